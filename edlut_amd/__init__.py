@@ -1,0 +1,31 @@
+"""MI355X (gfx950) implementation of EDLUT's time-driven neuron-update path.
+
+The compute path is the HIP library edlut_amd/libedlut_amd.so (C ABI in
+include/edlut_amd.h); this package is its host-side Python interface.
+"""
+from . import _capi
+from ._capi import EDLUTException, lib
+from .models import (
+    AdExTimeDrivenModel_GPU,
+    CreateNeuronModel,
+    LIFTimeDrivenModel_GPU,
+    REGISTRY,
+    TimeDrivenNeuronModel_GPU,
+    integration_method,
+)
+from .network import Network, binomial_thresholds, build_out_csr, from_spec
+
+__all__ = [
+    "EDLUTException",
+    "lib",
+    "AdExTimeDrivenModel_GPU",
+    "LIFTimeDrivenModel_GPU",
+    "TimeDrivenNeuronModel_GPU",
+    "CreateNeuronModel",
+    "REGISTRY",
+    "integration_method",
+    "Network",
+    "binomial_thresholds",
+    "build_out_csr",
+    "from_spec",
+]

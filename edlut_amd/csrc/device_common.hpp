@@ -1,0 +1,167 @@
+// device_common.hpp -- wave-level helpers and the ordered spike compaction
+// shared by every update kernel (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace edl {
+
+constexpr int kWave = 64;
+constexpr int kBlock = 256;               // threads per update workgroup
+constexpr int kWavesPerBlock = kBlock / kWave;
+
+// ---------------------------------------------------------------------------
+// Ordered compaction of spiking neuron indices, in two passes without atomics.
+//
+// Pass 1 (inside every update kernel): tile t = workgroup t covers neurons
+// [t*kBlock, (t+1)*kBlock).  A wave ballot gives each spiking lane its rank,
+// the four wave counts are combined in LDS, and the tile writes its count and
+// its spiking indices (ascending) into its slot of a staging array.
+//
+// Pass 2 (k_compact): each workgroup sums the counts of the tiles before its
+// range, scans its own range and copies every tile's indices to the output,
+// which is therefore ascending.  The last workgroup writes the total.
+// ---------------------------------------------------------------------------
+struct TileOut {
+    int32_t *tile_count;   // [n_tiles]
+    int32_t *staging;      // [n_tiles * tile_size]
+};
+
+// Item j of thread t in tile `tile` is neuron tile*kBlock*ITEMS + j*kBlock + t,
+// so ascending neuron order is item-major, then wave, then lane.
+template <int ITEMS>
+__device__ __forceinline__ void tile_spikes(const TileOut &o, int tile, const bool *spike,
+                                            const int *idx) {
+    __shared__ uint32_t s_wave[ITEMS * kWavesPerBlock];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    uint64_t ballot[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        ballot[j] = __ballot(spike[j]);
+        if (lane == 0) s_wave[j * kWavesPerBlock + wid] = (uint32_t)__popcll(ballot[j]);
+    }
+    __syncthreads();
+    uint32_t local = 0;
+#pragma unroll
+    for (int q = 0; q < ITEMS * kWavesPerBlock; q++) local += s_wave[q];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        if (spike[j]) {
+            uint32_t base = 0;
+            for (int q = 0; q < j * kWavesPerBlock + wid; q++) base += s_wave[q];
+            const uint32_t rank = (uint32_t)__popcll(ballot[j] & ((1ull << lane) - 1ull));
+            o.staging[(int64_t)tile * kBlock * ITEMS + base + rank] = idx[j];
+        }
+    }
+    if (threadIdx.x == 0) o.tile_count[tile] = (int32_t)local;
+}
+
+struct CompactArgs {
+    const int32_t *tile_count;
+    const int32_t *staging;
+    int n_tiles;
+    int tile_size;         // neurons per tile (staging stride)
+    int32_t *out_idx;      // ascending indices
+    int32_t *out_count;    // total
+    // optional spike log; the entry count is double-buffered by step parity:
+    // read log_count[step & 1], write log_count[(step + 1) & 1]
+    int32_t *log_step;
+    int32_t *log_idx;
+    int64_t *log_count;
+    int64_t log_cap;
+    uint32_t *err;         // bit 1: log overflow
+    int32_t step;
+};
+
+constexpr int kCompactTiles = kBlock;   // tiles per compaction workgroup
+
+__global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) {
+    __shared__ int64_t s_red[kWavesPerBlock];
+    __shared__ int32_t s_scan[kBlock];
+    __shared__ int64_t s_base;
+    const int t0 = blockIdx.x * kCompactTiles;
+    // 1. spikes in the tiles before this range
+    // tile_count is zero-padded to a multiple of 8*kBlock entries, and t0 is
+    // a multiple of kBlock: the eight loads of a round are unconditional.
+    int64_t before = 0;
+    for (int t = threadIdx.x; t < t0; t += 8 * kBlock) {
+        int32_t v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v[u] = a.tile_count[t + u * kBlock];
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+            if (t + u * kBlock < t0) before += v[u];
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) before += __shfl_xor(before, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) s_red[threadIdx.x / kWave] = before;
+    // 2. inclusive scan of this range's tile counts
+    const int my = t0 + (int)threadIdx.x;
+    s_scan[threadIdx.x] = my < a.n_tiles ? a.tile_count[my] : 0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int64_t b = 0;
+        for (int q = 0; q < kWavesPerBlock; q++) b += s_red[q];
+        s_base = b;
+    }
+    for (int off = 1; off < kBlock; off <<= 1) {
+        const int32_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
+        __syncthreads();
+        s_scan[threadIdx.x] += v;
+        __syncthreads();
+    }
+    const int64_t base = s_base;
+    const int64_t log_base = a.log_count ? a.log_count[a.step & 1] : 0;
+    // 3. copy: one thread per tile (most tiles hold no spike)
+    {
+        const int q = (int)threadIdx.x;
+        const int t = t0 + q;
+        const int32_t lo = q > 0 ? s_scan[q - 1] : 0;
+        const int32_t c = t < a.n_tiles ? s_scan[q] - lo : 0;
+        const int64_t start = base + lo;
+        for (int j = 0; j < c; j++) {
+            const int32_t id = a.staging[(int64_t)t * a.tile_size + j];
+            a.out_idx[start + j] = id;
+            if (a.log_idx) {
+                const int64_t lp = log_base + start + j;
+                if (lp < a.log_cap) {
+                    a.log_idx[lp] = id;
+                    a.log_step[lp] = a.step;
+                }
+            }
+        }
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const int64_t total = base + s_scan[kBlock - 1];
+        *a.out_count = (int32_t)total;
+        if (a.log_count) {
+            const int64_t nl = log_base + total;
+            if (nl > a.log_cap) atomicOr(a.err, 2u);
+            a.log_count[(a.step + 1) & 1] = nl;
+        }
+    }
+}
+
+// Counter-based hash of (seed, step, neuron) for the external drive.
+// mix32 is a 32-bit multiply-xorshift finaliser; the (seed, step) part is the
+// same for every neuron of a launch and is computed once.
+__host__ __device__ __forceinline__ uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+__host__ __device__ __forceinline__ uint32_t drive_step_key(uint64_t seed, int64_t step) {
+    return mix32((uint32_t)seed ^ mix32((uint32_t)(seed >> 32) ^ mix32((uint32_t)step)));
+}
+__host__ __device__ __forceinline__ uint32_t drive_hash_keyed(uint32_t step_key, uint32_t neuron) {
+    return mix32(neuron * 0x9E3779B9u + step_key);
+}
+__host__ __device__ __forceinline__ uint32_t drive_hash(uint64_t seed, int64_t step,
+                                                        uint32_t neuron) {
+    return drive_hash_keyed(drive_step_key(seed, step), neuron);
+}
+
+}  // namespace edl

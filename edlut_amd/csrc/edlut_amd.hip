@@ -1,0 +1,977 @@
+// edlut_amd.hip -- C ABI (include/edlut_amd.h) over the gfx950 kernels.
+//
+// Host-side objects own every device allocation (the reference builds device
+// objects with device-side `new`, AdExTimeDrivenModel_GPU_C_INTERFACE.cu:
+// 171-208; here they are plain host structs passed to kernels by value).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/edlut_amd.h"
+#include "kernels.hpp"
+
+using namespace edl;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (expr);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(EDLUT_E_HIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                        __FILE__, __LINE__);                                            \
+    } while (0)
+
+template <typename T> int dalloc(T **p, size_t count) {
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc((void **)p, count * sizeof(T)));
+    return 0;
+}
+
+uint64_t fnv1a(const void *data, size_t bytes) {
+    const unsigned char *b = (const unsigned char *)data;
+    uint64_t h = 1469598103934665603ull;
+    for (size_t i = 0; i < bytes; i++) {
+        h ^= b[i];
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+// ExponentialTable contents (ExponentialTable.h:60-66): double-precision exp
+// of each float grid point, stored as float.
+std::vector<float> build_exp_table() {
+    std::vector<float> t(kExpSize);
+    for (int i = 0; i < kExpSize; i++) {
+        float exponent = kExpMin + ((kExpMax - kExpMin) * i) / (kExpSize - 1);
+        t[i] = (float)std::exp((double)exponent);
+    }
+    return t;
+}
+
+// g_nmda_inf table (AdExTimeDrivenModel.cpp:27-35); exp(float) is the float
+// overload in the reference's model translation units.
+std::vector<float> build_nmda_table(float e_exc, float e_inh, float *aux) {
+    std::vector<float> t(kNmdaSize);
+    *aux = (kNmdaSize - 1) / (e_exc - e_inh);
+    for (int i = 0; i < kNmdaSize; i++) {
+        float V = e_inh + ((e_exc - e_inh) * i) / (kNmdaSize - 1);
+        t[i] = 1.0f / (1.0f + expf(-0.062f * V) * (1.2f / 3.57f));
+    }
+    return t;
+}
+
+// LIF refractory fractions (LIFTimeDrivenModel.cpp:189-201).  After a spike
+// at update s the clock holds 0 and gains (double)dt_s per update, so the
+// derivative of update s+1+j sees S_j = j-fold double sum of dt_s.  Entry j
+// is the fraction for S_j (negative: derivative zero); the table stops at the
+// first j with S_j*1000 > tau_ref, after which the full derivative applies.
+std::vector<float> build_refractory(float dt_s, float dt_ms, float tau_ref) {
+    std::vector<float> t;
+    const float time_scale = 1000.0f;
+    double S = 0.0;
+    for (int j = 0; j < (1 << 24); j++) {
+        double lst = S * time_scale;
+        if (lst > tau_ref) break;
+        if ((lst + dt_ms) > tau_ref) t.push_back((float)((lst + dt_ms - tau_ref) / dt_ms));
+        else t.push_back(-1.0f);
+        S += (double)dt_s;
+    }
+    return t;
+}
+
+inline float inv_of(float x) { return (float)(1.0 / (double)x); }
+
+// tile-count arrays are padded (and zeroed) to whole compaction chunks so the
+// compaction kernel reads them without bounds tests
+inline size_t padded_tiles(int n_tiles) {
+    const size_t chunk = 8 * (size_t)kBlock;
+    return ((size_t)n_tiles + chunk - 1) / chunk * chunk;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+struct edlut_population {
+    int model = 0, integ = 0, bdf_order = 0;
+    int n = 0, nv = 0, nd = 0;
+    int64_t stride = 0;
+    int n_tiles = 0;
+    int device = 0;
+    uint32_t mask = 0;
+    float dt_s = 0.0f, dt_ms = 0.0f;
+    std::vector<float> params;
+    ModelParams mp{};
+    StepConsts k{};
+    hipStream_t stream = nullptr;
+
+    float *state = nullptr;
+    int32_t *last_spike = nullptr;
+    float *exp_lut = nullptr;
+    float *nmda_lut = nullptr;
+    float *refr = nullptr;
+    int refr_len = 0;
+    uint64_t lut_sums[3] = {0, 0, 0};
+
+    // spikes of the last update (tile counts + staging, then compacted)
+    int32_t *tile_count = nullptr;
+    int32_t *staging = nullptr;
+    int32_t *spk_idx = nullptr;
+    int32_t *spk_cnt = nullptr;
+    int32_t *h_spk = nullptr;   // pinned
+    int32_t *h_cnt = nullptr;   // pinned
+    int last_n_spikes = 0;
+    uint32_t *err = nullptr;
+
+    // ordered input fold
+    uint32_t *cnt = nullptr, *fill = nullptr, *tile_flag = nullptr;
+    uint64_t *seg = nullptr, *bump = nullptr;
+    uint64_t upd_seq = 0;   // plug-in mode: parity of the fold counters
+    uint64_t *b_ord = nullptr;
+    float *b_w = nullptr;
+    uint64_t b_cap = 0;
+    std::vector<int32_t> q_tgt;
+    std::vector<uint8_t> q_type;
+    std::vector<float> q_w;
+    int32_t *e_tgt = nullptr;
+    uint8_t *e_type = nullptr;
+    float *e_w = nullptr;
+    int e_cap = 0;
+
+    edlut_network *net = nullptr;
+
+    FoldTarget fold_target(int par) const {
+        FoldTarget t;
+        t.state = state;
+        t.stride = stride;
+        t.ndiff = nd;
+        t.cnt = cnt;
+        t.fill = fill;
+        t.seg = seg;
+        t.tile_flag = tile_flag;
+        t.bump = bump;
+        t.par = par;
+        t.b_ord = b_ord;
+        t.b_w = b_w;
+        t.b_cap = b_cap;
+        t.err = err;
+        return t;
+    }
+
+    // Grow the fold's event buffer to hold `events` events.
+    int reserve_events(uint64_t events) {
+        if (events <= b_cap) return 0;
+        (void)hipStreamSynchronize(stream);
+        if (b_ord) (void)hipFree(b_ord);
+        if (b_w) (void)hipFree(b_w);
+        b_ord = nullptr;
+        b_w = nullptr;
+        b_cap = 0;
+        int rc;
+        if ((rc = dalloc(&b_ord, events)) || (rc = dalloc(&b_w, events))) return rc;
+        b_cap = events;
+        return 0;
+    }
+};
+
+struct Source {
+    bool self = false;
+    int64_t n = 0;
+    int32_t *ring_idx = nullptr;
+    int32_t *ring_cnt = nullptr;
+    int64_t *row_ptr = nullptr;
+    uint32_t *tgt_type = nullptr;
+    float *w = nullptr;
+    uint32_t *lcg = nullptr;
+    float rate = 0.0f;
+    int32_t *tile_count = nullptr;
+    int32_t *staging = nullptr;
+};
+
+struct edlut_network {
+    edlut_population *pop = nullptr;
+    int max_delay = 1;
+    int ring = 3;
+    std::vector<Source> sources;
+    int self_src = -1;
+    uint64_t total_syn = 0;
+    bool has_ext = false;
+    ExtDrive ext{};
+    // population-side ring when there is no recurrent source
+    int32_t *own_ring_idx = nullptr;
+    int32_t *own_ring_cnt = nullptr;
+    int32_t *tile_count = nullptr;
+    int32_t *staging = nullptr;
+    // spike log
+    int32_t *log_step = nullptr;
+    int32_t *log_idx = nullptr;
+    int64_t *log_count = nullptr;   // [2], by step parity
+    int64_t log_cap = 0;
+    int64_t last_step = 0;
+    // timing of the update kernel (event pool reserved by set_timing)
+    std::vector<hipEvent_t> ev;
+    int64_t timed = 0;
+    bool timing = false;
+};
+
+// ---------------------------------------------------------------------------
+// Kernel dispatch tables.
+// ---------------------------------------------------------------------------
+namespace {
+
+using UpdateFn = void (*)(UpdateArgs);
+
+template <class M, int I, unsigned MASK>
+void fill_one(UpdateFn *t) {
+    t[(I * 16 + (int)MASK) * 2 + 0] = k_update<M, I, MASK, false>;
+    t[(I * 16 + (int)MASK) * 2 + 1] = k_update<M, I, MASK, true>;
+}
+
+template <class M, int I, unsigned... MS>
+void fill_masks(UpdateFn *t, std::integer_sequence<unsigned, MS...>) {
+    (fill_one<M, I, MS>(t), ...);
+}
+
+template <class M>
+struct Table {
+    UpdateFn fn[3 * 16 * 2];
+    Table() {
+        fill_masks<M, kEuler>(fn, std::make_integer_sequence<unsigned, 16>{});
+        fill_masks<M, kRK2>(fn, std::make_integer_sequence<unsigned, 16>{});
+        fill_masks<M, kRK4>(fn, std::make_integer_sequence<unsigned, 16>{});
+    }
+};
+
+UpdateFn pick_update(int model, int integ, uint32_t mask, bool ext) {
+    static Table<AdExModel> adex;
+    static Table<LIFModel> lif;
+    const int idx = (integ * 16 + (int)(mask & 15u)) * 2 + (ext ? 1 : 0);
+    if (model == EDLUT_ADEX) return adex.fn[idx];
+    if (model == EDLUT_LIF) return lif.fn[idx];
+    return nullptr;
+}
+
+int check_err(edlut_population *p) {
+    uint32_t e = 0;
+    HIP_TRY(hipMemcpyAsync(&e, p->err, sizeof(uint32_t), hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    if (e & 1u) return fail(EDLUT_E_STATE, "spike compaction look-back exceeded its spin limit");
+    if (e & 2u) return fail(EDLUT_E_CAP, "spike log capacity exceeded");
+    if (e & 4u) return fail(EDLUT_E_CAP, "input event buffer capacity exceeded");
+    if (e & 8u) return fail(EDLUT_E_STATE, "input fold segment wait exceeded its spin limit");
+    return 0;
+}
+
+UpdateArgs make_update_args(edlut_population *p, int32_t step) {
+    UpdateArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.state = p->state;
+    a.stride = p->stride;
+    a.n = p->n;
+    a.n_tiles = p->n_tiles;
+    a.step = step;
+    a.mp = p->mp;
+    a.k = p->k;
+    a.exp_lut = p->exp_lut;
+    a.nmda_lut = p->nmda_lut;
+    a.refr = p->refr;
+    a.refr_len = p->refr_len;
+    a.last_spike = p->last_spike;
+    return a;
+}
+
+void launch_compact(hipStream_t st, int32_t *tile_count, int32_t *staging, int n_tiles,
+                    int tile_size, int32_t *out_idx, int32_t *out_count, int32_t *log_step, int32_t *log_idx,
+                    int64_t *log_count, int64_t log_cap, uint32_t *err, int32_t step) {
+    CompactArgs c;
+    c.tile_count = tile_count;
+    c.staging = staging;
+    c.n_tiles = n_tiles;
+    c.tile_size = tile_size;
+    c.out_idx = out_idx;
+    c.out_count = out_count;
+    c.log_step = log_step;
+    c.log_idx = log_idx;
+    c.log_count = log_count;
+    c.log_cap = log_cap;
+    c.err = err;
+    c.step = step;
+    const int blocks = (n_tiles + kCompactTiles - 1) / kCompactTiles;
+    hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(kBlock), 0, st, c);
+}
+
+int launch_fold(edlut_population *p, int par, int n_events, bool ring, const RingEvents *re,
+                const PushEvents *pe) {
+    FoldTarget t = p->fold_target(par);
+    if (ring) {
+        const int blocks = 512;
+        hipLaunchKernelGGL((k_fold_ring<kCount, 32>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
+        hipLaunchKernelGGL((k_fold_ring<kScatter, 32>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
+    } else {
+        if (n_events == 0) return 0;
+        int blocks = (n_events + kBlock - 1) / kBlock;
+        if (blocks > 4096) blocks = 4096;
+        hipLaunchKernelGGL((k_fold_push<kCount>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *pe);
+        hipLaunchKernelGGL((k_fold_push<kScatter>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *pe);
+    }
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char *edlut_last_error(void) { return g_err.c_str(); }
+
+const char *edlut_device_arch(void) { return "gfx950"; }
+
+uint32_t edlut_drive_hash(uint64_t seed, int64_t step, uint32_t neuron) {
+    return drive_hash(seed, step, neuron);
+}
+
+int edlut_population_create(int model, int integrator, int bdf_order, const float *params,
+                            int n_params, float dt_s, int n_neurons, int device,
+                            edlut_population **out) {
+    if (!out) return fail(EDLUT_E_ARG, "out is null");
+    *out = nullptr;
+    if (model != EDLUT_ADEX && model != EDLUT_LIF)
+        return fail(EDLUT_E_ARG, "neuron model %d is not available in this build", model);
+    if (integrator < EDLUT_EULER || integrator > EDLUT_RK4)
+        return fail(EDLUT_E_ARG, "integration method %d is not available for this model", integrator);
+    const int need = model == EDLUT_ADEX ? 14 : 10;
+    if (!params || n_params != need)
+        return fail(EDLUT_E_ARG, "model %d takes %d parameters, got %d", model, need, n_params);
+    if (!(dt_s > 0.0f)) return fail(EDLUT_E_ARG, "integration step must be positive");
+    if (n_neurons <= 0) return fail(EDLUT_E_ARG, "population size must be positive");
+    if (n_neurons > (1 << 30) - 1) return fail(EDLUT_E_ARG, "population too large");
+    for (int i = 0; i < n_params; i++)
+        if (!std::isfinite(params[i])) return fail(EDLUT_E_ARG, "parameter %d is not finite", i);
+
+    HIP_TRY(hipSetDevice(device));
+    edlut_population *p = new edlut_population;
+    p->model = model;
+    p->integ = integrator;
+    p->bdf_order = bdf_order;
+    p->n = n_neurons;
+    p->device = device;
+    p->params.assign(params, params + n_params);
+    p->dt_s = dt_s;
+    p->dt_ms = dt_s * 1000.0f;  // IntegrationMethodFast.h:191-193
+    p->k.dt = p->dt_ms;
+    p->k.h = p->dt_ms * 0.5f;
+    p->k.s6 = p->dt_ms * 0.166666666667f;
+    const float elapsed = integrator == EDLUT_RK4 ? p->dt_ms * 0.5f : p->dt_ms;
+
+    float e_exc, e_inh;
+    std::vector<float> init;
+    if (model == EDLUT_ADEX) {
+        AdExParams &q = p->mp.adex;
+        const float *x = params;
+        const float thr = x[2];
+        q.a = x[0];
+        q.b = x[1];
+        q.inv_thr = inv_of(thr);
+        q.v_thr = x[3];
+        q.v_cap = x[3] + 6 * thr;
+        q.inv_tau_w = inv_of(x[4]);
+        q.e_exc = e_exc = x[5];
+        q.e_inh = e_inh = x[6];
+        q.e_reset = x[7];
+        q.e_leak = x[8];
+        q.g_leak = x[9];
+        q.glk_thr = x[9] * thr;
+        q.inv_c_m = inv_of(x[10]);
+        q.cf[0] = expf(-elapsed * inv_of(x[11]));
+        q.cf[1] = expf(-elapsed * inv_of(x[12]));
+        q.cf[2] = expf(-elapsed * inv_of(x[13]));
+        init = {x[8], 0, 0, 0, 0, 0};
+        for (int j = 0; j < 6; j++) q.init[j] = init[j];
+        p->nv = 6;
+        p->nd = 2;
+    } else {
+        LIFParams &q = p->mp.lif;
+        const float *x = params;
+        q.e_exc = e_exc = x[0];
+        q.e_inh = e_inh = x[1];
+        q.e_leak = x[2];
+        q.v_thr = x[3];
+        q.inv_c_m = inv_of(x[4]);
+        q.g_leak = x[8];
+        q.cf[0] = expf(-elapsed * inv_of(x[5]));
+        q.cf[1] = expf(-elapsed * inv_of(x[6]));
+        q.cf[2] = expf(-elapsed * inv_of(x[9]));
+        init = {x[2], 0, 0, 0, 0};
+        for (int j = 0; j < 5; j++) q.init[j] = init[j];
+        p->nv = 5;
+        p->nd = 1;
+    }
+    float nmda_aux = 0.0f;
+    std::vector<float> nmda = build_nmda_table(e_exc, e_inh, &nmda_aux);
+    if (model == EDLUT_ADEX) p->mp.adex.nmda_aux = nmda_aux;
+    else p->mp.lif.nmda_aux = nmda_aux;
+    std::vector<float> expt = build_exp_table();
+    std::vector<float> refr;
+    if (model == EDLUT_LIF) refr = build_refractory(dt_s, p->dt_ms, params[7]);
+    p->refr_len = (int)refr.size();
+    p->lut_sums[0] = fnv1a(expt.data(), expt.size() * sizeof(float));
+    p->lut_sums[1] = fnv1a(nmda.data(), nmda.size() * sizeof(float));
+    p->lut_sums[2] = fnv1a(refr.data(), refr.size() * sizeof(float));
+
+    // rows padded to whole update tiles: every kernel load stays in bounds
+    p->stride = ((int64_t)n_neurons + kUpdTile - 1) / kUpdTile * kUpdTile;
+    p->n_tiles = (n_neurons + kUpdTile - 1) / kUpdTile;
+
+    int rc = 0;
+    auto cleanup = [&](int code) {
+        edlut_population_destroy(p);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking) != hipSuccess)
+        return cleanup(fail(EDLUT_E_HIP, "hipStreamCreate failed"));
+    if ((rc = dalloc(&p->state, (size_t)p->nv * p->stride))) return cleanup(rc);
+    if ((rc = dalloc(&p->exp_lut, kExpSize))) return cleanup(rc);
+    if ((rc = dalloc(&p->nmda_lut, kNmdaSize))) return cleanup(rc);
+    if ((rc = dalloc(&p->refr, refr.size() + 1))) return cleanup(rc);
+    if ((rc = dalloc(&p->tile_count, padded_tiles(p->n_tiles)))) return cleanup(rc);
+    if ((rc = dalloc(&p->staging, (size_t)p->n_tiles * kUpdTile))) return cleanup(rc);
+    if ((rc = dalloc(&p->spk_idx, n_neurons))) return cleanup(rc);
+    if ((rc = dalloc(&p->spk_cnt, 1))) return cleanup(rc);
+    if ((rc = dalloc(&p->err, 1))) return cleanup(rc);
+    if ((rc = dalloc(&p->cnt, (size_t)3 * p->stride))) return cleanup(rc);
+    if ((rc = dalloc(&p->fill, (size_t)3 * p->stride))) return cleanup(rc);
+    if ((rc = dalloc(&p->seg, (size_t)3 * p->stride))) return cleanup(rc);
+    if ((rc = dalloc(&p->tile_flag, p->n_tiles))) return cleanup(rc);
+    if ((rc = dalloc(&p->bump, 2))) return cleanup(rc);
+    if ((rc = p->reserve_events(1u << 16))) return cleanup(rc);
+    if (model == EDLUT_LIF) {
+        if ((rc = dalloc(&p->last_spike, p->stride))) return cleanup(rc);
+        std::vector<int32_t> never(p->stride, -(1 << 30));
+        if (hipMemcpy(p->last_spike, never.data(), sizeof(int32_t) * p->stride,
+                      hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(EDLUT_E_HIP, "hipMemcpy failed"));
+    }
+    if (hipHostMalloc((void **)&p->h_spk, sizeof(int32_t) * n_neurons, hipHostMallocDefault) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&p->h_cnt, sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(EDLUT_E_HIP, "hipHostMalloc failed"));
+
+    // initial state (VectorNeuronState_GPU_C_INTERFACE.cu:66-131)
+    std::vector<float> st((size_t)p->nv * p->stride, 0.0f);
+    for (int v = 0; v < p->nv; v++)
+        for (int i = 0; i < n_neurons; i++) st[(size_t)v * p->stride + i] = init[v];
+    bool ok = hipMemcpy(p->state, st.data(), st.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(p->exp_lut, expt.data(), expt.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+              hipMemcpy(p->nmda_lut, nmda.data(), nmda.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
+              (refr.empty() || hipMemcpy(p->refr, refr.data(), refr.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess) &&
+              hipMemset(p->tile_count, 0, sizeof(int32_t) * padded_tiles(p->n_tiles)) == hipSuccess &&
+              hipMemset(p->spk_cnt, 0, sizeof(int32_t)) == hipSuccess &&
+              hipMemset(p->err, 0, sizeof(uint32_t)) == hipSuccess &&
+              hipMemset(p->cnt, 0, sizeof(uint32_t) * 3 * p->stride) == hipSuccess &&
+              hipMemset(p->fill, 0, sizeof(uint32_t) * 3 * p->stride) == hipSuccess &&
+              hipMemset(p->tile_flag, 0, sizeof(uint32_t) * p->n_tiles) == hipSuccess &&
+              hipMemset(p->seg, 0xff, sizeof(uint64_t) * 3 * p->stride) == hipSuccess &&
+              hipMemset(p->bump, 0, 2 * sizeof(uint64_t)) == hipSuccess;
+    if (!ok) return cleanup(fail(EDLUT_E_HIP, "device initialisation failed"));
+    *out = p;
+    return 0;
+}
+
+int edlut_population_destroy(edlut_population *p) {
+    if (!p) return 0;
+    if (p->net) {
+        edlut_network_destroy(p->net);
+    }
+    (void)hipSetDevice(p->device);
+    if (p->stream) (void)hipStreamSynchronize(p->stream);
+    void *dev[] = {p->state, p->last_spike, p->exp_lut, p->nmda_lut, p->refr, p->tile_count,
+                   p->staging, p->spk_idx, p->spk_cnt, p->err, p->cnt, p->fill, p->seg,
+                   p->tile_flag, p->bump, p->b_ord, p->b_w, p->e_tgt,
+                   p->e_type, p->e_w};
+    for (void *q : dev)
+        if (q) (void)hipFree(q);
+    if (p->h_spk) (void)hipHostFree(p->h_spk);
+    if (p->h_cnt) (void)hipHostFree(p->h_cnt);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    delete p;
+    return 0;
+}
+
+int edlut_population_layout(const edlut_population *p, int *n_vars, int *n_diff, int64_t *stride) {
+    if (!p) return fail(EDLUT_E_ARG, "population is null");
+    if (n_vars) *n_vars = p->nv;
+    if (n_diff) *n_diff = p->nd;
+    if (stride) *stride = p->stride;
+    return 0;
+}
+
+int edlut_population_enable_synapses(edlut_population *p, uint32_t mask) {
+    if (!p) return fail(EDLUT_E_ARG, "population is null");
+    if (mask & ~15u) return fail(EDLUT_E_ARG, "unknown synapse type bits 0x%x", mask);
+    p->mask |= mask;
+    return 0;
+}
+
+int edlut_population_push_inputs(edlut_population *p, const int32_t *target, const uint8_t *type,
+                                 const float *weight, int n) {
+    if (!p) return fail(EDLUT_E_ARG, "population is null");
+    if (n < 0 || (n > 0 && (!target || !type || !weight))) return fail(EDLUT_E_ARG, "bad event arrays");
+    for (int i = 0; i < n; i++) {
+        if (target[i] < 0 || target[i] >= p->n)
+            return fail(EDLUT_E_ARG, "event %d: target %d out of range", i, target[i]);
+        if (type[i] > 2) return fail(EDLUT_E_ARG, "event %d: synapse type %d does not take spikes", i, type[i]);
+        if (!(p->mask & (1u << type[i])))
+            return fail(EDLUT_E_ARG, "event %d: synapse type %d is not enabled", i, type[i]);
+    }
+    p->q_tgt.insert(p->q_tgt.end(), target, target + n);
+    p->q_type.insert(p->q_type.end(), type, type + n);
+    p->q_w.insert(p->q_w.end(), weight, weight + n);
+    return 0;
+}
+
+int edlut_population_set_ext_current(edlut_population *p, const float *current) {
+    if (!p || !current) return fail(EDLUT_E_ARG, "null argument");
+    p->mask |= EDLUT_SYN_EXT_I;
+    const int var = p->nd + 3;
+    HIP_TRY(hipMemcpyAsync(p->state + (int64_t)var * p->stride, current, sizeof(float) * p->n,
+                           hipMemcpyHostToDevice, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
+    if (!p) return fail(EDLUT_E_ARG, "population is null");
+    if (step < 1 || step > (1ll << 30)) return fail(EDLUT_E_ARG, "update index out of range");
+    HIP_TRY(hipSetDevice(p->device));
+    // 1. ordered fold of the queued input spikes
+    const int ne = (int)p->q_tgt.size();
+    const int par = (int)(++p->upd_seq & 1u);
+    int rc0 = 0;
+    if (ne > 0) {
+        if (ne > p->e_cap) {
+            (void)hipFree(p->e_tgt);
+            (void)hipFree(p->e_type);
+            (void)hipFree(p->e_w);
+            p->e_tgt = nullptr;
+            p->e_type = nullptr;
+            p->e_w = nullptr;
+            int rc;
+            int cap = ne < 4096 ? 4096 : ne * 2;
+            if ((rc = dalloc(&p->e_tgt, cap)) || (rc = dalloc(&p->e_type, cap)) ||
+                (rc = dalloc(&p->e_w, cap)))
+                return rc;
+            p->e_cap = cap;
+        }
+        HIP_TRY(hipMemcpyAsync(p->e_tgt, p->q_tgt.data(), sizeof(int32_t) * ne, hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipMemcpyAsync(p->e_type, p->q_type.data(), ne, hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipMemcpyAsync(p->e_w, p->q_w.data(), sizeof(float) * ne, hipMemcpyHostToDevice, p->stream));
+        if ((rc0 = p->reserve_events((uint64_t)ne))) return rc0;
+        PushEvents pe{p->e_tgt, p->e_type, p->e_w, ne};
+        int rc = launch_fold(p, par, ne, false, nullptr, &pe);
+        if (rc) return rc;
+    }
+    // 2. integrate, detect, compact
+    UpdateArgs a = make_update_args(p, (int32_t)step);
+    a.out = TileOut{p->tile_count, p->staging};
+    a.fold = p->fold_target(par);
+    UpdateFn fn = pick_update(p->model, p->integ, p->mask, false);
+    hipLaunchKernelGGL(fn, dim3(p->n_tiles), dim3(kBlock), 0, p->stream, a);
+    launch_compact(p->stream, p->tile_count, p->staging, p->n_tiles, kUpdTile, p->spk_idx, p->spk_cnt,
+                   nullptr, nullptr, nullptr, 0, p->err, (int32_t)step);
+    HIP_TRY(hipGetLastError());
+    // 3. hand the compacted spikes back (synchronous, like the reference)
+    HIP_TRY(hipMemcpyAsync(p->h_cnt, p->spk_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    const int ns = *p->h_cnt;
+    if (ns > 0) {
+        HIP_TRY(hipMemcpyAsync(p->h_spk, p->spk_idx, sizeof(int32_t) * ns, hipMemcpyDeviceToHost, p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+    }
+    p->q_tgt.clear();
+    p->q_type.clear();
+    p->q_w.clear();
+    int rc = check_err(p);
+    if (rc) return rc;
+    p->last_n_spikes = ns;
+    if (n_spikes) *n_spikes = ns;
+    return 0;
+}
+
+int edlut_population_get_spikes(const edlut_population *p, int32_t *idx, int cap) {
+    if (!p || (!idx && p->last_n_spikes > 0)) return fail(EDLUT_E_ARG, "null argument");
+    if (cap < p->last_n_spikes) return fail(EDLUT_E_CAP, "spike buffer too small (%d < %d)", cap, p->last_n_spikes);
+    std::memcpy(idx, p->h_spk, sizeof(int32_t) * p->last_n_spikes);
+    return 0;
+}
+
+int edlut_population_get_state(const edlut_population *p, int var, float *out) {
+    if (!p || !out) return fail(EDLUT_E_ARG, "null argument");
+    if (var < 0 || var >= p->nv) return fail(EDLUT_E_ARG, "state variable %d out of range", var);
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipMemcpyAsync(out, p->state + (int64_t)var * p->stride, sizeof(float) * p->n,
+                           hipMemcpyDeviceToHost, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int edlut_population_set_state(edlut_population *p, int var, const float *in) {
+    if (!p || !in) return fail(EDLUT_E_ARG, "null argument");
+    if (var < 0 || var >= p->nv) return fail(EDLUT_E_ARG, "state variable %d out of range", var);
+    if (var == p->nd + 3) p->mask |= EDLUT_SYN_EXT_I;
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipMemcpyAsync(p->state + (int64_t)var * p->stride, in, sizeof(float) * p->n,
+                           hipMemcpyHostToDevice, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int edlut_population_device_state(const edlut_population *p, float **state) {
+    if (!p || !state) return fail(EDLUT_E_ARG, "null argument");
+    *state = p->state;
+    return 0;
+}
+
+int edlut_population_lut_checksums(const edlut_population *p, uint64_t *out3) {
+    if (!p || !out3) return fail(EDLUT_E_ARG, "null argument");
+    for (int i = 0; i < 3; i++) out3[i] = p->lut_sums[i];
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **out) {
+    if (!pop || !out) return fail(EDLUT_E_ARG, "null argument");
+    *out = nullptr;
+    if (pop->net) return fail(EDLUT_E_STATE, "population already has a network");
+    if (max_delay < 1 || max_delay > kMaxDelay)
+        return fail(EDLUT_E_ARG, "max_delay must be in [1, %d]", kMaxDelay);
+    HIP_TRY(hipSetDevice(pop->device));
+    edlut_network *n = new edlut_network;
+    n->pop = pop;
+    n->max_delay = max_delay;
+    n->ring = max_delay + 2;
+    n->log_cap = 1ll << 26;
+    int rc;
+    if ((rc = dalloc(&n->own_ring_idx, (size_t)n->ring * pop->n)) ||
+        (rc = dalloc(&n->own_ring_cnt, n->ring)) ||
+        (rc = dalloc(&n->tile_count, padded_tiles(pop->n_tiles))) ||
+        (rc = dalloc(&n->staging, (size_t)pop->n_tiles * kUpdTile)) ||
+        (rc = dalloc(&n->log_step, n->log_cap)) || (rc = dalloc(&n->log_idx, n->log_cap)) ||
+        (rc = dalloc(&n->log_count, 2))) {
+        delete n;
+        return rc;
+    }
+    HIP_TRY(hipMemset(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring));
+    HIP_TRY(hipMemset(n->log_count, 0, 2 * sizeof(int64_t)));
+    HIP_TRY(hipMemset(n->tile_count, 0, sizeof(int32_t) * padded_tiles(pop->n_tiles)));
+    pop->net = n;
+    *out = n;
+    return 0;
+}
+
+int edlut_network_destroy(edlut_network *n) {
+    if (!n) return 0;
+    (void)hipSetDevice(n->pop->device);
+    (void)hipStreamSynchronize(n->pop->stream);
+    for (Source &s : n->sources) {
+        void *dev[] = {s.self ? nullptr : s.ring_idx, s.self ? nullptr : s.ring_cnt, s.row_ptr,
+                       s.tgt_type, s.w, s.lcg, s.tile_count, s.staging};
+        for (void *q : dev)
+            if (q) (void)hipFree(q);
+    }
+    void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
+                   n->log_idx, n->log_count};
+    for (void *q : dev)
+        if (q) (void)hipFree(q);
+    for (hipEvent_t e : n->ev) (void)hipEventDestroy(e);
+    n->pop->net = nullptr;
+    delete n;
+    return 0;
+}
+
+static int add_synapses(edlut_network *n, Source &s, const int64_t *row_ptr,
+                        const uint32_t *tgt_type, const float *w) {
+    const size_t nrow = (size_t)s.n * n->max_delay + 1;
+    if (!row_ptr) return fail(EDLUT_E_ARG, "row_ptr is null");
+    if (row_ptr[0] != 0) return fail(EDLUT_E_ARG, "row_ptr[0] must be 0");
+    for (size_t i = 1; i < nrow; i++)
+        if (row_ptr[i] < row_ptr[i - 1]) return fail(EDLUT_E_ARG, "row_ptr must be non-decreasing");
+    const int64_t nsyn = row_ptr[nrow - 1];
+    if (nsyn >= (1ll << 40)) return fail(EDLUT_E_ARG, "too many synapses");
+    if (nsyn > 0 && (!tgt_type || !w)) return fail(EDLUT_E_ARG, "synapse arrays are null");
+    uint32_t types = 0;
+    for (int64_t q = 0; q < nsyn; q++) {
+        const uint32_t t = tgt_type[q] & 0x3fffffffu, ty = tgt_type[q] >> 30;
+        if ((int64_t)t >= n->pop->n) return fail(EDLUT_E_ARG, "synapse %lld: target out of range", (long long)q);
+        if (ty > 2) return fail(EDLUT_E_ARG, "synapse %lld: type %u does not take spikes", (long long)q, ty);
+        types |= 1u << ty;
+    }
+    // CheckSynapseType (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:239-262)
+    n->pop->mask |= types;
+    int rc;
+    if ((rc = dalloc(&s.row_ptr, nrow)) || (rc = dalloc(&s.tgt_type, nsyn)) || (rc = dalloc(&s.w, nsyn)))
+        return rc;
+    HIP_TRY(hipMemcpy(s.row_ptr, row_ptr, sizeof(int64_t) * nrow, hipMemcpyHostToDevice));
+    if (nsyn > 0) {
+        HIP_TRY(hipMemcpy(s.tgt_type, tgt_type, sizeof(uint32_t) * nsyn, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(s.w, w, sizeof(float) * nsyn, hipMemcpyHostToDevice));
+    }
+    return 0;
+}
+
+int edlut_network_add_recurrent(edlut_network *n, const int64_t *row_ptr, const uint32_t *tgt_type,
+                                const float *w) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (n->self_src >= 0) return fail(EDLUT_E_STATE, "recurrent synapses already added");
+    if ((int)n->sources.size() >= kMaxSources) return fail(EDLUT_E_CAP, "too many sources");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    Source s;
+    s.self = true;
+    s.n = n->pop->n;
+    s.ring_idx = n->own_ring_idx;
+    s.ring_cnt = n->own_ring_cnt;
+    int rc = add_synapses(n, s, row_ptr, tgt_type, w);
+    if (rc) return rc;
+    n->sources.push_back(s);
+    n->self_src = (int)n->sources.size() - 1;
+    n->total_syn += (uint64_t)row_ptr[(size_t)s.n * n->max_delay];
+    return n->pop->reserve_events(n->total_syn + 1);
+}
+
+int edlut_network_add_poisson(edlut_network *n, int n_src, const uint32_t *lcg_state, float rate_hz,
+                              const int64_t *row_ptr, const uint32_t *tgt_type, const float *w) {
+    if (!n || !lcg_state) return fail(EDLUT_E_ARG, "null argument");
+    if (n_src <= 0) return fail(EDLUT_E_ARG, "n_src must be positive");
+    if ((int)n->sources.size() >= kMaxSources) return fail(EDLUT_E_CAP, "too many sources");
+    if (!std::isfinite(rate_hz)) return fail(EDLUT_E_ARG, "rate must be finite");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    Source s;
+    s.self = false;
+    s.n = n_src;
+    s.rate = rate_hz;
+    int rc = add_synapses(n, s, row_ptr, tgt_type, w);
+    if (rc) return rc;
+    const int tiles = (n_src + kBlock - 1) / kBlock;
+    if ((rc = dalloc(&s.lcg, n_src)) || (rc = dalloc(&s.ring_idx, (size_t)n->ring * n_src)) ||
+        (rc = dalloc(&s.ring_cnt, n->ring)) || (rc = dalloc(&s.tile_count, padded_tiles(tiles))) ||
+        (rc = dalloc(&s.staging, (size_t)tiles * kBlock)))
+        return rc;
+    HIP_TRY(hipMemcpy(s.lcg, lcg_state, sizeof(uint32_t) * n_src, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(s.ring_cnt, 0, sizeof(int32_t) * n->ring));
+    HIP_TRY(hipMemset(s.tile_count, 0, sizeof(int32_t) * padded_tiles(tiles)));
+    n->sources.push_back(s);
+    n->total_syn += (uint64_t)row_ptr[(size_t)s.n * n->max_delay];
+    return n->pop->reserve_events(n->total_syn + 1);
+}
+
+int edlut_network_set_external(edlut_network *n, int n_thr, const uint32_t *thresholds, float w_ext,
+                               uint64_t seed) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (n_thr < 0 || n_thr > 16) return fail(EDLUT_E_ARG, "at most 16 thresholds");
+    if (n_thr > 0 && !thresholds) return fail(EDLUT_E_ARG, "thresholds are null");
+    for (int j = 1; j < n_thr; j++)
+        if (thresholds[j] > thresholds[j - 1]) return fail(EDLUT_E_ARG, "thresholds must not increase");
+    n->has_ext = n_thr > 0;
+    std::memset(&n->ext, 0, sizeof(n->ext));
+    n->ext.seed = seed;
+    n->ext.n_thr = n_thr;
+    n->ext.w = w_ext;
+    for (int j = 0; j < n_thr; j++) n->ext.thr[j] = thresholds[j];
+    if (n->has_ext) n->pop->mask |= EDLUT_SYN_EXC;
+    return 0;
+}
+
+int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (first_step < 1 || n_steps < 0 || first_step + n_steps > (1ll << 30))
+        return fail(EDLUT_E_ARG, "step range out of bounds");
+    edlut_population *p = n->pop;
+    HIP_TRY(hipSetDevice(p->device));
+    if (!p->q_tgt.empty()) return fail(EDLUT_E_STATE, "queued host inputs are not used in network mode");
+    if (first_step != n->last_step + 1) {
+        // Rings hold the spikes of the steps just before `first_step` only
+        // when the run continues the previous one.
+        for (Source &s : n->sources) HIP_TRY(hipMemsetAsync(s.ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
+        HIP_TRY(hipMemsetAsync(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
+        int64_t lc[2] = {0, 0};
+        HIP_TRY(hipMemcpyAsync(lc, n->log_count, sizeof(lc), hipMemcpyDeviceToHost, p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+        const int64_t total = lc[(n->last_step + 1) & 1];
+        lc[0] = lc[1] = total;
+        HIP_TRY(hipMemcpyAsync(n->log_count, lc, sizeof(lc), hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+    }
+    HIP_TRY(hipMemsetAsync(p->bump, 0, 2 * sizeof(uint64_t), p->stream));
+    RingEvents re;
+    std::memset(&re, 0, sizeof(re));
+    re.n_sources = (int)n->sources.size();
+    re.max_delay = n->max_delay;
+    re.ring = n->ring;
+    for (int q = 0; q < re.n_sources; q++) {
+        const Source &s = n->sources[q];
+        re.src[q] = RingSource{s.ring_idx, s.ring_cnt, s.row_ptr, s.tgt_type, s.w, s.n};
+    }
+    UpdateFn fn = pick_update(p->model, p->integ, p->mask, n->has_ext);
+    const bool timed = n->timing && 2 * (size_t)n_steps <= n->ev.size();
+    n->timed = timed ? n_steps : 0;
+    for (int64_t k = first_step; k < first_step + n_steps; k++) {
+        const int slot = (int)(k % n->ring);
+        // Poisson devices fire at update k (their spikes feed later updates).
+        for (Source &s : n->sources) {
+            if (s.self) continue;
+            PoissonArgs pa;
+            std::memset(&pa, 0, sizeof(pa));
+            pa.lcg = s.lcg;
+            pa.n = (int)s.n;
+            pa.n_tiles = (int)((s.n + kBlock - 1) / kBlock);
+            pa.rate = s.rate;
+            pa.rate_dt = (double)s.rate * (double)p->dt_s;
+            pa.out = TileOut{s.tile_count, s.staging};
+            hipLaunchKernelGGL(k_poisson, dim3(pa.n_tiles), dim3(kBlock), 0, p->stream, pa);
+            launch_compact(p->stream, s.tile_count, s.staging, pa.n_tiles, kBlock,
+                           s.ring_idx + (int64_t)slot * s.n, s.ring_cnt + slot, nullptr, nullptr,
+                           nullptr, 0, p->err, (int32_t)k);
+        }
+        // Deliver the events consumed by update k.
+        if (re.n_sources > 0) {
+            re.step = (int32_t)k;
+            int rc = launch_fold(p, (int)(k & 1), 0, true, &re, nullptr);
+            if (rc) return rc;
+        }
+        UpdateArgs a = make_update_args(p, (int32_t)k);
+        a.ext = n->ext;
+        a.out = TileOut{n->tile_count, n->staging};
+        a.fold = p->fold_target((int)(k & 1));
+        if (timed) HIP_TRY(hipEventRecord(n->ev[2 * (k - first_step)], p->stream));
+        hipLaunchKernelGGL(fn, dim3(p->n_tiles), dim3(kBlock), 0, p->stream, a);
+        if (timed) HIP_TRY(hipEventRecord(n->ev[2 * (k - first_step) + 1], p->stream));
+        launch_compact(p->stream, n->tile_count, n->staging, p->n_tiles, kUpdTile,
+                       n->own_ring_idx + (int64_t)slot * p->n, n->own_ring_cnt + slot, n->log_step,
+                       n->log_idx, n->log_count, n->log_cap, p->err, (int32_t)k);
+    }
+    HIP_TRY(hipGetLastError());
+    n->last_step = first_step + n_steps - 1;
+    return 0;
+}
+
+int edlut_network_synchronize(edlut_network *n) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    return check_err(n->pop);
+}
+
+int edlut_network_spike_log(edlut_network *n, int32_t *step, int32_t *idx, int64_t cap, int64_t *n_out) {
+    if (!n || !n_out) return fail(EDLUT_E_ARG, "null argument");
+    int rc = edlut_network_synchronize(n);
+    if (rc) return rc;
+    int64_t lc[2] = {0, 0};
+    HIP_TRY(hipMemcpy(lc, n->log_count, sizeof(lc), hipMemcpyDeviceToHost));
+    int64_t cnt = lc[(n->last_step + 1) & 1];
+    if (cnt > n->log_cap) cnt = n->log_cap;
+    *n_out = cnt;
+    if (cnt > cap) return fail(EDLUT_E_CAP, "spike log has %lld entries, buffer holds %lld", (long long)cnt, (long long)cap);
+    if (cnt > 0) {
+        HIP_TRY(hipMemcpy(step, n->log_step, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(idx, n->log_idx, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int edlut_network_clear_log(edlut_network *n) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipMemsetAsync(n->log_count, 0, 2 * sizeof(int64_t), n->pop->stream));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    return 0;
+}
+
+// Network::FindOutConnections (Network.cpp:70-123) sorts synapses with
+// qsort_inters (Network.cpp:45-68): target queue, source, delay, target.  With
+// one queue this is (source, delay, target); two stable counting passes
+// (target, then source*max_delay+delay) give that order in O(E + N), keeping
+// the input order of exact duplicates.
+int edlut_build_out_csr(int64_t n_src, int64_t n_tgt, int max_delay, int64_t n_edges,
+                        const int32_t *src, const int32_t *tgt, const int32_t *delay,
+                        const uint8_t *type, const float *w, int64_t *row_ptr,
+                        uint32_t *tgt_type, float *w_out) {
+    if (n_src <= 0 || n_tgt <= 0 || n_tgt >= (1ll << 30) || max_delay < 1 || n_edges < 0)
+        return fail(EDLUT_E_ARG, "bad sizes");
+    if (n_edges > 0 && (!src || !tgt || !delay || !type || !w || !tgt_type || !w_out))
+        return fail(EDLUT_E_ARG, "null edge array");
+    if (!row_ptr) return fail(EDLUT_E_ARG, "row_ptr is null");
+    for (int64_t e = 0; e < n_edges; e++) {
+        if (src[e] < 0 || src[e] >= n_src) return fail(EDLUT_E_ARG, "edge %lld: source out of range", (long long)e);
+        if (tgt[e] < 0 || tgt[e] >= n_tgt) return fail(EDLUT_E_ARG, "edge %lld: target out of range", (long long)e);
+        if (delay[e] < 1 || delay[e] > max_delay) return fail(EDLUT_E_ARG, "edge %lld: delay out of range", (long long)e);
+        if (type[e] > 2) return fail(EDLUT_E_ARG, "edge %lld: type must be 0, 1 or 2", (long long)e);
+    }
+    // pass 1: stable by target
+    std::vector<int64_t> off((size_t)n_tgt + 1, 0);
+    for (int64_t e = 0; e < n_edges; e++) off[(size_t)tgt[e] + 1]++;
+    for (int64_t t = 0; t < n_tgt; t++) off[(size_t)t + 1] += off[(size_t)t];
+    std::vector<int64_t> by_t((size_t)n_edges);
+    for (int64_t e = 0; e < n_edges; e++) by_t[(size_t)off[(size_t)tgt[e]]++] = e;
+    // pass 2: stable by (source, delay)
+    const int64_t nrow = n_src * (int64_t)max_delay;
+    std::fill(row_ptr, row_ptr + nrow + 1, 0);
+    for (int64_t e = 0; e < n_edges; e++) row_ptr[(int64_t)src[e] * max_delay + delay[e]]++;
+    for (int64_t r = 0; r < nrow; r++) row_ptr[r + 1] += row_ptr[r];
+    std::vector<int64_t> fill(row_ptr, row_ptr + nrow);
+    for (int64_t q = 0; q < n_edges; q++) {
+        const int64_t e = by_t[(size_t)q];
+        const int64_t r = (int64_t)src[e] * max_delay + delay[e] - 1;
+        const int64_t pos = fill[(size_t)r]++;
+        tgt_type[pos] = (uint32_t)tgt[e] | ((uint32_t)type[e] << 30);
+        w_out[pos] = w[e];
+    }
+    return 0;
+}
+
+int edlut_network_set_timing(edlut_network *n, int64_t max_steps) {
+    if (!n || max_steps < 0) return fail(EDLUT_E_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    for (hipEvent_t e : n->ev) (void)hipEventDestroy(e);
+    n->ev.clear();
+    n->timed = 0;
+    n->timing = max_steps > 0;
+    n->ev.assign(2 * (size_t)max_steps, nullptr);
+    for (auto &e : n->ev) HIP_TRY(hipEventCreate(&e));
+    return 0;
+}
+
+int edlut_network_update_timing(edlut_network *n, double *ms, int64_t *launches) {
+    if (!n || !ms || !launches) return fail(EDLUT_E_ARG, "null argument");
+    int rc = edlut_network_synchronize(n);
+    if (rc) return rc;
+    double total = 0.0;
+    for (int64_t i = 0; i < n->timed; i++) {
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, n->ev[2 * i], n->ev[2 * i + 1]));
+        total += t;
+    }
+    *ms = total;
+    *launches = n->timed;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,389 @@
+// kernels.hpp -- the per-step kernels of the time-driven neuron update path.
+//
+//   k_update     one step of every neuron: optional external drive, ODE step,
+//                conductance decay, threshold/reset, NaN reset and ordered
+//                spike compaction (one fused pass over the SoA state).
+//   k_poisson    one step of a Poisson generator device (per-source LCG).
+//   k_fold_*     ordered conductance accumulation of a step's input events.
+#pragma once
+#include "device_common.hpp"
+#include "device_models.hpp"
+
+namespace edl {
+
+enum Integ { kEuler = 0, kRK2 = 1, kRK4 = 2 };
+
+// ---------------------------------------------------------------------------
+// Ordered conductance accumulation.
+//
+// Every input event of a step is (key = type*stride + target, order, w); the
+// CPU reference adds them one by one into the state, in `order`.  The fold
+// reproduces that sequence exactly, for any event density, in two passes
+// plus a tail in the update kernel:
+//
+//   count    per-key event counts (atomic increments)
+//   scatter  keys with one event add it directly -- no ordering question;
+//            for the others the first arrival reserves a contiguous segment
+//            of the event buffer, every event drops (order, w) into it, and
+//            the target's tile is flagged
+//   update   in flagged tiles each neuron sorts its keys' segments by order
+//            and folds them sequentially before integrating, then clears the
+//            key's counters
+//
+// The event buffer can hold every event of a step (one per synapse) in the
+// network mode, so it cannot overflow there.
+// ---------------------------------------------------------------------------
+constexpr int kItems = 4;                    // neurons per thread in the update kernels
+constexpr int kUpdTile = kBlock * kItems;    // neurons per update tile
+constexpr uint64_t kSegEmpty = ~0ull;
+constexpr uint64_t kSegBusy = ~0ull - 1ull;
+
+struct FoldTarget {
+    float *state;
+    int64_t stride;
+    int ndiff;
+    uint32_t *cnt;       // per key: events this step
+    uint32_t *fill;      // per key: events scattered so far
+    uint64_t *seg;       // per key: segment start (kSegEmpty when none)
+    uint32_t *tile_flag; // per update tile: some key of the tile has >= 2 events
+    uint64_t *bump;      // [2] event-buffer allocator, by step parity
+    int par;             // parity of this step
+    uint64_t *b_ord;     // event buffer: order
+    float *b_w;          //               weight
+    uint64_t b_cap;
+    uint32_t *err;       // bit 2: event buffer overflow, bit 3: spin limit
+};
+
+__device__ __forceinline__ float *g_ptr(const FoldTarget &t, uint64_t key) {
+    const uint64_t type = key / (uint64_t)t.stride;
+    const uint64_t tgt = key - type * (uint64_t)t.stride;
+    return t.state + (int64_t)(t.ndiff + (int)type) * t.stride + (int64_t)tgt;
+}
+
+enum FoldPass { kCount = 0, kScatter = 1 };
+
+template <int PASS>
+__device__ __forceinline__ void fold_event(const FoldTarget &t, uint64_t key, uint64_t ord,
+                                           float w) {
+    if (PASS == kCount) {
+        atomicAdd(&t.cnt[key], 1u);
+        return;
+    }
+    const uint32_t c = t.cnt[key];
+    if (c == 1u) {
+        float *g = g_ptr(t, key);
+        *g = *g + w;
+        t.cnt[key] = 0u;
+        return;
+    }
+    // Reserve the key's segment (first arrival) -- every lane does this step
+    // before any lane waits, so a wave never waits on one of its own lanes.
+    const uint64_t prev = atomicCAS((unsigned long long *)&t.seg[key], kSegEmpty, kSegBusy);
+    if (prev == kSegEmpty) {
+        const uint64_t base = atomicAdd((unsigned long long *)&t.bump[t.par], (unsigned long long)c);
+        __hip_atomic_store(&t.seg[key], base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint64_t tgt = key % (uint64_t)t.stride;
+        t.tile_flag[tgt / kUpdTile] = 1u;
+    }
+    uint64_t base = __hip_atomic_load(&t.seg[key], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    uint32_t spins = 0;
+    while (base == kSegBusy) {
+        if (++spins > (1u << 22)) {
+            atomicOr(t.err, 8u);
+            return;
+        }
+        __builtin_amdgcn_s_sleep(1);
+        base = __hip_atomic_load(&t.seg[key], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    const uint64_t pos = base + atomicAdd(&t.fill[key], 1u);
+    if (pos < t.b_cap) {
+        t.b_ord[pos] = ord;
+        t.b_w[pos] = w;
+    } else {
+        atomicOr(t.err, 4u);
+    }
+}
+
+// Folds the segment of `key` (m >= 2 events) into g in order and clears the
+// key's counters.  Called by the update kernel for its own neuron, out of
+// line so the common path keeps its small register budget; the segment is
+// sorted in place in the event buffer (insertion sort: segments are short).
+__device__ __attribute__((noinline)) float fold_segment(uint32_t *cnt, uint32_t *fill,
+                                                        uint64_t *seg, uint64_t *b_ord,
+                                                        float *b_w, uint64_t b_cap,
+                                                        uint32_t *err, uint64_t key,
+                                                        uint32_t m, float g) {
+    const uint64_t b = seg[key];
+    if (b == kSegEmpty || b == kSegBusy || b + m > b_cap) {
+        atomicOr(err, 4u);
+    } else {
+        for (uint32_t j = 1; j < m; j++) {
+            const uint64_t ko = b_ord[b + j];
+            const float kw = b_w[b + j];
+            int64_t i = (int64_t)j - 1;
+            while (i >= 0 && b_ord[b + i] > ko) {
+                b_ord[b + i + 1] = b_ord[b + i];
+                b_w[b + i + 1] = b_w[b + i];
+                --i;
+            }
+            b_ord[b + i + 1] = ko;
+            b_w[b + i + 1] = kw;
+        }
+        for (uint32_t j = 0; j < m; j++) g = g + b_w[b + j];
+    }
+    cnt[key] = 0u;
+    fill[key] = 0u;
+    seg[key] = kSegEmpty;
+    return g;
+}
+
+struct ExtDrive {
+    uint64_t seed;
+    uint32_t thr[16];  // cumulative thresholds, decreasing
+    int n_thr;
+    float w;
+};
+
+struct UpdateArgs {
+    float *state;          // SoA, row stride `stride`
+    int64_t stride;
+    int n;                 // neurons
+    int n_tiles;
+    int32_t step;          // update index k (>= 1)
+    ModelParams mp;
+    StepConsts k;
+    const float *exp_lut;
+    const float *nmda_lut;
+    const float *refr;
+    int refr_len;
+    int32_t *last_spike;   // LIF only: update index of the last spike
+    ExtDrive ext;
+    TileOut out;
+    FoldTarget fold;       // ordered input fold of this step (multi-event keys)
+};
+
+// One tile = kUpdTile neurons; thread t handles neurons tile*kUpdTile +
+// n*kBlock + t for n < kItems (each n is one coalesced row access).  All
+// loads are issued first, the kItems neurons are integrated stage-major, and
+// the stores come last, so every thread keeps kItems independent dependency
+// chains in flight.
+template <class M, int INTEG, unsigned MASK, bool EXT>
+__global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
+    const int tile = blockIdx.x;
+    __shared__ uint32_t s_flag;
+    // the tile's fold flag is read alongside the state loads (the barrier
+    // that publishes it comes after them)
+    uint32_t flag = 0;
+    if (threadIdx.x == 0) flag = a.fold.tile_flag[tile];
+    // one row pointer per state variable (uniform), 32-bit neuron offsets
+    float *__restrict__ row[M::NV];
+#pragma unroll
+    for (int j = 0; j < M::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
+    const int64_t ld = a.stride;
+    const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, a.step) : 0u;
+    int idx[kItems];
+    bool live[kItems];
+    float s[kItems][M::NV];
+    RhsCtx c[kItems];
+#pragma unroll
+    for (int n = 0; n < kItems; n++) {
+        idx[n] = tile * kUpdTile + n * kBlock + (int)threadIdx.x;
+        live[n] = idx[n] < a.n;
+        // rows are padded to a multiple of kUpdTile, so the loads stay in bounds
+        const uint32_t i = (uint32_t)idx[n];
+#pragma unroll
+        for (int j = 0; j < M::ND; j++) s[n][j] = row[j][i];
+        s[n][M::EXC] = (MASK & 1u) ? row[M::EXC][i] : 0.0f;
+        s[n][M::INH] = (MASK & 2u) ? row[M::INH][i] : 0.0f;
+        s[n][M::NMDA] = (MASK & 4u) ? row[M::NMDA][i] : 0.0f;
+        s[n][M::EXT] = (MASK & 8u) ? row[M::EXT][i] : 0.0f;
+        c[n].exp_lut = a.exp_lut;
+        c[n].nmda_lut = a.nmda_lut;
+        c[n].refr = a.refr;
+        c[n].refr_len = a.refr_len;
+        c[n].since_spike = a.last_spike ? (a.step - 1 - a.last_spike[i]) : (1 << 30);
+    }
+    if (threadIdx.x == 0) {
+        s_flag = flag;
+        if (flag) a.fold.tile_flag[tile] = 0u;
+        // the next step's fold allocates from the other parity: clear it
+        if (tile == 0) a.fold.bump[a.fold.par ^ 1] = 0ull;
+    }
+    __syncthreads();
+    if (s_flag) {
+        // events of keys with several inputs this step, in delivery order
+#pragma unroll
+        for (int n = 0; n < kItems; n++) {
+            if (!live[n]) continue;
+#pragma unroll
+            for (int ty = 0; ty < 3; ty++) {
+                if (!(MASK & (1u << ty))) continue;
+                const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)idx[n];
+                const uint32_t m = a.fold.cnt[key];
+                if (m >= 2u)
+                    s[n][M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ord,
+                                                     a.fold.b_w, a.fold.b_cap, a.fold.err, key, m,
+                                                     s[n][M::EXC + ty]);
+            }
+        }
+    }
+    if (EXT) {
+        // External drive events of this interval, after the recurrent ones.
+#pragma unroll
+        for (int n = 0; n < kItems; n++) {
+            const uint32_t u = drive_hash_keyed(step_key, (uint32_t)idx[n]);
+            int cnt = 0;
+            while (cnt < a.ext.n_thr && u < a.ext.thr[cnt]) cnt++;
+            for (int q = 0; q < cnt; q++) s[n][M::EXC] += a.ext.w;
+        }
+    }
+    if (INTEG == kEuler) integrate_euler<M, MASK, kItems>(a.mp, c, a.k, s);
+    else if (INTEG == kRK2) integrate_rk2<M, MASK, kItems>(a.mp, c, a.k, s);
+    else integrate_rk4<M, MASK, kItems>(a.mp, c, a.k, s);
+
+    bool spk[kItems];
+#pragma unroll
+    for (int n = 0; n < kItems; n++) {
+        const uint32_t i = (uint32_t)idx[n];
+        spk[n] = live[n] && M::spike(a.mp, s[n]);
+        if (!live[n]) continue;
+        if (spk[n] && a.last_spike) a.last_spike[i] = a.step;
+        if (s[n][0] != s[n][0]) {
+            // TimeDrivenNeuronModel.cpp:75-88: NaN -> whole state to initial.
+#pragma unroll
+            for (int j = 0; j < M::NV; j++) row[j][i] = M::init(a.mp, j);
+        } else {
+#pragma unroll
+            for (int j = 0; j < M::ND; j++) row[j][i] = s[n][j];
+            if (MASK & 1u) row[M::EXC][i] = s[n][M::EXC];
+            if (MASK & 2u) row[M::INH][i] = s[n][M::INH];
+            if (MASK & 4u) row[M::NMDA][i] = s[n][M::NMDA];
+        }
+    }
+    tile_spikes<kItems>(a.out, tile, spk, idx);
+}
+
+// ---------------------------------------------------------------------------
+// Poisson generator device (PoissonGeneratorDeviceVector.cpp:56-70): per
+// source, if rate > 0 draw the next frand() of its LCG
+// (RandomGenerator.h:61-107) and fire when rate*step (double) exceeds it.
+// ---------------------------------------------------------------------------
+struct PoissonArgs {
+    uint32_t *lcg;
+    int n;
+    int n_tiles;
+    float rate;
+    double rate_dt;   // (double)rate * (double)step_s
+    TileOut out;
+};
+
+__global__ __launch_bounds__(kBlock) void k_poisson(PoissonArgs a) {
+    const int tile = blockIdx.x;
+    bool spk[1] = {false};
+    int idx[1] = {tile * kBlock + (int)threadIdx.x};
+    const int i = idx[0];
+    if (i < a.n && a.rate > 0.0f) {
+        uint32_t x = a.lcg[i];
+        x = (uint32_t)(((uint64_t)x * 1103515245ull + 12345ull) & 0x7fffffffull);
+        a.lcg[i] = x;
+        const float u = ((float)x) / 2147483647.0f;
+        spk[0] = a.rate_dt > (double)u;
+    }
+    tile_spikes<1>(a.out, tile, spk, idx);
+}
+
+// Events pushed by the host: order = position in the host's list.
+struct PushEvents {
+    const int32_t *tgt;
+    const uint8_t *type;
+    const float *w;
+    int n;
+};
+
+template <int PASS>
+__global__ __launch_bounds__(kBlock) void k_fold_push(FoldTarget t, PushEvents e) {
+    for (int q = blockIdx.x * kBlock + threadIdx.x; q < e.n; q += gridDim.x * kBlock) {
+        const uint64_t key = (uint64_t)e.type[q] * (uint64_t)t.stride + (uint64_t)e.tgt[q];
+        fold_event<PASS>(t, key, (uint64_t)q, e.w[q]);
+    }
+}
+
+// Events generated on the device from the spike rings of the sources.
+constexpr int kMaxSources = 4;
+constexpr int kMaxDelay = 256;
+struct RingSource {
+    const int32_t *ring_idx;   // [ring][n_src]
+    const int32_t *ring_cnt;   // [ring]
+    const int64_t *row_ptr;    // [n_src*max_delay+1]
+    const uint32_t *tgt_type;  // target | type << 30
+    const float *w;
+    int64_t n_src;
+};
+struct RingEvents {
+    RingSource src[kMaxSources];
+    int n_sources;
+    int max_delay;
+    int ring;
+    int32_t step;              // update index about to run
+};
+
+// Work item = one spike of one source at emission step k-1-d; a group of
+// LANES lanes walks its synapses of delay d.  Items are enumerated in
+// (d, source, rank) order from a prefix over the per-(d, source) counts.
+// order = ((d-1)*n_sources + source) << 40 | synapse position; for a fixed
+// (d, source) the synapse position rises with the source index and then the
+// target index, which is the reference's delivery order.
+template <int PASS, int LANES>
+__global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r) {
+    __shared__ int64_t s_pref[kMaxDelay * kMaxSources + 1];
+    __shared__ int s_slot[kMaxDelay * kMaxSources];
+    const int nseg = r.max_delay * r.n_sources;
+    for (int g = threadIdx.x; g < nseg; g += kBlock) {
+        const int d = g / r.n_sources + 1;
+        const int p = g % r.n_sources;
+        const int64_t s = (int64_t)r.step - 1 - d;
+        int slot = 0, c = 0;
+        if (s >= 1) {
+            slot = (int)(s % r.ring);
+            c = r.src[p].ring_cnt[slot];
+        }
+        s_slot[g] = slot;
+        s_pref[g + 1] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_pref[0] = 0;
+        for (int g = 0; g < nseg; g++) s_pref[g + 1] += s_pref[g];
+    }
+    __syncthreads();
+    const int64_t n_items = s_pref[nseg];
+    const int lane = threadIdx.x % LANES;
+    const int64_t group = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / LANES;
+    const int64_t n_groups = (int64_t)gridDim.x * kBlock / LANES;
+    for (int64_t it = group; it < n_items; it += n_groups) {
+        // last segment g with s_pref[g] <= it (segments may be empty)
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (s_pref[mid] <= it) lo = mid;
+            else hi = mid - 1;
+        }
+        const int g = lo;
+        const int d = g / r.n_sources + 1;
+        const int p = g % r.n_sources;
+        const RingSource &src = r.src[p];
+        const int64_t rank = it - s_pref[g];
+        const int32_t id = src.ring_idx[(int64_t)s_slot[g] * src.n_src + rank];
+        const int64_t b = src.row_ptr[(int64_t)id * r.max_delay + d - 1];
+        const int64_t e = src.row_ptr[(int64_t)id * r.max_delay + d];
+        const uint64_t ord_hi = ((uint64_t)((d - 1) * r.n_sources + p)) << 40;
+        for (int64_t q = b + lane; q < e; q += LANES) {
+            const uint32_t tt = src.tgt_type[q];
+            const uint64_t key =
+                (uint64_t)(tt >> 30) * (uint64_t)t.stride + (uint64_t)(tt & 0x3fffffffu);
+            fold_event<PASS>(t, key, ord_hi | (uint64_t)q, src.w[q]);
+        }
+    }
+}
+
+}  // namespace edl

@@ -1,0 +1,162 @@
+"""Device-resident network mode: spike sources, synapses and delivery on the GPU.
+
+A `Network` couples one GPU population (a TimeDrivenNeuronModel_GPU after
+InitializeStates) with its spike sources: the population itself (recurrent
+synapses), Poisson generator devices (PoissonGeneratorDeviceVector with one
+RandomGenerator per source) and a counter-based external drive.  Spikes never
+leave the device during a run; they are logged on the device and read back at
+the end.  Delivery follows the order of the reference's event queue (see
+include/edlut_amd.h, network mode).
+"""
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+
+from . import _capi as C
+from ._capi import EDLUTException
+
+
+def build_out_csr(n_src: int, n_tgt: int, max_delay: int, src, tgt, delay, syn_type, weight):
+    """Sort an edge list into the per-(source, delay) row layout, in the order of
+    Network::FindOutConnections (Kernel/src/spike/Network.cpp:45-123).
+
+    Returns (row_ptr int64[n_src*max_delay+1], tgt_type uint32[E], w float32[E]).
+    """
+    src = np.ascontiguousarray(src, np.int32)
+    tgt = np.ascontiguousarray(tgt, np.int32)
+    delay = np.ascontiguousarray(delay, np.int32)
+    syn_type = np.ascontiguousarray(syn_type, np.uint8)
+    weight = np.ascontiguousarray(weight, np.float32)
+    e = len(src)
+    if not (len(tgt) == len(delay) == len(syn_type) == len(weight) == e):
+        raise EDLUTException(C.EDLUT_E_ARG, "edge arrays differ in length")
+    row_ptr = np.zeros(n_src * max_delay + 1, np.int64)
+    tt = np.zeros(max(e, 1), np.uint32)
+    w = np.zeros(max(e, 1), np.float32)
+    C.check(
+        C.lib().edlut_build_out_csr(
+            n_src, n_tgt, max_delay, e, C.ptr(src), C.ptr(tgt), C.ptr(delay), C.ptr(syn_type),
+            C.ptr(weight), C.ptr(row_ptr), C.ptr(tt), C.ptr(w)
+        )
+    )
+    return row_ptr, tt[:e], w[:e]
+
+
+def binomial_thresholds(n_syn: int, p: float, max_terms: int = 16) -> np.ndarray:
+    """Cumulative thresholds for the external drive: thr[j-1] = round(P(K >= j) * 2^32)
+    for K ~ Binomial(n_syn, p), so that k = #{j : u < thr[j]} for a uniform
+    32-bit u has that distribution (terms below 2^-32 are dropped)."""
+    thr = []
+    pmf = [math.comb(n_syn, k) * p**k * (1 - p) ** (n_syn - k) for k in range(n_syn + 1)]
+    for j in range(1, min(n_syn, max_terms) + 1):
+        tail = sum(pmf[j:])
+        v = int(round(tail * 2.0**32))
+        if v <= 0:
+            break
+        thr.append(min(v, 2**32 - 1))
+    return np.array(thr, np.uint32)
+
+
+class Network:
+    def __init__(self, model, max_delay: int):
+        self.model = model
+        self.max_delay = int(max_delay)
+        h = ctypes.c_void_p()
+        C.check(C.lib().edlut_network_create(model.handle, self.max_delay, C.byref(h)))
+        self._h = h
+        self._keep = []
+        self.next_step = 1
+
+    def close(self):
+        # the population owns its network; destroying the model frees it too
+        if self._h is not None and self.model._pop is not None:
+            C.lib().edlut_network_destroy(self._h)
+        self._h = None
+
+    def add_recurrent(self, row_ptr, tgt_type, weight):
+        row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+        tgt_type = np.ascontiguousarray(tgt_type, np.uint32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        C.check(C.lib().edlut_network_add_recurrent(self._h, C.ptr(row_ptr), C.ptr(tgt_type), C.ptr(weight)))
+
+    def add_poisson(self, lcg_state, rate_hz, row_ptr, tgt_type, weight):
+        lcg_state = np.ascontiguousarray(lcg_state, np.uint32)
+        row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+        tgt_type = np.ascontiguousarray(tgt_type, np.uint32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        C.check(
+            C.lib().edlut_network_add_poisson(
+                self._h, len(lcg_state), C.ptr(lcg_state), float(rate_hz), C.ptr(row_ptr),
+                C.ptr(tgt_type), C.ptr(weight)
+            )
+        )
+
+    def set_external(self, thresholds, w_ext, seed):
+        thr = np.ascontiguousarray(thresholds, np.uint32)
+        C.check(C.lib().edlut_network_set_external(self._h, len(thr), C.ptr(thr), float(w_ext), int(seed)))
+
+    def run(self, n_steps: int, first_step: int | None = None, sync: bool = True):
+        if first_step is None:
+            first_step = self.next_step
+        C.check(C.lib().edlut_network_run(self._h, int(first_step), int(n_steps)))
+        self.next_step = first_step + n_steps
+        if sync:
+            self.synchronize()
+
+    def synchronize(self):
+        C.check(C.lib().edlut_network_synchronize(self._h))
+
+    def spikes(self):
+        """(step, index) arrays of every logged spike."""
+        cnt = ctypes.c_int64(0)
+        rc = C.lib().edlut_network_spike_log(self._h, None, None, 0, C.byref(cnt))
+        if rc not in (0, C.EDLUT_E_CAP):
+            C.check(rc)
+        m = cnt.value
+        st = np.zeros(max(m, 1), np.int32)
+        ix = np.zeros(max(m, 1), np.int32)
+        n = ctypes.c_int64(0)
+        C.check(C.lib().edlut_network_spike_log(self._h, C.ptr(st), C.ptr(ix), m, C.byref(n)))
+        return st[: n.value].copy(), ix[: n.value].copy()
+
+    def clear_log(self):
+        C.check(C.lib().edlut_network_clear_log(self._h))
+
+    def set_timing(self, max_steps: int):
+        """Time every update-kernel launch of runs of up to max_steps steps."""
+        C.check(C.lib().edlut_network_set_timing(self._h, int(max_steps)))
+
+    def update_timing(self):
+        """(total ms, launches) of the update kernel over the last run."""
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int64(0)
+        C.check(C.lib().edlut_network_update_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def from_spec(spec, device: int = 0, params: dict | None = None):
+    """Build the GPU population and its network for a workloads.NetworkSpec.
+
+    Returns (model, network)."""
+    from .models import CreateNeuronModel, integration_method
+
+    model = CreateNeuronModel(spec.model, dict(spec.params))
+    if params:
+        model.SetParameters(params)
+    model.SetParameters({"int_meth": integration_method(spec.integrator, float(np.float32(1e-4)))})
+    model.InitializeStates(spec.n, 0, device)
+    net = Network(model, spec.max_delay)
+    if spec.rec is not None:
+        s, t, d, y, w = spec.rec
+        rp, tt, ww = build_out_csr(spec.n, spec.n, spec.max_delay, s, t, d, y, w)
+        net.add_recurrent(rp, tt, ww)
+    for states, rate, (s, t, d, y, w) in spec.poisson:
+        rp, tt, ww = build_out_csr(len(states), spec.n, spec.max_delay, s, t, d, y, w)
+        net.add_poisson(states, rate, rp, tt, ww)
+    if spec.ext is not None:
+        thr, w_ext, seed = spec.ext
+        net.set_external(thr, w_ext, seed)
+    return model, net

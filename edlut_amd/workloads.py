@@ -1,0 +1,99 @@
+"""Seeded synthetic workloads of the benchmark configurations.
+
+Each builder returns plain numpy arrays (parameters, edge lists, RNG states) so
+the same inputs can be fed to the HIP path and to any CPU implementation.
+
+  lif_poisson      1,000 LIF neurons (Euler, dt 0.1 ms), each with 100 AMPA +
+                   25 GABA Poisson inputs at 10 Hz; weights U[0,0.5] / U[0,1] nS;
+                   delays U{1..5 ms} on the 0.1 ms grid.
+  adex_recurrent   AdEx network, RK4 dt 0.1 ms, fixed in-degree CSR (sources
+                   without replacement, no self-connections), 80/20 exc/inh
+                   sources, weights lognormal(-1, 0.5) nS, delays U{1..10}
+                   steps, external drive = 100 Poisson synapses at 5 Hz each.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .network import binomial_thresholds
+
+DT_S = np.float32(1e-4)
+
+
+@dataclass
+class NetworkSpec:
+    model: str
+    integrator: str
+    n: int
+    max_delay: int
+    params: dict = field(default_factory=dict)
+    # recurrent edges (src, tgt, delay steps, type, weight)
+    rec: tuple | None = None
+    # Poisson devices: list of (lcg_state, rate_hz, edges)
+    poisson: list = field(default_factory=list)
+    # external drive (thresholds, w_ext, seed)
+    ext: tuple | None = None
+    steps: int = 0
+
+
+def lcg_seeds(seed: int, n: int) -> np.ndarray:
+    """States of n RandomGenerator objects built after RandomGenerator::srand(seed)
+    (Kernel/include/simulation/RandomGenerator.h:48-57, RandomGenerator.cpp:21-24):
+    g_0 = seed, g_{i+1} = (765426813*g_i + 54321) mod 2^31."""
+    out = np.empty(n, np.uint32)
+    g = seed & 0xFFFFFFFF
+    a, c, m = 765426813, 54321, 0x7FFFFFFF
+    for i in range(n):
+        out[i] = g
+        g = (a * g + c) & m
+    return out
+
+
+def lif_poisson(n: int = 1000, n_exc: int = 100, n_inh: int = 25, rate: float = 10.0,
+                steps: int = 10000, seed: int = 1) -> NetworkSpec:
+    rng = np.random.default_rng(seed)
+    per = n_exc + n_inh
+    n_src = n * per
+    src = np.arange(n_src, dtype=np.int32)
+    tgt = np.repeat(np.arange(n, dtype=np.int32), per)
+    is_inh = np.tile(np.r_[np.zeros(n_exc, bool), np.ones(n_inh, bool)], n)
+    typ = is_inh.astype(np.uint8)
+    w = np.where(is_inh, rng.uniform(0.0, 1.0, n_src), rng.uniform(0.0, 0.5, n_src)).astype(np.float32)
+    delay = rng.integers(10, 51, n_src).astype(np.int32)  # 1..5 ms in 0.1 ms steps
+    states = lcg_seeds(seed * 7919 + 17, n_src)
+    spec = NetworkSpec("LIFTimeDrivenModel_GPU", "Euler", n, 50, steps=steps)
+    spec.poisson.append((states, rate, (src, tgt, delay, typ, w)))
+    return spec
+
+
+def adex_recurrent(n: int = 1_000_000, in_degree: int = 200, exc_frac: float = 0.8,
+                   max_delay: int = 10, n_ext: int = 100, ext_rate: float = 5.0,
+                   w_ext: float = 1.0, steps: int = 10000, seed: int = 2) -> NetworkSpec:
+    rng = np.random.default_rng(seed)
+    k = min(in_degree, n - 1)
+    src = rng.integers(0, n - 1, size=(n, k), dtype=np.int32)
+    # redraw duplicates until every row holds distinct sources
+    while True:
+        src.sort(axis=1)
+        dup = np.zeros_like(src, dtype=bool)
+        dup[:, 1:] = src[:, 1:] == src[:, :-1]
+        nd = int(dup.sum())
+        if nd == 0:
+            break
+        src[dup] = rng.integers(0, n - 1, size=nd, dtype=np.int32)
+    # no self-connections: sources drawn from n-1 values, shifted past the target
+    tgt = np.repeat(np.arange(n, dtype=np.int32), k).reshape(n, k)
+    src = src + (src >= tgt)
+    src = src.reshape(-1)
+    tgt = tgt.reshape(-1)
+    n_exc = int(round(exc_frac * n))
+    typ = (src >= n_exc).astype(np.uint8)
+    w = rng.lognormal(-1.0, 0.5, size=src.shape[0]).astype(np.float32)
+    delay = rng.integers(1, max_delay + 1, size=src.shape[0]).astype(np.int32)
+    thr = binomial_thresholds(n_ext, float(ext_rate) * float(DT_S))
+    spec = NetworkSpec("AdExTimeDrivenModel_GPU", "RK4", n, max_delay, steps=steps)
+    spec.rec = (src, tgt, delay, typ, w)
+    spec.ext = (thr, np.float32(w_ext), seed * 1000003 + 11)
+    return spec

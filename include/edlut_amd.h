@@ -1,0 +1,180 @@
+/*
+ * edlut_amd.h -- C ABI of the MI355X (gfx950) time-driven neuron-update path.
+ *
+ * The library replaces the device side of EDLUT's GPU neuron-model plug-ins
+ * (TimeDrivenNeuronModel_GPU_C_INTERFACE and its subclasses) and their
+ * integration-method plug-ins.  Plain pointers and sizes only; every call
+ * returns 0 on success or a negative EDLUT_E* code, with a message available
+ * from edlut_last_error() (the reference prints and exits instead,
+ * Kernel/include/cudaError.h:14-22; the C++ host layer turns codes into
+ * exceptions).
+ *
+ * Citations are relative to the reference repository root.
+ */
+#ifndef EDLUT_AMD_H_
+#define EDLUT_AMD_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- identifiers -------------------------------------------------------- */
+/* Neuron models; registry names as in Kernel/src/neuron_model/NeuronModelFactory.cu:66-91. */
+enum edlut_model {
+    EDLUT_LIF = 0,      /* "LIFTimeDrivenModel_GPU"  (LIFTimeDrivenModel_GPU_C_INTERFACE.cu)  */
+    EDLUT_ADEX = 1,     /* "AdExTimeDrivenModel_GPU" (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:568-570) */
+    EDLUT_HH = 2,       /* "HHTimeDrivenModel_GPU" */
+    EDLUT_GRANULE = 3,  /* "EgidioGranuleCell_TimeDriven_GPU" */
+    EDLUT_PURKINJE = 4  /* "TimeDrivenPurkinjeCell_GPU" */
+};
+/* Integration methods; names of IntegrationMethodFactory_GPU_C_INTERFACE.cu:40-145. */
+enum edlut_integrator { EDLUT_EULER = 0, EDLUT_RK2 = 1, EDLUT_RK4 = 2, EDLUT_BDF = 3 };
+/* Synapse-type bits (CheckSynapseType, AdExTimeDrivenModel_GPU_C_INTERFACE.cu:239-290). */
+enum edlut_synapse { EDLUT_SYN_EXC = 1, EDLUT_SYN_INH = 2, EDLUT_SYN_NMDA = 4, EDLUT_SYN_EXT_I = 8 };
+
+enum edlut_error {
+    EDLUT_OK = 0,
+    EDLUT_E_ARG = -1,     /* invalid argument (EDLUTException in the reference) */
+    EDLUT_E_HIP = -2,     /* HIP runtime error (HANDLE_ERROR in the reference)   */
+    EDLUT_E_CAP = -3,     /* a device buffer's capacity was exceeded             */
+    EDLUT_E_STATE = -4    /* call out of order (e.g. network already running)    */
+};
+
+typedef struct edlut_population edlut_population;
+typedef struct edlut_network edlut_network;
+
+/* Message of the last failing call on this thread. */
+const char *edlut_last_error(void);
+
+/* ---- population (one neuron-model plug-in instance) --------------------- */
+/*
+ * Replaces M_GPU_C_INTERFACE construction + SetParameters + InitializeStates
+ * (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:48-57,146-167) and
+ * VectorNeuronState_GPU_C_INTERFACE::InitializeStatesGPU
+ * (VectorNeuronState_GPU_C_INTERFACE.cu:66-131).  params holds the model's
+ * float parameters in the order of its GetDefaultParameters() map:
+ *   LIF : e_exc e_inh e_leak v_thr c_m tau_exc tau_inh tau_ref g_leak tau_nmda
+ *   AdEx: a b thr_slo_fac v_thr tau_w e_exc e_inh e_reset e_leak g_leak c_m
+ *         tau_exc tau_inh tau_nmda
+ * dt_s is the integration method's "step" parameter (seconds).  State lives on
+ * `device` as structure-of-arrays: variable v of neuron i at v*stride+i.
+ */
+int edlut_population_create(int model, int integrator, int bdf_order, const float *params,
+                            int n_params, float dt_s, int n_neurons, int device,
+                            edlut_population **out);
+int edlut_population_destroy(edlut_population *pop);
+
+/* Number of state variables and the SoA row stride (>= n_neurons). */
+int edlut_population_layout(const edlut_population *pop, int *n_vars, int *n_diff,
+                            int64_t *stride);
+
+/* Enable synapse types (bit mask of edlut_synapse); replaces
+ * SetEnabledSynapsis_GPU2 (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:232-262). */
+int edlut_population_enable_synapses(edlut_population *pop, uint32_t mask);
+
+/*
+ * Queue input spikes for the next update, in the host's delivery order.
+ * Replaces ProcessInputSpike (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:69-73):
+ * before the next update, state[ndiff+type][target] += w is applied once per
+ * event, in the order given, like the CPU model's direct increments
+ * (AdExTimeDrivenModel.cpp:74-79).  type is 0 (exc), 1 (inh) or 2 (nmda).
+ */
+int edlut_population_push_inputs(edlut_population *pop, const int32_t *target,
+                                 const uint8_t *type, const float *weight, int n);
+
+/* Total external current (pA) of every neuron; replaces ProcessInputCurrent
+ * (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:75-82).  Enables EXT_I. */
+int edlut_population_set_ext_current(edlut_population *pop, const float *current);
+
+/*
+ * Advance every neuron by one step (update index `step`, counted from 1) and
+ * return the number of spikes.  Replaces UpdateState
+ * (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:90-112); synchronous like it.
+ * Spiking neuron indices are compacted on the device in ascending order and
+ * read with edlut_population_get_spikes (replaces the host bool[] scan of
+ * TimeDrivenInternalSpike.cpp:132-160).
+ */
+int edlut_population_update(edlut_population *pop, int64_t step, int *n_spikes);
+int edlut_population_get_spikes(const edlut_population *pop, int32_t *idx, int cap);
+
+/* Copy one state variable to / from host memory (n_neurons floats); replaces
+ * the monitored-state copies (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:99-103). */
+int edlut_population_get_state(const edlut_population *pop, int var, float *out);
+int edlut_population_set_state(edlut_population *pop, int var, const float *in);
+
+/* Device pointer of the SoA state block (n_vars rows of `stride` floats). */
+int edlut_population_device_state(const edlut_population *pop, float **state);
+
+/* Checksums (FNV-1a over the bytes) of the look-up tables the kernels use:
+ * [0] ExponentialTable, [1] g_nmda_inf table, [2] refractory table. */
+int edlut_population_lut_checksums(const edlut_population *pop, uint64_t *out3);
+
+/* ---- network mode: device-resident spike delivery ----------------------- */
+/*
+ * Spikes stay on the device: each source keeps a ring of compacted spike
+ * lists, and before update k every event of a spike emitted at update
+ * s = k-1-d through a synapse of delay d is folded into its target's
+ * conductance, in the order the reference's event queue delivers them
+ * (ascending delay, then source model, then source index, then target index;
+ * Kernel/src/spike/PropagatedSpikeGroup.cpp:66-108,
+ * Kernel/src/spike/Network.cpp:45-68,
+ * Kernel/src/simulation/EventQueue.cpp:121-198).
+ *
+ * Synapse arrays use a per-(source, delay) row pointer: synapses of source s
+ * with delay d (1..max_delay steps) occupy [row_ptr[s*max_delay+d-1],
+ * row_ptr[s*max_delay+d]), ascending target; tgt_type packs
+ * target | type << 30.
+ */
+int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **out);
+int edlut_network_destroy(edlut_network *net);
+/* The population itself as a spike source (recurrent synapses). */
+int edlut_network_add_recurrent(edlut_network *net, const int64_t *row_ptr,
+                                const uint32_t *tgt_type, const float *weight);
+/* A Poisson generator device (PoissonGeneratorDeviceVector.cpp:56-70) of
+ * n_src sources with their RandomGenerator states (RandomGenerator.h:61-107). */
+int edlut_network_add_poisson(edlut_network *net, int n_src, const uint32_t *lcg_state,
+                              float rate_hz, const int64_t *row_ptr, const uint32_t *tgt_type,
+                              const float *weight);
+/* Counter-based external drive: per neuron and step, k events of weight w_ext
+ * into the excitatory conductance, k = #{j : hash(seed,step,i) < thr[j]},
+ * folded after the interval's recurrent events. */
+int edlut_network_set_external(edlut_network *net, int n_thr, const uint32_t *thresholds,
+                               float w_ext, uint64_t seed);
+/* Enqueue n_steps updates starting at update index first_step (asynchronous;
+ * spikes of the population are appended to the device spike log). */
+int edlut_network_run(edlut_network *net, int64_t first_step, int n_steps);
+int edlut_network_synchronize(edlut_network *net);
+/* Spike log (step, index) pairs, ascending within a step; returns the count. */
+int edlut_network_spike_log(edlut_network *net, int32_t *step, int32_t *idx, int64_t cap,
+                            int64_t *n_out);
+int edlut_network_clear_log(edlut_network *net);
+/* Reserve HIP events to time each update-kernel launch of runs of up to
+ * max_steps steps (0 switches timing off). */
+int edlut_network_set_timing(edlut_network *net, int64_t max_steps);
+/* Per-launch timing of the update kernel over the last run (HIP events on the
+ * library's stream): total milliseconds and launch count. */
+int edlut_network_update_timing(edlut_network *net, double *ms, int64_t *launches);
+
+/*
+ * Host helper: sort an edge list into the row layout above, in the order of
+ * Network::FindOutConnections / qsort_inters (Kernel/src/spike/Network.cpp:
+ * 45-123: source, then delay, then target).  delay is in steps (1..max_delay);
+ * row_ptr has n_src*max_delay+1 entries; tgt_type and w_out have n_edges.
+ */
+int edlut_build_out_csr(int64_t n_src, int64_t n_tgt, int max_delay, int64_t n_edges,
+                        const int32_t *src, const int32_t *tgt, const int32_t *delay,
+                        const uint8_t *type, const float *w, int64_t *row_ptr,
+                        uint32_t *tgt_type, float *w_out);
+
+/* ---- misc --------------------------------------------------------------- */
+/* Name of the compiled device architecture, e.g. "gfx950". */
+const char *edlut_device_arch(void);
+/* Counter-based hash used by the external drive (host copy, same definition). */
+uint32_t edlut_drive_hash(uint64_t seed, int64_t step, uint32_t neuron);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* EDLUT_AMD_H_ */

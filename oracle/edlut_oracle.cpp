@@ -1,0 +1,723 @@
+/*
+ * edlut_oracle.cpp -- CPU restatement of EDLUT's time-driven neuron update.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation.  Only
+ * tests/, the smoke entry point and bench.py's CPU-baseline leg use it.
+ *
+ * Layout follows the reference's CPU path: array-of-structures state
+ * (VectorNeuronState.cpp:244-276, index*NumberOfVariables+var), one loop over
+ * neurons per integration step, spikes appended in ascending index order.
+ * Float arithmetic keeps the reference's association; build with
+ * -ffp-contract=off so that no multiply-add is fused.
+ *
+ * Citations are relative to the reference repository root (Kernel/...).
+ */
+#include "edlut_oracle.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+#include <algorithm>
+
+namespace {
+
+/* ------------------------------------------------------------------------ */
+/* ExponentialTable: Kernel/include/simulation/ExponentialTable.h:60-103,    */
+/* Kernel/src/simulation/ExponentialTable.cpp:23-32.                        */
+/* 16384 entries on [-20, 20]; entry i holds exp of the float grid point     */
+/* Min + ((Max-Min)*i)/(TableSize-1), evaluated by the double-precision      */
+/* exp and stored as float.  GetResult truncates (x-Min)*aux to an int      */
+/* index, returns exp(x) above Max and 0 below Min.                          */
+/* ------------------------------------------------------------------------ */
+const float kExpMin = -20.0f;
+const float kExpMax = 20.0f;
+const int kExpSize = 1024 * 16;
+const float kExpAux = (kExpSize - 1) / (kExpMax - kExpMin);
+
+struct ExpTable {
+    float v[kExpSize];
+    ExpTable() {
+        for (int i = 0; i < kExpSize; i++) {
+            float exponent = kExpMin + ((kExpMax - kExpMin) * i) / (kExpSize - 1);
+            v[i] = (float)::exp((double)exponent);
+        }
+    }
+};
+const ExpTable &exp_table() {
+    static ExpTable t;
+    return t;
+}
+
+inline float exp_lut(float value) {
+    if (value >= kExpMin && value <= kExpMax) {
+        int position = int((value - kExpMin) * kExpAux);
+        return exp_table().v[position];
+    } else if (value > kExpMax) {
+        /* In the model translation units `using namespace std` is active
+         * (Kernel/include/neuron_model/NeuronModel.h), so exp(float) is the
+         * float overload. */
+        return std::exp(value);
+    }
+    return 0.0f;
+}
+
+/* float -> int conversion as compiled for x86-64 (cvttss2si): NaN and values
+ * outside the int32 range give INT_MIN.  Written out explicitly because the
+ * C++ conversion is undefined there and the reference's results depend on
+ * what the x86 instruction returns. */
+inline int trunc_i32_x86(float x) {
+    if (!(x < 2147483648.0f) || x < -2147483648.0f) return (int)0x80000000u;
+    return (int)x;
+}
+
+/* g_nmda_inf table: AdExTimeDrivenModel.cpp:27-47 (same in LIF/HH/granule),
+ * TableSizeNMDA = 256 (TimeDrivenNeuronModel.h:103).  exp(float) resolves to
+ * the float overload (using namespace std in scope). */
+const int kNmdaSize = 256;
+struct NmdaTable {
+    float v[kNmdaSize];
+    float aux;
+    void build(float e_exc, float e_inh) {
+        aux = (kNmdaSize - 1) / (e_exc - e_inh);
+        for (int i = 0; i < kNmdaSize; i++) {
+            float V = e_inh + ((e_exc - e_inh) * i) / (kNmdaSize - 1);
+            v[i] = 1.0f / (1.0f + std::exp(-0.062f * V) * (1.2f / 3.57f));
+        }
+    }
+    float get(float V_m, float e_inh) const {
+        int position = trunc_i32_x86((V_m - e_inh) * aux);
+        if (position < 0) position = 0;
+        else if (position > (kNmdaSize - 1)) position = kNmdaSize - 1;
+        return v[position];
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* Real-type traits: float = bit-exact restatement, double = FP64 oracle.   */
+/* ------------------------------------------------------------------------ */
+template <typename R> struct Num;
+template <> struct Num<float> {
+    static float lut_exp(float x) { return exp_lut(x); }
+    static float decay(float elapsed, float inv_tau) { return expf(-elapsed * inv_tau); }
+    /* inv_x = 1./x computed in double and stored in a float member
+     * (AdExTimeDrivenModel.cpp:347,362,405,413). */
+    static float inv(float x) { return (float)(1.0 / (double)x); }
+};
+template <> struct Num<double> {
+    static double lut_exp(double x) { return std::exp(x); }
+    static double decay(double elapsed, double inv_tau) { return std::exp(-elapsed * inv_tau); }
+    static double inv(double x) { return 1.0 / x; }
+};
+
+/* ------------------------------------------------------------------------ */
+/* Population: state and bookkeeping shared by every model.                 */
+/* ------------------------------------------------------------------------ */
+template <typename R> struct PopBase {
+    int N = 0;
+    int nvars = 0;
+    int ndiff = 0;
+    std::vector<R> state;          /* AoS: index*nvars + var */
+    std::vector<double> last_spike; /* LastSpikeTime, seconds */
+    std::vector<R> initial;
+    float dt_s = 1e-4f;
+    R dt_model = 0;                /* elapsedTimeInNeuronModelScale */
+    bool EXC = false, INH = false, NMDA = false, EXT_I = false;
+    std::vector<int32_t> spikes;   /* InternalSpikeIndexs of the last update */
+
+    void init_states(const R *init) {
+        state.assign((size_t)N * nvars, 0);
+        for (int i = 0; i < N; i++)
+            for (int j = 0; j < nvars; j++) state[(size_t)i * nvars + j] = init[j];
+        initial.assign(init, init + nvars);
+        /* VectorNeuronState.cpp:272-274 */
+        last_spike.assign(N, 100.0);
+    }
+    R *at(int i) { return &state[(size_t)i * nvars]; }
+    /* VectorNeuronState.cpp:221-224 */
+    void add_elapsed(int i) { last_spike[i] += (double)dt_s; }
+    /* VectorNeuronState.cpp:215-218 */
+    void fired(int i) { last_spike[i] = 0; }
+    /* VectorNeuronState.cpp:302-306 via TimeDrivenNeuronModel.cpp:75-88 */
+    void reset(int i) {
+        for (int j = 0; j < nvars; j++) state[(size_t)i * nvars + j] = initial[j];
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* AdEx: Kernel/src/neuron_model/AdExTimeDrivenModel.cpp                    */
+/* state: V, w | g_exc, g_inh, g_nmda, ext_I                                 */
+/* ------------------------------------------------------------------------ */
+template <typename R> struct AdEx : PopBase<R> {
+    R a, b, thr_slo_fac, inv_thr_slo_fac, v_thr, tau_w, inv_tau_w, e_exc, e_inh, e_reset,
+        e_leak, g_leak, c_m, inv_c_m, tau_exc, inv_tau_exc, tau_inh, inv_tau_inh, tau_nmda,
+        inv_tau_nmda;
+    NmdaTable nmda;
+    R cond[3]; /* conductance decay factors, elapsed-time index 0 */
+
+    /* Default parameters: AdExTimeDrivenModel.cpp:449-467 (order of p[]). */
+    void set_params(const float *p) {
+        a = p[0]; b = p[1]; thr_slo_fac = p[2]; v_thr = p[3]; tau_w = p[4]; e_exc = p[5];
+        e_inh = p[6]; e_reset = p[7]; e_leak = p[8]; g_leak = p[9]; c_m = p[10];
+        tau_exc = p[11]; tau_inh = p[12]; tau_nmda = p[13];
+        inv_thr_slo_fac = Num<R>::inv(thr_slo_fac);
+        inv_tau_w = Num<R>::inv(tau_w);
+        inv_c_m = Num<R>::inv(c_m);
+        inv_tau_exc = Num<R>::inv(tau_exc);
+        inv_tau_inh = Num<R>::inv(tau_inh);
+        inv_tau_nmda = Num<R>::inv(tau_nmda);
+        nmda.build(p[5], p[6]);
+        this->nvars = 6;
+        this->ndiff = 2;
+    }
+    R g_nmda_inf(R V) const {
+        if (sizeof(R) == sizeof(float)) return (R)nmda.get((float)V, (float)e_inh);
+        return 1.0 / (1.0 + std::exp(-0.062 * (double)V) * (1.2 / 3.57));
+    }
+    /* AdExTimeDrivenModel.cpp:136-148 */
+    void init_states() {
+        R init[6] = {e_leak, 0, 0, 0, 0, 0};
+        PopBase<R>::init_states(init);
+    }
+    /* AdExTimeDrivenModel.cpp:247-254 */
+    void calc_cond(R elapsed) {
+        cond[0] = Num<R>::decay(elapsed, inv_tau_exc);
+        cond[1] = Num<R>::decay(elapsed, inv_tau_inh);
+        cond[2] = Num<R>::decay(elapsed, inv_tau_nmda);
+    }
+    /* AdExTimeDrivenModel.cpp:170-218 */
+    void deriv(const R *S, R *D, int /*index*/, R /*elapsed*/) const {
+        if (S[0] <= v_thr + 6 * thr_slo_fac) {
+            R current = 0.0;
+            if (this->EXC) current += S[2] * (e_exc - S[0]);
+            if (this->INH) current += S[3] * (e_inh - S[0]);
+            if (this->NMDA) {
+                R ginf = g_nmda_inf(S[0]);
+                current += S[4] * ginf * (e_exc - S[0]);
+            }
+            current += S[5];
+            D[0] = (g_leak * (e_leak - S[0]) +
+                    g_leak * thr_slo_fac * Num<R>::lut_exp((S[0] - v_thr) * inv_thr_slo_fac) - S[1] +
+                    current) * inv_c_m;
+            D[1] = (a * (S[0] - e_leak) - S[1]) * inv_tau_w;
+        } else if (S[0] <= 0.0f) {
+            R Vm = v_thr + 6 * thr_slo_fac;
+            R current = 0.0;
+            if (this->EXC) current += S[2] * (e_exc - Vm);
+            if (this->INH) current += S[3] * (e_inh - Vm);
+            if (this->NMDA) {
+                R ginf = g_nmda_inf(Vm);
+                current += S[4] * ginf * (e_exc - Vm);
+            }
+            current += S[5];
+            D[0] = (g_leak * (e_leak - Vm) +
+                    g_leak * thr_slo_fac * Num<R>::lut_exp((Vm - v_thr) * inv_thr_slo_fac) - S[1] +
+                    current) * inv_c_m;
+            D[1] = (a * (S[0] - e_leak) - S[1]) * inv_tau_w;
+        } else {
+            D[0] = 0;
+            D[1] = 0;
+        }
+    }
+    /* AdExTimeDrivenModel.cpp:220-245 */
+    void decay(R *S) const {
+        const R limit = (R)1e-9f;
+        if (this->EXC) { if (S[2] < limit) S[2] = 0.0f; else S[2] *= cond[0]; }
+        if (this->INH) { if (S[3] < limit) S[3] = 0.0f; else S[3] *= cond[1]; }
+        if (this->NMDA) { if (S[4] < limit) S[4] = 0.0f; else S[4] *= cond[2]; }
+    }
+    /* AdExTimeDrivenModel.cpp:159-168 */
+    bool spike(R /*prev_V*/, R *S, int index) {
+        if (S[0] > 0.0f) {
+            S[0] = e_reset;
+            S[1] += b;
+            this->fired(index);
+            return true;
+        }
+        return false;
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* LIF: Kernel/src/neuron_model/LIFTimeDrivenModel.cpp                        */
+/* state: V | g_exc, g_inh, g_nmda, ext_I                                     */
+/* ------------------------------------------------------------------------ */
+template <typename R> struct LIF : PopBase<R> {
+    R e_exc, e_inh, e_leak, v_thr, c_m, inv_c_m, tau_exc, inv_tau_exc, tau_inh, inv_tau_inh,
+        tau_ref, g_leak, tau_nmda, inv_tau_nmda;
+    NmdaTable nmda;
+    R cond[3];
+    const float time_scale = 1000.0f; /* MilisecondScale (NeuronModel.h:71) */
+
+    /* Default parameters: LIFTimeDrivenModel.cpp:435-450 (order of p[]). */
+    void set_params(const float *p) {
+        e_exc = p[0]; e_inh = p[1]; e_leak = p[2]; v_thr = p[3]; c_m = p[4];
+        tau_exc = p[5]; tau_inh = p[6]; tau_ref = p[7]; g_leak = p[8]; tau_nmda = p[9];
+        inv_c_m = Num<R>::inv(c_m);
+        inv_tau_exc = Num<R>::inv(tau_exc);
+        inv_tau_inh = Num<R>::inv(tau_inh);
+        inv_tau_nmda = Num<R>::inv(tau_nmda);
+        nmda.build(p[0], p[1]);
+        this->nvars = 5;
+        this->ndiff = 1;
+    }
+    R g_nmda_inf(R V) const {
+        if (sizeof(R) == sizeof(float)) return (R)nmda.get((float)V, (float)e_inh);
+        return 1.0 / (1.0 + std::exp(-0.062 * (double)V) * (1.2 / 3.57));
+    }
+    /* LIFTimeDrivenModel.cpp:135-148 */
+    void init_states() {
+        R init[5] = {e_leak, 0, 0, 0, 0};
+        PopBase<R>::init_states(init);
+    }
+    void calc_cond(R elapsed) {
+        cond[0] = Num<R>::decay(elapsed, inv_tau_exc);
+        cond[1] = Num<R>::decay(elapsed, inv_tau_inh);
+        cond[2] = Num<R>::decay(elapsed, inv_tau_nmda);
+    }
+    /* LIFTimeDrivenModel.cpp:170-202.  The refractory test works on the
+     * double LastSpikeTime scaled to ms (double * float -> double). */
+    void deriv(const R *S, R *D, int index, R elapsed_time) const {
+        R current = 0.0;
+        if (this->EXC) current += S[1] * (e_exc - S[0]);
+        if (this->INH) current += S[2] * (e_inh - S[0]);
+        if (this->NMDA) {
+            R ginf = g_nmda_inf(S[0]);
+            current += S[3] * ginf * (e_exc - S[0]);
+        }
+        current += S[4];
+        double lst = this->last_spike[index] * time_scale;
+        if (lst > tau_ref) {
+            D[0] = (current + g_leak * (e_leak - S[0])) * inv_c_m;
+        } else if ((lst + elapsed_time) > tau_ref) {
+            R fraction = (R)((lst + elapsed_time - tau_ref) / elapsed_time);
+            D[0] = fraction * ((current + g_leak * (e_leak - S[0])) * inv_c_m);
+        } else {
+            D[0] = 0;
+        }
+    }
+    /* LIFTimeDrivenModel.cpp:204-242 */
+    void decay(R *S) const {
+        const R limit = (R)1e-9f;
+        if (this->EXC) { if (S[1] < limit) S[1] = 0.0f; else S[1] *= cond[0]; }
+        if (this->INH) { if (S[2] < limit) S[2] = 0.0f; else S[2] *= cond[1]; }
+        if (this->NMDA) { if (S[3] < limit) S[3] = 0.0f; else S[3] *= cond[2]; }
+    }
+    /* LIFTimeDrivenModel.cpp:158-168 */
+    bool spike(R /*prev_V*/, R *S, int index) {
+        if (S[0] > v_thr) {
+            S[0] = e_leak;
+            this->fired(index);
+            return true;
+        }
+        return false;
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* Fixed-step integrators over the whole population.                        */
+/* ------------------------------------------------------------------------ */
+const int kMaxVars = 32;
+
+/* Euler: Kernel/include/integration_method/Euler.h:123-146, decay factors
+ * for elapsed = dt (Euler.h:230-234). */
+template <typename M, typename R> void step_euler(M &m) {
+    R D[kMaxVars];
+    const R dt = m.dt_model;
+    for (int index = 0; index < m.N; index++) {
+        R *S = m.at(index);
+        R prev = S[0];
+        m.deriv(S, D, index, dt);
+        for (int j = 0; j < m.ndiff; j++) S[j] += dt * D[j];
+        m.decay(S);
+        m.add_elapsed(index);
+        if (m.spike(prev, S, index)) m.spikes.push_back(index);
+    }
+}
+
+/* RK2 (Heun): Kernel/include/integration_method/RK2.h:133-165, decay factors
+ * for elapsed = dt (RK2.h:257-261). */
+template <typename M, typename R> void step_rk2(M &m) {
+    R D1[kMaxVars], D2[kMaxVars];
+    const R dt = m.dt_model;
+    for (int index = 0; index < m.N; index++) {
+        R *S = m.at(index);
+        R prev = S[0];
+        m.deriv(S, D1, index, dt);
+        for (int j = 0; j < m.ndiff; j++) S[j] += D1[j] * dt;
+        m.decay(S);
+        m.deriv(S, D2, index, dt);
+        for (int j = 0; j < m.ndiff; j++) S[j] += (-D1[j] + D2[j]) * dt * 0.5f;
+        m.add_elapsed(index);
+        if (m.spike(prev, S, index)) m.spikes.push_back(index);
+    }
+}
+
+/* RK4: Kernel/include/integration_method/RK4.h:167-231, decay factors for
+ * elapsed = dt*0.5 applied twice (RK4.h:354-358). */
+template <typename M, typename R> void step_rk4(M &m) {
+    R A[kMaxVars], D1[kMaxVars], D2[kMaxVars], D3[kMaxVars], D4[kMaxVars];
+    const R dt = m.dt_model;
+    const R dt_0_5 = dt * 0.5f;
+    const R dt_0_16 = dt * 0.166666666667f;
+    for (int index = 0; index < m.N; index++) {
+        R *S = m.at(index);
+        R prev = S[0];
+        m.deriv(S, D1, index, dt);
+        for (int j = 0; j < m.ndiff; j++) A[j] = S[j] + D1[j] * dt_0_5;
+        for (int j = m.ndiff; j < m.nvars; j++) A[j] = S[j];
+        m.decay(A);
+        m.deriv(A, D2, index, dt);
+        for (int j = 0; j < m.ndiff; j++) A[j] = S[j] + D2[j] * dt_0_5;
+        m.deriv(A, D3, index, dt);
+        for (int j = 0; j < m.ndiff; j++) A[j] = S[j] + D3[j] * dt;
+        m.decay(A);
+        m.deriv(A, D4, index, dt);
+        for (int j = 0; j < m.ndiff; j++)
+            S[j] += (D1[j] + 2.0f * (D2[j] + D3[j]) + D4[j]) * dt_0_16;
+        for (int j = m.ndiff; j < m.nvars; j++) S[j] = A[j];
+        m.add_elapsed(index);
+        if (m.spike(prev, S, index)) m.spikes.push_back(index);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Type-erased population handle.                                           */
+/* ------------------------------------------------------------------------ */
+struct PopIface {
+    virtual ~PopIface() {}
+    virtual int nvars() = 0;
+    virtual int ndiff() = 0;
+    virtual int size() = 0;
+    virtual void get_state(int var, double *out) = 0;
+    virtual void set_state(int var, const double *in) = 0;
+    virtual void add_input(int target, int type, float w) = 0;
+    virtual void set_ext(const float *I) = 0;
+    virtual int update(int64_t step, int32_t *spikes) = 0;
+    virtual bool exc_enabled() = 0;
+    virtual float dt_s() = 0;
+};
+
+template <typename M, typename R> struct Pop : PopIface {
+    M m;
+    int integrator = ORC_EULER;
+
+    Pop(int integ, const float *params, float dt_s, int n, uint32_t mask) {
+        integrator = integ;
+        m.set_params(params);
+        m.N = n;
+        m.dt_s = dt_s;
+        /* IntegrationMethodFast.h:191-193: seconds * time scale, in float. */
+        float dt_model = dt_s * 1000.0f;
+        m.dt_model = (R)dt_model;
+        m.EXC = (mask & 1u) != 0;
+        m.INH = (mask & 2u) != 0;
+        m.NMDA = (mask & 4u) != 0;
+        m.EXT_I = (mask & 8u) != 0;
+        m.init_states();
+        if (integ == ORC_RK4) m.calc_cond((R)(dt_model * 0.5f));
+        else m.calc_cond((R)dt_model);
+    }
+    int nvars() override { return m.nvars; }
+    int ndiff() override { return m.ndiff; }
+    int size() override { return m.N; }
+    void get_state(int var, double *out) override {
+        for (int i = 0; i < m.N; i++) out[i] = (double)m.state[(size_t)i * m.nvars + var];
+    }
+    void set_state(int var, const double *in) override {
+        for (int i = 0; i < m.N; i++) m.state[(size_t)i * m.nvars + var] = (R)in[i];
+    }
+    /* AdExTimeDrivenModel.cpp:74-79: one add per input, in arrival order. */
+    void add_input(int target, int type, float w) override {
+        m.state[(size_t)target * m.nvars + m.ndiff + type] += (R)w;
+    }
+    /* AdExTimeDrivenModel.cpp:82-89: ext_I holds the total input current. */
+    void set_ext(const float *I) override {
+        for (int i = 0; i < m.N; i++) m.state[(size_t)i * m.nvars + m.ndiff + 3] = (R)I[i];
+    }
+    bool exc_enabled() override { return m.EXC; }
+    float dt_s() override { return m.dt_s; }
+    /* AdExTimeDrivenModel.cpp:92-101 + TimeDrivenNeuronModel.cpp:75-88. */
+    int update(int64_t /*step*/, int32_t *spikes) override {
+        m.spikes.clear();
+        if (integrator == ORC_EULER) step_euler<M, R>(m);
+        else if (integrator == ORC_RK2) step_rk2<M, R>(m);
+        else step_rk4<M, R>(m);
+        /* A NaN membrane potential resets that neuron to its initial state. */
+        for (int i = 0; i < m.N; i++) {
+            R v = m.state[(size_t)i * m.nvars];
+            if (v != v) m.reset(i);
+        }
+        int n = (int)m.spikes.size();
+        if (spikes) std::copy(m.spikes.begin(), m.spikes.end(), spikes);
+        return n;
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* RandomGenerator: Kernel/include/simulation/RandomGenerator.h:35-121,      */
+/* Kernel/src/simulation/RandomGenerator.cpp:21-27.  Both recurrences are    */
+/* taken modulo 2^31 (the & 0x7fffffff mask).                                */
+/* ------------------------------------------------------------------------ */
+inline uint32_t lcg_next(uint32_t x) {
+    return (uint32_t)(((uint64_t)x * 1103515245ull + 12345ull) & 0x7fffffffull);
+}
+inline uint32_t seed_next(uint32_t g) {
+    return (uint32_t)(((uint64_t)g * 765426813ull + 54321ull) & 0x7fffffffull);
+}
+inline float lcg_frand(uint32_t x) {
+    /* ((float)next_element)/MAX_RAND, MAX_RAND converted to float. */
+    return ((float)x) / 2147483647;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Network: canonical delivery order.                                        */
+/* A spike emitted at update s through a synapse of delay d steps arrives     */
+/* at t_s + d*dt (double), which falls strictly between updates s+d and      */
+/* s+d+1 because float(dt) < dt (TimeEventUpdateNeurons.cpp:78,              */
+/* Network.cpp:1311-1317, EventQueue.cpp:121-198 with TIMEEVENT priority      */
+/* above PROPAGATEDSPIKE, Event.h:37).  Within one inter-update interval the  */
+/* arrival times rise with falling emission step, i.e. with ascending d.     */
+/* Each group lists its sources in ascending neuron index                     */
+/* (TimeDrivenInternalSpike.cpp:109-160) and each source's synapses of that   */
+/* delay in ascending target index (Network.cpp:45-68).  Groups of equal     */
+/* arrival time from different source models are taken in the order the     */
+/* sources were added.                                                       */
+/* ------------------------------------------------------------------------ */
+struct Source {
+    bool self = true;
+    int n = 0;
+    std::vector<int64_t> row_ptr;
+    std::vector<uint32_t> tgt_type;
+    std::vector<float> w;
+    /* Poisson device */
+    std::vector<uint32_t> lcg;
+    float rate = 0.0f;
+    /* spikes per step: ring indexed by step % ring */
+    std::vector<std::vector<int32_t>> ring;
+};
+
+struct Net {
+    PopIface *pop = nullptr;
+    int max_delay = 1;
+    int ring = 2;
+    std::vector<Source> sources;
+    /* external drive */
+    std::vector<uint32_t> thr;
+    float w_ext = 0.0f;
+    uint64_t ext_seed = 0;
+    bool has_ext = false;
+};
+
+/* Counter-based external drive (the synthetic workloads' Poisson drive; not a
+ * reference feature): u = mix(neuron*0x9E3779B9 + key(seed, step)), where mix
+ * is a 32-bit multiply-xorshift finaliser. */
+inline uint32_t mix32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352du;
+    x ^= x >> 15;
+    x *= 0x846ca68bu;
+    x ^= x >> 16;
+    return x;
+}
+uint32_t drive_hash(uint64_t seed, int64_t step, uint32_t neuron) {
+    uint32_t key = mix32((uint32_t)seed ^ mix32((uint32_t)(seed >> 32) ^ mix32((uint32_t)step)));
+    return mix32(neuron * 0x9E3779B9u + key);
+}
+
+PopIface *make_pop(int model, int integ, const float *p, float dt_s, int n, uint32_t mask,
+                   bool fp64) {
+    if (model == ORC_ADEX) {
+        if (fp64) return new Pop<AdEx<double>, double>(integ, p, dt_s, n, mask);
+        return new Pop<AdEx<float>, float>(integ, p, dt_s, n, mask);
+    }
+    if (model == ORC_LIF) {
+        if (fp64) return new Pop<LIF<double>, double>(integ, p, dt_s, n, mask);
+        return new Pop<LIF<float>, float>(integ, p, dt_s, n, mask);
+    }
+    return nullptr;
+}
+
+}  // namespace
+
+/* ======================================================================== */
+extern "C" {
+
+void orc_exp_table(float *out) { std::memcpy(out, exp_table().v, sizeof(float) * kExpSize); }
+
+void orc_exp_lut(const float *x, float *out, int n) {
+    for (int i = 0; i < n; i++) out[i] = exp_lut(x[i]);
+}
+
+void orc_nmda_table(float e_exc, float e_inh, float *out) {
+    NmdaTable t;
+    t.build(e_exc, e_inh);
+    std::memcpy(out, t.v, sizeof(float) * kNmdaSize);
+}
+
+void orc_lcg_seeds(uint32_t seed, int n, uint32_t *out) {
+    /* RandomGenerator::srand then n constructions (RandomGenerator.cpp:21-24). */
+    uint32_t g = seed;
+    for (int i = 0; i < n; i++) {
+        out[i] = g;
+        g = seed_next(g);
+    }
+}
+
+void orc_lcg_frand(uint32_t x0, int n, float *out) {
+    uint32_t x = x0;
+    for (int i = 0; i < n; i++) {
+        x = lcg_next(x);
+        out[i] = lcg_frand(x);
+    }
+}
+
+uint32_t orc_drive_hash(uint64_t seed, int64_t step, uint32_t neuron) {
+    return drive_hash(seed, step, neuron);
+}
+
+void *orc_pop_create(int model, int integrator, int /*bdf_order*/, const float *params,
+                     int /*n_params*/, float dt_s, int n_neurons, uint32_t syn_mask, int fp64) {
+    return make_pop(model, integrator, params, dt_s, n_neurons, syn_mask, fp64 != 0);
+}
+
+void orc_pop_destroy(void *pop) { delete (PopIface *)pop; }
+int orc_pop_nvars(void *pop) { return ((PopIface *)pop)->nvars(); }
+void orc_pop_get_state(void *pop, int var, double *out) { ((PopIface *)pop)->get_state(var, out); }
+void orc_pop_set_state(void *pop, int var, const double *in) { ((PopIface *)pop)->set_state(var, in); }
+
+void orc_pop_add_inputs(void *pop, const int32_t *target, const uint8_t *type, const float *w,
+                        int n) {
+    PopIface *p = (PopIface *)pop;
+    for (int i = 0; i < n; i++) p->add_input(target[i], type[i], w[i]);
+}
+
+void orc_pop_set_ext_current(void *pop, const float *current) {
+    ((PopIface *)pop)->set_ext(current);
+}
+
+int orc_pop_update(void *pop, int64_t step, int32_t *spikes) {
+    return ((PopIface *)pop)->update(step, spikes);
+}
+
+void *orc_net_create(void *pop, int max_delay) {
+    Net *n = new Net;
+    n->pop = (PopIface *)pop;
+    n->max_delay = max_delay;
+    n->ring = max_delay + 2;
+    return n;
+}
+
+void orc_net_destroy(void *net) { delete (Net *)net; }
+
+static int add_source(Net *n, Source &s, const int64_t *row_ptr, const uint32_t *tgt_type,
+                      const float *w) {
+    size_t nrow = (size_t)s.n * n->max_delay + 1;
+    s.row_ptr.assign(row_ptr, row_ptr + nrow);
+    int64_t nsyn = s.row_ptr.back();
+    s.tgt_type.assign(tgt_type, tgt_type + nsyn);
+    s.w.assign(w, w + nsyn);
+    s.ring.assign(n->ring, std::vector<int32_t>());
+    n->sources.push_back(s);
+    return (int)n->sources.size() - 1;
+}
+
+int orc_net_add_source_self(void *net, const int64_t *row_ptr, const uint32_t *tgt_type,
+                            const float *w) {
+    Net *n = (Net *)net;
+    Source s;
+    s.self = true;
+    s.n = n->pop->size();
+    return add_source(n, s, row_ptr, tgt_type, w);
+}
+
+int orc_net_add_source_poisson(void *net, int n_src, const uint32_t *lcg_state, float rate_hz,
+                               const int64_t *row_ptr, const uint32_t *tgt_type,
+                               const float *w) {
+    Net *n = (Net *)net;
+    Source s;
+    s.self = false;
+    s.n = n_src;
+    s.lcg.assign(lcg_state, lcg_state + n_src);
+    s.rate = rate_hz;
+    return add_source(n, s, row_ptr, tgt_type, w);
+}
+
+void orc_net_set_external(void *net, int n_thr, const uint32_t *thresholds, float w_ext,
+                          uint64_t seed) {
+    Net *n = (Net *)net;
+    n->thr.assign(thresholds, thresholds + n_thr);
+    n->w_ext = w_ext;
+    n->ext_seed = seed;
+    n->has_ext = n_thr > 0;
+}
+
+int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_step,
+                    int32_t *spike_idx, int64_t cap) {
+    Net *n = (Net *)net;
+    PopIface *pop = n->pop;
+    const int N = pop->size();
+    const int D = n->max_delay;
+    std::vector<int32_t> spk(N);
+    int64_t total = 0;
+    for (int64_t k = first_step; k < first_step + n_steps; k++) {
+        /* 1. delivery of every event consumed by update k */
+        for (int d = 1; d <= D; d++) {
+            int64_t s = k - 1 - d;
+            if (s < 1) continue;
+            for (Source &src : n->sources) {
+                const std::vector<int32_t> &list = src.ring[s % n->ring];
+                for (int32_t id : list) {
+                    int64_t b = src.row_ptr[(size_t)id * D + d - 1];
+                    int64_t e = src.row_ptr[(size_t)id * D + d];
+                    for (int64_t q = b; q < e; q++) {
+                        uint32_t tt = src.tgt_type[q];
+                        pop->add_input((int)(tt & 0x3fffffffu), (int)(tt >> 30), src.w[q]);
+                    }
+                }
+            }
+        }
+        /* 2. external drive: after the recurrent events of the interval */
+        if (n->has_ext) {
+            for (int i = 0; i < N; i++) {
+                uint32_t u = drive_hash(n->ext_seed, k, (uint32_t)i);
+                int cnt = 0;
+                while (cnt < (int)n->thr.size() && u < n->thr[cnt]) cnt++;
+                for (int c = 0; c < cnt; c++) pop->add_input(i, 0, n->w_ext);
+            }
+        }
+        /* 3. Poisson devices at update k (PoissonGeneratorDeviceVector.cpp:56-70):
+         * a source fires when frequency * step (float * double) exceeds its
+         * next frand(); frand is not drawn when the frequency is not positive. */
+        for (Source &src : n->sources) {
+            if (src.self) continue;
+            std::vector<int32_t> &list = src.ring[k % n->ring];
+            list.clear();
+            const double step_s = (double)pop->dt_s();
+            for (int i = 0; i < src.n; i++) {
+                if (src.rate > 0.0f) {
+                    src.lcg[i] = lcg_next(src.lcg[i]);
+                    if ((double)src.rate * step_s > (double)lcg_frand(src.lcg[i]))
+                        list.push_back(i);
+                }
+            }
+        }
+        /* 4. population update k */
+        int ns = pop->update(k, spk.data());
+        for (Source &src : n->sources) {
+            if (!src.self) continue;
+            src.ring[k % n->ring].assign(spk.begin(), spk.begin() + ns);
+        }
+        for (int q = 0; q < ns; q++) {
+            if (total < cap) {
+                spike_step[total] = (int32_t)k;
+                spike_idx[total] = spk[q];
+            }
+            total++;
+        }
+    }
+    return total > cap ? -1 : total;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+/*
+ * edlut_oracle.h -- CPU restatement of EDLUT's time-driven neuron update.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP
+ * implementation in edlut_amd/ is compared against.  Only tests/, the smoke
+ * entry point and the CPU-baseline leg of bench.py may load it; the product
+ * path never calls it and has no CPU fallback.
+ *
+ * Every routine restates the reference's CPU algorithm (cited file:line,
+ * relative to the reference's repository root) with the reference's own
+ * floating-point association, look-up tables and integer indexing, so that a
+ * float build of this library reproduces the reference bit for bit.  The same
+ * code instantiated with double arithmetic and exact exponentials is the FP64
+ * oracle used to state each model's discretisation error.
+ *
+ * Plain C ABI so that Python (ctypes) and C++ test drivers can call it.
+ */
+#ifndef EDLUT_ORACLE_H_
+#define EDLUT_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Model and integrator identifiers (same numbering as include/edlut_amd.h). */
+enum { ORC_LIF = 0, ORC_ADEX = 1, ORC_HH = 2, ORC_GRANULE = 3, ORC_PURKINJE = 4 };
+enum { ORC_EULER = 0, ORC_RK2 = 1, ORC_RK4 = 2, ORC_BDF = 3 };
+
+/* ---- look-up tables ---------------------------------------------------- */
+/* ExponentialTable::LookUpTable (16384 floats). */
+void orc_exp_table(float *out);
+/* ExponentialTable::GetResult(x) for each x. */
+void orc_exp_lut(const float *x, float *out, int n);
+/* g_nmda_inf table (256 floats) for the given reversal potentials. */
+void orc_nmda_table(float e_exc, float e_inh, float *out);
+
+/* ---- RandomGenerator / Poisson device ----------------------------------- */
+/* Seeds handed to n successive RandomGenerator objects after srand(seed). */
+void orc_lcg_seeds(uint32_t seed, int n, uint32_t *out);
+/* n successive frand() values of one generator starting from state x0. */
+void orc_lcg_frand(uint32_t x0, int n, float *out);
+
+/* ---- one population ----------------------------------------------------- */
+/* fp64 != 0 builds the FP64 oracle (double arithmetic, exact exp). */
+void *orc_pop_create(int model, int integrator, int bdf_order, const float *params,
+                     int n_params, float dt_s, int n_neurons, uint32_t syn_mask, int fp64);
+void orc_pop_destroy(void *pop);
+int orc_pop_nvars(void *pop);
+/* State variable `var` of every neuron, widened to double. */
+void orc_pop_get_state(void *pop, int var, double *out);
+void orc_pop_set_state(void *pop, int var, const double *in);
+/* ProcessInputSpike: state[target][n_diff + type] += w, in call order. */
+void orc_pop_add_inputs(void *pop, const int32_t *target, const uint8_t *type,
+                        const float *w, int n);
+/* Total external current (pA) of each neuron (type 3 input). */
+void orc_pop_set_ext_current(void *pop, const float *current);
+/* Integrate one step; writes ascending spiking indices; returns their count. */
+int orc_pop_update(void *pop, int64_t step, int32_t *spikes);
+
+/* ---- network with canonical spike delivery ------------------------------ */
+void *orc_net_create(void *pop, int max_delay);
+void orc_net_destroy(void *net);
+/* Recurrent source: the population itself.  row_ptr has n*max_delay+1
+ * entries; synapses of source s with delay d occupy
+ * [row_ptr[s*max_delay+d-1], row_ptr[s*max_delay+d]) in ascending target
+ * order; tgt_type packs target | type << 30. */
+int orc_net_add_source_self(void *net, const int64_t *row_ptr, const uint32_t *tgt_type,
+                            const float *w);
+/* Poisson generator device (one RandomGenerator per source). */
+int orc_net_add_source_poisson(void *net, int n_src, const uint32_t *lcg_state,
+                               float rate_hz, const int64_t *row_ptr,
+                               const uint32_t *tgt_type, const float *w);
+/* Counter-based binomial external drive into the excitatory conductance. */
+void orc_net_set_external(void *net, int n_thr, const uint32_t *thresholds, float w_ext,
+                          uint64_t seed);
+/* Run n_steps updates starting at update index first_step (>= 1).  Writes
+ * (step, index) of every spike, ascending within a step; returns the spike
+ * count (or -1 if cap is exceeded). */
+int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_step,
+                    int32_t *spike_idx, int64_t cap);
+
+/* ---- counter-based drive (shared definition with the HIP path) ----------- */
+uint32_t orc_drive_hash(uint64_t seed, int64_t step, uint32_t neuron);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,209 @@
+"""ctypes interface of the oracle library (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke entry point and the CPU
+baseline of bench.py to check and time the HIP implementation; never by the
+product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DUMP = os.path.join(HERE, "_ref", "ref_dump")
+
+MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1}
+INTEGRATORS = {"Euler": 0, "RK2": 1, "RK4": 2, "BDF": 3}
+
+P = ctypes.c_void_p
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def build_ref(reference_kernel: str) -> bool:
+    """Compile oracle/_ref from the reference's own sources (needs the checkout)."""
+    if not os.path.isdir(reference_kernel):
+        return False
+    subprocess.run(["make", "-s", "-C", HERE, "ref", f"REF={reference_kernel}"], check=True)
+    return True
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        l = ctypes.CDLL(LIB)
+        l.orc_pop_create.restype = P
+        l.orc_pop_create.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, ctypes.c_int,
+                                     ctypes.c_float, ctypes.c_int, ctypes.c_uint32, ctypes.c_int]
+        l.orc_pop_destroy.argtypes = [P]
+        l.orc_pop_nvars.argtypes = [P]
+        l.orc_pop_get_state.argtypes = [P, ctypes.c_int, P]
+        l.orc_pop_set_state.argtypes = [P, ctypes.c_int, P]
+        l.orc_pop_add_inputs.argtypes = [P, P, P, P, ctypes.c_int]
+        l.orc_pop_set_ext_current.argtypes = [P, P]
+        l.orc_pop_update.argtypes = [P, ctypes.c_int64, P]
+        l.orc_exp_table.argtypes = [P]
+        l.orc_exp_lut.argtypes = [P, P, ctypes.c_int]
+        l.orc_nmda_table.argtypes = [ctypes.c_float, ctypes.c_float, P]
+        l.orc_lcg_seeds.argtypes = [ctypes.c_uint32, ctypes.c_int, P]
+        l.orc_lcg_frand.argtypes = [ctypes.c_uint32, ctypes.c_int, P]
+        l.orc_drive_hash.restype = ctypes.c_uint32
+        l.orc_drive_hash.argtypes = [ctypes.c_uint64, ctypes.c_int64, ctypes.c_uint32]
+        l.orc_net_create.restype = P
+        l.orc_net_create.argtypes = [P, ctypes.c_int]
+        l.orc_net_destroy.argtypes = [P]
+        l.orc_net_add_source_self.argtypes = [P, P, P, P]
+        l.orc_net_add_source_poisson.argtypes = [P, ctypes.c_int, P, ctypes.c_float, P, P, P]
+        l.orc_net_set_external.argtypes = [P, ctypes.c_int, P, ctypes.c_float, ctypes.c_uint64]
+        l.orc_net_run.restype = ctypes.c_int64
+        l.orc_net_run.argtypes = [P, ctypes.c_int64, ctypes.c_int, P, P, ctypes.c_int64]
+        _lib = l
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def exp_table() -> np.ndarray:
+    t = np.zeros(16384, np.float32)
+    lib().orc_exp_table(_p(t))
+    return t
+
+
+def exp_lut(x) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib().orc_exp_lut(_p(x), _p(out), len(x))
+    return out
+
+
+def nmda_table(e_exc: float, e_inh: float) -> np.ndarray:
+    t = np.zeros(256, np.float32)
+    lib().orc_nmda_table(e_exc, e_inh, _p(t))
+    return t
+
+
+def lcg_seeds(seed: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().orc_lcg_seeds(seed, n, _p(out))
+    return out
+
+
+def lcg_frand(x0: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.float32)
+    lib().orc_lcg_frand(x0, n, _p(out))
+    return out
+
+
+def drive_hash(seed: int, step: int, neuron: int) -> int:
+    return int(lib().orc_drive_hash(seed, step, neuron))
+
+
+class Population:
+    """One population of the oracle (float restatement, or FP64 with fp64=True)."""
+
+    def __init__(self, model: str, integrator: str, params, n: int, dt_s=np.float32(1e-4),
+                 mask: int = 3, fp64: bool = False, bdf_order: int = 1):
+        p = np.ascontiguousarray(params, np.float32)
+        self.n = n
+        self.h = lib().orc_pop_create(MODELS[model], INTEGRATORS[integrator], bdf_order, _p(p), len(p),
+                                      float(np.float32(dt_s)), n, mask, int(fp64))
+        if not self.h:
+            raise ValueError("oracle does not implement this model")
+        self.nv = lib().orc_pop_nvars(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_pop_destroy(self.h)
+            self.h = None
+
+    def state(self, var: int) -> np.ndarray:
+        out = np.zeros(self.n, np.float64)
+        lib().orc_pop_get_state(self.h, var, _p(out))
+        return out
+
+    def set_state(self, var: int, values) -> None:
+        v = np.ascontiguousarray(values, np.float64)
+        lib().orc_pop_set_state(self.h, var, _p(v))
+
+    def add_inputs(self, tgt, typ, w) -> None:
+        t = np.ascontiguousarray(tgt, np.int32)
+        y = np.ascontiguousarray(typ, np.uint8)
+        ww = np.ascontiguousarray(w, np.float32)
+        lib().orc_pop_add_inputs(self.h, _p(t), _p(y), _p(ww), len(t))
+
+    def set_ext_current(self, cur) -> None:
+        c = np.ascontiguousarray(cur, np.float32)
+        lib().orc_pop_set_ext_current(self.h, _p(c))
+
+    def update(self, step: int) -> np.ndarray:
+        out = np.zeros(self.n, np.int32)
+        k = lib().orc_pop_update(self.h, step, _p(out))
+        return out[:k].copy()
+
+
+class Net:
+    def __init__(self, pop: Population, max_delay: int):
+        self.pop = pop
+        self.h = lib().orc_net_create(pop.h, max_delay)
+        self._keep = []
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_net_destroy(self.h)
+            self.h = None
+
+    def add_self(self, row_ptr, tgt_type, w):
+        a = [np.ascontiguousarray(row_ptr, np.int64), np.ascontiguousarray(tgt_type, np.uint32),
+             np.ascontiguousarray(w, np.float32)]
+        lib().orc_net_add_source_self(self.h, *[_p(x) for x in a])
+
+    def add_poisson(self, states, rate, row_ptr, tgt_type, w):
+        s = np.ascontiguousarray(states, np.uint32)
+        a = [np.ascontiguousarray(row_ptr, np.int64), np.ascontiguousarray(tgt_type, np.uint32),
+             np.ascontiguousarray(w, np.float32)]
+        lib().orc_net_add_source_poisson(self.h, len(s), _p(s), float(rate), *[_p(x) for x in a])
+
+    def set_external(self, thr, w_ext, seed):
+        t = np.ascontiguousarray(thr, np.uint32)
+        lib().orc_net_set_external(self.h, len(t), _p(t), float(w_ext), int(seed))
+
+    def run(self, first_step: int, n_steps: int, cap: int = 1 << 24):
+        st = np.zeros(cap, np.int32)
+        ix = np.zeros(cap, np.int32)
+        n = lib().orc_net_run(self.h, first_step, n_steps, _p(st), _p(ix), cap)
+        if n < 0:
+            raise RuntimeError("oracle spike buffer too small")
+        return st[:n].copy(), ix[:n].copy()
+
+
+def from_spec(spec, params: np.ndarray, csr_builder, fp64: bool = False):
+    """Oracle population + network for a workloads.NetworkSpec.  csr_builder
+    sorts edges into rows (the tests check it against an independent sort)."""
+    mask = 0
+    if spec.rec is not None:
+        mask |= int(np.bitwise_or.reduce(1 << np.unique(spec.rec[3]).astype(np.int64)))
+    for _, _, e in spec.poisson:
+        mask |= int(np.bitwise_or.reduce(1 << np.unique(e[3]).astype(np.int64)))
+    if spec.ext is not None:
+        mask |= 1
+    pop = Population(spec.model, spec.integrator, params, spec.n, mask=mask, fp64=fp64)
+    net = Net(pop, spec.max_delay)
+    if spec.rec is not None:
+        net.add_self(*csr_builder(spec.n, spec.n, spec.max_delay, *spec.rec))
+    for states, rate, e in spec.poisson:
+        net.add_poisson(states, rate, *csr_builder(len(states), spec.n, spec.max_delay, *e))
+    if spec.ext is not None:
+        thr, w_ext, seed = spec.ext
+        net.set_external(thr, w_ext, seed)
+    return pop, net

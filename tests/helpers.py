@@ -1,0 +1,27 @@
+"""Shared helpers of the test suite."""
+from __future__ import annotations
+
+import numpy as np
+
+ADEX_DEFAULTS = np.array([1.0, 9.0, 2.0, -50.0, 50.0, 0.0, -80.0, -80.0, -65.0, 10.0, 110.0, 5.0, 10.0, 20.0],
+                         np.float32)
+LIF_DEFAULTS = np.array([0.0, -80.0, -65.0, -50.0, 2.0, 5.0, 10.0, 1.0, 0.2, 20.0], np.float32)
+
+
+def np_out_csr(n_src, n_tgt, max_delay, src, tgt, delay, typ, w):
+    """Independent numpy restatement of qsort_inters (Kernel/src/spike/Network.cpp:45-68)
+    with one queue: order by source, then delay, then target (stable)."""
+    src = np.asarray(src, np.int64)
+    tgt = np.asarray(tgt, np.int64)
+    delay = np.asarray(delay, np.int64)
+    order = np.lexsort((tgt, delay, src))
+    row = src[order] * max_delay + (delay[order] - 1)
+    row_ptr = np.zeros(n_src * max_delay + 1, np.int64)
+    np.add.at(row_ptr, row + 1, 1)
+    row_ptr = np.cumsum(row_ptr)
+    tt = (tgt[order].astype(np.uint32) | (np.asarray(typ, np.uint32)[order] << np.uint32(30))).astype(np.uint32)
+    return row_ptr, tt, np.asarray(w, np.float32)[order]
+
+
+def bits(a) -> np.ndarray:
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
