@@ -498,7 +498,16 @@ struct Source {
     std::vector<std::vector<int32_t>> ring;
 };
 
+struct TraceRec {
+    int32_t step;
+    int32_t tgt;
+    uint8_t type;
+    float w;
+};
+
 struct Net {
+    bool trace = false;
+    std::vector<TraceRec> trace_buf;
     PopIface *pop = nullptr;
     int max_delay = 1;
     int ring = 2;
@@ -653,6 +662,25 @@ void orc_net_set_external(void *net, int n_thr, const uint32_t *thresholds, floa
     n->has_ext = n_thr > 0;
 }
 
+void orc_net_set_trace(void *net, int on) {
+    Net *n = (Net *)net;
+    n->trace = on != 0;
+    n->trace_buf.clear();
+}
+
+int64_t orc_net_get_trace(void *net, int32_t *step, int32_t *tgt, uint8_t *type, float *w,
+                          int64_t cap) {
+    Net *n = (Net *)net;
+    const int64_t m = (int64_t)n->trace_buf.size();
+    for (int64_t i = 0; i < m && i < cap; i++) {
+        step[i] = n->trace_buf[i].step;
+        tgt[i] = n->trace_buf[i].tgt;
+        type[i] = n->trace_buf[i].type;
+        w[i] = n->trace_buf[i].w;
+    }
+    return m;
+}
+
 int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_step,
                     int32_t *spike_idx, int64_t cap) {
     Net *n = (Net *)net;
@@ -674,6 +702,9 @@ int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_s
                     for (int64_t q = b; q < e; q++) {
                         uint32_t tt = src.tgt_type[q];
                         pop->add_input((int)(tt & 0x3fffffffu), (int)(tt >> 30), src.w[q]);
+                        if (n->trace)
+                            n->trace_buf.push_back({(int32_t)k, (int32_t)(tt & 0x3fffffffu),
+                                                    (uint8_t)(tt >> 30), src.w[q]});
                     }
                 }
             }

@@ -81,6 +81,12 @@ void orc_net_set_external(void *net, int n_thr, const uint32_t *thresholds, floa
 int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_step,
                     int32_t *spike_idx, int64_t cap);
 
+/* Record every delivered recurrent/device event (step, target, type, w) in
+ * delivery order, for the ordering tests. */
+void orc_net_set_trace(void *net, int on);
+int64_t orc_net_get_trace(void *net, int32_t *step, int32_t *tgt, uint8_t *type, float *w,
+                          int64_t cap);
+
 /* ---- counter-based drive (shared definition with the HIP path) ----------- */
 uint32_t orc_drive_hash(uint64_t seed, int64_t step, uint32_t neuron);
 

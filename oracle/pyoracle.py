@@ -64,6 +64,9 @@ def lib():
         l.orc_net_add_source_self.argtypes = [P, P, P, P]
         l.orc_net_add_source_poisson.argtypes = [P, ctypes.c_int, P, ctypes.c_float, P, P, P]
         l.orc_net_set_external.argtypes = [P, ctypes.c_int, P, ctypes.c_float, ctypes.c_uint64]
+        l.orc_net_set_trace.argtypes = [P, ctypes.c_int]
+        l.orc_net_get_trace.restype = ctypes.c_int64
+        l.orc_net_get_trace.argtypes = [P, P, P, P, P, ctypes.c_int64]
         l.orc_net_run.restype = ctypes.c_int64
         l.orc_net_run.argtypes = [P, ctypes.c_int64, ctypes.c_int, P, P, ctypes.c_int64]
         _lib = l
@@ -177,6 +180,18 @@ class Net:
     def set_external(self, thr, w_ext, seed):
         t = np.ascontiguousarray(thr, np.uint32)
         lib().orc_net_set_external(self.h, len(t), _p(t), float(w_ext), int(seed))
+
+    def set_trace(self, on: bool):
+        lib().orc_net_set_trace(self.h, int(on))
+
+    def trace(self):
+        n = lib().orc_net_get_trace(self.h, None, None, None, None, 0)
+        st = np.zeros(n, np.int32)
+        tg = np.zeros(n, np.int32)
+        ty = np.zeros(n, np.uint8)
+        w = np.zeros(n, np.float32)
+        lib().orc_net_get_trace(self.h, _p(st), _p(tg), _p(ty), _p(w), n)
+        return st, tg, ty, w
 
     def run(self, first_step: int, n_steps: int, cap: int = 1 << 24):
         st = np.zeros(cap, np.int32)
