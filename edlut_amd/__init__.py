@@ -8,6 +8,7 @@ from ._capi import EDLUTException, lib
 from .models import (
     AdExTimeDrivenModel_GPU,
     CreateNeuronModel,
+    HHTimeDrivenModel_GPU,
     LIFTimeDrivenModel_GPU,
     REGISTRY,
     TimeDrivenNeuronModel_GPU,
@@ -20,6 +21,7 @@ __all__ = [
     "lib",
     "AdExTimeDrivenModel_GPU",
     "LIFTimeDrivenModel_GPU",
+    "HHTimeDrivenModel_GPU",
     "TimeDrivenNeuronModel_GPU",
     "CreateNeuronModel",
     "REGISTRY",

@@ -27,6 +27,7 @@ EXPORTS = (
     "edlut_population_get_spikes",
     "edlut_population_get_state",
     "edlut_population_set_state",
+    "edlut_population_set_newton_fp64",
     "edlut_population_device_state",
     "edlut_population_lut_checksums",
     "edlut_network_create",
@@ -85,6 +86,7 @@ def _sig(lib):
         "edlut_population_get_spikes": (I32, [P, P, I32]),
         "edlut_population_get_state": (I32, [P, I32, P]),
         "edlut_population_set_state": (I32, [P, I32, P]),
+        "edlut_population_set_newton_fp64": (I32, [P, I32]),
         "edlut_population_device_state": (I32, [P, P]),
         "edlut_population_lut_checksums": (I32, [P, P]),
         "edlut_network_create": (I32, [P, I32, P]),

@@ -50,6 +50,9 @@ __device__ __forceinline__ float nmda_inf(const float *__restrict__ t, float aux
 __device__ __forceinline__ float decay_g(float g, float f) {
     return (g < 1e-9f) ? 0.0f : g * f;
 }
+__device__ __forceinline__ double decay_g(double g, double f) {
+    return (g < (double)1e-9f) ? 0.0 : g * f;
+}
 
 // ---------------------------------------------------------------------------
 // Model descriptors.  Each exposes NV (state variables), ND (differential
@@ -69,13 +72,23 @@ struct LIFParams {
     float init[5];
 };
 
+struct HHParams {
+    float e_exc, e_inh, e_leak, g_leak, inv_c_m, g_na, g_kd, e_na, e_k, nmda_aux;
+    float cf[3];
+    float init[8];
+    double inv_c_m_d;   // FP64 Newton mode: 1/c_m and decay factors in double
+    double cf_d[3];
+};
+
 struct ModelParams {
     AdExParams adex;
     LIFParams lif;
+    HHParams hh;
 };
 
 // Per-thread context the right-hand sides need besides the state.
 struct RhsCtx {
+    const float *chan;   // HH: channel-rate table, 6 floats per entry
     const float *exp_lut;
     const float *nmda_lut;
     const float *refr;   // LIF: refractory fractions by steps since the last spike
@@ -85,12 +98,13 @@ struct RhsCtx {
 
 struct AdExModel {
     static constexpr int NV = 6, ND = 2;
+    static constexpr bool kResetsIntegrator = true;   // spike -> resetState
     static constexpr int EXC = 2, INH = 3, NMDA = 4, EXT = 5;
 
     // AdExTimeDrivenModel.cpp:170-218.
-    template <unsigned MASK>
-    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c,
-                                               const float *s, float *d) {
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s,
+                                               T *d) {
         const AdExParams &p = mp.adex;
         const float V = s[0];
         if (V <= p.v_cap) {
@@ -123,14 +137,15 @@ struct AdExModel {
             d[1] = 0.0f;
         }
     }
-    template <unsigned MASK>
-    __device__ static __forceinline__ void decay(const ModelParams &mp, float *s) {
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
         if (MASK & 1u) s[EXC] = decay_g(s[EXC], mp.adex.cf[0]);
         if (MASK & 2u) s[INH] = decay_g(s[INH], mp.adex.cf[1]);
         if (MASK & 4u) s[NMDA] = decay_g(s[NMDA], mp.adex.cf[2]);
     }
     // AdExTimeDrivenModel.cpp:159-168.
-    __device__ static __forceinline__ bool spike(const ModelParams &mp, float *s) {
+    template <typename T = float>
+    __device__ static __forceinline__ bool spike(const ModelParams &mp, T prev_v, T *s) {
         if (s[0] > 0.0f) {
             s[0] = mp.adex.e_reset;
             s[1] += mp.adex.b;
@@ -145,6 +160,7 @@ struct AdExModel {
 
 struct LIFModel {
     static constexpr int NV = 5, ND = 1;
+    static constexpr bool kResetsIntegrator = true;
     static constexpr int EXC = 1, INH = 2, NMDA = 3, EXT = 4;
 
     // LIFTimeDrivenModel.cpp:170-202.  The double-precision refractory clock
@@ -152,9 +168,9 @@ struct LIFModel {
     // last spike and a host-built table of the fraction the clock yields for
     // each count (negative = derivative forced to zero); counts past the
     // table's end are outside the refractory period.
-    template <unsigned MASK>
-    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c,
-                                               const float *s, float *d) {
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s,
+                                               T *d) {
         const LIFParams &p = mp.lif;
         const float V = s[0];
         float cur = 0.0f;
@@ -172,14 +188,15 @@ struct LIFModel {
             d[0] = (f < 0.0f) ? 0.0f : f * ((cur + p.g_leak * (p.e_leak - V)) * p.inv_c_m);
         }
     }
-    template <unsigned MASK>
-    __device__ static __forceinline__ void decay(const ModelParams &mp, float *s) {
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
         if (MASK & 1u) s[EXC] = decay_g(s[EXC], mp.lif.cf[0]);
         if (MASK & 2u) s[INH] = decay_g(s[INH], mp.lif.cf[1]);
         if (MASK & 4u) s[NMDA] = decay_g(s[NMDA], mp.lif.cf[2]);
     }
     // LIFTimeDrivenModel.cpp:158-168.
-    __device__ static __forceinline__ bool spike(const ModelParams &mp, float *s) {
+    template <typename T = float>
+    __device__ static __forceinline__ bool spike(const ModelParams &mp, T prev_v, T *s) {
         if (s[0] > mp.lif.v_thr) {
             s[0] = mp.lif.e_leak;
             return true;
@@ -188,6 +205,72 @@ struct LIFModel {
     }
     __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
         return mp.lif.init[v];
+    }
+};
+
+
+// HH channel-rate table geometry (HHTimeDrivenModel.h:64-79, .cpp:27-30).
+constexpr int kHHTable = 1024 * 256;
+constexpr float kHHMinV = -90.0f;
+constexpr float kHHMaxV = 50.0f;
+constexpr float kHHAux = (kHHTable - 1) / (kHHMaxV - kHHMinV);
+
+struct HHModel {
+    static constexpr int NV = 8, ND = 4;
+    static constexpr bool kResetsIntegrator = false;
+    static constexpr int EXC = 4, INH = 5, NMDA = 6, EXT = 7;
+
+    // HHTimeDrivenModel.cpp:197-264.  The state it is given is clamped in
+    // place (V to [e_k, e_na-1], gates to [0,1], n to 0.72), as in the
+    // reference, where the first stage receives the neuron's own state.
+    // T = double is the FP64 Newton mode: same expression order in double,
+    // channel rates from the same float table.
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s,
+                                               T *d) {
+        const HHParams &p = mp.hh;
+        if (s[0] < (T)p.e_k) s[0] = p.e_k;
+        if (s[0] > (T)(p.e_na - 1.0f)) s[0] = p.e_na - 1.0f;
+        if (s[1] < 0) s[1] = 0.0f;
+        if (s[1] > 1) s[1] = 1.0f;
+        if (s[2] < 0) s[2] = 0.0f;
+        if (s[2] > 1) s[2] = 1.0f;
+        if (s[3] < 0) s[3] = 0.0f;
+        if ((double)s[3] > 0.72) s[3] = 0.72f;
+        const T V = s[0], m = s[1], h = s[2], n = s[3];
+        int pos = trunc_i32_x86(((float)V - kHHMinV) * kHHAux);
+        pos = pos < 0 ? 0 : (pos > kHHTable - 1 ? kHHTable - 1 : pos);
+        const float *v = c.chan + (int64_t)pos * 6;
+        T cur = 0.0f;
+        if (MASK & 1u) cur += s[EXC] * ((T)p.e_exc - s[0]);
+        if (MASK & 2u) cur += s[INH] * ((T)p.e_inh - s[0]);
+        if (MASK & 4u) {
+            const T gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, (float)s[0]);
+            cur += s[NMDA] * gi * ((T)p.e_exc - s[0]);
+        }
+        if (MASK & 8u) cur += s[EXT];
+        const T inv_c_m = sizeof(T) == sizeof(double) ? (T)p.inv_c_m_d : (T)p.inv_c_m;
+        d[0] = (cur + (T)p.g_leak * ((T)p.e_leak - V) + (T)p.g_na * m * m * m * h * ((T)p.e_na - V) +
+                (T)p.g_kd * n * n * n * n * ((T)p.e_k - V)) * inv_c_m;
+        const T am = v[0], bm = v[1], ah = v[2], bh = v[3], an = v[4], bn = v[5];
+        d[1] = (am * (1.0f - m) - bm * m);
+        d[2] = (ah * (1.0f - h) - bh * h);
+        d[3] = (an * (1.0f - n) - bn * n);
+    }
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
+        const bool dbl = sizeof(T) == sizeof(double);
+        if (MASK & 1u) s[EXC] = decay_g(s[EXC], dbl ? (T)mp.hh.cf_d[0] : (T)mp.hh.cf[0]);
+        if (MASK & 2u) s[INH] = decay_g(s[INH], dbl ? (T)mp.hh.cf_d[1] : (T)mp.hh.cf[1]);
+        if (MASK & 4u) s[NMDA] = decay_g(s[NMDA], dbl ? (T)mp.hh.cf_d[2] : (T)mp.hh.cf[2]);
+    }
+    // HHTimeDrivenModel.cpp:187-193: upward crossing of -30 mV, no reset.
+    template <typename T = float>
+    __device__ static __forceinline__ bool spike(const ModelParams &, T prev_v, T *s) {
+        return s[0] > -30.0f && prev_v < -30.0f;
+    }
+    __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
+        return mp.hh.init[v];
     }
 };
 
@@ -277,6 +360,172 @@ __device__ __forceinline__ void integrate_rk4(const ModelParams &mp, const RhsCt
 #pragma unroll
         for (int j = M::ND; j < M::NV; j++) s[n][j] = a[n][j];
     }
+}
+
+
+// ---------------------------------------------------------------------------
+// BDF (BDFn.h:288-423, Jacobian 682-722, invermat 724-786) on one neuron
+// held in registers, in arithmetic type T (float: the reference's; double:
+// the FP64 Newton mode).  D and Prev are the difference table and previous
+// states (row-major in the history index, SoA over neurons).
+// ---------------------------------------------------------------------------
+template <int N, typename T>
+__device__ __forceinline__ void invermat_dev(const T *a, T *ainv) {
+    if (N == 1) {
+        ainv[0] = (T)1.0f / a[0];
+        return;
+    }
+    T la[N * N], li[N * N];
+#pragma unroll
+    for (int q = 0; q < N * N; q++) {
+        la[q] = a[q];
+        li[q] = 0;
+    }
+#pragma unroll
+    for (int q = 0; q < N; q++) li[q * N + q] = 1.0f;
+#pragma unroll
+    for (int s = 0; s < N; s++) {
+        T element = la[s * N + s];
+        if (element == 0) {
+            bool done = false;
+#pragma unroll
+            for (int r = s + 1; r < N; r++) {
+                if (!done && la[r * N + s] != 0) {
+                    element = la[r * N + s];
+#pragma unroll
+                    for (int m = 0; m < N; m++) {
+                        T v = la[r * N + m];
+                        la[r * N + m] = la[s * N + m];
+                        la[s * N + m] = v;
+                        v = li[r * N + m];
+                        li[r * N + m] = li[s * N + m];
+                        li[s * N + m] = v;
+                    }
+                    done = true;
+                }
+            }
+        }
+        const T inv_element = (T)1.0f / element;
+#pragma unroll
+        for (int j = 0; j < N; j++) {
+            la[s * N + j] *= inv_element;
+            li[s * N + j] *= inv_element;
+        }
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            if (i != s) {
+                const T coef = -la[i * N + s];
+#pragma unroll
+                for (int j = 0; j < N; j++) {
+                    la[i * N + j] += la[s * N + j] * coef;
+                    li[i * N + j] += li[s * N + j] * coef;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < N * N; q++) ainv[q] = li[q];
+}
+
+struct BdfState {
+    int order;
+    const float *C;    // [7][7] coefficients (BDFn.h:604-669)
+    void *d;           // T[order][ND][stride]
+    void *prev;        // T[order-1][ND][stride]
+    int32_t *st;       // [stride]
+};
+
+template <class M, unsigned MASK, typename T>
+__device__ __forceinline__ void integrate_bdf(const ModelParams &mp, const RhsCtx &c, T dt,
+                                              const BdfState &b, int64_t ld, uint32_t i,
+                                              int &st, T *S) {
+    constexpr int ND = M::ND, NV = M::NV;
+    T *Dh = (T *)b.d;
+    T *Ph = (T *)b.prev;
+    T F[ND], P[NV], P1[ND], Cc[ND], jac[ND * ND], J[ND * ND], iJ[ND * ND], X[NV], Fp[ND], Fn[ND];
+    if (st == 0) {
+        M::template rhs<MASK, T>(mp, c, S, F);
+#pragma unroll
+        for (int j = 0; j < ND; j++) P[j] = S[j] + dt * F[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < ND; j++) {
+            P[j] = S[j];
+            for (int q = 0; q < st; q++) P[j] += Dh[((int64_t)q * ND + j) * ld + i];
+        }
+    }
+#pragma unroll
+    for (int j = ND; j < NV; j++) P[j] = S[j];
+    M::template decay<MASK, T>(mp, P);
+    T epsi = 1.0f;
+    int k = 0;
+    const float *Cs = b.C + st * 7;
+    while (epsi > 1e-6 && k < 10) {
+        M::template rhs<MASK, T>(mp, c, P, F);
+#pragma unroll
+        for (int j = 0; j < ND; j++) {
+            Cc[j] = (T)Cs[0] * dt * F[j] + (T)Cs[1] * S[j];
+            for (int q = 1; q < st; q++) Cc[j] += (T)Cs[q + 1] * Ph[((int64_t)(q - 1) * ND + j) * ld + i];
+        }
+        const T eps = dt * 0.1f;
+        const T inv_eps = (T)1.0f / eps;
+#pragma unroll
+        for (int q = 0; q < NV; q++) X[q] = P[q];
+        M::template rhs<MASK, T>(mp, c, X, Fp);
+#pragma unroll
+        for (int j = 0; j < ND; j++) {
+#pragma unroll
+            for (int q = 0; q < NV; q++) X[q] = P[q];
+            X[j] -= eps;
+            M::template rhs<MASK, T>(mp, c, X, Fn);
+#pragma unroll
+            for (int z = 0; z < ND; z++) jac[z * ND + j] = (Fp[z] - Fn[z]) * inv_eps;
+        }
+#pragma unroll
+        for (int z = 0; z < ND; z++)
+#pragma unroll
+            for (int t = 0; t < ND; t++) {
+                J[z * ND + t] = (T)Cs[0] * dt * jac[z * ND + t];
+                if (z == t) J[z * ND + t] -= 1;
+            }
+        invermat_dev<ND, T>(J, iJ);
+#pragma unroll
+        for (int z = 0; z < ND; z++) {
+            T aux = 0.0f;
+#pragma unroll
+            for (int t = 0; t < ND; t++) aux += iJ[z * ND + t] * (P[t] - Cc[t]);
+            P1[z] = aux + P[z];
+        }
+        T aux2 = 0.0f;
+#pragma unroll
+        for (int z = 0; z < ND; z++) {
+            const T aux = fabs(P1[z] - P[z]);
+            if (aux > aux2) aux2 = aux;
+        }
+#pragma unroll
+        for (int z = 0; z < ND; z++) P[z] = P1[z];
+        epsi = aux2;
+        k++;
+    }
+    if (st < b.order) st++;
+#pragma unroll
+    for (int j = 0; j < ND; j++) {
+        for (int q = st - 1; q > 0; q--) Dh[((int64_t)q * ND + j) * ld + i] = -Dh[((int64_t)(q - 1) * ND + j) * ld + i];
+        Dh[(int64_t)j * ld + i] = P[j] - S[j];
+        for (int q = 1; q < st; q++)
+            Dh[((int64_t)q * ND + j) * ld + i] += Dh[((int64_t)(q - 1) * ND + j) * ld + i];
+    }
+    if (st > 1) {
+        for (int q = st - 2; q > 0; q--)
+#pragma unroll
+            for (int j = 0; j < ND; j++)
+                Ph[((int64_t)q * ND + j) * ld + i] = Ph[((int64_t)(q - 1) * ND + j) * ld + i];
+#pragma unroll
+        for (int j = 0; j < ND; j++) Ph[(int64_t)j * ld + i] = S[j];
+    }
+#pragma unroll
+    for (int j = 0; j < ND; j++) S[j] = P[j];
+    M::template decay<MASK, T>(mp, S);
 }
 
 }  // namespace edl

@@ -100,6 +100,54 @@ std::vector<float> build_refractory(float dt_s, float dt_ms, float tau_ref) {
 
 inline float inv_of(float x) { return (float)(1.0 / (double)x); }
 
+// HH channel-rate table (HHTimeDrivenModel.h:372-421): 262144 entries of six
+// rates on [-90, 50] mV, evaluated in float with the float exp.
+std::vector<float> build_hh_table(float v_thr) {
+    std::vector<float> t((size_t)kHHTable * 6);
+    for (int i = 0; i < kHHTable; i++) {
+        const float V = kHHMinV + ((kHHMaxV - kHHMinV) * i) / (kHHTable - 1);
+        float am, bm, an;
+        if ((13.0f - V + v_thr) == 0.0) am = 1.279999894092334f;
+        else am = 0.32f * (13.0f - V + v_thr) / (expf((13.0f - V + v_thr) * 0.25f) - 1.0f);
+        if ((V - v_thr - 40.0f) == 0.0) bm = 1.400030971093304f;
+        else bm = 0.28f * (V - v_thr - 40.0f) / (expf((V - v_thr - 40.0f) * 0.2f) - 1.0f);
+        const float ah = 0.128f * expf((17.0f - V + v_thr) * 0.05555555f);
+        const float bh = 4.0f / (1.0f + expf((40.0f - V + v_thr) * 0.2f));
+        if ((15.0f - V + v_thr) == 0.0) an = 0.1599999867615418f;
+        else an = 0.032f * (15.0f - V + v_thr) / (expf((15.0f - V + v_thr) * 0.2f) - 1.0f);
+        const float bn = 0.5f * expf((10.0f - V + v_thr) * 0.025f);
+        float *o = &t[(size_t)i * 6];
+        o[0] = am; o[1] = bm; o[2] = ah; o[3] = bh; o[4] = an; o[5] = bn;
+    }
+    return t;
+}
+
+inline int hh_index(float V) {
+    int pos;
+    const float x = (V - kHHMinV) * kHHAux;
+    if (!(x < 2147483648.0f) || x < -2147483648.0f) pos = (int)0x80000000u;
+    else pos = (int)x;
+    return pos < 0 ? 0 : (pos > kHHTable - 1 ? kHHTable - 1 : pos);
+}
+
+// BDF coefficients (BDFn.h:604-669), row = method state (0..6)
+void bdf_coefficients(float *C) {
+    std::memset(C, 0, sizeof(float) * 49);
+    auto set = [&](int r, std::initializer_list<float> v) {
+        int c = 0;
+        for (float x : v) C[r * 7 + c++] = x;
+    };
+    set(0, {1.0f, 1.0f});
+    set(1, {1.0f, 1.0f});
+    set(2, {2.0f / 3.0f, 4.0f / 3.0f, -1.0f / 3.0f});
+    set(3, {6.0f / 11.0f, 18.0f / 11.0f, -9.0f / 11.0f, 2.0f / 11.0f});
+    set(4, {12.0f / 25.0f, 48.0f / 25.0f, -36.0f / 25.0f, 16.0f / 25.0f, -3.0f / 25.0f});
+    set(5, {60.0f / 137.0f, 300.0f / 137.0f, -300.0f / 137.0f, 200.0f / 137.0f, -75.0f / 137.0f,
+            12.0f / 137.0f});
+    set(6, {60.0f / 147.0f, 360.0f / 147.0f, -450.0f / 147.0f, 400.0f / 147.0f, -225.0f / 147.0f,
+            72.0f / 147.0f, -10.0f / 147.0f});
+}
+
 // tile-count arrays are padded (and zeroed) to whole compaction chunks so the
 // compaction kernel reads them without bounds tests
 inline size_t padded_tiles(int n_tiles) {
@@ -125,6 +173,13 @@ struct edlut_population {
 
     float *state = nullptr;
     int32_t *last_spike = nullptr;
+    float *chan = nullptr;           // HH channel-rate table
+    float *bdf_c = nullptr;          // BDF coefficients [7][7]
+    void *bdf_d = nullptr;           // BDF difference table
+    void *bdf_prev = nullptr;        // BDF previous states
+    int32_t *bdf_st = nullptr;       // BDF method state per neuron
+    bool newton_fp64 = false;
+    bool started = false;            // an update has run (BDF precision is fixed then)
     float *exp_lut = nullptr;
     float *nmda_lut = nullptr;
     float *refr = nullptr;
@@ -174,6 +229,20 @@ struct edlut_population {
         t.b_cap = b_cap;
         t.err = err;
         return t;
+    }
+
+    // BDF history: bdf_order difference rows and bdf_order-1 previous states
+    // of the nd differential variables, in `elem`-byte words.
+    int alloc_bdf_history(size_t elem) {
+        if (bdf_d) (void)hipFree(bdf_d);
+        if (bdf_prev) (void)hipFree(bdf_prev);
+        bdf_d = bdf_prev = nullptr;
+        const size_t nd_rows = (size_t)bdf_order * nd, np_rows = (size_t)(bdf_order > 1 ? bdf_order - 1 : 1) * nd;
+        HIP_TRY(hipMalloc(&bdf_d, nd_rows * stride * elem));
+        HIP_TRY(hipMalloc(&bdf_prev, np_rows * stride * elem));
+        HIP_TRY(hipMemset(bdf_d, 0, nd_rows * stride * elem));
+        HIP_TRY(hipMemset(bdf_prev, 0, np_rows * stride * elem));
+        return 0;
     }
 
     // Grow the fold's event buffer to hold `events` events.
@@ -260,12 +329,37 @@ struct Table {
     }
 };
 
-UpdateFn pick_update(int model, int integ, uint32_t mask, bool ext) {
+template <class M, typename T, unsigned... MS>
+void fill_bdf(UpdateFn *t, std::integer_sequence<unsigned, MS...>) {
+    ((t[MS * 2 + 0] = k_update_bdf<M, MS, false, T>, t[MS * 2 + 1] = k_update_bdf<M, MS, true, T>), ...);
+}
+
+template <class M, typename T>
+struct BdfTable {
+    UpdateFn fn[16 * 2];
+    BdfTable() { fill_bdf<M, T>(fn, std::make_integer_sequence<unsigned, 16>{}); }
+};
+
+UpdateFn pick_update(int model, int integ, uint32_t mask, bool ext, bool fp64 = false) {
+    const int m = (int)(mask & 15u);
+    if (integ == EDLUT_BDF) {
+        static BdfTable<AdExModel, float> adex_b;
+        static BdfTable<LIFModel, float> lif_b;
+        static BdfTable<HHModel, float> hh_b;
+        static BdfTable<HHModel, double> hh_b64;
+        const int idx = m * 2 + (ext ? 1 : 0);
+        if (model == EDLUT_ADEX) return adex_b.fn[idx];
+        if (model == EDLUT_LIF) return lif_b.fn[idx];
+        if (model == EDLUT_HH) return fp64 ? hh_b64.fn[idx] : hh_b.fn[idx];
+        return nullptr;
+    }
     static Table<AdExModel> adex;
     static Table<LIFModel> lif;
-    const int idx = (integ * 16 + (int)(mask & 15u)) * 2 + (ext ? 1 : 0);
+    static Table<HHModel> hh;
+    const int idx = (integ * 16 + m) * 2 + (ext ? 1 : 0);
     if (model == EDLUT_ADEX) return adex.fn[idx];
     if (model == EDLUT_LIF) return lif.fn[idx];
+    if (model == EDLUT_HH) return hh.fn[idx];
     return nullptr;
 }
 
@@ -295,6 +389,12 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.refr = p->refr;
     a.refr_len = p->refr_len;
     a.last_spike = p->last_spike;
+    a.chan = p->chan;
+    a.bdf.order = p->bdf_order;
+    a.bdf.C = p->bdf_c;
+    a.bdf.d = p->bdf_d;
+    a.bdf.prev = p->bdf_prev;
+    a.bdf.st = p->bdf_st;
     return a;
 }
 
@@ -354,11 +454,13 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
                             edlut_population **out) {
     if (!out) return fail(EDLUT_E_ARG, "out is null");
     *out = nullptr;
-    if (model != EDLUT_ADEX && model != EDLUT_LIF)
+    if (model != EDLUT_ADEX && model != EDLUT_LIF && model != EDLUT_HH)
         return fail(EDLUT_E_ARG, "neuron model %d is not available in this build", model);
-    if (integrator < EDLUT_EULER || integrator > EDLUT_RK4)
+    if (integrator < EDLUT_EULER || integrator > EDLUT_BDF)
         return fail(EDLUT_E_ARG, "integration method %d is not available for this model", integrator);
-    const int need = model == EDLUT_ADEX ? 14 : 10;
+    if (integrator == EDLUT_BDF && (bdf_order < 1 || bdf_order > 6))
+        return fail(EDLUT_E_ARG, "bdf_order must be in [1, 6] (BDFn_GPU_C_INTERFACE.cuh:275-288)");
+    const int need = model == EDLUT_ADEX ? 14 : (model == EDLUT_HH ? 13 : 10);
     if (!params || n_params != need)
         return fail(EDLUT_E_ARG, "model %d takes %d parameters, got %d", model, need, n_params);
     if (!(dt_s > 0.0f)) return fail(EDLUT_E_ARG, "integration step must be positive");
@@ -408,6 +510,26 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
         for (int j = 0; j < 6; j++) q.init[j] = init[j];
         p->nv = 6;
         p->nd = 2;
+    } else if (model == EDLUT_HH) {
+        HHParams &q = p->mp.hh;
+        const float *x = params;   // e_exc e_inh e_leak g_leak c_m v_thr tau_exc tau_inh tau_nmda g_na g_kd e_na e_k
+        q.e_exc = e_exc = x[0];
+        q.e_inh = e_inh = x[1];
+        q.e_leak = x[2];
+        q.g_leak = x[3];
+        q.inv_c_m = inv_of(x[4]);
+        q.inv_c_m_d = 1.0 / (double)x[4];
+        q.g_na = x[9];
+        q.g_kd = x[10];
+        q.e_na = x[11];
+        q.e_k = x[12];
+        const float tau[3] = {x[6], x[7], x[8]};
+        for (int j = 0; j < 3; j++) {
+            q.cf[j] = expf(-elapsed * inv_of(tau[j]));
+            q.cf_d[j] = std::exp(-(double)elapsed * (1.0 / (double)tau[j]));
+        }
+        p->nv = 8;
+        p->nd = 4;
     } else {
         LIFParams &q = p->mp.lif;
         const float *x = params;
@@ -428,7 +550,19 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
     float nmda_aux = 0.0f;
     std::vector<float> nmda = build_nmda_table(e_exc, e_inh, &nmda_aux);
     if (model == EDLUT_ADEX) p->mp.adex.nmda_aux = nmda_aux;
+    else if (model == EDLUT_HH) p->mp.hh.nmda_aux = nmda_aux;
     else p->mp.lif.nmda_aux = nmda_aux;
+    std::vector<float> chan;
+    if (model == EDLUT_HH) {
+        chan = build_hh_table(params[5]);
+        // HHTimeDrivenModel.cpp:136-160: gates at their steady state at e_leak
+        const float *c = &chan[(size_t)hh_index(params[2]) * 6];
+        const float m_inf = c[0] / (c[0] + c[1]);
+        const float h_inf = c[2] / (c[2] + c[3]);
+        const float n_inf = c[4] / (c[4] + c[5]);
+        init = {params[2], m_inf, h_inf, n_inf, 0, 0, 0, 0};
+        for (int j = 0; j < 8; j++) p->mp.hh.init[j] = init[j];
+    }
     std::vector<float> expt = build_exp_table();
     std::vector<float> refr;
     if (model == EDLUT_LIF) refr = build_refractory(dt_s, p->dt_ms, params[7]);
@@ -452,6 +586,21 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
     if ((rc = dalloc(&p->exp_lut, kExpSize))) return cleanup(rc);
     if ((rc = dalloc(&p->nmda_lut, kNmdaSize))) return cleanup(rc);
     if ((rc = dalloc(&p->refr, refr.size() + 1))) return cleanup(rc);
+    if (model == EDLUT_HH) {
+        if ((rc = dalloc(&p->chan, chan.size()))) return cleanup(rc);
+        if (hipMemcpy(p->chan, chan.data(), chan.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(EDLUT_E_HIP, "hipMemcpy failed"));
+    }
+    if (integrator == EDLUT_BDF) {
+        float C[49];
+        bdf_coefficients(C);
+        if ((rc = dalloc(&p->bdf_c, 49))) return cleanup(rc);
+        if ((rc = dalloc(&p->bdf_st, p->stride))) return cleanup(rc);
+        if ((rc = p->alloc_bdf_history(sizeof(float)))) return cleanup(rc);
+        if (hipMemcpy(p->bdf_c, C, sizeof(C), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(p->bdf_st, 0, sizeof(int32_t) * p->stride) != hipSuccess)
+            return cleanup(fail(EDLUT_E_HIP, "BDF initialisation failed"));
+    }
     if ((rc = dalloc(&p->tile_count, padded_tiles(p->n_tiles)))) return cleanup(rc);
     if ((rc = dalloc(&p->staging, (size_t)p->n_tiles * kUpdTile))) return cleanup(rc);
     if ((rc = dalloc(&p->spk_idx, n_neurons))) return cleanup(rc);
@@ -503,7 +652,8 @@ int edlut_population_destroy(edlut_population *p) {
     }
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    void *dev[] = {p->state, p->last_spike, p->exp_lut, p->nmda_lut, p->refr, p->tile_count,
+    void *dev[] = {p->chan, p->bdf_c, p->bdf_d, p->bdf_prev, p->bdf_st,
+                   p->state, p->last_spike, p->exp_lut, p->nmda_lut, p->refr, p->tile_count,
                    p->staging, p->spk_idx, p->spk_cnt, p->err, p->cnt, p->fill, p->seg,
                    p->tile_flag, p->bump, p->b_ord, p->b_w, p->e_tgt,
                    p->e_type, p->e_w};
@@ -593,7 +743,8 @@ int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
     UpdateArgs a = make_update_args(p, (int32_t)step);
     a.out = TileOut{p->tile_count, p->staging};
     a.fold = p->fold_target(par);
-    UpdateFn fn = pick_update(p->model, p->integ, p->mask, false);
+    UpdateFn fn = pick_update(p->model, p->integ, p->mask, false, p->newton_fp64);
+    p->started = true;
     hipLaunchKernelGGL(fn, dim3(p->n_tiles), dim3(kBlock), 0, p->stream, a);
     launch_compact(p->stream, p->tile_count, p->staging, p->n_tiles, kUpdTile, p->spk_idx, p->spk_cnt,
                    nullptr, nullptr, nullptr, 0, p->err, (int32_t)step);
@@ -641,6 +792,19 @@ int edlut_population_set_state(edlut_population *p, int var, const float *in) {
     HIP_TRY(hipMemcpyAsync(p->state + (int64_t)var * p->stride, in, sizeof(float) * p->n,
                            hipMemcpyHostToDevice, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
+    return 0;
+}
+
+int edlut_population_set_newton_fp64(edlut_population *p, int on) {
+    if (!p) return fail(EDLUT_E_ARG, "population is null");
+    if (p->integ != EDLUT_BDF || p->model != EDLUT_HH)
+        return fail(EDLUT_E_ARG, "the FP64 Newton step applies to HH populations under BDF");
+    if (p->started) return fail(EDLUT_E_STATE, "the Newton precision is fixed once updates have run");
+    if ((bool)on == p->newton_fp64) return 0;
+    HIP_TRY(hipSetDevice(p->device));
+    int rc = p->alloc_bdf_history(on ? sizeof(double) : sizeof(float));
+    if (rc) return rc;
+    p->newton_fp64 = on != 0;
     return 0;
 }
 
@@ -829,7 +993,8 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
         const Source &s = n->sources[q];
         re.src[q] = RingSource{s.ring_idx, s.ring_cnt, s.row_ptr, s.tgt_type, s.w, s.n};
     }
-    UpdateFn fn = pick_update(p->model, p->integ, p->mask, n->has_ext);
+    UpdateFn fn = pick_update(p->model, p->integ, p->mask, n->has_ext, p->newton_fp64);
+    p->started = true;
     const bool timed = n->timing && 2 * (size_t)n_steps <= n->ev.size();
     n->timed = timed ? n_steps : 0;
     for (int64_t k = first_step; k < first_step + n_steps; k++) {

@@ -152,11 +152,13 @@ struct UpdateArgs {
     int32_t step;          // update index k (>= 1)
     ModelParams mp;
     StepConsts k;
+    const float *chan;     // HH channel-rate table
     const float *exp_lut;
     const float *nmda_lut;
     const float *refr;
     int refr_len;
     int32_t *last_spike;   // LIF only: update index of the last spike
+    BdfState bdf;          // BDF integrator history
     ExtDrive ext;
     TileOut out;
     FoldTarget fold;       // ordered input fold of this step (multi-event keys)
@@ -197,6 +199,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
         s[n][M::INH] = (MASK & 2u) ? row[M::INH][i] : 0.0f;
         s[n][M::NMDA] = (MASK & 4u) ? row[M::NMDA][i] : 0.0f;
         s[n][M::EXT] = (MASK & 8u) ? row[M::EXT][i] : 0.0f;
+        c[n].chan = a.chan;
         c[n].exp_lut = a.exp_lut;
         c[n].nmda_lut = a.nmda_lut;
         c[n].refr = a.refr;
@@ -237,6 +240,9 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
             for (int q = 0; q < cnt; q++) s[n][M::EXC] += a.ext.w;
         }
     }
+    float prev_v[kItems];
+#pragma unroll
+    for (int n = 0; n < kItems; n++) prev_v[n] = s[n][0];
     if (INTEG == kEuler) integrate_euler<M, MASK, kItems>(a.mp, c, a.k, s);
     else if (INTEG == kRK2) integrate_rk2<M, MASK, kItems>(a.mp, c, a.k, s);
     else integrate_rk4<M, MASK, kItems>(a.mp, c, a.k, s);
@@ -245,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
 #pragma unroll
     for (int n = 0; n < kItems; n++) {
         const uint32_t i = (uint32_t)idx[n];
-        spk[n] = live[n] && M::spike(a.mp, s[n]);
+        spk[n] = live[n] && M::spike(a.mp, prev_v[n], s[n]);
         if (!live[n]) continue;
         if (spk[n] && a.last_spike) a.last_spike[i] = a.step;
         if (s[n][0] != s[n][0]) {
@@ -258,6 +264,93 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
             if (MASK & 1u) row[M::EXC][i] = s[n][M::EXC];
             if (MASK & 2u) row[M::INH][i] = s[n][M::INH];
             if (MASK & 4u) row[M::NMDA][i] = s[n][M::NMDA];
+        }
+    }
+    tile_spikes<kItems>(a.out, tile, spk, idx);
+}
+
+
+// BDF update: one neuron at a time per thread (data-dependent Newton
+// iterations), kItems neurons per thread in the same tile layout as k_update.
+// T = double runs the whole BDF step in double (FP64 Newton mode); the state
+// is stored back in float.
+template <class M, unsigned MASK, bool EXT, typename T>
+__global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
+    const int tile = blockIdx.x;
+    __shared__ uint32_t s_flag;
+    if (threadIdx.x == 0) {
+        s_flag = a.fold.tile_flag[tile];
+        if (s_flag) a.fold.tile_flag[tile] = 0u;
+        if (tile == 0) a.fold.bump[a.fold.par ^ 1] = 0ull;
+    }
+    __syncthreads();
+    float *__restrict__ row[M::NV];
+#pragma unroll
+    for (int j = 0; j < M::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
+    const int64_t ld = a.stride;
+    const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, a.step) : 0u;
+    bool spk[kItems];
+    int idx[kItems];
+    for (int n = 0; n < kItems; n++) {
+        idx[n] = tile * kUpdTile + n * kBlock + (int)threadIdx.x;
+        spk[n] = false;
+        if (idx[n] >= a.n) continue;
+        const uint32_t i = (uint32_t)idx[n];
+        float sf[M::NV];
+#pragma unroll
+        for (int j = 0; j < M::ND; j++) sf[j] = row[j][i];
+        sf[M::EXC] = (MASK & 1u) ? row[M::EXC][i] : 0.0f;
+        sf[M::INH] = (MASK & 2u) ? row[M::INH][i] : 0.0f;
+        sf[M::NMDA] = (MASK & 4u) ? row[M::NMDA][i] : 0.0f;
+        sf[M::EXT] = (MASK & 8u) ? row[M::EXT][i] : 0.0f;
+        if (s_flag) {
+#pragma unroll
+            for (int ty = 0; ty < 3; ty++) {
+                if (!(MASK & (1u << ty))) continue;
+                const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)i;
+                const uint32_t m = a.fold.cnt[key];
+                if (m >= 2u)
+                    sf[M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ord,
+                                                   a.fold.b_w, a.fold.b_cap, a.fold.err, key, m,
+                                                   sf[M::EXC + ty]);
+            }
+        }
+        if (EXT) {
+            const uint32_t u = drive_hash_keyed(step_key, i);
+            int cnt = 0;
+            while (cnt < a.ext.n_thr && u < a.ext.thr[cnt]) cnt++;
+            for (int q = 0; q < cnt; q++) sf[M::EXC] += a.ext.w;
+        }
+        T S[M::NV];
+#pragma unroll
+        for (int j = 0; j < M::NV; j++) S[j] = sf[j];
+        const T prev = S[0];
+        RhsCtx c;
+        c.chan = a.chan;
+        c.exp_lut = a.exp_lut;
+        c.nmda_lut = a.nmda_lut;
+        c.refr = a.refr;
+        c.refr_len = a.refr_len;
+        c.since_spike = a.last_spike ? (a.step - 1 - a.last_spike[i]) : (1 << 30);
+        int st = a.bdf.st[i];
+        integrate_bdf<M, MASK, T>(a.mp, c, (T)a.k.dt, a.bdf, ld, i, st, S);
+        const bool fired = M::spike(a.mp, prev, S);
+        if (fired && M::kResetsIntegrator) st = 0;
+        a.bdf.st[i] = st;
+        spk[n] = fired;
+        if (fired && a.last_spike) a.last_spike[i] = a.step;
+        float out[M::NV];
+#pragma unroll
+        for (int j = 0; j < M::NV; j++) out[j] = (float)S[j];
+        if (out[0] != out[0]) {
+#pragma unroll
+            for (int j = 0; j < M::NV; j++) row[j][i] = M::init(a.mp, j);
+        } else {
+#pragma unroll
+            for (int j = 0; j < M::ND; j++) row[j][i] = out[j];
+            if (MASK & 1u) row[M::EXC][i] = out[M::EXC];
+            if (MASK & 2u) row[M::INH][i] = out[M::INH];
+            if (MASK & 4u) row[M::NMDA][i] = out[M::NMDA];
         }
     }
     tile_spikes<kItems>(a.out, tile, spk, idx);

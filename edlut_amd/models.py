@@ -279,10 +279,41 @@ class AdExTimeDrivenModel_GPU(TimeDrivenNeuronModel_GPU):
     N_DIFF = 2
 
 
+class HHTimeDrivenModel_GPU(TimeDrivenNeuronModel_GPU):
+    """Hodgkin-Huxley (Kernel/src/neuron_model/HHTimeDrivenModel_GPU_C_INTERFACE.cu;
+    defaults of HHTimeDrivenModel.cpp GetDefaultParameters).  Under BDF the
+    Newton step can run in FP64 (SetNewtonFP64)."""
+
+    NAME = "HHTimeDrivenModel_GPU"
+    MODEL = C.EDLUT_HH
+    PARAMS = (
+        ("e_exc", 0.0),
+        ("e_inh", -80.0),
+        ("e_leak", -65.0),
+        ("g_leak", 10.0),
+        ("c_m", 120.0),
+        ("v_thr", -52.0),
+        ("tau_exc", 5.0),
+        ("tau_inh", 10.0),
+        ("tau_nmda", 20.0),
+        ("g_na", 20000.0),
+        ("g_kd", 6000.0),
+        ("e_na", 50.0),
+        ("e_k", -90.0),
+    )
+    STATE_VARS = ("V_m", "m", "h", "n", "g_exc", "g_inh", "g_nmda", "ext_I")
+    N_DIFF = 4
+
+    def SetNewtonFP64(self, on: bool = True) -> None:
+        self._need()
+        C.check(C.lib().edlut_population_set_newton_fp64(self._pop, int(bool(on))))
+
+
 # NeuronModelFactory registry (Kernel/src/neuron_model/NeuronModelFactory.cu:66-91)
 REGISTRY = {
     LIFTimeDrivenModel_GPU.NAME: LIFTimeDrivenModel_GPU,
     AdExTimeDrivenModel_GPU.NAME: AdExTimeDrivenModel_GPU,
+    HHTimeDrivenModel_GPU.NAME: HHTimeDrivenModel_GPU,
 }
 
 

@@ -58,6 +58,10 @@ const char *edlut_last_error(void);
  *   LIF : e_exc e_inh e_leak v_thr c_m tau_exc tau_inh tau_ref g_leak tau_nmda
  *   AdEx: a b thr_slo_fac v_thr tau_w e_exc e_inh e_reset e_leak g_leak c_m
  *         tau_exc tau_inh tau_nmda
+ *   HH  : e_exc e_inh e_leak g_leak c_m v_thr tau_exc tau_inh tau_nmda g_na g_kd
+ *         e_na e_k
+ * bdf_order (1..6) is the BDF method's order ("bdf_order" parameter,
+ * BDFn_GPU_C_INTERFACE.cuh:275-288); ignored by the other integrators.
  * dt_s is the integration method's "step" parameter (seconds).  State lives on
  * `device` as structure-of-arrays: variable v of neuron i at v*stride+i.
  */
@@ -103,6 +107,12 @@ int edlut_population_get_spikes(const edlut_population *pop, int32_t *idx, int c
  * the monitored-state copies (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:99-103). */
 int edlut_population_get_state(const edlut_population *pop, int var, float *out);
 int edlut_population_set_state(edlut_population *pop, int var, const float *in);
+
+/* HH under BDF: run the whole BDF step (predictor, Newton iterations,
+ * Jacobian, Gauss-Jordan inverse) in FP64, state kept in float between
+ * steps.  Must precede the first update.  No counterpart flag in the
+ * reference, whose BDF is float throughout (BDFn.h:288-423). */
+int edlut_population_set_newton_fp64(edlut_population *pop, int on);
 
 /* Device pointer of the SoA state block (n_vars rows of `stride` floats). */
 int edlut_population_device_state(const edlut_population *pop, float **state);

@@ -125,6 +125,11 @@ template <typename R> struct PopBase {
     R dt_model = 0;                /* elapsedTimeInNeuronModelScale */
     bool EXC = false, INH = false, NMDA = false, EXT_I = false;
     std::vector<int32_t> spikes;   /* InternalSpikeIndexs of the last update */
+    std::vector<int> bdf_state;     /* BDFn::state, empty for other integrators */
+    /* integration_method->resetState(index) (BDFn.h resetState: state = 0) */
+    void reset_integrator(int i) {
+        if (!bdf_state.empty()) bdf_state[i] = 0;
+    }
 
     void init_states(const R *init) {
         state.assign((size_t)N * nvars, 0);
@@ -187,7 +192,7 @@ template <typename R> struct AdEx : PopBase<R> {
         cond[2] = Num<R>::decay(elapsed, inv_tau_nmda);
     }
     /* AdExTimeDrivenModel.cpp:170-218 */
-    void deriv(const R *S, R *D, int /*index*/, R /*elapsed*/) const {
+    void deriv(R *S, R *D, int /*index*/, R /*elapsed*/) const {
         if (S[0] <= v_thr + 6 * thr_slo_fac) {
             R current = 0.0;
             if (this->EXC) current += S[2] * (e_exc - S[0]);
@@ -233,6 +238,7 @@ template <typename R> struct AdEx : PopBase<R> {
             S[0] = e_reset;
             S[1] += b;
             this->fired(index);
+            this->reset_integrator(index);
             return true;
         }
         return false;
@@ -278,7 +284,7 @@ template <typename R> struct LIF : PopBase<R> {
     }
     /* LIFTimeDrivenModel.cpp:170-202.  The refractory test works on the
      * double LastSpikeTime scaled to ms (double * float -> double). */
-    void deriv(const R *S, R *D, int index, R elapsed_time) const {
+    void deriv(R *S, R *D, int index, R elapsed_time) const {
         R current = 0.0;
         if (this->EXC) current += S[1] * (e_exc - S[0]);
         if (this->INH) current += S[2] * (e_inh - S[0]);
@@ -308,6 +314,131 @@ template <typename R> struct LIF : PopBase<R> {
     bool spike(R /*prev_V*/, R *S, int index) {
         if (S[0] > v_thr) {
             S[0] = e_leak;
+            this->fired(index);
+            this->reset_integrator(index);
+            return true;
+        }
+        return false;
+    }
+};
+
+
+/* ------------------------------------------------------------------------ */
+/* HH: Kernel/src/neuron_model/HHTimeDrivenModel.cpp,                         */
+/* Kernel/include/neuron_model/HHTimeDrivenModel.h:372-441                    */
+/* state: V, m, h, n | g_exc, g_inh, g_nmda, ext_I                           */
+/* Channel rates come from a 262144-entry table on [-90, 50] mV built with    */
+/* the float exp (the header's inline generator runs in a translation unit    */
+/* with `using namespace std`).                                              */
+/* ------------------------------------------------------------------------ */
+const int kHHTable = 1024 * 256;
+const float kHHMinV = -90.0f;
+const float kHHMaxV = 50.0f;
+const float kHHAux = (kHHTable - 1) / (kHHMaxV - kHHMinV);
+
+struct HHTable {
+    std::vector<float> v;
+    void build(float v_thr) {
+        v.assign((size_t)kHHTable * 6, 0.0f);
+        for (int i = 0; i < kHHTable; i++) {
+            float V = kHHMinV + ((kHHMaxV - kHHMinV) * i) / (kHHTable - 1);
+            float am, bm, an;
+            if ((13.0f - V + v_thr) == 0.0) am = 1.279999894092334f;
+            else am = 0.32f * (13.0f - V + v_thr) / (std::exp((13.0f - V + v_thr) * 0.25f) - 1.0f);
+            if ((V - v_thr - 40.0f) == 0.0) bm = 1.400030971093304f;
+            else bm = 0.28f * (V - v_thr - 40.0f) / (std::exp((V - v_thr - 40.0f) * 0.2f) - 1.0f);
+            float ah = 0.128f * std::exp((17.0f - V + v_thr) * 0.05555555f);
+            float bh = 4.0f / (1.0f + std::exp((40.0f - V + v_thr) * 0.2f));
+            if ((15.0f - V + v_thr) == 0.0) an = 0.1599999867615418f;
+            else an = 0.032f * (15.0f - V + v_thr) / (std::exp((15.0f - V + v_thr) * 0.2f) - 1.0f);
+            float bn = 0.5f * std::exp((10.0f - V + v_thr) * 0.025f);
+            float *o = &v[(size_t)i * 6];
+            o[0] = am; o[1] = bm; o[2] = ah; o[3] = bh; o[4] = an; o[5] = bn;
+        }
+    }
+    /* Get_channel_values: clamped truncating index */
+    const float *at(float V) const {
+        int position = trunc_i32_x86((V - kHHMinV) * kHHAux);
+        if (position < 0) position = 0;
+        else if (position > (kHHTable - 1)) position = kHHTable - 1;
+        return &v[(size_t)position * 6];
+    }
+};
+
+template <typename R> struct HH : PopBase<R> {
+    R e_exc, e_inh, e_leak, g_leak, c_m, inv_c_m, v_thr, tau_exc, inv_tau_exc, tau_inh,
+        inv_tau_inh, tau_nmda, inv_tau_nmda, g_na, g_kd, e_na, e_k;
+    NmdaTable nmda;
+    HHTable table;
+    R cond[3];
+
+    /* Default parameters: HHTimeDrivenModel.cpp GetDefaultParameters (order of p[]):
+     * e_exc e_inh e_leak g_leak c_m v_thr tau_exc tau_inh tau_nmda g_na g_kd e_na e_k */
+    void set_params(const float *p) {
+        e_exc = p[0]; e_inh = p[1]; e_leak = p[2]; g_leak = p[3]; c_m = p[4]; v_thr = p[5];
+        tau_exc = p[6]; tau_inh = p[7]; tau_nmda = p[8]; g_na = p[9]; g_kd = p[10];
+        e_na = p[11]; e_k = p[12];
+        inv_c_m = Num<R>::inv(c_m);
+        inv_tau_exc = Num<R>::inv(tau_exc);
+        inv_tau_inh = Num<R>::inv(tau_inh);
+        inv_tau_nmda = Num<R>::inv(tau_nmda);
+        nmda.build(p[0], p[1]);
+        table.build(p[5]);
+        this->nvars = 8;
+        this->ndiff = 4;
+    }
+    R g_nmda_inf(R V) const { return (R)nmda.get((float)V, (float)e_inh); }
+    /* HHTimeDrivenModel.cpp:136-160: gates start at their steady state at e_leak */
+    void init_states() {
+        const float *c = table.at((float)e_leak);
+        float m_inf = c[0] / (c[0] + c[1]);
+        float h_inf = c[2] / (c[2] + c[3]);
+        float n_inf = c[4] / (c[4] + c[5]);
+        R init[8] = {e_leak, (R)m_inf, (R)h_inf, (R)n_inf, 0, 0, 0, 0};
+        PopBase<R>::init_states(init);
+    }
+    void calc_cond(R elapsed) {
+        cond[0] = Num<R>::decay(elapsed, inv_tau_exc);
+        cond[1] = Num<R>::decay(elapsed, inv_tau_inh);
+        cond[2] = Num<R>::decay(elapsed, inv_tau_nmda);
+    }
+    /* HHTimeDrivenModel.cpp:197-264: clamps the state it is given, in place. */
+    void deriv(R *S, R *D, int /*index*/, R /*elapsed*/) const {
+        if (S[0] < e_k) S[0] = e_k;
+        if (S[0] > e_na - 1.0f) S[0] = e_na - 1.0f;
+        if (S[1] < 0) S[1] = 0.0f;
+        if (S[1] > 1) S[1] = 1.0f;
+        if (S[2] < 0) S[2] = 0.0f;
+        if (S[2] > 1) S[2] = 1.0f;
+        if (S[3] < 0) S[3] = 0.0f;
+        if (S[3] > 0.72) S[3] = 0.72f;
+        const R V = S[0], m = S[1], h = S[2], n = S[3];
+        const float *values = table.at((float)V);
+        R current = 0.0;
+        if (this->EXC) current += S[4] * (e_exc - S[0]);
+        if (this->INH) current += S[5] * (e_inh - S[0]);
+        if (this->NMDA) {
+            R ginf = g_nmda_inf(S[0]);
+            current += S[6] * ginf * (e_exc - S[0]);
+        }
+        current += S[7];
+        D[0] = (current + g_leak * (e_leak - V) + g_na * m * m * m * h * (e_na - V) +
+                g_kd * n * n * n * n * (e_k - V)) * inv_c_m;
+        const R am = values[0], bm = values[1], ah = values[2], bh = values[3], an = values[4],
+                bn = values[5];
+        D[1] = (am * (1.0f - m) - bm * m);
+        D[2] = (ah * (1.0f - h) - bh * h);
+        D[3] = (an * (1.0f - n) - bn * n);
+    }
+    void decay(R *S) const {
+        const R limit = (R)1e-9f;
+        if (this->EXC) { if (S[4] < limit) S[4] = 0.0f; else S[4] *= cond[0]; }
+        if (this->INH) { if (S[5] < limit) S[5] = 0.0f; else S[5] *= cond[1]; }
+        if (this->NMDA) { if (S[6] < limit) S[6] = 0.0f; else S[6] *= cond[2]; }
+    }
+    /* HHTimeDrivenModel.cpp:187-193: upward crossing of -30 mV, no reset. */
+    bool spike(R prev_V, R *S, int index) {
+        if (S[0] > -30.0f && prev_V < -30.0f) {
             this->fired(index);
             return true;
         }
@@ -382,6 +513,158 @@ template <typename M, typename R> void step_rk4(M &m) {
     }
 }
 
+
+/* BDFn: Kernel/include/integration_method/BDFn.h:288-423 (step), 604-669
+ * (coefficients), 682-722 (Jacobian), 724-786 (Gauss-Jordan inverse).
+ * Per neuron: Euler predictor while the method has no history (state 0),
+ * else the difference-table predictor; then up to 10 Newton iterations on
+ * the BDF corrector until the max-norm change is <= 1e-6; the order rises by
+ * one per step up to bdf_order and restarts at 0 on a spike.  The
+ * conductances of the iterate are decayed copies; the stored ones decay once
+ * after the update. */
+template <typename R> struct BdfStore {
+    int order = 1;
+    std::vector<std::vector<R>> D, Prev;   /* [order][N*nd], [order-1][N*nd] */
+    float C[7][7];
+    void init(int ord, int N, int nd) {
+        order = ord;
+        D.assign(ord, std::vector<R>((size_t)N * nd, 0));
+        Prev.assign(ord > 1 ? ord - 1 : 0, std::vector<R>((size_t)N * nd, 0));
+        std::memset(C, 0, sizeof(C));
+        C[0][0] = 1.0f; C[0][1] = 1.0f;
+        C[1][0] = 1.0f; C[1][1] = 1.0f;
+        C[2][0] = 2.0f / 3.0f; C[2][1] = 4.0f / 3.0f; C[2][2] = -1.0f / 3.0f;
+        C[3][0] = 6.0f / 11.0f; C[3][1] = 18.0f / 11.0f; C[3][2] = -9.0f / 11.0f; C[3][3] = 2.0f / 11.0f;
+        C[4][0] = 12.0f / 25.0f; C[4][1] = 48.0f / 25.0f; C[4][2] = -36.0f / 25.0f;
+        C[4][3] = 16.0f / 25.0f; C[4][4] = -3.0f / 25.0f;
+        C[5][0] = 60.0f / 137.0f; C[5][1] = 300.0f / 137.0f; C[5][2] = -300.0f / 137.0f;
+        C[5][3] = 200.0f / 137.0f; C[5][4] = -75.0f / 137.0f; C[5][5] = 12.0f / 137.0f;
+        C[6][0] = 60.0f / 147.0f; C[6][1] = 360.0f / 147.0f; C[6][2] = -450.0f / 147.0f;
+        C[6][3] = 400.0f / 147.0f; C[6][4] = -225.0f / 147.0f; C[6][5] = 72.0f / 147.0f;
+        C[6][6] = -10.0f / 147.0f;
+    }
+};
+
+/* invermat (BDFn.h:724-786): Gauss-Jordan with row swap on a zero pivot. */
+template <typename R> void invermat(int n, const R *a, R *ainv) {
+    if (n == 1) {
+        ainv[0] = 1.0f / a[0];
+        return;
+    }
+    R la[kMaxVars * kMaxVars], li[kMaxVars * kMaxVars];
+    for (int i = 0; i < n * n; i++) { la[i] = a[i]; li[i] = 0; }
+    for (int i = 0; i < n; i++) li[i * n + i] = 1.0f;
+    for (int s = 0; s < n; s++) {
+        R element = la[s * n + s];
+        if (element == 0) {
+            for (int r = s + 1; r < n; r++) {
+                element = la[r * n + s];
+                if (element != 0) {
+                    for (int m = 0; m < n; m++) {
+                        R v = la[r * n + m]; la[r * n + m] = la[s * n + m]; la[s * n + m] = v;
+                        v = li[r * n + m]; li[r * n + m] = li[s * n + m]; li[s * n + m] = v;
+                    }
+                    break;
+                }
+            }
+        }
+        R inv_element = 1.0f / element;
+        for (int j = 0; j < n; j++) {
+            la[s * n + j] *= inv_element;
+            li[s * n + j] *= inv_element;
+        }
+        for (int i = 0; i < n; i++) {
+            if (i != s) {
+                R coef = -la[i * n + s];
+                for (int j = 0; j < n; j++) {
+                    la[i * n + j] += la[s * n + j] * coef;
+                    li[i * n + j] += li[s * n + j] * coef;
+                }
+            }
+        }
+    }
+    for (int i = 0; i < n * n; i++) ainv[i] = li[i];
+}
+
+template <typename M, typename R> void step_bdf(M &m, BdfStore<R> &b) {
+    const int nd = m.ndiff, nv = m.nvars;
+    const R dt = m.dt_model;
+    R F[kMaxVars], P[kMaxVars], P1[kMaxVars], Cc[kMaxVars];
+    R jac[kMaxVars * kMaxVars], J[kMaxVars * kMaxVars], iJ[kMaxVars * kMaxVars];
+    R Jx[kMaxVars], Jpos[kMaxVars], Jneg[kMaxVars];
+    for (int index = 0; index < m.N; index++) {
+        R *S = m.at(index);
+        R prev = S[0];
+        int &st = m.bdf_state[index];
+        if (st == 0) {
+            m.deriv(S, F, index, dt);
+            for (int j = 0; j < nd; j++) P[j] = S[j] + dt * F[j];
+        } else {
+            for (int j = 0; j < nd; j++) {
+                P[j] = S[j];
+                for (int i = 0; i < st; i++) P[j] += b.D[i][(size_t)index * nd + j];
+            }
+        }
+        for (int i = nd; i < nv; i++) P[i] = S[i];
+        m.decay(P);
+        R epsi = 1.0f;
+        int k = 0;
+        while (epsi > 1e-6 && k < 10) {
+            m.deriv(P, F, index, dt);
+            for (int j = 0; j < nd; j++) {
+                Cc[j] = b.C[st][0] * dt * F[j] + b.C[st][1] * S[j];
+                for (int i = 1; i < st; i++) Cc[j] += b.C[st][i + 1] * b.Prev[i - 1][(size_t)index * nd + j];
+            }
+            /* Jacobian (one-sided, eps = 0.1 dt) at the iterate */
+            const R eps = dt * 0.1f;
+            const R inv_eps = 1.0f / eps;
+            for (int q = 0; q < nv; q++) Jx[q] = P[q];
+            m.deriv(Jx, Jpos, index, dt);
+            for (int j = 0; j < nd; j++) {
+                for (int q = 0; q < nv; q++) Jx[q] = P[q];
+                Jx[j] -= eps;
+                m.deriv(Jx, Jneg, index, dt);
+                for (int z = 0; z < nd; z++) jac[z * nd + j] = (Jpos[z] - Jneg[z]) * inv_eps;
+            }
+            for (int z = 0; z < nd; z++)
+                for (int t = 0; t < nd; t++) {
+                    J[z * nd + t] = b.C[st][0] * dt * jac[z * nd + t];
+                    if (z == t) J[z * nd + t] -= 1;
+                }
+            invermat<R>(nd, J, iJ);
+            for (int z = 0; z < nd; z++) {
+                R aux = 0.0;
+                for (int t = 0; t < nd; t++) aux += iJ[z * nd + t] * (P[t] - Cc[t]);
+                P1[z] = aux + P[z];
+            }
+            R aux2 = 0.0f;
+            for (int z = 0; z < nd; z++) {
+                R aux = std::fabs(P1[z] - P[z]);
+                if (aux > aux2) aux2 = aux;
+            }
+            for (int z = 0; z < nd; z++) P[z] = P1[z];
+            epsi = aux2;
+            k++;
+        }
+        if (st < b.order) st++;
+        for (int j = 0; j < nd; j++) {
+            const size_t q = (size_t)index * nd + j;
+            for (int i = st - 1; i > 0; i--) b.D[i][q] = -b.D[i - 1][q];
+            b.D[0][q] = P[j] - S[j];
+            for (int i = 1; i < st; i++) b.D[i][q] += b.D[i - 1][q];
+        }
+        if (st > 1) {
+            for (int i = st - 2; i > 0; i--)
+                for (int j = 0; j < nd; j++) b.Prev[i][(size_t)index * nd + j] = b.Prev[i - 1][(size_t)index * nd + j];
+            for (int j = 0; j < nd; j++) b.Prev[0][(size_t)index * nd + j] = S[j];
+        }
+        for (int j = 0; j < nd; j++) S[j] = P[j];
+        m.decay(S);
+        m.add_elapsed(index);
+        if (m.spike(prev, S, index)) m.spikes.push_back(index);
+    }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Type-erased population handle.                                           */
 /* ------------------------------------------------------------------------ */
@@ -402,8 +685,10 @@ struct PopIface {
 template <typename M, typename R> struct Pop : PopIface {
     M m;
     int integrator = ORC_EULER;
+    BdfStore<R> bdf;
+    bool round_to_float = false;   /* FP64 Newton mode: state kept in float between steps */
 
-    Pop(int integ, const float *params, float dt_s, int n, uint32_t mask) {
+    Pop(int integ, int bdf_order, const float *params, float dt_s, int n, uint32_t mask) {
         integrator = integ;
         m.set_params(params);
         m.N = n;
@@ -418,6 +703,10 @@ template <typename M, typename R> struct Pop : PopIface {
         m.init_states();
         if (integ == ORC_RK4) m.calc_cond((R)(dt_model * 0.5f));
         else m.calc_cond((R)dt_model);
+        if (integ == ORC_BDF) {
+            bdf.init(bdf_order, n, m.ndiff);
+            m.bdf_state.assign(n, 0);
+        }
     }
     int nvars() override { return m.nvars; }
     int ndiff() override { return m.ndiff; }
@@ -430,7 +719,9 @@ template <typename M, typename R> struct Pop : PopIface {
     }
     /* AdExTimeDrivenModel.cpp:74-79: one add per input, in arrival order. */
     void add_input(int target, int type, float w) override {
-        m.state[(size_t)target * m.nvars + m.ndiff + type] += (R)w;
+        R &g = m.state[(size_t)target * m.nvars + m.ndiff + type];
+        if (round_to_float) g = (R)((float)g + w);   /* state held in float */
+        else g += (R)w;
     }
     /* AdExTimeDrivenModel.cpp:82-89: ext_I holds the total input current. */
     void set_ext(const float *I) override {
@@ -443,7 +734,10 @@ template <typename M, typename R> struct Pop : PopIface {
         m.spikes.clear();
         if (integrator == ORC_EULER) step_euler<M, R>(m);
         else if (integrator == ORC_RK2) step_rk2<M, R>(m);
-        else step_rk4<M, R>(m);
+        else if (integrator == ORC_RK4) step_rk4<M, R>(m);
+        else step_bdf<M, R>(m, bdf);
+        if (round_to_float)
+            for (auto &v : m.state) v = (R)(float)v;
         /* A NaN membrane potential resets that neuron to its initial state. */
         for (int i = 0; i < m.N; i++) {
             R v = m.state[(size_t)i * m.nvars];
@@ -535,15 +829,19 @@ uint32_t drive_hash(uint64_t seed, int64_t step, uint32_t neuron) {
     return mix32(neuron * 0x9E3779B9u + key);
 }
 
-PopIface *make_pop(int model, int integ, const float *p, float dt_s, int n, uint32_t mask,
-                   bool fp64) {
+PopIface *make_pop(int model, int integ, int bdf_order, const float *p, float dt_s, int n,
+                   uint32_t mask, bool fp64) {
     if (model == ORC_ADEX) {
-        if (fp64) return new Pop<AdEx<double>, double>(integ, p, dt_s, n, mask);
-        return new Pop<AdEx<float>, float>(integ, p, dt_s, n, mask);
+        if (fp64) return new Pop<AdEx<double>, double>(integ, bdf_order, p, dt_s, n, mask);
+        return new Pop<AdEx<float>, float>(integ, bdf_order, p, dt_s, n, mask);
     }
     if (model == ORC_LIF) {
-        if (fp64) return new Pop<LIF<double>, double>(integ, p, dt_s, n, mask);
-        return new Pop<LIF<float>, float>(integ, p, dt_s, n, mask);
+        if (fp64) return new Pop<LIF<double>, double>(integ, bdf_order, p, dt_s, n, mask);
+        return new Pop<LIF<float>, float>(integ, bdf_order, p, dt_s, n, mask);
+    }
+    if (model == ORC_HH) {
+        if (fp64) return new Pop<HH<double>, double>(integ, bdf_order, p, dt_s, n, mask);
+        return new Pop<HH<float>, float>(integ, bdf_order, p, dt_s, n, mask);
     }
     return nullptr;
 }
@@ -586,9 +884,20 @@ uint32_t orc_drive_hash(uint64_t seed, int64_t step, uint32_t neuron) {
     return drive_hash(seed, step, neuron);
 }
 
-void *orc_pop_create(int model, int integrator, int /*bdf_order*/, const float *params,
+void *orc_pop_create(int model, int integrator, int bdf_order, const float *params,
                      int /*n_params*/, float dt_s, int n_neurons, uint32_t syn_mask, int fp64) {
-    return make_pop(model, integrator, params, dt_s, n_neurons, syn_mask, fp64 != 0);
+    if (integrator == ORC_BDF && (bdf_order < 1 || bdf_order > 6)) return nullptr;
+    /* fp64: 0 float restatement, 1 FP64 oracle, 2 FP64 with the state rounded
+     * to float after every step (the HIP path's FP64 Newton mode for HH). */
+    PopIface *pi = make_pop(model, integrator, bdf_order, params, dt_s, n_neurons, syn_mask, fp64 != 0);
+    if (pi && fp64 == 2) {
+        if (model != ORC_HH) {
+            delete pi;
+            return nullptr;
+        }
+        ((Pop<HH<double>, double> *)pi)->round_to_float = true;
+    }
+    return pi;
 }
 
 void orc_pop_destroy(void *pop) { delete (PopIface *)pop; }

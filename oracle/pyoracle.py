@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_DUMP = os.path.join(HERE, "_ref", "ref_dump")
 
-MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1}
+MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1, "HHTimeDrivenModel_GPU": 2}
 INTEGRATORS = {"Euler": 0, "RK2": 1, "RK4": 2, "BDF": 3}
 
 P = ctypes.c_void_p
@@ -116,7 +116,7 @@ class Population:
     """One population of the oracle (float restatement, or FP64 with fp64=True)."""
 
     def __init__(self, model: str, integrator: str, params, n: int, dt_s=np.float32(1e-4),
-                 mask: int = 3, fp64: bool = False, bdf_order: int = 1):
+                 mask: int = 3, fp64=False, bdf_order: int = 1):
         p = np.ascontiguousarray(params, np.float32)
         self.n = n
         self.h = lib().orc_pop_create(MODELS[model], INTEGRATORS[integrator], bdf_order, _p(p), len(p),

@@ -6,6 +6,13 @@ import numpy as np
 ADEX_DEFAULTS = np.array([1.0, 9.0, 2.0, -50.0, 50.0, 0.0, -80.0, -80.0, -65.0, 10.0, 110.0, 5.0, 10.0, 20.0],
                          np.float32)
 LIF_DEFAULTS = np.array([0.0, -80.0, -65.0, -50.0, 2.0, 5.0, 10.0, 1.0, 0.2, 20.0], np.float32)
+HH_DEFAULTS = np.array([0.0, -80.0, -65.0, 10.0, 120.0, -52.0, 5.0, 10.0, 20.0, 20000.0, 6000.0, 50.0, -90.0],
+                       np.float32)
+
+
+def defaults(model: str) -> np.ndarray:
+    return {"AdExTimeDrivenModel_GPU": ADEX_DEFAULTS, "LIFTimeDrivenModel_GPU": LIF_DEFAULTS,
+            "HHTimeDrivenModel_GPU": HH_DEFAULTS}[model]
 
 
 def np_out_csr(n_src, n_tgt, max_delay, src, tgt, delay, typ, w):
