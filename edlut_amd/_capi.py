@@ -36,6 +36,7 @@ EXPORTS = (
     "edlut_network_add_poisson",
     "edlut_network_set_external",
     "edlut_network_run",
+    "edlut_network_set_graph",
     "edlut_network_synchronize",
     "edlut_network_spike_log",
     "edlut_network_clear_log",
@@ -95,6 +96,7 @@ def _sig(lib):
         "edlut_network_add_poisson": (I32, [P, I32, P, F32, P, P, P]),
         "edlut_network_set_external": (I32, [P, I32, P, F32, U64]),
         "edlut_network_run": (I32, [P, I64, I32]),
+        "edlut_network_set_graph": (I32, [P, I32]),
         "edlut_network_synchronize": (I32, [P]),
         "edlut_network_spike_log": (I32, [P, P, P, I64, P]),
         "edlut_network_clear_log": (I32, [P]),

@@ -22,6 +22,14 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // range, scans its own range and copies every tile's indices to the output,
 // which is therefore ascending.  The last workgroup writes the total.
 // ---------------------------------------------------------------------------
+// Update index of a launch: an immediate value, or read from device memory
+// (graph replays advance the counter on the device between steps).
+struct StepRef {
+    const int32_t *ptr;
+    int32_t imm;
+    __device__ __forceinline__ int32_t get() const { return ptr ? *ptr : imm; }
+};
+
 struct TileOut {
     int32_t *tile_count;   // [n_tiles]
     int32_t *staging;      // [n_tiles * tile_size]
@@ -62,8 +70,11 @@ struct CompactArgs {
     const int32_t *staging;
     int n_tiles;
     int tile_size;         // neurons per tile (staging stride)
+    // output slot (step % ring) of a ring of `ring` lists of `slot_len` indices
     int32_t *out_idx;      // ascending indices
     int32_t *out_count;    // total
+    int ring;
+    int64_t slot_len;
     // optional spike log; the entry count is double-buffered by step parity:
     // read log_count[step & 1], write log_count[(step + 1) & 1]
     int32_t *log_step;
@@ -71,12 +82,15 @@ struct CompactArgs {
     int64_t *log_count;
     int64_t log_cap;
     uint32_t *err;         // bit 1: log overflow
-    int32_t step;
+    StepRef step;
 };
 
 constexpr int kCompactTiles = kBlock;   // tiles per compaction workgroup
 
 __global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) {
+    const int32_t step = a.step.get();
+    const int slot = step % a.ring;
+    int32_t *const out_idx = a.out_idx + (int64_t)slot * a.slot_len;
     __shared__ int64_t s_red[kWavesPerBlock];
     __shared__ int32_t s_scan[kBlock];
     __shared__ int64_t s_base;
@@ -111,7 +125,7 @@ __global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) {
         __syncthreads();
     }
     const int64_t base = s_base;
-    const int64_t log_base = a.log_count ? a.log_count[a.step & 1] : 0;
+    const int64_t log_base = a.log_count ? a.log_count[step & 1] : 0;
     // 3. copy: one thread per tile (most tiles hold no spike)
     {
         const int q = (int)threadIdx.x;
@@ -121,25 +135,30 @@ __global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) {
         const int64_t start = base + lo;
         for (int j = 0; j < c; j++) {
             const int32_t id = a.staging[(int64_t)t * a.tile_size + j];
-            a.out_idx[start + j] = id;
+            out_idx[start + j] = id;
             if (a.log_idx) {
                 const int64_t lp = log_base + start + j;
                 if (lp < a.log_cap) {
                     a.log_idx[lp] = id;
-                    a.log_step[lp] = a.step;
+                    a.log_step[lp] = step;
                 }
             }
         }
     }
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
         const int64_t total = base + s_scan[kBlock - 1];
-        *a.out_count = (int32_t)total;
+        a.out_count[slot] = (int32_t)total;
         if (a.log_count) {
             const int64_t nl = log_base + total;
             if (nl > a.log_cap) atomicOr(a.err, 2u);
-            a.log_count[(a.step + 1) & 1] = nl;
+            a.log_count[(step + 1) & 1] = nl;
         }
     }
+}
+
+// Advances a device step counter (last node of a captured step).
+__global__ void k_advance_step(int32_t *step) {
+    if (threadIdx.x == 0) *step = *step + 1;
 }
 
 // Counter-based hash of (seed, step, neuron) for the external drive.

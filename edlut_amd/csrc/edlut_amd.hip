@@ -213,8 +213,9 @@ struct edlut_population {
 
     edlut_network *net = nullptr;
 
-    FoldTarget fold_target(int par) const {
+    FoldTarget fold_target(int par, bool by_step = false) const {
         FoldTarget t;
+        t.par_by_step = by_step ? 1 : 0;
         t.state = state;
         t.stride = stride;
         t.ndiff = nd;
@@ -299,6 +300,15 @@ struct edlut_network {
     std::vector<hipEvent_t> ev;
     int64_t timed = 0;
     bool timing = false;
+    // graph replay: a chunk of `chunk` captured steps reading the update
+    // index from d_step; rebuilt when the network's configuration changes
+    int32_t *d_step = nullptr;
+    hipGraphExec_t graph = nullptr;
+    void (*graph_fn)(UpdateArgs) = nullptr;
+    int graph_chunk = 0;
+    bool graph_timed = false;
+    bool dirty = true;
+    bool use_graph = true;
 };
 
 // ---------------------------------------------------------------------------
@@ -381,7 +391,7 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.stride = p->stride;
     a.n = p->n;
     a.n_tiles = p->n_tiles;
-    a.step = step;
+    a.step = StepRef{nullptr, step};
     a.mp = p->mp;
     a.k = p->k;
     a.exp_lut = p->exp_lut;
@@ -399,8 +409,9 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
 }
 
 void launch_compact(hipStream_t st, int32_t *tile_count, int32_t *staging, int n_tiles,
-                    int tile_size, int32_t *out_idx, int32_t *out_count, int32_t *log_step, int32_t *log_idx,
-                    int64_t *log_count, int64_t log_cap, uint32_t *err, int32_t step) {
+                    int tile_size, int32_t *out_idx, int32_t *out_count, int ring, int64_t slot_len,
+                    int32_t *log_step, int32_t *log_idx, int64_t *log_count, int64_t log_cap,
+                    uint32_t *err, StepRef step) {
     CompactArgs c;
     c.tile_count = tile_count;
     c.staging = staging;
@@ -408,6 +419,8 @@ void launch_compact(hipStream_t st, int32_t *tile_count, int32_t *staging, int n
     c.tile_size = tile_size;
     c.out_idx = out_idx;
     c.out_count = out_count;
+    c.ring = ring;
+    c.slot_len = slot_len;
     c.log_step = log_step;
     c.log_idx = log_idx;
     c.log_count = log_count;
@@ -420,7 +433,7 @@ void launch_compact(hipStream_t st, int32_t *tile_count, int32_t *staging, int n
 
 int launch_fold(edlut_population *p, int par, int n_events, bool ring, const RingEvents *re,
                 const PushEvents *pe) {
-    FoldTarget t = p->fold_target(par);
+    FoldTarget t = p->fold_target(par, ring);
     if (ring) {
         const int blocks = 512;
         hipLaunchKernelGGL((k_fold_ring<kCount, 32>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
@@ -747,7 +760,7 @@ int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
     p->started = true;
     hipLaunchKernelGGL(fn, dim3(p->n_tiles), dim3(kBlock), 0, p->stream, a);
     launch_compact(p->stream, p->tile_count, p->staging, p->n_tiles, kUpdTile, p->spk_idx, p->spk_cnt,
-                   nullptr, nullptr, nullptr, 0, p->err, (int32_t)step);
+                   1, 0, nullptr, nullptr, nullptr, 0, p->err, StepRef{nullptr, (int32_t)step});
     HIP_TRY(hipGetLastError());
     // 3. hand the compacted spikes back (synchronous, like the reference)
     HIP_TRY(hipMemcpyAsync(p->h_cnt, p->spk_cnt, sizeof(int32_t), hipMemcpyDeviceToHost, p->stream));
@@ -839,7 +852,7 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
         (rc = dalloc(&n->tile_count, padded_tiles(pop->n_tiles))) ||
         (rc = dalloc(&n->staging, (size_t)pop->n_tiles * kUpdTile)) ||
         (rc = dalloc(&n->log_step, n->log_cap)) || (rc = dalloc(&n->log_idx, n->log_cap)) ||
-        (rc = dalloc(&n->log_count, 2))) {
+        (rc = dalloc(&n->log_count, 2)) || (rc = dalloc(&n->d_step, 1))) {
         delete n;
         return rc;
     }
@@ -861,8 +874,9 @@ int edlut_network_destroy(edlut_network *n) {
         for (void *q : dev)
             if (q) (void)hipFree(q);
     }
+    if (n->graph) (void)hipGraphExecDestroy(n->graph);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
-                   n->log_idx, n->log_count};
+                   n->log_idx, n->log_count, n->d_step};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     for (hipEvent_t e : n->ev) (void)hipEventDestroy(e);
@@ -915,6 +929,7 @@ int edlut_network_add_recurrent(edlut_network *n, const int64_t *row_ptr, const 
     int rc = add_synapses(n, s, row_ptr, tgt_type, w);
     if (rc) return rc;
     n->sources.push_back(s);
+    n->dirty = true;
     n->self_src = (int)n->sources.size() - 1;
     n->total_syn += (uint64_t)row_ptr[(size_t)s.n * n->max_delay];
     return n->pop->reserve_events(n->total_syn + 1);
@@ -942,6 +957,7 @@ int edlut_network_add_poisson(edlut_network *n, int n_src, const uint32_t *lcg_s
     HIP_TRY(hipMemset(s.ring_cnt, 0, sizeof(int32_t) * n->ring));
     HIP_TRY(hipMemset(s.tile_count, 0, sizeof(int32_t) * padded_tiles(tiles)));
     n->sources.push_back(s);
+    n->dirty = true;
     n->total_syn += (uint64_t)row_ptr[(size_t)s.n * n->max_delay];
     return n->pop->reserve_events(n->total_syn + 1);
 }
@@ -960,6 +976,95 @@ int edlut_network_set_external(edlut_network *n, int n_thr, const uint32_t *thre
     n->ext.w = w_ext;
     for (int j = 0; j < n_thr; j++) n->ext.thr[j] = thresholds[j];
     if (n->has_ext) n->pop->mask |= EDLUT_SYN_EXC;
+    n->dirty = true;
+    return 0;
+}
+
+// Enqueues every kernel of one network step on the population's stream.
+static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingEvents &re0,
+                        hipEvent_t ev0, hipEvent_t ev1) {
+    edlut_population *p = n->pop;
+    // Poisson devices fire at update k (their spikes feed later updates).
+    for (Source &s : n->sources) {
+        if (s.self) continue;
+        PoissonArgs pa;
+        std::memset(&pa, 0, sizeof(pa));
+        pa.lcg = s.lcg;
+        pa.n = (int)s.n;
+        pa.n_tiles = (int)((s.n + kBlock - 1) / kBlock);
+        pa.rate = s.rate;
+        pa.rate_dt = (double)s.rate * (double)p->dt_s;
+        pa.out = TileOut{s.tile_count, s.staging};
+        hipLaunchKernelGGL(k_poisson, dim3(pa.n_tiles), dim3(kBlock), 0, p->stream, pa);
+        launch_compact(p->stream, s.tile_count, s.staging, pa.n_tiles, kBlock, s.ring_idx, s.ring_cnt,
+                       n->ring, s.n, nullptr, nullptr, nullptr, 0, p->err, step);
+    }
+    // Deliver the events consumed by update k.
+    if (re0.n_sources > 0) {
+        RingEvents re = re0;
+        re.step = step;
+        int rc = launch_fold(p, 0, 0, true, &re, nullptr);
+        if (rc) return rc;
+    }
+    UpdateArgs a = make_update_args(p, 0);
+    a.step = step;
+    a.ext = n->ext;
+    a.out = TileOut{n->tile_count, n->staging};
+    a.fold = p->fold_target(0, true);
+    if (ev0) HIP_TRY(hipEventRecord(ev0, p->stream));
+    hipLaunchKernelGGL(fn, dim3(p->n_tiles), dim3(kBlock), 0, p->stream, a);
+    if (ev1) HIP_TRY(hipEventRecord(ev1, p->stream));
+    launch_compact(p->stream, n->tile_count, n->staging, p->n_tiles, kUpdTile, n->own_ring_idx,
+                   n->own_ring_cnt, n->ring, p->n, n->log_step, n->log_idx, n->log_count, n->log_cap,
+                   p->err, step);
+    return 0;
+}
+
+static RingEvents ring_events(edlut_network *n) {
+    RingEvents re;
+    std::memset(&re, 0, sizeof(re));
+    re.n_sources = (int)n->sources.size();
+    re.max_delay = n->max_delay;
+    re.ring = n->ring;
+    for (int q = 0; q < re.n_sources; q++) {
+        const Source &s = n->sources[q];
+        re.src[q] = RingSource{s.ring_idx, s.ring_cnt, s.row_ptr, s.tgt_type, s.w, s.n};
+    }
+    return re;
+}
+
+static constexpr int kGraphChunk = 16;
+
+// Captures kGraphChunk steps that read the update index from d_step and
+// advance it; per-step timing events bracket each update kernel.
+static int build_graph(edlut_network *n, UpdateFn fn, bool timed) {
+    edlut_population *p = n->pop;
+    if (n->graph) {
+        (void)hipGraphExecDestroy(n->graph);
+        n->graph = nullptr;
+    }
+    const RingEvents re = ring_events(n);
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int i = 0; i < kGraphChunk && rc == 0; i++) {
+        hipEvent_t e0 = timed ? n->ev[2 * i] : nullptr, e1 = timed ? n->ev[2 * i + 1] : nullptr;
+        rc = enqueue_step(n, StepRef{n->d_step, 0}, fn, re, e0, e1);
+        hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, p->stream, n->d_step);
+    }
+    hipError_t e = hipStreamEndCapture(p->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(EDLUT_E_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    e = hipGraphInstantiate(&n->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(EDLUT_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
+    n->graph_chunk = kGraphChunk;
+    n->graph_fn = fn;
+    n->graph_timed = timed;
+    n->dirty = false;
     return 0;
 }
 
@@ -984,56 +1089,39 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
         HIP_TRY(hipStreamSynchronize(p->stream));
     }
     HIP_TRY(hipMemsetAsync(p->bump, 0, 2 * sizeof(uint64_t), p->stream));
-    RingEvents re;
-    std::memset(&re, 0, sizeof(re));
-    re.n_sources = (int)n->sources.size();
-    re.max_delay = n->max_delay;
-    re.ring = n->ring;
-    for (int q = 0; q < re.n_sources; q++) {
-        const Source &s = n->sources[q];
-        re.src[q] = RingSource{s.ring_idx, s.ring_cnt, s.row_ptr, s.tgt_type, s.w, s.n};
-    }
     UpdateFn fn = pick_update(p->model, p->integ, p->mask, n->has_ext, p->newton_fp64);
+    if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model/integrator");
     p->started = true;
-    const bool timed = n->timing && 2 * (size_t)n_steps <= n->ev.size();
-    n->timed = timed ? n_steps : 0;
-    for (int64_t k = first_step; k < first_step + n_steps; k++) {
-        const int slot = (int)(k % n->ring);
-        // Poisson devices fire at update k (their spikes feed later updates).
-        for (Source &s : n->sources) {
-            if (s.self) continue;
-            PoissonArgs pa;
-            std::memset(&pa, 0, sizeof(pa));
-            pa.lcg = s.lcg;
-            pa.n = (int)s.n;
-            pa.n_tiles = (int)((s.n + kBlock - 1) / kBlock);
-            pa.rate = s.rate;
-            pa.rate_dt = (double)s.rate * (double)p->dt_s;
-            pa.out = TileOut{s.tile_count, s.staging};
-            hipLaunchKernelGGL(k_poisson, dim3(pa.n_tiles), dim3(kBlock), 0, p->stream, pa);
-            launch_compact(p->stream, s.tile_count, s.staging, pa.n_tiles, kBlock,
-                           s.ring_idx + (int64_t)slot * s.n, s.ring_cnt + slot, nullptr, nullptr,
-                           nullptr, 0, p->err, (int32_t)k);
-        }
-        // Deliver the events consumed by update k.
-        if (re.n_sources > 0) {
-            re.step = (int32_t)k;
-            int rc = launch_fold(p, (int)(k & 1), 0, true, &re, nullptr);
+    const bool want_timing = n->timing && n->ev.size() >= 2 * (size_t)kGraphChunk;
+    int64_t k = first_step;
+    n->timed = 0;
+    if (n->use_graph && n_steps >= kGraphChunk) {
+        if (n->dirty || !n->graph || n->graph_timed != want_timing || n->graph_fn != fn) {
+            int rc = build_graph(n, fn, want_timing);
             if (rc) return rc;
         }
-        UpdateArgs a = make_update_args(p, (int32_t)k);
-        a.ext = n->ext;
-        a.out = TileOut{n->tile_count, n->staging};
-        a.fold = p->fold_target((int)(k & 1));
-        if (timed) HIP_TRY(hipEventRecord(n->ev[2 * (k - first_step)], p->stream));
-        hipLaunchKernelGGL(fn, dim3(p->n_tiles), dim3(kBlock), 0, p->stream, a);
-        if (timed) HIP_TRY(hipEventRecord(n->ev[2 * (k - first_step) + 1], p->stream));
-        launch_compact(p->stream, n->tile_count, n->staging, p->n_tiles, kUpdTile,
-                       n->own_ring_idx + (int64_t)slot * p->n, n->own_ring_cnt + slot, n->log_step,
-                       n->log_idx, n->log_count, n->log_cap, p->err, (int32_t)k);
+        int32_t k32 = (int32_t)k;
+        HIP_TRY(hipMemcpyAsync(n->d_step, &k32, sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+        while (k + kGraphChunk <= first_step + n_steps) {
+            HIP_TRY(hipGraphLaunch(n->graph, p->stream));
+            k += kGraphChunk;
+        }
+        n->timed = want_timing ? kGraphChunk : 0;
+    }
+    const RingEvents re = ring_events(n);
+    for (; k < first_step + n_steps; k++) {
+        int rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, nullptr, nullptr);
+        if (rc) return rc;
     }
     HIP_TRY(hipGetLastError());
     n->last_step = first_step + n_steps - 1;
+    return 0;
+}
+
+int edlut_network_set_graph(edlut_network *n, int on) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    n->use_graph = on != 0;
     return 0;
 }
 
@@ -1119,7 +1207,9 @@ int edlut_network_set_timing(edlut_network *n, int64_t max_steps) {
     n->ev.clear();
     n->timed = 0;
     n->timing = max_steps > 0;
+    if (n->timing && max_steps < kGraphChunk) max_steps = kGraphChunk;
     n->ev.assign(2 * (size_t)max_steps, nullptr);
+    n->dirty = true;
     for (auto &e : n->ev) HIP_TRY(hipEventCreate(&e));
     return 0;
 }

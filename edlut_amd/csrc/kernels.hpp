@@ -47,7 +47,8 @@ struct FoldTarget {
     uint64_t *seg;       // per key: segment start (kSegEmpty when none)
     uint32_t *tile_flag; // per update tile: some key of the tile has >= 2 events
     uint64_t *bump;      // [2] event-buffer allocator, by step parity
-    int par;             // parity of this step
+    int par;             // parity of this step, unless par_by_step
+    int par_by_step;     // parity = update index & 1 (network mode)
     uint64_t *b_ord;     // event buffer: order
     float *b_w;          //               weight
     uint64_t b_cap;
@@ -63,7 +64,7 @@ __device__ __forceinline__ float *g_ptr(const FoldTarget &t, uint64_t key) {
 enum FoldPass { kCount = 0, kScatter = 1 };
 
 template <int PASS>
-__device__ __forceinline__ void fold_event(const FoldTarget &t, uint64_t key, uint64_t ord,
+__device__ __forceinline__ void fold_event(const FoldTarget &t, int par, uint64_t key, uint64_t ord,
                                            float w) {
     if (PASS == kCount) {
         atomicAdd(&t.cnt[key], 1u);
@@ -80,7 +81,7 @@ __device__ __forceinline__ void fold_event(const FoldTarget &t, uint64_t key, ui
     // before any lane waits, so a wave never waits on one of its own lanes.
     const uint64_t prev = atomicCAS((unsigned long long *)&t.seg[key], kSegEmpty, kSegBusy);
     if (prev == kSegEmpty) {
-        const uint64_t base = atomicAdd((unsigned long long *)&t.bump[t.par], (unsigned long long)c);
+        const uint64_t base = atomicAdd((unsigned long long *)&t.bump[par], (unsigned long long)c);
         __hip_atomic_store(&t.seg[key], base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t tgt = key % (uint64_t)t.stride;
         t.tile_flag[tgt / kUpdTile] = 1u;
@@ -149,7 +150,7 @@ struct UpdateArgs {
     int64_t stride;
     int n;                 // neurons
     int n_tiles;
-    int32_t step;          // update index k (>= 1)
+    StepRef step;          // update index k (>= 1)
     ModelParams mp;
     StepConsts k;
     const float *chan;     // HH channel-rate table
@@ -172,6 +173,8 @@ struct UpdateArgs {
 template <class M, int INTEG, unsigned MASK, bool EXT>
 __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     const int tile = blockIdx.x;
+    const int32_t step = a.step.get();
+    const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
     __shared__ uint32_t s_flag;
     // the tile's fold flag is read alongside the state loads (the barrier
     // that publishes it comes after them)
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
 #pragma unroll
     for (int j = 0; j < M::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
     const int64_t ld = a.stride;
-    const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, a.step) : 0u;
+    const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, step) : 0u;
     int idx[kItems];
     bool live[kItems];
     float s[kItems][M::NV];
@@ -204,13 +207,13 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
         c[n].nmda_lut = a.nmda_lut;
         c[n].refr = a.refr;
         c[n].refr_len = a.refr_len;
-        c[n].since_spike = a.last_spike ? (a.step - 1 - a.last_spike[i]) : (1 << 30);
+        c[n].since_spike = a.last_spike ? (step - 1 - a.last_spike[i]) : (1 << 30);
     }
     if (threadIdx.x == 0) {
         s_flag = flag;
         if (flag) a.fold.tile_flag[tile] = 0u;
         // the next step's fold allocates from the other parity: clear it
-        if (tile == 0) a.fold.bump[a.fold.par ^ 1] = 0ull;
+        if (tile == 0) a.fold.bump[par ^ 1] = 0ull;
     }
     __syncthreads();
     if (s_flag) {
@@ -253,7 +256,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
         const uint32_t i = (uint32_t)idx[n];
         spk[n] = live[n] && M::spike(a.mp, prev_v[n], s[n]);
         if (!live[n]) continue;
-        if (spk[n] && a.last_spike) a.last_spike[i] = a.step;
+        if (spk[n] && a.last_spike) a.last_spike[i] = step;
         if (s[n][0] != s[n][0]) {
             // TimeDrivenNeuronModel.cpp:75-88: NaN -> whole state to initial.
 #pragma unroll
@@ -277,18 +280,20 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
 template <class M, unsigned MASK, bool EXT, typename T>
 __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     const int tile = blockIdx.x;
+    const int32_t step = a.step.get();
+    const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
     __shared__ uint32_t s_flag;
     if (threadIdx.x == 0) {
         s_flag = a.fold.tile_flag[tile];
         if (s_flag) a.fold.tile_flag[tile] = 0u;
-        if (tile == 0) a.fold.bump[a.fold.par ^ 1] = 0ull;
+        if (tile == 0) a.fold.bump[par ^ 1] = 0ull;
     }
     __syncthreads();
     float *__restrict__ row[M::NV];
 #pragma unroll
     for (int j = 0; j < M::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
     const int64_t ld = a.stride;
-    const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, a.step) : 0u;
+    const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, step) : 0u;
     bool spk[kItems];
     int idx[kItems];
     for (int n = 0; n < kItems; n++) {
@@ -331,14 +336,14 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         c.nmda_lut = a.nmda_lut;
         c.refr = a.refr;
         c.refr_len = a.refr_len;
-        c.since_spike = a.last_spike ? (a.step - 1 - a.last_spike[i]) : (1 << 30);
+        c.since_spike = a.last_spike ? (step - 1 - a.last_spike[i]) : (1 << 30);
         int st = a.bdf.st[i];
         integrate_bdf<M, MASK, T>(a.mp, c, (T)a.k.dt, a.bdf, ld, i, st, S);
         const bool fired = M::spike(a.mp, prev, S);
         if (fired && M::kResetsIntegrator) st = 0;
         a.bdf.st[i] = st;
         spk[n] = fired;
-        if (fired && a.last_spike) a.last_spike[i] = a.step;
+        if (fired && a.last_spike) a.last_spike[i] = step;
         float out[M::NV];
 #pragma unroll
         for (int j = 0; j < M::NV; j++) out[j] = (float)S[j];
@@ -397,7 +402,7 @@ template <int PASS>
 __global__ __launch_bounds__(kBlock) void k_fold_push(FoldTarget t, PushEvents e) {
     for (int q = blockIdx.x * kBlock + threadIdx.x; q < e.n; q += gridDim.x * kBlock) {
         const uint64_t key = (uint64_t)e.type[q] * (uint64_t)t.stride + (uint64_t)e.tgt[q];
-        fold_event<PASS>(t, key, (uint64_t)q, e.w[q]);
+        fold_event<PASS>(t, t.par, key, (uint64_t)q, e.w[q]);
     }
 }
 
@@ -417,7 +422,7 @@ struct RingEvents {
     int n_sources;
     int max_delay;
     int ring;
-    int32_t step;              // update index about to run
+    StepRef step;              // update index about to run
 };
 
 // Work item = one spike of one source at emission step k-1-d; a group of
@@ -431,10 +436,12 @@ __global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r
     __shared__ int64_t s_pref[kMaxDelay * kMaxSources + 1];
     __shared__ int s_slot[kMaxDelay * kMaxSources];
     const int nseg = r.max_delay * r.n_sources;
+    const int32_t step = r.step.get();
+    const int par = t.par_by_step ? (step & 1) : t.par;
     for (int g = threadIdx.x; g < nseg; g += kBlock) {
         const int d = g / r.n_sources + 1;
         const int p = g % r.n_sources;
-        const int64_t s = (int64_t)r.step - 1 - d;
+        const int64_t s = (int64_t)step - 1 - d;
         int slot = 0, c = 0;
         if (s >= 1) {
             slot = (int)(s % r.ring);
@@ -474,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r
             const uint32_t tt = src.tgt_type[q];
             const uint64_t key =
                 (uint64_t)(tt >> 30) * (uint64_t)t.stride + (uint64_t)(tt & 0x3fffffffu);
-            fold_event<PASS>(t, key, ord_hi | (uint64_t)q, src.w[q]);
+            fold_event<PASS>(t, par, key, ord_hi | (uint64_t)q, src.w[q]);
         }
     }
 }

@@ -106,6 +106,10 @@ class Network:
         if sync:
             self.synchronize()
 
+    def set_graph(self, on: bool):
+        """Replay captured 16-step hipGraphs (default) or launch every kernel."""
+        C.check(C.lib().edlut_network_set_graph(self._h, int(bool(on))))
+
     def synchronize(self):
         C.check(C.lib().edlut_network_synchronize(self._h))
 

@@ -153,8 +153,15 @@ int edlut_network_add_poisson(edlut_network *net, int n_src, const uint32_t *lcg
 int edlut_network_set_external(edlut_network *net, int n_thr, const uint32_t *thresholds,
                                float w_ext, uint64_t seed);
 /* Enqueue n_steps updates starting at update index first_step (asynchronous;
- * spikes of the population are appended to the device spike log). */
+ * spikes of the population are appended to the device spike log).  Whole
+ * chunks of 16 steps are replayed from a captured hipGraph whose kernels
+ * read the update index from device memory; the remainder is launched
+ * directly.  Replaces the per-step TimeEventUpdateNeurons / UpdateState /
+ * TimeDrivenInternalSpike / PropagatedSpikeGroup cycle for device-resident
+ * networks (TimeEventUpdateNeurons.cpp:45-79). */
 int edlut_network_run(edlut_network *net, int64_t first_step, int n_steps);
+/* Graph replay on (default) or off (every step launched directly). */
+int edlut_network_set_graph(edlut_network *net, int on);
 int edlut_network_synchronize(edlut_network *net);
 /* Spike log (step, index) pairs, ascending within a step; returns the count. */
 int edlut_network_spike_log(edlut_network *net, int32_t *step, int32_t *idx, int64_t cap,
