@@ -8,6 +8,13 @@ update on the device: ordered delivery of the spikes due, the fused update
 kernel (integrate, decay, threshold/reset, compaction) and spike logging.
 Inputs are resident in HBM before the timed region.
 
+A second leg measures the batched parameter sweep: 4,096 independent AdEx
+networks (lognormal sizes, median 5,000, clipped to [1,000, 20,000], in-degree
+100, per-network g_leak, E_leak, a, b, tau_w within +-30%), RK4, 0.5 s,
+advanced together as one grouped population with the autotuner's launch
+shape and hipGraph-replayed steps, next to the same networks run one after
+another with plain (directly launched) steps, on a size-stratified sample.
+
 Prints one JSON line (see the repository's benchmark contract).
 """
 from __future__ import annotations
@@ -69,6 +76,81 @@ def cpu_baseline(spec, steps: int):
     return spec.n * steps / dt, dt
 
 
+def bench_sweep(n_networks: int, steps: int, seq_sample: int, seq_steps: int):
+    """Batched sweep vs sequential plain launches (see the module docstring)."""
+    import edlut_amd as E
+    from edlut_amd import workloads
+    from edlut_amd.network import Network
+
+    spec = workloads.adex_sweep(n_networks=n_networks, steps=steps)
+    thr, w_ext, ext_seed = spec.ext
+    t0 = time.perf_counter()
+    m = E.CreateNeuronModel("AdExTimeDrivenModel_GPU")
+    m.SetParameters({"int_meth": E.integration_method("RK4", 1e-4)})
+    m.InitializeGroups(spec.sizes, spec.params)
+    net = Network(m, spec.max_delay)
+    net.generate_recurrent(spec.in_degree, spec.exc_fraction, spec.w_mu, spec.w_sigma, spec.wiring_seed)
+    net.set_external(thr, w_ext, ext_seed)
+    net.synchronize()
+    setup = time.perf_counter() - t0
+    items, lanes, blocks, tune_ms = net.autotune(64)
+    print(f"sweep setup {setup:.1f}s, autotune -> items {items} lanes {lanes} blocks {blocks} "
+          f"({tune_ms:.4f} ms/step)", file=sys.stderr, flush=True)
+    net.synchronize()
+    t0 = time.perf_counter()
+    net.run(steps, sync=False)
+    net.synchronize()
+    wall = time.perf_counter() - t0
+    st, ix = net.spikes()
+    real = int(spec.sizes.sum())
+    batched = real * steps / wall
+    rate_hz = len(st) / (real * steps * 1e-4)
+    net.close()
+    m.close()
+
+    # sequential plain launches: one network at a time, every kernel launched
+    # directly; a sample spread evenly over the size distribution
+    order = np.argsort(spec.sizes, kind="stable")
+    pick = order[np.linspace(0, len(order) - 1, seq_sample).round().astype(int)]
+    seq_updates, seq_time = 0, 0.0
+    for g in pick:
+        pm = E.CreateNeuronModel("AdExTimeDrivenModel_GPU")
+        names = [k for k, _ in type(pm).PARAMS]
+        pm.SetParameters({k: float(v) for k, v in zip(names, spec.params[g])})
+        pm.SetParameters({"int_meth": E.integration_method("RK4", 1e-4)})
+        pm.InitializeStates(int(spec.sizes[g]))
+        pn = Network(pm, spec.max_delay)
+        pn.generate_recurrent(spec.in_degree, spec.exc_fraction, spec.w_mu, spec.w_sigma,
+                              spec.wiring_seed + int(g))
+        pn.set_external(thr, w_ext, ext_seed + int(g))
+        pn.set_graph(False)
+        pn.run(16)
+        t0 = time.perf_counter()
+        pn.run(seq_steps, sync=False)
+        pn.synchronize()
+        seq_time += time.perf_counter() - t0
+        seq_updates += int(spec.sizes[g]) * seq_steps
+        pn.close()
+        pm.close()
+    sequential = seq_updates / seq_time
+    return {
+        "workload": f"{n_networks} AdEx networks, RK4 dt 0.1 ms, {steps} steps, in-degree {spec.in_degree}",
+        "neurons": real,
+        "value": batched,
+        "unit": "neuron-updates/s",
+        "ms_per_step": wall * 1e3 / steps,
+        "mean_rate_hz": round(float(rate_hz), 3),
+        "launch": {"items_per_thread": items, "fold_lanes": lanes, "fold_blocks": blocks,
+                   "autotune_ms_per_step": tune_ms},
+        "sequential_plain": {
+            "value": sequential,
+            "unit": "neuron-updates/s",
+            "sample": f"{seq_sample} networks spread over the size distribution, {seq_steps} steps each",
+        },
+        "speedup_vs_sequential": batched / sequential,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,10 +162,14 @@ def main():
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device copy of this size (for counter profiles)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--sweep-networks", type=int, default=4096)
+    ap.add_argument("--sweep-steps", type=int, default=5000)
+    ap.add_argument("--seq-sample", type=int, default=16)
+    ap.add_argument("--seq-steps", type=int, default=500)
+    ap.add_argument("--no-sweep", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one GPU only: the library has no multi-GPU path.", file=sys.stderr)
         sys.exit(2)
 
     import edlut_amd as E
@@ -146,6 +232,10 @@ def main():
             "traffic": traffic,
         },
     }
+    net.close()
+    model.close()
+    if not args.no_sweep:
+        out["batched"] = bench_sweep(args.sweep_networks, args.sweep_steps, args.seq_sample, args.seq_steps)
     if not args.no_cpu:
         cpu_spec = workloads.adex_recurrent(n=args.neurons, steps=args.cpu_steps, w_ext=args.w_ext)
         v, dt = cpu_baseline(cpu_spec, args.cpu_steps)

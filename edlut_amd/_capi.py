@@ -18,6 +18,8 @@ LIB_PATH = os.path.join(_HERE, "libedlut_amd.so")
 EXPORTS = (
     "edlut_last_error",
     "edlut_population_create",
+    "edlut_population_create_groups",
+    "edlut_population_groups",
     "edlut_population_destroy",
     "edlut_population_layout",
     "edlut_population_enable_synapses",
@@ -42,6 +44,12 @@ EXPORTS = (
     "edlut_network_clear_log",
     "edlut_network_set_timing",
     "edlut_network_update_timing",
+    "edlut_network_generate_recurrent",
+    "edlut_network_get_synapses",
+    "edlut_network_reset",
+    "edlut_network_set_launch",
+    "edlut_network_get_launch",
+    "edlut_network_autotune",
     "edlut_device_arch",
     "edlut_drive_hash",
     "edlut_build_out_csr",
@@ -78,6 +86,8 @@ def _sig(lib):
         "edlut_device_arch": (ctypes.c_char_p, []),
         "edlut_drive_hash": (U32, [U64, I64, U32]),
         "edlut_population_create": (I32, [I32, I32, I32, P, I32, F32, I32, I32, P]),
+        "edlut_population_create_groups": (I32, [I32, I32, P, I32, F32, I32, P, I32, P]),
+        "edlut_population_groups": (I32, [P, P, P, P]),
         "edlut_population_destroy": (I32, [P]),
         "edlut_population_layout": (I32, [P, P, P, P]),
         "edlut_population_enable_synapses": (I32, [P, U32]),
@@ -102,6 +112,12 @@ def _sig(lib):
         "edlut_network_clear_log": (I32, [P]),
         "edlut_network_set_timing": (I32, [P, I64]),
         "edlut_network_update_timing": (I32, [P, P, P]),
+        "edlut_network_generate_recurrent": (I32, [P, I32, F32, F32, F32, U64]),
+        "edlut_network_get_synapses": (I32, [P, I32, P, P, P, P, P]),
+        "edlut_network_reset": (I32, [P]),
+        "edlut_network_set_launch": (I32, [P, I32, I32, I32]),
+        "edlut_network_get_launch": (I32, [P, P, P, P]),
+        "edlut_network_autotune": (I32, [P, I32, P, P, P, P]),
         "edlut_build_out_csr": (I32, [I64, I64, I32, I64, P, P, P, P, P, P, P, P]),
     }
     for name, (res, args) in s.items():

@@ -14,9 +14,10 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // Ordered compaction of spiking neuron indices, in two passes without atomics.
 //
 // Pass 1 (inside every update kernel): tile t = workgroup t covers neurons
-// [t*kBlock, (t+1)*kBlock).  A wave ballot gives each spiking lane its rank,
-// the four wave counts are combined in LDS, and the tile writes its count and
-// its spiking indices (ascending) into its slot of a staging array.
+// [t*T, (t+1)*T) with T = kBlock*ITEMS.  A wave ballot per item gives each
+// spiking lane its rank, the wave counts are combined in LDS, and the tile
+// writes its count and its spiking indices (ascending) into its slot of a
+// staging array.
 //
 // Pass 2 (k_compact): each workgroup sums the counts of the tiles before its
 // range, scans its own range and copies every tile's indices to the output,
@@ -35,8 +36,17 @@ struct TileOut {
     int32_t *staging;      // [n_tiles * tile_size]
 };
 
-// Item j of thread t in tile `tile` is neuron tile*kBlock*ITEMS + j*kBlock + t,
-// so ascending neuron order is item-major, then wave, then lane.
+// Tiles are wave-major: item j of lane l in wave w of tile `tile` is neuron
+// tile*kBlock*ITEMS + w*ITEMS*kWave + j*kWave + l, so ascending neuron order
+// is wave, then item, then lane, and each wave covers one contiguous span of
+// ITEMS*kWave neurons.
+template <int ITEMS>
+__device__ __forceinline__ int tile_neuron(int tile, int j) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    return tile * kBlock * ITEMS + wid * ITEMS * kWave + j * kWave + lane;
+}
+
 template <int ITEMS>
 __device__ __forceinline__ void tile_spikes(const TileOut &o, int tile, const bool *spike,
                                             const int *idx) {
@@ -47,7 +57,7 @@ __device__ __forceinline__ void tile_spikes(const TileOut &o, int tile, const bo
 #pragma unroll
     for (int j = 0; j < ITEMS; j++) {
         ballot[j] = __ballot(spike[j]);
-        if (lane == 0) s_wave[j * kWavesPerBlock + wid] = (uint32_t)__popcll(ballot[j]);
+        if (lane == 0) s_wave[wid * ITEMS + j] = (uint32_t)__popcll(ballot[j]);
     }
     __syncthreads();
     uint32_t local = 0;
@@ -57,7 +67,7 @@ __device__ __forceinline__ void tile_spikes(const TileOut &o, int tile, const bo
     for (int j = 0; j < ITEMS; j++) {
         if (spike[j]) {
             uint32_t base = 0;
-            for (int q = 0; q < j * kWavesPerBlock + wid; q++) base += s_wave[q];
+            for (int q = 0; q < wid * ITEMS + j; q++) base += s_wave[q];
             const uint32_t rank = (uint32_t)__popcll(ballot[j] & ((1ull << lane) - 1ull));
             o.staging[(int64_t)tile * kBlock * ITEMS + base + rank] = idx[j];
         }

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/edlut_amd.h"
+#include "generate.hpp"
 #include "kernels.hpp"
 
 using namespace edl;
@@ -213,6 +214,16 @@ struct edlut_population {
 
     edlut_network *net = nullptr;
 
+    // grouped populations: per-group parameters on the device
+    bool grouped = false;
+    std::vector<int32_t> g_start, g_size;
+    GroupInfo *d_groups = nullptr;
+    uint32_t *d_chunk_group = nullptr;
+    std::vector<float> init_state;   // initial SoA state (host copy)
+    int items = kItems;              // neurons per thread of the update launch
+
+    int update_tiles() const { return (int)(stride / ((int64_t)kBlock * items)); }
+
     FoldTarget fold_target(int par, bool by_step = false) const {
         FoldTarget t;
         t.par_by_step = by_step ? 1 : 0;
@@ -271,6 +282,7 @@ struct Source {
     uint32_t *tgt_type = nullptr;
     float *w = nullptr;
     uint32_t *lcg = nullptr;
+    std::vector<uint32_t> lcg0;   // initial generator states (reset)
     float rate = 0.0f;
     int32_t *tile_count = nullptr;
     int32_t *staging = nullptr;
@@ -309,6 +321,9 @@ struct edlut_network {
     bool graph_timed = false;
     bool dirty = true;
     bool use_graph = true;
+    // launch shape of the ring fold (set by hand or by the autotuner)
+    int fold_lanes = 32;
+    int fold_blocks = 512;
 };
 
 // ---------------------------------------------------------------------------
@@ -320,8 +335,8 @@ using UpdateFn = void (*)(UpdateArgs);
 
 template <class M, int I, unsigned MASK>
 void fill_one(UpdateFn *t) {
-    t[(I * 16 + (int)MASK) * 2 + 0] = k_update<M, I, MASK, false>;
-    t[(I * 16 + (int)MASK) * 2 + 1] = k_update<M, I, MASK, true>;
+    t[(I * 16 + (int)MASK) * 2 + 0] = k_update<M, I, MASK, false, kItems, false>;
+    t[(I * 16 + (int)MASK) * 2 + 1] = k_update<M, I, MASK, true, kItems, false>;
 }
 
 template <class M, int I, unsigned... MS>
@@ -339,6 +354,30 @@ struct Table {
     }
 };
 
+// Grouped populations: spike-driven synapse types (mask 0..3), with and
+// without external drive, two tile shapes (items per thread 2 or 4).
+constexpr int kGroupItems[2] = {2, 4};
+template <class M, int I, unsigned MASK>
+void fill_grouped_one(UpdateFn *t) {
+    t[((I * 4 + (int)MASK) * 2 + 0) * 2 + 0] = k_update<M, I, MASK, false, 2, true>;
+    t[((I * 4 + (int)MASK) * 2 + 1) * 2 + 0] = k_update<M, I, MASK, true, 2, true>;
+    t[((I * 4 + (int)MASK) * 2 + 0) * 2 + 1] = k_update<M, I, MASK, false, 4, true>;
+    t[((I * 4 + (int)MASK) * 2 + 1) * 2 + 1] = k_update<M, I, MASK, true, 4, true>;
+}
+template <class M, int I, unsigned... MS>
+void fill_grouped(UpdateFn *t, std::integer_sequence<unsigned, MS...>) {
+    (fill_grouped_one<M, I, MS>(t), ...);
+}
+template <class M>
+struct GroupedTable {
+    UpdateFn fn[3 * 4 * 2 * 2];
+    GroupedTable() {
+        fill_grouped<M, kEuler>(fn, std::make_integer_sequence<unsigned, 4>{});
+        fill_grouped<M, kRK2>(fn, std::make_integer_sequence<unsigned, 4>{});
+        fill_grouped<M, kRK4>(fn, std::make_integer_sequence<unsigned, 4>{});
+    }
+};
+
 template <class M, typename T, unsigned... MS>
 void fill_bdf(UpdateFn *t, std::integer_sequence<unsigned, MS...>) {
     ((t[MS * 2 + 0] = k_update_bdf<M, MS, false, T>, t[MS * 2 + 1] = k_update_bdf<M, MS, true, T>), ...);
@@ -350,8 +389,23 @@ struct BdfTable {
     BdfTable() { fill_bdf<M, T>(fn, std::make_integer_sequence<unsigned, 16>{}); }
 };
 
-UpdateFn pick_update(int model, int integ, uint32_t mask, bool ext, bool fp64 = false) {
-    const int m = (int)(mask & 15u);
+// Update kernel of a population: null when no kernel covers the combination
+// (grouped populations with NMDA or external-current synapses).
+UpdateFn pick_update(const edlut_population *p, bool ext) {
+    const int m = (int)(p->mask & 15u);
+    const int integ = p->integ, model = p->model;
+    if (p->grouped) {
+        if (m > 3 || integ == EDLUT_BDF) return nullptr;
+        static GroupedTable<AdExModel> adex_g;
+        static GroupedTable<LIFModel> lif_g;
+        const int it = p->items == 2 ? 0 : (p->items == 4 ? 1 : -1);
+        if (it < 0) return nullptr;
+        const int idx = ((integ * 4 + m) * 2 + (ext ? 1 : 0)) * 2 + it;
+        if (model == EDLUT_ADEX) return adex_g.fn[idx];
+        if (model == EDLUT_LIF) return lif_g.fn[idx];
+        return nullptr;
+    }
+    if (p->items != kItems) return nullptr;
     if (integ == EDLUT_BDF) {
         static BdfTable<AdExModel, float> adex_b;
         static BdfTable<LIFModel, float> lif_b;
@@ -360,7 +414,7 @@ UpdateFn pick_update(int model, int integ, uint32_t mask, bool ext, bool fp64 = 
         const int idx = m * 2 + (ext ? 1 : 0);
         if (model == EDLUT_ADEX) return adex_b.fn[idx];
         if (model == EDLUT_LIF) return lif_b.fn[idx];
-        if (model == EDLUT_HH) return fp64 ? hh_b64.fn[idx] : hh_b.fn[idx];
+        if (model == EDLUT_HH) return p->newton_fp64 ? hh_b64.fn[idx] : hh_b.fn[idx];
         return nullptr;
     }
     static Table<AdExModel> adex;
@@ -405,6 +459,8 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.bdf.d = p->bdf_d;
     a.bdf.prev = p->bdf_prev;
     a.bdf.st = p->bdf_st;
+    a.grp.tab = p->d_groups;
+    a.grp.chunk_group = p->d_chunk_group;
     return a;
 }
 
@@ -431,22 +487,43 @@ void launch_compact(hipStream_t st, int32_t *tile_count, int32_t *staging, int n
     hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(kBlock), 0, st, c);
 }
 
+using RingFoldFn = void (*)(FoldTarget, RingEvents);
+template <int LANES>
+RingFoldFn ring_fold_fn(int pass) {
+    return pass == kCount ? k_fold_ring<kCount, LANES> : k_fold_ring<kScatter, LANES>;
+}
+RingFoldFn pick_ring_fold(int pass, int lanes) {
+    switch (lanes) {
+        case 8: return ring_fold_fn<8>(pass);
+        case 16: return ring_fold_fn<16>(pass);
+        case 32: return ring_fold_fn<32>(pass);
+        case 64: return ring_fold_fn<64>(pass);
+        default: return nullptr;
+    }
+}
+
 int launch_fold(edlut_population *p, int par, int n_events, bool ring, const RingEvents *re,
-                const PushEvents *pe) {
+                const PushEvents *pe, int lanes = 32, int blocks = 512) {
     FoldTarget t = p->fold_target(par, ring);
     if (ring) {
-        const int blocks = 512;
-        hipLaunchKernelGGL((k_fold_ring<kCount, 32>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
-        hipLaunchKernelGGL((k_fold_ring<kScatter, 32>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
+        RingFoldFn c = pick_ring_fold(kCount, lanes), sc = pick_ring_fold(kScatter, lanes);
+        if (!c || !sc) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
+        hipLaunchKernelGGL(c, dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
+        hipLaunchKernelGGL(sc, dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
     } else {
         if (n_events == 0) return 0;
-        int blocks = (n_events + kBlock - 1) / kBlock;
-        if (blocks > 4096) blocks = 4096;
-        hipLaunchKernelGGL((k_fold_push<kCount>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *pe);
-        hipLaunchKernelGGL((k_fold_push<kScatter>), dim3(blocks), dim3(kBlock), 0, p->stream, t, *pe);
+        int nb = (n_events + kBlock - 1) / kBlock;
+        if (nb > 4096) nb = 4096;
+        hipLaunchKernelGGL((k_fold_push<kCount>), dim3(nb), dim3(kBlock), 0, p->stream, t, *pe);
+        hipLaunchKernelGGL((k_fold_push<kScatter>), dim3(nb), dim3(kBlock), 0, p->stream, t, *pe);
     }
     HIP_TRY(hipGetLastError());
     return 0;
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 }  // namespace
@@ -462,46 +539,18 @@ uint32_t edlut_drive_hash(uint64_t seed, int64_t step, uint32_t neuron) {
     return drive_hash(seed, step, neuron);
 }
 
-int edlut_population_create(int model, int integrator, int bdf_order, const float *params,
-                            int n_params, float dt_s, int n_neurons, int device,
-                            edlut_population **out) {
-    if (!out) return fail(EDLUT_E_ARG, "out is null");
-    *out = nullptr;
-    if (model != EDLUT_ADEX && model != EDLUT_LIF && model != EDLUT_HH)
-        return fail(EDLUT_E_ARG, "neuron model %d is not available in this build", model);
-    if (integrator < EDLUT_EULER || integrator > EDLUT_BDF)
-        return fail(EDLUT_E_ARG, "integration method %d is not available for this model", integrator);
-    if (integrator == EDLUT_BDF && (bdf_order < 1 || bdf_order > 6))
-        return fail(EDLUT_E_ARG, "bdf_order must be in [1, 6] (BDFn_GPU_C_INTERFACE.cuh:275-288)");
-    const int need = model == EDLUT_ADEX ? 14 : (model == EDLUT_HH ? 13 : 10);
-    if (!params || n_params != need)
-        return fail(EDLUT_E_ARG, "model %d takes %d parameters, got %d", model, need, n_params);
-    if (!(dt_s > 0.0f)) return fail(EDLUT_E_ARG, "integration step must be positive");
-    if (n_neurons <= 0) return fail(EDLUT_E_ARG, "population size must be positive");
-    if (n_neurons > (1 << 30) - 1) return fail(EDLUT_E_ARG, "population too large");
-    for (int i = 0; i < n_params; i++)
-        if (!std::isfinite(params[i])) return fail(EDLUT_E_ARG, "parameter %d is not finite", i);
+}  // extern "C"
 
-    HIP_TRY(hipSetDevice(device));
-    edlut_population *p = new edlut_population;
-    p->model = model;
-    p->integ = integrator;
-    p->bdf_order = bdf_order;
-    p->n = n_neurons;
-    p->device = device;
-    p->params.assign(params, params + n_params);
-    p->dt_s = dt_s;
-    p->dt_ms = dt_s * 1000.0f;  // IntegrationMethodFast.h:191-193
-    p->k.dt = p->dt_ms;
-    p->k.h = p->dt_ms * 0.5f;
-    p->k.s6 = p->dt_ms * 0.166666666667f;
-    const float elapsed = integrator == EDLUT_RK4 ? p->dt_ms * 0.5f : p->dt_ms;
+namespace {
 
-    float e_exc, e_inh;
-    std::vector<float> init;
+// Derived per-model constants of one parameter vector (the reference's
+// InitializeStates / loadNeuronModel, e.g. AdExTimeDrivenModel.cpp:250-330),
+// plus the initial state (VectorNeuronState_GPU_C_INTERFACE.cu:66-131).
+void model_params(int model, int integrator, const float *x, float dt_ms, ModelParams &mp,
+                  std::vector<float> &init, float &e_exc, float &e_inh) {
+    const float elapsed = integrator == EDLUT_RK4 ? dt_ms * 0.5f : dt_ms;
     if (model == EDLUT_ADEX) {
-        AdExParams &q = p->mp.adex;
-        const float *x = params;
+        AdExParams &q = mp.adex;
         const float thr = x[2];
         q.a = x[0];
         q.b = x[1];
@@ -521,11 +570,8 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
         q.cf[2] = expf(-elapsed * inv_of(x[13]));
         init = {x[8], 0, 0, 0, 0, 0};
         for (int j = 0; j < 6; j++) q.init[j] = init[j];
-        p->nv = 6;
-        p->nd = 2;
     } else if (model == EDLUT_HH) {
-        HHParams &q = p->mp.hh;
-        const float *x = params;   // e_exc e_inh e_leak g_leak c_m v_thr tau_exc tau_inh tau_nmda g_na g_kd e_na e_k
+        HHParams &q = mp.hh;   // e_exc e_inh e_leak g_leak c_m v_thr tau_exc tau_inh tau_nmda g_na g_kd e_na e_k
         q.e_exc = e_exc = x[0];
         q.e_inh = e_inh = x[1];
         q.e_leak = x[2];
@@ -541,11 +587,16 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
             q.cf[j] = expf(-elapsed * inv_of(tau[j]));
             q.cf_d[j] = std::exp(-(double)elapsed * (1.0 / (double)tau[j]));
         }
-        p->nv = 8;
-        p->nd = 4;
+        // HHTimeDrivenModel.cpp:136-160: gates at their steady state at e_leak
+        const std::vector<float> chan = build_hh_table(x[5]);
+        const float *c = &chan[(size_t)hh_index(x[2]) * 6];
+        const float m_inf = c[0] / (c[0] + c[1]);
+        const float h_inf = c[2] / (c[2] + c[3]);
+        const float n_inf = c[4] / (c[4] + c[5]);
+        init = {x[2], m_inf, h_inf, n_inf, 0, 0, 0, 0};
+        for (int j = 0; j < 8; j++) q.init[j] = init[j];
     } else {
-        LIFParams &q = p->mp.lif;
-        const float *x = params;
+        LIFParams &q = mp.lif;
         q.e_exc = e_exc = x[0];
         q.e_inh = e_inh = x[1];
         q.e_leak = x[2];
@@ -557,25 +608,99 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
         q.cf[2] = expf(-elapsed * inv_of(x[9]));
         init = {x[2], 0, 0, 0, 0};
         for (int j = 0; j < 5; j++) q.init[j] = init[j];
-        p->nv = 5;
-        p->nd = 1;
     }
+}
+
+// Common constructor of plain (one group, no padding) and grouped populations.
+int create_population(int model, int integrator, int bdf_order, const float *params, int n_params,
+                      float dt_s, int n_groups, const int32_t *sizes, bool grouped, int device,
+                      edlut_population **out) {
+    if (!out) return fail(EDLUT_E_ARG, "out is null");
+    *out = nullptr;
+    if (model != EDLUT_ADEX && model != EDLUT_LIF && model != EDLUT_HH)
+        return fail(EDLUT_E_ARG, "neuron model %d is not available in this build", model);
+    if (integrator < EDLUT_EULER || integrator > EDLUT_BDF)
+        return fail(EDLUT_E_ARG, "integration method %d is not available for this model", integrator);
+    if (integrator == EDLUT_BDF && (bdf_order < 1 || bdf_order > 6))
+        return fail(EDLUT_E_ARG, "bdf_order must be in [1, 6] (BDFn_GPU_C_INTERFACE.cuh:275-288)");
+    const int need = model == EDLUT_ADEX ? 14 : (model == EDLUT_HH ? 13 : 10);
+    if (!params || n_params != need)
+        return fail(EDLUT_E_ARG, "model %d takes %d parameters, got %d", model, need, n_params);
+    if (!(dt_s > 0.0f)) return fail(EDLUT_E_ARG, "integration step must be positive");
+    if (n_groups <= 0 || !sizes) return fail(EDLUT_E_ARG, "no groups");
+    if (grouped) {
+        if (model == EDLUT_HH || integrator == EDLUT_BDF)
+            return fail(EDLUT_E_ARG, "grouped populations take AdEx or LIF under Euler, RK2 or RK4");
+    }
+    int64_t total = 0;
+    std::vector<int32_t> start(n_groups);
+    for (int g = 0; g < n_groups; g++) {
+        if (sizes[g] <= 0) return fail(EDLUT_E_ARG, "group %d: size must be positive", g);
+        start[g] = (int32_t)total;
+        total += grouped ? ((int64_t)sizes[g] + kGroupAlign - 1) / kGroupAlign * kGroupAlign : sizes[g];
+        if (total > (1 << 30) - 1) return fail(EDLUT_E_ARG, "population too large");
+    }
+    for (int i = 0; i < n_params * n_groups; i++)
+        if (!std::isfinite(params[i])) return fail(EDLUT_E_ARG, "parameter %d is not finite", i % n_params);
+    const int n_neurons = (int)total;
+
+    float e_exc = 0, e_inh = 0;
+    std::vector<float> init;
+    ModelParams mp0;
+    std::memset(&mp0, 0, sizeof(mp0));
+    const float dt_ms = dt_s * 1000.0f;
+    model_params(model, integrator, params, dt_ms, mp0, init, e_exc, e_inh);
+    std::vector<GroupInfo> gtab;
+    std::vector<float> st_init;   // per group initial state
+    if (grouped) {
+        gtab.resize(n_groups);
+        for (int g = 0; g < n_groups; g++) {
+            const float *x = params + (size_t)g * n_params;
+            float ee, ei;
+            std::vector<float> gi;
+            std::memset(&gtab[g], 0, sizeof(GroupInfo));
+            model_params(model, integrator, x, dt_ms, gtab[g].mp, gi, ee, ei);
+            // shared tables: NMDA (reversal potentials), refractory clock
+            if (ee != e_exc || ei != e_inh)
+                return fail(EDLUT_E_ARG, "group %d: e_exc/e_inh must equal group 0's (shared NMDA table)", g);
+            if (model == EDLUT_LIF && x[7] != params[7])
+                return fail(EDLUT_E_ARG, "group %d: tau_ref must equal group 0's (shared refractory table)", g);
+            gtab[g].start = start[g];
+            gtab[g].n = sizes[g];
+            st_init.insert(st_init.end(), gi.begin(), gi.end());
+        }
+    }
+    HIP_TRY(hipSetDevice(device));
+    edlut_population *p = new edlut_population;
+    p->model = model;
+    p->integ = integrator;
+    p->bdf_order = bdf_order;
+    p->n = n_neurons;
+    p->device = device;
+    p->params.assign(params, params + (size_t)n_params * n_groups);
+    p->dt_s = dt_s;
+    p->dt_ms = dt_s * 1000.0f;  // IntegrationMethodFast.h:191-193
+    p->k.dt = p->dt_ms;
+    p->k.h = p->dt_ms * 0.5f;
+    p->k.s6 = p->dt_ms * 0.166666666667f;
+    p->grouped = grouped;
+    p->g_start = start;
+    p->g_size.assign(sizes, sizes + n_groups);
+    p->mp = mp0;
+
     float nmda_aux = 0.0f;
     std::vector<float> nmda = build_nmda_table(e_exc, e_inh, &nmda_aux);
     if (model == EDLUT_ADEX) p->mp.adex.nmda_aux = nmda_aux;
     else if (model == EDLUT_HH) p->mp.hh.nmda_aux = nmda_aux;
     else p->mp.lif.nmda_aux = nmda_aux;
-    std::vector<float> chan;
-    if (model == EDLUT_HH) {
-        chan = build_hh_table(params[5]);
-        // HHTimeDrivenModel.cpp:136-160: gates at their steady state at e_leak
-        const float *c = &chan[(size_t)hh_index(params[2]) * 6];
-        const float m_inf = c[0] / (c[0] + c[1]);
-        const float h_inf = c[2] / (c[2] + c[3]);
-        const float n_inf = c[4] / (c[4] + c[5]);
-        init = {params[2], m_inf, h_inf, n_inf, 0, 0, 0, 0};
-        for (int j = 0; j < 8; j++) p->mp.hh.init[j] = init[j];
+    for (GroupInfo &g : gtab) {
+        if (model == EDLUT_ADEX) g.mp.adex.nmda_aux = nmda_aux;
+        else g.mp.lif.nmda_aux = nmda_aux;
     }
+    p->nv = model == EDLUT_ADEX ? 6 : (model == EDLUT_HH ? 8 : 5);
+    p->nd = model == EDLUT_ADEX ? 2 : (model == EDLUT_HH ? 4 : 1);
+    std::vector<float> chan;
+    if (model == EDLUT_HH) chan = build_hh_table(params[5]);
     std::vector<float> expt = build_exp_table();
     std::vector<float> refr;
     if (model == EDLUT_LIF) refr = build_refractory(dt_s, p->dt_ms, params[7]);
@@ -587,6 +712,7 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
     // rows padded to whole update tiles: every kernel load stays in bounds
     p->stride = ((int64_t)n_neurons + kUpdTile - 1) / kUpdTile * kUpdTile;
     p->n_tiles = (n_neurons + kUpdTile - 1) / kUpdTile;
+    const size_t max_tiles = (size_t)(p->stride / kFlagTile);
 
     int rc = 0;
     auto cleanup = [&](int code) {
@@ -604,6 +730,20 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
         if (hipMemcpy(p->chan, chan.data(), chan.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(EDLUT_E_HIP, "hipMemcpy failed"));
     }
+    if (grouped) {
+        const size_t chunks = (size_t)(p->stride / kGroupAlign);
+        std::vector<uint32_t> cg(chunks, (uint32_t)(n_groups - 1));
+        for (int g = 0; g < n_groups; g++) {
+            const int64_t c0 = start[g] / kGroupAlign;
+            const int64_t c1 = g + 1 < n_groups ? start[g + 1] / kGroupAlign : (int64_t)chunks;
+            for (int64_t c = c0; c < c1; c++) cg[(size_t)c] = (uint32_t)g;
+        }
+        if ((rc = dalloc(&p->d_groups, gtab.size())) || (rc = dalloc(&p->d_chunk_group, chunks)))
+            return cleanup(rc);
+        if (hipMemcpy(p->d_groups, gtab.data(), gtab.size() * sizeof(GroupInfo), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(p->d_chunk_group, cg.data(), chunks * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+            return cleanup(fail(EDLUT_E_HIP, "hipMemcpy failed"));
+    }
     if (integrator == EDLUT_BDF) {
         float C[49];
         bdf_coefficients(C);
@@ -614,15 +754,15 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
             hipMemset(p->bdf_st, 0, sizeof(int32_t) * p->stride) != hipSuccess)
             return cleanup(fail(EDLUT_E_HIP, "BDF initialisation failed"));
     }
-    if ((rc = dalloc(&p->tile_count, padded_tiles(p->n_tiles)))) return cleanup(rc);
-    if ((rc = dalloc(&p->staging, (size_t)p->n_tiles * kUpdTile))) return cleanup(rc);
+    if ((rc = dalloc(&p->tile_count, padded_tiles((int)max_tiles)))) return cleanup(rc);
+    if ((rc = dalloc(&p->staging, (size_t)p->stride))) return cleanup(rc);
     if ((rc = dalloc(&p->spk_idx, n_neurons))) return cleanup(rc);
     if ((rc = dalloc(&p->spk_cnt, 1))) return cleanup(rc);
     if ((rc = dalloc(&p->err, 1))) return cleanup(rc);
     if ((rc = dalloc(&p->cnt, (size_t)3 * p->stride))) return cleanup(rc);
     if ((rc = dalloc(&p->fill, (size_t)3 * p->stride))) return cleanup(rc);
     if ((rc = dalloc(&p->seg, (size_t)3 * p->stride))) return cleanup(rc);
-    if ((rc = dalloc(&p->tile_flag, p->n_tiles))) return cleanup(rc);
+    if ((rc = dalloc(&p->tile_flag, max_tiles))) return cleanup(rc);
     if ((rc = dalloc(&p->bump, 2))) return cleanup(rc);
     if ((rc = p->reserve_events(1u << 16))) return cleanup(rc);
     if (model == EDLUT_LIF) {
@@ -637,24 +777,61 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
         hipHostMalloc((void **)&p->h_cnt, sizeof(int32_t), hipHostMallocDefault) != hipSuccess)
         return cleanup(fail(EDLUT_E_HIP, "hipHostMalloc failed"));
 
-    // initial state (VectorNeuronState_GPU_C_INTERFACE.cu:66-131)
+    // initial state (VectorNeuronState_GPU_C_INTERFACE.cu:66-131), per group
     std::vector<float> st((size_t)p->nv * p->stride, 0.0f);
-    for (int v = 0; v < p->nv; v++)
-        for (int i = 0; i < n_neurons; i++) st[(size_t)v * p->stride + i] = init[v];
+    for (int g = 0; g < n_groups; g++) {
+        const float *gi = grouped ? &st_init[(size_t)g * p->nv] : init.data();
+        const int64_t b = start[g];
+        const int64_t e = g + 1 < n_groups ? start[g + 1] : n_neurons;
+        for (int v = 0; v < p->nv; v++)
+            for (int64_t i = b; i < e; i++) st[(size_t)v * p->stride + i] = gi[v];
+    }
+    p->init_state = st;
     bool ok = hipMemcpy(p->state, st.data(), st.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(p->exp_lut, expt.data(), expt.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
               hipMemcpy(p->nmda_lut, nmda.data(), nmda.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess &&
               (refr.empty() || hipMemcpy(p->refr, refr.data(), refr.size() * sizeof(float), hipMemcpyHostToDevice) == hipSuccess) &&
-              hipMemset(p->tile_count, 0, sizeof(int32_t) * padded_tiles(p->n_tiles)) == hipSuccess &&
+              hipMemset(p->tile_count, 0, sizeof(int32_t) * padded_tiles((int)max_tiles)) == hipSuccess &&
               hipMemset(p->spk_cnt, 0, sizeof(int32_t)) == hipSuccess &&
               hipMemset(p->err, 0, sizeof(uint32_t)) == hipSuccess &&
               hipMemset(p->cnt, 0, sizeof(uint32_t) * 3 * p->stride) == hipSuccess &&
               hipMemset(p->fill, 0, sizeof(uint32_t) * 3 * p->stride) == hipSuccess &&
-              hipMemset(p->tile_flag, 0, sizeof(uint32_t) * p->n_tiles) == hipSuccess &&
+              hipMemset(p->tile_flag, 0, sizeof(uint32_t) * max_tiles) == hipSuccess &&
               hipMemset(p->seg, 0xff, sizeof(uint64_t) * 3 * p->stride) == hipSuccess &&
               hipMemset(p->bump, 0, 2 * sizeof(uint64_t)) == hipSuccess;
     if (!ok) return cleanup(fail(EDLUT_E_HIP, "device initialisation failed"));
     *out = p;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edlut_population_create(int model, int integrator, int bdf_order, const float *params,
+                            int n_params, float dt_s, int n_neurons, int device,
+                            edlut_population **out) {
+    const int32_t size = n_neurons;
+    if (n_neurons <= 0) {
+        if (out) *out = nullptr;
+        return fail(EDLUT_E_ARG, "population size must be positive");
+    }
+    return create_population(model, integrator, bdf_order, params, n_params, dt_s, 1, &size, false,
+                             device, out);
+}
+
+int edlut_population_create_groups(int model, int integrator, const float *params, int n_params,
+                                   float dt_s, int n_groups, const int32_t *sizes, int device,
+                                   edlut_population **out) {
+    return create_population(model, integrator, 0, params, n_params, dt_s, n_groups, sizes, true,
+                             device, out);
+}
+
+int edlut_population_groups(const edlut_population *p, int *n_groups, int32_t *start, int32_t *size) {
+    if (!p || !n_groups) return fail(EDLUT_E_ARG, "null argument");
+    *n_groups = (int)p->g_start.size();
+    if (start) std::memcpy(start, p->g_start.data(), sizeof(int32_t) * p->g_start.size());
+    if (size) std::memcpy(size, p->g_size.data(), sizeof(int32_t) * p->g_size.size());
     return 0;
 }
 
@@ -669,7 +846,7 @@ int edlut_population_destroy(edlut_population *p) {
                    p->state, p->last_spike, p->exp_lut, p->nmda_lut, p->refr, p->tile_count,
                    p->staging, p->spk_idx, p->spk_cnt, p->err, p->cnt, p->fill, p->seg,
                    p->tile_flag, p->bump, p->b_ord, p->b_w, p->e_tgt,
-                   p->e_type, p->e_w};
+                   p->e_type, p->e_w, p->d_groups, p->d_chunk_group};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     if (p->h_spk) (void)hipHostFree(p->h_spk);
@@ -756,10 +933,12 @@ int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
     UpdateArgs a = make_update_args(p, (int32_t)step);
     a.out = TileOut{p->tile_count, p->staging};
     a.fold = p->fold_target(par);
-    UpdateFn fn = pick_update(p->model, p->integ, p->mask, false, p->newton_fp64);
+    UpdateFn fn = pick_update(p, false);
+    if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
     p->started = true;
-    hipLaunchKernelGGL(fn, dim3(p->n_tiles), dim3(kBlock), 0, p->stream, a);
-    launch_compact(p->stream, p->tile_count, p->staging, p->n_tiles, kUpdTile, p->spk_idx, p->spk_cnt,
+    const int tiles = p->update_tiles();
+    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), 0, p->stream, a);
+    launch_compact(p->stream, p->tile_count, p->staging, tiles, kBlock * p->items, p->spk_idx, p->spk_cnt,
                    1, 0, nullptr, nullptr, nullptr, 0, p->err, StepRef{nullptr, (int32_t)step});
     HIP_TRY(hipGetLastError());
     // 3. hand the compacted spikes back (synchronous, like the reference)
@@ -849,8 +1028,8 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
     int rc;
     if ((rc = dalloc(&n->own_ring_idx, (size_t)n->ring * pop->n)) ||
         (rc = dalloc(&n->own_ring_cnt, n->ring)) ||
-        (rc = dalloc(&n->tile_count, padded_tiles(pop->n_tiles))) ||
-        (rc = dalloc(&n->staging, (size_t)pop->n_tiles * kUpdTile)) ||
+        (rc = dalloc(&n->tile_count, padded_tiles((int)(pop->stride / kFlagTile)))) ||
+        (rc = dalloc(&n->staging, (size_t)pop->stride)) ||
         (rc = dalloc(&n->log_step, n->log_cap)) || (rc = dalloc(&n->log_idx, n->log_cap)) ||
         (rc = dalloc(&n->log_count, 2)) || (rc = dalloc(&n->d_step, 1))) {
         delete n;
@@ -858,7 +1037,7 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
     }
     HIP_TRY(hipMemset(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring));
     HIP_TRY(hipMemset(n->log_count, 0, 2 * sizeof(int64_t)));
-    HIP_TRY(hipMemset(n->tile_count, 0, sizeof(int32_t) * padded_tiles(pop->n_tiles)));
+    HIP_TRY(hipMemset(n->tile_count, 0, sizeof(int32_t) * padded_tiles((int)(pop->stride / kFlagTile))));
     pop->net = n;
     *out = n;
     return 0;
@@ -954,6 +1133,7 @@ int edlut_network_add_poisson(edlut_network *n, int n_src, const uint32_t *lcg_s
         (rc = dalloc(&s.staging, (size_t)tiles * kBlock)))
         return rc;
     HIP_TRY(hipMemcpy(s.lcg, lcg_state, sizeof(uint32_t) * n_src, hipMemcpyHostToDevice));
+    s.lcg0.assign(lcg_state, lcg_state + n_src);
     HIP_TRY(hipMemset(s.ring_cnt, 0, sizeof(int32_t) * n->ring));
     HIP_TRY(hipMemset(s.tile_count, 0, sizeof(int32_t) * padded_tiles(tiles)));
     n->sources.push_back(s);
@@ -1003,7 +1183,7 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     if (re0.n_sources > 0) {
         RingEvents re = re0;
         re.step = step;
-        int rc = launch_fold(p, 0, 0, true, &re, nullptr);
+        int rc = launch_fold(p, 0, 0, true, &re, nullptr, n->fold_lanes, n->fold_blocks);
         if (rc) return rc;
     }
     UpdateArgs a = make_update_args(p, 0);
@@ -1011,10 +1191,11 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     a.ext = n->ext;
     a.out = TileOut{n->tile_count, n->staging};
     a.fold = p->fold_target(0, true);
+    const int tiles = p->update_tiles();
     if (ev0) HIP_TRY(hipEventRecord(ev0, p->stream));
-    hipLaunchKernelGGL(fn, dim3(p->n_tiles), dim3(kBlock), 0, p->stream, a);
+    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), 0, p->stream, a);
     if (ev1) HIP_TRY(hipEventRecord(ev1, p->stream));
-    launch_compact(p->stream, n->tile_count, n->staging, p->n_tiles, kUpdTile, n->own_ring_idx,
+    launch_compact(p->stream, n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
                    n->own_ring_cnt, n->ring, p->n, n->log_step, n->log_idx, n->log_count, n->log_cap,
                    p->err, step);
     return 0;
@@ -1089,8 +1270,8 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
         HIP_TRY(hipStreamSynchronize(p->stream));
     }
     HIP_TRY(hipMemsetAsync(p->bump, 0, 2 * sizeof(uint64_t), p->stream));
-    UpdateFn fn = pick_update(p->model, p->integ, p->mask, n->has_ext, p->newton_fp64);
-    if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model/integrator");
+    UpdateFn fn = pick_update(p, n->has_ext);
+    if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
     p->started = true;
     const bool want_timing = n->timing && n->ev.size() >= 2 * (size_t)kGraphChunk;
     int64_t k = first_step;
@@ -1226,6 +1407,237 @@ int edlut_network_update_timing(edlut_network *n, double *ms, int64_t *launches)
     }
     *ms = total;
     *launches = n->timed;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Synthetic recurrent wiring generated on the device (one source, like
+// edlut_network_add_recurrent): see generate.hpp for the distribution.
+int edlut_network_generate_recurrent(edlut_network *n, int in_degree, float exc_fraction, float w_mu,
+                                     float w_sigma, uint64_t seed) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (n->self_src >= 0) return fail(EDLUT_E_STATE, "recurrent synapses already added");
+    if ((int)n->sources.size() >= kMaxSources) return fail(EDLUT_E_CAP, "too many sources");
+    if (in_degree < 1 || in_degree > 256) return fail(EDLUT_E_ARG, "in_degree must be in [1, 256]");
+    if (!(exc_fraction >= 0.0f && exc_fraction <= 1.0f)) return fail(EDLUT_E_ARG, "exc_fraction must be in [0, 1]");
+    if (!std::isfinite(w_mu) || !std::isfinite(w_sigma) || w_sigma < 0.0f)
+        return fail(EDLUT_E_ARG, "bad weight distribution");
+    edlut_population *p = n->pop;
+    HIP_TRY(hipSetDevice(p->device));
+    const int ng = (int)p->g_start.size();
+    std::vector<int32_t> g_exc(ng);
+    int64_t n_syn = 0;
+    uint32_t types = 0;
+    for (int g = 0; g < ng; g++) {
+        if (p->g_size[g] < 2) return fail(EDLUT_E_ARG, "group %d has fewer than 2 neurons", g);
+        g_exc[g] = (int32_t)std::lround((double)exc_fraction * p->g_size[g]);
+        n_syn += (int64_t)p->g_size[g] * in_degree;
+        if (g_exc[g] > 0) types |= EDLUT_SYN_EXC;
+        if (g_exc[g] < p->g_size[g]) types |= EDLUT_SYN_INH;
+    }
+    if (n_syn >= (1ll << 40)) return fail(EDLUT_E_ARG, "too many synapses");
+    const int64_t rows = (int64_t)p->n * n->max_delay;
+    const int64_t nch = (rows + kScanChunk - 1) / kScanChunk;
+    int32_t *d_start = nullptr, *d_size = nullptr, *d_exc = nullptr;
+    uint32_t *row_cnt = nullptr;
+    int64_t *sums = nullptr;
+    uint64_t *key = nullptr;
+    Source s;
+    s.self = true;
+    s.n = p->n;
+    s.ring_idx = n->own_ring_idx;
+    s.ring_cnt = n->own_ring_cnt;
+    auto release = [&](int code) {
+        void *tmp[] = {d_start, d_size, d_exc, row_cnt, sums, key};
+        for (void *q : tmp)
+            if (q) (void)hipFree(q);
+        if (code) {
+            void *kept[] = {s.row_ptr, s.tgt_type, s.w};
+            for (void *q : kept)
+                if (q) (void)hipFree(q);
+        }
+        return code;
+    };
+    int rc;
+    if ((rc = dalloc(&d_start, ng)) || (rc = dalloc(&d_size, ng)) || (rc = dalloc(&d_exc, ng)) ||
+        (rc = dalloc(&row_cnt, rows)) || (rc = dalloc(&sums, nch + 1)) || (rc = dalloc(&key, n_syn)) ||
+        (rc = dalloc(&s.row_ptr, rows + 1)) || (rc = dalloc(&s.tgt_type, n_syn)) || (rc = dalloc(&s.w, n_syn)))
+        return release(rc);
+    if (hipMemcpy(d_start, p->g_start.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_size, p->g_size.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_exc, g_exc.data(), sizeof(int32_t) * ng, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemsetAsync(row_cnt, 0, sizeof(uint32_t) * rows, p->stream) != hipSuccess)
+        return release(fail(EDLUT_E_HIP, "generator set-up failed"));
+    GenArgs ga;
+    ga.g_start = d_start;
+    ga.g_size = d_size;
+    ga.g_exc = d_exc;
+    ga.chunk_group = p->grouped ? p->d_chunk_group : nullptr;
+    ga.n = p->n;
+    ga.in_degree = in_degree;
+    ga.max_delay = n->max_delay;
+    ga.w_mu = w_mu;
+    ga.w_sigma = w_sigma;
+    ga.seed = seed;
+    ga.row_cnt = row_cnt;
+    ga.row_ptr = s.row_ptr;
+    ga.key = key;
+    ga.tgt_type = s.tgt_type;
+    ga.w = s.w;
+    const unsigned tb = (unsigned)((p->n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL((k_gen_edges<false>), dim3(tb), dim3(kBlock), 0, p->stream, ga);
+    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nch), dim3(kBlock), 0, p->stream, row_cnt, rows, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, p->stream, sums, nch);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nch), dim3(kBlock), 0, p->stream, row_cnt, rows,
+                       (const int64_t *)sums, s.row_ptr);
+    if (hipMemsetAsync(row_cnt, 0, sizeof(uint32_t) * rows, p->stream) != hipSuccess)
+        return release(fail(EDLUT_E_HIP, "hipMemsetAsync failed"));
+    hipLaunchKernelGGL((k_gen_edges<true>), dim3(tb), dim3(kBlock), 0, p->stream, ga);
+    hipLaunchKernelGGL(k_gen_rows, dim3((unsigned)((rows + kBlock - 1) / kBlock)), dim3(kBlock), 0,
+                       p->stream, ga);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess)
+        return release(fail(EDLUT_E_HIP, "connectivity generation failed"));
+    int64_t total = 0;
+    if (hipMemcpy(&total, s.row_ptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return release(fail(EDLUT_E_HIP, "hipMemcpy failed"));
+    if (total != n_syn) return release(fail(EDLUT_E_STATE, "generated %lld synapses, expected %lld",
+                                            (long long)total, (long long)n_syn));
+    release(0);
+    p->mask |= types;
+    n->sources.push_back(s);
+    n->dirty = true;
+    n->self_src = (int)n->sources.size() - 1;
+    n->total_syn += (uint64_t)n_syn;
+    return p->reserve_events(n->total_syn + 1);
+}
+
+int edlut_network_get_synapses(edlut_network *n, int source, int64_t *row_ptr, uint32_t *tgt_type,
+                               float *w, int64_t *n_rows, int64_t *n_syn) {
+    if (!n || !n_rows || !n_syn) return fail(EDLUT_E_ARG, "null argument");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    const Source &s = n->sources[source];
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    const int64_t rows = s.n * n->max_delay;
+    int64_t total = 0;
+    HIP_TRY(hipMemcpy(&total, s.row_ptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost));
+    *n_rows = rows;
+    *n_syn = total;
+    if (row_ptr) HIP_TRY(hipMemcpy(row_ptr, s.row_ptr, sizeof(int64_t) * (rows + 1), hipMemcpyDeviceToHost));
+    if (tgt_type && total) HIP_TRY(hipMemcpy(tgt_type, s.tgt_type, sizeof(uint32_t) * total, hipMemcpyDeviceToHost));
+    if (w && total) HIP_TRY(hipMemcpy(w, s.w, sizeof(float) * total, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Back to the state after construction: initial neuron state, generators at
+// their initial states, empty spike rings and log, next run starts at step 1.
+int edlut_network_reset(edlut_network *n) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    edlut_population *p = n->pop;
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    HIP_TRY(hipMemcpy(p->state, p->init_state.data(), sizeof(float) * p->init_state.size(), hipMemcpyHostToDevice));
+    if (p->last_spike) {
+        hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(kBlock), 0, p->stream, p->last_spike, p->stride,
+                           -(1 << 30));
+        HIP_TRY(hipGetLastError());
+    }
+    if (p->bdf_st) {
+        HIP_TRY(hipMemsetAsync(p->bdf_st, 0, sizeof(int32_t) * p->stride, p->stream));
+        int rc = p->alloc_bdf_history(p->newton_fp64 ? sizeof(double) : sizeof(float));
+        if (rc) return rc;
+    }
+    for (Source &s : n->sources) {
+        if (!s.self && !s.lcg0.empty())
+            HIP_TRY(hipMemcpy(s.lcg, s.lcg0.data(), sizeof(uint32_t) * s.lcg0.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(s.ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
+    }
+    HIP_TRY(hipMemsetAsync(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
+    HIP_TRY(hipMemsetAsync(n->log_count, 0, 2 * sizeof(int64_t), p->stream));
+    HIP_TRY(hipMemsetAsync(p->err, 0, sizeof(uint32_t), p->stream));
+    HIP_TRY(hipMemsetAsync(p->cnt, 0, sizeof(uint32_t) * 3 * p->stride, p->stream));
+    HIP_TRY(hipMemsetAsync(p->fill, 0, sizeof(uint32_t) * 3 * p->stride, p->stream));
+    HIP_TRY(hipMemsetAsync(p->seg, 0xff, sizeof(uint64_t) * 3 * p->stride, p->stream));
+    HIP_TRY(hipMemsetAsync(p->tile_flag, 0, sizeof(uint32_t) * (p->stride / kFlagTile), p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    n->last_step = 0;
+    return 0;
+}
+
+int edlut_network_set_launch(edlut_network *n, int items, int fold_lanes, int fold_blocks) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    edlut_population *p = n->pop;
+    if (p->grouped ? (items != 2 && items != 4) : items != kItems)
+        return fail(EDLUT_E_ARG, "items per thread must be %s", p->grouped ? "2 or 4" : "4");
+    if (!pick_ring_fold(kCount, fold_lanes)) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
+    if (fold_blocks < 1 || fold_blocks > 65536) return fail(EDLUT_E_ARG, "fold blocks must be in [1, 65536]");
+    p->items = items;
+    n->fold_lanes = fold_lanes;
+    n->fold_blocks = fold_blocks;
+    n->dirty = true;
+    return 0;
+}
+
+int edlut_network_get_launch(const edlut_network *n, int *items, int *fold_lanes, int *fold_blocks) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (items) *items = n->pop->items;
+    if (fold_lanes) *fold_lanes = n->fold_lanes;
+    if (fold_blocks) *fold_blocks = n->fold_blocks;
+    return 0;
+}
+
+// Times `trial_steps` steps (after 16 warm-up steps) of every launch shape
+// and keeps the fastest; the network is reset before each trial and after
+// the search.  The results of a run do not depend on the launch shape.
+int edlut_network_autotune(edlut_network *n, int trial_steps, int *items, int *fold_lanes,
+                           int *fold_blocks, double *ms_per_step) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (trial_steps < 16 || trial_steps > 100000) return fail(EDLUT_E_ARG, "trial_steps must be in [16, 100000]");
+    edlut_population *p = n->pop;
+    HIP_TRY(hipSetDevice(p->device));
+    std::vector<int> it_c = p->grouped ? std::vector<int>{2, 4} : std::vector<int>{kItems};
+    const int lanes_c[] = {16, 32, 64};
+    const int blocks_c[] = {256, 1024, 4096};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best = 1e300;
+    int b_it = p->items, b_l = n->fold_lanes, b_b = n->fold_blocks;
+    int rc = 0;
+    for (int it : it_c)
+        for (int l : lanes_c)
+            for (int b : blocks_c) {
+                if ((rc = edlut_network_set_launch(n, it, l, b)) || (rc = edlut_network_reset(n)) ||
+                    (rc = edlut_network_run(n, 1, 16)))
+                    goto done;
+                if (hipEventRecord(e0, p->stream) != hipSuccess) {
+                    rc = fail(EDLUT_E_HIP, "hipEventRecord failed");
+                    goto done;
+                }
+                if ((rc = edlut_network_run(n, 17, trial_steps))) goto done;
+                if (hipEventRecord(e1, p->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+                    rc = fail(EDLUT_E_HIP, "timing failed");
+                    goto done;
+                }
+                if ((rc = check_err(p))) goto done;
+                float ms = 0.0f;
+                (void)hipEventElapsedTime(&ms, e0, e1);
+                if (ms < best) {
+                    best = ms;
+                    b_it = it;
+                    b_l = l;
+                    b_b = b;
+                }
+            }
+done:
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if ((rc = edlut_network_set_launch(n, b_it, b_l, b_b)) || (rc = edlut_network_reset(n))) return rc;
+    if (items) *items = b_it;
+    if (fold_lanes) *fold_lanes = b_l;
+    if (fold_blocks) *fold_blocks = b_b;
+    if (ms_per_step) *ms_per_step = best / trial_steps;
     return 0;
 }
 

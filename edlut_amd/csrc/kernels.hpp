@@ -34,7 +34,9 @@ enum Integ { kEuler = 0, kRK2 = 1, kRK4 = 2 };
 // network mode, so it cannot overflow there.
 // ---------------------------------------------------------------------------
 constexpr int kItems = 4;                    // neurons per thread in the update kernels
-constexpr int kUpdTile = kBlock * kItems;    // neurons per update tile
+constexpr int kUpdTile = kBlock * kItems;    // neurons per update tile (kItems per thread)
+constexpr int kFlagTile = 512;               // neurons per fold flag (smallest update tile)
+constexpr int kGroupAlign = 256;             // grouped populations: groups start on this
 constexpr uint64_t kSegEmpty = ~0ull;
 constexpr uint64_t kSegBusy = ~0ull - 1ull;
 
@@ -45,7 +47,7 @@ struct FoldTarget {
     uint32_t *cnt;       // per key: events this step
     uint32_t *fill;      // per key: events scattered so far
     uint64_t *seg;       // per key: segment start (kSegEmpty when none)
-    uint32_t *tile_flag; // per update tile: some key of the tile has >= 2 events
+    uint32_t *tile_flag; // per kFlagTile neurons: some key there has >= 2 events
     uint64_t *bump;      // [2] event-buffer allocator, by step parity
     int par;             // parity of this step, unless par_by_step
     int par_by_step;     // parity = update index & 1 (network mode)
@@ -84,7 +86,7 @@ __device__ __forceinline__ void fold_event(const FoldTarget &t, int par, uint64_
         const uint64_t base = atomicAdd((unsigned long long *)&t.bump[par], (unsigned long long)c);
         __hip_atomic_store(&t.seg[key], base, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         const uint64_t tgt = key % (uint64_t)t.stride;
-        t.tile_flag[tgt / kUpdTile] = 1u;
+        t.tile_flag[tgt / kFlagTile] = 1u;
     }
     uint64_t base = __hip_atomic_load(&t.seg[key], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     uint32_t spins = 0;
@@ -145,6 +147,20 @@ struct ExtDrive {
     float w;
 };
 
+// Grouped populations (many independent networks in one index space): group
+// g owns neurons [start, start + n) and its own model parameters.  Groups
+// start on multiples of kGroupAlign, so every wave's span of a tile lies in
+// one group and the group's parameters are wave-uniform (scalar loads).
+struct GroupInfo {
+    ModelParams mp;
+    int32_t start;
+    int32_t n;
+};
+struct GroupTable {
+    const GroupInfo *tab;
+    const uint32_t *chunk_group;   // group of each kGroupAlign-neuron chunk
+};
+
 struct UpdateArgs {
     float *state;          // SoA, row stride `stride`
     int64_t stride;
@@ -163,15 +179,36 @@ struct UpdateArgs {
     ExtDrive ext;
     TileOut out;
     FoldTarget fold;       // ordered input fold of this step (multi-event keys)
+    GroupTable grp;        // grouped populations only
 };
 
-// One tile = kUpdTile neurons; thread t handles neurons tile*kUpdTile +
-// n*kBlock + t for n < kItems (each n is one coalesced row access).  All
-// loads are issued first, the kItems neurons are integrated stage-major, and
-// the stores come last, so every thread keeps kItems independent dependency
-// chains in flight.
-template <class M, int INTEG, unsigned MASK, bool EXT>
+// Reads (and clears) the fold flags covering a tile of T neurons.
+template <int T>
+__device__ __forceinline__ uint32_t take_tile_flag(uint32_t *flags, int tile) {
+    constexpr int F = T / kFlagTile;
+    uint32_t f = 0;
+#pragma unroll
+    for (int q = 0; q < F; q++) f |= flags[tile * F + q];
+    return f;
+}
+template <int T>
+__device__ __forceinline__ void clear_tile_flag(uint32_t *flags, int tile) {
+    constexpr int F = T / kFlagTile;
+#pragma unroll
+    for (int q = 0; q < F; q++) flags[tile * F + q] = 0u;
+}
+
+// One tile = kBlock*ITEMS neurons in the wave-major layout of tile_neuron:
+// each lane handles ITEMS neurons kWave apart (each item is one coalesced row
+// access per wave).  All loads are issued first, the ITEMS neurons are
+// integrated stage-major, and the stores come last, so every thread keeps
+// ITEMS independent dependency chains in flight.  GROUPED takes the model
+// parameters, the live range and the drive seed from the wave's group.
+template <class M, int INTEG, unsigned MASK, bool EXT, int ITEMS, bool GROUPED>
 __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
+    constexpr int T = kBlock * ITEMS;
+    static_assert(T % kFlagTile == 0, "update tiles cover whole fold flags");
+    static_assert(kGroupAlign % (ITEMS * kWave) == 0, "a wave's span lies in one group");
     const int tile = blockIdx.x;
     const int32_t step = a.step.get();
     const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
@@ -179,21 +216,34 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     // the tile's fold flag is read alongside the state loads (the barrier
     // that publishes it comes after them)
     uint32_t flag = 0;
-    if (threadIdx.x == 0) flag = a.fold.tile_flag[tile];
+    if (threadIdx.x == 0) flag = take_tile_flag<T>(a.fold.tile_flag, tile);
+    ModelParams gmp;
+    int32_t gstart = 0, gn = a.n;
+    uint64_t dseed = a.ext.seed;
+    if (GROUPED) {
+        const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+        const int first = tile * T + wid * ITEMS * kWave;
+        const uint32_t g = a.grp.chunk_group[first / kGroupAlign];
+        gmp = a.grp.tab[g].mp;
+        gstart = a.grp.tab[g].start;
+        gn = a.grp.tab[g].n;
+        dseed = a.ext.seed + g;
+    }
+    const ModelParams &mp = GROUPED ? gmp : a.mp;
     // one row pointer per state variable (uniform), 32-bit neuron offsets
     float *__restrict__ row[M::NV];
 #pragma unroll
     for (int j = 0; j < M::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
     const int64_t ld = a.stride;
-    const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, step) : 0u;
-    int idx[kItems];
-    bool live[kItems];
-    float s[kItems][M::NV];
-    RhsCtx c[kItems];
+    const uint32_t step_key = EXT ? drive_step_key(dseed, step) : 0u;
+    int idx[ITEMS];
+    bool live[ITEMS];
+    float s[ITEMS][M::NV];
+    RhsCtx c[ITEMS];
 #pragma unroll
-    for (int n = 0; n < kItems; n++) {
-        idx[n] = tile * kUpdTile + n * kBlock + (int)threadIdx.x;
-        live[n] = idx[n] < a.n;
+    for (int n = 0; n < ITEMS; n++) {
+        idx[n] = tile_neuron<ITEMS>(tile, n);
+        live[n] = idx[n] - gstart < gn;
         // rows are padded to a multiple of kUpdTile, so the loads stay in bounds
         const uint32_t i = (uint32_t)idx[n];
 #pragma unroll
@@ -211,7 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     }
     if (threadIdx.x == 0) {
         s_flag = flag;
-        if (flag) a.fold.tile_flag[tile] = 0u;
+        if (flag) clear_tile_flag<T>(a.fold.tile_flag, tile);
         // the next step's fold allocates from the other parity: clear it
         if (tile == 0) a.fold.bump[par ^ 1] = 0ull;
     }
@@ -219,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     if (s_flag) {
         // events of keys with several inputs this step, in delivery order
 #pragma unroll
-        for (int n = 0; n < kItems; n++) {
+        for (int n = 0; n < ITEMS; n++) {
             if (!live[n]) continue;
 #pragma unroll
             for (int ty = 0; ty < 3; ty++) {
@@ -234,33 +284,34 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
         }
     }
     if (EXT) {
-        // External drive events of this interval, after the recurrent ones.
+        // External drive events of this interval, after the recurrent ones;
+        // the hash takes the neuron's index within its group.
 #pragma unroll
-        for (int n = 0; n < kItems; n++) {
-            const uint32_t u = drive_hash_keyed(step_key, (uint32_t)idx[n]);
+        for (int n = 0; n < ITEMS; n++) {
+            const uint32_t u = drive_hash_keyed(step_key, (uint32_t)(idx[n] - gstart));
             int cnt = 0;
             while (cnt < a.ext.n_thr && u < a.ext.thr[cnt]) cnt++;
             for (int q = 0; q < cnt; q++) s[n][M::EXC] += a.ext.w;
         }
     }
-    float prev_v[kItems];
+    float prev_v[ITEMS];
 #pragma unroll
-    for (int n = 0; n < kItems; n++) prev_v[n] = s[n][0];
-    if (INTEG == kEuler) integrate_euler<M, MASK, kItems>(a.mp, c, a.k, s);
-    else if (INTEG == kRK2) integrate_rk2<M, MASK, kItems>(a.mp, c, a.k, s);
-    else integrate_rk4<M, MASK, kItems>(a.mp, c, a.k, s);
+    for (int n = 0; n < ITEMS; n++) prev_v[n] = s[n][0];
+    if (INTEG == kEuler) integrate_euler<M, MASK, ITEMS>(mp, c, a.k, s);
+    else if (INTEG == kRK2) integrate_rk2<M, MASK, ITEMS>(mp, c, a.k, s);
+    else integrate_rk4<M, MASK, ITEMS>(mp, c, a.k, s);
 
-    bool spk[kItems];
+    bool spk[ITEMS];
 #pragma unroll
-    for (int n = 0; n < kItems; n++) {
+    for (int n = 0; n < ITEMS; n++) {
         const uint32_t i = (uint32_t)idx[n];
-        spk[n] = live[n] && M::spike(a.mp, prev_v[n], s[n]);
+        spk[n] = live[n] && M::spike(mp, prev_v[n], s[n]);
         if (!live[n]) continue;
         if (spk[n] && a.last_spike) a.last_spike[i] = step;
         if (s[n][0] != s[n][0]) {
             // TimeDrivenNeuronModel.cpp:75-88: NaN -> whole state to initial.
 #pragma unroll
-            for (int j = 0; j < M::NV; j++) row[j][i] = M::init(a.mp, j);
+            for (int j = 0; j < M::NV; j++) row[j][i] = M::init(mp, j);
         } else {
 #pragma unroll
             for (int j = 0; j < M::ND; j++) row[j][i] = s[n][j];
@@ -269,7 +320,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
             if (MASK & 4u) row[M::NMDA][i] = s[n][M::NMDA];
         }
     }
-    tile_spikes<kItems>(a.out, tile, spk, idx);
+    tile_spikes<ITEMS>(a.out, tile, spk, idx);
 }
 
 
@@ -284,8 +335,8 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
     __shared__ uint32_t s_flag;
     if (threadIdx.x == 0) {
-        s_flag = a.fold.tile_flag[tile];
-        if (s_flag) a.fold.tile_flag[tile] = 0u;
+        s_flag = take_tile_flag<kUpdTile>(a.fold.tile_flag, tile);
+        if (s_flag) clear_tile_flag<kUpdTile>(a.fold.tile_flag, tile);
         if (tile == 0) a.fold.bump[par ^ 1] = 0ull;
     }
     __syncthreads();
@@ -297,7 +348,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     bool spk[kItems];
     int idx[kItems];
     for (int n = 0; n < kItems; n++) {
-        idx[n] = tile * kUpdTile + n * kBlock + (int)threadIdx.x;
+        idx[n] = tile_neuron<kItems>(tile, n);
         spk[n] = false;
         if (idx[n] >= a.n) continue;
         const uint32_t i = (uint32_t)idx[n];

@@ -115,6 +115,58 @@ class TimeDrivenNeuronModel_GPU:
         if self._mask:
             C.check(C.lib().edlut_population_enable_synapses(self._pop, self._mask))
 
+    def InitializeGroups(self, sizes, group_params=None, device: int = 0) -> None:
+        """Grouped population: len(sizes) independent groups of this model in
+        one index space (parameter sweeps).  group_params is a list of dicts
+        overriding this model's parameters per group, or an array of shape
+        (n_groups, n_params) in GetDefaultParameters() order.  Group g owns
+        neurons [start[g], start[g] + sizes[g]) (GetGroups); the state arrays
+        cover the whole index space, padding included."""
+        if self._pop is not None:
+            raise EDLUTException(C.EDLUT_E_STATE, "states already initialised")
+        sizes = np.ascontiguousarray(sizes, np.int32)
+        ng = len(sizes)
+        base = np.array(list(self._p.values()), np.float32)
+        if group_params is None:
+            params = np.tile(base, (ng, 1))
+        elif isinstance(group_params, np.ndarray):
+            params = np.ascontiguousarray(group_params, np.float32)
+            if params.shape != (ng, len(base)):
+                raise EDLUTException(C.EDLUT_E_ARG, f"group parameters must have shape ({ng}, {len(base)})")
+        else:
+            if len(group_params) != ng:
+                raise EDLUTException(C.EDLUT_E_ARG, "one parameter dict per group expected")
+            names = list(self._p.keys())
+            params = np.tile(base, (ng, 1))
+            for g, d in enumerate(group_params):
+                for k, v in d.items():
+                    if k not in names:
+                        raise EDLUTException(C.EDLUT_E_ARG, f"unknown parameter {k!r} for {self.NAME}")
+                    params[g, names.index(k)] = np.float32(v)
+        params = np.ascontiguousarray(params, np.float32)
+        h = ctypes.c_void_p()
+        C.check(
+            C.lib().edlut_population_create_groups(
+                self.MODEL, _INTEGRATORS[self._int["name"]], C.ptr(params), params.shape[1],
+                float(np.float32(self._int["step"])), ng, C.ptr(sizes), int(device), C.byref(h)
+            )
+        )
+        self._pop = h
+        self._dev = device
+        start = np.zeros(ng, np.int32)
+        size = np.zeros(ng, np.int32)
+        cnt = ctypes.c_int(0)
+        C.check(C.lib().edlut_population_groups(self._pop, C.byref(cnt), C.ptr(start), C.ptr(size)))
+        self._groups = (start, size)
+        self._n = int(start[-1]) + ((int(size[-1]) + 255) // 256) * 256
+        if self._mask:
+            C.check(C.lib().edlut_population_enable_synapses(self._pop, self._mask))
+
+    def GetGroups(self):
+        """(start, size) arrays of the groups (one group for a plain population)."""
+        self._need()
+        return getattr(self, "_groups", (np.zeros(1, np.int32), np.array([self._n], np.int32)))
+
     def close(self) -> None:
         if self._pop is not None:
             C.lib().edlut_population_destroy(self._pop)

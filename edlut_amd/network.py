@@ -94,6 +94,50 @@ class Network:
             )
         )
 
+    def generate_recurrent(self, in_degree: int, exc_fraction: float, w_mu: float, w_sigma: float,
+                           seed: int):
+        """Fixed in-degree random wiring built on the device (per group for a
+        grouped population; see include/edlut_amd.h)."""
+        C.check(C.lib().edlut_network_generate_recurrent(self._h, int(in_degree), float(exc_fraction),
+                                                         float(w_mu), float(w_sigma), int(seed)))
+
+    def synapses(self, source: int = 0):
+        """(row_ptr, tgt_type, weight) of a source, copied from the device."""
+        rows = ctypes.c_int64(0)
+        nsyn = ctypes.c_int64(0)
+        C.check(C.lib().edlut_network_get_synapses(self._h, int(source), None, None, None,
+                                                   C.byref(rows), C.byref(nsyn)))
+        rp = np.zeros(rows.value + 1, np.int64)
+        tt = np.zeros(max(nsyn.value, 1), np.uint32)
+        w = np.zeros(max(nsyn.value, 1), np.float32)
+        C.check(C.lib().edlut_network_get_synapses(self._h, int(source), C.ptr(rp), C.ptr(tt), C.ptr(w),
+                                                   C.byref(rows), C.byref(nsyn)))
+        return rp, tt[: nsyn.value], w[: nsyn.value]
+
+    def reset(self):
+        """Back to the initial state; the next run starts at update 1."""
+        C.check(C.lib().edlut_network_reset(self._h))
+        self.next_step = 1
+
+    def set_launch(self, items: int, fold_lanes: int, fold_blocks: int):
+        C.check(C.lib().edlut_network_set_launch(self._h, int(items), int(fold_lanes), int(fold_blocks)))
+
+    def launch(self):
+        """(items per thread, fold lanes, fold workgroups) of the current launch shape."""
+        a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        C.check(C.lib().edlut_network_get_launch(self._h, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    def autotune(self, trial_steps: int = 64):
+        """Pick the fastest launch shape by timing trial runs (the network is
+        reset afterwards).  Returns (items, lanes, blocks, ms_per_step)."""
+        a, b, c = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+        ms = ctypes.c_double(0)
+        C.check(C.lib().edlut_network_autotune(self._h, int(trial_steps), C.byref(a), C.byref(b),
+                                               C.byref(c), C.byref(ms)))
+        self.next_step = 1
+        return a.value, b.value, c.value, ms.value
+
     def set_external(self, thresholds, w_ext, seed):
         thr = np.ascontiguousarray(thresholds, np.uint32)
         C.check(C.lib().edlut_network_set_external(self._h, len(thr), C.ptr(thr), float(w_ext), int(seed)))

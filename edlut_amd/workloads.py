@@ -10,6 +10,11 @@ the same inputs can be fed to the HIP path and to any CPU implementation.
                    without replacement, no self-connections), 80/20 exc/inh
                    sources, weights lognormal(-1, 0.5) nS, delays U{1..10}
                    steps, external drive = 100 Poisson synapses at 5 Hz each.
+  adex_sweep       parameter sweep: many independent AdEx networks (RK4,
+                   dt 0.1 ms) of lognormal sizes, each with its own g_leak,
+                   E_leak, a, b, tau_w drawn uniformly within +-30% of the
+                   defaults; in-degree 100 wiring and the external drive are
+                   generated per network from its own seed.
 """
 from __future__ import annotations
 
@@ -97,3 +102,38 @@ def adex_recurrent(n: int = 1_000_000, in_degree: int = 200, exc_frac: float = 0
     spec.rec = (src, tgt, delay, typ, w)
     spec.ext = (thr, np.float32(w_ext), seed * 1000003 + 11)
     return spec
+
+
+@dataclass
+class SweepSpec:
+    sizes: np.ndarray            # int32 [networks]
+    params: np.ndarray           # float32 [networks, 14] (AdEx parameter order)
+    in_degree: int = 100
+    exc_fraction: float = 0.8
+    w_mu: float = -1.0
+    w_sigma: float = 0.5
+    max_delay: int = 10
+    ext: tuple = ()              # (thresholds, w_ext, seed)
+    wiring_seed: int = 0
+    steps: int = 5000
+
+
+SWEPT = ("g_leak", "e_leak", "a", "b", "tau_w")
+
+
+def adex_sweep(n_networks: int = 4096, median: float = 5000.0, sigma: float = 0.5,
+               lo: int = 1000, hi: int = 20000, spread: float = 0.3, w_ext: float = 1.0,
+               steps: int = 5000, seed: int = 4) -> SweepSpec:
+    from .models import AdExTimeDrivenModel_GPU
+
+    rng = np.random.default_rng(seed)
+    sizes = np.clip(np.round(rng.lognormal(np.log(median), sigma, n_networks)), lo, hi).astype(np.int32)
+    names = [k for k, _ in AdExTimeDrivenModel_GPU.PARAMS]
+    base = np.array([v for _, v in AdExTimeDrivenModel_GPU.PARAMS], np.float32)
+    params = np.tile(base, (n_networks, 1))
+    for k in SWEPT:
+        j = names.index(k)
+        params[:, j] = (base[j] * rng.uniform(1.0 - spread, 1.0 + spread, n_networks)).astype(np.float32)
+    thr = binomial_thresholds(100, 5.0 * float(DT_S))
+    return SweepSpec(sizes=sizes, params=params, ext=(thr, np.float32(w_ext), seed * 1000003 + 29),
+                     wiring_seed=seed * 7777 + 3, steps=steps)

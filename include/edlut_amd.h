@@ -70,6 +70,26 @@ int edlut_population_create(int model, int integrator, int bdf_order, const floa
                             edlut_population **out);
 int edlut_population_destroy(edlut_population *pop);
 
+/*
+ * Grouped population: n_groups independent groups (e.g. the networks of a
+ * parameter sweep) of one model and integrator in one index space, so one
+ * launch per kernel advances all of them.  params holds n_groups parameter
+ * vectors (n_params each, order as above); groups may differ in every
+ * parameter except e_exc/e_inh (shared g_nmda_inf table) and, for LIF,
+ * tau_ref (shared refractory table).  Group g starts at start[g] (a multiple
+ * of 256; edlut_population_groups) and owns sizes[g] neurons; the padding
+ * neurons between groups never update or spike.  AdEx and LIF under Euler,
+ * RK2 and RK4, with excitatory/inhibitory synapses and the external drive.
+ * No counterpart in the reference, which builds one network per simulation.
+ */
+int edlut_population_create_groups(int model, int integrator, const float *params, int n_params,
+                                   float dt_s, int n_groups, const int32_t *sizes, int device,
+                                   edlut_population **out);
+/* Group layout: count, and (if non-null) first neuron and size of each group.
+ * A plain population is one group starting at 0. */
+int edlut_population_groups(const edlut_population *pop, int *n_groups, int32_t *start,
+                            int32_t *size);
+
 /* Number of state variables and the SoA row stride (>= n_neurons). */
 int edlut_population_layout(const edlut_population *pop, int *n_vars, int *n_diff,
                             int64_t *stride);
@@ -149,7 +169,9 @@ int edlut_network_add_poisson(edlut_network *net, int n_src, const uint32_t *lcg
                               const float *weight);
 /* Counter-based external drive: per neuron and step, k events of weight w_ext
  * into the excitatory conductance, k = #{j : hash(seed,step,i) < thr[j]},
- * folded after the interval's recurrent events. */
+ * folded after the interval's recurrent events.  In a grouped population
+ * neuron i of group g uses seed + g and its index within the group, so each
+ * group is driven as the same network built on its own would be. */
 int edlut_network_set_external(edlut_network *net, int n_thr, const uint32_t *thresholds,
                                float w_ext, uint64_t seed);
 /* Enqueue n_steps updates starting at update index first_step (asynchronous;
@@ -173,6 +195,41 @@ int edlut_network_set_timing(edlut_network *net, int64_t max_steps);
 /* Per-launch timing of the update kernel over the last run (HIP events on the
  * library's stream): total milliseconds and launch count. */
 int edlut_network_update_timing(edlut_network *net, double *ms, int64_t *launches);
+
+/*
+ * Synthetic recurrent wiring generated on the device (replaces building and
+ * uploading a host edge list for large synthetic networks; no counterpart in
+ * the reference, whose networks come from files, Network.cpp:445-640).
+ * Every neuron receives in_degree synapses from sources drawn uniformly
+ * within its own group, delays uniform in [1, max_delay], weights
+ * lognormal(w_mu, w_sigma); the first round(exc_fraction * size) neurons of a
+ * group are excitatory sources, the rest inhibitory.  Group g is generated
+ * from seed + g and local indices, so it equals the wiring of a plain
+ * population of its size generated with that seed.
+ */
+int edlut_network_generate_recurrent(edlut_network *net, int in_degree, float exc_fraction,
+                                     float w_mu, float w_sigma, uint64_t seed);
+/* Copy a source's synapse arrays to the host (row layout above).  Sizes are
+ * returned in n_rows (row_ptr holds n_rows+1 entries) and n_syn; null arrays
+ * are skipped, so a first call with nulls sizes the buffers. */
+int edlut_network_get_synapses(edlut_network *net, int source, int64_t *row_ptr,
+                               uint32_t *tgt_type, float *weight, int64_t *n_rows,
+                               int64_t *n_syn);
+/* Back to the initial state: neuron state as after construction, Poisson
+ * generators at their initial states, empty spike rings and log; the next
+ * run starts at update 1. */
+int edlut_network_reset(edlut_network *net);
+/* Launch shape: neurons per thread of the update kernel (4; 2 or 4 for a
+ * grouped population), lanes per spike and workgroups of the ring fold
+ * (lanes 8, 16, 32 or 64).  Results do not depend on the launch shape. */
+int edlut_network_set_launch(edlut_network *net, int items, int fold_lanes, int fold_blocks);
+int edlut_network_get_launch(const edlut_network *net, int *items, int *fold_lanes,
+                             int *fold_blocks);
+/* Autotuner: times trial_steps steps of every launch shape on this network
+ * and keeps the fastest (returned, with its time per step); resets the
+ * network before each trial and after the search. */
+int edlut_network_autotune(edlut_network *net, int trial_steps, int *items, int *fold_lanes,
+                           int *fold_blocks, double *ms_per_step);
 
 /*
  * Host helper: sort an edge list into the row layout above, in the order of

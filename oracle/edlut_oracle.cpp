@@ -863,6 +863,12 @@ void orc_nmda_table(float e_exc, float e_inh, float *out) {
     std::memcpy(out, t.v, sizeof(float) * kNmdaSize);
 }
 
+void orc_hh_table(float v_thr, float *out) {
+    HHTable t;
+    t.build(v_thr);
+    std::memcpy(out, t.v.data(), sizeof(float) * t.v.size());
+}
+
 void orc_lcg_seeds(uint32_t seed, int n, uint32_t *out) {
     /* RandomGenerator::srand then n constructions (RandomGenerator.cpp:21-24). */
     uint32_t g = seed;

@@ -36,6 +36,10 @@ void orc_exp_lut(const float *x, float *out, int n);
 /* g_nmda_inf table (256 floats) for the given reversal potentials. */
 void orc_nmda_table(float e_exc, float e_inh, float *out);
 
+/* HH channel-rate table (262144 x {alpha_m, beta_m, alpha_h, beta_h,
+ * alpha_n, beta_n} on [-90, 50] mV) for the given v_thr. */
+void orc_hh_table(float v_thr, float *out);
+
 /* ---- RandomGenerator / Poisson device ----------------------------------- */
 /* Seeds handed to n successive RandomGenerator objects after srand(seed). */
 void orc_lcg_seeds(uint32_t seed, int n, uint32_t *out);

@@ -54,6 +54,7 @@ def lib():
         l.orc_exp_table.argtypes = [P]
         l.orc_exp_lut.argtypes = [P, P, ctypes.c_int]
         l.orc_nmda_table.argtypes = [ctypes.c_float, ctypes.c_float, P]
+        l.orc_hh_table.argtypes = [ctypes.c_float, P]
         l.orc_lcg_seeds.argtypes = [ctypes.c_uint32, ctypes.c_int, P]
         l.orc_lcg_frand.argtypes = [ctypes.c_uint32, ctypes.c_int, P]
         l.orc_drive_hash.restype = ctypes.c_uint32
@@ -93,6 +94,12 @@ def exp_lut(x) -> np.ndarray:
 def nmda_table(e_exc: float, e_inh: float) -> np.ndarray:
     t = np.zeros(256, np.float32)
     lib().orc_nmda_table(e_exc, e_inh, _p(t))
+    return t
+
+
+def hh_table(v_thr: float) -> np.ndarray:
+    t = np.zeros((262144, 6), np.float32)
+    lib().orc_hh_table(v_thr, _p(t))
     return t
 
 

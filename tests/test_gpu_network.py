@@ -80,7 +80,7 @@ def test_graph_replay_equals_direct_launches():
     for graph in (True, False):
         model, net = E.from_spec(spec)
         net.set_graph(graph)
-        for n in (37, 16, 5, 64, 1):
+        for n in (370, 16, 5, 641, 1, 300):
             net.run(n)
         st, ix = net.spikes()
         res.append((st, ix, [model.GetVectorNeuronState(v) for v in range(4)]))

@@ -1,0 +1,168 @@
+"""Grouped populations (batched parameter sweeps) on the GPU.
+
+A grouped population advances many independent networks in one launch per
+kernel.  Each group must behave exactly like the same network built on its
+own (plain population, wiring generated from the group's seed, drive seeded
+with seed + g): same spike raster, bit-identical state.  The plain networks
+are in turn checked against the oracle, and the launch shape chosen by the
+autotuner must not change any result.
+"""
+import numpy as np
+import pytest
+
+import edlut_amd as E
+from edlut_amd.network import Network, binomial_thresholds
+from tests.helpers import ADEX_DEFAULTS, LIF_DEFAULTS, bits
+
+pytestmark = pytest.mark.gpu
+
+SIZES = np.array([300, 1000, 257, 2000, 512, 77, 1500], np.int32)
+IN_DEG, MAX_D, W_MU, W_SIG, SEED, EXT_SEED = 20, 6, -1.0, 0.5, 1234, 777
+
+
+def _params(model, n_groups, seed=0):
+    base = {"AdExTimeDrivenModel_GPU": ADEX_DEFAULTS, "LIFTimeDrivenModel_GPU": LIF_DEFAULTS}[model]
+    rng = np.random.default_rng(seed)
+    p = np.tile(base, (n_groups, 1))
+    if model == "AdExTimeDrivenModel_GPU":
+        cols = [9, 8, 0, 1, 4]          # g_leak e_leak a b tau_w
+    else:
+        cols = [8, 2, 4]                # g_leak e_leak c_m
+    for j in cols:
+        p[:, j] = (base[j] * rng.uniform(0.7, 1.3, n_groups)).astype(np.float32)
+    return p
+
+
+def _grouped(model, integ, params, sizes, w_ext, items=None):
+    m = E.CreateNeuronModel(model)
+    m.SetParameters({"int_meth": E.integration_method(integ, 1e-4)})
+    m.InitializeGroups(sizes, params)
+    net = Network(m, MAX_D)
+    net.generate_recurrent(IN_DEG, 0.8, W_MU, W_SIG, SEED)
+    net.set_external(binomial_thresholds(100, 5.0 * 1e-4), w_ext, EXT_SEED)
+    if items is not None:
+        net.set_launch(*items)
+    return m, net
+
+
+def _plain(model, integ, params_g, n, g, w_ext):
+    m = E.CreateNeuronModel(model)
+    names = [k for k, _ in type(m).PARAMS]
+    m.SetParameters({k: float(v) for k, v in zip(names, params_g)})
+    m.SetParameters({"int_meth": E.integration_method(integ, 1e-4)})
+    m.InitializeStates(int(n))
+    net = Network(m, MAX_D)
+    net.generate_recurrent(IN_DEG, 0.8, W_MU, W_SIG, SEED + g)
+    net.set_external(binomial_thresholds(100, 5.0 * 1e-4), w_ext, EXT_SEED + g)
+    return m, net
+
+
+@pytest.mark.parametrize("model,integ,w_ext", [("AdExTimeDrivenModel_GPU", "RK4", 2.0),
+                                               ("LIFTimeDrivenModel_GPU", "Euler", 0.6)])
+def test_groups_equal_separate_networks(oracle, model, integ, w_ext):
+    steps = 600
+    params = _params(model, len(SIZES), seed=11)
+    print(f"case grouped {model} {integ} groups={len(SIZES)} steps={steps}", flush=True)
+    gm, gnet = _grouped(model, integ, params, SIZES, w_ext)
+    gnet.run(steps)
+    gst, gix = gnet.spikes()
+    start, size = gm.GetGroups()
+    assert np.array_equal(size, SIZES)
+    assert np.all(start % 256 == 0)
+    gstate = [gm.GetVectorNeuronState(v) for v in range(type(gm).N_DIFF + 2)]
+    total = 0
+    for g in range(len(SIZES)):
+        sel = (gix >= start[g]) & (gix < start[g] + SIZES[g])
+        pm, pnet = _plain(model, integ, params[g], SIZES[g], g, w_ext)
+        pnet.run(steps)
+        pst, pix = pnet.spikes()
+        assert np.array_equal(gst[sel], pst), f"group {g}: spike steps"
+        assert np.array_equal(gix[sel] - start[g], pix), f"group {g}: spike indices"
+        for v, full in enumerate(gstate):
+            assert np.array_equal(bits(full[start[g]:start[g] + SIZES[g]]),
+                                  bits(pm.GetVectorNeuronState(v))), f"group {g} variable {v}"
+        total += len(pst)
+        if g in (0, 3):
+            # the plain network against the oracle, on the wiring the device built
+            pop = oracle.Population(model, integ, params[g], int(SIZES[g]), mask=3)
+            onet = oracle.Net(pop, MAX_D)
+            onet.add_self(*pnet.synapses(0))
+            onet.set_external(binomial_thresholds(100, 5.0 * 1e-4), w_ext, EXT_SEED + g)
+            ost, oix = onet.run(1, steps)
+            assert np.array_equal(ost, pst) and np.array_equal(oix, pix), f"group {g}: oracle raster"
+            for v in range(pop.nv - 2):
+                assert np.array_equal(bits(pop.state(v)), bits(pm.GetVectorNeuronState(v))), f"oracle var {v}"
+        pnet.close()
+        pm.close()
+    # padding neurons never spike
+    inside = np.zeros(len(gix), bool)
+    for g in range(len(SIZES)):
+        inside |= (gix >= start[g]) & (gix < start[g] + SIZES[g])
+    assert inside.all()
+    assert total == len(gst) and total > 200, "the groups should spike"
+
+
+def test_generated_wiring_properties():
+    sizes = np.array([700, 2, 1300, 256], np.int32)
+    params = _params("AdExTimeDrivenModel_GPU", len(sizes))
+    m, net = _grouped("AdExTimeDrivenModel_GPU", "RK4", params, sizes, 1.0)
+    rp, tt, w = net.synapses(0)
+    start, size = m.GetGroups()
+    n = m.size
+    assert len(rp) == n * MAX_D + 1 and rp[-1] == len(tt) == IN_DEG * sizes.sum()
+    tgt = (tt & 0x3FFFFFFF).astype(np.int64)
+    typ = tt >> 30
+    src = np.repeat(np.arange(n * MAX_D) // MAX_D, np.diff(rp))
+    assert np.all(np.bincount(tgt, minlength=n)[np.concatenate(
+        [np.arange(s, s + z) for s, z in zip(start, size)])] == IN_DEG)
+    grp_t = np.searchsorted(start, tgt, side="right") - 1
+    grp_s = np.searchsorted(start, src, side="right") - 1
+    assert np.all(grp_t == grp_s), "synapses stay inside their group"
+    assert np.all(tgt - start[grp_t] < size[grp_t]) and np.all(src - start[grp_s] < size[grp_s])
+    assert np.all(src != tgt), "no self-connections"
+    n_exc = np.round(0.8 * size).astype(np.int64)
+    assert np.array_equal(typ, (src - start[grp_s] >= n_exc[grp_s]).astype(np.uint32))
+    for r in range(0, n * MAX_D, 997):
+        seg = tgt[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(seg) >= 0), "targets ascending within a row"
+    assert np.all(w > 0) and abs(np.median(np.log(w)) - W_MU) < 0.05
+    net.close()
+    m.close()
+
+
+def test_launch_shape_and_autotune_do_not_change_results():
+    sizes = np.array([900, 3000, 400, 2200], np.int32)
+    params = _params("AdExTimeDrivenModel_GPU", len(sizes), seed=5)
+    out = []
+    for shape in [(4, 32, 512), (2, 16, 256), (4, 64, 4096), None]:
+        print(f"case launch shape {shape}", flush=True)
+        m, net = _grouped("AdExTimeDrivenModel_GPU", "RK4", params, sizes, 2.0, items=shape)
+        if shape is None:
+            it, lanes, blocks, ms = net.autotune(32)
+            assert net.launch() == (it, lanes, blocks) and ms > 0
+        net.run(500)
+        st, ix = net.spikes()
+        out.append((st, ix, m.GetVectorNeuronState(0), m.GetVectorNeuronState(1)))
+        # reset restores the initial state: the same run again gives the same raster
+        net.reset()
+        net.run(500)
+        st2, ix2 = net.spikes()
+        assert np.array_equal(st, st2) and np.array_equal(ix, ix2)
+        net.close()
+        m.close()
+    a = out[0]
+    assert len(a[0]) > 100
+    for b in out[1:]:
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        assert np.array_equal(bits(a[2]), bits(b[2])) and np.array_equal(bits(a[3]), bits(b[3]))
+
+
+def test_grouped_rejects_unsupported():
+    with pytest.raises(E.EDLUTException):
+        m = E.CreateNeuronModel("HHTimeDrivenModel_GPU")
+        m.InitializeGroups([300, 400])
+    m = E.CreateNeuronModel("AdExTimeDrivenModel_GPU")
+    p = np.tile(ADEX_DEFAULTS, (2, 1))
+    p[1, 5] = 5.0   # e_exc differs: shared NMDA table
+    with pytest.raises(E.EDLUTException):
+        m.InitializeGroups([300, 400], p)
