@@ -47,6 +47,8 @@ EXPORTS = (
     "edlut_network_generate_recurrent",
     "edlut_network_get_synapses",
     "edlut_network_reset",
+    "edlut_network_set_stdp",
+    "edlut_network_get_stdp",
     "edlut_network_set_launch",
     "edlut_network_get_launch",
     "edlut_network_autotune",
@@ -115,6 +117,8 @@ def _sig(lib):
         "edlut_network_generate_recurrent": (I32, [P, I32, F32, F32, F32, U64]),
         "edlut_network_get_synapses": (I32, [P, I32, P, P, P, P, P]),
         "edlut_network_reset": (I32, [P]),
+        "edlut_network_set_stdp": (I32, [P, I32, F32, F32, F32, F32, F32]),
+        "edlut_network_get_stdp": (I32, [P, I32, P, P, P]),
         "edlut_network_set_launch": (I32, [P, I32, I32, I32]),
         "edlut_network_get_launch": (I32, [P, P, P, P]),
         "edlut_network_autotune": (I32, [P, I32, P, P, P, P]),

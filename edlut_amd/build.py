@@ -29,6 +29,23 @@ FLAGS = [
 ]
 
 
+DRIVER_SRC = os.path.join(HERE, "host", "plugin_driver.cpp")
+DRIVER = os.path.join(HERE, "host", "plugin_driver")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The C++ plug-in driver (host code only, g++), linked to the HIP library."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-o", DRIVER, DRIVER_SRC, "-L", HERE, "-ledlut_amd",
+           "-Wl,-rpath,$ORIGIN/..", "-I", os.path.join(ROOT, "include")]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("g++ failed for " + DRIVER_SRC)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
+
+
 def build(verbose: bool = False) -> str:
     cmd = [HIPCC, *FLAGS, "-o", OUT + ".tmp", SRC]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -38,6 +55,7 @@ def build(verbose: bool = False) -> str:
     os.replace(OUT + ".tmp", OUT)
     if verbose:
         print("built", OUT)
+    build_driver(verbose)
     return OUT
 
 

@@ -284,6 +284,13 @@ struct Source {
     uint32_t *lcg = nullptr;
     std::vector<uint32_t> lcg0;   // initial generator states (reset)
     float rate = 0.0f;
+    // STDP (edlut_network_set_stdp)
+    bool stdp = false;
+    StdpSyn syn{};
+    std::vector<float> w0;        // weights when STDP was switched on (reset)
+    int64_t *in_ptr = nullptr;
+    int64_t *in_syn = nullptr;
+    int64_t nsyn = 0;
     int32_t *tile_count = nullptr;
     int32_t *staging = nullptr;
 };
@@ -324,6 +331,21 @@ struct edlut_network {
     // launch shape of the ring fold (set by hand or by the autotuner)
     int fold_lanes = 32;
     int fold_blocks = 512;
+    // STDP: update time of each ring slot and delays in seconds
+    bool plastic = false;
+    double *d_time = nullptr;        // [ring]
+    double *d_delay_s = nullptr;     // [max_delay + 1]
+    int64_t t_cache_k = 0;           // host copy of t_k for k = t_cache_k
+    double t_cache = 0.0;
+    double time_of(int64_t k) {
+        // t_k = t_{k-1} + double(float dt), t_0 = 0 (TimeEventUpdateNeurons.cpp:78)
+        if (k < t_cache_k) {
+            t_cache_k = 0;
+            t_cache = 0.0;
+        }
+        for (; t_cache_k < k; t_cache_k++) t_cache += (double)pop->dt_s;
+        return t_cache;
+    }
 };
 
 // ---------------------------------------------------------------------------
@@ -1031,9 +1053,21 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
         (rc = dalloc(&n->tile_count, padded_tiles((int)(pop->stride / kFlagTile)))) ||
         (rc = dalloc(&n->staging, (size_t)pop->stride)) ||
         (rc = dalloc(&n->log_step, n->log_cap)) || (rc = dalloc(&n->log_idx, n->log_cap)) ||
-        (rc = dalloc(&n->log_count, 2)) || (rc = dalloc(&n->d_step, 1))) {
+        (rc = dalloc(&n->log_count, 2)) || (rc = dalloc(&n->d_step, 1)) ||
+        (rc = dalloc(&n->d_time, n->ring)) || (rc = dalloc(&n->d_delay_s, max_delay + 1))) {
         delete n;
         return rc;
+    }
+    {
+        // Network::RoundPropagationDelay (Network.cpp:1311-1317) of d steps
+        std::vector<double> ds(max_delay + 1, 0.0);
+        for (int d = 1; d <= max_delay; d++) {
+            const double m = 0.0001;
+            const double r = std::floor((double)d * (double)pop->dt_s * (1.0 / 0.0001) + m * 0.5) * m;
+            ds[d] = r <= 0 ? m : r;
+        }
+        HIP_TRY(hipMemcpy(n->d_delay_s, ds.data(), sizeof(double) * ds.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemset(n->d_time, 0, sizeof(double) * n->ring));
     }
     HIP_TRY(hipMemset(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring));
     HIP_TRY(hipMemset(n->log_count, 0, 2 * sizeof(int64_t)));
@@ -1049,13 +1083,14 @@ int edlut_network_destroy(edlut_network *n) {
     (void)hipStreamSynchronize(n->pop->stream);
     for (Source &s : n->sources) {
         void *dev[] = {s.self ? nullptr : s.ring_idx, s.self ? nullptr : s.ring_cnt, s.row_ptr,
-                       s.tgt_type, s.w, s.lcg, s.tile_count, s.staging};
+                       s.tgt_type, s.w, s.lcg, s.tile_count, s.staging, s.syn.pre, s.syn.post,
+                       s.syn.last, s.in_ptr, s.in_syn};
         for (void *q : dev)
             if (q) (void)hipFree(q);
     }
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
-                   n->log_idx, n->log_count, n->d_step};
+                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_delay_s};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     for (hipEvent_t e : n->ev) (void)hipEventDestroy(e);
@@ -1087,6 +1122,7 @@ static int add_synapses(edlut_network *n, Source &s, const int64_t *row_ptr,
     if ((rc = dalloc(&s.row_ptr, nrow)) || (rc = dalloc(&s.tgt_type, nsyn)) || (rc = dalloc(&s.w, nsyn)))
         return rc;
     HIP_TRY(hipMemcpy(s.row_ptr, row_ptr, sizeof(int64_t) * nrow, hipMemcpyHostToDevice));
+    s.nsyn = nsyn;
     if (nsyn > 0) {
         HIP_TRY(hipMemcpy(s.tgt_type, tgt_type, sizeof(uint32_t) * nsyn, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(s.w, w, sizeof(float) * nsyn, hipMemcpyHostToDevice));
@@ -1164,6 +1200,8 @@ int edlut_network_set_external(edlut_network *n, int n_thr, const uint32_t *thre
 static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingEvents &re0,
                         hipEvent_t ev0, hipEvent_t ev1) {
     edlut_population *p = n->pop;
+    if (n->plastic)
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, p->stream, step, n->d_time, n->ring, (double)p->dt_s);
     // Poisson devices fire at update k (their spikes feed later updates).
     for (Source &s : n->sources) {
         if (s.self) continue;
@@ -1198,6 +1236,23 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     launch_compact(p->stream, n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
                    n->own_ring_cnt, n->ring, p->n, n->log_step, n->log_idx, n->log_count, n->log_cap,
                    p->err, step);
+    // post-synaptic learning of the neurons that fired at this update
+    for (Source &s : n->sources) {
+        if (!s.stdp) continue;
+        StdpPostArgs pa;
+        pa.spk_idx = n->own_ring_idx;
+        pa.spk_cnt = n->own_ring_cnt;
+        pa.slot_len = p->n;
+        pa.ring = n->ring;
+        pa.step = step;
+        pa.in_ptr = s.in_ptr;
+        pa.in_syn = s.in_syn;
+        pa.w = s.w;
+        pa.syn = s.syn;
+        pa.time_ring = n->d_time;
+        pa.exp_lut = p->exp_lut;
+        hipLaunchKernelGGL(k_stdp_post<64>, dim3(64), dim3(kBlock), 0, p->stream, pa);
+    }
     return 0;
 }
 
@@ -1207,9 +1262,13 @@ static RingEvents ring_events(edlut_network *n) {
     re.n_sources = (int)n->sources.size();
     re.max_delay = n->max_delay;
     re.ring = n->ring;
+    re.time_ring = n->d_time;
+    re.delay_s = n->d_delay_s;
+    re.exp_lut = n->pop->exp_lut;
     for (int q = 0; q < re.n_sources; q++) {
         const Source &s = n->sources[q];
-        re.src[q] = RingSource{s.ring_idx, s.ring_cnt, s.row_ptr, s.tgt_type, s.w, s.n};
+        re.src[q] = RingSource{s.ring_idx, s.ring_cnt, s.row_ptr, s.tgt_type, s.w, s.n,
+                               s.stdp ? 1 : 0, s.syn};
     }
     return re;
 }
@@ -1270,6 +1329,13 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
         HIP_TRY(hipStreamSynchronize(p->stream));
     }
     HIP_TRY(hipMemsetAsync(p->bump, 0, 2 * sizeof(uint64_t), p->stream));
+    if (n->plastic) {
+        // update time of the step before the run, in its ring slot
+        const double t0 = n->time_of(first_step - 1);
+        HIP_TRY(hipMemcpyAsync(n->d_time + (first_step - 1) % n->ring, &t0, sizeof(double),
+                               hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipStreamSynchronize(p->stream));
+    }
     UpdateFn fn = pick_update(p, n->has_ext);
     if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
     p->started = true;
@@ -1503,6 +1569,7 @@ int edlut_network_generate_recurrent(edlut_network *n, int in_degree, float exc_
     if (total != n_syn) return release(fail(EDLUT_E_STATE, "generated %lld synapses, expected %lld",
                                             (long long)total, (long long)n_syn));
     release(0);
+    s.nsyn = n_syn;
     p->mask |= types;
     n->sources.push_back(s);
     n->dirty = true;
@@ -1555,6 +1622,15 @@ int edlut_network_reset(edlut_network *n) {
     HIP_TRY(hipMemsetAsync(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
     HIP_TRY(hipMemsetAsync(n->log_count, 0, 2 * sizeof(int64_t), p->stream));
     HIP_TRY(hipMemsetAsync(p->err, 0, sizeof(uint32_t), p->stream));
+    HIP_TRY(hipMemsetAsync(n->d_time, 0, sizeof(double) * n->ring, p->stream));
+    for (Source &s : n->sources) {
+        if (!s.stdp) continue;
+        if (s.w0.size() == (size_t)s.nsyn && s.nsyn)
+            HIP_TRY(hipMemcpy(s.w, s.w0.data(), sizeof(float) * s.nsyn, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemsetAsync(s.syn.pre, 0, sizeof(float) * s.nsyn, p->stream));
+        HIP_TRY(hipMemsetAsync(s.syn.post, 0, sizeof(float) * s.nsyn, p->stream));
+        HIP_TRY(hipMemsetAsync(s.syn.last, 0, sizeof(double) * s.nsyn, p->stream));
+    }
     HIP_TRY(hipMemsetAsync(p->cnt, 0, sizeof(uint32_t) * 3 * p->stride, p->stream));
     HIP_TRY(hipMemsetAsync(p->fill, 0, sizeof(uint32_t) * 3 * p->stride, p->stream));
     HIP_TRY(hipMemsetAsync(p->seg, 0xff, sizeof(uint64_t) * 3 * p->stride, p->stream));
@@ -1638,6 +1714,71 @@ done:
     if (fold_lanes) *fold_lanes = b_l;
     if (fold_blocks) *fold_blocks = b_b;
     if (ms_per_step) *ms_per_step = best / trial_steps;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Additive STDP on every synapse of one source (STDPWeightChange): times in
+// seconds, weights clamped to [0, max_weight].
+int edlut_network_set_stdp(edlut_network *n, int source, float max_ltp, float tau_ltp, float max_ltd,
+                           float tau_ltd, float max_weight) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    // STDPWeightChange::SetParameters (STDPWeightChange.cpp:107-160)
+    if (!(tau_ltp > 0.0f) || !(tau_ltd > 0.0f)) return fail(EDLUT_E_ARG, "STDP time constants must be positive");
+    if (!std::isfinite(max_ltp) || !std::isfinite(max_ltd) || !std::isfinite(max_weight) || max_weight < 0.0f)
+        return fail(EDLUT_E_ARG, "bad STDP amplitudes or maximum weight");
+    Source &s = n->sources[(size_t)source];
+    if (s.stdp) return fail(EDLUT_E_STATE, "source %d already has STDP", source);
+    edlut_population *p = n->pop;
+    HIP_TRY(hipSetDevice(p->device));
+    HIP_TRY(hipStreamSynchronize(p->stream));
+    // targets of every synapse -> per-target list of synapse positions
+    std::vector<uint32_t> tt((size_t)s.nsyn);
+    s.w0.resize((size_t)s.nsyn);
+    if (s.nsyn) {
+        HIP_TRY(hipMemcpy(tt.data(), s.tgt_type, sizeof(uint32_t) * s.nsyn, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(s.w0.data(), s.w, sizeof(float) * s.nsyn, hipMemcpyDeviceToHost));
+    }
+    std::vector<int64_t> ptr((size_t)p->n + 1, 0), syn((size_t)s.nsyn);
+    for (int64_t q = 0; q < s.nsyn; q++) ptr[(tt[(size_t)q] & 0x3fffffffu) + 1]++;
+    for (int i = 0; i < p->n; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
+    std::vector<int64_t> fillp(ptr.begin(), ptr.end() - 1);
+    for (int64_t q = 0; q < s.nsyn; q++) syn[(size_t)fillp[tt[(size_t)q] & 0x3fffffffu]++] = q;
+    int rc;
+    const size_t m = s.nsyn > 0 ? (size_t)s.nsyn : 1;
+    if ((rc = dalloc(&s.in_ptr, ptr.size())) || (rc = dalloc(&s.in_syn, m)) || (rc = dalloc(&s.syn.pre, m)) ||
+        (rc = dalloc(&s.syn.post, m)) || (rc = dalloc(&s.syn.last, m)))
+        return rc;
+    HIP_TRY(hipMemcpy(s.in_ptr, ptr.data(), sizeof(int64_t) * ptr.size(), hipMemcpyHostToDevice));
+    if (s.nsyn) HIP_TRY(hipMemcpy(s.in_syn, syn.data(), sizeof(int64_t) * s.nsyn, hipMemcpyHostToDevice));
+    // ConnectionState.cpp:22: traces and last update times start at zero
+    HIP_TRY(hipMemset(s.syn.pre, 0, sizeof(float) * m));
+    HIP_TRY(hipMemset(s.syn.post, 0, sizeof(float) * m));
+    HIP_TRY(hipMemset(s.syn.last, 0, sizeof(double) * m));
+    s.syn.a_plus = max_ltp;
+    s.syn.a_minus = max_ltd;
+    // STDPState.cpp:24-27
+    s.syn.inv_tau_plus = 1.0f / tau_ltp;
+    s.syn.inv_tau_minus = 1.0f / tau_ltd;
+    s.syn.w_max = max_weight;
+    s.stdp = true;
+    n->plastic = true;
+    n->dirty = true;
+    return 0;
+}
+
+int edlut_network_get_stdp(edlut_network *n, int source, float *pre, float *post, double *last) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    const Source &s = n->sources[(size_t)source];
+    if (!s.stdp) return fail(EDLUT_E_STATE, "source %d has no STDP", source);
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    if (s.nsyn == 0) return 0;
+    if (pre) HIP_TRY(hipMemcpy(pre, s.syn.pre, sizeof(float) * s.nsyn, hipMemcpyDeviceToHost));
+    if (post) HIP_TRY(hipMemcpy(post, s.syn.post, sizeof(float) * s.nsyn, hipMemcpyDeviceToHost));
+    if (last) HIP_TRY(hipMemcpy(last, s.syn.last, sizeof(double) * s.nsyn, hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -460,13 +460,27 @@ __global__ __launch_bounds__(kBlock) void k_fold_push(FoldTarget t, PushEvents e
 // Events generated on the device from the spike rings of the sources.
 constexpr int kMaxSources = 4;
 constexpr int kMaxDelay = 256;
+// Additive STDP on the synapses of a source (STDPWeightChange / STDPState):
+// per synapse a presynaptic and a postsynaptic trace and the time of its
+// last update; weights change in place.
+struct StdpSyn {
+    float *pre;
+    float *post;
+    double *last;
+    float a_plus, a_minus;           // max_LTP, max_LTD
+    float inv_tau_plus, inv_tau_minus;
+    float w_max;
+};
+
 struct RingSource {
     const int32_t *ring_idx;   // [ring][n_src]
     const int32_t *ring_cnt;   // [ring]
     const int64_t *row_ptr;    // [n_src*max_delay+1]
     const uint32_t *tgt_type;  // target | type << 30
-    const float *w;
+    float *w;
     int64_t n_src;
+    int stdp;                  // plastic synapses (StdpSyn below)
+    StdpSyn syn;
 };
 struct RingEvents {
     RingSource src[kMaxSources];
@@ -474,7 +488,27 @@ struct RingEvents {
     int max_delay;
     int ring;
     StepRef step;              // update index about to run
+    // STDP only: update time of each ring slot, delay of d steps in seconds
+    const double *time_ring;   // [ring]
+    const double *delay_s;     // [max_delay + 1]
+    const float *exp_lut;
 };
+
+// STDPState::SetNewUpdateTime (STDPState.cpp:74-82): both traces decay by
+// ExponentialTable factors of the float elapsed time.
+__device__ __forceinline__ void stdp_decay(const StdpSyn &s, const float *exp_lut, int64_t q, double t) {
+    const float elapsed = (float)(t - s.last[q]);
+    s.pre[q] = s.pre[q] * lut_exp(exp_lut, -elapsed * s.inv_tau_plus);
+    s.post[q] = s.post[q] * lut_exp(exp_lut, -elapsed * s.inv_tau_minus);
+    s.last[q] = t;
+}
+// Interconnection::IncrementWeight (Interconnection.h:400-409)
+__device__ __forceinline__ void stdp_increment(const StdpSyn &s, float *w, int64_t q, float inc) {
+    float x = w[q] + inc;
+    if (x > s.w_max) x = s.w_max;
+    else if (x < 0.0f) x = 0.0f;
+    w[q] = x;
+}
 
 // Work item = one spike of one source at emission step k-1-d; a group of
 // LANES lanes walks its synapses of delay d.  Items are enumerated in
@@ -528,12 +562,75 @@ __global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r
         const int64_t b = src.row_ptr[(int64_t)id * r.max_delay + d - 1];
         const int64_t e = src.row_ptr[(int64_t)id * r.max_delay + d];
         const uint64_t ord_hi = ((uint64_t)((d - 1) * r.n_sources + p)) << 40;
+        double t_arr = 0.0;
+        if (PASS == kScatter && src.stdp) t_arr = r.time_ring[s_slot[g]] + r.delay_s[d];
         for (int64_t q = b + lane; q < e; q += LANES) {
             const uint32_t tt = src.tgt_type[q];
             const uint64_t key =
                 (uint64_t)(tt >> 30) * (uint64_t)t.stride + (uint64_t)(tt & 0x3fffffffu);
             fold_event<PASS>(t, par, key, ord_hi | (uint64_t)q, src.w[q]);
+            if (PASS == kScatter && src.stdp) {
+                // STDPWeightChange::ApplyPreSynapticSpike (STDPWeightChange.cpp:43-55),
+                // after the event used the weight (PropagatedSpikeGroup.cpp:76-97).
+                // A synapse carries at most one event per step: no other lane
+                // touches q.
+                stdp_decay(src.syn, r.exp_lut, q, t_arr);
+                src.syn.pre[q] = src.syn.pre[q] + 1.0f;
+                stdp_increment(src.syn, src.w, q, -src.syn.a_minus * src.syn.post[q]);
+            }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Post-synaptic STDP (STDPWeightChange::ApplyPostSynapticSpike,
+// STDPWeightChange.cpp:57-76, called for every neuron that fired at update k,
+// TimeDrivenInternalSpike.cpp:94-103): every plastic input synapse of a
+// spiking neuron decays to the update time, gains a postsynaptic trace unit
+// and potentiates by max_LTP times its presynaptic trace.  LANES lanes walk
+// one spiking neuron's inputs (in_syn lists synapse positions by target).
+// ---------------------------------------------------------------------------
+struct StdpPostArgs {
+    const int32_t *spk_idx;    // own spike ring, slot k % ring
+    const int32_t *spk_cnt;
+    int64_t slot_len;
+    int ring;
+    StepRef step;
+    const int64_t *in_ptr;     // [n_tgt + 1]
+    const int64_t *in_syn;     // synapse positions grouped by target
+    float *w;
+    StdpSyn syn;
+    const double *time_ring;
+    const float *exp_lut;
+};
+
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void k_stdp_post(StdpPostArgs a) {
+    const int32_t step = a.step.get();
+    const int slot = step % a.ring;
+    const int32_t n = a.spk_cnt[slot];
+    const int32_t *ids = a.spk_idx + (int64_t)slot * a.slot_len;
+    const double tk = a.time_ring[slot];
+    const int lane = threadIdx.x % LANES;
+    const int64_t group = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / LANES;
+    const int64_t n_groups = (int64_t)gridDim.x * kBlock / LANES;
+    for (int64_t it = group; it < n; it += n_groups) {
+        const int32_t i = ids[it];
+        for (int64_t x = a.in_ptr[i] + lane; x < a.in_ptr[i + 1]; x += LANES) {
+            const int64_t q = a.in_syn[x];
+            stdp_decay(a.syn, a.exp_lut, q, tk);
+            a.syn.post[q] = a.syn.post[q] + 1.0f;
+            stdp_increment(a.syn, a.w, q, a.syn.a_plus * a.syn.pre[q]);
+        }
+    }
+}
+
+// Update time of step k: t_k = t_{k-1} + double(float dt), kept per ring slot
+// (TimeEventUpdateNeurons.cpp:78).  First node of a step of a plastic network.
+__global__ void k_tick(StepRef step, double *time_ring, int ring, double dt) {
+    if (threadIdx.x == 0) {
+        const int32_t k = step.get();
+        time_ring[k % ring] = time_ring[(k - 1) % ring] + dt;
     }
 }
 

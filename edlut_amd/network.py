@@ -114,6 +114,23 @@ class Network:
                                                    C.byref(rows), C.byref(nsyn)))
         return rp, tt[: nsyn.value], w[: nsyn.value]
 
+    def set_stdp(self, source: int, max_ltp: float, tau_ltp: float, max_ltd: float, tau_ltd: float,
+                 max_weight: float):
+        """Additive STDP (STDPWeightChange parameters max_LTP, tau_LTP, max_LTD,
+        tau_LTD, in seconds) on every synapse of a source."""
+        C.check(C.lib().edlut_network_set_stdp(self._h, int(source), float(max_ltp), float(tau_ltp),
+                                               float(max_ltd), float(tau_ltd), float(max_weight)))
+
+    def stdp_state(self, source: int):
+        """(pre, post, last) of a plastic source's synapses, in row order."""
+        _, tt, _ = self.synapses(source)
+        m = len(tt)
+        pre = np.zeros(max(m, 1), np.float32)
+        post = np.zeros(max(m, 1), np.float32)
+        last = np.zeros(max(m, 1), np.float64)
+        C.check(C.lib().edlut_network_get_stdp(self._h, int(source), C.ptr(pre), C.ptr(post), C.ptr(last)))
+        return pre[:m], post[:m], last[:m]
+
     def reset(self):
         """Back to the initial state; the next run starts at update 1."""
         C.check(C.lib().edlut_network_reset(self._h))

@@ -215,9 +215,28 @@ int edlut_network_generate_recurrent(edlut_network *net, int in_degree, float ex
 int edlut_network_get_synapses(edlut_network *net, int source, int64_t *row_ptr,
                                uint32_t *tgt_type, float *weight, int64_t *n_rows,
                                int64_t *n_syn);
+/*
+ * Additive STDP on every synapse of a source (source index = order of the
+ * add/generate calls), replacing STDPWeightChange / STDPState
+ * (STDPWeightChange.cpp:43-76, STDPState.cpp:74-97) for device-resident
+ * synapses.  A delivered spike first adds the synapse's current weight to
+ * the target, then both traces decay by ExponentialTable factors to its
+ * arrival time, pre += 1 and w -= max_ltd * post; a target's spike at
+ * update k decays the traces of its plastic inputs to t_k, post += 1 and
+ * w += max_ltp * pre.  Weights stay in [0, max_weight]
+ * (Interconnection.h:400-409).  Times in seconds: update k at
+ * t_k = t_{k-1} + double(float dt), arrival = t_s + the delay rounded by
+ * Network::RoundPropagationDelay.  Weights are read back with
+ * edlut_network_get_synapses.
+ */
+int edlut_network_set_stdp(edlut_network *net, int source, float max_ltp, float tau_ltp,
+                           float max_ltd, float tau_ltd, float max_weight);
+/* Traces and last update times of a plastic source's synapses (row order). */
+int edlut_network_get_stdp(edlut_network *net, int source, float *pre, float *post, double *last);
 /* Back to the initial state: neuron state as after construction, Poisson
- * generators at their initial states, empty spike rings and log; the next
- * run starts at update 1. */
+ * generators at their initial states, plastic weights and traces as when
+ * STDP was switched on, empty spike rings and log; the next run starts at
+ * update 1. */
 int edlut_network_reset(edlut_network *net);
 /* Launch shape: neurons per thread of the update kernel (4; 2 or 4 for a
  * grouped population), lanes per spike and workgroups of the ring fold

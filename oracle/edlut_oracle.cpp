@@ -790,7 +790,50 @@ struct Source {
     float rate = 0.0f;
     /* spikes per step: ring indexed by step % ring */
     std::vector<std::vector<int32_t>> ring;
+    /* additive STDP on every synapse of the source (STDPWeightChange,
+     * STDPState): traces, last update time, and per target the positions of
+     * its plastic input synapses */
+    bool stdp = false;
+    float a_plus = 0, inv_tau_plus = 0, a_minus = 0, inv_tau_minus = 0, w_max = 0;
+    std::vector<float> pre, post;
+    std::vector<double> last;
+    std::vector<std::vector<int64_t>> in_syn;
 };
+
+/* STDPState::SetNewUpdateTime (STDPState.cpp:74-82): both traces decay by
+ * ExponentialTable::GetResult(-elapsed / tau) with elapsed = float(new - last). */
+inline void stdp_decay(Source &s, int64_t q, double t) {
+    const float elapsed = (float)(t - s.last[q]);
+    s.pre[q] *= exp_lut(-elapsed * s.inv_tau_plus);
+    s.post[q] *= exp_lut(-elapsed * s.inv_tau_minus);
+    s.last[q] = t;
+}
+/* Interconnection::IncrementWeight (Interconnection.h:400-409) */
+inline void stdp_increment(Source &s, int64_t q, float inc) {
+    s.w[q] += inc;
+    if (s.w[q] > s.w_max) s.w[q] = s.w_max;
+    else if (s.w[q] < 0.0f) s.w[q] = 0.0f;
+}
+/* STDPWeightChange::ApplyPreSynapticSpike (STDPWeightChange.cpp:43-55) */
+inline void stdp_pre(Source &s, int64_t q, double t) {
+    stdp_decay(s, q, t);
+    s.pre[q] += 1.0f;
+    stdp_increment(s, q, -s.a_minus * s.post[q]);
+}
+/* STDPWeightChange::ApplyPostSynapticSpike (STDPWeightChange.cpp:57-76) */
+inline void stdp_post(Source &s, int64_t q, double t) {
+    stdp_decay(s, q, t);
+    s.post[q] += 1.0f;
+    stdp_increment(s, q, s.a_plus * s.pre[q]);
+}
+
+/* Network::RoundPropagationDelay (Network.cpp:1311-1317) of a delay of d
+ * steps, minpropagationdelay = 0.0001 s. */
+inline double delay_seconds(int d, float dt_s) {
+    const double m = 0.0001;
+    double r = std::floor((double)d * (double)dt_s * (1.0 / 0.0001) + m * 0.5) * m;
+    return r <= 0 ? m : r;
+}
 
 struct TraceRec {
     int32_t step;
@@ -811,6 +854,14 @@ struct Net {
     float w_ext = 0.0f;
     uint64_t ext_seed = 0;
     bool has_ext = false;
+    /* update times: t[k] = t[k-1] + double(float dt), t[0] = 0
+     * (TimeEventUpdateNeurons.cpp:78, Simulation.cpp:228-268) */
+    std::vector<double> t;
+    double time_of(int64_t k) {
+        if (t.empty()) t.push_back(0.0);
+        while ((int64_t)t.size() <= k) t.push_back(t.back() + (double)pop->dt_s());
+        return t[(size_t)k];
+    }
 };
 
 /* Counter-based external drive (the synthetic workloads' Poisson drive; not a
@@ -977,6 +1028,40 @@ void orc_net_set_external(void *net, int n_thr, const uint32_t *thresholds, floa
     n->has_ext = n_thr > 0;
 }
 
+int orc_net_set_stdp(void *net, int source, float max_ltp, float tau_ltp, float max_ltd,
+                     float tau_ltd, float max_weight) {
+    Net *n = (Net *)net;
+    if (source < 0 || source >= (int)n->sources.size() || !(tau_ltp > 0) || !(tau_ltd > 0)) return -1;
+    Source &s = n->sources[(size_t)source];
+    s.stdp = true;
+    s.a_plus = max_ltp;
+    s.a_minus = max_ltd;
+    /* STDPState.cpp:24-27 */
+    s.inv_tau_plus = 1.0f / tau_ltp;
+    s.inv_tau_minus = 1.0f / tau_ltd;
+    s.w_max = max_weight;
+    const size_t nsyn = s.w.size();
+    s.pre.assign(nsyn, 0.0f);
+    s.post.assign(nsyn, 0.0f);
+    s.last.assign(nsyn, 0.0);   /* ConnectionState.cpp:22 */
+    s.in_syn.assign((size_t)n->pop->size(), std::vector<int64_t>());
+    for (size_t q = 0; q < nsyn; q++) s.in_syn[s.tgt_type[q] & 0x3fffffffu].push_back((int64_t)q);
+    return 0;
+}
+
+int orc_net_get_synapses(void *net, int source, float *w, float *pre, float *post, double *last) {
+    Net *n = (Net *)net;
+    if (source < 0 || source >= (int)n->sources.size()) return -1;
+    Source &s = n->sources[(size_t)source];
+    for (size_t q = 0; q < s.w.size(); q++) {
+        if (w) w[q] = s.w[q];
+        if (pre) pre[q] = s.stdp ? s.pre[q] : 0.0f;
+        if (post) post[q] = s.stdp ? s.post[q] : 0.0f;
+        if (last) last[q] = s.stdp ? s.last[q] : 0.0;
+    }
+    return (int)s.w.size();
+}
+
 void orc_net_set_trace(void *net, int on) {
     Net *n = (Net *)net;
     n->trace = on != 0;
@@ -1020,6 +1105,9 @@ int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_s
                         if (n->trace)
                             n->trace_buf.push_back({(int32_t)k, (int32_t)(tt & 0x3fffffffu),
                                                     (uint8_t)(tt >> 30), src.w[q]});
+                        /* PropagatedSpikeGroup.cpp:76-97: the learning rule runs
+                         * after ProcessInputSpike, at the arrival time */
+                        if (src.stdp) stdp_pre(src, q, n->time_of(s) + delay_seconds(d, pop->dt_s()));
                     }
                 }
             }
@@ -1054,6 +1142,14 @@ int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_s
         for (Source &src : n->sources) {
             if (!src.self) continue;
             src.ring[k % n->ring].assign(spk.begin(), spk.begin() + ns);
+        }
+        /* post-synaptic learning of the neurons that fired, at the update
+         * time (TimeDrivenInternalSpike.cpp:94-103) */
+        for (Source &src : n->sources) {
+            if (!src.stdp) continue;
+            const double tk = n->time_of(k);
+            for (int q = 0; q < ns; q++)
+                for (int64_t syn : src.in_syn[(size_t)spk[q]]) stdp_post(src, syn, tk);
         }
         for (int q = 0; q < ns; q++) {
             if (total < cap) {

@@ -85,6 +85,14 @@ void orc_net_set_external(void *net, int n_thr, const uint32_t *thresholds, floa
 int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_step,
                     int32_t *spike_idx, int64_t cap);
 
+/* Additive STDP (STDPWeightChange) on every synapse of a source (index in
+ * the order sources were added): traces decay by ExponentialTable factors,
+ * weights clamped to [0, max_weight].  Times in seconds. */
+int orc_net_set_stdp(void *net, int source, float max_ltp, float tau_ltp, float max_ltd,
+                     float tau_ltd, float max_weight);
+/* Current weights and STDP state of a source's synapses (row order). */
+int orc_net_get_synapses(void *net, int source, float *w, float *pre, float *post, double *last);
+
 /* Record every delivered recurrent/device event (step, target, type, w) in
  * delivery order, for the ordering tests. */
 void orc_net_set_trace(void *net, int on);

@@ -66,6 +66,8 @@ def lib():
         l.orc_net_add_source_poisson.argtypes = [P, ctypes.c_int, P, ctypes.c_float, P, P, P]
         l.orc_net_set_external.argtypes = [P, ctypes.c_int, P, ctypes.c_float, ctypes.c_uint64]
         l.orc_net_set_trace.argtypes = [P, ctypes.c_int]
+        l.orc_net_set_stdp.argtypes = [P, ctypes.c_int] + [ctypes.c_float] * 5
+        l.orc_net_get_synapses.argtypes = [P, ctypes.c_int, P, P, P, P]
         l.orc_net_get_trace.restype = ctypes.c_int64
         l.orc_net_get_trace.argtypes = [P, P, P, P, P, ctypes.c_int64]
         l.orc_net_run.restype = ctypes.c_int64
@@ -187,6 +189,20 @@ class Net:
     def set_external(self, thr, w_ext, seed):
         t = np.ascontiguousarray(thr, np.uint32)
         lib().orc_net_set_external(self.h, len(t), _p(t), float(w_ext), int(seed))
+
+    def set_stdp(self, source: int, max_ltp, tau_ltp, max_ltd, tau_ltd, max_weight):
+        if lib().orc_net_set_stdp(self.h, source, max_ltp, tau_ltp, max_ltd, tau_ltd, max_weight) != 0:
+            raise ValueError("bad STDP arguments")
+
+    def synapses(self, source: int):
+        """(w, pre, post, last) of every synapse of a source, in row order."""
+        m = lib().orc_net_get_synapses(self.h, source, None, None, None, None)
+        w = np.zeros(m, np.float32)
+        pre = np.zeros(m, np.float32)
+        post = np.zeros(m, np.float32)
+        last = np.zeros(m, np.float64)
+        lib().orc_net_get_synapses(self.h, source, _p(w), _p(pre), _p(post), _p(last))
+        return w, pre, post, last
 
     def set_trace(self, on: bool):
         lib().orc_net_set_trace(self.h, int(on))
