@@ -52,6 +52,11 @@ EXPORTS = (
     "edlut_network_set_launch",
     "edlut_network_get_launch",
     "edlut_network_autotune",
+    "edlut_fit_create",
+    "edlut_fit_destroy",
+    "edlut_fit_forward",
+    "edlut_fit_backward",
+    "edlut_refractory_table",
     "edlut_device_arch",
     "edlut_drive_hash",
     "edlut_build_out_csr",
@@ -122,6 +127,11 @@ def _sig(lib):
         "edlut_network_set_launch": (I32, [P, I32, I32, I32]),
         "edlut_network_get_launch": (I32, [P, P, P, P]),
         "edlut_network_autotune": (I32, [P, I32, P, P, P, P]),
+        "edlut_fit_create": (I32, [I32, P, I32, F32, I32, I32, I32, I32, I32, P]),
+        "edlut_fit_destroy": (I32, [P]),
+        "edlut_fit_forward": (I32, [P, P, P]),
+        "edlut_fit_backward": (I32, [P, P, P]),
+        "edlut_refractory_table": (I32, [F32, F32, P, I32, P]),
         "edlut_build_out_csr": (I32, [I64, I64, I32, I64, P, P, P, P, P, P, P, P]),
     }
     for name, (res, args) in s.items():

@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/edlut_amd.h"
+#include "adjoint.hpp"
 #include "generate.hpp"
 #include "kernels.hpp"
 
@@ -1779,6 +1780,169 @@ int edlut_network_get_stdp(edlut_network *n, int source, float *pre, float *post
     if (pre) HIP_TRY(hipMemcpy(pre, s.syn.pre, sizeof(float) * s.nsyn, hipMemcpyDeviceToHost));
     if (post) HIP_TRY(hipMemcpy(post, s.syn.post, sizeof(float) * s.nsyn, hipMemcpyDeviceToHost));
     if (last) HIP_TRY(hipMemcpy(last, s.syn.last, sizeof(double) * s.nsyn, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Trace fitting: forward simulation and discrete adjoint (adjoint.hpp).
+}  // extern "C"
+
+struct edlut_fit {
+    int model = 0, n = 0, steps = 0, n_syn = 0, ckpt = 100, device = 0;
+    FitArgs base{};
+    float *refr = nullptr;
+    float *ck_state = nullptr;
+    int32_t *ck_int = nullptr;
+    float *scratch = nullptr;
+    int32_t *scratch_int = nullptr;
+    bool forward_done = false;
+};
+
+extern "C" {
+
+int edlut_refractory_table(float dt_s, float tau_ref, float *out, int cap, int *len) {
+    if (!len || !(dt_s > 0.0f)) return fail(EDLUT_E_ARG, "bad argument");
+    std::vector<float> r = build_refractory(dt_s, dt_s * 1000.0f, tau_ref);
+    *len = (int)r.size();
+    if (out) {
+        if (cap < (int)r.size()) return fail(EDLUT_E_CAP, "refractory table has %d entries", (int)r.size());
+        std::memcpy(out, r.data(), sizeof(float) * r.size());
+    }
+    return 0;
+}
+
+int edlut_fit_create(int model, const float *params, int n_params, float dt_s, int n_neurons, int steps,
+                     int n_syn, int checkpoint, int device, edlut_fit **out) {
+    if (!out) return fail(EDLUT_E_ARG, "out is null");
+    *out = nullptr;
+    if (model != EDLUT_LIF && model != EDLUT_ADEX)
+        return fail(EDLUT_E_ARG, "trace fitting takes LIF (Euler) or AdEx (RK4)");
+    const int need = model == EDLUT_ADEX ? 14 : 10;
+    if (!params || n_params != need) return fail(EDLUT_E_ARG, "model takes %d parameters", need);
+    if (!(dt_s > 0.0f) || n_neurons <= 0 || steps <= 0 || n_syn <= 0 || checkpoint <= 0)
+        return fail(EDLUT_E_ARG, "sizes must be positive");
+    for (int i = 0; i < n_params; i++)
+        if (!std::isfinite(params[i])) return fail(EDLUT_E_ARG, "parameter %d is not finite", i);
+    HIP_TRY(hipSetDevice(device));
+    edlut_fit *f = new edlut_fit;
+    f->model = model;
+    f->n = n_neurons;
+    f->steps = steps;
+    f->n_syn = n_syn;
+    f->ckpt = checkpoint < steps ? checkpoint : steps;
+    f->device = device;
+    FitArgs &a = f->base;
+    std::memset(&a, 0, sizeof(a));
+    a.model = model == EDLUT_LIF ? 0 : 1;
+    a.n = n_neurons;
+    a.steps = steps;
+    a.n_syn = n_syn;
+    a.ckpt = f->ckpt;
+    a.dt = dt_s * 1000.0f;
+    a.h = model == EDLUT_ADEX ? a.dt * 0.5f : a.dt;
+    a.s6 = a.dt * 0.166666666667f;
+    std::vector<float> refr;
+    if (model == EDLUT_LIF) {
+        // e_exc e_inh e_leak v_thr c_m tau_exc tau_inh tau_ref g_leak tau_nmda
+        a.e_exc = params[0];
+        a.e_inh = params[1];
+        a.v_thr = params[3];
+        a.inv_c_m = inv_of(params[4]);
+        refr = build_refractory(dt_s, dt_s * 1000.0f, params[7]);
+    } else {
+        // a b thr_slo_fac v_thr tau_w e_exc e_inh e_reset e_leak g_leak c_m tau_exc tau_inh tau_nmda
+        a.a = params[0];
+        a.b = params[1];
+        a.thr = params[2];
+        a.inv_thr = inv_of(params[2]);
+        a.v_thr = params[3];
+        a.inv_tau_w = inv_of(params[4]);
+        a.e_exc = params[5];
+        a.e_inh = params[6];
+        a.e_reset = params[7];
+        a.inv_c_m = inv_of(params[10]);
+        a.v_cap = params[3] + 6 * params[2];
+    }
+    a.refr_len = (int)refr.size();
+    const int segs = (steps + f->ckpt - 1) / f->ckpt;
+    int rc;
+    if ((rc = dalloc(&f->refr, refr.size() + 1)) || (rc = dalloc(&f->ck_state, (size_t)(segs + 1) * 4 * n_neurons)) ||
+        (rc = dalloc(&f->ck_int, (size_t)(segs + 1) * 2 * n_neurons)) ||
+        (rc = dalloc(&f->scratch, (size_t)f->ckpt * kFitStore * n_neurons)) ||
+        (rc = dalloc(&f->scratch_int, (size_t)f->ckpt * n_neurons))) {
+        edlut_fit_destroy(f);
+        return rc;
+    }
+    if (!refr.empty() &&
+        hipMemcpy(f->refr, refr.data(), sizeof(float) * refr.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        edlut_fit_destroy(f);
+        return fail(EDLUT_E_HIP, "hipMemcpy failed");
+    }
+    a.refr = f->refr;
+    a.ck_state = f->ck_state;
+    a.ck_int = f->ck_int;
+    a.scratch = f->scratch;
+    a.scratch_int = f->scratch_int;
+    *out = f;
+    return 0;
+}
+
+int edlut_fit_destroy(edlut_fit *f) {
+    if (!f) return 0;
+    (void)hipSetDevice(f->device);
+    (void)hipDeviceSynchronize();
+    void *dev[] = {f->refr, f->ck_state, f->ck_int, f->scratch, f->scratch_int};
+    for (void *q : dev)
+        if (q) (void)hipFree(q);
+    delete f;
+    return 0;
+}
+
+static int fit_args(edlut_fit *f, const edlut_fit_tensors *t, FitArgs &a) {
+    if (!f || !t) return fail(EDLUT_E_ARG, "null argument");
+    if (!t->g_leak || !t->e_leak || !t->tau_exc || !t->tau_inh || !t->weight || !t->syn_type || !t->ev_ptr ||
+        !t->ev_step || !t->ev_syn)
+        return fail(EDLUT_E_ARG, "null tensor");
+    a = f->base;
+    a.gl = t->g_leak;
+    a.el = t->e_leak;
+    a.te = t->tau_exc;
+    a.ti = t->tau_inh;
+    a.w = t->weight;
+    a.syn_type = t->syn_type;
+    a.ev_ptr = t->ev_ptr;
+    a.ev_step = t->ev_step;
+    a.ev_syn = t->ev_syn;
+    a.v_trace = t->v_trace;
+    a.grad_v = t->grad_v;
+    a.grad_par = t->grad_par;
+    a.grad_w = t->grad_w;
+    return 0;
+}
+
+int edlut_fit_forward(edlut_fit *f, const edlut_fit_tensors *t, void *stream) {
+    FitArgs a;
+    int rc = fit_args(f, t, a);
+    if (rc) return rc;
+    if (!t->v_trace) return fail(EDLUT_E_ARG, "v_trace is null");
+    HIP_TRY(hipSetDevice(f->device));
+    const int blocks = (f->n + kFitBlock - 1) / kFitBlock;
+    hipLaunchKernelGGL(k_fit_forward, dim3(blocks), dim3(kFitBlock), 0, (hipStream_t)stream, a);
+    HIP_TRY(hipGetLastError());
+    f->forward_done = true;
+    return 0;
+}
+
+int edlut_fit_backward(edlut_fit *f, const edlut_fit_tensors *t, void *stream) {
+    FitArgs a;
+    int rc = fit_args(f, t, a);
+    if (rc) return rc;
+    if (!t->grad_v || !t->grad_par || !t->grad_w) return fail(EDLUT_E_ARG, "gradient tensors are null");
+    if (!f->forward_done) return fail(EDLUT_E_STATE, "the backward pass needs the forward pass's checkpoints");
+    HIP_TRY(hipSetDevice(f->device));
+    const int blocks = (f->n + kFitBlock - 1) / kFitBlock;
+    hipLaunchKernelGGL(k_fit_backward, dim3(blocks), dim3(kFitBlock), 0, (hipStream_t)stream, a);
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

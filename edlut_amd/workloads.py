@@ -15,6 +15,11 @@ the same inputs can be fed to the HIP path and to any CPU implementation.
                    E_leak, a, b, tau_w drawn uniformly within +-30% of the
                    defaults; in-degree 100 wiring and the external drive are
                    generated per network from its own seed.
+  adjoint_fit      trace fitting: independent LIF (Euler) or AdEx (RK4)
+                   neurons, 50 input synapses each (40 exc, 10 inh) driven by
+                   fixed seeded Poisson trains at 20 Hz; per-neuron g_leak,
+                   e_leak, tau_exc, tau_inh and per-synapse weights, with a
+                   target parameter set perturbed by up to +-20%.
 """
 from __future__ import annotations
 
@@ -137,3 +142,61 @@ def adex_sweep(n_networks: int = 4096, median: float = 5000.0, sigma: float = 0.
     thr = binomial_thresholds(100, 5.0 * float(DT_S))
     return SweepSpec(sizes=sizes, params=params, ext=(thr, np.float32(w_ext), seed * 1000003 + 29),
                      wiring_seed=seed * 7777 + 3, steps=steps)
+
+
+@dataclass
+class FitSpec:
+    model: str
+    params: np.ndarray          # model parameter vector (GetDefaultParameters order)
+    n: int
+    steps: int
+    syn_type: np.ndarray        # [n, n_syn] uint8
+    events: tuple               # (ev_ptr [n+1], ev_step, ev_syn) int32
+    start: dict                 # g_leak, e_leak, tau_exc, tau_inh [n], weight [n, n_syn]
+    target: dict                # the perturbed set the target trace comes from
+
+
+FIT_SCALE = {"LIFTimeDrivenModel_GPU": (0.03, 0.06), "AdExTimeDrivenModel_GPU": (2.0, 4.0)}
+
+
+def adjoint_fit(model: str, n: int = 256, n_syn: int = 50, n_exc: int = 40, rate: float = 20.0,
+                steps: int = 10000, perturb: float = 0.2, seed: int = 5) -> FitSpec:
+    from .models import REGISTRY
+
+    cls = REGISTRY[model]
+    names = [k for k, _ in cls.PARAMS]
+    params = np.array([v for _, v in cls.PARAMS], np.float32)
+    rng = np.random.default_rng(seed)
+    syn_type = np.zeros((n, n_syn), np.uint8)
+    syn_type[:, n_exc:] = 1
+    # fixed Poisson trains: each synapse fires before update k with probability rate*dt
+    p = rate * float(DT_S)
+    ev_neuron, ev_step, ev_syn = [], [], []
+    chunk = 1000
+    for k0 in range(0, steps, chunk):
+        m = min(chunk, steps - k0)
+        hit = rng.random((m, n, n_syn)) < p
+        kk, ii, ss = np.nonzero(hit)
+        ev_step.append(kk + k0 + 1)
+        ev_neuron.append(ii)
+        ev_syn.append(ss)
+    ev_neuron = np.concatenate(ev_neuron)
+    ev_step = np.concatenate(ev_step)
+    ev_syn = np.concatenate(ev_syn)
+    order = np.lexsort((ev_syn, ev_step, ev_neuron))
+    ev_neuron, ev_step, ev_syn = ev_neuron[order], ev_step[order], ev_syn[order]
+    ev_ptr = np.zeros(n + 1, np.int64)
+    np.add.at(ev_ptr, ev_neuron + 1, 1)
+    ev_ptr = np.cumsum(ev_ptr)
+    we, wi = FIT_SCALE[model]
+    weight = np.where(syn_type == 0, rng.uniform(0, we, (n, n_syn)), rng.uniform(0, wi, (n, n_syn)))
+    start = {
+        "g_leak": np.full(n, params[names.index("g_leak")], np.float32),
+        "e_leak": np.full(n, params[names.index("e_leak")], np.float32),
+        "tau_exc": np.full(n, params[names.index("tau_exc")], np.float32),
+        "tau_inh": np.full(n, params[names.index("tau_inh")], np.float32),
+        "weight": weight.astype(np.float32),
+    }
+    target = {k: (v * rng.uniform(1 - perturb, 1 + perturb, v.shape)).astype(np.float32) for k, v in start.items()}
+    return FitSpec(model, params, n, steps, syn_type,
+                   (ev_ptr.astype(np.int32), ev_step.astype(np.int32), ev_syn.astype(np.int32)), start, target)

@@ -250,6 +250,46 @@ int edlut_network_get_launch(const edlut_network *net, int *items, int *fold_lan
 int edlut_network_autotune(edlut_network *net, int trial_steps, int *items, int *fold_lanes,
                            int *fold_blocks, double *ms_per_step);
 
+/* ---- trace fitting: forward simulation and discrete adjoint ------------- */
+/*
+ * No counterpart in the reference (it has no gradient path).  n independent
+ * neurons of one model, LIF under Euler or AdEx under RK4, each with n_syn
+ * input synapses driven by fixed spike trains, run `steps` steps of
+ * dt_s.  params holds the model's parameter vector (order of
+ * edlut_population_create); g_leak, e_leak, tau_exc and tau_inh are taken
+ * per neuron from the tensors instead.  The forward pass writes the V trace
+ * and keeps the state every `checkpoint` steps; the backward pass takes
+ * dL/dV for every step and writes dL/d(g_leak, e_leak, tau_exc, tau_inh)
+ * per neuron and dL/dweight per synapse.  It is the reverse-mode derivative
+ * of the discrete forward map, with the forward run's spike pattern held
+ * fixed; it replays each checkpointed segment before sweeping it backwards.
+ * The AdEx exponential term uses expf (not the look-up table, whose
+ * derivative vanishes) and decay factors are exp(-h/tau) of each neuron.
+ */
+typedef struct edlut_fit edlut_fit;
+typedef struct edlut_fit_tensors {
+    const float *g_leak, *e_leak, *tau_exc, *tau_inh; /* [n] device */
+    const float *weight;                              /* [n][n_syn] device */
+    const uint8_t *syn_type;                          /* [n][n_syn]: 0 exc, 1 inh */
+    const int32_t *ev_ptr;                            /* [n+1]: events of neuron i */
+    const int32_t *ev_step;                           /* update index (1..steps) the event precedes */
+    const int32_t *ev_syn;                            /* synapse of the event (0..n_syn-1) */
+    float *v_trace;                                   /* [steps][n]: V after each update */
+    const float *grad_v;                              /* [steps][n]: dL/dV (backward) */
+    float *grad_par;                                  /* [4][n] (backward output) */
+    float *grad_w;                                    /* [n][n_syn] (backward output) */
+} edlut_fit_tensors;
+int edlut_fit_create(int model, const float *params, int n_params, float dt_s, int n_neurons,
+                     int steps, int n_syn, int checkpoint, int device, edlut_fit **out);
+int edlut_fit_destroy(edlut_fit *fit);
+/* Asynchronous on `stream` (a hipStream_t; null = the default stream). */
+int edlut_fit_forward(edlut_fit *fit, const edlut_fit_tensors *t, void *stream);
+/* Needs the checkpoints of a forward pass with the same tensors. */
+int edlut_fit_backward(edlut_fit *fit, const edlut_fit_tensors *t, void *stream);
+/* LIF refractory fraction by steps since the last spike (host helper; the
+ * double clock of LIFTimeDrivenModel.cpp:189-201). */
+int edlut_refractory_table(float dt_s, float tau_ref, float *out, int cap, int *len);
+
 /*
  * Host helper: sort an edge list into the row layout above, in the order of
  * Network::FindOutConnections / qsort_inters (Kernel/src/spike/Network.cpp:

@@ -66,6 +66,10 @@ def lib():
         l.orc_net_add_source_poisson.argtypes = [P, ctypes.c_int, P, ctypes.c_float, P, P, P]
         l.orc_net_set_external.argtypes = [P, ctypes.c_int, P, ctypes.c_float, ctypes.c_uint64]
         l.orc_net_set_trace.argtypes = [P, ctypes.c_int]
+        fit = [ctypes.c_int, P, P, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+               P, P, P, P, P, P]
+        l.orc_fit_forward.argtypes = fit + [ctypes.c_int, P, P]
+        l.orc_fit_backward.argtypes = fit + [P, P, P]
         l.orc_net_set_stdp.argtypes = [P, ctypes.c_int] + [ctypes.c_float] * 5
         l.orc_net_get_synapses.argtypes = [P, ctypes.c_int, P, P, P, P]
         l.orc_net_get_trace.restype = ctypes.c_int64
@@ -245,3 +249,50 @@ def from_spec(spec, params: np.ndarray, csr_builder, fp64: bool = False):
         thr, w_ext, seed = spec.ext
         net.set_external(thr, w_ext, seed)
     return pop, net
+
+
+class FitOracle:
+    """FP64 (or float) forward simulation and FP64 adjoint of the trace-fitting
+    path, for a workloads.FitSpec-like set of inputs."""
+
+    MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1}
+
+    def __init__(self, model, params, n, steps, syn_type, events, refr=None, dt_s=np.float32(1e-4)):
+        self.model = self.MODELS[model]
+        p = np.asarray(params, np.float32)
+        if self.model == 0:   # LIF: e_exc e_inh v_thr c_m
+            fixed = p[[0, 1, 3, 4]]
+        else:                 # AdEx: a b thr_slo_fac v_thr tau_w e_exc e_inh e_reset c_m
+            fixed = p[[0, 1, 2, 3, 4, 5, 6, 7, 10]]
+        self.fixed = np.ascontiguousarray(fixed, np.float32)
+        self.refr = np.ascontiguousarray(refr if refr is not None else np.zeros(0), np.float32)
+        self.n, self.steps = n, steps
+        self.syn_type = np.ascontiguousarray(syn_type, np.uint8)
+        self.n_syn = self.syn_type.shape[1]
+        self.ev = [np.ascontiguousarray(e, np.int32) for e in events]
+        self.dt_s = float(np.float32(dt_s))
+
+    def _common(self):
+        return [self.model, _p(self.fixed), _p(self.refr) if len(self.refr) else None, len(self.refr),
+                self.n, self.steps, self.n_syn, self.dt_s, *[_p(e) for e in self.ev], _p(self.syn_type)]
+
+    @staticmethod
+    def _par(gl, el, te, ti):
+        return np.ascontiguousarray(np.stack([gl, el, te, ti]).astype(np.float64))
+
+    def forward(self, gl, el, te, ti, w, fp64=True):
+        par = self._par(gl, el, te, ti)
+        ww = np.ascontiguousarray(w, np.float64)
+        v = np.zeros((self.steps, self.n), np.float64)
+        spk = np.zeros((self.steps, self.n), np.uint8)
+        lib().orc_fit_forward(*self._common(), _p(par), _p(ww), int(fp64), _p(v), _p(spk))
+        return v, spk
+
+    def backward(self, gl, el, te, ti, w, grad_v):
+        par = self._par(gl, el, te, ti)
+        ww = np.ascontiguousarray(w, np.float64)
+        gv = np.ascontiguousarray(grad_v, np.float64)
+        gp = np.zeros((4, self.n), np.float64)
+        gw = np.zeros((self.n, self.n_syn), np.float64)
+        lib().orc_fit_backward(*self._common(), _p(par), _p(ww), _p(gv), _p(gp), _p(gw))
+        return gp, gw
