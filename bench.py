@@ -89,6 +89,7 @@ def bench_sweep(n_networks: int, steps: int, seq_sample: int, seq_steps: int):
     m.SetParameters({"int_meth": E.integration_method("RK4", 1e-4)})
     m.InitializeGroups(spec.sizes, spec.params)
     net = Network(m, spec.max_delay)
+    net.set_log_capacity(0)                      # spike count only
     net.generate_recurrent(spec.in_degree, spec.exc_fraction, spec.w_mu, spec.w_sigma, spec.wiring_seed)
     net.set_external(thr, w_ext, ext_seed)
     net.synchronize()
@@ -98,13 +99,17 @@ def bench_sweep(n_networks: int, steps: int, seq_sample: int, seq_steps: int):
           f"({tune_ms:.4f} ms/step)", file=sys.stderr, flush=True)
     net.synchronize()
     t0 = time.perf_counter()
-    net.run(steps, sync=False)
-    net.synchronize()
+    chunk = 1000
+    for k in range(0, steps, chunk):     # a progress line per chunk (one sync each)
+        net.run(min(chunk, steps - k), sync=False)
+        net.synchronize()
+        print(f"sweep: {k + min(chunk, steps - k)} of {steps} steps, {time.perf_counter() - t0:.1f}s",
+              file=sys.stderr, flush=True)
     wall = time.perf_counter() - t0
-    st, ix = net.spikes()
+    n_spk = net.spike_count()
     real = int(spec.sizes.sum())
     batched = real * steps / wall
-    rate_hz = len(st) / (real * steps * 1e-4)
+    rate_hz = n_spk / (real * steps * 1e-4)
     net.close()
     m.close()
 
@@ -120,6 +125,7 @@ def bench_sweep(n_networks: int, steps: int, seq_sample: int, seq_steps: int):
         pm.SetParameters({"int_meth": E.integration_method("RK4", 1e-4)})
         pm.InitializeStates(int(spec.sizes[g]))
         pn = Network(pm, spec.max_delay)
+        pn.set_log_capacity(0)
         pn.generate_recurrent(spec.in_degree, spec.exc_fraction, spec.w_mu, spec.w_sigma,
                               spec.wiring_seed + int(g))
         pn.set_external(thr, w_ext, ext_seed + int(g))
@@ -130,6 +136,7 @@ def bench_sweep(n_networks: int, steps: int, seq_sample: int, seq_steps: int):
         pn.synchronize()
         seq_time += time.perf_counter() - t0
         seq_updates += int(spec.sizes[g]) * seq_steps
+        print(f"sequential: network {g} ({spec.sizes[g]} neurons) {seq_time:.1f}s", file=sys.stderr, flush=True)
         pn.close()
         pm.close()
     sequential = seq_updates / seq_time
@@ -158,6 +165,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--neurons", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=60)
+    ap.add_argument("--kernel-steps", type=int, default=1000,
+                    help="steps after the timed region whose update kernels are timed with HIP events")
     ap.add_argument("--w-ext", type=float, default=0.65)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device copy of this size (for counter profiles)")
@@ -183,17 +192,22 @@ def main():
     model, net = E.from_spec(spec)
     print(f"setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
+    print("timed region", file=sys.stderr, flush=True)
     net.run(args.warmup)                       # untimed warmup (synchronised)
-    net.set_timing(args.steps)
     net.synchronize()
     t0 = time.perf_counter()
-    net.run(args.steps, sync=False)
+    net.run(args.steps, sync=False)            # hipGraph replays
     net.synchronize()
     wall = time.perf_counter() - t0
-    ms_launch, launches = net.update_timing()
     st, ix = net.spikes()
     timed = st > args.warmup
     rate_hz = timed.sum() / (spec.n * args.steps * 1e-4)
+    # update-kernel duration: HIP events around every update kernel of the
+    # next kernel_steps steps (launched directly, same stream and kernel)
+    net.set_timing(args.kernel_steps)
+    net.run(args.kernel_steps)
+    ms_launch, launches = net.update_timing()
+    net.set_timing(0)
 
     updates = spec.n * args.steps
     value = updates / wall
@@ -222,6 +236,7 @@ def main():
             "external_drive": f"100 synapses x 5 Hz, w {args.w_ext} nS",
             "mean_rate_hz": round(float(rate_hz), 3),
             "update_kernel_us": round(avg_launch_s * 1e6, 3),
+            "update_kernel_timed_launches": launches,
         },
         "roofline": {
             "bound": "hbm",
@@ -237,6 +252,7 @@ def main():
     if not args.no_sweep:
         out["batched"] = bench_sweep(args.sweep_networks, args.sweep_steps, args.seq_sample, args.seq_steps)
     if not args.no_cpu:
+        print("cpu baseline", file=sys.stderr, flush=True)
         cpu_spec = workloads.adex_recurrent(n=args.neurons, steps=args.cpu_steps, w_ext=args.w_ext)
         v, dt = cpu_baseline(cpu_spec, args.cpu_steps)
         out["cpu_baseline"] = {

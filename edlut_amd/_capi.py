@@ -42,6 +42,8 @@ EXPORTS = (
     "edlut_network_synchronize",
     "edlut_network_spike_log",
     "edlut_network_clear_log",
+    "edlut_network_set_log_capacity",
+    "edlut_network_spike_count",
     "edlut_network_set_timing",
     "edlut_network_update_timing",
     "edlut_network_generate_recurrent",
@@ -117,6 +119,8 @@ def _sig(lib):
         "edlut_network_synchronize": (I32, [P]),
         "edlut_network_spike_log": (I32, [P, P, P, I64, P]),
         "edlut_network_clear_log": (I32, [P]),
+        "edlut_network_set_log_capacity": (I32, [P, I64]),
+        "edlut_network_spike_count": (I32, [P, P]),
         "edlut_network_set_timing": (I32, [P, I64]),
         "edlut_network_update_timing": (I32, [P, P, P]),
         "edlut_network_generate_recurrent": (I32, [P, I32, F32, F32, F32, U64]),

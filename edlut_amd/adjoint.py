@@ -26,7 +26,7 @@ _MODELS = {"LIFTimeDrivenModel_GPU": C.EDLUT_LIF, "AdExTimeDrivenModel_GPU": C.E
 class _Tensors(ctypes.Structure):
     _fields_ = [(name, ctypes.c_void_p) for name in (
         "g_leak", "e_leak", "tau_exc", "tau_inh", "weight", "syn_type", "ev_ptr", "ev_step", "ev_syn",
-        "v_trace", "grad_v", "grad_par", "grad_w")]
+        "v_trace", "spikes", "traj", "traj_since", "grad_v", "grad_par", "grad_w")]
 
 
 def _key(*ts):
@@ -77,7 +77,8 @@ class FitProblem:
         except Exception:
             pass
 
-    def _tensors(self, gl, el, te, ti, w, v=None, gv=None, gp=None, gw=None):
+    def _tensors(self, gl, el, te, ti, w, v=None, gv=None, gp=None, gw=None, spk=None, traj=None,
+                 since=None):
         for x in (gl, el, te, ti, w):
             if x.dtype != torch.float32 or not x.is_contiguous() or x.device != self.dev:
                 raise EDLUTException(C.EDLUT_E_ARG, "parameters must be contiguous float32 on the GPU")
@@ -89,15 +90,30 @@ class FitProblem:
         t.syn_type = _ptr(self.syn_type)
         t.ev_ptr, t.ev_step, t.ev_syn = _ptr(self.ev_ptr), _ptr(self.ev_step), _ptr(self.ev_syn)
         t.v_trace, t.grad_v, t.grad_par, t.grad_w = _ptr(v), _ptr(gv), _ptr(gp), _ptr(gw)
+        t.spikes = _ptr(spk)
+        t.traj, t.traj_since = _ptr(traj), _ptr(since)
         return t
 
-    def forward_raw(self, gl, el, te, ti, w):
+    def forward_raw(self, gl, el, te, ti, w, spikes: bool = False, trajectory: bool = False):
+        """V trace [steps, n]; with spikes, also the spike flags [steps, n]; with
+        trajectory, also the states before each step's integration
+        [steps, 4, n] and the refractory counts [steps, n]."""
         v = torch.empty((self.steps, self.n), dtype=torch.float32, device=self.dev)
-        t = self._tensors(gl, el, te, ti, w, v=v)
+        spk = torch.empty((self.steps, self.n), dtype=torch.uint8, device=self.dev) if spikes else None
+        traj = since = None
+        if trajectory:
+            traj = torch.empty((self.steps, 4, self.n), dtype=torch.float32, device=self.dev)
+            since = torch.empty((self.steps, self.n), dtype=torch.int32, device=self.dev)
+        t = self._tensors(gl, el, te, ti, w, v=v, spk=spk, traj=traj, since=since)
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.dev).cuda_stream)
         C.check(C.lib().edlut_fit_forward(self._h, ctypes.byref(t), stream))
         self._last = _key(gl, el, te, ti, w)
-        return v
+        out = [v]
+        if spikes:
+            out.append(spk)
+        if trajectory:
+            out += [traj, since]
+        return tuple(out) if len(out) > 1 else v
 
     def backward_raw(self, gl, el, te, ti, w, grad_v):
         grad_v = grad_v.contiguous().to(torch.float32)

@@ -47,6 +47,9 @@ struct FitArgs {
     const int32_t *ev_ptr, *ev_step, *ev_syn;
     // forward outputs
     float *v_trace;                              // [steps][n]
+    uint8_t *spikes;                             // [steps][n] (optional)
+    float *traj;                                 // [steps][4][n] state before integration (optional)
+    int32_t *traj_since;                         // [steps][n] refractory count (optional)
     float *ck_state;                             // [segments+1][4][n]
     int32_t *ck_int;                             // [segments+1][2][n]: event cursor, last spike
     // backward
@@ -182,8 +185,18 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_forward(FitArgs a) {
             e++;
             next_step = e < e_end ? a.ev_step[e] : 0x7fffffff;
         }
-        if (fit_step(a, gl, el, cfe, cfi, k - 1 - last, V, W, ge, gi, nullptr)) last = k;
+        if (a.traj) {
+            float *y = a.traj + (int64_t)(k - 1) * 4 * a.n + i;
+            y[0] = V;
+            y[a.n] = W;
+            y[2 * (int64_t)a.n] = ge;
+            y[3 * (int64_t)a.n] = gi;
+            a.traj_since[(int64_t)(k - 1) * a.n + i] = k - 1 - last;
+        }
+        const bool fired = fit_step(a, gl, el, cfe, cfi, k - 1 - last, V, W, ge, gi, nullptr);
+        if (fired) last = k;
         a.v_trace[(int64_t)(k - 1) * a.n + i] = V;
+        if (a.spikes) a.spikes[(int64_t)(k - 1) * a.n + i] = fired ? 1 : 0;
     }
 }
 
@@ -229,17 +242,43 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_backward(FitArgs a) {
             if (fired) last = k;
             float *sc = a.scratch + (int64_t)(k - k0) * kFitStore * n + i;
             const int nst = a.model == 0 ? 4 : kFitStore;
-            for (int q = 0; q < nst; q++) sc[(int64_t)q * n] = st[q];
+            for (int q = 0; q < nst; q++) sc[(int64_t)q * n] = st[q];   // LIF needs the state only
             a.scratch_int[(int64_t)(k - k0) * n + i] = (since < (1 << 29) ? since : (1 << 29)) | (fired ? (1 << 30) : 0);
         }
-        // 2. reverse sweep over the segment
+        // 2. reverse sweep over the segment.  The stored values, dL/dV and
+        // the event cursor of step k-1 are fetched while step k is
+        // differentiated: nothing on the adjoint's dependency chain waits on
+        // memory.
         int er = e - 1;   // last event of the segment
+        int er_step = er >= e_begin ? a.ev_step[er] : -1;
+        const int nst = a.model == 0 ? 4 : kFitStore;
+        float nx[kFitStore];
+        float gv_n;
+        int32_t si_n;
+        {
+            const float *sc = a.scratch + (int64_t)(k1 - k0) * kFitStore * n + i;
+#pragma unroll
+            for (int q = 0; q < kFitStore; q++) nx[q] = q < nst ? sc[(int64_t)q * n] : 0.0f;
+            gv_n = a.grad_v[(int64_t)(k1 - 1) * n + i];
+            si_n = a.scratch_int[(int64_t)(k1 - k0) * n + i];
+        }
         for (int k = k1; k >= k0; k--) {
-            lV += a.grad_v[(int64_t)(k - 1) * n + i];
-            const float *sc = a.scratch + (int64_t)(k - k0) * kFitStore * n + i;
-            const int32_t si = a.scratch_int[(int64_t)(k - k0) * n + i];
+            float cur[kFitStore];
+#pragma unroll
+            for (int q = 0; q < kFitStore; q++) cur[q] = nx[q];
+            const float gvk = gv_n;
+            const int32_t si = si_n;
+            if (k > k0) {
+                const float *sc = a.scratch + (int64_t)(k - 1 - k0) * kFitStore * n + i;
+#pragma unroll
+                for (int q = 0; q < kFitStore; q++) nx[q] = q < nst ? sc[(int64_t)q * n] : 0.0f;
+                gv_n = a.grad_v[(int64_t)(k - 2) * n + i];
+                si_n = a.scratch_int[(int64_t)(k - 1 - k0) * n + i];
+            }
+            lV += gvk;
             const bool fired = (si >> 30) & 1;
-            const float Vy = sc[0], Wy = sc[(int64_t)n], gey = sc[2 * (int64_t)n], giy = sc[3 * (int64_t)n];
+            const float Vy = cur[0], Wy = cur[1], gey = cur[2], giy = cur[3];
+            (void)Wy;
             if (fired) {
                 if (a.model == 0) lp[1] += (double)lV;
                 lV = 0.0f;
@@ -262,14 +301,8 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_backward(FitArgs a) {
                     lcfi += (double)(lgi * giy);
                 }
             } else {
-                const float a1V = sc[4 * (int64_t)n], a1W = sc[5 * (int64_t)n];
-                const float a2V = sc[6 * (int64_t)n], a2W = sc[7 * (int64_t)n];
-                const float a3V = sc[8 * (int64_t)n], a3W = sc[9 * (int64_t)n];
-                const float E1 = sc[10 * (int64_t)n], E2 = sc[11 * (int64_t)n];
-                const float E3 = sc[12 * (int64_t)n], E4 = sc[13 * (int64_t)n];
-                (void)a1W;
-                (void)a2W;
-                (void)a3W;
+                const float a1V = cur[4], a2V = cur[6], a3V = cur[8];
+                const float E1 = cur[10], E2 = cur[11], E3 = cur[12], E4 = cur[13];
                 const float gem = fit_decay(gey, cfe), gim = fit_decay(giy, cfi);
                 const float ge2 = fit_decay(gem, cfe), gi2 = fit_decay(gim, cfi);
                 float l4V = lV * a.s6, l4W = lW * a.s6;
@@ -306,9 +339,11 @@ __global__ __launch_bounds__(kFitBlock) void k_fit_backward(FitArgs a) {
             lW = nW;
             lge = nge;
             lgi = ngi;
-            for (; er >= e_begin && a.ev_step[er] == k; er--) {
+            while (er_step == k) {
                 const int s = a.ev_syn[er];
                 gw[s] += ty[s] == 0 ? lge : lgi;
+                er--;
+                er_step = er >= e_begin ? a.ev_step[er] : -1;
             }
         }
     }

@@ -23,12 +23,13 @@ constexpr int kWavesPerBlock = kBlock / kWave;
 // range, scans its own range and copies every tile's indices to the output,
 // which is therefore ascending.  The last workgroup writes the total.
 // ---------------------------------------------------------------------------
-// Update index of a launch: an immediate value, or read from device memory
-// (graph replays advance the counter on the device between steps).
+// Update index of a launch: an immediate value, or a device-memory base
+// plus the launch's offset within a captured chunk (graph replays advance the
+// base once per chunk).
 struct StepRef {
     const int32_t *ptr;
     int32_t imm;
-    __device__ __forceinline__ int32_t get() const { return ptr ? *ptr : imm; }
+    __device__ __forceinline__ int32_t get() const { return ptr ? *ptr + imm : imm; }
 };
 
 struct TileOut {
@@ -86,7 +87,8 @@ struct CompactArgs {
     int ring;
     int64_t slot_len;
     // optional spike log; the entry count is double-buffered by step parity:
-    // read log_count[step & 1], write log_count[(step + 1) & 1]
+    // read log_count[step & 1], write log_count[(step + 1) & 1].  With
+    // log_idx null only the count is kept.
     int32_t *log_step;
     int32_t *log_idx;
     int64_t *log_count;
@@ -97,14 +99,16 @@ struct CompactArgs {
 
 constexpr int kCompactTiles = kBlock;   // tiles per compaction workgroup
 
-__global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) {
+// Body of the compaction for workgroup `bid` of `nb` (also run by the fused
+// compaction + input-count kernel).
+__device__ __forceinline__ void compact_body(const CompactArgs &a, int bid, int nb) {
     const int32_t step = a.step.get();
     const int slot = step % a.ring;
     int32_t *const out_idx = a.out_idx + (int64_t)slot * a.slot_len;
     __shared__ int64_t s_red[kWavesPerBlock];
     __shared__ int32_t s_scan[kBlock];
     __shared__ int64_t s_base;
-    const int t0 = blockIdx.x * kCompactTiles;
+    const int t0 = bid * kCompactTiles;
     // 1. spikes in the tiles before this range
     // tile_count is zero-padded to a multiple of 8*kBlock entries, and t0 is
     // a multiple of kBlock: the eight loads of a round are unconditional.
@@ -155,20 +159,22 @@ __global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) {
             }
         }
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+    if (bid == nb - 1 && threadIdx.x == 0) {
         const int64_t total = base + s_scan[kBlock - 1];
         a.out_count[slot] = (int32_t)total;
         if (a.log_count) {
             const int64_t nl = log_base + total;
-            if (nl > a.log_cap) atomicOr(a.err, 2u);
+            if (a.log_idx && nl > a.log_cap) atomicOr(a.err, 2u);
             a.log_count[(step + 1) & 1] = nl;
         }
     }
 }
 
-// Advances a device step counter (last node of a captured step).
-__global__ void k_advance_step(int32_t *step) {
-    if (threadIdx.x == 0) *step = *step + 1;
+__global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) { compact_body(a, blockIdx.x, gridDim.x); }
+
+// Advances a device step base (last node of a captured chunk).
+__global__ void k_advance_step(int32_t *step, int32_t by) {
+    if (threadIdx.x == 0) *step = *step + by;
 }
 
 // Counter-based hash of (seed, step, neuron) for the external drive.

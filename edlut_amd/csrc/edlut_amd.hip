@@ -227,6 +227,7 @@ struct edlut_population {
 
     FoldTarget fold_target(int par, bool by_step = false) const {
         FoldTarget t;
+        t.seg_off = nullptr;
         t.par_by_step = by_step ? 1 : 0;
         t.state = state;
         t.stride = stride;
@@ -326,12 +327,15 @@ struct edlut_network {
     hipGraphExec_t graph = nullptr;
     void (*graph_fn)(UpdateArgs) = nullptr;
     int graph_chunk = 0;
-    bool graph_timed = false;
     bool dirty = true;
     bool use_graph = true;
     // launch shape of the ring fold (set by hand or by the autotuner)
     int fold_lanes = 32;
     int fold_blocks = 512;
+    // fixed fold segment of every (type, target) key: prefix of the keys'
+    // synapse counts over all sources (rebuilt when sources change)
+    uint64_t *seg_off = nullptr;
+    bool seg_dirty = true;
     // STDP: update time of each ring slot and delays in seconds
     bool plastic = false;
     double *d_time = nullptr;        // [ring]
@@ -487,10 +491,9 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     return a;
 }
 
-void launch_compact(hipStream_t st, int32_t *tile_count, int32_t *staging, int n_tiles,
-                    int tile_size, int32_t *out_idx, int32_t *out_count, int ring, int64_t slot_len,
-                    int32_t *log_step, int32_t *log_idx, int64_t *log_count, int64_t log_cap,
-                    uint32_t *err, StepRef step) {
+CompactArgs compact_args(int32_t *tile_count, int32_t *staging, int n_tiles, int tile_size, int32_t *out_idx,
+                         int32_t *out_count, int ring, int64_t slot_len, int32_t *log_step, int32_t *log_idx,
+                         int64_t *log_count, int64_t log_cap, uint32_t *err, StepRef step) {
     CompactArgs c;
     c.tile_count = tile_count;
     c.staging = staging;
@@ -506,14 +509,26 @@ void launch_compact(hipStream_t st, int32_t *tile_count, int32_t *staging, int n
     c.log_cap = log_cap;
     c.err = err;
     c.step = step;
-    const int blocks = (n_tiles + kCompactTiles - 1) / kCompactTiles;
-    hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(kBlock), 0, st, c);
+    return c;
+}
+
+inline int compact_blocks(int n_tiles) { return (n_tiles + kCompactTiles - 1) / kCompactTiles; }
+
+void launch_compact(hipStream_t st, int32_t *tile_count, int32_t *staging, int n_tiles,
+                    int tile_size, int32_t *out_idx, int32_t *out_count, int ring, int64_t slot_len,
+                    int32_t *log_step, int32_t *log_idx, int64_t *log_count, int64_t log_cap,
+                    uint32_t *err, StepRef step) {
+    CompactArgs c = compact_args(tile_count, staging, n_tiles, tile_size, out_idx, out_count, ring, slot_len,
+                                 log_step, log_idx, log_count, log_cap, err, step);
+    hipLaunchKernelGGL(k_compact, dim3(compact_blocks(n_tiles)), dim3(kBlock), 0, st, c);
 }
 
 using RingFoldFn = void (*)(FoldTarget, RingEvents);
 template <int LANES>
 RingFoldFn ring_fold_fn(int pass) {
-    return pass == kCount ? k_fold_ring<kCount, LANES> : k_fold_ring<kScatter, LANES>;
+    if (pass == kCount) return k_fold_ring<kCount, LANES>;
+    if (pass == kScatter) return k_fold_ring<kScatter, LANES>;
+    return k_fold_ring<kStatic, LANES>;
 }
 RingFoldFn pick_ring_fold(int pass, int lanes) {
     switch (lanes) {
@@ -524,22 +539,22 @@ RingFoldFn pick_ring_fold(int pass, int lanes) {
         default: return nullptr;
     }
 }
+// The network-mode fold (one pass into the keys' fixed segments).
+int launch_ring_fold(const FoldTarget &t, hipStream_t st, const RingEvents &re, int lanes, int blocks) {
+    RingFoldFn f = pick_ring_fold(kStatic, lanes);
+    if (!f) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
+    hipLaunchKernelGGL(f, dim3(blocks), dim3(kBlock), 0, st, t, re);
+    return 0;
+}
 
-int launch_fold(edlut_population *p, int par, int n_events, bool ring, const RingEvents *re,
-                const PushEvents *pe, int lanes = 32, int blocks = 512) {
-    FoldTarget t = p->fold_target(par, ring);
-    if (ring) {
-        RingFoldFn c = pick_ring_fold(kCount, lanes), sc = pick_ring_fold(kScatter, lanes);
-        if (!c || !sc) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
-        hipLaunchKernelGGL(c, dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
-        hipLaunchKernelGGL(sc, dim3(blocks), dim3(kBlock), 0, p->stream, t, *re);
-    } else {
-        if (n_events == 0) return 0;
-        int nb = (n_events + kBlock - 1) / kBlock;
-        if (nb > 4096) nb = 4096;
-        hipLaunchKernelGGL((k_fold_push<kCount>), dim3(nb), dim3(kBlock), 0, p->stream, t, *pe);
-        hipLaunchKernelGGL((k_fold_push<kScatter>), dim3(nb), dim3(kBlock), 0, p->stream, t, *pe);
-    }
+// Both passes of the fold of host-pushed events (plug-in mode).
+int launch_push_fold(edlut_population *p, int par, int n_events, const PushEvents *pe) {
+    if (n_events == 0) return 0;
+    FoldTarget t = p->fold_target(par, false);
+    int nb = (n_events + kBlock - 1) / kBlock;
+    if (nb > 4096) nb = 4096;
+    hipLaunchKernelGGL((k_fold_push<kCount>), dim3(nb), dim3(kBlock), 0, p->stream, t, *pe);
+    hipLaunchKernelGGL((k_fold_push<kScatter>), dim3(nb), dim3(kBlock), 0, p->stream, t, *pe);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -949,7 +964,7 @@ int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
         HIP_TRY(hipMemcpyAsync(p->e_w, p->q_w.data(), sizeof(float) * ne, hipMemcpyHostToDevice, p->stream));
         if ((rc0 = p->reserve_events((uint64_t)ne))) return rc0;
         PushEvents pe{p->e_tgt, p->e_type, p->e_w, ne};
-        int rc = launch_fold(p, par, ne, false, nullptr, &pe);
+        int rc = launch_push_fold(p, par, ne, &pe);
         if (rc) return rc;
     }
     // 2. integrate, detect, compact
@@ -1091,7 +1106,7 @@ int edlut_network_destroy(edlut_network *n) {
     }
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
-                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_delay_s};
+                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_delay_s, n->seg_off};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     for (hipEvent_t e : n->ev) (void)hipEventDestroy(e);
@@ -1146,6 +1161,7 @@ int edlut_network_add_recurrent(edlut_network *n, const int64_t *row_ptr, const 
     if (rc) return rc;
     n->sources.push_back(s);
     n->dirty = true;
+    n->seg_dirty = true;
     n->self_src = (int)n->sources.size() - 1;
     n->total_syn += (uint64_t)row_ptr[(size_t)s.n * n->max_delay];
     return n->pop->reserve_events(n->total_syn + 1);
@@ -1175,6 +1191,7 @@ int edlut_network_add_poisson(edlut_network *n, int n_src, const uint32_t *lcg_s
     HIP_TRY(hipMemset(s.tile_count, 0, sizeof(int32_t) * padded_tiles(tiles)));
     n->sources.push_back(s);
     n->dirty = true;
+    n->seg_dirty = true;
     n->total_syn += (uint64_t)row_ptr[(size_t)s.n * n->max_delay];
     return n->pop->reserve_events(n->total_syn + 1);
 }
@@ -1218,25 +1235,27 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         launch_compact(p->stream, s.tile_count, s.staging, pa.n_tiles, kBlock, s.ring_idx, s.ring_cnt,
                        n->ring, s.n, nullptr, nullptr, nullptr, 0, p->err, step);
     }
-    // Deliver the events consumed by update k.
+    // Deliver the events consumed by update k into their keys' segments.
+    FoldTarget ft = p->fold_target(0, true);
+    ft.seg_off = n->seg_off;
     if (re0.n_sources > 0) {
         RingEvents re = re0;
         re.step = step;
-        int rc = launch_fold(p, 0, 0, true, &re, nullptr, n->fold_lanes, n->fold_blocks);
+        int rc = launch_ring_fold(ft, p->stream, re, n->fold_lanes, n->fold_blocks);
         if (rc) return rc;
     }
     UpdateArgs a = make_update_args(p, 0);
     a.step = step;
     a.ext = n->ext;
     a.out = TileOut{n->tile_count, n->staging};
-    a.fold = p->fold_target(0, true);
+    a.fold = ft;
     const int tiles = p->update_tiles();
     if (ev0) HIP_TRY(hipEventRecord(ev0, p->stream));
     hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), 0, p->stream, a);
     if (ev1) HIP_TRY(hipEventRecord(ev1, p->stream));
     launch_compact(p->stream, n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
-                   n->own_ring_cnt, n->ring, p->n, n->log_step, n->log_idx, n->log_count, n->log_cap,
-                   p->err, step);
+                   n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
+                   n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);
     // post-synaptic learning of the neurons that fired at this update
     for (Source &s : n->sources) {
         if (!s.stdp) continue;
@@ -1276,9 +1295,9 @@ static RingEvents ring_events(edlut_network *n) {
 
 static constexpr int kGraphChunk = 16;
 
-// Captures kGraphChunk steps that read the update index from d_step and
-// advance it; per-step timing events bracket each update kernel.
-static int build_graph(edlut_network *n, UpdateFn fn, bool timed) {
+// Captures kGraphChunk steps; step i of the chunk reads its update index as
+// d_step + i, and the chunk's last node advances d_step by kGraphChunk.
+static int build_graph(edlut_network *n, UpdateFn fn) {
     edlut_population *p = n->pop;
     if (n->graph) {
         (void)hipGraphExecDestroy(n->graph);
@@ -1288,11 +1307,9 @@ static int build_graph(edlut_network *n, UpdateFn fn, bool timed) {
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int i = 0; i < kGraphChunk && rc == 0; i++) {
-        hipEvent_t e0 = timed ? n->ev[2 * i] : nullptr, e1 = timed ? n->ev[2 * i + 1] : nullptr;
-        rc = enqueue_step(n, StepRef{n->d_step, 0}, fn, re, e0, e1);
-        hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, p->stream, n->d_step);
-    }
+    for (int i = 0; i < kGraphChunk && rc == 0; i++)
+        rc = enqueue_step(n, StepRef{n->d_step, i}, fn, re, nullptr, nullptr);
+    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, p->stream, n->d_step, (int32_t)kGraphChunk);
     hipError_t e = hipStreamEndCapture(p->stream, &g);
     if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -1304,8 +1321,43 @@ static int build_graph(edlut_network *n, UpdateFn fn, bool timed) {
     if (e != hipSuccess) return fail(EDLUT_E_HIP, "graph instantiation failed: %s", hipGetErrorString(e));
     n->graph_chunk = kGraphChunk;
     n->graph_fn = fn;
-    n->graph_timed = timed;
     n->dirty = false;
+    return 0;
+}
+
+// Fixed fold segments: per (type, target) key, the count of synapses of all
+// sources ending there, as an exclusive prefix (events of one step never
+// exceed it).
+static int build_segments(edlut_network *n) {
+    edlut_population *p = n->pop;
+    const int64_t keys = 3 * p->stride;
+    const int64_t nch = (keys + kScanChunk - 1) / kScanChunk;
+    uint32_t *deg = nullptr;
+    int64_t *sums = nullptr;
+    int rc;
+    if (!n->seg_off && (rc = dalloc(&n->seg_off, keys + 1))) return rc;
+    if ((rc = dalloc(&deg, keys)) || (rc = dalloc(&sums, nch + 1))) {
+        if (deg) (void)hipFree(deg);
+        return rc;
+    }
+    hipError_t e = hipMemsetAsync(deg, 0, sizeof(uint32_t) * keys, p->stream);
+    for (const Source &s : n->sources) {
+        if (s.nsyn == 0) continue;
+        int64_t nb = (s.nsyn + kBlock - 1) / kBlock;
+        if (nb > 65536) nb = 65536;
+        hipLaunchKernelGGL(k_key_degree, dim3((unsigned)nb), dim3(kBlock), 0, p->stream, s.tgt_type, s.nsyn,
+                           p->stride, deg);
+    }
+    hipLaunchKernelGGL(k_scan_sums, dim3((unsigned)nch), dim3(kBlock), 0, p->stream, deg, keys, sums);
+    hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(kBlock), 0, p->stream, sums, nch);
+    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nch), dim3(kBlock), 0, p->stream, deg, keys,
+                       (const int64_t *)sums, (int64_t *)n->seg_off);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    (void)hipFree(deg);
+    (void)hipFree(sums);
+    if (e != hipSuccess) return fail(EDLUT_E_HIP, "fold segment set-up failed: %s", hipGetErrorString(e));
+    n->seg_dirty = false;
     return 0;
 }
 
@@ -1339,13 +1391,21 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     }
     UpdateFn fn = pick_update(p, n->has_ext);
     if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
+    if (n->seg_dirty) {
+        int rc = build_segments(n);
+        if (rc) return rc;
+        n->dirty = true;
+    }
     p->started = true;
-    const bool want_timing = n->timing && n->ev.size() >= 2 * (size_t)kGraphChunk;
+    // timed runs launch every step directly, with events around each update
+    // kernel; untimed runs replay captured chunks
+    const bool timing = n->timing && !n->ev.empty();
+    const RingEvents re = ring_events(n);
     int64_t k = first_step;
     n->timed = 0;
-    if (n->use_graph && n_steps >= kGraphChunk) {
-        if (n->dirty || !n->graph || n->graph_timed != want_timing || n->graph_fn != fn) {
-            int rc = build_graph(n, fn, want_timing);
+    if (n->use_graph && !timing && n_steps >= kGraphChunk) {
+        if (n->dirty || !n->graph || n->graph_fn != fn) {
+            int rc = build_graph(n, fn);
             if (rc) return rc;
         }
         int32_t k32 = (int32_t)k;
@@ -1355,11 +1415,15 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
             HIP_TRY(hipGraphLaunch(n->graph, p->stream));
             k += kGraphChunk;
         }
-        n->timed = want_timing ? kGraphChunk : 0;
     }
-    const RingEvents re = ring_events(n);
     for (; k < first_step + n_steps; k++) {
-        int rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, nullptr, nullptr);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        if (timing && (size_t)(2 * n->timed + 1) < n->ev.size()) {
+            e0 = n->ev[2 * n->timed];
+            e1 = n->ev[2 * n->timed + 1];
+            n->timed++;
+        }
+        int rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, e0, e1);
         if (rc) return rc;
     }
     HIP_TRY(hipGetLastError());
@@ -1387,7 +1451,7 @@ int edlut_network_spike_log(edlut_network *n, int32_t *step, int32_t *idx, int64
     int64_t lc[2] = {0, 0};
     HIP_TRY(hipMemcpy(lc, n->log_count, sizeof(lc), hipMemcpyDeviceToHost));
     int64_t cnt = lc[(n->last_step + 1) & 1];
-    if (cnt > n->log_cap) cnt = n->log_cap;
+    if (cnt > n->log_cap) cnt = n->log_cap;   // 0 when only the count is kept
     *n_out = cnt;
     if (cnt > cap) return fail(EDLUT_E_CAP, "spike log has %lld entries, buffer holds %lld", (long long)cnt, (long long)cap);
     if (cnt > 0) {
@@ -1455,9 +1519,7 @@ int edlut_network_set_timing(edlut_network *n, int64_t max_steps) {
     n->ev.clear();
     n->timed = 0;
     n->timing = max_steps > 0;
-    if (n->timing && max_steps < kGraphChunk) max_steps = kGraphChunk;
     n->ev.assign(2 * (size_t)max_steps, nullptr);
-    n->dirty = true;
     for (auto &e : n->ev) HIP_TRY(hipEventCreate(&e));
     return 0;
 }
@@ -1574,6 +1636,7 @@ int edlut_network_generate_recurrent(edlut_network *n, int in_degree, float exc_
     p->mask |= types;
     n->sources.push_back(s);
     n->dirty = true;
+    n->seg_dirty = true;
     n->self_src = (int)n->sources.size() - 1;
     n->total_syn += (uint64_t)n_syn;
     return p->reserve_events(n->total_syn + 1);
@@ -1604,7 +1667,9 @@ int edlut_network_reset(edlut_network *n) {
     edlut_population *p = n->pop;
     HIP_TRY(hipSetDevice(p->device));
     HIP_TRY(hipStreamSynchronize(p->stream));
-    HIP_TRY(hipMemcpy(p->state, p->init_state.data(), sizeof(float) * p->init_state.size(), hipMemcpyHostToDevice));
+    // every state row except the external current (an input, kept as set)
+    HIP_TRY(hipMemcpy(p->state, p->init_state.data(), sizeof(float) * (size_t)(p->nv - 1) * p->stride,
+                      hipMemcpyHostToDevice));
     if (p->last_spike) {
         hipLaunchKernelGGL(k_fill_i32, dim3(1024), dim3(kBlock), 0, p->stream, p->last_spike, p->stride,
                            -(1 << 30));
@@ -1914,6 +1979,11 @@ static int fit_args(edlut_fit *f, const edlut_fit_tensors *t, FitArgs &a) {
     a.ev_step = t->ev_step;
     a.ev_syn = t->ev_syn;
     a.v_trace = t->v_trace;
+    a.spikes = t->spikes;
+    a.traj = t->traj;
+    a.traj_since = t->traj_since;
+    if ((a.traj == nullptr) != (a.traj_since == nullptr))
+        return fail(EDLUT_E_ARG, "traj and traj_since go together");
     a.grad_v = t->grad_v;
     a.grad_par = t->grad_par;
     a.grad_w = t->grad_w;
@@ -1943,6 +2013,34 @@ int edlut_fit_backward(edlut_fit *f, const edlut_fit_tensors *t, void *stream) {
     const int blocks = (f->n + kFitBlock - 1) / kFitBlock;
     hipLaunchKernelGGL(k_fit_backward, dim3(blocks), dim3(kFitBlock), 0, (hipStream_t)stream, a);
     HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+// Spike log capacity (entries); 0 keeps only the spike count.
+int edlut_network_set_log_capacity(edlut_network *n, int64_t cap) {
+    if (!n || cap < 0) return fail(EDLUT_E_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    if (n->log_step) (void)hipFree(n->log_step);
+    if (n->log_idx) (void)hipFree(n->log_idx);
+    n->log_step = nullptr;
+    n->log_idx = nullptr;
+    n->log_cap = 0;
+    int rc;
+    if (cap > 0 && ((rc = dalloc(&n->log_step, cap)) || (rc = dalloc(&n->log_idx, cap)))) return rc;
+    n->log_cap = cap;
+    n->dirty = true;
+    HIP_TRY(hipMemset(n->log_count, 0, 2 * sizeof(int64_t)));
+    return 0;
+}
+
+int edlut_network_spike_count(edlut_network *n, int64_t *count) {
+    if (!n || !count) return fail(EDLUT_E_ARG, "null argument");
+    int rc = edlut_network_synchronize(n);
+    if (rc) return rc;
+    int64_t lc[2] = {0, 0};
+    HIP_TRY(hipMemcpy(lc, n->log_count, sizeof(lc), hipMemcpyDeviceToHost));
+    *count = lc[(n->last_step + 1) & 1];
     return 0;
 }
 

@@ -55,6 +55,9 @@ struct FoldTarget {
     float *b_w;          //               weight
     uint64_t b_cap;
     uint32_t *err;       // bit 2: event buffer overflow, bit 3: spin limit
+    // network mode: fixed segment of each key (prefix of the keys' synapse
+    // counts; a synapse carries at most one event per step), null otherwise
+    const uint64_t *seg_off;
 };
 
 __device__ __forceinline__ float *g_ptr(const FoldTarget &t, uint64_t key) {
@@ -63,11 +66,20 @@ __device__ __forceinline__ float *g_ptr(const FoldTarget &t, uint64_t key) {
     return t.state + (int64_t)(t.ndiff + (int)type) * t.stride + (int64_t)tgt;
 }
 
-enum FoldPass { kCount = 0, kScatter = 1 };
+enum FoldPass { kCount = 0, kScatter = 1, kStatic = 2 };
 
 template <int PASS>
 __device__ __forceinline__ void fold_event(const FoldTarget &t, int par, uint64_t key, uint64_t ord,
                                            float w) {
+    if (PASS == kStatic) {
+        // network mode, one pass: the key's fixed segment, next free entry
+        const uint32_t pos = atomicAdd(&t.fill[key], 1u);
+        const uint64_t at = t.seg_off[key] + pos;
+        t.b_ord[at] = ord;
+        t.b_w[at] = w;
+        if (pos == 0u) t.tile_flag[(key % (uint64_t)t.stride) / kFlagTile] = 1u;
+        return;
+    }
     if (PASS == kCount) {
         atomicAdd(&t.cnt[key], 1u);
         return;
@@ -105,6 +117,29 @@ __device__ __forceinline__ void fold_event(const FoldTarget &t, int par, uint64_
     } else {
         atomicOr(t.err, 4u);
     }
+}
+
+// Network mode: folds the m >= 1 events of `key` (its fixed segment) into g
+// in order and clears the key's fill count.
+__device__ __attribute__((noinline)) float fold_static(uint32_t *fill, const uint64_t *seg_off,
+                                                       uint64_t *b_ord, float *b_w, uint64_t key,
+                                                       uint32_t m, float g) {
+    const uint64_t b = seg_off[key];
+    for (uint32_t j = 1; j < m; j++) {
+        const uint64_t ko = b_ord[b + j];
+        const float kw = b_w[b + j];
+        int64_t i = (int64_t)j - 1;
+        while (i >= 0 && b_ord[b + i] > ko) {
+            b_ord[b + i + 1] = b_ord[b + i];
+            b_w[b + i + 1] = b_w[b + i];
+            --i;
+        }
+        b_ord[b + i + 1] = ko;
+        b_w[b + i + 1] = kw;
+    }
+    for (uint32_t j = 0; j < m; j++) g = g + b_w[b + j];
+    fill[key] = 0u;
+    return g;
 }
 
 // Folds the segment of `key` (m >= 2 events) into g in order and clears the
@@ -275,6 +310,13 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
             for (int ty = 0; ty < 3; ty++) {
                 if (!(MASK & (1u << ty))) continue;
                 const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)idx[n];
+                if (a.fold.seg_off) {
+                    const uint32_t m = a.fold.fill[key];
+                    if (m >= 1u)
+                        s[n][M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
+                                                        key, m, s[n][M::EXC + ty]);
+                    continue;
+                }
                 const uint32_t m = a.fold.cnt[key];
                 if (m >= 2u)
                     s[n][M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ord,
@@ -364,6 +406,13 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
             for (int ty = 0; ty < 3; ty++) {
                 if (!(MASK & (1u << ty))) continue;
                 const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)i;
+                if (a.fold.seg_off) {
+                    const uint32_t m = a.fold.fill[key];
+                    if (m >= 1u)
+                        sf[M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
+                                                      key, m, sf[M::EXC + ty]);
+                    continue;
+                }
                 const uint32_t m = a.fold.cnt[key];
                 if (m >= 2u)
                     sf[M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ord,
@@ -517,7 +566,7 @@ __device__ __forceinline__ void stdp_increment(const StdpSyn &s, float *w, int64
 // (d, source) the synapse position rises with the source index and then the
 // target index, which is the reference's delivery order.
 template <int PASS, int LANES>
-__global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r) {
+__device__ __forceinline__ void fold_ring_body(const FoldTarget &t, const RingEvents &r, int bid, int nb) {
     __shared__ int64_t s_pref[kMaxDelay * kMaxSources + 1];
     __shared__ int s_slot[kMaxDelay * kMaxSources];
     const int nseg = r.max_delay * r.n_sources;
@@ -543,8 +592,8 @@ __global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r
     __syncthreads();
     const int64_t n_items = s_pref[nseg];
     const int lane = threadIdx.x % LANES;
-    const int64_t group = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / LANES;
-    const int64_t n_groups = (int64_t)gridDim.x * kBlock / LANES;
+    const int64_t group = (int64_t)(bid * kBlock + (int)threadIdx.x) / LANES;
+    const int64_t n_groups = (int64_t)nb * kBlock / LANES;
     for (int64_t it = group; it < n_items; it += n_groups) {
         // last segment g with s_pref[g] <= it (segments may be empty)
         int lo = 0, hi = nseg - 1;
@@ -563,13 +612,13 @@ __global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r
         const int64_t e = src.row_ptr[(int64_t)id * r.max_delay + d];
         const uint64_t ord_hi = ((uint64_t)((d - 1) * r.n_sources + p)) << 40;
         double t_arr = 0.0;
-        if (PASS == kScatter && src.stdp) t_arr = r.time_ring[s_slot[g]] + r.delay_s[d];
+        if (PASS != kCount && src.stdp) t_arr = r.time_ring[s_slot[g]] + r.delay_s[d];
         for (int64_t q = b + lane; q < e; q += LANES) {
             const uint32_t tt = src.tgt_type[q];
             const uint64_t key =
                 (uint64_t)(tt >> 30) * (uint64_t)t.stride + (uint64_t)(tt & 0x3fffffffu);
             fold_event<PASS>(t, par, key, ord_hi | (uint64_t)q, src.w[q]);
-            if (PASS == kScatter && src.stdp) {
+            if (PASS != kCount && src.stdp) {
                 // STDPWeightChange::ApplyPreSynapticSpike (STDPWeightChange.cpp:43-55),
                 // after the event used the weight (PropagatedSpikeGroup.cpp:76-97).
                 // A synapse carries at most one event per step: no other lane
@@ -579,6 +628,21 @@ __global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r
                 stdp_increment(src.syn, src.w, q, -src.syn.a_minus * src.syn.post[q]);
             }
         }
+    }
+}
+
+template <int PASS, int LANES>
+__global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r) {
+    fold_ring_body<PASS, LANES>(t, r, blockIdx.x, gridDim.x);
+}
+
+// Synapses per (type, target) key of one source, for the fixed segments of
+// the network-mode fold.
+__global__ __launch_bounds__(kBlock) void k_key_degree(const uint32_t *tgt_type, int64_t nsyn, int64_t stride,
+                                                       uint32_t *deg) {
+    for (int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x; q < nsyn; q += (int64_t)gridDim.x * kBlock) {
+        const uint32_t tt = tgt_type[q];
+        atomicAdd(&deg[(uint64_t)(tt >> 30) * (uint64_t)stride + (tt & 0x3fffffffu)], 1u);
     }
 }
 

@@ -187,6 +187,15 @@ class Network:
         C.check(C.lib().edlut_network_spike_log(self._h, C.ptr(st), C.ptr(ix), m, C.byref(n)))
         return st[: n.value].copy(), ix[: n.value].copy()
 
+    def set_log_capacity(self, cap: int):
+        """Spike log capacity in entries; 0 keeps only the spike count."""
+        C.check(C.lib().edlut_network_set_log_capacity(self._h, int(cap)))
+
+    def spike_count(self) -> int:
+        c = ctypes.c_int64(0)
+        C.check(C.lib().edlut_network_spike_count(self._h, C.byref(c)))
+        return c.value
+
     def clear_log(self):
         C.check(C.lib().edlut_network_clear_log(self._h))
 

@@ -127,7 +127,7 @@ SWEPT = ("g_leak", "e_leak", "a", "b", "tau_w")
 
 
 def adex_sweep(n_networks: int = 4096, median: float = 5000.0, sigma: float = 0.5,
-               lo: int = 1000, hi: int = 20000, spread: float = 0.3, w_ext: float = 1.0,
+               lo: int = 1000, hi: int = 20000, spread: float = 0.3, w_ext: float = 0.7,
                steps: int = 5000, seed: int = 4) -> SweepSpec:
     from .models import AdExTimeDrivenModel_GPU
 

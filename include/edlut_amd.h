@@ -177,8 +177,8 @@ int edlut_network_set_external(edlut_network *net, int n_thr, const uint32_t *th
 /* Enqueue n_steps updates starting at update index first_step (asynchronous;
  * spikes of the population are appended to the device spike log).  Whole
  * chunks of 16 steps are replayed from a captured hipGraph whose kernels
- * read the update index from device memory; the remainder is launched
- * directly.  Replaces the per-step TimeEventUpdateNeurons / UpdateState /
+ * read the update index from device memory; the remainder (and every step
+ * of a timed run) is launched directly.  Replaces the per-step TimeEventUpdateNeurons / UpdateState /
  * TimeDrivenInternalSpike / PropagatedSpikeGroup cycle for device-resident
  * networks (TimeEventUpdateNeurons.cpp:45-79). */
 int edlut_network_run(edlut_network *net, int64_t first_step, int n_steps);
@@ -189,8 +189,15 @@ int edlut_network_synchronize(edlut_network *net);
 int edlut_network_spike_log(edlut_network *net, int32_t *step, int32_t *idx, int64_t cap,
                             int64_t *n_out);
 int edlut_network_clear_log(edlut_network *net);
-/* Reserve HIP events to time each update-kernel launch of runs of up to
- * max_steps steps (0 switches timing off). */
+/* Spike log capacity in entries (default 2^26); 0 keeps only the count.
+ * Clears the log. */
+int edlut_network_set_log_capacity(edlut_network *net, int64_t cap);
+/* Spikes logged (or counted) since the log was last cleared. */
+int edlut_network_spike_count(edlut_network *net, int64_t *count);
+/* Reserve HIP events to time each update-kernel launch of the next runs, up
+ * to max_steps steps (0 switches timing off).  While timing is on, steps are
+ * launched directly (no graph replay), with an event pair around each update
+ * kernel on the library's stream. */
 int edlut_network_set_timing(edlut_network *net, int64_t max_steps);
 /* Per-launch timing of the update kernel over the last run (HIP events on the
  * library's stream): total milliseconds and launch count. */
@@ -233,7 +240,8 @@ int edlut_network_set_stdp(edlut_network *net, int source, float max_ltp, float 
                            float max_ltd, float tau_ltd, float max_weight);
 /* Traces and last update times of a plastic source's synapses (row order). */
 int edlut_network_get_stdp(edlut_network *net, int source, float *pre, float *post, double *last);
-/* Back to the initial state: neuron state as after construction, Poisson
+/* Back to the initial state: neuron state as after construction (the
+ * external-current row, an input, is kept), Poisson
  * generators at their initial states, plastic weights and traces as when
  * STDP was switched on, empty spike rings and log; the next run starts at
  * update 1. */
@@ -275,6 +283,10 @@ typedef struct edlut_fit_tensors {
     const int32_t *ev_step;                           /* update index (1..steps) the event precedes */
     const int32_t *ev_syn;                            /* synapse of the event (0..n_syn-1) */
     float *v_trace;                                   /* [steps][n]: V after each update */
+    uint8_t *spikes;                                  /* [steps][n]: 1 where a spike reset V (optional) */
+    float *traj;                                      /* [steps][4][n]: V, w, g_exc, g_inh before each
+                                                         step's integration (optional) */
+    int32_t *traj_since;                              /* [steps][n]: steps since the last spike (with traj) */
     const float *grad_v;                              /* [steps][n]: dL/dV (backward) */
     float *grad_par;                                  /* [4][n] (backward output) */
     float *grad_w;                                    /* [n][n_syn] (backward output) */

@@ -168,14 +168,27 @@ template <typename T> void forward_neuron(const Problem<T> &P, int i, const Par<
     }
 }
 
-/* adjoint of one neuron; grad_v[(k-1)*n + i] = dL/dV_k */
+/* adjoint of one neuron; grad_v[(k-1)*n + i] = dL/dV_k.  With `traj` the
+ * states before each step's integration, the spike flags and the refractory
+ * counts come from a given run ([steps][4][n], [steps][n], [steps][n])
+ * instead of this precision's own forward pass. */
 template <typename T> void backward_neuron(const Problem<T> &P, int i, const Par<T> &p, const T *w,
-                                           const T *grad_v, T *gp /*[4]*/, T *gw /*[n_syn]*/) {
+                                           const T *grad_v, T *gp /*[4]*/, T *gw /*[n_syn]*/,
+                                           const float *traj = nullptr, const uint8_t *traj_spk = nullptr,
+                                           const int32_t *traj_since = nullptr) {
     const bool adex = P.model == 1;
     std::vector<T> y((size_t)P.steps * 4), vtr((size_t)P.steps);
     std::vector<uint8_t> spk((size_t)P.steps);
     std::vector<int> since((size_t)P.steps);
-    forward_neuron(P, i, p, w, vtr.data(), spk.data(), 1, 0, &y, &since);
+    if (traj) {
+        for (int k = 0; k < P.steps; k++) {
+            for (int q = 0; q < 4; q++) y[(size_t)k * 4 + q] = (T)traj[((size_t)k * 4 + q) * P.n + i];
+            spk[(size_t)k] = traj_spk[(size_t)k * P.n + i];
+            since[(size_t)k] = traj_since ? traj_since[(size_t)k * P.n + i] : 1 << 30;
+        }
+    } else {
+        forward_neuron(P, i, p, w, vtr.data(), spk.data(), 1, 0, &y, &since);
+    }
     const T h = adex ? P.dt * (T)0.5 : P.dt;
     const T cfe = cf_of(h, p.te), cfi = cf_of(h, p.ti);
     T lV = 0, lW = 0, lge = 0, lgi = 0;
@@ -330,6 +343,49 @@ void orc_fit_forward(int model, const float *fixed, const float *refr, int refr_
                            (std::vector<float> *)nullptr, (std::vector<int> *)nullptr);
         }
         for (size_t q = 0; q < vf.size(); q++) v_out[q] = vf[q];
+    }
+}
+
+/* Float forward that also records the states before each step's
+ * integration ([steps][4][n]) and the refractory counts ([steps][n]). */
+void orc_fit_trajectory(int model, const float *fixed, const float *refr, int refr_len, int n, int steps,
+                        int n_syn, float dt_s, const int32_t *ev_ptr, const int32_t *ev_step,
+                        const int32_t *ev_syn, const uint8_t *syn_type, const double *par, const double *w,
+                        float *traj, uint8_t *spk, int32_t *since_out) {
+    Problem<float> P = make_problem<float>(model, fixed, refr, refr_len, n, steps, n_syn, dt_s, ev_ptr,
+                                           ev_step, ev_syn, syn_type);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < n; i++) {
+        Par<float> p{(float)par[i], (float)par[n + i], (float)par[2 * n + i], (float)par[3 * n + i]};
+        std::vector<float> wf(w + (size_t)i * n_syn, w + (size_t)(i + 1) * n_syn);
+        std::vector<float> y((size_t)steps * 4), v((size_t)steps);
+        std::vector<uint8_t> sp((size_t)steps);
+        std::vector<int> since((size_t)steps);
+        forward_neuron(P, i, p, wf.data(), v.data(), sp.data(), 1, 0, &y, &since);
+        for (int k = 0; k < steps; k++) {
+            for (int q = 0; q < 4; q++) traj[((size_t)k * 4 + q) * n + i] = y[(size_t)k * 4 + q];
+            spk[(size_t)k * n + i] = sp[(size_t)k];
+            since_out[(size_t)k * n + i] = since[(size_t)k];
+        }
+    }
+}
+
+/* FP64 adjoint along a given trajectory (orc_fit_trajectory's layout). */
+void orc_fit_backward_along(int model, const float *fixed, const float *refr, int refr_len, int n, int steps,
+                            int n_syn, float dt_s, const int32_t *ev_ptr, const int32_t *ev_step,
+                            const int32_t *ev_syn, const uint8_t *syn_type, const double *par,
+                            const double *w, const double *grad_v, const float *traj, const uint8_t *spk,
+                            const int32_t *since, double *grad_par, double *grad_w) {
+    Problem<double> P = make_problem<double>(model, fixed, refr, refr_len, n, steps, n_syn, dt_s, ev_ptr,
+                                             ev_step, ev_syn, syn_type);
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int i = 0; i < n; i++) {
+        Par<double> p{par[i], par[n + i], par[2 * n + i], par[3 * n + i]};
+        double gp[4];
+        std::vector<double> gw((size_t)n_syn, 0.0);
+        backward_neuron(P, i, p, w + (size_t)i * n_syn, grad_v, gp, gw.data(), traj, spk, since);
+        for (int q = 0; q < 4; q++) grad_par[(size_t)q * n + i] = gp[q];
+        for (int s = 0; s < n_syn; s++) grad_w[(size_t)i * n_syn + s] = gw[(size_t)s];
     }
 }
 

@@ -70,6 +70,8 @@ def lib():
                P, P, P, P, P, P]
         l.orc_fit_forward.argtypes = fit + [ctypes.c_int, P, P]
         l.orc_fit_backward.argtypes = fit + [P, P, P]
+        l.orc_fit_trajectory.argtypes = fit + [P, P, P]
+        l.orc_fit_backward_along.argtypes = fit + [P, P, P, P, P, P]
         l.orc_net_set_stdp.argtypes = [P, ctypes.c_int] + [ctypes.c_float] * 5
         l.orc_net_get_synapses.argtypes = [P, ctypes.c_int, P, P, P, P]
         l.orc_net_get_trace.restype = ctypes.c_int64
@@ -287,6 +289,30 @@ class FitOracle:
         spk = np.zeros((self.steps, self.n), np.uint8)
         lib().orc_fit_forward(*self._common(), _p(par), _p(ww), int(fp64), _p(v), _p(spk))
         return v, spk
+
+    def trajectory(self, gl, el, te, ti, w):
+        """Float run: states before each step's integration [steps, 4, n],
+        spike flags [steps, n], refractory counts [steps, n]."""
+        par = self._par(gl, el, te, ti)
+        ww = np.ascontiguousarray(w, np.float64)
+        traj = np.zeros((self.steps, 4, self.n), np.float32)
+        spk = np.zeros((self.steps, self.n), np.uint8)
+        since = np.zeros((self.steps, self.n), np.int32)
+        lib().orc_fit_trajectory(*self._common(), _p(par), _p(ww), _p(traj), _p(spk), _p(since))
+        return traj, spk, since
+
+    def backward_along(self, gl, el, te, ti, w, grad_v, traj, spk, since):
+        """FP64 adjoint evaluated along a given (e.g. float) trajectory."""
+        par = self._par(gl, el, te, ti)
+        ww = np.ascontiguousarray(w, np.float64)
+        gv = np.ascontiguousarray(grad_v, np.float64)
+        gp = np.zeros((4, self.n), np.float64)
+        gw = np.zeros((self.n, self.n_syn), np.float64)
+        lib().orc_fit_backward_along(*self._common(), _p(par), _p(ww), _p(gv),
+                                     _p(np.ascontiguousarray(traj, np.float32)),
+                                     _p(np.ascontiguousarray(spk, np.uint8)),
+                                     _p(np.ascontiguousarray(since, np.int32)), _p(gp), _p(gw))
+        return gp, gw
 
     def backward(self, gl, el, te, ti, w, grad_v):
         par = self._par(gl, el, te, ti)

@@ -94,3 +94,34 @@ def test_fp64_adjoint_check_discriminates(oracle):
     # the same derivative with V continuing through the resets differs
     contrib = (gv[:, i] * (spk[:, i] == 1)).sum()
     assert abs(contrib) > 1e-3 * abs(fd)
+
+
+@pytest.mark.parametrize("model,bound", [("LIFTimeDrivenModel_GPU", 1e-5), ("AdExTimeDrivenModel_GPU", None)])
+def test_gradient_conditioning_float_vs_fp64_trajectory(oracle, model, bound):
+    """How much float rounding of the forward trajectory alone moves the
+    gradient: the FP64 adjoint along a float run vs along the FP64 run (same
+    spike trains).  LIF: ~1e-6.  AdEx: tens of percent, from the steps on the
+    exponential upswing before each spike; this is why the GPU adjoint is
+    checked against the FP64 adjoint along its own trajectory."""
+    n, steps = 32, 10000
+    spec = workloads.adjoint_fit(model, n=n, steps=steps, seed=7)
+    refr = None
+    if model == "LIFTimeDrivenModel_GPU":
+        from edlut_amd.adjoint import refractory_table
+
+        refr = refractory_table(1e-4, float(spec.params[7]))
+    o = oracle.FitOracle(model, spec.params, n, steps, spec.syn_type, spec.events, refr)
+    x = [spec.start[k] for k in ("g_leak", "e_leak", "tau_exc", "tau_inh", "weight")]
+    tgt, _ = o.forward(*[spec.target[k] for k in ("g_leak", "e_leak", "tau_exc", "tau_inh", "weight")], fp64=False)
+    v64, s64 = o.forward(*x, fp64=True)
+    gp, _ = o.backward(*x, 2 * (v64 - tgt) / v64.size)
+    traj, sf, since = o.trajectory(*x)
+    vf, _ = o.forward(*x, fp64=False)
+    gp2, _ = o.backward_along(*x, 2 * (vf - tgt) / vf.size, traj, sf, since)
+    same = np.all(sf == s64, axis=0)
+    assert same.mean() > 0.9 and s64.sum() > 20
+    err = np.linalg.norm(gp2[0][same] - gp[0][same]) / np.linalg.norm(gp[0][same])
+    if bound is not None:
+        assert err < bound, err
+    else:
+        assert err > 1e-3, err   # ill-conditioned: far above float rounding
