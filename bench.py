@@ -165,8 +165,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--neurons", type=int, default=1_000_000)
     ap.add_argument("--cpu-steps", type=int, default=60)
-    ap.add_argument("--kernel-steps", type=int, default=1000,
-                    help="steps after the timed region whose update kernels are timed with HIP events")
+    ap.add_argument("--kernel-steps", type=int, default=2000,
+                    help="timed steps whose update kernels stamp their workgroups on the device clock")
     ap.add_argument("--w-ext", type=float, default=0.65)
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device copy of this size (for counter profiles)")
@@ -174,7 +174,8 @@ def main():
     ap.add_argument("--sweep-networks", type=int, default=4096)
     ap.add_argument("--sweep-steps", type=int, default=5000)
     ap.add_argument("--seq-sample", type=int, default=16)
-    ap.add_argument("--seq-steps", type=int, default=500)
+    ap.add_argument("--seq-steps", type=int, default=0,
+                    help="steps per sequential network (default: the sweep's steps)")
     ap.add_argument("--no-sweep", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -192,22 +193,24 @@ def main():
     model, net = E.from_spec(spec)
     print(f"setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
+    items, lanes, blocks, tune_ms = net.autotune(64)   # launch shape (the network is reset after)
+    print(f"autotune -> items {items} lanes {lanes} blocks {blocks} ({tune_ms:.4f} ms/step)",
+          file=sys.stderr, flush=True)
     print("timed region", file=sys.stderr, flush=True)
     net.run(args.warmup)                       # untimed warmup (synchronised)
+    net.set_timing(min(args.steps, args.kernel_steps))
     net.synchronize()
     t0 = time.perf_counter()
     net.run(args.steps, sync=False)            # hipGraph replays
     net.synchronize()
     wall = time.perf_counter() - t0
+    # update-kernel duration in the timed region: every workgroup of the first
+    # kernel_steps update kernels stamps its start and end on the device clock
+    ms_launch, launches = net.update_timing()
+    net.set_timing(0)
     st, ix = net.spikes()
     timed = st > args.warmup
     rate_hz = timed.sum() / (spec.n * args.steps * 1e-4)
-    # update-kernel duration: HIP events around every update kernel of the
-    # next kernel_steps steps (launched directly, same stream and kernel)
-    net.set_timing(args.kernel_steps)
-    net.run(args.kernel_steps)
-    ms_launch, launches = net.update_timing()
-    net.set_timing(0)
 
     updates = spec.n * args.steps
     value = updates / wall
@@ -237,6 +240,7 @@ def main():
             "mean_rate_hz": round(float(rate_hz), 3),
             "update_kernel_us": round(avg_launch_s * 1e6, 3),
             "update_kernel_timed_launches": launches,
+            "launch": {"items_per_thread": items, "fold_lanes": lanes, "fold_blocks": blocks},
         },
         "roofline": {
             "bound": "hbm",
@@ -250,7 +254,8 @@ def main():
     net.close()
     model.close()
     if not args.no_sweep:
-        out["batched"] = bench_sweep(args.sweep_networks, args.sweep_steps, args.seq_sample, args.seq_steps)
+        out["batched"] = bench_sweep(args.sweep_networks, args.sweep_steps, args.seq_sample,
+                                     args.seq_steps or args.sweep_steps)
     if not args.no_cpu:
         print("cpu baseline", file=sys.stderr, flush=True)
         cpu_spec = workloads.adex_recurrent(n=args.neurons, steps=args.cpu_steps, w_ext=args.w_ext)

@@ -318,15 +318,19 @@ struct edlut_network {
     int64_t log_cap = 0;
     int64_t last_step = 0;
     // timing of the update kernel (event pool reserved by set_timing)
-    std::vector<hipEvent_t> ev;
     int64_t timed = 0;
-    bool timing = false;
     // graph replay: a chunk of `chunk` captured steps reading the update
     // index from d_step; rebuilt when the network's configuration changes
     int32_t *d_step = nullptr;
     hipGraphExec_t graph = nullptr;
     void (*graph_fn)(UpdateArgs) = nullptr;
     int graph_chunk = 0;
+    bool graph_fused = false;        // captured steps fold the next step's events
+    // timed runs: per-workgroup [start, end] wall-clock stamps of the update
+    // kernels of steps [clk_first, clk_first + clk_count)
+    unsigned long long *clk = nullptr;
+    int64_t clk_cap = 0, clk_tiles_max = 0, clk_first = 0, clk_count = 0, clk_blocks = 0;
+    int64_t fold_ready = -1;         // update whose events are already folded
     bool dirty = true;
     bool use_graph = true;
     // launch shape of the ring fold (set by hand or by the autotuner)
@@ -539,6 +543,17 @@ RingFoldFn pick_ring_fold(int pass, int lanes) {
         default: return nullptr;
     }
 }
+using CompactFoldFn = void (*)(CompactArgs, FoldTarget, RingEvents, int);
+CompactFoldFn pick_compact_fold(int lanes) {
+    switch (lanes) {
+        case 8: return k_compact_fold<8>;
+        case 16: return k_compact_fold<16>;
+        case 32: return k_compact_fold<32>;
+        case 64: return k_compact_fold<64>;
+        default: return nullptr;
+    }
+}
+
 // The network-mode fold (one pass into the keys' fixed segments).
 int launch_ring_fold(const FoldTarget &t, hipStream_t st, const RingEvents &re, int lanes, int blocks) {
     RingFoldFn f = pick_ring_fold(kStatic, lanes);
@@ -1106,10 +1121,9 @@ int edlut_network_destroy(edlut_network *n) {
     }
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
-                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_delay_s, n->seg_off};
+                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_delay_s, n->seg_off, n->clk};
     for (void *q : dev)
         if (q) (void)hipFree(q);
-    for (hipEvent_t e : n->ev) (void)hipEventDestroy(e);
     n->pop->net = nullptr;
     delete n;
     return 0;
@@ -1215,8 +1229,11 @@ int edlut_network_set_external(edlut_network *n, int n_thr, const uint32_t *thre
 }
 
 // Enqueues every kernel of one network step on the population's stream.
+// fold_done: the fold of update k's events already ran (fused into the
+// previous step's compaction).  Returns through *fused whether this step's
+// compaction also folded the events of update k + 1.
 static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingEvents &re0,
-                        hipEvent_t ev0, hipEvent_t ev1) {
+                        bool fold_done, bool *fused) {
     edlut_population *p = n->pop;
     if (n->plastic)
         hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, p->stream, step, n->d_time, n->ring, (double)p->dt_s);
@@ -1238,7 +1255,8 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     // Deliver the events consumed by update k into their keys' segments.
     FoldTarget ft = p->fold_target(0, true);
     ft.seg_off = n->seg_off;
-    if (re0.n_sources > 0) {
+    const bool ring = re0.n_sources > 0;
+    if (ring && !fold_done) {
         RingEvents re = re0;
         re.step = step;
         int rc = launch_ring_fold(ft, p->stream, re, n->fold_lanes, n->fold_blocks);
@@ -1250,12 +1268,29 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     a.out = TileOut{n->tile_count, n->staging};
     a.fold = ft;
     const int tiles = p->update_tiles();
-    if (ev0) HIP_TRY(hipEventRecord(ev0, p->stream));
+    if (n->clk) {
+        // timed run: every workgroup stamps its start and end
+        a.clk = n->clk;
+        a.clk_first = (int32_t)n->clk_first;
+        a.clk_count = (int32_t)n->clk_count;
+    }
     hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), 0, p->stream, a);
-    if (ev1) HIP_TRY(hipEventRecord(ev1, p->stream));
-    launch_compact(p->stream, n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
-                   n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
-                   n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);
+    CompactArgs c = compact_args(n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
+                                 n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
+                                 n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);
+    const int nbc = compact_blocks(tiles);
+    // STDP orders the post-synaptic learning of update k before the arrivals
+    // of update k + 1, so plastic networks fold in their own step.
+    *fused = ring && !n->plastic;
+    if (*fused) {
+        RingEvents rn = re0;
+        rn.step = StepRef{step.ptr, step.imm + 1};
+        CompactFoldFn f = pick_compact_fold(n->fold_lanes);
+        if (!f) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
+        hipLaunchKernelGGL(f, dim3(nbc + n->fold_blocks), dim3(kBlock), 0, p->stream, c, ft, rn, nbc);
+    } else {
+        hipLaunchKernelGGL(k_compact, dim3(nbc), dim3(kBlock), 0, p->stream, c);
+    }
     // post-synaptic learning of the neurons that fired at this update
     for (Source &s : n->sources) {
         if (!s.stdp) continue;
@@ -1307,8 +1342,13 @@ static int build_graph(edlut_network *n, UpdateFn fn) {
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
+    // the fold of each captured step ran in the step before it when fused
+    // (the run launches the fold of the chunk's first step when needed)
+    const bool will_fuse = re.n_sources > 0 && !n->plastic;
+    bool fused = will_fuse;
     for (int i = 0; i < kGraphChunk && rc == 0; i++)
-        rc = enqueue_step(n, StepRef{n->d_step, i}, fn, re, nullptr, nullptr);
+        rc = enqueue_step(n, StepRef{n->d_step, i}, fn, re, fused, &fused);
+    n->graph_fused = will_fuse;
     hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, p->stream, n->d_step, (int32_t)kGraphChunk);
     hipError_t e = hipStreamEndCapture(p->stream, &g);
     if (rc) {
@@ -1368,6 +1408,13 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     edlut_population *p = n->pop;
     HIP_TRY(hipSetDevice(p->device));
     if (!p->q_tgt.empty()) return fail(EDLUT_E_STATE, "queued host inputs are not used in network mode");
+    if (n->fold_ready >= 0 && (first_step != n->fold_ready || n->seg_dirty)) {
+        // events folded ahead for an update that will not run next (or into
+        // segments about to change): drop them, the fold is redone
+        HIP_TRY(hipMemsetAsync(p->fill, 0, sizeof(uint32_t) * 3 * p->stride, p->stream));
+        HIP_TRY(hipMemsetAsync(p->tile_flag, 0, sizeof(uint32_t) * (p->stride / kFlagTile), p->stream));
+        n->fold_ready = -1;
+    }
     if (first_step != n->last_step + 1) {
         // Rings hold the spikes of the steps just before `first_step` only
         // when the run continues the previous one.
@@ -1397,15 +1444,30 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
         n->dirty = true;
     }
     p->started = true;
-    // timed runs launch every step directly, with events around each update
-    // kernel; untimed runs replay captured chunks
-    const bool timing = n->timing && !n->ev.empty();
+    // timed runs: the update kernels of the first clk_cap steps stamp their
+    // workgroups' start and end (captured chunks included)
     const RingEvents re = ring_events(n);
     int64_t k = first_step;
     n->timed = 0;
-    if (n->use_graph && !timing && n_steps >= kGraphChunk) {
+    if (n->clk) {
+        n->clk_first = first_step;
+        n->clk_count = n_steps < n->clk_cap ? n_steps : n->clk_cap;
+        n->clk_blocks = p->update_tiles();
+        n->dirty = true;   // captured kernel arguments carry the timing window
+        HIP_TRY(hipMemsetAsync(n->clk, 0, sizeof(unsigned long long) * 2 * n->clk_cap * n->clk_tiles_max,
+                               p->stream));
+    }
+    if (n->use_graph && n_steps >= kGraphChunk) {
         if (n->dirty || !n->graph || n->graph_fn != fn) {
             int rc = build_graph(n, fn);
+            if (rc) return rc;
+        }
+        if (n->graph_fused && n->fold_ready != k) {
+            RingEvents rk = re;
+            rk.step = StepRef{nullptr, (int32_t)k};
+            FoldTarget ft = p->fold_target(0, true);
+            ft.seg_off = n->seg_off;
+            int rc = launch_ring_fold(ft, p->stream, rk, n->fold_lanes, n->fold_blocks);
             if (rc) return rc;
         }
         int32_t k32 = (int32_t)k;
@@ -1415,18 +1477,16 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
             HIP_TRY(hipGraphLaunch(n->graph, p->stream));
             k += kGraphChunk;
         }
+        n->fold_ready = n->graph_fused ? k : -1;
     }
     for (; k < first_step + n_steps; k++) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timing && (size_t)(2 * n->timed + 1) < n->ev.size()) {
-            e0 = n->ev[2 * n->timed];
-            e1 = n->ev[2 * n->timed + 1];
-            n->timed++;
-        }
-        int rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, e0, e1);
+        bool fused = false;
+        int rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, n->fold_ready == k, &fused);
         if (rc) return rc;
+        n->fold_ready = fused ? k + 1 : -1;
     }
     HIP_TRY(hipGetLastError());
+    if (n->clk) n->timed = n->clk_count;
     n->last_step = first_step + n_steps - 1;
     return 0;
 }
@@ -1515,27 +1575,44 @@ int edlut_network_set_timing(edlut_network *n, int64_t max_steps) {
     if (!n || max_steps < 0) return fail(EDLUT_E_ARG, "bad argument");
     HIP_TRY(hipSetDevice(n->pop->device));
     HIP_TRY(hipStreamSynchronize(n->pop->stream));
-    for (hipEvent_t e : n->ev) (void)hipEventDestroy(e);
-    n->ev.clear();
+    if (n->clk) (void)hipFree(n->clk);
+    n->clk = nullptr;
     n->timed = 0;
-    n->timing = max_steps > 0;
-    n->ev.assign(2 * (size_t)max_steps, nullptr);
-    for (auto &e : n->ev) HIP_TRY(hipEventCreate(&e));
+    n->clk_cap = 0;
+    n->dirty = true;
+    if (max_steps == 0) return 0;
+    n->clk_tiles_max = n->pop->stride / kFlagTile;   // the most workgroups of any launch shape
+    int rc;
+    if ((rc = dalloc(&n->clk, 2 * (size_t)max_steps * (size_t)n->clk_tiles_max))) return rc;
+    n->clk_cap = max_steps;
     return 0;
 }
 
-int edlut_network_update_timing(edlut_network *n, double *ms, int64_t *launches) {
-    if (!n || !ms || !launches) return fail(EDLUT_E_ARG, "null argument");
+int edlut_network_update_timing(edlut_network *n, double *ms_clock, int64_t *launches) {
+    if (!n || !launches) return fail(EDLUT_E_ARG, "null argument");
     int rc = edlut_network_synchronize(n);
     if (rc) return rc;
-    double total = 0.0;
-    for (int64_t i = 0; i < n->timed; i++) {
-        float t = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&t, n->ev[2 * i], n->ev[2 * i + 1]));
-        total += t;
-    }
-    *ms = total;
     *launches = n->timed;
+    double t_clock = 0.0;
+    if (n->timed > 0) {
+        const size_t nb = (size_t)n->clk_blocks;
+        std::vector<unsigned long long> c(2 * (size_t)n->timed * nb);
+        HIP_TRY(hipMemcpy(c.data(), n->clk, sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost));
+        int khz = 0;
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, n->pop->device));
+        if (khz <= 0) return fail(EDLUT_E_HIP, "device wall-clock rate unavailable");
+        for (int64_t i = 0; i < n->timed; i++) {
+            unsigned long long lo = ~0ull, hi = 0;
+            for (size_t b = 0; b < nb; b++) {
+                const unsigned long long s0 = c[(i * nb + b) * 2], s1 = c[(i * nb + b) * 2 + 1];
+                if (s0 == 0 || s1 == 0) return fail(EDLUT_E_STATE, "a timed launch left no clock stamp");
+                lo = s0 < lo ? s0 : lo;
+                hi = s1 > hi ? s1 : hi;
+            }
+            t_clock += (double)(hi - lo) / (double)khz;   // ms
+        }
+    }
+    if (ms_clock) *ms_clock = t_clock;
     return 0;
 }
 
@@ -1701,6 +1778,7 @@ int edlut_network_reset(edlut_network *n) {
     HIP_TRY(hipMemsetAsync(p->fill, 0, sizeof(uint32_t) * 3 * p->stride, p->stream));
     HIP_TRY(hipMemsetAsync(p->seg, 0xff, sizeof(uint64_t) * 3 * p->stride, p->stream));
     HIP_TRY(hipMemsetAsync(p->tile_flag, 0, sizeof(uint32_t) * (p->stride / kFlagTile), p->stream));
+    n->fold_ready = -1;
     HIP_TRY(hipStreamSynchronize(p->stream));
     n->last_step = 0;
     return 0;
@@ -1739,7 +1817,7 @@ int edlut_network_autotune(edlut_network *n, int trial_steps, int *items, int *f
     HIP_TRY(hipSetDevice(p->device));
     std::vector<int> it_c = p->grouped ? std::vector<int>{2, 4} : std::vector<int>{kItems};
     const int lanes_c[] = {16, 32, 64};
-    const int blocks_c[] = {256, 1024, 4096};
+    const int blocks_c[] = {64, 256, 1024, 4096};
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
@@ -1799,6 +1877,12 @@ int edlut_network_set_stdp(edlut_network *n, int source, float max_ltp, float ta
     edlut_population *p = n->pop;
     HIP_TRY(hipSetDevice(p->device));
     HIP_TRY(hipStreamSynchronize(p->stream));
+    if (n->fold_ready >= 0) {
+        // events folded ahead without the learning rule: fold them again
+        HIP_TRY(hipMemset(p->fill, 0, sizeof(uint32_t) * 3 * p->stride));
+        HIP_TRY(hipMemset(p->tile_flag, 0, sizeof(uint32_t) * (p->stride / kFlagTile)));
+        n->fold_ready = -1;
+    }
     // targets of every synapse -> per-target list of synapse positions
     std::vector<uint32_t> tt((size_t)s.nsyn);
     s.w0.resize((size_t)s.nsyn);

@@ -215,7 +215,30 @@ struct UpdateArgs {
     TileOut out;
     FoldTarget fold;       // ordered input fold of this step (multi-event keys)
     GroupTable grp;        // grouped populations only
+    // timed runs: every workgroup stores its start and end on the device
+    // wall clock at clk[((step - clk_first) * gridDim.x + block) * 2 + {0,1}]
+    // for steps in [clk_first, clk_first + clk_count) (clk null: untimed)
+    unsigned long long *clk;
+    int32_t clk_first;
+    int32_t clk_count;
 };
+
+__device__ __forceinline__ unsigned long long *clock_slot(const UpdateArgs &a, int32_t step) {
+    const int32_t k = step - a.clk_first;
+    if (!a.clk || k < 0 || k >= a.clk_count) return nullptr;
+    return a.clk + ((size_t)k * gridDim.x + blockIdx.x) * 2;
+}
+__device__ __forceinline__ void clock_begin(const UpdateArgs &a, int32_t step) {
+    unsigned long long *c = clock_slot(a, step);
+    if (c && threadIdx.x == 0) c[0] = (unsigned long long)wall_clock64();
+}
+__device__ __forceinline__ void clock_finish(const UpdateArgs &a, int32_t step) {
+    unsigned long long *c = clock_slot(a, step);
+    if (c) {
+        __syncthreads();
+        if (threadIdx.x == 0) c[1] = (unsigned long long)wall_clock64();
+    }
+}
 
 // Reads (and clears) the fold flags covering a tile of T neurons.
 template <int T>
@@ -246,6 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     static_assert(kGroupAlign % (ITEMS * kWave) == 0, "a wave's span lies in one group");
     const int tile = blockIdx.x;
     const int32_t step = a.step.get();
+    clock_begin(a, step);
     const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
     __shared__ uint32_t s_flag;
     // the tile's fold flag is read alongside the state loads (the barrier
@@ -327,14 +351,30 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     }
     if (EXT) {
         // External drive events of this interval, after the recurrent ones;
-        // the hash takes the neuron's index within its group.
+        // the hash takes the neuron's index within its group.  The event
+        // count is #{j : u < thr[j]} (thresholds decrease); the loop walks
+        // the thresholds in wave-uniform order and stops once no lane hits.
+        uint32_t u[ITEMS];
+        int cnt[ITEMS];
 #pragma unroll
         for (int n = 0; n < ITEMS; n++) {
-            const uint32_t u = drive_hash_keyed(step_key, (uint32_t)(idx[n] - gstart));
-            int cnt = 0;
-            while (cnt < a.ext.n_thr && u < a.ext.thr[cnt]) cnt++;
-            for (int q = 0; q < cnt; q++) s[n][M::EXC] += a.ext.w;
+            u[n] = drive_hash_keyed(step_key, (uint32_t)(idx[n] - gstart));
+            cnt[n] = 0;
         }
+        for (int j = 0; j < a.ext.n_thr; j++) {
+            const uint32_t t = a.ext.thr[j];
+            bool hit = false;
+#pragma unroll
+            for (int n = 0; n < ITEMS; n++) {
+                const bool h = u[n] < t;
+                cnt[n] += h ? 1 : 0;
+                hit = hit || h;
+            }
+            if (!__any(hit)) break;
+        }
+#pragma unroll
+        for (int n = 0; n < ITEMS; n++)
+            for (int q = 0; q < cnt[n]; q++) s[n][M::EXC] += a.ext.w;
     }
     float prev_v[ITEMS];
 #pragma unroll
@@ -363,6 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
         }
     }
     tile_spikes<ITEMS>(a.out, tile, spk, idx);
+    clock_finish(a, step);
 }
 
 
@@ -374,6 +415,7 @@ template <class M, unsigned MASK, bool EXT, typename T>
 __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     const int tile = blockIdx.x;
     const int32_t step = a.step.get();
+    clock_begin(a, step);
     const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
     __shared__ uint32_t s_flag;
     if (threadIdx.x == 0) {
@@ -459,6 +501,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         }
     }
     tile_spikes<kItems>(a.out, tile, spk, idx);
+    clock_finish(a, step);
 }
 
 // ---------------------------------------------------------------------------
@@ -634,6 +677,17 @@ __device__ __forceinline__ void fold_ring_body(const FoldTarget &t, const RingEv
 template <int PASS, int LANES>
 __global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r) {
     fold_ring_body<PASS, LANES>(t, r, blockIdx.x, gridDim.x);
+}
+
+// Compaction of update k's spikes and the fold of the events update k + 1
+// consumes, in one launch (network mode without STDP): the fold reads the
+// spike lists of updates <= k - 1 only, and the fold buffers it fills were
+// drained by update k.  Workgroups [0, nb_compact) compact, the rest fold.
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void k_compact_fold(CompactArgs c, FoldTarget t, RingEvents r,
+                                                         int nb_compact) {
+    if ((int)blockIdx.x < nb_compact) compact_body(c, blockIdx.x, nb_compact);
+    else fold_ring_body<kStatic, LANES>(t, r, blockIdx.x - nb_compact, gridDim.x - nb_compact);
 }
 
 // Synapses per (type, target) key of one source, for the fixed segments of

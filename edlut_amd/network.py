@@ -200,11 +200,14 @@ class Network:
         C.check(C.lib().edlut_network_clear_log(self._h))
 
     def set_timing(self, max_steps: int):
-        """Time every update-kernel launch of runs of up to max_steps steps."""
+        """Time the update kernels of the first max_steps steps of the next runs
+        (per-workgroup device-clock stamps; graph replays included)."""
         C.check(C.lib().edlut_network_set_timing(self._h, int(max_steps)))
 
     def update_timing(self):
-        """(total ms, launches) of the update kernel over the last run."""
+        """(total ms, launches) of the timed update kernels of the last run
+        (device-clock span from each launch's first workgroup start to its last
+        workgroup end)."""
         ms = ctypes.c_double(0)
         n = ctypes.c_int64(0)
         C.check(C.lib().edlut_network_update_timing(self._h, C.byref(ms), C.byref(n)))

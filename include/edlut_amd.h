@@ -194,13 +194,13 @@ int edlut_network_clear_log(edlut_network *net);
 int edlut_network_set_log_capacity(edlut_network *net, int64_t cap);
 /* Spikes logged (or counted) since the log was last cleared. */
 int edlut_network_spike_count(edlut_network *net, int64_t *count);
-/* Reserve HIP events to time each update-kernel launch of the next runs, up
- * to max_steps steps (0 switches timing off).  While timing is on, steps are
- * launched directly (no graph replay), with an event pair around each update
- * kernel on the library's stream. */
+/* Time the update kernels of the next runs' first max_steps steps (0
+ * switches timing off): every workgroup of a timed update kernel stores its
+ * start and end on the device wall clock (graph replays included); a
+ * launch's duration is the span from its first start to its last end. */
 int edlut_network_set_timing(edlut_network *net, int64_t max_steps);
-/* Per-launch timing of the update kernel over the last run (HIP events on the
- * library's stream): total milliseconds and launch count. */
+/* Total milliseconds of the timed update-kernel launches of the last run and
+ * their count. */
 int edlut_network_update_timing(edlut_network *net, double *ms, int64_t *launches);
 
 /*
