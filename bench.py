@@ -158,6 +158,47 @@ def bench_sweep(n_networks: int, steps: int, seq_sample: int, seq_steps: int):
     }
 
 
+def bench_adjoint(n: int, steps: int, checkpoint: int, reps: int = 3):
+    """Trace-fitting forward and adjoint kernels (256 LIF/Euler + 256 AdEx/RK4
+    neurons, 50 Poisson inputs at 20 Hz, 10,000 steps, checkpoints every 100
+    steps): kernel time of the backward pass over the forward pass."""
+    import torch
+
+    from edlut_amd import workloads
+    from edlut_amd.adjoint import FitProblem
+
+    out = {"workload": f"{n} LIF (Euler) + {n} AdEx (RK4) neurons, 50 inputs at 20 Hz, {steps} steps, "
+                       f"checkpoint every {checkpoint}"}
+    total_f = total_b = 0.0
+    for model, key in (("LIFTimeDrivenModel_GPU", "lif"), ("AdExTimeDrivenModel_GPU", "adex")):
+        spec = workloads.adjoint_fit(model, n=n, steps=steps, seed=5)
+        prob = FitProblem(model, spec.params, n, steps, spec.syn_type, spec.events, checkpoint=checkpoint)
+        x = [torch.tensor(spec.start[k], dtype=torch.float32, device="cuda")
+             for k in ("g_leak", "e_leak", "tau_exc", "tau_inh", "weight")]
+        gv = torch.randn(steps, n, device="cuda")
+        prob.forward_raw(*x)
+        prob.backward_raw(*x, gv)
+        fs, bs = [], []
+        for _ in range(reps):
+            e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+            e[0].record()
+            prob.forward_raw(*x)
+            e[1].record()
+            prob.backward_raw(*x, gv)
+            e[2].record()
+            torch.cuda.synchronize()
+            fs.append(e[0].elapsed_time(e[1]))
+            bs.append(e[1].elapsed_time(e[2]))
+        f, b = min(fs), min(bs)
+        out[key] = {"forward_ms": f, "backward_ms": b, "ratio": b / f}
+        total_f += f
+        total_b += b
+        prob.close()
+        print(f"adjoint {key}: forward {f:.3f} ms, backward {b:.3f} ms", file=sys.stderr, flush=True)
+    out["cost_ratio"] = total_b / total_f
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -177,6 +218,7 @@ def main():
     ap.add_argument("--seq-steps", type=int, default=0,
                     help="steps per sequential network (default: the sweep's steps)")
     ap.add_argument("--no-sweep", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path.", file=sys.stderr)
@@ -253,6 +295,8 @@ def main():
     }
     net.close()
     model.close()
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(256, 10000, 100)
     if not args.no_sweep:
         out["batched"] = bench_sweep(args.sweep_networks, args.sweep_steps, args.seq_sample,
                                      args.seq_steps or args.sweep_steps)

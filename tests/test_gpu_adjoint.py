@@ -92,16 +92,21 @@ def test_gpu_adjoint_cost_ratio_and_fit(oracle, model):
     for _ in range(2):
         prob.forward_raw(*x)
         prob.backward_raw(*x, gv)
-    e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
-    e[0].record()
-    prob.forward_raw(*x)
-    e[1].record()
-    prob.backward_raw(*x, gv)
-    e[2].record()
-    torch.cuda.synchronize()
-    fwd, bwd = e[0].elapsed_time(e[1]), e[1].elapsed_time(e[2])
+    fs, bs = [], []
+    for _ in range(3):
+        e = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        e[0].record()
+        prob.forward_raw(*x)
+        e[1].record()
+        prob.backward_raw(*x, gv)
+        e[2].record()
+        torch.cuda.synchronize()
+        fs.append(e[0].elapsed_time(e[1]))
+        bs.append(e[1].elapsed_time(e[2]))
+    fwd, bwd = min(fs), min(bs)
     print(f"{model}: forward {fwd:.3f} ms, backward {bwd:.3f} ms, ratio {bwd / fwd:.2f}")
-    assert bwd / fwd < 4.0
+    # the contract's bound: the adjoint costs at most three forward passes
+    assert bwd / fwd < 3.0
     # a few Adam steps on the parameters reduce the trace MSE
     xs = [t.clone().requires_grad_(True) for t in x]
     opt = torch.optim.Adam(xs, lr=1e-3)
