@@ -8,10 +8,12 @@ from ._capi import EDLUTException, lib
 from .models import (
     AdExTimeDrivenModel_GPU,
     CreateNeuronModel,
+    EgidioGranuleCell_TimeDriven_GPU,
     HHTimeDrivenModel_GPU,
     LIFTimeDrivenModel_GPU,
     REGISTRY,
     TimeDrivenNeuronModel_GPU,
+    TimeDrivenPurkinjeCell_GPU,
     integration_method,
 )
 from .network import Network, binomial_thresholds, build_out_csr, from_spec
@@ -22,6 +24,8 @@ __all__ = [
     "AdExTimeDrivenModel_GPU",
     "LIFTimeDrivenModel_GPU",
     "HHTimeDrivenModel_GPU",
+    "EgidioGranuleCell_TimeDriven_GPU",
+    "TimeDrivenPurkinjeCell_GPU",
     "TimeDrivenNeuronModel_GPU",
     "CreateNeuronModel",
     "REGISTRY",

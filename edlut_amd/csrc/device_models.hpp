@@ -80,25 +80,49 @@ struct HHParams {
     double cf_d[3];
 };
 
+// Granule and Purkinje cells keep the enabled synapse types as a run-time
+// mask (`mask`, EDLUT_SYN_* bits): one kernel serves every combination.
+struct GranuleParams {
+    float e_exc, e_inh, nmda_aux;
+    float a_xkca, b_xkca;   // Q10_30*2.5f, Q10_30*1.5f
+    float k_nernst;         // 1000*(R*(temper+273.15)/F)/z for calcium (z = 2)
+    float cf[3];
+    float init[19];
+    uint32_t mask;
+};
+
+struct PurkinjeParams {
+    float e_exc, e_inh, e_leak, v_thr, nmda_aux;
+    float rise, fall;       // refractory dV/dt: (spk_peak-v_thr)/tau_ref_0_5, (e_leak-spk_peak)/tau_ref_0_5
+    double cur_scale;       // 1e-6 * inv_area (pA -> model units, in double)
+    float cf[3];
+    float init[7];
+    uint32_t mask;
+};
+
 struct ModelParams {
     AdExParams adex;
     LIFParams lif;
     HHParams hh;
+    GranuleParams gran;
+    PurkinjeParams pkj;
 };
 
 // Per-thread context the right-hand sides need besides the state.
 struct RhsCtx {
-    const float *chan;   // HH: channel-rate table, 6 floats per entry
+    const float *chan;   // channel-rate table (HH 6, granule 26, Purkinje 4 floats per entry)
     const float *exp_lut;
     const float *nmda_lut;
-    const float *refr;   // LIF: refractory fractions by steps since the last spike
+    const float *refr;   // LIF: refractory fractions by steps since the last spike;
+                         // Purkinje: (derivative mode, V override) pairs (see PurkinjeModel)
     int refr_len;
-    int since_spike;     // LIF: steps since the last spike (see refractory table)
+    int since_spike;     // steps since the last spike (see the clock tables)
 };
 
 struct AdExModel {
     static constexpr int NV = 6, ND = 2;
     static constexpr bool kResetsIntegrator = true;   // spike -> resetState
+    static constexpr bool kClockOverride = false;     // no post-spike V waveform
     static constexpr int EXC = 2, INH = 3, NMDA = 4, EXT = 5;
 
     // AdExTimeDrivenModel.cpp:170-218.
@@ -161,6 +185,7 @@ struct AdExModel {
 struct LIFModel {
     static constexpr int NV = 5, ND = 1;
     static constexpr bool kResetsIntegrator = true;
+    static constexpr bool kClockOverride = false;
     static constexpr int EXC = 1, INH = 2, NMDA = 3, EXT = 4;
 
     // LIFTimeDrivenModel.cpp:170-202.  The double-precision refractory clock
@@ -218,6 +243,7 @@ constexpr float kHHAux = (kHHTable - 1) / (kHHMaxV - kHHMinV);
 struct HHModel {
     static constexpr int NV = 8, ND = 4;
     static constexpr bool kResetsIntegrator = false;
+    static constexpr bool kClockOverride = false;
     static constexpr int EXC = 4, INH = 5, NMDA = 6, EXT = 7;
 
     // HHTimeDrivenModel.cpp:197-264.  The state it is given is clamped in
@@ -271,6 +297,225 @@ struct HHModel {
     }
     __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
         return mp.hh.init[v];
+    }
+};
+
+// Natural logarithm of a positive float, evaluated in double and rounded to
+// float: x = 2^e m with m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s) with
+// s = (m-1)/(m+1) (|s| <= 0.1716), atanh by its Taylor series through s^23
+// (truncation below 1e-19), e ln 2 with ln 2 split so that e*ln2_hi is exact.
+// The result is the correctly rounded logf except where the exact value lies
+// within ~1e-16 (relative) of a rounding midpoint.  Only +,-,*,/ in a fixed
+// order, so a host build with the same operation order gives the same bits.
+__device__ __forceinline__ float log_f(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -__builtin_inff() : __builtin_nanf("");
+    if (x == __builtin_inff()) return x;
+    uint32_t b = __float_as_uint(x);
+    int e = 0;
+    if (b < 0x00800000u) {   // subnormal: scale by 2^23 (exact)
+        x *= 8388608.0f;
+        b = __float_as_uint(x);
+        e = -23;
+    }
+    e += (int)(b >> 23) - 127;
+    double m = (double)__uint_as_float((b & 0x007fffffu) | 0x3f800000u);
+    if (m > 1.4142135623730951) {
+        m *= 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 23.0;
+    p = p * s2 + 1.0 / 21.0;
+    p = p * s2 + 1.0 / 19.0;
+    p = p * s2 + 1.0 / 17.0;
+    p = p * s2 + 1.0 / 15.0;
+    p = p * s2 + 1.0 / 13.0;
+    p = p * s2 + 1.0 / 11.0;
+    p = p * s2 + 1.0 / 9.0;
+    p = p * s2 + 1.0 / 7.0;
+    p = p * s2 + 1.0 / 5.0;
+    p = p * s2 + 1.0 / 3.0;
+    p = p * s2 + 1.0;
+    const double lm = (2.0 * s) * p;
+    const double ln2_hi = 6.93147180369123816490e-01;   // 0x3fe62e42fee00000
+    const double ln2_lo = 1.90821492927058770002e-10;
+    const double de = (double)e;
+    return (float)(de * ln2_hi + (de * ln2_lo + lm));
+}
+
+// Granule channel table (EgidioGranuleCell_TimeDriven.h:133-146,402-552):
+// 1024 entries of 26 floats on [-100, 50] mV.
+constexpr int kGrTable = 1024;
+constexpr int kGrVars = 26;
+constexpr float kGrMinV = -100.0f;
+constexpr float kGrMaxV = 50.0f;
+constexpr float kGrAux = (kGrTable - 1) / (kGrMaxV - kGrMinV);
+
+struct GranuleModel {
+    // V, xNa_f, yNa_f, xNa_r, yNa_r, xNa_p, xK_V, xK_A, yK_A, xK_IR, xK_Ca,
+    // xCa, yCa, xK_sl, Ca | g_exc, g_inh, g_nmda, ext_I
+    static constexpr int NV = 19, ND = 15;
+    static constexpr bool kResetsIntegrator = false;
+    static constexpr bool kClockOverride = false;
+    static constexpr int EXC = 15, INH = 16, NMDA = 17, EXT = 18;
+
+    // EgidioGranuleCell_TimeDriven.cpp:252-365 (constants :26-58).  The
+    // synapse types come from the run-time mask; MASK is unused.
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s,
+                                               T *d) {
+        const GranuleParams &p = mp.gran;
+        const float V = s[0];
+        float cur = 0.0f;
+        if (p.mask & 1u) cur += s[EXC] * (p.e_exc - V);
+        if (p.mask & 2u) cur += s[INH] * (p.e_inh - V);
+        if (p.mask & 4u) {
+            const float gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, V);
+            cur += s[NMDA] * gi * (p.e_exc - V);
+        }
+        cur += s[EXT];
+        cur = (float)((double)cur * 3.34156941e-4);   // float *= double literal
+        const float pv = V;
+        const float vca = p.k_nernst * log_f(fabsf(2.0f / s[14]));   // nernst(Ca, cao = 2, z = 2, 30 C)
+        int pos = trunc_i32_x86((pv - kGrMinV) * kGrAux);
+        pos = pos < 0 ? 0 : (pos > kGrTable - 1 ? kGrTable - 1 : pos);
+        // a row is 104 bytes: 13 aligned float2 loads
+        const float2 *r2 = (const float2 *)(c.chan + pos * kGrVars);
+        float v[kGrVars];
+#pragma unroll
+        for (int q = 0; q < kGrVars / 2; q++) {
+            const float2 t = r2[q];
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+        const float ca = s[14];
+        const float a_kca = p.a_xkca / (1.0f + v[18] / ca);
+        const float b_kca = p.b_xkca / (1.0f + ca * v[19]);
+        const float xkca_inf = a_kca / (a_kca + b_kca);
+        const float inv_tau_xkca = a_kca + b_kca;
+        const float gNa_f = 0.013f * s[1] * s[1] * s[1] * s[2];
+        const float gNa_r = 0.0005f * s[3] * s[4];
+        const float gNa_p = 0.0002f * s[5];
+        const float gK_V = 0.003f * s[6] * s[6] * s[6] * s[6];
+        const float gK_A = 0.004f * s[7] * s[7] * s[7] * s[8];
+        const float gK_IR = 0.0009f * s[9];
+        const float gK_Ca = 0.004f * s[10];
+        const float gCa = 0.00046f * s[11] * s[11] * s[12];
+        const float gK_sl = 0.00035f * s[13];
+        d[1] = (v[0] - s[1]) * v[1];
+        d[2] = (v[2] - s[2]) * v[3];
+        d[3] = (v[4] - s[3]) * v[5];
+        d[4] = (v[6] - s[4]) * v[7];
+        d[5] = (v[8] - s[5]) * v[9];
+        d[6] = (v[10] - s[6]) * v[11];
+        d[7] = (v[12] - s[7]) * v[13];
+        d[8] = (v[14] - s[8]) * v[15];
+        d[9] = (v[16] - s[9]) * v[17];
+        d[10] = (xkca_inf - s[10]) * inv_tau_xkca;
+        d[11] = (v[20] - s[11]) * v[21];
+        d[12] = (v[22] - s[12]) * v[23];
+        d[13] = (v[24] - s[13]) * v[25];
+        // 2*F*A*d = ((2*96485.309f)*1e-04f)*0.2f; betaCa = 1.5, Ca0 = 1e-4
+        constexpr float kFAd = ((2 * 96485.309f) * 1e-04f) * 0.2f;
+        d[14] = (-gCa * (pv - vca) / kFAd - (1.5f * (ca - 1e-04f)));
+        const float VNa = 87.39f, VK = -84.69f;
+        d[0] = (cur + gNa_f * (VNa - pv) + gNa_r * (VNa - pv) + gNa_p * (VNa - pv) + gK_V * (VK - pv) +
+                gK_A * (VK - pv) + gK_IR * (VK - pv) + gK_Ca * (VK - pv) + gCa * (vca - pv) +
+                gK_sl * (VK - pv) + 5.68e-5f * (-58.0f - pv) + 2.17e-5f * (-65.0f - pv)) *
+               (1.0f / 1.0e-3f);
+    }
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
+        const GranuleParams &p = mp.gran;
+        if (p.mask & 1u) s[EXC] = decay_g(s[EXC], p.cf[0]);
+        if (p.mask & 2u) s[INH] = decay_g(s[INH], p.cf[1]);
+        if (p.mask & 4u) s[NMDA] = decay_g(s[NMDA], p.cf[2]);
+    }
+    // EgidioGranuleCell_TimeDriven.cpp:243-249: upward crossing of v_thr = 0 mV.
+    template <typename T = float>
+    __device__ static __forceinline__ bool spike(const ModelParams &, T prev_v, T *s) {
+        return s[0] > 0.0f && prev_v < 0.0f;
+    }
+    __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
+        return mp.gran.init[v];
+    }
+};
+
+// Purkinje channel table (TimeDrivenPurkinjeCell.h:76-91,396-441): 1024
+// entries of (alpha_ca, inv_tau_ca, alpha_M, inv_tau_M) on [-100, 35] mV.
+constexpr int kPkTable = 1024;
+constexpr float kPkMinV = -100.0f;
+constexpr float kPkMaxV = 35.0f;
+constexpr float kPkAux = (kPkTable - 1) / (kPkMaxV - kPkMinV);
+
+struct PurkinjeModel {
+    // V, ca, M | g_exc, g_inh, g_nmda, ext_I
+    static constexpr int NV = 7, ND = 3;
+    static constexpr bool kResetsIntegrator = false;
+    static constexpr bool kClockOverride = true;
+    static constexpr int EXC = 3, INH = 4, NMDA = 5, EXT = 6;
+
+    // The reference's double LastSpikeTime is replaced by the count of steps
+    // since the last spike and a host-built table over that count: entry j
+    // (clock S_j = j-fold double sum of the step, 0 right after a spike)
+    // holds the derivative mode of S_j (0 full, 1 rising, 2 falling flank;
+    // TimeDrivenPurkinjeCell.cpp:218-224) and the membrane potential the
+    // spike condition imposes at S_j (NaN: none; :181-189).  Counts past the
+    // table's end are outside the refractory period.
+    __device__ static __forceinline__ float clock_entry(const RhsCtx &c, int j, int q, float dflt) {
+        return (j >= 0 && j < c.refr_len) ? c.refr[2 * j + q] : dflt;
+    }
+
+    // TimeDrivenPurkinjeCell.cpp:193-239.
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s,
+                                               T *d) {
+        const PurkinjeParams &p = mp.pkj;
+        const float V = s[0], ca = s[1], M = s[2];
+        float cur = 0.0f;
+        if (p.mask & 1u) cur += s[EXC] * (p.e_exc - V);
+        if (p.mask & 2u) cur += s[INH] * (p.e_inh - V);
+        if (p.mask & 4u) {
+            const float gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, V);
+            cur += s[NMDA] * gi * (p.e_exc - V);
+        }
+        cur += s[EXT];
+        cur = (float)((double)cur * p.cur_scale);
+        const float mode = clock_entry(c, c.since_spike, 0, 0.0f);
+        if (mode == 0.0f) {
+            d[0] = (-0.02f * (V + 70.0f) - 0.001f * ca * ca * (V - 125.0f) - 0.75f * M * (V + 95.0f) + cur) *
+                   (1.0f / 0.95f);
+        } else {
+            d[0] = mode == 1.0f ? p.rise : p.fall;
+        }
+        int pos = trunc_i32_x86((V - kPkMinV) * kPkAux);
+        pos = pos < 0 ? 0 : (pos > kPkTable - 1 ? kPkTable - 1 : pos);
+        const float4 t = *(const float4 *)(c.chan + pos * 4);
+        d[1] = t.x - ca * t.y;
+        d[2] = t.z - M * t.w;
+    }
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
+        const PurkinjeParams &p = mp.pkj;
+        if (p.mask & 1u) s[EXC] = decay_g(s[EXC], p.cf[0]);
+        if (p.mask & 2u) s[INH] = decay_g(s[INH], p.cf[1]);
+        if (p.mask & 4u) s[NMDA] = decay_g(s[NMDA], p.cf[2]);
+    }
+    // TimeDrivenPurkinjeCell.cpp:172-176: V reaches v_thr from below.
+    template <typename T = float>
+    __device__ static __forceinline__ bool spike(const ModelParams &mp, T prev_v, T *s) {
+        return s[0] >= mp.pkj.v_thr && prev_v < mp.pkj.v_thr;
+    }
+    // :177-189: after the clock advanced (or restarted at a spike), the
+    // waveform value of the new clock replaces V.
+    template <typename T = float>
+    __device__ static __forceinline__ void clock_override(const RhsCtx &c, bool fired, T *s) {
+        const float v = clock_entry(c, fired ? 0 : c.since_spike + 1, 1, __builtin_nanf(""));
+        if (v == v) s[0] = v;
+    }
+    __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
+        return mp.pkj.init[v];
     }
 };
 

@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <algorithm>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -130,6 +131,110 @@ inline int hh_index(float V) {
     if (!(x < 2147483648.0f) || x < -2147483648.0f) pos = (int)0x80000000u;
     else pos = (int)x;
     return pos < 0 ? 0 : (pos > kHHTable - 1 ? kHHTable - 1 : pos);
+}
+
+// Clamped truncating index of a 1024-entry channel table on [lo, hi].
+inline int chan_index(float V, float lo, float aux) {
+    const float x = (V - lo) * aux;
+    int pos = (!(x < 2147483648.0f) || x < -2147483648.0f) ? (int)0x80000000u : (int)x;
+    return pos < 0 ? 0 : (pos > 1023 ? 1023 : pos);
+}
+
+// x/(exp(x/y)-1), its limit y(1-x/y/2) near x = 0
+// (EgidioGranuleCell_TimeDriven.h:170-178).
+inline float gr_linoid(float x, float y) {
+    if (fabsf(x / y) < 1e-06f) return y * (1 - x / y / 2);
+    return x / (expf(x / y) - 1);
+}
+
+// Temperature factors 3^((30 - T0)/10) of the granule model (.cpp:54-57).
+inline float gr_q10(float t0) { return powf(3.0f, (30.0f - t0) / 10.0f); }
+
+// Granule channel table (EgidioGranuleCell_TimeDriven.h:402-552): per grid
+// point, (steady state, 1/tau) of every voltage-gated variable, and the two
+// voltage factors of the Ca-dependent K gate, all in float with the float exp.
+std::vector<float> build_granule_table() {
+    const float q20 = gr_q10(20.0f), q22 = gr_q10(22.0f), q30 = gr_q10(30.0f), q63 = gr_q10(6.3f);
+    std::vector<float> t((size_t)kGrTable * kGrVars);
+    for (int i = 0; i < kGrTable; i++) {
+        const float V = kGrMinV + ((kGrMaxV - kGrMinV) * i) / (kGrTable - 1);
+        float *o = &t[(size_t)i * kGrVars];
+        auto pair = [&](int k, float a, float b) {   // x_inf = a/(a+b), 1/tau = a+b
+            o[k] = a / (a + b);
+            o[k + 1] = a + b;
+        };
+        pair(0, q20 * (-0.3f) * gr_linoid(V + 19.0f, -10.0f), q20 * 12.0f * expf(-(V + 44.0f) / 18.182f));
+        pair(2, q20 * 0.105f * expf(-(V + 44.0f) / 3.333f), q20 * 1.5f / (1.0f + expf(-(V + 11.0f) / 5.0f)));
+        pair(4, q20 * (0.00008f - 0.00493f * gr_linoid(V - 4.48754f, -6.81881f)),
+             q20 * (0.04752f + 0.01558f * gr_linoid(V + 43.97494f, 0.10818f)));
+        pair(6, q20 * 0.31836f * expf(-(V + 80.0f) / 62.52621f), q20 * 0.01014f * expf((V + 83.3332f) / 16.05379f));
+        o[8] = 1.0f / (1.0f + expf(-(V + 42.0f) / 5.0f));
+        o[9] = (q30 * (-0.091f) * gr_linoid(V + 42.0f, -5.0f) + q30 * 0.062f * gr_linoid(V + 42.0f, 5.0f)) * 0.2f;
+        pair(10, q63 * (-0.01f) * gr_linoid(V + 25.0f, -10.0f), q63 * 0.125f * expf(-0.0125f * (V + 35.0f)));
+        o[12] = 1.0f / (1.0f + expf((V - (-46.7f)) / (-19.8f)));
+        o[13] = (q20 * 4.88826f) / (1 + expf(-(V + 9.17203f) / 23.32708f)) +
+                (q20 * 0.99285f) / expf((V + 18.27914f) / 19.47175f);
+        o[14] = 1.0f / (1.0f + expf((V - (-78.8f)) / 8.4f));
+        o[15] = (q20 * 0.11042f) / (1.0f + expf((V + 111.33209f) / 12.8433f)) +
+                (q20 * 0.10353f) / (1.0f + expf(-(V + 49.9537f) / 8.90123f));
+        pair(16, q20 * 0.13289f * expf(-(V + 83.94f) / 24.3902f), q20 * 0.16994f * expf((V + 83.94f) / 35.714f));
+        o[18] = 0.0015f * expf(-V / 11.765f);
+        o[19] = 1.0f / (0.00015f * expf(-V / 11.765f));
+        pair(20, q20 * 0.04944f * expf((V + 29.06f) / 15.87301587302f), q20 * 0.08298f * expf(-(V + 18.66f) / 25.641f));
+        pair(22, q20 * 0.0013f * expf(-(V + 48.0f) / 18.183f), q20 * 0.0013f * expf((V + 48.0f) / 83.33f));
+        o[24] = 1.0f / (1.0f + expf(-(V - (-30.0f)) / 6.0f));
+        o[25] = q22 * 0.0033f * expf((V + 30.0f) / 40.0f) + q22 * 0.0033f * (-(V + 30.0f) / 20.0f);
+    }
+    return t;
+}
+
+// Purkinje channel table (TimeDrivenPurkinjeCell.h:396-421).
+std::vector<float> build_purkinje_table() {
+    std::vector<float> t((size_t)kPkTable * 4);
+    for (int i = 0; i < kPkTable; i++) {
+        const float V = kPkMinV + ((kPkMaxV - kPkMinV) * i) / (kPkTable - 1);
+        const float alpha_ca = 1.6f / (1 + expf(-0.072f * (V - 5.0f)));
+        const float beta_ca = (0.02f * (V + 8.9f)) / (expf((V + 8.9f) * 0.2f) - 1.0f);
+        const float alpha_m = 0.3f / (1 + expf((-V - 2.0f) * 0.2f));
+        const float beta_m = 0.001f * expf((-V - 60.0f) * 0.055555555555555f);
+        float *o = &t[(size_t)i * 4];
+        o[0] = alpha_ca;
+        o[1] = alpha_ca + beta_ca;
+        o[2] = alpha_m;
+        o[3] = alpha_m + beta_m;
+    }
+    return t;
+}
+
+// Purkinje refractory clock (TimeDrivenPurkinjeCell.cpp:172-224).  After a
+// spike the clock S holds 0 and gains (double)dt_s per update; entry j holds,
+// for S_j, the derivative mode the right-hand side takes (from the float
+// clock in ms: 0 full, 1 rising flank, 2 falling flank) and the membrane
+// potential the spike condition imposes (from the double clock; NaN: none).
+// The table stops where neither applies any more.
+std::vector<float> build_purkinje_clock(float dt_s, float dt_ms, float v_thr, float e_leak, float tau_ref) {
+    const float spk_peak = 31.0f, time_scale = 1000.0f;
+    const float tau_ref_0_5 = tau_ref / 2;
+    const float inv_half = (float)(1.0 / (double)tau_ref_0_5);
+    std::vector<float> t;
+    double S = 0.0;
+    for (int j = 0; j < (1 << 24); j++) {
+        const float ls_f = (float)(time_scale * S);
+        const float mode = ls_f >= tau_ref ? 0.0f : (ls_f <= tau_ref_0_5 ? 1.0f : 2.0f);
+        const double ls = S * time_scale;
+        float v = std::numeric_limits<float>::quiet_NaN();
+        if (ls < tau_ref) {
+            if (ls <= tau_ref_0_5) v = (float)(v_thr + (spk_peak - v_thr) * (ls * inv_half));
+            else v = (float)(spk_peak - (spk_peak - e_leak) * ((ls - tau_ref_0_5) * inv_half));
+        } else if ((ls - tau_ref) < dt_ms) {
+            v = e_leak;
+        }
+        if (mode == 0.0f && v != v) break;
+        t.push_back(mode);
+        t.push_back(v);
+        S += (double)dt_s;
+    }
+    return t;
 }
 
 // BDF coefficients (BDFn.h:604-669), row = method state (0..6)
@@ -364,6 +469,19 @@ namespace {
 
 using UpdateFn = void (*)(UpdateArgs);
 
+// Float parameters of each model's GetDefaultParameters() vector.
+int model_param_count(int model) {
+    static const int n[5] = {10, 14, 13, 0, 8};
+    return model >= 0 && model < 5 ? n[model] : -1;
+}
+
+// Neurons per thread of an ungrouped population's update kernel: the
+// granule's Runge-Kutta kernels hold two (19 state variables each), every
+// other kernel four.
+int plain_items(int model, int integrator) {
+    return (model == EDLUT_GRANULE && integrator != EDLUT_BDF) ? 2 : kItems;
+}
+
 template <class M, int I, unsigned MASK>
 void fill_one(UpdateFn *t) {
     t[(I * 16 + (int)MASK) * 2 + 0] = k_update<M, I, MASK, false, kItems, false>;
@@ -409,6 +527,23 @@ struct GroupedTable {
     }
 };
 
+// Granule and Purkinje cells: one kernel per integrator and drive (the
+// synapse types are a run-time mask), ITEMS neurons per thread.
+template <class M, int ITEMS>
+struct MaskFreeTable {
+    UpdateFn fn[4 * 2];
+    MaskFreeTable() {
+        fn[0] = k_update<M, kEuler, 15u, false, ITEMS, false>;
+        fn[1] = k_update<M, kEuler, 15u, true, ITEMS, false>;
+        fn[2] = k_update<M, kRK2, 15u, false, ITEMS, false>;
+        fn[3] = k_update<M, kRK2, 15u, true, ITEMS, false>;
+        fn[4] = k_update<M, kRK4, 15u, false, ITEMS, false>;
+        fn[5] = k_update<M, kRK4, 15u, true, ITEMS, false>;
+        fn[6] = k_update_bdf<M, 15u, false, float>;
+        fn[7] = k_update_bdf<M, 15u, true, float>;
+    }
+};
+
 template <class M, typename T, unsigned... MS>
 void fill_bdf(UpdateFn *t, std::integer_sequence<unsigned, MS...>) {
     ((t[MS * 2 + 0] = k_update_bdf<M, MS, false, T>, t[MS * 2 + 1] = k_update_bdf<M, MS, true, T>), ...);
@@ -435,6 +570,13 @@ UpdateFn pick_update(const edlut_population *p, bool ext) {
         if (model == EDLUT_ADEX) return adex_g.fn[idx];
         if (model == EDLUT_LIF) return lif_g.fn[idx];
         return nullptr;
+    }
+    if (model == EDLUT_GRANULE || model == EDLUT_PURKINJE) {
+        if (p->items != plain_items(model, integ)) return nullptr;
+        static MaskFreeTable<GranuleModel, 2> gr;
+        static MaskFreeTable<PurkinjeModel, kItems> pk;
+        const int idx = integ * 2 + (ext ? 1 : 0);
+        return model == EDLUT_GRANULE ? gr.fn[idx] : pk.fn[idx];
     }
     if (p->items != kItems) return nullptr;
     if (integ == EDLUT_BDF) {
@@ -648,6 +790,46 @@ void model_params(int model, int integrator, const float *x, float dt_ms, ModelP
         const float n_inf = c[4] / (c[4] + c[5]);
         init = {x[2], m_inf, h_inf, n_inf, 0, 0, 0, 0};
         for (int j = 0; j < 8; j++) q.init[j] = init[j];
+    } else if (model == EDLUT_GRANULE) {
+        // EgidioGranuleCell_TimeDriven.cpp:26-58,90-93,144-200,382-389: no free
+        // parameters; tau_exc 0.5, tau_inh 10, tau_nmda 15 ms (exp(-t/tau))
+        GranuleParams &q = mp.gran;
+        q.e_exc = e_exc = 0.0f;
+        q.e_inh = e_inh = -80.0f;
+        q.a_xkca = gr_q10(30.0f) * 2.5f;
+        q.b_xkca = gr_q10(30.0f) * 1.5f;
+        q.k_nernst = 1000 * (8.3134f * (30.0f + 273.15f) / 96485.309f) / 2.0f;
+        q.cf[0] = expf(-elapsed / 0.5f);
+        q.cf[1] = expf(-elapsed / 10.0f);
+        q.cf[2] = expf(-elapsed / 15.0f);
+        init = {-80.0f,       0.00047309535f, 1.0f,         0.00013423511f, 0.96227829f,
+                0.00050020111f, 0.010183001f, 0.15685486f,  0.53565367f,    0.37337035f,
+                0.00012384122f, 0.0021951104f, 0.89509747f, 0.00024031171f, 1e-04f,
+                0.0f,         0.0f,           0.0f,         0.0f};
+        for (int j = 0; j < 19; j++) q.init[j] = init[j];
+    } else if (model == EDLUT_PURKINJE) {
+        // e_exc e_inh v_thr e_leak tau_exc tau_inh tau_nmda tau_ref
+        // (TimeDrivenPurkinjeCell.cpp:342-404; constants :62-68)
+        PurkinjeParams &q = mp.pkj;
+        q.e_exc = e_exc = x[0];
+        q.e_inh = e_inh = x[1];
+        q.v_thr = x[2];
+        q.e_leak = x[3];
+        const float spk_peak = 31.0f;
+        const float tau_ref_0_5 = x[7] / 2;
+        const float inv_half = inv_of(tau_ref_0_5);
+        q.rise = (spk_peak - x[2]) * inv_half;
+        q.fall = (x[3] - spk_peak) * inv_half;
+        const float inv_area = 1.0f / (3.141592f * 0.0015f * 2.0f * 0.0008f);
+        q.cur_scale = 1e-6 * inv_area;
+        q.cf[0] = expf(-elapsed * inv_of(x[4]));
+        q.cf[1] = expf(-elapsed * inv_of(x[5]));
+        q.cf[2] = expf(-elapsed * inv_of(x[6]));
+        // :139-153: Ca and M start at their steady state at e_leak
+        const std::vector<float> chan = build_purkinje_table();
+        const float *c = &chan[(size_t)chan_index(x[3], kPkMinV, kPkAux) * 4];
+        init = {x[3], c[0] / c[1], c[2] / c[3], 0, 0, 0, 0};
+        for (int j = 0; j < 7; j++) q.init[j] = init[j];
     } else {
         LIFParams &q = mp.lif;
         q.e_exc = e_exc = x[0];
@@ -670,19 +852,19 @@ int create_population(int model, int integrator, int bdf_order, const float *par
                       edlut_population **out) {
     if (!out) return fail(EDLUT_E_ARG, "out is null");
     *out = nullptr;
-    if (model != EDLUT_ADEX && model != EDLUT_LIF && model != EDLUT_HH)
+    if (model < EDLUT_LIF || model > EDLUT_PURKINJE)
         return fail(EDLUT_E_ARG, "neuron model %d is not available in this build", model);
     if (integrator < EDLUT_EULER || integrator > EDLUT_BDF)
         return fail(EDLUT_E_ARG, "integration method %d is not available for this model", integrator);
     if (integrator == EDLUT_BDF && (bdf_order < 1 || bdf_order > 6))
         return fail(EDLUT_E_ARG, "bdf_order must be in [1, 6] (BDFn_GPU_C_INTERFACE.cuh:275-288)");
-    const int need = model == EDLUT_ADEX ? 14 : (model == EDLUT_HH ? 13 : 10);
-    if (!params || n_params != need)
+    const int need = model_param_count(model);
+    if (n_params != need || (need > 0 && !params))
         return fail(EDLUT_E_ARG, "model %d takes %d parameters, got %d", model, need, n_params);
     if (!(dt_s > 0.0f)) return fail(EDLUT_E_ARG, "integration step must be positive");
     if (n_groups <= 0 || !sizes) return fail(EDLUT_E_ARG, "no groups");
     if (grouped) {
-        if (model == EDLUT_HH || integrator == EDLUT_BDF)
+        if ((model != EDLUT_ADEX && model != EDLUT_LIF) || integrator == EDLUT_BDF)
             return fail(EDLUT_E_ARG, "grouped populations take AdEx or LIF under Euler, RK2 or RK4");
     }
     int64_t total = 0;
@@ -745,19 +927,29 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     std::vector<float> nmda = build_nmda_table(e_exc, e_inh, &nmda_aux);
     if (model == EDLUT_ADEX) p->mp.adex.nmda_aux = nmda_aux;
     else if (model == EDLUT_HH) p->mp.hh.nmda_aux = nmda_aux;
+    else if (model == EDLUT_GRANULE) p->mp.gran.nmda_aux = nmda_aux;
+    else if (model == EDLUT_PURKINJE) p->mp.pkj.nmda_aux = nmda_aux;
     else p->mp.lif.nmda_aux = nmda_aux;
     for (GroupInfo &g : gtab) {
         if (model == EDLUT_ADEX) g.mp.adex.nmda_aux = nmda_aux;
         else g.mp.lif.nmda_aux = nmda_aux;
     }
-    p->nv = model == EDLUT_ADEX ? 6 : (model == EDLUT_HH ? 8 : 5);
-    p->nd = model == EDLUT_ADEX ? 2 : (model == EDLUT_HH ? 4 : 1);
+    static const int kNv[5] = {5, 6, 8, 19, 7}, kNd[5] = {1, 2, 4, 15, 3};
+    p->nv = kNv[model];
+    p->nd = kNd[model];
+    // the granule's Runge-Kutta kernels hold two neurons per thread (19 state
+    // variables each); every other kernel holds four
+    p->items = plain_items(model, integrator);
     std::vector<float> chan;
     if (model == EDLUT_HH) chan = build_hh_table(params[5]);
+    if (model == EDLUT_GRANULE) chan = build_granule_table();
+    if (model == EDLUT_PURKINJE) chan = build_purkinje_table();
     std::vector<float> expt = build_exp_table();
     std::vector<float> refr;
     if (model == EDLUT_LIF) refr = build_refractory(dt_s, p->dt_ms, params[7]);
-    p->refr_len = (int)refr.size();
+    if (model == EDLUT_PURKINJE) refr = build_purkinje_clock(dt_s, p->dt_ms, params[2], params[3], params[7]);
+    // entries: LIF one fraction each, Purkinje a (mode, V) pair each
+    p->refr_len = (int)refr.size() / (model == EDLUT_PURKINJE ? 2 : 1);
     p->lut_sums[0] = fnv1a(expt.data(), expt.size() * sizeof(float));
     p->lut_sums[1] = fnv1a(nmda.data(), nmda.size() * sizeof(float));
     p->lut_sums[2] = fnv1a(refr.data(), refr.size() * sizeof(float));
@@ -778,7 +970,7 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     if ((rc = dalloc(&p->exp_lut, kExpSize))) return cleanup(rc);
     if ((rc = dalloc(&p->nmda_lut, kNmdaSize))) return cleanup(rc);
     if ((rc = dalloc(&p->refr, refr.size() + 1))) return cleanup(rc);
-    if (model == EDLUT_HH) {
+    if (!chan.empty()) {
         if ((rc = dalloc(&p->chan, chan.size()))) return cleanup(rc);
         if (hipMemcpy(p->chan, chan.data(), chan.size() * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
             return cleanup(fail(EDLUT_E_HIP, "hipMemcpy failed"));
@@ -818,7 +1010,7 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     if ((rc = dalloc(&p->tile_flag, max_tiles))) return cleanup(rc);
     if ((rc = dalloc(&p->bump, 2))) return cleanup(rc);
     if ((rc = p->reserve_events(1u << 16))) return cleanup(rc);
-    if (model == EDLUT_LIF) {
+    if (model == EDLUT_LIF || model == EDLUT_PURKINJE) {
         if ((rc = dalloc(&p->last_spike, p->stride))) return cleanup(rc);
         std::vector<int32_t> never(p->stride, -(1 << 30));
         if (hipMemcpy(p->last_spike, never.data(), sizeof(int32_t) * p->stride,
@@ -921,6 +1113,7 @@ int edlut_population_enable_synapses(edlut_population *p, uint32_t mask) {
     if (!p) return fail(EDLUT_E_ARG, "population is null");
     if (mask & ~15u) return fail(EDLUT_E_ARG, "unknown synapse type bits 0x%x", mask);
     p->mask |= mask;
+    p->mp.gran.mask = p->mp.pkj.mask = p->mask;   // run-time mask of these two models
     return 0;
 }
 
@@ -1787,8 +1980,9 @@ int edlut_network_reset(edlut_network *n) {
 int edlut_network_set_launch(edlut_network *n, int items, int fold_lanes, int fold_blocks) {
     if (!n) return fail(EDLUT_E_ARG, "network is null");
     edlut_population *p = n->pop;
-    if (p->grouped ? (items != 2 && items != 4) : items != kItems)
-        return fail(EDLUT_E_ARG, "items per thread must be %s", p->grouped ? "2 or 4" : "4");
+    const int fixed = plain_items(p->model, p->integ);
+    if (p->grouped ? (items != 2 && items != 4) : items != fixed)
+        return fail(EDLUT_E_ARG, "items per thread must be %s", p->grouped ? "2 or 4" : (fixed == 2 ? "2" : "4"));
     if (!pick_ring_fold(kCount, fold_lanes)) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
     if (fold_blocks < 1 || fold_blocks > 65536) return fail(EDLUT_E_ARG, "fold blocks must be in [1, 65536]");
     p->items = items;
@@ -1815,7 +2009,7 @@ int edlut_network_autotune(edlut_network *n, int trial_steps, int *items, int *f
     if (trial_steps < 16 || trial_steps > 100000) return fail(EDLUT_E_ARG, "trial_steps must be in [16, 100000]");
     edlut_population *p = n->pop;
     HIP_TRY(hipSetDevice(p->device));
-    std::vector<int> it_c = p->grouped ? std::vector<int>{2, 4} : std::vector<int>{kItems};
+    std::vector<int> it_c = p->grouped ? std::vector<int>{2, 4} : std::vector<int>{plain_items(p->model, p->integ)};
     const int lanes_c[] = {16, 32, 64};
     const int blocks_c[] = {64, 256, 1024, 4096};
     hipEvent_t e0, e1;

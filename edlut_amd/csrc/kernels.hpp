@@ -204,12 +204,12 @@ struct UpdateArgs {
     StepRef step;          // update index k (>= 1)
     ModelParams mp;
     StepConsts k;
-    const float *chan;     // HH channel-rate table
+    const float *chan;     // channel-rate table (HH, granule, Purkinje)
     const float *exp_lut;
     const float *nmda_lut;
     const float *refr;
     int refr_len;
-    int32_t *last_spike;   // LIF only: update index of the last spike
+    int32_t *last_spike;   // LIF, Purkinje: update index of the last spike
     BdfState bdf;          // BDF integrator history
     ExtDrive ext;
     TileOut out;
@@ -389,6 +389,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
         const uint32_t i = (uint32_t)idx[n];
         spk[n] = live[n] && M::spike(mp, prev_v[n], s[n]);
         if (!live[n]) continue;
+        if constexpr (M::kClockOverride) M::clock_override(c[n], spk[n], s[n]);
         if (spk[n] && a.last_spike) a.last_spike[i] = step;
         if (s[n][0] != s[n][0]) {
             // TimeDrivenNeuronModel.cpp:75-88: NaN -> whole state to initial.
@@ -482,6 +483,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         int st = a.bdf.st[i];
         integrate_bdf<M, MASK, T>(a.mp, c, (T)a.k.dt, a.bdf, ld, i, st, S);
         const bool fired = M::spike(a.mp, prev, S);
+        if constexpr (M::kClockOverride) M::clock_override(c, fired, S);
         if (fired && M::kResetsIntegrator) st = 0;
         a.bdf.st[i] = st;
         spk[n] = fired;

@@ -260,4 +260,22 @@ class HHTimeDrivenModel_GPU : public TimeDrivenNeuronModel_GPU {
     void SetNewtonFP64(bool on) { check(edlut_population_set_newton_fp64(handle(), on ? 1 : 0)); }
 };
 
+// EgidioGranuleCell_TimeDriven_GPU: fixed constants, no parameters besides
+// the integration method (EgidioGranuleCell_TimeDriven.cpp:26-58).
+class EgidioGranuleCell_TimeDriven_GPU : public TimeDrivenNeuronModel_GPU {
+  public:
+    EgidioGranuleCell_TimeDriven_GPU() : TimeDrivenNeuronModel_GPU(EDLUT_GRANULE, {}) {}
+    std::string GetName() const override { return "EgidioGranuleCell_TimeDriven_GPU"; }
+};
+
+// TimeDrivenPurkinjeCell_GPU (defaults of TimeDrivenPurkinjeCell.cpp:423-434).
+class TimeDrivenPurkinjeCell_GPU : public TimeDrivenNeuronModel_GPU {
+  public:
+    TimeDrivenPurkinjeCell_GPU()
+        : TimeDrivenNeuronModel_GPU(EDLUT_PURKINJE, {{"e_exc", 0.0f}, {"e_inh", -80.0f}, {"v_thr", -35.0f},
+                                                     {"e_leak", -70.0f}, {"tau_exc", 1.0f}, {"tau_inh", 2.0f},
+                                                     {"tau_nmda", 20.0f}, {"tau_ref", 1.35f}}) {}
+    std::string GetName() const override { return "TimeDrivenPurkinjeCell_GPU"; }
+};
+
 }  // namespace edlut_amd

@@ -26,6 +26,8 @@ static std::unique_ptr<TimeDrivenNeuronModel_GPU> make(const std::string &name) 
     if (name == "AdExTimeDrivenModel_GPU") return std::make_unique<AdExTimeDrivenModel_GPU>();
     if (name == "LIFTimeDrivenModel_GPU") return std::make_unique<LIFTimeDrivenModel_GPU>();
     if (name == "HHTimeDrivenModel_GPU") return std::make_unique<HHTimeDrivenModel_GPU>();
+    if (name == "EgidioGranuleCell_TimeDriven_GPU") return std::make_unique<EgidioGranuleCell_TimeDriven_GPU>();
+    if (name == "TimeDrivenPurkinjeCell_GPU") return std::make_unique<TimeDrivenPurkinjeCell_GPU>();
     throw EDLUTException(EDLUT_E_ARG, "unknown neuron model " + name);
 }
 

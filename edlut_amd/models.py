@@ -361,11 +361,48 @@ class HHTimeDrivenModel_GPU(TimeDrivenNeuronModel_GPU):
         C.check(C.lib().edlut_population_set_newton_fp64(self._pop, int(bool(on))))
 
 
+class EgidioGranuleCell_TimeDriven_GPU(TimeDrivenNeuronModel_GPU):
+    """Cerebellar granule cell with fifteen differential equations
+    (Kernel/src/neuron_model/EgidioGranuleCell_TimeDriven_GPU_C_INTERFACE.cu;
+    CPU model EgidioGranuleCell_TimeDriven.cpp).  Its constants are fixed: the
+    model has no parameters besides the integration method."""
+
+    NAME = "EgidioGranuleCell_TimeDriven_GPU"
+    MODEL = C.EDLUT_GRANULE
+    PARAMS = ()
+    STATE_VARS = ("V_m", "xNa_f", "yNa_f", "xNa_r", "yNa_r", "xNa_p", "xK_V", "xK_A", "yK_A", "xK_IR",
+                  "xK_Ca", "xCa", "yCa", "xK_sl", "Ca", "g_exc", "g_inh", "g_nmda", "ext_I")
+    N_DIFF = 15
+
+
+class TimeDrivenPurkinjeCell_GPU(TimeDrivenNeuronModel_GPU):
+    """Purkinje cell with calcium and muscarinic currents and a stereotyped
+    spike waveform (Kernel/src/neuron_model/TimeDrivenPurkinjeCell_GPU_C_INTERFACE.cu;
+    defaults of TimeDrivenPurkinjeCell.cpp:423-434)."""
+
+    NAME = "TimeDrivenPurkinjeCell_GPU"
+    MODEL = C.EDLUT_PURKINJE
+    PARAMS = (
+        ("e_exc", 0.0),
+        ("e_inh", -80.0),
+        ("v_thr", -35.0),
+        ("e_leak", -70.0),
+        ("tau_exc", 1.0),
+        ("tau_inh", 2.0),
+        ("tau_nmda", 20.0),
+        ("tau_ref", 1.35),
+    )
+    STATE_VARS = ("V_m", "Ca", "M", "g_exc", "g_inh", "g_nmda", "ext_I")
+    N_DIFF = 3
+
+
 # NeuronModelFactory registry (Kernel/src/neuron_model/NeuronModelFactory.cu:66-91)
 REGISTRY = {
     LIFTimeDrivenModel_GPU.NAME: LIFTimeDrivenModel_GPU,
     AdExTimeDrivenModel_GPU.NAME: AdExTimeDrivenModel_GPU,
     HHTimeDrivenModel_GPU.NAME: HHTimeDrivenModel_GPU,
+    EgidioGranuleCell_TimeDriven_GPU.NAME: EgidioGranuleCell_TimeDriven_GPU,
+    TimeDrivenPurkinjeCell_GPU.NAME: TimeDrivenPurkinjeCell_GPU,
 }
 
 

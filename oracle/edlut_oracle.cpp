@@ -447,6 +447,375 @@ template <typename R> struct HH : PopBase<R> {
 };
 
 /* ------------------------------------------------------------------------ */
+/* logf of the granule's Nernst potential.  The reference calls glibc's      */
+/* logf (EgidioGranuleCell_TimeDriven.h:157-159, float overload of log under */
+/* `using namespace std`).  That routine is not correctly rounded in every   */
+/* case and has no published bit-exact description, so the restatement     */
+/* uses the correctly rounded logarithm instead: ln x in double from         */
+/* x = 2^e m (m in [sqrt(1/2), sqrt(2))), ln m = 2 atanh((m-1)/(m+1)) by its */
+/* series through s^23, e ln 2 with a split ln 2, rounded to float.  The    */
+/* pin tests measure how often glibc's logf differs from it (by one ulp).   */
+/* ------------------------------------------------------------------------ */
+float log_cr(float x) {
+    if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
+    if (std::isinf(x)) return x;
+    uint32_t b;
+    std::memcpy(&b, &x, 4);
+    int e = 0;
+    if (b < 0x00800000u) {
+        x *= 8388608.0f;
+        std::memcpy(&b, &x, 4);
+        e = -23;
+    }
+    e += (int)(b >> 23) - 127;
+    const uint32_t mb = (b & 0x007fffffu) | 0x3f800000u;
+    float mf;
+    std::memcpy(&mf, &mb, 4);
+    double m = mf;
+    if (m > 1.4142135623730951) {
+        m *= 0.5;
+        e += 1;
+    }
+    const double s = (m - 1.0) / (m + 1.0), s2 = s * s;
+    static const double inv_odd[12] = {1.0 / 23.0, 1.0 / 21.0, 1.0 / 19.0, 1.0 / 17.0,
+                                       1.0 / 15.0, 1.0 / 13.0, 1.0 / 11.0, 1.0 / 9.0,
+                                       1.0 / 7.0,  1.0 / 5.0,  1.0 / 3.0,  1.0};
+    double p = inv_odd[0];
+    for (int k = 1; k < 12; k++) p = p * s2 + inv_odd[k];
+    const double lm = (2.0 * s) * p;
+    const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+    return (float)((double)e * ln2_hi + ((double)e * ln2_lo + lm));
+}
+
+/* ------------------------------------------------------------------------ */
+/* Egidio granule cell: Kernel/src/neuron_model/EgidioGranuleCell_TimeDriven */
+/* .cpp, Kernel/include/neuron_model/EgidioGranuleCell_TimeDriven.h.        */
+/* state: V, xNa_f, yNa_f, xNa_r, yNa_r, xNa_p, xK_V, xK_A, yK_A, xK_IR,      */
+/*        xK_Ca, xCa, yCa, xK_sl, Ca | g_exc, g_inh, g_nmda, ext_I           */
+/* Millisecond model with fixed constants (.cpp:26-58).  Channel rates come  */
+/* from a 1024 x 26 table on [-100, 50] mV (.h:402-552, float exp).          */
+/* ------------------------------------------------------------------------ */
+namespace gr {
+const float gMAXNa_f = 0.013f, gMAXNa_r = 0.0005f, gMAXNa_p = 0.0002f, gMAXK_V = 0.003f,
+            gMAXK_A = 0.004f, gMAXK_IR = 0.0009f, gMAXK_Ca = 0.004f, gMAXCa = 0.00046f,
+            gMAXK_sl = 0.00035f, gLkg1 = 5.68e-5f, gLkg2 = 2.17e-5f, VNa = 87.39f, VK = -84.69f,
+            VLkg1 = -58.0f, VLkg2 = -65.0f, V0_xK_Ai = -46.7f, K_xK_Ai = -19.8f,
+            V0_yK_Ai = -78.8f, K_yK_Ai = 8.4f, V0_xK_sli = -30.0f, B_xK_sli = 6.0f,
+            F = 96485.309f, A = 1e-04f, d = 0.2f, betaCa = 1.5f, Ca0 = 1e-04f, R = 8.3134f,
+            cao = 2.0f, Cm = 1.0e-3f, inv_Cm = 1.0f / 1.0e-3f, temper = 30.0f;
+const int kTable = 1024, kVars = 26;
+const float Max_V = 50.0f, Min_V = -100.0f;
+const float aux = (kTable - 1) / (Max_V - Min_V);
+inline float q10(float t0) { return std::pow(3.0f, (temper - t0) / 10.0f); }
+/* .h:170-178 */
+inline float linoid(float x, float y) {
+    if (std::fabs(x / y) < 1e-06f) return y * (1 - x / y / 2);
+    return x / (std::exp(x / y) - 1);
+}
+}  // namespace gr
+
+struct GranuleTable {
+    std::vector<float> v;
+    void build() {
+        using namespace gr;
+        const float Q10_20 = q10(20.0f), Q10_22 = q10(22.0f), Q10_30 = q10(30.0f), Q10_6_3 = q10(6.3f);
+        v.assign((size_t)kTable * kVars, 0.0f);
+        for (int i = 0; i < kTable; i++) {
+            const float V = Min_V + ((Max_V - Min_V) * i) / (kTable - 1);
+            float *t = &v[(size_t)i * kVars];
+            float a, b;
+            a = Q10_20 * (-0.3f) * linoid(V + 19.0f, -10.0f);
+            b = Q10_20 * 12.0f * std::exp(-(V + 44.0f) / 18.182f);
+            t[0] = a / (a + b); t[1] = a + b;
+            a = Q10_20 * 0.105f * std::exp(-(V + 44.0f) / 3.333f);
+            b = Q10_20 * 1.5f / (1.0f + std::exp(-(V + 11.0f) / 5.0f));
+            t[2] = a / (a + b); t[3] = a + b;
+            a = Q10_20 * (0.00008f - 0.00493f * linoid(V - 4.48754f, -6.81881f));
+            b = Q10_20 * (0.04752f + 0.01558f * linoid(V + 43.97494f, 0.10818f));
+            t[4] = a / (a + b); t[5] = a + b;
+            a = Q10_20 * 0.31836f * std::exp(-(V + 80.0f) / 62.52621f);
+            b = Q10_20 * 0.01014f * std::exp((V + 83.3332f) / 16.05379f);
+            t[6] = a / (a + b); t[7] = a + b;
+            t[8] = 1.0f / (1.0f + std::exp(-(V + 42.0f) / 5.0f));
+            a = Q10_30 * (-0.091f) * linoid(V + 42.0f, -5.0f);
+            b = Q10_30 * 0.062f * linoid(V + 42.0f, 5.0f);
+            t[9] = (a + b) * 0.2f;
+            a = Q10_6_3 * (-0.01f) * linoid(V + 25.0f, -10.0f);
+            b = Q10_6_3 * 0.125f * std::exp(-0.0125f * (V + 35.0f));
+            t[10] = a / (a + b); t[11] = a + b;
+            t[12] = 1.0f / (1.0f + std::exp((V - V0_xK_Ai) / K_xK_Ai));
+            a = (Q10_20 * 4.88826f) / (1 + std::exp(-(V + 9.17203f) / 23.32708f));
+            b = (Q10_20 * 0.99285f) / std::exp((V + 18.27914f) / 19.47175f);
+            t[13] = a + b;
+            t[14] = 1.0f / (1.0f + std::exp((V - V0_yK_Ai) / K_yK_Ai));
+            a = (Q10_20 * 0.11042f) / (1.0f + std::exp((V + 111.33209f) / 12.8433f));
+            b = (Q10_20 * 0.10353f) / (1.0f + std::exp(-(V + 49.9537f) / 8.90123f));
+            t[15] = a + b;
+            a = Q10_20 * 0.13289f * std::exp(-(V + 83.94f) / 24.3902f);
+            b = Q10_20 * 0.16994f * std::exp((V + 83.94f) / 35.714f);
+            t[16] = a / (a + b); t[17] = a + b;
+            t[18] = 0.0015f * std::exp(-V / 11.765f);
+            t[19] = 1.0f / (0.00015f * std::exp(-V / 11.765f));
+            a = Q10_20 * 0.04944f * std::exp((V + 29.06f) / 15.87301587302f);
+            b = Q10_20 * 0.08298f * std::exp(-(V + 18.66f) / 25.641f);
+            t[20] = a / (a + b); t[21] = a + b;
+            a = Q10_20 * 0.0013f * std::exp(-(V + 48.0f) / 18.183f);
+            b = Q10_20 * 0.0013f * std::exp((V + 48.0f) / 83.33f);
+            t[22] = a / (a + b); t[23] = a + b;
+            t[24] = 1.0f / (1.0f + std::exp(-(V - V0_xK_sli) / B_xK_sli));
+            a = Q10_22 * 0.0033f * std::exp((V + 30.0f) / 40.0f);
+            b = Q10_22 * 0.0033f * (-(V + 30.0f) / 20.0f);
+            t[25] = a + b;
+        }
+    }
+    /* Get_channel_values (.h:563-571) */
+    const float *at(float V) const {
+        int position = trunc_i32_x86((V - gr::Min_V) * gr::aux);
+        if (position < 0) position = 0;
+        else if (position > gr::kTable - 1) position = gr::kTable - 1;
+        return &v[(size_t)position * gr::kVars];
+    }
+};
+
+template <typename R> struct Granule : PopBase<R> {
+    const float e_exc = 0.0f, e_inh = -80.0f, tau_exc = 0.5f, tau_inh = 10.0f, tau_nmda = 15.0f,
+                v_thr = 0.0f;
+    float Q10_30 = 1.0f;
+    NmdaTable nmda;
+    GranuleTable table;
+    R cond[3];
+
+    void set_params(const float *) {
+        Q10_30 = gr::q10(30.0f);
+        nmda.build(e_exc, e_inh);
+        table.build();
+        this->nvars = 19;
+        this->ndiff = 15;
+    }
+    R g_nmda_inf(R V) const { return (R)nmda.get((float)V, e_inh); }
+    /* .cpp:176-235 */
+    void init_states() {
+        const R init[19] = {-80.0f, 0.00047309535f, 1.0f, 0.00013423511f, 0.96227829f,
+                            0.00050020111f, 0.010183001f, 0.15685486f, 0.53565367f, 0.37337035f,
+                            0.00012384122f, 0.0021951104f, 0.89509747f, 0.00024031171f, gr::Ca0,
+                            0.0f, 0.0f, 0.0f, 0.0f};
+        PopBase<R>::init_states(init);
+    }
+    /* .cpp:382-389: exp(-t/tau) (float division, float exp) */
+    void calc_cond(R elapsed) {
+        if (sizeof(R) == sizeof(float)) {
+            cond[0] = std::exp(-(float)elapsed / tau_exc);
+            cond[1] = std::exp(-(float)elapsed / tau_inh);
+            cond[2] = expf(-(float)elapsed / tau_nmda);
+        } else {
+            cond[0] = std::exp(-(double)elapsed / tau_exc);
+            cond[1] = std::exp(-(double)elapsed / tau_inh);
+            cond[2] = std::exp(-(double)elapsed / tau_nmda);
+        }
+    }
+    /* nernst(Ca, cao, 2, temper) (.h:157-159) */
+    R nernst(R ci) const {
+        using namespace gr;
+        const float k = 1000 * (R_gas() * (temper + 273.15f) / F) / 2.0f;
+        if (sizeof(R) == sizeof(float)) return (R)(k * log_cr(std::fabs(cao / (float)ci)));
+        return (R)k * std::log(std::fabs((double)cao / (double)ci));
+    }
+    static float R_gas() { return gr::R; }
+    /* .cpp:252-365 */
+    void deriv(R *S, R *D, int /*index*/, R /*elapsed*/) const {
+        using namespace gr;
+        R current = 0.0;
+        if (this->EXC) current += S[15] * (e_exc - S[0]);
+        if (this->INH) current += S[16] * (e_inh - S[0]);
+        if (this->NMDA) {
+            R ginf = g_nmda_inf(S[0]);
+            current += S[17] * ginf * (e_exc - S[0]);
+        }
+        current += S[18];
+        current = (R)(current * 3.34156941e-4);   /* float *= double literal */
+        const R pV = S[0];
+        const R VCa = nernst(S[14]);
+        const float *values = table.at((float)pV);
+        const R alpha_xK_Ca = (Q10_30 * 2.5f) / (1.0f + values[18] / S[14]);
+        const R beta_xK_Ca = (Q10_30 * 1.5f) / (1.0f + S[14] * values[19]);
+        const R xK_Ca_inf = alpha_xK_Ca / (alpha_xK_Ca + beta_xK_Ca);
+        const R inv_tau_xK_Ca = (alpha_xK_Ca + beta_xK_Ca);
+        const R gNa_f = gMAXNa_f * S[1] * S[1] * S[1] * S[2];
+        const R gNa_r = gMAXNa_r * S[3] * S[4];
+        const R gNa_p = gMAXNa_p * S[5];
+        const R gK_V = gMAXK_V * S[6] * S[6] * S[6] * S[6];
+        const R gK_A = gMAXK_A * S[7] * S[7] * S[7] * S[8];
+        const R gK_IR = gMAXK_IR * S[9];
+        const R gK_Ca = gMAXK_Ca * S[10];
+        const R gCa = gMAXCa * S[11] * S[11] * S[12];
+        const R gK_sl = gMAXK_sl * S[13];
+        D[1] = (values[0] - S[1]) * values[1];
+        D[2] = (values[2] - S[2]) * values[3];
+        D[3] = (values[4] - S[3]) * values[5];
+        D[4] = (values[6] - S[4]) * values[7];
+        D[5] = (values[8] - S[5]) * values[9];
+        D[6] = (values[10] - S[6]) * values[11];
+        D[7] = (values[12] - S[7]) * values[13];
+        D[8] = (values[14] - S[8]) * values[15];
+        D[9] = (values[16] - S[9]) * values[17];
+        D[10] = (xK_Ca_inf - S[10]) * inv_tau_xK_Ca;
+        D[11] = (values[20] - S[11]) * values[21];
+        D[12] = (values[22] - S[12]) * values[23];
+        D[13] = (values[24] - S[13]) * values[25];
+        D[14] = (-gCa * (pV - VCa) / (2 * F * A * d) - (betaCa * (S[14] - Ca0)));
+        D[0] = (current + gNa_f * (VNa - pV) + gNa_r * (VNa - pV) + gNa_p * (VNa - pV) +
+                gK_V * (VK - pV) + gK_A * (VK - pV) + gK_IR * (VK - pV) + gK_Ca * (VK - pV) +
+                gCa * (VCa - pV) + gK_sl * (VK - pV) + gLkg1 * (VLkg1 - pV) + gLkg2 * (VLkg2 - pV)) *
+               inv_Cm;
+    }
+    /* .cpp:368-380 */
+    void decay(R *S) const {
+        const R limit = (R)1e-9f;
+        if (this->EXC) { if (S[15] < limit) S[15] = 0.0f; else S[15] *= cond[0]; }
+        if (this->INH) { if (S[16] < limit) S[16] = 0.0f; else S[16] *= cond[1]; }
+        if (this->NMDA) { if (S[17] < limit) S[17] = 0.0f; else S[17] *= cond[2]; }
+    }
+    /* .cpp:243-249: upward crossing of v_thr, no reset */
+    bool spike(R prev_V, R *S, int index) {
+        if (S[0] > v_thr && prev_V < v_thr) {
+            this->fired(index);
+            return true;
+        }
+        return false;
+    }
+};
+
+/* ------------------------------------------------------------------------ */
+/* Purkinje cell: Kernel/src/neuron_model/TimeDrivenPurkinjeCell.cpp,        */
+/* Kernel/include/neuron_model/TimeDrivenPurkinjeCell.h:396-441.             */
+/* state: V, ca, M | g_exc, g_inh, g_nmda, ext_I                             */
+/* A spike starts a stereotyped waveform (linear rise to spk_peak, linear    */
+/* fall to e_leak over tau_ref) timed by the double LastSpikeTime.           */
+/* ------------------------------------------------------------------------ */
+namespace pk {
+const int kTable = 1024;
+const float Max_V = 35.0f, Min_V = -100.0f;
+const float aux = (kTable - 1) / (Max_V - Min_V);
+}  // namespace pk
+
+struct PurkinjeTable {
+    std::vector<float> v;
+    void build() {
+        v.assign((size_t)pk::kTable * 4, 0.0f);
+        for (int i = 0; i < pk::kTable; i++) {
+            const float V = pk::Min_V + ((pk::Max_V - pk::Min_V) * i) / (pk::kTable - 1);
+            const float alpha_ca = 1.6f / (1 + std::exp(-0.072f * (V - 5.0f)));
+            const float beta_ca = (0.02f * (V + 8.9f)) / (std::exp((V + 8.9f) * 0.2f) - 1.0f);
+            const float alpha_M = 0.3f / (1 + std::exp((-V - 2.0f) * 0.2f));
+            const float beta_M = 0.001f * std::exp((-V - 60.0f) * 0.055555555555555f);
+            float *t = &v[(size_t)i * 4];
+            t[0] = alpha_ca;
+            t[1] = alpha_ca + beta_ca;
+            t[2] = alpha_M;
+            t[3] = alpha_M + beta_M;
+        }
+    }
+    const float *at(float V) const {
+        int position = trunc_i32_x86((V - pk::Min_V) * pk::aux);
+        if (position < 0) position = 0;
+        else if (position > pk::kTable - 1) position = pk::kTable - 1;
+        return &v[(size_t)position * 4];
+    }
+};
+
+template <typename R> struct Purkinje : PopBase<R> {
+    const float g_leak = 0.02f, g_Ca = 0.001f, g_M = 0.75f, c_m = 0.95f, inv_c_m = 1.0f / 0.95f,
+                spk_peak = 31.0f;
+    const float inv_area = 1.0f / (3.141592f * 0.0015f * 2.0f * 0.0008f);
+    const float time_scale = 1000.0f;
+    float e_exc, e_inh, v_thr, e_leak, tau_exc, inv_tau_exc, tau_inh, inv_tau_inh, tau_nmda,
+        inv_tau_nmda, tau_ref, tau_ref_0_5, inv_tau_ref_0_5;
+    NmdaTable nmda;
+    PurkinjeTable table;
+    R cond[3];
+
+    /* Parameters (order of p[]): e_exc e_inh v_thr e_leak tau_exc tau_inh
+     * tau_nmda tau_ref (.cpp:342-404, defaults :423-434) */
+    void set_params(const float *p) {
+        e_exc = p[0]; e_inh = p[1]; v_thr = p[2]; e_leak = p[3];
+        tau_exc = p[4]; inv_tau_exc = (float)(1.0 / p[4]);
+        tau_inh = p[5]; inv_tau_inh = (float)(1.0 / p[5]);
+        tau_nmda = p[6]; inv_tau_nmda = (float)(1.0 / p[6]);
+        tau_ref = p[7]; tau_ref_0_5 = p[7] / 2; inv_tau_ref_0_5 = (float)(1.0 / tau_ref_0_5);
+        nmda.build(e_exc, e_inh);
+        table.build();
+        this->nvars = 7;
+        this->ndiff = 3;
+    }
+    R g_nmda_inf(R V) const { return (R)nmda.get((float)V, e_inh); }
+    /* .cpp:139-161 */
+    void init_states() {
+        const float *values = table.at(e_leak);
+        const float c_inf = values[0] / values[1];
+        const float M_inf = values[2] / values[3];
+        const R init[7] = {e_leak, c_inf, M_inf, 0, 0, 0, 0};
+        PopBase<R>::init_states(init);
+    }
+    /* .cpp:270-274 */
+    void calc_cond(R elapsed) {
+        cond[0] = Num<R>::decay(elapsed, inv_tau_exc);
+        cond[1] = Num<R>::decay(elapsed, inv_tau_inh);
+        cond[2] = Num<R>::decay(elapsed, inv_tau_nmda);
+    }
+    /* .cpp:193-239 */
+    void deriv(R *S, R *D, int index, R /*elapsed*/) const {
+        const R V = S[0], ca = S[1], M = S[2];
+        const float last_spike = (float)(time_scale * this->last_spike[index]);
+        R current = 0.0;
+        if (this->EXC) current += S[3] * (e_exc - S[0]);
+        if (this->INH) current += S[4] * (e_inh - S[0]);
+        if (this->NMDA) {
+            R ginf = g_nmda_inf(S[0]);
+            current += S[5] * ginf * (e_exc - S[0]);
+        }
+        current += S[6];
+        current = (R)(current * (1e-6 * inv_area));
+        if (last_spike >= tau_ref) {
+            D[0] = (-g_leak * (V + 70.0f) - g_Ca * ca * ca * (V - 125.0f) - g_M * M * (V + 95.0f) + current) *
+                   inv_c_m;
+        } else if (last_spike <= tau_ref_0_5) {
+            D[0] = (spk_peak - v_thr) * inv_tau_ref_0_5;
+        } else {
+            D[0] = (e_leak - spk_peak) * inv_tau_ref_0_5;
+        }
+        const float *values = table.at((float)V);
+        D[1] = values[0] - ca * values[1];
+        D[2] = values[2] - M * values[3];
+    }
+    /* .cpp:241-268 */
+    void decay(R *S) const {
+        const R limit = (R)1e-9f;
+        if (this->EXC) { if (S[3] < limit) S[3] = 0.0f; else S[3] *= cond[0]; }
+        if (this->INH) { if (S[4] < limit) S[4] = 0.0f; else S[4] *= cond[1]; }
+        if (this->NMDA) { if (S[5] < limit) S[5] = 0.0f; else S[5] *= cond[2]; }
+    }
+    /* .cpp:172-190: the crossing fires; then the clock (advanced this step,
+     * or restarted by the spike) imposes the waveform on V. */
+    bool spike(R prev_V, R *S, int index) {
+        bool f = false;
+        if (S[0] >= v_thr && prev_V < v_thr) {
+            this->fired(index);
+            f = true;
+        }
+        const double last_spike = this->last_spike[index] * time_scale;
+        if (last_spike < tau_ref) {
+            if (last_spike <= tau_ref_0_5)
+                S[0] = (R)(v_thr + (spk_peak - v_thr) * (last_spike * inv_tau_ref_0_5));
+            else
+                S[0] = (R)(spk_peak - (spk_peak - e_leak) * ((last_spike - tau_ref_0_5) * inv_tau_ref_0_5));
+        } else if ((last_spike - tau_ref) < this->dt_model) {
+            S[0] = e_leak;
+        }
+        return f;
+    }
+};
+
+/* ------------------------------------------------------------------------ */
 /* Fixed-step integrators over the whole population.                        */
 /* ------------------------------------------------------------------------ */
 const int kMaxVars = 32;
@@ -894,6 +1263,14 @@ PopIface *make_pop(int model, int integ, int bdf_order, const float *p, float dt
         if (fp64) return new Pop<HH<double>, double>(integ, bdf_order, p, dt_s, n, mask);
         return new Pop<HH<float>, float>(integ, bdf_order, p, dt_s, n, mask);
     }
+    if (model == ORC_GRANULE) {
+        if (fp64) return new Pop<Granule<double>, double>(integ, bdf_order, p, dt_s, n, mask);
+        return new Pop<Granule<float>, float>(integ, bdf_order, p, dt_s, n, mask);
+    }
+    if (model == ORC_PURKINJE) {
+        if (fp64) return new Pop<Purkinje<double>, double>(integ, bdf_order, p, dt_s, n, mask);
+        return new Pop<Purkinje<float>, float>(integ, bdf_order, p, dt_s, n, mask);
+    }
     return nullptr;
 }
 
@@ -918,6 +1295,22 @@ void orc_hh_table(float v_thr, float *out) {
     HHTable t;
     t.build(v_thr);
     std::memcpy(out, t.v.data(), sizeof(float) * t.v.size());
+}
+
+void orc_granule_table(float *out) {
+    GranuleTable t;
+    t.build();
+    std::memcpy(out, t.v.data(), sizeof(float) * t.v.size());
+}
+
+void orc_purkinje_table(float *out) {
+    PurkinjeTable t;
+    t.build();
+    std::memcpy(out, t.v.data(), sizeof(float) * t.v.size());
+}
+
+void orc_log_cr(const float *x, float *out, int n) {
+    for (int i = 0; i < n; i++) out[i] = log_cr(x[i]);
 }
 
 void orc_lcg_seeds(uint32_t seed, int n, uint32_t *out) {

@@ -40,6 +40,13 @@ void orc_nmda_table(float e_exc, float e_inh, float *out);
  * alpha_n, beta_n} on [-90, 50] mV) for the given v_thr. */
 void orc_hh_table(float v_thr, float *out);
 
+/* Granule channel table (1024 x 26 on [-100, 50] mV) and Purkinje channel
+ * table (1024 x {alpha_ca, inv_tau_ca, alpha_M, inv_tau_M} on [-100, 35] mV). */
+void orc_granule_table(float *out);
+void orc_purkinje_table(float *out);
+/* The correctly rounded float logarithm the granule restatement uses. */
+void orc_log_cr(const float *x, float *out, int n);
+
 /* ---- RandomGenerator / Poisson device ----------------------------------- */
 /* Seeds handed to n successive RandomGenerator objects after srand(seed). */
 void orc_lcg_seeds(uint32_t seed, int n, uint32_t *out);

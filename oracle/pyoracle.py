@@ -16,7 +16,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_DUMP = os.path.join(HERE, "_ref", "ref_dump")
 
-MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1, "HHTimeDrivenModel_GPU": 2}
+MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1, "HHTimeDrivenModel_GPU": 2,
+          "EgidioGranuleCell_TimeDriven_GPU": 3, "TimeDrivenPurkinjeCell_GPU": 4}
 INTEGRATORS = {"Euler": 0, "RK2": 1, "RK4": 2, "BDF": 3}
 
 P = ctypes.c_void_p
@@ -55,6 +56,9 @@ def lib():
         l.orc_exp_lut.argtypes = [P, P, ctypes.c_int]
         l.orc_nmda_table.argtypes = [ctypes.c_float, ctypes.c_float, P]
         l.orc_hh_table.argtypes = [ctypes.c_float, P]
+        l.orc_granule_table.argtypes = [P]
+        l.orc_purkinje_table.argtypes = [P]
+        l.orc_log_cr.argtypes = [P, P, ctypes.c_int]
         l.orc_lcg_seeds.argtypes = [ctypes.c_uint32, ctypes.c_int, P]
         l.orc_lcg_frand.argtypes = [ctypes.c_uint32, ctypes.c_int, P]
         l.orc_drive_hash.restype = ctypes.c_uint32
@@ -109,6 +113,25 @@ def hh_table(v_thr: float) -> np.ndarray:
     t = np.zeros((262144, 6), np.float32)
     lib().orc_hh_table(v_thr, _p(t))
     return t
+
+
+def granule_table() -> np.ndarray:
+    t = np.zeros((1024, 26), np.float32)
+    lib().orc_granule_table(_p(t))
+    return t
+
+
+def purkinje_table() -> np.ndarray:
+    t = np.zeros((1024, 4), np.float32)
+    lib().orc_purkinje_table(_p(t))
+    return t
+
+
+def log_cr(x) -> np.ndarray:
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.zeros_like(x)
+    lib().orc_log_cr(_p(x), _p(out), len(x))
+    return out
 
 
 def lcg_seeds(seed: int, n: int) -> np.ndarray:
