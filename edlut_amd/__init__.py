@@ -16,7 +16,7 @@ from .models import (
     TimeDrivenPurkinjeCell_GPU,
     integration_method,
 )
-from .network import Network, binomial_thresholds, build_out_csr, from_spec
+from .network import Circuit, Network, binomial_thresholds, build_out_csr, from_spec
 
 __all__ = [
     "EDLUTException",
@@ -31,6 +31,7 @@ __all__ = [
     "REGISTRY",
     "integration_method",
     "Network",
+    "Circuit",
     "binomial_thresholds",
     "build_out_csr",
     "from_spec",

@@ -37,6 +37,12 @@ EXPORTS = (
     "edlut_network_add_recurrent",
     "edlut_network_add_poisson",
     "edlut_network_set_external",
+    "edlut_network_set_poisson_rates",
+    "edlut_network_add_projection",
+    "edlut_circuit_create",
+    "edlut_circuit_run",
+    "edlut_circuit_synchronize",
+    "edlut_circuit_destroy",
     "edlut_network_run",
     "edlut_network_set_graph",
     "edlut_network_synchronize",
@@ -114,6 +120,12 @@ def _sig(lib):
         "edlut_network_add_recurrent": (I32, [P, P, P, P]),
         "edlut_network_add_poisson": (I32, [P, I32, P, F32, P, P, P]),
         "edlut_network_set_external": (I32, [P, I32, P, F32, U64]),
+        "edlut_network_set_poisson_rates": (I32, [P, I32, P]),
+        "edlut_network_add_projection": (I32, [P, P, P, P, P]),
+        "edlut_circuit_create": (I32, [P, I32, P]),
+        "edlut_circuit_run": (I32, [P, I64, I32]),
+        "edlut_circuit_synchronize": (I32, [P]),
+        "edlut_circuit_destroy": (I32, [P]),
         "edlut_network_run": (I32, [P, I64, I32]),
         "edlut_network_set_graph": (I32, [P, I32]),
         "edlut_network_synchronize": (I32, [P]),

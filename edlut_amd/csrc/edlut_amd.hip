@@ -380,8 +380,15 @@ struct edlut_population {
     }
 };
 
+struct edlut_network;
+struct edlut_circuit;
+
 struct Source {
     bool self = false;
+    // projection from another network's population (edlut_network_add_projection):
+    // its spikes are that network's own ring
+    edlut_network *from = nullptr;
+    float *rates = nullptr;       // per-generator Poisson frequencies (or null)
     int64_t n = 0;
     int32_t *ring_idx = nullptr;
     int32_t *ring_cnt = nullptr;
@@ -404,6 +411,8 @@ struct Source {
 
 struct edlut_network {
     edlut_population *pop = nullptr;
+    edlut_circuit *circuit = nullptr;   // lockstep group this network belongs to
+    int links_out = 0;                  // projections of this population into other networks
     int max_delay = 1;
     int ring = 3;
     std::vector<Source> sources;
@@ -1083,7 +1092,9 @@ int edlut_population_groups(const edlut_population *p, int *n_groups, int32_t *s
 int edlut_population_destroy(edlut_population *p) {
     if (!p) return 0;
     if (p->net) {
-        edlut_network_destroy(p->net);
+        // refused while the network is in a circuit or feeds projections
+        int rc = edlut_network_destroy(p->net);
+        if (rc) return rc;
     }
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
@@ -1303,14 +1314,21 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
 
 int edlut_network_destroy(edlut_network *n) {
     if (!n) return 0;
+    if (n->circuit) return fail(EDLUT_E_STATE, "the network belongs to a circuit: destroy the circuit first");
+    if (n->links_out > 0)
+        return fail(EDLUT_E_STATE, "%d projections read this network's spikes: destroy their networks first",
+                    n->links_out);
     (void)hipSetDevice(n->pop->device);
     (void)hipStreamSynchronize(n->pop->stream);
     for (Source &s : n->sources) {
-        void *dev[] = {s.self ? nullptr : s.ring_idx, s.self ? nullptr : s.ring_cnt, s.row_ptr,
+        // the rings of recurrent and projected sources belong to their populations' networks
+        const bool own_ring = !s.self && !s.from;
+        void *dev[] = {own_ring ? s.ring_idx : nullptr, own_ring ? s.ring_cnt : nullptr, s.row_ptr,
                        s.tgt_type, s.w, s.lcg, s.tile_count, s.staging, s.syn.pre, s.syn.post,
-                       s.syn.last, s.in_ptr, s.in_syn};
+                       s.syn.last, s.in_ptr, s.in_syn, s.rates};
         for (void *q : dev)
             if (q) (void)hipFree(q);
+        if (s.from) s.from->links_out--;
     }
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
@@ -1403,6 +1421,46 @@ int edlut_network_add_poisson(edlut_network *n, int n_src, const uint32_t *lcg_s
     return n->pop->reserve_events(n->total_syn + 1);
 }
 
+int edlut_network_add_projection(edlut_network *n, edlut_network *from, const int64_t *row_ptr,
+                                 const uint32_t *tgt_type, const float *w) {
+    if (!n || !from) return fail(EDLUT_E_ARG, "network is null");
+    if (from == n) return fail(EDLUT_E_ARG, "a population's own synapses are added with add_recurrent");
+    if ((int)n->sources.size() >= kMaxSources) return fail(EDLUT_E_CAP, "too many sources");
+    if (from->pop->device != n->pop->device) return fail(EDLUT_E_ARG, "networks on different devices");
+    if (from->max_delay != n->max_delay) return fail(EDLUT_E_ARG, "networks of a projection must share max_delay");
+    if (from->pop->dt_s != n->pop->dt_s) return fail(EDLUT_E_ARG, "networks of a projection must share the step");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    Source s;
+    s.self = false;
+    s.from = from;
+    s.n = from->pop->n;
+    s.ring_idx = from->own_ring_idx;
+    s.ring_cnt = from->own_ring_cnt;
+    int rc = add_synapses(n, s, row_ptr, tgt_type, w);
+    if (rc) return rc;
+    n->sources.push_back(s);
+    from->links_out++;
+    n->dirty = true;
+    n->seg_dirty = true;
+    n->total_syn += (uint64_t)row_ptr[(size_t)s.n * n->max_delay];
+    return n->pop->reserve_events(n->total_syn + 1);
+}
+
+int edlut_network_set_poisson_rates(edlut_network *n, int source, const float *rates_hz) {
+    if (!n || !rates_hz) return fail(EDLUT_E_ARG, "null argument");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    Source &s = n->sources[(size_t)source];
+    if (s.self || s.from) return fail(EDLUT_E_ARG, "source %d is not a Poisson device", source);
+    for (int64_t i = 0; i < s.n; i++)
+        if (!std::isfinite(rates_hz[i])) return fail(EDLUT_E_ARG, "rate %lld is not finite", (long long)i);
+    HIP_TRY(hipSetDevice(n->pop->device));
+    int rc;
+    if (!s.rates && (rc = dalloc(&s.rates, (size_t)s.n))) return rc;
+    HIP_TRY(hipMemcpy(s.rates, rates_hz, sizeof(float) * s.n, hipMemcpyHostToDevice));
+    n->dirty = true;
+    return 0;
+}
+
 int edlut_network_set_external(edlut_network *n, int n_thr, const uint32_t *thresholds, float w_ext,
                                uint64_t seed) {
     if (!n) return fail(EDLUT_E_ARG, "network is null");
@@ -1432,14 +1490,15 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, p->stream, step, n->d_time, n->ring, (double)p->dt_s);
     // Poisson devices fire at update k (their spikes feed later updates).
     for (Source &s : n->sources) {
-        if (s.self) continue;
+        if (s.self || s.from) continue;
         PoissonArgs pa;
         std::memset(&pa, 0, sizeof(pa));
         pa.lcg = s.lcg;
         pa.n = (int)s.n;
         pa.n_tiles = (int)((s.n + kBlock - 1) / kBlock);
         pa.rate = s.rate;
-        pa.rate_dt = (double)s.rate * (double)p->dt_s;
+        pa.rates = s.rates;
+        pa.step_s = (double)p->dt_s;
         pa.out = TileOut{s.tile_count, s.staging};
         hipLaunchKernelGGL(k_poisson, dim3(pa.n_tiles), dim3(kBlock), 0, p->stream, pa);
         launch_compact(p->stream, s.tile_count, s.staging, pa.n_tiles, kBlock, s.ring_idx, s.ring_cnt,
@@ -1594,12 +1653,11 @@ static int build_segments(edlut_network *n) {
     return 0;
 }
 
-int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
-    if (!n) return fail(EDLUT_E_ARG, "network is null");
-    if (first_step < 1 || n_steps < 0 || first_step + n_steps > (1ll << 30))
-        return fail(EDLUT_E_ARG, "step range out of bounds");
+// Everything a run needs before its first step: stale folds dropped, rings
+// cleared when the run does not continue the previous one, segments built,
+// the timing window set.  Returns the update kernel through *fn.
+static int run_prepare(edlut_network *n, int64_t first_step, int n_steps, UpdateFn *fn) {
     edlut_population *p = n->pop;
-    HIP_TRY(hipSetDevice(p->device));
     if (!p->q_tgt.empty()) return fail(EDLUT_E_STATE, "queued host inputs are not used in network mode");
     if (n->fold_ready >= 0 && (first_step != n->fold_ready || n->seg_dirty)) {
         // events folded ahead for an update that will not run next (or into
@@ -1611,7 +1669,8 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     if (first_step != n->last_step + 1) {
         // Rings hold the spikes of the steps just before `first_step` only
         // when the run continues the previous one.
-        for (Source &s : n->sources) HIP_TRY(hipMemsetAsync(s.ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
+        for (Source &s : n->sources)
+            if (!s.from) HIP_TRY(hipMemsetAsync(s.ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
         HIP_TRY(hipMemsetAsync(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
         int64_t lc[2] = {0, 0};
         HIP_TRY(hipMemcpyAsync(lc, n->log_count, sizeof(lc), hipMemcpyDeviceToHost, p->stream));
@@ -1629,8 +1688,8 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
                                hipMemcpyHostToDevice, p->stream));
         HIP_TRY(hipStreamSynchronize(p->stream));
     }
-    UpdateFn fn = pick_update(p, n->has_ext);
-    if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
+    *fn = pick_update(p, n->has_ext);
+    if (!*fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
     if (n->seg_dirty) {
         int rc = build_segments(n);
         if (rc) return rc;
@@ -1639,8 +1698,6 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     p->started = true;
     // timed runs: the update kernels of the first clk_cap steps stamp their
     // workgroups' start and end (captured chunks included)
-    const RingEvents re = ring_events(n);
-    int64_t k = first_step;
     n->timed = 0;
     if (n->clk) {
         n->clk_first = first_step;
@@ -1650,19 +1707,44 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
         HIP_TRY(hipMemsetAsync(n->clk, 0, sizeof(unsigned long long) * 2 * n->clk_cap * n->clk_tiles_max,
                                p->stream));
     }
+    return 0;
+}
+
+// Before a chunk of captured steps starting at update k: when the captured
+// steps expect their events folded already, fold update k's.
+static int fold_ahead(edlut_network *n, const RingEvents &re, int64_t k) {
+    if (!n->graph_fused || n->fold_ready == k) return 0;
+    RingEvents rk = re;
+    rk.step = StepRef{nullptr, (int32_t)k};
+    FoldTarget ft = n->pop->fold_target(0, true);
+    ft.seg_off = n->seg_off;
+    return launch_ring_fold(ft, n->pop->stream, rk, n->fold_lanes, n->fold_blocks);
+}
+
+static void run_finish(edlut_network *n, int64_t first_step, int n_steps) {
+    if (n->clk) n->timed = n->clk_count;
+    n->last_step = first_step + n_steps - 1;
+}
+
+int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (n->circuit) return fail(EDLUT_E_STATE, "the network belongs to a circuit: run the circuit");
+    if (first_step < 1 || n_steps < 0 || first_step + n_steps > (1ll << 30))
+        return fail(EDLUT_E_ARG, "step range out of bounds");
+    for (const Source &s : n->sources)
+        if (s.from) return fail(EDLUT_E_STATE, "a network with projections from other networks runs in a circuit");
+    edlut_population *p = n->pop;
+    HIP_TRY(hipSetDevice(p->device));
+    UpdateFn fn = nullptr;
+    int rc = run_prepare(n, first_step, n_steps, &fn);
+    if (rc) return rc;
+    const RingEvents re = ring_events(n);
+    int64_t k = first_step;
     if (n->use_graph && n_steps >= kGraphChunk) {
         if (n->dirty || !n->graph || n->graph_fn != fn) {
-            int rc = build_graph(n, fn);
-            if (rc) return rc;
+            if ((rc = build_graph(n, fn))) return rc;
         }
-        if (n->graph_fused && n->fold_ready != k) {
-            RingEvents rk = re;
-            rk.step = StepRef{nullptr, (int32_t)k};
-            FoldTarget ft = p->fold_target(0, true);
-            ft.seg_off = n->seg_off;
-            int rc = launch_ring_fold(ft, p->stream, rk, n->fold_lanes, n->fold_blocks);
-            if (rc) return rc;
-        }
+        if ((rc = fold_ahead(n, re, k))) return rc;
         int32_t k32 = (int32_t)k;
         HIP_TRY(hipMemcpyAsync(n->d_step, &k32, sizeof(int32_t), hipMemcpyHostToDevice, p->stream));
         HIP_TRY(hipStreamSynchronize(p->stream));
@@ -1674,13 +1756,183 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     }
     for (; k < first_step + n_steps; k++) {
         bool fused = false;
-        int rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, n->fold_ready == k, &fused);
-        if (rc) return rc;
+        if ((rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, n->fold_ready == k, &fused))) return rc;
         n->fold_ready = fused ? k + 1 : -1;
     }
     HIP_TRY(hipGetLastError());
-    if (n->clk) n->timed = n->clk_count;
-    n->last_step = first_step + n_steps - 1;
+    run_finish(n, first_step, n_steps);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Circuits: several networks advanced in lockstep on one stream (update k of
+// every network, in the order given, before update k + 1 of any), so that
+// projections between them deliver in the integer order of a single network.
+struct edlut_circuit {
+    std::vector<edlut_network *> nets;
+    std::vector<hipStream_t> own_streams;   // the populations' streams, restored on destroy
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int32_t *d_step = nullptr;
+    hipGraphExec_t graph = nullptr;
+    std::vector<UpdateFn> graph_fns;
+};
+
+int edlut_circuit_create(edlut_network *const *nets, int n_nets, edlut_circuit **out) {
+    if (!nets || !out || n_nets <= 0) return fail(EDLUT_E_ARG, "no networks");
+    *out = nullptr;
+    for (int i = 0; i < n_nets; i++) {
+        if (!nets[i]) return fail(EDLUT_E_ARG, "network %d is null", i);
+        if (nets[i]->circuit) return fail(EDLUT_E_STATE, "network %d already belongs to a circuit", i);
+        if (nets[i]->pop->device != nets[0]->pop->device) return fail(EDLUT_E_ARG, "networks on different devices");
+        for (int j = 0; j < i; j++)
+            if (nets[j] == nets[i]) return fail(EDLUT_E_ARG, "network %d appears twice", i);
+    }
+    // every projection must come from a member of the circuit
+    for (int i = 0; i < n_nets; i++)
+        for (const Source &s : nets[i]->sources) {
+            if (!s.from) continue;
+            bool member = false;
+            for (int j = 0; j < n_nets; j++) member = member || nets[j] == s.from;
+            if (!member) return fail(EDLUT_E_ARG, "network %d has a projection from a network outside the circuit", i);
+        }
+    edlut_circuit *c = new edlut_circuit;
+    c->device = nets[0]->pop->device;
+    HIP_TRY(hipSetDevice(c->device));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return fail(EDLUT_E_HIP, "hipStreamCreate failed");
+    }
+    int rc = dalloc(&c->d_step, 1);
+    if (rc) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return rc;
+    }
+    for (int i = 0; i < n_nets; i++) {
+        edlut_network *n = nets[i];
+        (void)hipStreamSynchronize(n->pop->stream);
+        c->nets.push_back(n);
+        c->own_streams.push_back(n->pop->stream);
+        n->pop->stream = c->stream;
+        n->circuit = c;
+        n->dirty = true;
+    }
+    *out = c;
+    return 0;
+}
+
+int edlut_circuit_destroy(edlut_circuit *c) {
+    if (!c) return 0;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i < c->nets.size(); i++) {
+        c->nets[i]->pop->stream = c->own_streams[i];
+        c->nets[i]->circuit = nullptr;
+        c->nets[i]->dirty = true;
+    }
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
+    if (c->d_step) (void)hipFree(c->d_step);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return 0;
+}
+
+// Captures kGraphChunk lockstep steps of every member network.
+static int build_circuit_graph(edlut_circuit *c, const std::vector<UpdateFn> &fns,
+                               const std::vector<RingEvents> &res) {
+    if (c->graph) {
+        (void)hipGraphExecDestroy(c->graph);
+        c->graph = nullptr;
+    }
+    const size_t m = c->nets.size();
+    std::vector<bool> fused(m);
+    for (size_t j = 0; j < m; j++) fused[j] = res[j].n_sources > 0 && !c->nets[j]->plastic;
+    hipGraph_t g = nullptr;
+    HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = 0;
+    for (int i = 0; i < kGraphChunk && rc == 0; i++)
+        for (size_t j = 0; j < m && rc == 0; j++) {
+            bool f = fused[j];
+            rc = enqueue_step(c->nets[j], StepRef{c->d_step, i}, fns[j], res[j], f, &f);
+            fused[j] = f;
+        }
+    hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, c->stream, c->d_step, (int32_t)kGraphChunk);
+    hipError_t e = hipStreamEndCapture(c->stream, &g);
+    if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return fail(EDLUT_E_HIP, "graph capture failed");
+    e = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return fail(EDLUT_E_HIP, "graph instantiation failed");
+    for (size_t j = 0; j < m; j++) {
+        edlut_network *n = c->nets[j];
+        n->graph_fused = res[j].n_sources > 0 && !n->plastic;
+        // a network's own graph is stale once it has run inside the circuit
+        if (n->graph) (void)hipGraphExecDestroy(n->graph);
+        n->graph = nullptr;
+        n->dirty = false;
+    }
+    c->graph_fns = fns;
+    return 0;
+}
+
+int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps) {
+    if (!c) return fail(EDLUT_E_ARG, "circuit is null");
+    if (first_step < 1 || n_steps < 0 || first_step + n_steps > (1ll << 30))
+        return fail(EDLUT_E_ARG, "step range out of bounds");
+    HIP_TRY(hipSetDevice(c->device));
+    const size_t m = c->nets.size();
+    std::vector<UpdateFn> fns(m);
+    std::vector<RingEvents> res(m);
+    bool use_graph = true, rebuild = !c->graph;
+    for (size_t j = 0; j < m; j++) {
+        edlut_network *n = c->nets[j];
+        int rc = run_prepare(n, first_step, n_steps, &fns[j]);
+        if (rc) return rc;
+        res[j] = ring_events(n);
+        use_graph = use_graph && n->use_graph;
+        rebuild = rebuild || n->dirty;
+    }
+    rebuild = rebuild || fns != c->graph_fns;
+    int64_t k = first_step;
+    int rc = 0;
+    if (use_graph && n_steps >= kGraphChunk) {
+        if (rebuild && (rc = build_circuit_graph(c, fns, res))) return rc;
+        for (size_t j = 0; j < m; j++)
+            if ((rc = fold_ahead(c->nets[j], res[j], k))) return rc;
+        int32_t k32 = (int32_t)k;
+        HIP_TRY(hipMemcpyAsync(c->d_step, &k32, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        while (k + kGraphChunk <= first_step + n_steps) {
+            HIP_TRY(hipGraphLaunch(c->graph, c->stream));
+            k += kGraphChunk;
+        }
+        for (size_t j = 0; j < m; j++) c->nets[j]->fold_ready = c->nets[j]->graph_fused ? k : -1;
+    }
+    for (; k < first_step + n_steps; k++)
+        for (size_t j = 0; j < m; j++) {
+            edlut_network *n = c->nets[j];
+            bool fused = false;
+            if ((rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fns[j], res[j], n->fold_ready == k, &fused)))
+                return rc;
+            n->fold_ready = fused ? k + 1 : -1;
+        }
+    HIP_TRY(hipGetLastError());
+    for (size_t j = 0; j < m; j++) run_finish(c->nets[j], first_step, n_steps);
+    return 0;
+}
+
+int edlut_circuit_synchronize(edlut_circuit *c) {
+    if (!c) return fail(EDLUT_E_ARG, "circuit is null");
+    HIP_TRY(hipSetDevice(c->device));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (edlut_network *n : c->nets) {
+        int rc = check_err(n->pop);
+        if (rc) return rc;
+    }
     return 0;
 }
 

@@ -515,8 +515,9 @@ struct PoissonArgs {
     uint32_t *lcg;
     int n;
     int n_tiles;
-    float rate;
-    double rate_dt;   // (double)rate * (double)step_s
+    float rate;             // frequency of every generator (Hz) when rates is null
+    const float *rates;     // per-generator frequencies (PoissonGeneratorDeviceVector)
+    double step_s;          // time-driven step (s)
     TileOut out;
 };
 
@@ -525,12 +526,13 @@ __global__ __launch_bounds__(kBlock) void k_poisson(PoissonArgs a) {
     bool spk[1] = {false};
     int idx[1] = {tile * kBlock + (int)threadIdx.x};
     const int i = idx[0];
-    if (i < a.n && a.rate > 0.0f) {
+    const float f = (i < a.n && a.rates) ? a.rates[i] : a.rate;
+    if (i < a.n && f > 0.0f) {
         uint32_t x = a.lcg[i];
         x = (uint32_t)(((uint64_t)x * 1103515245ull + 12345ull) & 0x7fffffffull);
         a.lcg[i] = x;
         const float u = ((float)x) / 2147483647.0f;
-        spk[0] = a.rate_dt > (double)u;
+        spk[0] = (double)f * a.step_s > (double)u;
     }
     tile_spikes<1>(a.out, tile, spk, idx);
 }

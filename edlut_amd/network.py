@@ -94,6 +94,23 @@ class Network:
             )
         )
 
+    def set_poisson_rates(self, source: int, rates_hz):
+        """One frequency per generator of Poisson source `source`
+        (PoissonGeneratorDeviceVector)."""
+        r = np.ascontiguousarray(rates_hz, np.float32)
+        C.check(C.lib().edlut_network_set_poisson_rates(self._h, int(source), C.ptr(r)))
+
+    def add_projection(self, source: "Network", row_ptr, tgt_type, weight):
+        """Synapses from `source`'s population onto this network's population
+        (rows over the source neurons).  Networks with projections run in a
+        Circuit with their sources."""
+        row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+        tgt_type = np.ascontiguousarray(tgt_type, np.uint32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        C.check(C.lib().edlut_network_add_projection(self._h, source._h, C.ptr(row_ptr), C.ptr(tgt_type),
+                                                     C.ptr(weight)))
+        self._keep.append(source)   # the source network outlives this one
+
     def generate_recurrent(self, in_degree: int, exc_fraction: float, w_mu: float, w_sigma: float,
                            seed: int):
         """Fixed in-degree random wiring built on the device (per group for a
@@ -237,3 +254,65 @@ def from_spec(spec, device: int = 0, params: dict | None = None):
         thr, w_ext, seed = spec.ext
         net.set_external(thr, w_ext, seed)
     return model, net
+
+
+def cerebellum_from_spec(spec, device: int = 0):
+    """GPU circuit of a workloads.CerebellumSpec: granule cells (BDF2) fed
+    by the mossy-fibre Poisson devices, Golgi cells (LIF, Euler) and
+    Purkinje cells (HH under BDF2 with the FP64 Newton step, or the
+    Purkinje-cell model under RK4) fed by projections from the granule
+    cells, additive STDP on granule -> Purkinje.
+
+    Returns (models, networks, circuit) in the order granule, Golgi, Purkinje."""
+    from .models import CreateNeuronModel, integration_method
+
+    dt = float(np.float32(1e-4))
+    D = spec.max_delay
+    hh = spec.pc_model == "HHTimeDrivenModel_GPU"
+    gr = CreateNeuronModel("EgidioGranuleCell_TimeDriven_GPU", {"int_meth": integration_method("BDF", dt, 2)})
+    go = CreateNeuronModel("LIFTimeDrivenModel_GPU", {"int_meth": integration_method("Euler", dt)})
+    pc = CreateNeuronModel(spec.pc_model, {"int_meth": integration_method("BDF" if hh else "RK4", dt, 2)})
+    for m, n in ((gr, spec.n_grc), (go, spec.n_goc), (pc, spec.n_pc)):
+        m.CheckSynapseType(0)
+        m.InitializeStates(n, 0, device)
+    if hh:
+        pc.SetNewtonFP64(True)
+    ng, no, npc = Network(gr, D), Network(go, D), Network(pc, D)
+    ng.add_poisson(spec.mf_states, 0.0, *build_out_csr(spec.n_mf, spec.n_grc, D, *spec.mf_grc))
+    ng.set_poisson_rates(0, spec.mf_rates)
+    no.add_projection(ng, *build_out_csr(spec.n_grc, spec.n_goc, D, *spec.grc_goc))
+    npc.add_projection(ng, *build_out_csr(spec.n_grc, spec.n_pc, D, *spec.grc_pc))
+    npc.set_stdp(0, *spec.stdp)
+    circuit = Circuit([ng, no, npc])
+    return (gr, go, pc), (ng, no, npc), circuit
+
+
+class Circuit:
+    """Networks advanced in lockstep on one stream: update k of every member,
+    in the order given, before update k + 1 of any (edlut_circuit_*)."""
+
+    def __init__(self, nets):
+        self.nets = list(nets)
+        arr = (ctypes.c_void_p * len(self.nets))(*[n._h for n in self.nets])
+        h = ctypes.c_void_p()
+        C.check(C.lib().edlut_circuit_create(arr, len(self.nets), C.byref(h)))
+        self._h = h
+        self.next_step = 1
+
+    def run(self, n_steps: int, first_step: int | None = None, sync: bool = True):
+        if first_step is None:
+            first_step = self.next_step
+        C.check(C.lib().edlut_circuit_run(self._h, int(first_step), int(n_steps)))
+        self.next_step = first_step + n_steps
+        for n in self.nets:
+            n.next_step = self.next_step
+        if sync:
+            C.check(C.lib().edlut_circuit_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None:
+            C.lib().edlut_circuit_destroy(self._h)
+        self._h = None
+
+    def __del__(self):
+        self.close()

@@ -15,6 +15,12 @@ the same inputs can be fed to the HIP path and to any CPU implementation.
                    E_leak, a, b, tau_w drawn uniformly within +-30% of the
                    defaults; in-degree 100 wiring and the external drive are
                    generated per network from its own seed.
+  cerebellum       layered cerebellar network: mossy fibres (Poisson, one
+                   rate per fibre, U[20,80] Hz) -> granule cells (Egidio model,
+                   BDF2; in-degree 4, AMPA) -> Golgi cells (LIF, Euler;
+                   in-degree 100, AMPA) and Purkinje cells (HH, BDF2 with the
+                   FP64 Newton step, or the TimeDrivenPurkinjeCell model;
+                   in-degree 2,000, AMPA, additive STDP); dt 0.1 ms.
   adjoint_fit      trace fitting: independent LIF (Euler) or AdEx (RK4)
                    neurons, 50 input synapses each (40 exc, 10 inh) driven by
                    fixed seeded Poisson trains at 20 Hz; per-neuron g_leak,
@@ -107,6 +113,55 @@ def adex_recurrent(n: int = 1_000_000, in_degree: int = 200, exc_frac: float = 0
     spec.rec = (src, tgt, delay, typ, w)
     spec.ext = (thr, np.float32(w_ext), seed * 1000003 + 11)
     return spec
+
+
+@dataclass
+class CerebellumSpec:
+    n_mf: int
+    n_grc: int
+    n_goc: int
+    n_pc: int
+    pc_model: str
+    max_delay: int
+    mf_states: np.ndarray
+    mf_rates: np.ndarray
+    # edge lists (src, tgt, delay steps, type, weight)
+    mf_grc: tuple
+    grc_goc: tuple
+    grc_pc: tuple
+    # additive STDP on granule -> Purkinje: (max_LTP, tau_LTP, max_LTD, tau_LTD, max_weight)
+    stdp: tuple
+    steps: int
+
+
+def _fixed_in_degree(rng, n_src, n_tgt, k, d_lo, d_hi, w_lo, w_hi):
+    src = rng.integers(0, n_src, n_tgt * k).astype(np.int32)
+    tgt = np.repeat(np.arange(n_tgt, dtype=np.int32), k)
+    delay = rng.integers(d_lo, d_hi + 1, n_tgt * k).astype(np.int32)
+    typ = np.zeros(n_tgt * k, np.uint8)
+    w = rng.uniform(w_lo, w_hi, n_tgt * k).astype(np.float32)
+    return src, tgt, delay, typ, w
+
+
+def cerebellum(n_grc: int = 200_000, n_goc: int = 2_000, n_pc: int = 100, n_mf: int = 10_000,
+               k_grc: int = 4, k_goc: int = 100, k_pc: int = 2_000, pc_model: str = "HHTimeDrivenModel_GPU",
+               max_delay: int = 10, steps: int = 20_000, seed: int = 3, w_goc: float = 0.02,
+               w_pc: float | None = None) -> CerebellumSpec:
+    """Layered cerebellar network.  Weights (nS): mossy fibre -> granule
+    U[0.5, 1.5], granule -> Golgi U[0, w_goc], granule -> Purkinje U[0, w_pc]
+    (default 0.3 for HH, 0.05 for the Purkinje-cell model), clamped by STDP
+    to [0, 2 w_pc]; delays 1-5 steps into granule cells, 1-max_delay elsewhere."""
+    rng = np.random.default_rng(seed)
+    pc_w = w_pc if w_pc is not None else (0.3 if pc_model == "HHTimeDrivenModel_GPU" else 0.05)
+    return CerebellumSpec(
+        n_mf=n_mf, n_grc=n_grc, n_goc=n_goc, n_pc=n_pc, pc_model=pc_model, max_delay=max_delay,
+        mf_states=lcg_seeds(seed * 7919 + 5, n_mf),
+        mf_rates=rng.uniform(20.0, 80.0, n_mf).astype(np.float32),
+        mf_grc=_fixed_in_degree(rng, n_mf, n_grc, k_grc, 1, min(5, max_delay), 0.5, 1.5),
+        grc_goc=_fixed_in_degree(rng, n_grc, n_goc, k_goc, 1, max_delay, 0.0, w_goc),
+        grc_pc=_fixed_in_degree(rng, n_grc, n_pc, k_pc, 1, max_delay, 0.0, pc_w),
+        stdp=(0.005 * pc_w, 0.020, 0.0055 * pc_w, 0.020, 2.0 * pc_w),
+        steps=steps)
 
 
 @dataclass

@@ -44,6 +44,7 @@ enum edlut_error {
 
 typedef struct edlut_population edlut_population;
 typedef struct edlut_network edlut_network;
+typedef struct edlut_circuit edlut_circuit;
 
 /* Message of the last failing call on this thread. */
 const char *edlut_last_error(void);
@@ -171,6 +172,43 @@ int edlut_network_add_recurrent(edlut_network *net, const int64_t *row_ptr,
 int edlut_network_add_poisson(edlut_network *net, int n_src, const uint32_t *lcg_state,
                               float rate_hz, const int64_t *row_ptr, const uint32_t *tgt_type,
                               const float *weight);
+/*
+ * Per-generator frequencies (Hz) of Poisson source `source` (n_src values):
+ * PoissonGeneratorDeviceVector keeps one frequency per generator
+ * (PoissonGeneratorDeviceVector.cpp:56-70,95-99); a generator fires when
+ * frequency * step > frand() and draws nothing when its frequency is not
+ * positive.
+ */
+int edlut_network_set_poisson_rates(edlut_network *net, int source, const float *rates_hz);
+
+/*
+ * Projection of the population of network `from` onto the population of
+ * `net` (another neuron model, e.g. granule -> Purkinje): CSR rows over the
+ * source population's neurons, the same layout as edlut_network_add_recurrent.
+ * Replaces the reference's inter-population Interconnections delivered by
+ * PropagatedSpikeGroup (PropagatedSpikeGroup.cpp:66-108) for GPU->GPU
+ * synapses: the spikes come from `from`'s device ring.  Both networks must
+ * share the device, the step and max_delay, and run in one circuit.  Events
+ * of one update reach the target by delay, then source (in the order the
+ * sources were added: recurrent, Poisson devices, projections), then
+ * spiking source index, then synapse.
+ */
+int edlut_network_add_projection(edlut_network *net, edlut_network *from, const int64_t *row_ptr,
+                                 const uint32_t *tgt_type, const float *w);
+
+/*
+ * Circuit: networks advanced in lockstep on one stream, update k of every
+ * member (in the order given) before update k + 1 of any; every projection
+ * must come from a member.  While a network belongs to a circuit it is run
+ * only through the circuit (its state, spike log and synapses stay readable).
+ * The reference advances its populations through one event queue
+ * (TimeEventUpdateNeurons.cpp:45-79, one update event per model).
+ */
+int edlut_circuit_create(edlut_network *const *nets, int n_nets, edlut_circuit **out);
+int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps);
+int edlut_circuit_synchronize(edlut_circuit *c);
+int edlut_circuit_destroy(edlut_circuit *c);
+
 /* Counter-based external drive: per neuron and step, k events of weight w_ext
  * into the excitatory conductance, k = #{j : hash(seed,step,i) < thr[j]},
  * folded after the interval's recurrent events.  In a grouped population

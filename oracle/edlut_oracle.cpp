@@ -1154,9 +1154,13 @@ struct Source {
     std::vector<int64_t> row_ptr;
     std::vector<uint32_t> tgt_type;
     std::vector<float> w;
-    /* Poisson device */
+    /* Poisson device (PoissonGeneratorDeviceVector: one frequency per
+     * generator, `rates`, or `rate` for all) */
     std::vector<uint32_t> lcg;
     float rate = 0.0f;
+    std::vector<float> rates;
+    /* projection from another network's population: its spikes */
+    struct Net *from = nullptr;
     /* spikes per step: ring indexed by step % ring */
     std::vector<std::vector<int32_t>> ring;
     /* additive STDP on every synapse of the source (STDPWeightChange,
@@ -1223,6 +1227,10 @@ struct Net {
     float w_ext = 0.0f;
     uint64_t ext_seed = 0;
     bool has_ext = false;
+    /* the population's spikes of each step, by step % ring (read by
+     * projections into other networks), and the spikes logged by circuit runs */
+    std::vector<std::vector<int32_t>> out_ring;
+    std::vector<std::pair<int32_t, int32_t>> log;
     /* update times: t[k] = t[k-1] + double(float dt), t[0] = 0
      * (TimeEventUpdateNeurons.cpp:78, Simulation.cpp:228-268) */
     std::vector<double> t;
@@ -1474,76 +1482,88 @@ int64_t orc_net_get_trace(void *net, int32_t *step, int32_t *tgt, uint8_t *type,
     return m;
 }
 
-int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_step,
-                    int32_t *spike_idx, int64_t cap) {
-    Net *n = (Net *)net;
+/* One update k of a network: delivery, external drive, Poisson devices,
+ * population update, post-synaptic learning.  Returns the spike count; the
+ * spikes are in spk[0..ns) and in out_ring[k % ring]. */
+static int net_step(Net *n, int64_t k, std::vector<int32_t> &spk) {
     PopIface *pop = n->pop;
     const int N = pop->size();
     const int D = n->max_delay;
-    std::vector<int32_t> spk(N);
+    /* 1. delivery of every event consumed by update k: by delay, then source
+     * (in the order added), then spiking source index, then synapse */
+    for (int d = 1; d <= D; d++) {
+        int64_t s = k - 1 - d;
+        if (s < 1) continue;
+        for (Source &src : n->sources) {
+            const std::vector<int32_t> &list =
+                src.from ? src.from->out_ring[s % src.from->ring] : src.ring[s % n->ring];
+            for (int32_t id : list) {
+                int64_t b = src.row_ptr[(size_t)id * D + d - 1];
+                int64_t e = src.row_ptr[(size_t)id * D + d];
+                for (int64_t q = b; q < e; q++) {
+                    uint32_t tt = src.tgt_type[q];
+                    pop->add_input((int)(tt & 0x3fffffffu), (int)(tt >> 30), src.w[q]);
+                    if (n->trace)
+                        n->trace_buf.push_back({(int32_t)k, (int32_t)(tt & 0x3fffffffu),
+                                                (uint8_t)(tt >> 30), src.w[q]});
+                    /* PropagatedSpikeGroup.cpp:76-97: the learning rule runs
+                     * after ProcessInputSpike, at the arrival time */
+                    if (src.stdp) stdp_pre(src, q, n->time_of(s) + delay_seconds(d, pop->dt_s()));
+                }
+            }
+        }
+    }
+    /* 2. external drive: after the recurrent events of the interval */
+    if (n->has_ext) {
+        for (int i = 0; i < N; i++) {
+            uint32_t u = drive_hash(n->ext_seed, k, (uint32_t)i);
+            int cnt = 0;
+            while (cnt < (int)n->thr.size() && u < n->thr[cnt]) cnt++;
+            for (int c = 0; c < cnt; c++) pop->add_input(i, 0, n->w_ext);
+        }
+    }
+    /* 3. Poisson devices at update k (PoissonGeneratorDeviceVector.cpp:56-70):
+     * a source fires when frequency * step (float * double) exceeds its
+     * next frand(); frand is not drawn when the frequency is not positive. */
+    for (Source &src : n->sources) {
+        if (src.self || src.from) continue;
+        std::vector<int32_t> &list = src.ring[k % n->ring];
+        list.clear();
+        const double step_s = (double)pop->dt_s();
+        for (int i = 0; i < src.n; i++) {
+            const float f = src.rates.empty() ? src.rate : src.rates[(size_t)i];
+            if (f > 0.0f) {
+                src.lcg[i] = lcg_next(src.lcg[i]);
+                if ((double)f * step_s > (double)lcg_frand(src.lcg[i])) list.push_back(i);
+            }
+        }
+    }
+    /* 4. population update k */
+    int ns = pop->update(k, spk.data());
+    for (Source &src : n->sources) {
+        if (!src.self) continue;
+        src.ring[k % n->ring].assign(spk.begin(), spk.begin() + ns);
+    }
+    if (n->out_ring.empty()) n->out_ring.assign(n->ring, std::vector<int32_t>());
+    n->out_ring[k % n->ring].assign(spk.begin(), spk.begin() + ns);
+    /* post-synaptic learning of the neurons that fired, at the update
+     * time (TimeDrivenInternalSpike.cpp:94-103) */
+    for (Source &src : n->sources) {
+        if (!src.stdp) continue;
+        const double tk = n->time_of(k);
+        for (int q = 0; q < ns; q++)
+            for (int64_t syn : src.in_syn[(size_t)spk[q]]) stdp_post(src, syn, tk);
+    }
+    return ns;
+}
+
+int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_step,
+                    int32_t *spike_idx, int64_t cap) {
+    Net *n = (Net *)net;
+    std::vector<int32_t> spk(n->pop->size());
     int64_t total = 0;
     for (int64_t k = first_step; k < first_step + n_steps; k++) {
-        /* 1. delivery of every event consumed by update k */
-        for (int d = 1; d <= D; d++) {
-            int64_t s = k - 1 - d;
-            if (s < 1) continue;
-            for (Source &src : n->sources) {
-                const std::vector<int32_t> &list = src.ring[s % n->ring];
-                for (int32_t id : list) {
-                    int64_t b = src.row_ptr[(size_t)id * D + d - 1];
-                    int64_t e = src.row_ptr[(size_t)id * D + d];
-                    for (int64_t q = b; q < e; q++) {
-                        uint32_t tt = src.tgt_type[q];
-                        pop->add_input((int)(tt & 0x3fffffffu), (int)(tt >> 30), src.w[q]);
-                        if (n->trace)
-                            n->trace_buf.push_back({(int32_t)k, (int32_t)(tt & 0x3fffffffu),
-                                                    (uint8_t)(tt >> 30), src.w[q]});
-                        /* PropagatedSpikeGroup.cpp:76-97: the learning rule runs
-                         * after ProcessInputSpike, at the arrival time */
-                        if (src.stdp) stdp_pre(src, q, n->time_of(s) + delay_seconds(d, pop->dt_s()));
-                    }
-                }
-            }
-        }
-        /* 2. external drive: after the recurrent events of the interval */
-        if (n->has_ext) {
-            for (int i = 0; i < N; i++) {
-                uint32_t u = drive_hash(n->ext_seed, k, (uint32_t)i);
-                int cnt = 0;
-                while (cnt < (int)n->thr.size() && u < n->thr[cnt]) cnt++;
-                for (int c = 0; c < cnt; c++) pop->add_input(i, 0, n->w_ext);
-            }
-        }
-        /* 3. Poisson devices at update k (PoissonGeneratorDeviceVector.cpp:56-70):
-         * a source fires when frequency * step (float * double) exceeds its
-         * next frand(); frand is not drawn when the frequency is not positive. */
-        for (Source &src : n->sources) {
-            if (src.self) continue;
-            std::vector<int32_t> &list = src.ring[k % n->ring];
-            list.clear();
-            const double step_s = (double)pop->dt_s();
-            for (int i = 0; i < src.n; i++) {
-                if (src.rate > 0.0f) {
-                    src.lcg[i] = lcg_next(src.lcg[i]);
-                    if ((double)src.rate * step_s > (double)lcg_frand(src.lcg[i]))
-                        list.push_back(i);
-                }
-            }
-        }
-        /* 4. population update k */
-        int ns = pop->update(k, spk.data());
-        for (Source &src : n->sources) {
-            if (!src.self) continue;
-            src.ring[k % n->ring].assign(spk.begin(), spk.begin() + ns);
-        }
-        /* post-synaptic learning of the neurons that fired, at the update
-         * time (TimeDrivenInternalSpike.cpp:94-103) */
-        for (Source &src : n->sources) {
-            if (!src.stdp) continue;
-            const double tk = n->time_of(k);
-            for (int q = 0; q < ns; q++)
-                for (int64_t syn : src.in_syn[(size_t)spk[q]]) stdp_post(src, syn, tk);
-        }
+        const int ns = net_step(n, k, spk);
         for (int q = 0; q < ns; q++) {
             if (total < cap) {
                 spike_step[total] = (int32_t)k;
@@ -1553,6 +1573,59 @@ int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_s
         }
     }
     return total > cap ? -1 : total;
+}
+
+/* Projection of `source`'s population onto `net`'s (its CSR rows are the
+ * source population's neurons).  Both networks must have the same maximum
+ * delay and step. */
+int orc_net_add_link(void *net, void *source, const int64_t *row_ptr, const uint32_t *tgt_type,
+                     const float *w) {
+    Net *n = (Net *)net;
+    Net *f = (Net *)source;
+    if (f->max_delay != n->max_delay) return -1;
+    Source s;
+    s.self = false;
+    s.from = f;
+    s.n = f->pop->size();
+    if (f->out_ring.empty()) f->out_ring.assign(f->ring, std::vector<int32_t>());
+    return add_source(n, s, row_ptr, tgt_type, w);
+}
+
+/* One frequency per generator of a Poisson source. */
+int orc_net_set_poisson_rates(void *net, int source, const float *rates) {
+    Net *n = (Net *)net;
+    if (source < 0 || source >= (int)n->sources.size()) return -1;
+    Source &s = n->sources[(size_t)source];
+    if (s.self || s.from) return -1;
+    s.rates.assign(rates, rates + s.n);
+    return 0;
+}
+
+/* Lockstep run of several networks: update k of every network (in the order
+ * given) before update k + 1 of any; the spikes go to each network's log. */
+void orc_circuit_run(void **nets, int n_nets, int64_t first_step, int n_steps) {
+    std::vector<std::vector<int32_t>> spk((size_t)n_nets);
+    for (int i = 0; i < n_nets; i++) spk[(size_t)i].resize(((Net *)nets[i])->pop->size());
+    for (int64_t k = first_step; k < first_step + n_steps; k++)
+        for (int i = 0; i < n_nets; i++) {
+            Net *n = (Net *)nets[i];
+            const int ns = net_step(n, k, spk[(size_t)i]);
+            for (int q = 0; q < ns; q++) n->log.push_back({(int32_t)k, spk[(size_t)i][(size_t)q]});
+        }
+}
+
+/* Moves the logged spikes out when cap holds them all (else leaves them);
+ * returns the number logged. */
+int64_t orc_net_take_log(void *net, int32_t *step, int32_t *idx, int64_t cap) {
+    Net *n = (Net *)net;
+    const int64_t m = (int64_t)n->log.size();
+    if (cap < m) return m;
+    for (int64_t i = 0; i < m; i++) {
+        step[i] = n->log[(size_t)i].first;
+        idx[i] = n->log[(size_t)i].second;
+    }
+    n->log.clear();
+    return m;
 }
 
 }  // extern "C"

@@ -109,6 +109,18 @@ int64_t orc_net_get_trace(void *net, int32_t *step, int32_t *tgt, uint8_t *type,
 /* ---- counter-based drive (shared definition with the HIP path) ----------- */
 uint32_t orc_drive_hash(uint64_t seed, int64_t step, uint32_t neuron);
 
+/* ---- several populations ------------------------------------------------ */
+/* Projection from `source`'s population onto `net`'s population (CSR over the
+ * source's neurons, same layout as the other sources); same max_delay. */
+int orc_net_add_link(void *net, void *source, const int64_t *row_ptr, const uint32_t *tgt_type,
+                     const float *w);
+/* Per-generator frequencies (Hz) of Poisson source `source`. */
+int orc_net_set_poisson_rates(void *net, int source, const float *rates);
+/* Lockstep run: update k of every network, in the order given, before update
+ * k + 1 of any; spikes go to each network's log (orc_net_take_log). */
+void orc_circuit_run(void **nets, int n_nets, int64_t first_step, int n_steps);
+int64_t orc_net_take_log(void *net, int32_t *step, int32_t *idx, int64_t cap);
+
 #ifdef __cplusplus
 }
 #endif

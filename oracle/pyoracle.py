@@ -80,6 +80,11 @@ def lib():
         l.orc_net_get_synapses.argtypes = [P, ctypes.c_int, P, P, P, P]
         l.orc_net_get_trace.restype = ctypes.c_int64
         l.orc_net_get_trace.argtypes = [P, P, P, P, P, ctypes.c_int64]
+        l.orc_net_add_link.argtypes = [P, P, P, P, P]
+        l.orc_net_set_poisson_rates.argtypes = [P, ctypes.c_int, P]
+        l.orc_circuit_run.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_int]
+        l.orc_net_take_log.restype = ctypes.c_int64
+        l.orc_net_take_log.argtypes = [P, P, P, ctypes.c_int64]
         l.orc_net_run.restype = ctypes.c_int64
         l.orc_net_run.argtypes = [P, ctypes.c_int64, ctypes.c_int, P, P, ctypes.c_int64]
         _lib = l
@@ -245,6 +250,27 @@ class Net:
         lib().orc_net_get_trace(self.h, _p(st), _p(tg), _p(ty), _p(w), n)
         return st, tg, ty, w
 
+    def add_link(self, source: "Net", row_ptr, tgt_type, w):
+        """Projection of `source`'s population onto this one."""
+        a = [np.ascontiguousarray(row_ptr, np.int64), np.ascontiguousarray(tgt_type, np.uint32),
+             np.ascontiguousarray(w, np.float32)]
+        if lib().orc_net_add_link(self.h, source.h, *[_p(x) for x in a]) < 0:
+            raise ValueError("networks of a projection must share max_delay")
+        self._keep.append(source)
+
+    def set_poisson_rates(self, source: int, rates):
+        r = np.ascontiguousarray(rates, np.float32)
+        if lib().orc_net_set_poisson_rates(self.h, source, _p(r)) != 0:
+            raise ValueError("not a Poisson source")
+        self._keep.append(r)
+
+    def take_log(self):
+        m = lib().orc_net_take_log(self.h, None, None, 0)
+        st = np.zeros(max(m, 1), np.int32)
+        ix = np.zeros(max(m, 1), np.int32)
+        lib().orc_net_take_log(self.h, _p(st), _p(ix), m)
+        return st[:m].copy(), ix[:m].copy()
+
     def run(self, first_step: int, n_steps: int, cap: int = 1 << 24):
         st = np.zeros(cap, np.int32)
         ix = np.zeros(cap, np.int32)
@@ -252,6 +278,12 @@ class Net:
         if n < 0:
             raise RuntimeError("oracle spike buffer too small")
         return st[:n].copy(), ix[:n].copy()
+
+
+def circuit_run(nets, first_step: int, n_steps: int) -> None:
+    """Lockstep run of several networks (update k of each, in order)."""
+    arr = (ctypes.c_void_p * len(nets))(*[n.h for n in nets])
+    lib().orc_circuit_run(arr, len(nets), first_step, n_steps)
 
 
 def from_spec(spec, params: np.ndarray, csr_builder, fp64: bool = False):
