@@ -199,6 +199,84 @@ def bench_adjoint(n: int, steps: int, checkpoint: int, reps: int = 3):
     return out
 
 
+def bench_cerebellum(steps: int, warmup: int, cpu_steps: int):
+    """Layered cerebellar circuit at full size (200,000 granule cells under
+    BDF2, 2,000 Golgi LIF, 100 HH Purkinje cells under BDF2 with the FP64
+    Newton step, 10,000 mossy fibres at U[20,80] Hz, STDP on the 200,000
+    granule -> Purkinje synapses): lockstep circuit steps, hipGraph replays."""
+    from edlut_amd import workloads
+    from edlut_amd.network import cerebellum_from_spec
+
+    spec = workloads.cerebellum(steps=warmup + steps)
+    t_setup = time.perf_counter()
+    models, nets, circuit = cerebellum_from_spec(spec)
+    print(f"cerebellum setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
+    circuit.run(warmup)
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:   # progress every 1000 steps
+        k = min(1000, steps - done)
+        circuit.run(k)
+        done += k
+        print(f"cerebellum {done}/{steps} steps", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t0
+    sizes = (spec.n_grc, spec.n_goc, spec.n_pc)
+    rates = {}
+    for name, n, size in zip(("granule", "golgi", "purkinje"), nets, sizes):
+        st, _ = n.spikes()
+        rates[name] = round(float((st > warmup).sum() / (size * steps * 1e-4)), 3)
+    updates = sum(sizes) * steps
+    out = {
+        "workload": f"{spec.n_grc} granule (BDF2) + {spec.n_goc} Golgi (LIF, Euler) + {spec.n_pc} Purkinje "
+                    f"(HH, BDF2, FP64 Newton); {spec.n_mf} mossy fibres U[20,80] Hz; in-degree 4 / 100 / 2000; "
+                    f"STDP granule->Purkinje; dt 0.1 ms",
+        "value": updates / wall,
+        "unit": "neuron-updates/s",
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": wall * 1e3 / steps,
+        "rates_hz": rates,
+    }
+    circuit.close()
+    for n in reversed(nets):
+        n.close()
+    for m in models:
+        m.close()
+    if cpu_steps > 0:
+        import edlut_amd as E
+        from oracle import pyoracle
+
+        def defaults(name):
+            return np.array([v for _, v in E.REGISTRY[name].PARAMS], np.float32)
+
+        np_out_csr = E.build_out_csr
+        D = spec.max_delay
+        t_c = time.perf_counter()
+        gr = pyoracle.Population("EgidioGranuleCell_TimeDriven_GPU", "BDF", [], spec.n_grc, mask=1, bdf_order=2)
+        go = pyoracle.Population("LIFTimeDrivenModel_GPU", "Euler", defaults("LIFTimeDrivenModel_GPU"),
+                                 spec.n_goc, mask=1)
+        pc = pyoracle.Population(spec.pc_model, "BDF", defaults(spec.pc_model), spec.n_pc, mask=1, fp64=2,
+                                 bdf_order=2)
+        ng, no, npc = pyoracle.Net(gr, D), pyoracle.Net(go, D), pyoracle.Net(pc, D)
+        ng.add_poisson(spec.mf_states, 0.0, *np_out_csr(spec.n_mf, spec.n_grc, D, *spec.mf_grc))
+        ng.set_poisson_rates(0, spec.mf_rates)
+        no.add_link(ng, *np_out_csr(spec.n_grc, spec.n_goc, D, *spec.grc_goc))
+        npc.add_link(ng, *np_out_csr(spec.n_grc, spec.n_pc, D, *spec.grc_pc))
+        npc.set_stdp(0, *spec.stdp)
+        t0 = time.perf_counter()
+        pyoracle.circuit_run([ng, no, npc], 1, cpu_steps)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": sum(sizes) * cpu_steps / dt,
+            "unit": "neuron-updates/s",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"oracle C++ restatement, first {cpu_steps} updates of the same circuit ({dt:.1f} s; "
+                      f"setup {t0 - t_c:.1f} s)",
+        }
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -219,6 +297,9 @@ def main():
                     help="steps per sequential network (default: the sweep's steps)")
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-cerebellum", action="store_true")
+    ap.add_argument("--cerebellum-steps", type=int, default=2000)
+    ap.add_argument("--cerebellum-cpu-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: the library has no multi-GPU path.", file=sys.stderr)
@@ -297,6 +378,9 @@ def main():
     model.close()
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(256, 10000, 100)
+    if not args.no_cerebellum:
+        out["cerebellum"] = bench_cerebellum(args.cerebellum_steps, 100,
+                                             0 if args.no_cpu else args.cerebellum_cpu_steps)
     if not args.no_sweep:
         out["batched"] = bench_sweep(args.sweep_networks, args.sweep_steps, args.seq_sample,
                                      args.seq_steps or args.sweep_steps)

@@ -360,11 +360,38 @@ struct GranuleModel {
     static constexpr bool kClockOverride = false;
     static constexpr int EXC = 15, INH = 16, NMDA = 17, EXT = 18;
 
-    // EgidioGranuleCell_TimeDriven.cpp:252-365 (constants :26-58).  The
-    // synapse types come from the run-time mask; MASK is unused.
-    template <unsigned MASK, typename T = float>
-    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s,
-                                               T *d) {
+    // EgidioGranuleCell_TimeDriven.cpp:252-365 (constants :26-58), split into
+    // the parts the BDF Newton step reuses: the voltage-indexed table row and
+    // the calcium Nernst potential (Terms), the membrane equation (dv) and
+    // the gate / calcium equations.  The synapse types come from the run-time
+    // mask; MASK is unused.
+    struct Terms {
+        float v[kGrVars];   // channel table row at V
+        float vca;          // nernst(Ca, cao = 2, z = 2, 30 C)
+    };
+    __device__ static __forceinline__ void load_row(const RhsCtx &c, float V, float *v) {
+        int pos = trunc_i32_x86((V - kGrMinV) * kGrAux);
+        pos = pos < 0 ? 0 : (pos > kGrTable - 1 ? kGrTable - 1 : pos);
+        // a row is 104 bytes: 13 aligned float2 loads
+        const float2 *r2 = (const float2 *)(c.chan + pos * kGrVars);
+#pragma unroll
+        for (int q = 0; q < kGrVars / 2; q++) {
+            const float2 t = r2[q];
+            v[2 * q] = t.x;
+            v[2 * q + 1] = t.y;
+        }
+    }
+    __device__ static __forceinline__ float nernst_ca(const GranuleParams &p, float ca) {
+        return p.k_nernst * log_f(fabsf(2.0f / ca));
+    }
+    __device__ static __forceinline__ void prep(const ModelParams &mp, const RhsCtx &c, const float *s,
+                                                Terms &t) {
+        load_row(c, s[0], t.v);
+        t.vca = nernst_ca(mp.gran, s[14]);
+    }
+    // dV/dt (.cpp:342-348 with the synaptic current of :255-272)
+    __device__ static __forceinline__ float dv(const ModelParams &mp, const RhsCtx &c, const float *s,
+                                               float vca) {
         const GranuleParams &p = mp.gran;
         const float V = s[0];
         float cur = 0.0f;
@@ -377,23 +404,6 @@ struct GranuleModel {
         cur += s[EXT];
         cur = (float)((double)cur * 3.34156941e-4);   // float *= double literal
         const float pv = V;
-        const float vca = p.k_nernst * log_f(fabsf(2.0f / s[14]));   // nernst(Ca, cao = 2, z = 2, 30 C)
-        int pos = trunc_i32_x86((pv - kGrMinV) * kGrAux);
-        pos = pos < 0 ? 0 : (pos > kGrTable - 1 ? kGrTable - 1 : pos);
-        // a row is 104 bytes: 13 aligned float2 loads
-        const float2 *r2 = (const float2 *)(c.chan + pos * kGrVars);
-        float v[kGrVars];
-#pragma unroll
-        for (int q = 0; q < kGrVars / 2; q++) {
-            const float2 t = r2[q];
-            v[2 * q] = t.x;
-            v[2 * q + 1] = t.y;
-        }
-        const float ca = s[14];
-        const float a_kca = p.a_xkca / (1.0f + v[18] / ca);
-        const float b_kca = p.b_xkca / (1.0f + ca * v[19]);
-        const float xkca_inf = a_kca / (a_kca + b_kca);
-        const float inv_tau_xkca = a_kca + b_kca;
         const float gNa_f = 0.013f * s[1] * s[1] * s[1] * s[2];
         const float gNa_r = 0.0005f * s[3] * s[4];
         const float gNa_p = 0.0002f * s[5];
@@ -403,27 +413,44 @@ struct GranuleModel {
         const float gK_Ca = 0.004f * s[10];
         const float gCa = 0.00046f * s[11] * s[11] * s[12];
         const float gK_sl = 0.00035f * s[13];
-        d[1] = (v[0] - s[1]) * v[1];
-        d[2] = (v[2] - s[2]) * v[3];
-        d[3] = (v[4] - s[3]) * v[5];
-        d[4] = (v[6] - s[4]) * v[7];
-        d[5] = (v[8] - s[5]) * v[9];
-        d[6] = (v[10] - s[6]) * v[11];
-        d[7] = (v[12] - s[7]) * v[13];
-        d[8] = (v[14] - s[8]) * v[15];
-        d[9] = (v[16] - s[9]) * v[17];
-        d[10] = (xkca_inf - s[10]) * inv_tau_xkca;
-        d[11] = (v[20] - s[11]) * v[21];
-        d[12] = (v[22] - s[12]) * v[23];
-        d[13] = (v[24] - s[13]) * v[25];
-        // 2*F*A*d = ((2*96485.309f)*1e-04f)*0.2f; betaCa = 1.5, Ca0 = 1e-4
-        constexpr float kFAd = ((2 * 96485.309f) * 1e-04f) * 0.2f;
-        d[14] = (-gCa * (pv - vca) / kFAd - (1.5f * (ca - 1e-04f)));
         const float VNa = 87.39f, VK = -84.69f;
-        d[0] = (cur + gNa_f * (VNa - pv) + gNa_r * (VNa - pv) + gNa_p * (VNa - pv) + gK_V * (VK - pv) +
+        return (cur + gNa_f * (VNa - pv) + gNa_r * (VNa - pv) + gNa_p * (VNa - pv) + gK_V * (VK - pv) +
                 gK_A * (VK - pv) + gK_IR * (VK - pv) + gK_Ca * (VK - pv) + gCa * (vca - pv) +
                 gK_sl * (VK - pv) + 5.68e-5f * (-58.0f - pv) + 2.17e-5f * (-65.0f - pv)) *
                (1.0f / 1.0e-3f);
+    }
+    // voltage-gated variable k (1..9, 11..13): (x_inf - x)/tau from its table columns
+    __device__ static __forceinline__ float dgate(int k, const float *s, const float *v) {
+        const int col = k < 10 ? 2 * (k - 1) : 2 * k - 2;
+        return (v[col] - s[k]) * v[col + 1];
+    }
+    // Ca-dependent K gate (.cpp:300-305)
+    __device__ static __forceinline__ float dxkca(const GranuleParams &p, const float *s, const float *v) {
+        const float ca = s[14];
+        const float a_kca = p.a_xkca / (1.0f + v[18] / ca);
+        const float b_kca = p.b_xkca / (1.0f + ca * v[19]);
+        const float xkca_inf = a_kca / (a_kca + b_kca);
+        const float inv_tau_xkca = a_kca + b_kca;
+        return (xkca_inf - s[10]) * inv_tau_xkca;
+    }
+    // calcium (.cpp:340-341): 2*F*A*d = ((2*96485.309f)*1e-04f)*0.2f, betaCa 1.5, Ca0 1e-4
+    __device__ static __forceinline__ float dca(const float *s, float vca) {
+        constexpr float kFAd = ((2 * 96485.309f) * 1e-04f) * 0.2f;
+        const float gCa = 0.00046f * s[11] * s[11] * s[12];
+        return (-gCa * (s[0] - vca) / kFAd - (1.5f * (s[14] - 1e-04f)));
+    }
+    __device__ static __forceinline__ void rhs_terms(const ModelParams &mp, const RhsCtx &c, const float *s,
+                                                     const Terms &t, float *d) {
+#pragma unroll
+        for (int k = 1; k < 14; k++) d[k] = k == 10 ? dxkca(mp.gran, s, t.v) : dgate(k, s, t.v);
+        d[14] = dca(s, t.vca);
+        d[0] = dv(mp, c, s, t.vca);
+    }
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s, T *d) {
+        Terms t;
+        prep(mp, c, s, t);
+        rhs_terms(mp, c, s, t, d);
     }
     template <unsigned MASK, typename T = float>
     __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
@@ -771,6 +798,220 @@ __device__ __forceinline__ void integrate_bdf(const ModelParams &mp, const RhsCt
 #pragma unroll
     for (int j = 0; j < ND; j++) S[j] = P[j];
     M::template decay<MASK, T>(mp, S);
+}
+
+// ---------------------------------------------------------------------------
+// Granule BDF step (the same algorithm as integrate_bdf, BDFn.h:288-423 and
+// 682-786), arranged for a 15-variable system held in registers:
+//   * the Jacobian column of a gate perturbs one variable: only the rows
+//     whose equations read it are evaluated (the others difference to +0
+//     exactly, as in the reference), and the table row and the Nernst
+//     potential are reused unless V or Ca is the perturbed variable;
+//   * the inverse is formed in place: at pivot s the dead column s of the
+//     eliminated matrix becomes column s of the inverse, and the inverse's
+//     still-unit columns are not stored.  Every stored value takes exactly
+//     the reference's operations (li[i][s] = 0 + li[s][s]*coef included);
+//   * a zero pivot (the reference then swaps rows) or a non-finite value
+//     sends that Newton iteration through the reference arrangement
+//     (granule_newton_reference).
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ bool invert_in_place(float *A) {
+    bool ok = true;
+#pragma unroll
+    for (int s = 0; s < N; s++) {
+        const float element = A[s * N + s];
+        ok = ok && element != 0.0f;
+        const float inv = 1.0f / element;
+#pragma unroll
+        for (int j = 0; j < N; j++)
+            if (j != s) A[s * N + j] *= inv;
+        A[s * N + s] = inv;
+#pragma unroll
+        for (int i = 0; i < N; i++) {
+            if (i == s) continue;
+            const float coef = -A[i * N + s];
+#pragma unroll
+            for (int j = 0; j < N; j++)
+                if (j != s) A[i * N + j] += A[s * N + j] * coef;
+            A[i * N + s] = 0.0f + A[s * N + s] * coef;
+        }
+    }
+    bool fin = true;
+#pragma unroll
+    for (int q = 0; q < N * N; q++) fin = fin && __builtin_isfinite(A[q]);
+    return ok && fin;
+}
+
+struct GranuleNewtonIn {
+    float P[GranuleModel::NV];
+    float Cc[GranuleModel::ND];
+    float F[GranuleModel::ND];
+};
+struct GranuleNewtonOut {
+    float P1[GranuleModel::ND];
+};
+
+// One Newton correction exactly as the reference arranges it: full Jacobian
+// by one-sided differences, Gauss-Jordan inverse with row swaps, product.
+__device__ __noinline__ GranuleNewtonOut granule_newton_reference(const ModelParams &mp, const RhsCtx &c, float dt,
+                                                                  float c0, GranuleNewtonIn in) {
+    constexpr int ND = GranuleModel::ND, NV = GranuleModel::NV;
+    float jac[ND * ND], J[ND * ND], iJ[ND * ND], X[NV], Fp[ND], Fn[ND];
+    const float eps = dt * 0.1f;
+    const float inv_eps = 1.0f / eps;
+    for (int q = 0; q < NV; q++) X[q] = in.P[q];
+    GranuleModel::rhs<15u>(mp, c, X, Fp);
+    for (int j = 0; j < ND; j++) {
+        for (int q = 0; q < NV; q++) X[q] = in.P[q];
+        X[j] -= eps;
+        GranuleModel::rhs<15u>(mp, c, X, Fn);
+        for (int z = 0; z < ND; z++) jac[z * ND + j] = (Fp[z] - Fn[z]) * inv_eps;
+    }
+    for (int z = 0; z < ND; z++)
+        for (int t = 0; t < ND; t++) {
+            J[z * ND + t] = c0 * dt * jac[z * ND + t];
+            if (z == t) J[z * ND + t] -= 1;
+        }
+    invermat_dev<ND, float>(J, iJ);
+    GranuleNewtonOut out;
+    for (int z = 0; z < ND; z++) {
+        float aux = 0.0f;
+        for (int t = 0; t < ND; t++) aux += iJ[z * ND + t] * (in.P[t] - in.Cc[t]);
+        out.P1[z] = aux + in.P[z];
+    }
+    return out;
+}
+
+template <unsigned MASK>
+__device__ __forceinline__ void integrate_bdf_granule(const ModelParams &mp, const RhsCtx &c, float dt,
+                                                      const BdfState &b, int64_t ld, uint32_t i, int &st,
+                                                      float *S) {
+    using G = GranuleModel;
+    constexpr int ND = G::ND, NV = G::NV;
+    float *Dh = (float *)b.d;
+    float *Ph = (float *)b.prev;
+    float F[ND], P[NV], P1[ND], Cc[ND], A[ND * ND];
+    if (st == 0) {
+        G::rhs<MASK>(mp, c, S, F);
+#pragma unroll
+        for (int j = 0; j < ND; j++) P[j] = S[j] + dt * F[j];
+    } else {
+#pragma unroll
+        for (int j = 0; j < ND; j++) {
+            P[j] = S[j];
+            for (int q = 0; q < st; q++) P[j] += Dh[((int64_t)q * ND + j) * ld + i];
+        }
+    }
+#pragma unroll
+    for (int j = ND; j < NV; j++) P[j] = S[j];
+    G::decay<MASK>(mp, P);
+    float epsi = 1.0f;
+    int k = 0;
+    const float *Cs = b.C + st * 7;
+    const float c0 = Cs[0], c0dt = c0 * dt;
+    const float eps = dt * 0.1f;
+    const float inv_eps = 1.0f / eps;
+    while (epsi > 1e-6 && k < 10) {
+        G::Terms t;
+        G::prep(mp, c, P, t);
+        G::rhs_terms(mp, c, P, t, F);
+        bool fin = true;
+#pragma unroll
+        for (int j = 0; j < ND; j++) {
+            Cc[j] = c0 * dt * F[j] + Cs[1] * S[j];
+            for (int q = 1; q < st; q++) Cc[j] += Cs[q + 1] * Ph[((int64_t)(q - 1) * ND + j) * ld + i];
+            fin = fin && __builtin_isfinite(F[j]);
+        }
+        // J = c0*dt*jac - I, jac[z][j] = (F[z] - Fn_j[z]) / eps (one-sided, at P)
+#pragma unroll
+        for (int q = 0; q < ND * ND; q++) A[q] = 0.0f;
+        {   // V: every equation reads it (table row at the perturbed V)
+            float X[NV], Fn[ND];
+            G::Terms tv;
+#pragma unroll
+            for (int q = 0; q < NV; q++) X[q] = P[q];
+            X[0] -= eps;
+            G::load_row(c, X[0], tv.v);
+            tv.vca = t.vca;
+            G::rhs_terms(mp, c, X, tv, Fn);
+#pragma unroll
+            for (int z = 0; z < ND; z++) A[z * ND] = c0dt * ((F[z] - Fn[z]) * inv_eps);
+        }
+#pragma unroll
+        for (int j = 1; j < 14; j++) {   // gates: the membrane equation and their own (+ calcium for xCa, yCa)
+            float X[NV];
+#pragma unroll
+            for (int q = 0; q < NV; q++) X[q] = P[q];
+            X[j] -= eps;
+            A[j] = c0dt * ((F[0] - G::dv(mp, c, X, t.vca)) * inv_eps);
+            const float fj = j == 10 ? G::dxkca(mp.gran, X, t.v) : G::dgate(j, X, t.v);
+            A[j * ND + j] = c0dt * ((F[j] - fj) * inv_eps);
+            if (j == 11 || j == 12) A[14 * ND + j] = c0dt * ((F[14] - G::dca(X, t.vca)) * inv_eps);
+        }
+        {   // Ca: the membrane equation (Nernst potential), the Ca-dependent K gate, calcium
+            float X[NV];
+#pragma unroll
+            for (int q = 0; q < NV; q++) X[q] = P[q];
+            X[14] -= eps;
+            const float vca = G::nernst_ca(mp.gran, X[14]);
+            A[14] = c0dt * ((F[0] - G::dv(mp, c, X, vca)) * inv_eps);
+            A[10 * ND + 14] = c0dt * ((F[10] - G::dxkca(mp.gran, X, t.v)) * inv_eps);
+            A[14 * ND + 14] = c0dt * ((F[14] - G::dca(X, vca)) * inv_eps);
+        }
+#pragma unroll
+        for (int z = 0; z < ND; z++) A[z * ND + z] -= 1;
+        if (fin && invert_in_place<ND>(A)) {
+#pragma unroll
+            for (int z = 0; z < ND; z++) {
+                float aux = 0.0f;
+#pragma unroll
+                for (int q = 0; q < ND; q++) aux += A[z * ND + q] * (P[q] - Cc[q]);
+                P1[z] = aux + P[z];
+            }
+        } else {
+            GranuleNewtonIn in;
+#pragma unroll
+            for (int q = 0; q < NV; q++) in.P[q] = P[q];
+#pragma unroll
+            for (int q = 0; q < ND; q++) {
+                in.Cc[q] = Cc[q];
+                in.F[q] = F[q];
+            }
+            const GranuleNewtonOut o = granule_newton_reference(mp, c, dt, c0, in);
+#pragma unroll
+            for (int q = 0; q < ND; q++) P1[q] = o.P1[q];
+        }
+        float aux2 = 0.0f;
+#pragma unroll
+        for (int z = 0; z < ND; z++) {
+            const float aux = fabsf(P1[z] - P[z]);
+            if (aux > aux2) aux2 = aux;
+        }
+#pragma unroll
+        for (int z = 0; z < ND; z++) P[z] = P1[z];
+        epsi = aux2;
+        k++;
+    }
+    if (st < b.order) st++;
+#pragma unroll
+    for (int j = 0; j < ND; j++) {
+        for (int q = st - 1; q > 0; q--) Dh[((int64_t)q * ND + j) * ld + i] = -Dh[((int64_t)(q - 1) * ND + j) * ld + i];
+        Dh[(int64_t)j * ld + i] = P[j] - S[j];
+        for (int q = 1; q < st; q++)
+            Dh[((int64_t)q * ND + j) * ld + i] += Dh[((int64_t)(q - 1) * ND + j) * ld + i];
+    }
+    if (st > 1) {
+        for (int q = st - 2; q > 0; q--)
+#pragma unroll
+            for (int j = 0; j < ND; j++)
+                Ph[((int64_t)q * ND + j) * ld + i] = Ph[((int64_t)(q - 1) * ND + j) * ld + i];
+#pragma unroll
+        for (int j = 0; j < ND; j++) Ph[(int64_t)j * ld + i] = S[j];
+    }
+#pragma unroll
+    for (int j = 0; j < ND; j++) S[j] = P[j];
+    G::decay<MASK>(mp, S);
 }
 
 }  // namespace edl

@@ -485,10 +485,11 @@ int model_param_count(int model) {
 }
 
 // Neurons per thread of an ungrouped population's update kernel: the
-// granule's Runge-Kutta kernels hold two (19 state variables each), every
-// other kernel four.
+// granule's kernels hold two (19 state variables each; the BDF step keeps a
+// 15 x 15 matrix per neuron), every other kernel four.
 int plain_items(int model, int integrator) {
-    return (model == EDLUT_GRANULE && integrator != EDLUT_BDF) ? 2 : kItems;
+    (void)integrator;
+    return model == EDLUT_GRANULE ? 2 : kItems;
 }
 
 template <class M, int I, unsigned MASK>
@@ -548,8 +549,8 @@ struct MaskFreeTable {
         fn[3] = k_update<M, kRK2, 15u, true, ITEMS, false>;
         fn[4] = k_update<M, kRK4, 15u, false, ITEMS, false>;
         fn[5] = k_update<M, kRK4, 15u, true, ITEMS, false>;
-        fn[6] = k_update_bdf<M, 15u, false, float>;
-        fn[7] = k_update_bdf<M, 15u, true, float>;
+        fn[6] = k_update_bdf<M, 15u, false, float, ITEMS>;
+        fn[7] = k_update_bdf<M, 15u, true, float, ITEMS>;
     }
 };
 
@@ -946,8 +947,8 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     static const int kNv[5] = {5, 6, 8, 19, 7}, kNd[5] = {1, 2, 4, 15, 3};
     p->nv = kNv[model];
     p->nd = kNd[model];
-    // the granule's Runge-Kutta kernels hold two neurons per thread (19 state
-    // variables each); every other kernel holds four
+    // the granule's kernels hold two neurons per thread (19 state variables
+    // each); every other kernel holds four
     p->items = plain_items(model, integrator);
     std::vector<float> chan;
     if (model == EDLUT_HH) chan = build_hh_table(params[5]);

@@ -6,6 +6,7 @@
 //   k_poisson    one step of a Poisson generator device (per-source LCG).
 //   k_fold_*     ordered conductance accumulation of a step's input events.
 #pragma once
+#include <type_traits>
 #include "device_common.hpp"
 #include "device_models.hpp"
 
@@ -412,16 +413,18 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
 // iterations), kItems neurons per thread in the same tile layout as k_update.
 // T = double runs the whole BDF step in double (FP64 Newton mode); the state
 // is stored back in float.
-template <class M, unsigned MASK, bool EXT, typename T>
+template <class M, unsigned MASK, bool EXT, typename T, int ITEMS = kItems>
 __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
+    constexpr int TILE = kBlock * ITEMS;
+    static_assert(TILE % kFlagTile == 0, "update tiles cover whole fold flags");
     const int tile = blockIdx.x;
     const int32_t step = a.step.get();
     clock_begin(a, step);
     const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
     __shared__ uint32_t s_flag;
     if (threadIdx.x == 0) {
-        s_flag = take_tile_flag<kUpdTile>(a.fold.tile_flag, tile);
-        if (s_flag) clear_tile_flag<kUpdTile>(a.fold.tile_flag, tile);
+        s_flag = take_tile_flag<TILE>(a.fold.tile_flag, tile);
+        if (s_flag) clear_tile_flag<TILE>(a.fold.tile_flag, tile);
         if (tile == 0) a.fold.bump[par ^ 1] = 0ull;
     }
     __syncthreads();
@@ -430,10 +433,10 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     for (int j = 0; j < M::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
     const int64_t ld = a.stride;
     const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, step) : 0u;
-    bool spk[kItems];
-    int idx[kItems];
-    for (int n = 0; n < kItems; n++) {
-        idx[n] = tile_neuron<kItems>(tile, n);
+    bool spk[ITEMS];
+    int idx[ITEMS];
+    for (int n = 0; n < ITEMS; n++) {
+        idx[n] = tile_neuron<ITEMS>(tile, n);
         spk[n] = false;
         if (idx[n] >= a.n) continue;
         const uint32_t i = (uint32_t)idx[n];
@@ -481,7 +484,10 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         c.refr_len = a.refr_len;
         c.since_spike = a.last_spike ? (step - 1 - a.last_spike[i]) : (1 << 30);
         int st = a.bdf.st[i];
-        integrate_bdf<M, MASK, T>(a.mp, c, (T)a.k.dt, a.bdf, ld, i, st, S);
+        if constexpr (std::is_same<M, GranuleModel>::value)
+            integrate_bdf_granule<MASK>(a.mp, c, a.k.dt, a.bdf, ld, i, st, S);
+        else
+            integrate_bdf<M, MASK, T>(a.mp, c, (T)a.k.dt, a.bdf, ld, i, st, S);
         const bool fired = M::spike(a.mp, prev, S);
         if constexpr (M::kClockOverride) M::clock_override(c, fired, S);
         if (fired && M::kResetsIntegrator) st = 0;
@@ -502,7 +508,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
             if (MASK & 4u) row[M::NMDA][i] = out[M::NMDA];
         }
     }
-    tile_spikes<kItems>(a.out, tile, spk, idx);
+    tile_spikes<ITEMS>(a.out, tile, spk, idx);
     clock_finish(a, step);
 }
 
