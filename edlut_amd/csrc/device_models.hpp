@@ -705,6 +705,12 @@ struct BdfState {
     void *d;           // T[order][ND][stride]
     void *prev;        // T[order-1][ND][stride]
     int32_t *st;       // [stride]
+    // granule: working storage of the reference-arranged Newton correction,
+    // ws_cap slots per step handed out by ws_count[step parity]
+    float *ws;
+    uint32_t *ws_count;
+    int ws_cap;
+    uint32_t *err;
 };
 
 template <class M, unsigned MASK, typename T>
@@ -813,7 +819,8 @@ __device__ __forceinline__ void integrate_bdf(const ModelParams &mp, const RhsCt
 //     the reference's operations (li[i][s] = 0 + li[s][s]*coef included);
 //   * a zero pivot (the reference then swaps rows) or a non-finite value
 //     sends that Newton iteration through the reference arrangement
-//     (granule_newton_reference).
+//     (granule_newton_reference) on a global working-storage slot, so that
+//     the kernel keeps no large private frame.
 // ---------------------------------------------------------------------------
 template <int N>
 __device__ __forceinline__ bool invert_in_place(float *A) {
@@ -843,50 +850,75 @@ __device__ __forceinline__ bool invert_in_place(float *A) {
     return ok && fin;
 }
 
-struct GranuleNewtonIn {
-    float P[GranuleModel::NV];
-    float Cc[GranuleModel::ND];
-    float F[GranuleModel::ND];
-};
-struct GranuleNewtonOut {
-    float P1[GranuleModel::ND];
-};
+// Floats of one working-storage slot: la, li (15 x 15 each), X (19), Fp, Fn,
+// Cc, P1 (15 each).
+constexpr int kGrWsFloats = 2 * 225 + 19 + 4 * 15;
 
-// One Newton correction exactly as the reference arranges it: full Jacobian
-// by one-sided differences, Gauss-Jordan inverse with row swaps, product.
-__device__ __noinline__ GranuleNewtonOut granule_newton_reference(const ModelParams &mp, const RhsCtx &c, float dt,
-                                                                  float c0, GranuleNewtonIn in) {
+// One Newton correction exactly as the reference arranges it (BDFn.h:
+// 372-400, 682-786): full Jacobian by one-sided differences, Gauss-Jordan
+// inverse with row swaps, product.  Works on a global working-storage slot
+// w, whose X holds the iterate and Cc the corrector constants on entry and
+// whose P1 receives the corrected iterate.
+__device__ __forceinline__ void granule_newton_reference(const ModelParams &mp, const RhsCtx &c, float dt,
+                                                         float c0, float *w) {
     constexpr int ND = GranuleModel::ND, NV = GranuleModel::NV;
-    float jac[ND * ND], J[ND * ND], iJ[ND * ND], X[NV], Fp[ND], Fn[ND];
+    float *la = w, *li = w + 225, *X = w + 450, *Fp = X + NV, *Fn = Fp + ND, *Cc = Fn + ND, *P1 = Cc + ND;
     const float eps = dt * 0.1f;
     const float inv_eps = 1.0f / eps;
-    for (int q = 0; q < NV; q++) X[q] = in.P[q];
     GranuleModel::rhs<15u>(mp, c, X, Fp);
     for (int j = 0; j < ND; j++) {
-        for (int q = 0; q < NV; q++) X[q] = in.P[q];
-        X[j] -= eps;
+        const float xj = X[j];
+        X[j] = xj - eps;
         GranuleModel::rhs<15u>(mp, c, X, Fn);
-        for (int z = 0; z < ND; z++) jac[z * ND + j] = (Fp[z] - Fn[z]) * inv_eps;
+        X[j] = xj;
+        for (int z = 0; z < ND; z++) la[z * ND + j] = c0 * dt * ((Fp[z] - Fn[z]) * inv_eps);
     }
-    for (int z = 0; z < ND; z++)
-        for (int t = 0; t < ND; t++) {
-            J[z * ND + t] = c0 * dt * jac[z * ND + t];
-            if (z == t) J[z * ND + t] -= 1;
+    for (int z = 0; z < ND; z++) la[z * ND + z] -= 1;
+    for (int q = 0; q < ND * ND; q++) li[q] = 0;
+    for (int q = 0; q < ND; q++) li[q * ND + q] = 1.0f;
+    for (int s = 0; s < ND; s++) {
+        float element = la[s * ND + s];
+        if (element == 0) {
+            for (int r = s + 1; r < ND; r++) {
+                element = la[r * ND + s];
+                if (element != 0) {
+                    for (int m = 0; m < ND; m++) {
+                        float v = la[r * ND + m];
+                        la[r * ND + m] = la[s * ND + m];
+                        la[s * ND + m] = v;
+                        v = li[r * ND + m];
+                        li[r * ND + m] = li[s * ND + m];
+                        li[s * ND + m] = v;
+                    }
+                    break;
+                }
+            }
         }
-    invermat_dev<ND, float>(J, iJ);
-    GranuleNewtonOut out;
+        const float inv_element = 1.0f / element;
+        for (int j = 0; j < ND; j++) {
+            la[s * ND + j] *= inv_element;
+            li[s * ND + j] *= inv_element;
+        }
+        for (int i = 0; i < ND; i++) {
+            if (i == s) continue;
+            const float coef = -la[i * ND + s];
+            for (int j = 0; j < ND; j++) {
+                la[i * ND + j] += la[s * ND + j] * coef;
+                li[i * ND + j] += li[s * ND + j] * coef;
+            }
+        }
+    }
     for (int z = 0; z < ND; z++) {
         float aux = 0.0f;
-        for (int t = 0; t < ND; t++) aux += iJ[z * ND + t] * (in.P[t] - in.Cc[t]);
-        out.P1[z] = aux + in.P[z];
+        for (int t = 0; t < ND; t++) aux += li[z * ND + t] * (X[t] - Cc[t]);
+        P1[z] = aux + X[z];
     }
-    return out;
 }
 
 template <unsigned MASK>
 __device__ __forceinline__ void integrate_bdf_granule(const ModelParams &mp, const RhsCtx &c, float dt,
-                                                      const BdfState &b, int64_t ld, uint32_t i, int &st,
-                                                      float *S) {
+                                                      const BdfState &b, int par, int64_t ld, uint32_t i,
+                                                      int &st, float *S) {
     using G = GranuleModel;
     constexpr int ND = G::ND, NV = G::NV;
     float *Dh = (float *)b.d;
@@ -970,17 +1002,22 @@ __device__ __forceinline__ void integrate_bdf_granule(const ModelParams &mp, con
                 P1[z] = aux + P[z];
             }
         } else {
-            GranuleNewtonIn in;
+            const uint32_t slot = atomicAdd(&b.ws_count[par], 1u);
+            if (slot < (uint32_t)b.ws_cap) {
+                float *w = b.ws + (size_t)slot * kGrWsFloats;
+                float *X = w + 450, *Cw = X + NV + 2 * ND, *P1w = Cw + ND;
 #pragma unroll
-            for (int q = 0; q < NV; q++) in.P[q] = P[q];
+                for (int q = 0; q < NV; q++) X[q] = P[q];
 #pragma unroll
-            for (int q = 0; q < ND; q++) {
-                in.Cc[q] = Cc[q];
-                in.F[q] = F[q];
+                for (int q = 0; q < ND; q++) Cw[q] = Cc[q];
+                granule_newton_reference(mp, c, dt, c0, w);
+#pragma unroll
+                for (int q = 0; q < ND; q++) P1[q] = P1w[q];
+            } else {
+                atomicOr(b.err, 16u);   // reported to the host; the step is not the reference's
+#pragma unroll
+                for (int q = 0; q < ND; q++) P1[q] = P[q];
             }
-            const GranuleNewtonOut o = granule_newton_reference(mp, c, dt, c0, in);
-#pragma unroll
-            for (int q = 0; q < ND; q++) P1[q] = o.P1[q];
         }
         float aux2 = 0.0f;
 #pragma unroll

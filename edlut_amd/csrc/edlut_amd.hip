@@ -291,6 +291,8 @@ struct edlut_population {
     float *nmda_lut = nullptr;
     float *refr = nullptr;
     int refr_len = 0;
+    float *bdf_ws = nullptr;          // granule BDF: fallback working storage
+    uint32_t *bdf_ws_count = nullptr; // [2] slots handed out, by step parity
     uint64_t lut_sums[3] = {0, 0, 0};
 
     // spikes of the last update (tile counts + staging, then compacted)
@@ -478,6 +480,11 @@ namespace {
 
 using UpdateFn = void (*)(UpdateArgs);
 
+// Granule BDF: Newton corrections per step that can take the reference's
+// row-swapping arrangement (zero pivot or non-finite values; in practice
+// none); more in one step is reported as a capacity error.
+constexpr int kBdfWsSlots = 4096;
+
 // Float parameters of each model's GetDefaultParameters() vector.
 int model_param_count(int model) {
     static const int n[5] = {10, 14, 13, 0, 8};
@@ -618,6 +625,7 @@ int check_err(edlut_population *p) {
     if (e & 2u) return fail(EDLUT_E_CAP, "spike log capacity exceeded");
     if (e & 4u) return fail(EDLUT_E_CAP, "input event buffer capacity exceeded");
     if (e & 8u) return fail(EDLUT_E_STATE, "input fold segment wait exceeded its spin limit");
+    if (e & 16u) return fail(EDLUT_E_CAP, "BDF fallback working storage exhausted in one step");
     return 0;
 }
 
@@ -642,6 +650,10 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.bdf.d = p->bdf_d;
     a.bdf.prev = p->bdf_prev;
     a.bdf.st = p->bdf_st;
+    a.bdf.ws = p->bdf_ws;
+    a.bdf.ws_count = p->bdf_ws_count;
+    a.bdf.ws_cap = p->bdf_ws ? kBdfWsSlots : 0;
+    a.bdf.err = p->err;
     a.grp.tab = p->d_groups;
     a.grp.chunk_group = p->d_chunk_group;
     return a;
@@ -1008,6 +1020,12 @@ int create_population(int model, int integrator, int bdf_order, const float *par
         if (hipMemcpy(p->bdf_c, C, sizeof(C), hipMemcpyHostToDevice) != hipSuccess ||
             hipMemset(p->bdf_st, 0, sizeof(int32_t) * p->stride) != hipSuccess)
             return cleanup(fail(EDLUT_E_HIP, "BDF initialisation failed"));
+        if (model == EDLUT_GRANULE) {
+            if ((rc = dalloc(&p->bdf_ws, (size_t)kBdfWsSlots * kGrWsFloats)) || (rc = dalloc(&p->bdf_ws_count, 2)))
+                return cleanup(rc);
+            if (hipMemset(p->bdf_ws_count, 0, 2 * sizeof(uint32_t)) != hipSuccess)
+                return cleanup(fail(EDLUT_E_HIP, "BDF initialisation failed"));
+        }
     }
     if ((rc = dalloc(&p->tile_count, padded_tiles((int)max_tiles)))) return cleanup(rc);
     if ((rc = dalloc(&p->staging, (size_t)p->stride))) return cleanup(rc);
@@ -1099,7 +1117,7 @@ int edlut_population_destroy(edlut_population *p) {
     }
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
-    void *dev[] = {p->chan, p->bdf_c, p->bdf_d, p->bdf_prev, p->bdf_st,
+    void *dev[] = {p->chan, p->bdf_c, p->bdf_d, p->bdf_prev, p->bdf_st, p->bdf_ws, p->bdf_ws_count,
                    p->state, p->last_spike, p->exp_lut, p->nmda_lut, p->refr, p->tile_count,
                    p->staging, p->spk_idx, p->spk_cnt, p->err, p->cnt, p->fill, p->seg,
                    p->tile_flag, p->bump, p->b_ord, p->b_w, p->e_tgt,

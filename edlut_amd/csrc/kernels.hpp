@@ -425,7 +425,11 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     if (threadIdx.x == 0) {
         s_flag = take_tile_flag<TILE>(a.fold.tile_flag, tile);
         if (s_flag) clear_tile_flag<TILE>(a.fold.tile_flag, tile);
-        if (tile == 0) a.fold.bump[par ^ 1] = 0ull;
+        if (tile == 0) {
+            a.fold.bump[par ^ 1] = 0ull;
+            // the next step hands out working-storage slots from the other parity
+            if (a.bdf.ws_count) a.bdf.ws_count[par ^ 1] = 0u;
+        }
     }
     __syncthreads();
     float *__restrict__ row[M::NV];
@@ -485,7 +489,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         c.since_spike = a.last_spike ? (step - 1 - a.last_spike[i]) : (1 << 30);
         int st = a.bdf.st[i];
         if constexpr (std::is_same<M, GranuleModel>::value)
-            integrate_bdf_granule<MASK>(a.mp, c, a.k.dt, a.bdf, ld, i, st, S);
+            integrate_bdf_granule<MASK>(a.mp, c, a.k.dt, a.bdf, par, ld, i, st, S);
         else
             integrate_bdf<M, MASK, T>(a.mp, c, (T)a.k.dt, a.bdf, ld, i, st, S);
         const bool fired = M::spike(a.mp, prev, S);
