@@ -164,6 +164,14 @@ def lib():
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'`"
             )
+        # PyTorch ships its own HIP runtime under the same soname.  Loading it
+        # first makes this library bind to that one runtime; loaded the other
+        # way round, torch finds no GPU once this library has initialised the
+        # system runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         l = ctypes.CDLL(LIB_PATH)
         _sig(l)
         _lib = l
