@@ -298,6 +298,8 @@ def main():
     ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-cerebellum", action="store_true")
+    ap.add_argument("--cerebellum-only", action="store_true",
+                    help="run only the cerebellum leg (for profiling its kernels)")
     ap.add_argument("--cerebellum-steps", type=int, default=2000)
     ap.add_argument("--cerebellum-cpu-steps", type=int, default=3)
     args = ap.parse_args()
@@ -310,6 +312,9 @@ def main():
 
     if args.calibrate_copy_mb:
         calibration_copy(args.calibrate_copy_mb << 20)
+    if args.cerebellum_only:
+        print(json.dumps({"cerebellum": bench_cerebellum(args.cerebellum_steps, 100, 0)}))
+        return
 
     t_setup = time.perf_counter()
     spec = workloads.adex_recurrent(n=args.neurons, steps=args.warmup + args.steps, w_ext=args.w_ext)
