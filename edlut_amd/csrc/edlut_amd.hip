@@ -638,6 +638,9 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.n_tiles = p->n_tiles;
     a.step = StepRef{nullptr, step};
     a.mp = p->mp;
+    // the granule and Purkinje kernels read the enabled synapse types at run
+    // time: take them from the population (connections may have enabled more)
+    a.mp.gran.mask = a.mp.pkj.mask = p->mask;
     a.k = p->k;
     a.exp_lut = p->exp_lut;
     a.nmda_lut = p->nmda_lut;
