@@ -485,6 +485,12 @@ using UpdateFn = void (*)(UpdateArgs);
 // none); more in one step is reported as a capacity error.
 constexpr int kBdfWsSlots = 4096;
 
+// Dynamic LDS of a population's update kernel: the granule's BDF kernel
+// stages the channel table (1024 x 26 floats) there.
+size_t update_lds_bytes(const edlut_population *p) {
+    return (p->model == EDLUT_GRANULE && p->integ == EDLUT_BDF) ? sizeof(float) * kGrTable * kGrVars : 0;
+}
+
 // Float parameters of each model's GetDefaultParameters() vector.
 int model_param_count(int model) {
     static const int n[5] = {10, 14, 13, 0, 8};
@@ -592,6 +598,15 @@ UpdateFn pick_update(const edlut_population *p, bool ext) {
         if (p->items != plain_items(model, integ)) return nullptr;
         static MaskFreeTable<GranuleModel, 2> gr;
         static MaskFreeTable<PurkinjeModel, kItems> pk;
+        static const bool lds_ok = [] {
+            // more than the default 64 KB of dynamic LDS for the table-staging kernels
+            bool ok = true;
+            for (int q = 6; q < 8; q++)
+                ok = ok && hipFuncSetAttribute((const void *)gr.fn[q], hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)(sizeof(float) * kGrTable * kGrVars)) == hipSuccess;
+            return ok;
+        }();
+        if (model == EDLUT_GRANULE && integ == EDLUT_BDF && !lds_ok) return nullptr;
         const int idx = integ * 2 + (ext ? 1 : 0);
         return model == EDLUT_GRANULE ? gr.fn[idx] : pk.fn[idx];
     }
@@ -1216,7 +1231,7 @@ int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
     if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
     p->started = true;
     const int tiles = p->update_tiles();
-    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), 0, p->stream, a);
+    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
     launch_compact(p->stream, p->tile_count, p->staging, tiles, kBlock * p->items, p->spk_idx, p->spk_cnt,
                    1, 0, nullptr, nullptr, nullptr, 0, p->err, StepRef{nullptr, (int32_t)step});
     HIP_TRY(hipGetLastError());
@@ -1548,7 +1563,7 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         a.clk_first = (int32_t)n->clk_first;
         a.clk_count = (int32_t)n->clk_count;
     }
-    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), 0, p->stream, a);
+    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
     CompactArgs c = compact_args(n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
                                  n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
                                  n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);

@@ -437,6 +437,17 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     for (int j = 0; j < M::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
     const int64_t ld = a.stride;
     const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, step) : 0u;
+    // The granule's channel table (104 KB) is staged in LDS: its workgroups
+    // run one per CU (the Newton step fills the register file), so the LDS
+    // is free, and every right-hand side gathers a 26-float row from it.
+    const float *chan = a.chan;
+    if constexpr (std::is_same<M, GranuleModel>::value) {
+        extern __shared__ float4 s_chan[];
+        const float4 *g4 = (const float4 *)a.chan;
+        for (int q = threadIdx.x; q < kGrTable * kGrVars / 4; q += kBlock) s_chan[q] = g4[q];
+        __syncthreads();
+        chan = (const float *)s_chan;
+    }
     bool spk[ITEMS];
     int idx[ITEMS];
     for (int n = 0; n < ITEMS; n++) {
@@ -481,7 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         for (int j = 0; j < M::NV; j++) S[j] = sf[j];
         const T prev = S[0];
         RhsCtx c;
-        c.chan = a.chan;
+        c.chan = chan;
         c.exp_lut = a.exp_lut;
         c.nmda_lut = a.nmda_lut;
         c.refr = a.refr;
