@@ -10,6 +10,7 @@ from .models import (
     CreateNeuronModel,
     EgidioGranuleCell_TimeDriven_GPU,
     HHTimeDrivenModel_GPU,
+    IzhikevichTimeDrivenModel_GPU,
     LIFTimeDrivenModel_GPU,
     REGISTRY,
     TimeDrivenNeuronModel_GPU,
@@ -26,6 +27,7 @@ __all__ = [
     "HHTimeDrivenModel_GPU",
     "EgidioGranuleCell_TimeDriven_GPU",
     "TimeDrivenPurkinjeCell_GPU",
+    "IzhikevichTimeDrivenModel_GPU",
     "TimeDrivenNeuronModel_GPU",
     "CreateNeuronModel",
     "REGISTRY",

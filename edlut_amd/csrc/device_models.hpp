@@ -100,12 +100,20 @@ struct PurkinjeParams {
     uint32_t mask;
 };
 
+struct IzhParams {
+    float a, b, c, d, e_exc, e_inh, inv_c_m, nmda_aux;
+    float cf[3];
+    float init[6];
+    uint32_t mask;   // run-time synapse mask, as for the granule
+};
+
 struct ModelParams {
     AdExParams adex;
     LIFParams lif;
     HHParams hh;
     GranuleParams gran;
     PurkinjeParams pkj;
+    IzhParams izh;
 };
 
 // Per-thread context the right-hand sides need besides the state.
@@ -297,6 +305,56 @@ struct HHModel {
     }
     __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
         return mp.hh.init[v];
+    }
+};
+
+// Izhikevich (IzhikevichTimeDrivenModel.cpp): V, u | g_exc, g_inh, g_nmda, ext_I.
+struct IzhModel {
+    static constexpr int NV = 6, ND = 2;
+    static constexpr bool kResetsIntegrator = true;
+    static constexpr bool kClockOverride = false;
+    static constexpr int EXC = 2, INH = 3, NMDA = 4, EXT = 5;
+
+    // IzhikevichTimeDrivenModel.cpp:169-195 (run-time synapse mask; MASK unused)
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s, T *d) {
+        const IzhParams &p = mp.izh;
+        const float V = s[0];
+        float cur = 0.0f;
+        if (p.mask & 1u) cur += s[EXC] * (p.e_exc - V);
+        if (p.mask & 2u) cur += s[INH] * (p.e_inh - V);
+        if (p.mask & 4u) {
+            const float gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, V);
+            cur += s[NMDA] * gi * (p.e_exc - V);
+        }
+        cur += s[EXT];
+        if (V <= 30.0f) {
+            d[0] = 0.04f * V * V + 5 * V + 140 - s[1] + (cur * p.inv_c_m);
+            d[1] = p.a * (p.b * V - s[1]);
+        } else {
+            d[0] = 0.0f;
+            d[1] = 0.0f;
+        }
+    }
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
+        const IzhParams &p = mp.izh;
+        if (p.mask & 1u) s[EXC] = decay_g(s[EXC], p.cf[0]);
+        if (p.mask & 2u) s[INH] = decay_g(s[INH], p.cf[1]);
+        if (p.mask & 4u) s[NMDA] = decay_g(s[NMDA], p.cf[2]);
+    }
+    // :156-165: above 30 mV, V resets to c and u grows by d
+    template <typename T = float>
+    __device__ static __forceinline__ bool spike(const ModelParams &mp, T, T *s) {
+        if (s[0] > 30.0f) {
+            s[0] = mp.izh.c;
+            s[1] += mp.izh.d;
+            return true;
+        }
+        return false;
+    }
+    __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
+        return mp.izh.init[v];
     }
 };
 

@@ -493,8 +493,8 @@ size_t update_lds_bytes(const edlut_population *p) {
 
 // Float parameters of each model's GetDefaultParameters() vector.
 int model_param_count(int model) {
-    static const int n[5] = {10, 14, 13, 0, 8};
-    return model >= 0 && model < 5 ? n[model] : -1;
+    static const int n[6] = {10, 14, 13, 0, 8, 10};
+    return model >= 0 && model < 6 ? n[model] : -1;
 }
 
 // Neurons per thread of an ungrouped population's update kernel: the
@@ -594,6 +594,11 @@ UpdateFn pick_update(const edlut_population *p, bool ext) {
         if (model == EDLUT_LIF) return lif_g.fn[idx];
         return nullptr;
     }
+    if (model == EDLUT_IZHIKEVICH) {
+        if (p->items != kItems) return nullptr;
+        static MaskFreeTable<IzhModel, kItems> iz;
+        return iz.fn[integ * 2 + (ext ? 1 : 0)];
+    }
     if (model == EDLUT_GRANULE || model == EDLUT_PURKINJE) {
         if (p->items != plain_items(model, integ)) return nullptr;
         static MaskFreeTable<GranuleModel, 2> gr;
@@ -655,7 +660,7 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.mp = p->mp;
     // the granule and Purkinje kernels read the enabled synapse types at run
     // time: take them from the population (connections may have enabled more)
-    a.mp.gran.mask = a.mp.pkj.mask = p->mask;
+    a.mp.gran.mask = a.mp.pkj.mask = a.mp.izh.mask = p->mask;
     a.k = p->k;
     a.exp_lut = p->exp_lut;
     a.nmda_lut = p->nmda_lut;
@@ -870,6 +875,25 @@ void model_params(int model, int integrator, const float *x, float dt_ms, ModelP
         const float *c = &chan[(size_t)chan_index(x[3], kPkMinV, kPkAux) * 4];
         init = {x[3], c[0] / c[1], c[2] / c[3], 0, 0, 0, 0};
         for (int j = 0; j < 7; j++) q.init[j] = init[j];
+    } else if (model == EDLUT_IZHIKEVICH) {
+        // a b c d e_exc e_inh c_m tau_exc tau_inh tau_nmda
+        // (IzhikevichTimeDrivenModel.cpp:298-375, defaults :389-401)
+        IzhParams &q = mp.izh;
+        q.a = x[0];
+        q.b = x[1];
+        q.c = x[2];
+        q.d = x[3];
+        q.e_exc = e_exc = x[4];
+        q.e_inh = e_inh = x[5];
+        q.inv_c_m = inv_of(x[6]);
+        q.cf[0] = expf(-elapsed * inv_of(x[7]));
+        q.cf[1] = expf(-elapsed * inv_of(x[8]));
+        q.cf[2] = expf(-elapsed * inv_of(x[9]));
+        // :131-135: resting equilibrium (lower root of 0.04V^2 + (5-b)V + 140 = 0, u = bV)
+        const float b = x[1];
+        const float veq = (((b - 5.0f) - sqrtf((5.0f - b) * (5.0f - b) - 22.4f)) / 0.08f);
+        init = {veq, veq * b, 0, 0, 0, 0};
+        for (int j = 0; j < 6; j++) q.init[j] = init[j];
     } else {
         LIFParams &q = mp.lif;
         q.e_exc = e_exc = x[0];
@@ -892,7 +916,7 @@ int create_population(int model, int integrator, int bdf_order, const float *par
                       edlut_population **out) {
     if (!out) return fail(EDLUT_E_ARG, "out is null");
     *out = nullptr;
-    if (model < EDLUT_LIF || model > EDLUT_PURKINJE)
+    if (model < EDLUT_LIF || model > EDLUT_IZHIKEVICH)
         return fail(EDLUT_E_ARG, "neuron model %d is not available in this build", model);
     if (integrator < EDLUT_EULER || integrator > EDLUT_BDF)
         return fail(EDLUT_E_ARG, "integration method %d is not available for this model", integrator);
@@ -969,12 +993,13 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     else if (model == EDLUT_HH) p->mp.hh.nmda_aux = nmda_aux;
     else if (model == EDLUT_GRANULE) p->mp.gran.nmda_aux = nmda_aux;
     else if (model == EDLUT_PURKINJE) p->mp.pkj.nmda_aux = nmda_aux;
+    else if (model == EDLUT_IZHIKEVICH) p->mp.izh.nmda_aux = nmda_aux;
     else p->mp.lif.nmda_aux = nmda_aux;
     for (GroupInfo &g : gtab) {
         if (model == EDLUT_ADEX) g.mp.adex.nmda_aux = nmda_aux;
         else g.mp.lif.nmda_aux = nmda_aux;
     }
-    static const int kNv[5] = {5, 6, 8, 19, 7}, kNd[5] = {1, 2, 4, 15, 3};
+    static const int kNv[6] = {5, 6, 8, 19, 7, 6}, kNd[6] = {1, 2, 4, 15, 3, 2};
     p->nv = kNv[model];
     p->nd = kNd[model];
     // the granule's kernels hold two neurons per thread (19 state variables
@@ -1161,7 +1186,7 @@ int edlut_population_enable_synapses(edlut_population *p, uint32_t mask) {
     if (!p) return fail(EDLUT_E_ARG, "population is null");
     if (mask & ~15u) return fail(EDLUT_E_ARG, "unknown synapse type bits 0x%x", mask);
     p->mask |= mask;
-    p->mp.gran.mask = p->mp.pkj.mask = p->mask;   // run-time mask of these two models
+    p->mp.gran.mask = p->mp.pkj.mask = p->mp.izh.mask = p->mask;   // models with a run-time mask
     return 0;
 }
 

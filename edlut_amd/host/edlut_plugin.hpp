@@ -278,4 +278,15 @@ class TimeDrivenPurkinjeCell_GPU : public TimeDrivenNeuronModel_GPU {
     std::string GetName() const override { return "TimeDrivenPurkinjeCell_GPU"; }
 };
 
+// IzhikevichTimeDrivenModel_GPU (defaults of IzhikevichTimeDrivenModel.cpp:389-401).
+class IzhikevichTimeDrivenModel_GPU : public TimeDrivenNeuronModel_GPU {
+  public:
+    IzhikevichTimeDrivenModel_GPU()
+        : TimeDrivenNeuronModel_GPU(EDLUT_IZHIKEVICH, {{"a", 0.1f}, {"b", 0.23f}, {"c", -65.0f}, {"d", 0.2f},
+                                                       {"e_exc", 0.0f}, {"e_inh", -80.0f}, {"c_m", 100.0f},
+                                                       {"tau_exc", 5.0f}, {"tau_inh", 10.0f},
+                                                       {"tau_nmda", 20.0f}}) {}
+    std::string GetName() const override { return "IzhikevichTimeDrivenModel_GPU"; }
+};
+
 }  // namespace edlut_amd

@@ -396,6 +396,29 @@ class TimeDrivenPurkinjeCell_GPU(TimeDrivenNeuronModel_GPU):
     N_DIFF = 3
 
 
+class IzhikevichTimeDrivenModel_GPU(TimeDrivenNeuronModel_GPU):
+    """Izhikevich's simple model with conductance-based synapses
+    (Kernel/src/neuron_model/IzhikevichTimeDrivenModel_GPU_C_INTERFACE.cu;
+    defaults of IzhikevichTimeDrivenModel.cpp:389-401)."""
+
+    NAME = "IzhikevichTimeDrivenModel_GPU"
+    MODEL = C.EDLUT_IZHIKEVICH
+    PARAMS = (
+        ("a", 0.1),
+        ("b", 0.23),
+        ("c", -65.0),
+        ("d", 0.2),
+        ("e_exc", 0.0),
+        ("e_inh", -80.0),
+        ("c_m", 100.0),
+        ("tau_exc", 5.0),
+        ("tau_inh", 10.0),
+        ("tau_nmda", 20.0),
+    )
+    STATE_VARS = ("V_m", "u", "g_exc", "g_inh", "g_nmda", "ext_I")
+    N_DIFF = 2
+
+
 # NeuronModelFactory registry (Kernel/src/neuron_model/NeuronModelFactory.cu:66-91)
 REGISTRY = {
     LIFTimeDrivenModel_GPU.NAME: LIFTimeDrivenModel_GPU,
@@ -403,6 +426,7 @@ REGISTRY = {
     HHTimeDrivenModel_GPU.NAME: HHTimeDrivenModel_GPU,
     EgidioGranuleCell_TimeDriven_GPU.NAME: EgidioGranuleCell_TimeDriven_GPU,
     TimeDrivenPurkinjeCell_GPU.NAME: TimeDrivenPurkinjeCell_GPU,
+    IzhikevichTimeDrivenModel_GPU.NAME: IzhikevichTimeDrivenModel_GPU,
 }
 
 

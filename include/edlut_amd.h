@@ -27,7 +27,8 @@ enum edlut_model {
     EDLUT_ADEX = 1,     /* "AdExTimeDrivenModel_GPU" (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:568-570) */
     EDLUT_HH = 2,       /* "HHTimeDrivenModel_GPU" */
     EDLUT_GRANULE = 3,  /* "EgidioGranuleCell_TimeDriven_GPU" (EgidioGranuleCell_TimeDriven_GPU_C_INTERFACE.cu) */
-    EDLUT_PURKINJE = 4  /* "TimeDrivenPurkinjeCell_GPU" (TimeDrivenPurkinjeCell_GPU_C_INTERFACE.cu) */
+    EDLUT_PURKINJE = 4, /* "TimeDrivenPurkinjeCell_GPU" (TimeDrivenPurkinjeCell_GPU_C_INTERFACE.cu) */
+    EDLUT_IZHIKEVICH = 5 /* "IzhikevichTimeDrivenModel_GPU" (IzhikevichTimeDrivenModel_GPU_C_INTERFACE.cu) */
 };
 /* Integration methods; names of IntegrationMethodFactory_GPU_C_INTERFACE.cu:40-145. */
 enum edlut_integrator { EDLUT_EULER = 0, EDLUT_RK2 = 1, EDLUT_RK4 = 2, EDLUT_BDF = 3 };
@@ -65,6 +66,8 @@ const char *edlut_last_error(void);
  *         be null); its constants are fixed (EgidioGranuleCell_TimeDriven.cpp:26-58)
  *   Purkinje (TimeDrivenPurkinjeCell_GPU): e_exc e_inh v_thr e_leak tau_exc
  *         tau_inh tau_nmda tau_ref (TimeDrivenPurkinjeCell.cpp:423-434)
+ *   Izhikevich (IzhikevichTimeDrivenModel_GPU): a b c d e_exc e_inh c_m tau_exc
+ *         tau_inh tau_nmda (IzhikevichTimeDrivenModel.cpp:389-401)
  * bdf_order (1..6) is the BDF method's order ("bdf_order" parameter,
  * BDFn_GPU_C_INTERFACE.cuh:275-288); ignored by the other integrators.
  * dt_s is the integration method's "step" parameter (seconds).  State lives on

@@ -816,6 +816,80 @@ template <typename R> struct Purkinje : PopBase<R> {
 };
 
 /* ------------------------------------------------------------------------ */
+/* Izhikevich: Kernel/src/neuron_model/IzhikevichTimeDrivenModel.cpp          */
+/* state: V, u | g_exc, g_inh, g_nmda, ext_I                                  */
+/* ------------------------------------------------------------------------ */
+template <typename R> struct Izhikevich : PopBase<R> {
+    float a, b, c, d, e_exc, e_inh, c_m, inv_c_m, tau_exc, inv_tau_exc, tau_inh, inv_tau_inh,
+        tau_nmda, inv_tau_nmda;
+    NmdaTable nmda;
+    R cond[3];
+
+    /* Parameters (order of p[]): a b c d e_exc e_inh c_m tau_exc tau_inh
+     * tau_nmda (.cpp:298-375, defaults :389-401) */
+    void set_params(const float *p) {
+        a = p[0]; b = p[1]; c = p[2]; d = p[3]; e_exc = p[4]; e_inh = p[5];
+        c_m = p[6]; inv_c_m = (float)(1. / c_m);
+        tau_exc = p[7]; inv_tau_exc = (float)(1.0 / tau_exc);
+        tau_inh = p[8]; inv_tau_inh = (float)(1.0 / tau_inh);
+        tau_nmda = p[9]; inv_tau_nmda = (float)(1.0 / tau_nmda);
+        nmda.build(e_exc, e_inh);
+        this->nvars = 6;
+        this->ndiff = 2;
+    }
+    R g_nmda_inf(R V) const { return (R)nmda.get((float)V, e_inh); }
+    /* .cpp:131-136: the resting equilibrium */
+    void init_states() {
+        const float Veq = (((b - 5.0f) - std::sqrt((5.0f - b) * (5.0f - b) - 22.4f)) / 0.08f);
+        const float Ueq = Veq * b;
+        const R init[6] = {Veq, Ueq, 0, 0, 0, 0};
+        PopBase<R>::init_states(init);
+    }
+    /* .cpp:224-231 */
+    void calc_cond(R elapsed) {
+        cond[0] = Num<R>::decay(elapsed, inv_tau_exc);
+        cond[1] = Num<R>::decay(elapsed, inv_tau_inh);
+        cond[2] = Num<R>::decay(elapsed, inv_tau_nmda);
+    }
+    /* .cpp:169-195 */
+    void deriv(R *S, R *D, int, R) const {
+        R current = 0.0;
+        if (this->EXC) current += S[2] * (e_exc - S[0]);
+        if (this->INH) current += S[3] * (e_inh - S[0]);
+        if (this->NMDA) {
+            R ginf = g_nmda_inf(S[0]);
+            current += S[4] * ginf * (e_exc - S[0]);
+        }
+        current += S[5];
+        if (S[0] <= 30.0f) {
+            D[0] = 0.04f * S[0] * S[0] + 5 * S[0] + 140 - S[1] + (current * inv_c_m);
+            D[1] = a * (b * S[0] - S[1]);
+        } else {
+            D[0] = 0;
+            D[1] = 0;
+        }
+    }
+    /* .cpp:197-222 */
+    void decay(R *S) const {
+        const R limit = (R)1e-9f;
+        if (this->EXC) { if (S[2] < limit) S[2] = 0.0f; else S[2] *= cond[0]; }
+        if (this->INH) { if (S[3] < limit) S[3] = 0.0f; else S[3] *= cond[1]; }
+        if (this->NMDA) { if (S[4] < limit) S[4] = 0.0f; else S[4] *= cond[2]; }
+    }
+    /* .cpp:156-165 */
+    bool spike(R, R *S, int index) {
+        if (S[0] > 30.0f) {
+            S[0] = c;
+            S[1] += d;
+            this->fired(index);
+            this->reset_integrator(index);
+            return true;
+        }
+        return false;
+    }
+};
+
+/* ------------------------------------------------------------------------ */
 /* Fixed-step integrators over the whole population.                        */
 /* ------------------------------------------------------------------------ */
 const int kMaxVars = 32;
@@ -1278,6 +1352,10 @@ PopIface *make_pop(int model, int integ, int bdf_order, const float *p, float dt
     if (model == ORC_PURKINJE) {
         if (fp64) return new Pop<Purkinje<double>, double>(integ, bdf_order, p, dt_s, n, mask);
         return new Pop<Purkinje<float>, float>(integ, bdf_order, p, dt_s, n, mask);
+    }
+    if (model == ORC_IZHIKEVICH) {
+        if (fp64) return new Pop<Izhikevich<double>, double>(integ, bdf_order, p, dt_s, n, mask);
+        return new Pop<Izhikevich<float>, float>(integ, bdf_order, p, dt_s, n, mask);
     }
     return nullptr;
 }

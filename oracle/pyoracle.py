@@ -17,7 +17,8 @@ LIB = os.path.join(HERE, "liboracle.so")
 REF_DUMP = os.path.join(HERE, "_ref", "ref_dump")
 
 MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1, "HHTimeDrivenModel_GPU": 2,
-          "EgidioGranuleCell_TimeDriven_GPU": 3, "TimeDrivenPurkinjeCell_GPU": 4}
+          "EgidioGranuleCell_TimeDriven_GPU": 3, "TimeDrivenPurkinjeCell_GPU": 4,
+          "IzhikevichTimeDrivenModel_GPU": 5}
 INTEGRATORS = {"Euler": 0, "RK2": 1, "RK4": 2, "BDF": 3}
 
 P = ctypes.c_void_p
