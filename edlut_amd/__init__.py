@@ -12,6 +12,7 @@ from .models import (
     HHTimeDrivenModel_GPU,
     IzhikevichTimeDrivenModel_GPU,
     LIFTimeDrivenModel_GPU,
+    LIFTimeDrivenModel_IS_GPU,
     REGISTRY,
     TimeDrivenNeuronModel_GPU,
     TimeDrivenPurkinjeCell_GPU,
@@ -28,6 +29,7 @@ __all__ = [
     "EgidioGranuleCell_TimeDriven_GPU",
     "TimeDrivenPurkinjeCell_GPU",
     "IzhikevichTimeDrivenModel_GPU",
+    "LIFTimeDrivenModel_IS_GPU",
     "TimeDrivenNeuronModel_GPU",
     "CreateNeuronModel",
     "REGISTRY",

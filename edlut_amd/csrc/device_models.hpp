@@ -107,6 +107,13 @@ struct IzhParams {
     uint32_t mask;   // run-time synapse mask, as for the granule
 };
 
+struct LifIsParams {
+    float e_exc, e_inh, e_leak, v_thr, g_leak_nS, inv_c_m_nF, nmda_aux;
+    float cf[3];
+    float init[5];
+    uint32_t mask;   // run-time synapse mask
+};
+
 struct ModelParams {
     AdExParams adex;
     LIFParams lif;
@@ -114,6 +121,7 @@ struct ModelParams {
     GranuleParams gran;
     PurkinjeParams pkj;
     IzhParams izh;
+    LifIsParams lifis;
 };
 
 // Per-thread context the right-hand sides need besides the state.
@@ -305,6 +313,58 @@ struct HHModel {
     }
     __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
         return mp.hh.init[v];
+    }
+};
+
+// LIF in SI units (LIFTimeDrivenModel_IS.cpp, second time scale): V | g_exc,
+// g_inh, g_nmda, ext_I.  Conductances in nS, V in volts, the external current
+// in pA (taken as nA x 1000), the refractory clock in seconds (table over
+// steps since the spike, as for LIFModel).
+struct LifIsModel {
+    static constexpr int NV = 5, ND = 1;
+    static constexpr bool kResetsIntegrator = true;
+    static constexpr bool kClockOverride = false;
+    static constexpr int EXC = 1, INH = 2, NMDA = 3, EXT = 4;
+
+    // LIFTimeDrivenModel_IS.cpp:166-193
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void rhs(const ModelParams &mp, const RhsCtx &c, T *s, T *d) {
+        const LifIsParams &p = mp.lifis;
+        const float V = s[0];
+        float cur = 0.0f;
+        if (p.mask & 1u) cur += s[EXC] * (p.e_exc - V);
+        if (p.mask & 2u) cur += s[INH] * (p.e_inh - V);
+        if (p.mask & 4u) {
+            const float gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, V);
+            cur += s[NMDA] * gi * (p.e_exc - V);
+        }
+        const float ext_na = (float)(0.001 * (double)s[EXT]);
+        cur += ext_na;
+        if (c.since_spike >= c.refr_len) {
+            d[0] = (cur + p.g_leak_nS * (p.e_leak - V)) * p.inv_c_m_nF;
+        } else {
+            const float f = c.refr[c.since_spike];
+            d[0] = (f < 0.0f) ? 0.0f : f * ((cur + p.g_leak_nS * (p.e_leak - V)) * p.inv_c_m_nF);
+        }
+    }
+    template <unsigned MASK, typename T = float>
+    __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
+        const LifIsParams &p = mp.lifis;
+        if (p.mask & 1u) s[EXC] = decay_g(s[EXC], p.cf[0]);
+        if (p.mask & 2u) s[INH] = decay_g(s[INH], p.cf[1]);
+        if (p.mask & 4u) s[NMDA] = decay_g(s[NMDA], p.cf[2]);
+    }
+    // :155-163
+    template <typename T = float>
+    __device__ static __forceinline__ bool spike(const ModelParams &mp, T, T *s) {
+        if (s[0] > mp.lifis.v_thr) {
+            s[0] = mp.lifis.e_leak;
+            return true;
+        }
+        return false;
+    }
+    __device__ static __forceinline__ float init(const ModelParams &mp, int v) {
+        return mp.lifis.init[v];
     }
 };
 

@@ -72,12 +72,13 @@ std::vector<float> build_exp_table() {
 
 // g_nmda_inf table (AdExTimeDrivenModel.cpp:27-35); exp(float) is the float
 // overload in the reference's model translation units.
-std::vector<float> build_nmda_table(float e_exc, float e_inh, float *aux) {
+// k: -0.062 per mV, or -62 per V for the SI-unit LIF (LIFTimeDrivenModel_IS.cpp:25-33).
+std::vector<float> build_nmda_table(float e_exc, float e_inh, float *aux, float k = -0.062f) {
     std::vector<float> t(kNmdaSize);
     *aux = (kNmdaSize - 1) / (e_exc - e_inh);
     for (int i = 0; i < kNmdaSize; i++) {
         float V = e_inh + ((e_exc - e_inh) * i) / (kNmdaSize - 1);
-        t[i] = 1.0f / (1.0f + expf(-0.062f * V) * (1.2f / 3.57f));
+        t[i] = 1.0f / (1.0f + expf(k * V) * (1.2f / 3.57f));
     }
     return t;
 }
@@ -86,10 +87,11 @@ std::vector<float> build_nmda_table(float e_exc, float e_inh, float *aux) {
 // at update s the clock holds 0 and gains (double)dt_s per update, so the
 // derivative of update s+1+j sees S_j = j-fold double sum of dt_s.  Entry j
 // is the fraction for S_j (negative: derivative zero); the table stops at the
-// first j with S_j*1000 > tau_ref, after which the full derivative applies.
-std::vector<float> build_refractory(float dt_s, float dt_ms, float tau_ref) {
+// first j with S_j*time_scale > tau_ref, after which the full derivative
+// applies.  dt_ms is the step in the model's time scale (ms, or s for the
+// SI-unit LIF, LIFTimeDrivenModel_IS.cpp:183-192, time_scale 1).
+std::vector<float> build_refractory(float dt_s, float dt_ms, float tau_ref, float time_scale = 1000.0f) {
     std::vector<float> t;
-    const float time_scale = 1000.0f;
     double S = 0.0;
     for (int j = 0; j < (1 << 24); j++) {
         double lst = S * time_scale;
@@ -493,8 +495,8 @@ size_t update_lds_bytes(const edlut_population *p) {
 
 // Float parameters of each model's GetDefaultParameters() vector.
 int model_param_count(int model) {
-    static const int n[6] = {10, 14, 13, 0, 8, 10};
-    return model >= 0 && model < 6 ? n[model] : -1;
+    static const int n[7] = {10, 14, 13, 0, 8, 10, 10};
+    return model >= 0 && model < 7 ? n[model] : -1;
 }
 
 // Neurons per thread of an ungrouped population's update kernel: the
@@ -594,10 +596,11 @@ UpdateFn pick_update(const edlut_population *p, bool ext) {
         if (model == EDLUT_LIF) return lif_g.fn[idx];
         return nullptr;
     }
-    if (model == EDLUT_IZHIKEVICH) {
+    if (model == EDLUT_IZHIKEVICH || model == EDLUT_LIF_IS) {
         if (p->items != kItems) return nullptr;
         static MaskFreeTable<IzhModel, kItems> iz;
-        return iz.fn[integ * 2 + (ext ? 1 : 0)];
+        static MaskFreeTable<LifIsModel, kItems> li;
+        return (model == EDLUT_IZHIKEVICH ? iz.fn : li.fn)[integ * 2 + (ext ? 1 : 0)];
     }
     if (model == EDLUT_GRANULE || model == EDLUT_PURKINJE) {
         if (p->items != plain_items(model, integ)) return nullptr;
@@ -660,7 +663,7 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.mp = p->mp;
     // the granule and Purkinje kernels read the enabled synapse types at run
     // time: take them from the population (connections may have enabled more)
-    a.mp.gran.mask = a.mp.pkj.mask = a.mp.izh.mask = p->mask;
+    a.mp.gran.mask = a.mp.pkj.mask = a.mp.izh.mask = a.mp.lifis.mask = p->mask;
     a.k = p->k;
     a.exp_lut = p->exp_lut;
     a.nmda_lut = p->nmda_lut;
@@ -875,6 +878,21 @@ void model_params(int model, int integrator, const float *x, float dt_ms, ModelP
         const float *c = &chan[(size_t)chan_index(x[3], kPkMinV, kPkAux) * 4];
         init = {x[3], c[0] / c[1], c[2] / c[3], 0, 0, 0, 0};
         for (int j = 0; j < 7; j++) q.init[j] = init[j];
+    } else if (model == EDLUT_LIF_IS) {
+        // e_exc e_inh e_leak v_thr c_m tau_exc tau_inh tau_ref g_leak tau_nmda in SI
+        // units (LIFTimeDrivenModel_IS.cpp:298-375, defaults :392-404)
+        LifIsParams &q = mp.lifis;
+        q.e_exc = e_exc = x[0];
+        q.e_inh = e_inh = x[1];
+        q.e_leak = x[2];
+        q.v_thr = x[3];
+        q.inv_c_m_nF = (float)(1. / ((double)x[4] * 1.e9));
+        q.g_leak_nS = (float)((double)x[8] * 1.e9);
+        q.cf[0] = expf(-elapsed * inv_of(x[5]));
+        q.cf[1] = expf(-elapsed * inv_of(x[6]));
+        q.cf[2] = expf(-elapsed * inv_of(x[9]));
+        init = {x[2], 0, 0, 0, 0};
+        for (int j = 0; j < 5; j++) q.init[j] = init[j];
     } else if (model == EDLUT_IZHIKEVICH) {
         // a b c d e_exc e_inh c_m tau_exc tau_inh tau_nmda
         // (IzhikevichTimeDrivenModel.cpp:298-375, defaults :389-401)
@@ -916,7 +934,7 @@ int create_population(int model, int integrator, int bdf_order, const float *par
                       edlut_population **out) {
     if (!out) return fail(EDLUT_E_ARG, "out is null");
     *out = nullptr;
-    if (model < EDLUT_LIF || model > EDLUT_IZHIKEVICH)
+    if (model < EDLUT_LIF || model > EDLUT_LIF_IS)
         return fail(EDLUT_E_ARG, "neuron model %d is not available in this build", model);
     if (integrator < EDLUT_EULER || integrator > EDLUT_BDF)
         return fail(EDLUT_E_ARG, "integration method %d is not available for this model", integrator);
@@ -947,7 +965,10 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     std::vector<float> init;
     ModelParams mp0;
     std::memset(&mp0, 0, sizeof(mp0));
-    const float dt_ms = dt_s * 1000.0f;
+    // IntegrationMethodFast.h:191-193: the step in the model's time scale
+    // (milliseconds; seconds for the SI-unit LIF)
+    const float time_scale = model == EDLUT_LIF_IS ? 1.0f : 1000.0f;
+    const float dt_ms = dt_s * time_scale;
     model_params(model, integrator, params, dt_ms, mp0, init, e_exc, e_inh);
     std::vector<GroupInfo> gtab;
     std::vector<float> st_init;   // per group initial state
@@ -978,7 +999,7 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     p->device = device;
     p->params.assign(params, params + (size_t)n_params * n_groups);
     p->dt_s = dt_s;
-    p->dt_ms = dt_s * 1000.0f;  // IntegrationMethodFast.h:191-193
+    p->dt_ms = dt_ms;
     p->k.dt = p->dt_ms;
     p->k.h = p->dt_ms * 0.5f;
     p->k.s6 = p->dt_ms * 0.166666666667f;
@@ -988,18 +1009,19 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     p->mp = mp0;
 
     float nmda_aux = 0.0f;
-    std::vector<float> nmda = build_nmda_table(e_exc, e_inh, &nmda_aux);
+    std::vector<float> nmda = build_nmda_table(e_exc, e_inh, &nmda_aux, model == EDLUT_LIF_IS ? -62.0f : -0.062f);
     if (model == EDLUT_ADEX) p->mp.adex.nmda_aux = nmda_aux;
     else if (model == EDLUT_HH) p->mp.hh.nmda_aux = nmda_aux;
     else if (model == EDLUT_GRANULE) p->mp.gran.nmda_aux = nmda_aux;
     else if (model == EDLUT_PURKINJE) p->mp.pkj.nmda_aux = nmda_aux;
     else if (model == EDLUT_IZHIKEVICH) p->mp.izh.nmda_aux = nmda_aux;
+    else if (model == EDLUT_LIF_IS) p->mp.lifis.nmda_aux = nmda_aux;
     else p->mp.lif.nmda_aux = nmda_aux;
     for (GroupInfo &g : gtab) {
         if (model == EDLUT_ADEX) g.mp.adex.nmda_aux = nmda_aux;
         else g.mp.lif.nmda_aux = nmda_aux;
     }
-    static const int kNv[6] = {5, 6, 8, 19, 7, 6}, kNd[6] = {1, 2, 4, 15, 3, 2};
+    static const int kNv[7] = {5, 6, 8, 19, 7, 6, 5}, kNd[7] = {1, 2, 4, 15, 3, 2, 1};
     p->nv = kNv[model];
     p->nd = kNd[model];
     // the granule's kernels hold two neurons per thread (19 state variables
@@ -1012,6 +1034,7 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     std::vector<float> expt = build_exp_table();
     std::vector<float> refr;
     if (model == EDLUT_LIF) refr = build_refractory(dt_s, p->dt_ms, params[7]);
+    if (model == EDLUT_LIF_IS) refr = build_refractory(dt_s, p->dt_ms, params[7], 1.0f);
     if (model == EDLUT_PURKINJE) refr = build_purkinje_clock(dt_s, p->dt_ms, params[2], params[3], params[7]);
     // entries: LIF one fraction each, Purkinje a (mode, V) pair each
     p->refr_len = (int)refr.size() / (model == EDLUT_PURKINJE ? 2 : 1);
@@ -1081,7 +1104,7 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     if ((rc = dalloc(&p->tile_flag, max_tiles))) return cleanup(rc);
     if ((rc = dalloc(&p->bump, 2))) return cleanup(rc);
     if ((rc = p->reserve_events(1u << 16))) return cleanup(rc);
-    if (model == EDLUT_LIF || model == EDLUT_PURKINJE) {
+    if (model == EDLUT_LIF || model == EDLUT_PURKINJE || model == EDLUT_LIF_IS) {
         if ((rc = dalloc(&p->last_spike, p->stride))) return cleanup(rc);
         std::vector<int32_t> never(p->stride, -(1 << 30));
         if (hipMemcpy(p->last_spike, never.data(), sizeof(int32_t) * p->stride,
@@ -1186,7 +1209,7 @@ int edlut_population_enable_synapses(edlut_population *p, uint32_t mask) {
     if (!p) return fail(EDLUT_E_ARG, "population is null");
     if (mask & ~15u) return fail(EDLUT_E_ARG, "unknown synapse type bits 0x%x", mask);
     p->mask |= mask;
-    p->mp.gran.mask = p->mp.pkj.mask = p->mp.izh.mask = p->mask;   // models with a run-time mask
+    p->mp.gran.mask = p->mp.pkj.mask = p->mp.izh.mask = p->mp.lifis.mask = p->mask;   // models with a run-time mask
     return 0;
 }
 

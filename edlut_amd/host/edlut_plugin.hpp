@@ -289,4 +289,15 @@ class IzhikevichTimeDrivenModel_GPU : public TimeDrivenNeuronModel_GPU {
     std::string GetName() const override { return "IzhikevichTimeDrivenModel_GPU"; }
 };
 
+// LIFTimeDrivenModel_IS_GPU: SI units (defaults of LIFTimeDrivenModel_IS.cpp:392-404).
+class LIFTimeDrivenModel_IS_GPU : public TimeDrivenNeuronModel_GPU {
+  public:
+    LIFTimeDrivenModel_IS_GPU()
+        : TimeDrivenNeuronModel_GPU(EDLUT_LIF_IS, {{"e_exc", 0.0f}, {"e_inh", -80.0e-3f}, {"e_leak", -65.0e-3f},
+                                                   {"v_thr", -50.0e-3f}, {"c_m", 2.0e-12f}, {"tau_exc", 5.0e-3f},
+                                                   {"tau_inh", 10.0e-3f}, {"tau_ref", 1.0e-3f},
+                                                   {"g_leak", 0.2e-9f}, {"tau_nmda", 20.0e-3f}}) {}
+    std::string GetName() const override { return "LIFTimeDrivenModel_IS_GPU"; }
+};
+
 }  // namespace edlut_amd

@@ -29,6 +29,7 @@ static std::unique_ptr<TimeDrivenNeuronModel_GPU> make(const std::string &name) 
     if (name == "EgidioGranuleCell_TimeDriven_GPU") return std::make_unique<EgidioGranuleCell_TimeDriven_GPU>();
     if (name == "TimeDrivenPurkinjeCell_GPU") return std::make_unique<TimeDrivenPurkinjeCell_GPU>();
     if (name == "IzhikevichTimeDrivenModel_GPU") return std::make_unique<IzhikevichTimeDrivenModel_GPU>();
+    if (name == "LIFTimeDrivenModel_IS_GPU") return std::make_unique<LIFTimeDrivenModel_IS_GPU>();
     throw EDLUTException(EDLUT_E_ARG, "unknown neuron model " + name);
 }
 

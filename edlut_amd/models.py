@@ -419,6 +419,29 @@ class IzhikevichTimeDrivenModel_GPU(TimeDrivenNeuronModel_GPU):
     N_DIFF = 2
 
 
+class LIFTimeDrivenModel_IS_GPU(TimeDrivenNeuronModel_GPU):
+    """Leaky integrate-and-fire in SI units: volts, farads, siemens, seconds
+    (Kernel/src/neuron_model/LIFTimeDrivenModel_IS_GPU_C_INTERFACE.cu;
+    defaults of LIFTimeDrivenModel_IS.cpp:392-404)."""
+
+    NAME = "LIFTimeDrivenModel_IS_GPU"
+    MODEL = C.EDLUT_LIF_IS
+    PARAMS = (
+        ("e_exc", 0.0),
+        ("e_inh", -80.0e-3),
+        ("e_leak", -65.0e-3),
+        ("v_thr", -50.0e-3),
+        ("c_m", 2.0e-12),
+        ("tau_exc", 5.0e-3),
+        ("tau_inh", 10.0e-3),
+        ("tau_ref", 1.0e-3),
+        ("g_leak", 0.2e-9),
+        ("tau_nmda", 20.0e-3),
+    )
+    STATE_VARS = ("V_m", "g_exc", "g_inh", "g_nmda", "ext_I")
+    N_DIFF = 1
+
+
 # NeuronModelFactory registry (Kernel/src/neuron_model/NeuronModelFactory.cu:66-91)
 REGISTRY = {
     LIFTimeDrivenModel_GPU.NAME: LIFTimeDrivenModel_GPU,
@@ -427,6 +450,7 @@ REGISTRY = {
     EgidioGranuleCell_TimeDriven_GPU.NAME: EgidioGranuleCell_TimeDriven_GPU,
     TimeDrivenPurkinjeCell_GPU.NAME: TimeDrivenPurkinjeCell_GPU,
     IzhikevichTimeDrivenModel_GPU.NAME: IzhikevichTimeDrivenModel_GPU,
+    LIFTimeDrivenModel_IS_GPU.NAME: LIFTimeDrivenModel_IS_GPU,
 }
 
 

@@ -28,7 +28,8 @@ enum edlut_model {
     EDLUT_HH = 2,       /* "HHTimeDrivenModel_GPU" */
     EDLUT_GRANULE = 3,  /* "EgidioGranuleCell_TimeDriven_GPU" (EgidioGranuleCell_TimeDriven_GPU_C_INTERFACE.cu) */
     EDLUT_PURKINJE = 4, /* "TimeDrivenPurkinjeCell_GPU" (TimeDrivenPurkinjeCell_GPU_C_INTERFACE.cu) */
-    EDLUT_IZHIKEVICH = 5 /* "IzhikevichTimeDrivenModel_GPU" (IzhikevichTimeDrivenModel_GPU_C_INTERFACE.cu) */
+    EDLUT_IZHIKEVICH = 5, /* "IzhikevichTimeDrivenModel_GPU" (IzhikevichTimeDrivenModel_GPU_C_INTERFACE.cu) */
+    EDLUT_LIF_IS = 6     /* "LIFTimeDrivenModel_IS_GPU" (LIFTimeDrivenModel_IS_GPU_C_INTERFACE.cu), SI units */
 };
 /* Integration methods; names of IntegrationMethodFactory_GPU_C_INTERFACE.cu:40-145. */
 enum edlut_integrator { EDLUT_EULER = 0, EDLUT_RK2 = 1, EDLUT_RK4 = 2, EDLUT_BDF = 3 };
@@ -68,6 +69,8 @@ const char *edlut_last_error(void);
  *         tau_inh tau_nmda tau_ref (TimeDrivenPurkinjeCell.cpp:423-434)
  *   Izhikevich (IzhikevichTimeDrivenModel_GPU): a b c d e_exc e_inh c_m tau_exc
  *         tau_inh tau_nmda (IzhikevichTimeDrivenModel.cpp:389-401)
+ *   LIF_IS (LIFTimeDrivenModel_IS_GPU): the LIF order in SI units (V, F, s, S)
+ *         (LIFTimeDrivenModel_IS.cpp:392-404)
  * bdf_order (1..6) is the BDF method's order ("bdf_order" parameter,
  * BDFn_GPU_C_INTERFACE.cuh:275-288); ignored by the other integrators.
  * dt_s is the integration method's "step" parameter (seconds).  State lives on

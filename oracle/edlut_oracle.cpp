@@ -79,11 +79,13 @@ const int kNmdaSize = 256;
 struct NmdaTable {
     float v[kNmdaSize];
     float aux;
-    void build(float e_exc, float e_inh) {
+    /* k = -0.062 per mV; the SI-unit LIF uses -62 per V
+     * (LIFTimeDrivenModel_IS.cpp:25-33) */
+    void build(float e_exc, float e_inh, float k = -0.062f) {
         aux = (kNmdaSize - 1) / (e_exc - e_inh);
         for (int i = 0; i < kNmdaSize; i++) {
             float V = e_inh + ((e_exc - e_inh) * i) / (kNmdaSize - 1);
-            v[i] = 1.0f / (1.0f + std::exp(-0.062f * V) * (1.2f / 3.57f));
+            v[i] = 1.0f / (1.0f + std::exp(k * V) * (1.2f / 3.57f));
         }
     }
     float get(float V_m, float e_inh) const {
@@ -140,6 +142,8 @@ template <typename R> struct PopBase {
         last_spike.assign(N, 100.0);
     }
     R *at(int i) { return &state[(size_t)i * nvars]; }
+    /* NeuronModel time scale: milliseconds (MilisecondScale, NeuronModel.h:71) */
+    float time_scale_factor() const { return 1000.0f; }
     /* VectorNeuronState.cpp:221-224 */
     void add_elapsed(int i) { last_spike[i] += (double)dt_s; }
     /* VectorNeuronState.cpp:215-218 */
@@ -816,6 +820,83 @@ template <typename R> struct Purkinje : PopBase<R> {
 };
 
 /* ------------------------------------------------------------------------ */
+/* LIF in SI units: Kernel/src/neuron_model/LIFTimeDrivenModel_IS.cpp         */
+/* (second time scale; V, F, S, s).  state: V | g_exc, g_inh, g_nmda, ext_I   */
+/* ------------------------------------------------------------------------ */
+template <typename R> struct LIF_IS : PopBase<R> {
+    float e_exc, e_inh, e_leak, v_thr, c_m, inv_c_m_nF, tau_exc, inv_tau_exc, tau_inh, inv_tau_inh,
+        tau_ref, g_leak, g_leak_nS, tau_nmda, inv_tau_nmda;
+    NmdaTable nmda;
+    R cond[3];
+
+    /* SecondScale (.cpp:56) */
+    float time_scale_factor() const { return 1.0f; }
+    /* Parameters (order of p[]): e_exc e_inh e_leak v_thr c_m tau_exc tau_inh
+     * tau_ref g_leak tau_nmda (.cpp:298-375, defaults :392-404) */
+    void set_params(const float *p) {
+        e_exc = p[0]; e_inh = p[1]; e_leak = p[2]; v_thr = p[3];
+        c_m = p[4]; inv_c_m_nF = (float)(1. / (c_m * 1.e9));
+        tau_exc = p[5]; inv_tau_exc = (float)(1.0 / tau_exc);
+        tau_inh = p[6]; inv_tau_inh = (float)(1.0 / tau_inh);
+        tau_ref = p[7];
+        g_leak = p[8]; g_leak_nS = (float)(g_leak * 1.e9);
+        tau_nmda = p[9]; inv_tau_nmda = (float)(1.0 / tau_nmda);
+        nmda.build(e_exc, e_inh, -62.0f);
+        this->nvars = 5;
+        this->ndiff = 1;
+    }
+    R g_nmda_inf(R V) const { return (R)nmda.get((float)V, e_inh); }
+    /* .cpp:132-136 */
+    void init_states() {
+        R init[5] = {e_leak, 0, 0, 0, 0};
+        PopBase<R>::init_states(init);
+    }
+    void calc_cond(R elapsed) {
+        cond[0] = Num<R>::decay(elapsed, inv_tau_exc);
+        cond[1] = Num<R>::decay(elapsed, inv_tau_inh);
+        cond[2] = Num<R>::decay(elapsed, inv_tau_nmda);
+    }
+    /* .cpp:166-193: the clock in seconds, compared with tau_ref directly */
+    void deriv(R *S, R *D, int index, R elapsed_time) const {
+        R current = 0.0;
+        if (this->EXC) current += S[1] * (e_exc - S[0]);
+        if (this->INH) current += S[2] * (e_inh - S[0]);
+        if (this->NMDA) {
+            R ginf = g_nmda_inf(S[0]);
+            current += S[3] * ginf * (e_exc - S[0]);
+        }
+        const R external_current_nA = (R)(0.001 * S[4]);
+        current += external_current_nA;
+        const double lst = this->last_spike[index];
+        if (lst > tau_ref) {
+            D[0] = (current + g_leak_nS * (e_leak - S[0])) * inv_c_m_nF;
+        } else if ((lst + elapsed_time) > tau_ref) {
+            R fraction = (R)((lst + elapsed_time - tau_ref) / elapsed_time);
+            D[0] = fraction * ((current + g_leak_nS * (e_leak - S[0])) * inv_c_m_nF);
+        } else {
+            D[0] = 0;
+        }
+    }
+    /* .cpp:195-220 */
+    void decay(R *S) const {
+        const R limit = (R)1e-9f;
+        if (this->EXC) { if (S[1] < limit) S[1] = 0.0f; else S[1] *= cond[0]; }
+        if (this->INH) { if (S[2] < limit) S[2] = 0.0f; else S[2] *= cond[1]; }
+        if (this->NMDA) { if (S[3] < limit) S[3] = 0.0f; else S[3] *= cond[2]; }
+    }
+    /* .cpp:155-163 */
+    bool spike(R, R *S, int index) {
+        if (S[0] > v_thr) {
+            S[0] = e_leak;
+            this->fired(index);
+            this->reset_integrator(index);
+            return true;
+        }
+        return false;
+    }
+};
+
+/* ------------------------------------------------------------------------ */
 /* Izhikevich: Kernel/src/neuron_model/IzhikevichTimeDrivenModel.cpp          */
 /* state: V, u | g_exc, g_inh, g_nmda, ext_I                                  */
 /* ------------------------------------------------------------------------ */
@@ -1137,7 +1218,7 @@ template <typename M, typename R> struct Pop : PopIface {
         m.N = n;
         m.dt_s = dt_s;
         /* IntegrationMethodFast.h:191-193: seconds * time scale, in float. */
-        float dt_model = dt_s * 1000.0f;
+        float dt_model = dt_s * m.time_scale_factor();
         m.dt_model = (R)dt_model;
         m.EXC = (mask & 1u) != 0;
         m.INH = (mask & 2u) != 0;
@@ -1352,6 +1433,10 @@ PopIface *make_pop(int model, int integ, int bdf_order, const float *p, float dt
     if (model == ORC_PURKINJE) {
         if (fp64) return new Pop<Purkinje<double>, double>(integ, bdf_order, p, dt_s, n, mask);
         return new Pop<Purkinje<float>, float>(integ, bdf_order, p, dt_s, n, mask);
+    }
+    if (model == ORC_LIF_IS) {
+        if (fp64) return new Pop<LIF_IS<double>, double>(integ, bdf_order, p, dt_s, n, mask);
+        return new Pop<LIF_IS<float>, float>(integ, bdf_order, p, dt_s, n, mask);
     }
     if (model == ORC_IZHIKEVICH) {
         if (fp64) return new Pop<Izhikevich<double>, double>(integ, bdf_order, p, dt_s, n, mask);

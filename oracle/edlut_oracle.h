@@ -25,7 +25,8 @@ extern "C" {
 #endif
 
 /* Model and integrator identifiers (same numbering as include/edlut_amd.h). */
-enum { ORC_LIF = 0, ORC_ADEX = 1, ORC_HH = 2, ORC_GRANULE = 3, ORC_PURKINJE = 4, ORC_IZHIKEVICH = 5 };
+enum { ORC_LIF = 0, ORC_ADEX = 1, ORC_HH = 2, ORC_GRANULE = 3, ORC_PURKINJE = 4, ORC_IZHIKEVICH = 5,
+       ORC_LIF_IS = 6 };
 enum { ORC_EULER = 0, ORC_RK2 = 1, ORC_RK4 = 2, ORC_BDF = 3 };
 
 /* ---- look-up tables ---------------------------------------------------- */

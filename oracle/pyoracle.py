@@ -18,7 +18,7 @@ REF_DUMP = os.path.join(HERE, "_ref", "ref_dump")
 
 MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1, "HHTimeDrivenModel_GPU": 2,
           "EgidioGranuleCell_TimeDriven_GPU": 3, "TimeDrivenPurkinjeCell_GPU": 4,
-          "IzhikevichTimeDrivenModel_GPU": 5}
+          "IzhikevichTimeDrivenModel_GPU": 5, "LIFTimeDrivenModel_IS_GPU": 6}
 INTEGRATORS = {"Euler": 0, "RK2": 1, "RK4": 2, "BDF": 3}
 
 P = ctypes.c_void_p

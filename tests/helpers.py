@@ -13,12 +13,15 @@ HH_DEFAULTS = np.array([0.0, -80.0, -65.0, 10.0, 120.0, -52.0, 5.0, 10.0, 20.0, 
 GRANULE_DEFAULTS = np.zeros(0, np.float32)
 PURKINJE_DEFAULTS = np.array([0.0, -80.0, -35.0, -70.0, 1.0, 2.0, 20.0, 1.35], np.float32)
 IZH_DEFAULTS = np.array([0.1, 0.23, -65.0, 0.2, 0.0, -80.0, 100.0, 5.0, 10.0, 20.0], np.float32)
+LIF_IS_DEFAULTS = np.array([0.0, -80.0e-3, -65.0e-3, -50.0e-3, 2.0e-12, 5.0e-3, 10.0e-3, 1.0e-3, 0.2e-9, 20.0e-3],
+                           np.float32)
 
 
 def defaults(model: str) -> np.ndarray:
     return {"AdExTimeDrivenModel_GPU": ADEX_DEFAULTS, "LIFTimeDrivenModel_GPU": LIF_DEFAULTS,
             "HHTimeDrivenModel_GPU": HH_DEFAULTS, "EgidioGranuleCell_TimeDriven_GPU": GRANULE_DEFAULTS,
-            "TimeDrivenPurkinjeCell_GPU": PURKINJE_DEFAULTS, "IzhikevichTimeDrivenModel_GPU": IZH_DEFAULTS}[model]
+            "TimeDrivenPurkinjeCell_GPU": PURKINJE_DEFAULTS, "IzhikevichTimeDrivenModel_GPU": IZH_DEFAULTS,
+            "LIFTimeDrivenModel_IS_GPU": LIF_IS_DEFAULTS}[model]
 
 
 def np_out_csr(n_src, n_tgt, max_delay, src, tgt, delay, typ, w):
