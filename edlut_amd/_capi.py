@@ -71,7 +71,7 @@ EXPORTS = (
 )
 
 EDLUT_LIF, EDLUT_ADEX, EDLUT_HH, EDLUT_GRANULE, EDLUT_PURKINJE, EDLUT_IZHIKEVICH, EDLUT_LIF_IS = range(7)
-EDLUT_EULER, EDLUT_RK2, EDLUT_RK4, EDLUT_BDF = range(4)
+EDLUT_EULER, EDLUT_RK2, EDLUT_RK4, EDLUT_BDF, EDLUT_BIFIXED_EULER, EDLUT_BIFIXED_RK2, EDLUT_BIFIXED_RK4 = range(7)
 SYN_EXC, SYN_INH, SYN_NMDA, SYN_EXT_I = 1, 2, 4, 8
 EDLUT_E_ARG, EDLUT_E_HIP, EDLUT_E_CAP, EDLUT_E_STATE = -1, -2, -3, -4
 

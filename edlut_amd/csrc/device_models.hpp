@@ -54,6 +54,15 @@ __device__ __forceinline__ double decay_g(double g, double f) {
     return (g < (double)1e-9f) ? 0.0 : g * f;
 }
 
+// Enabled synapse types.  AdEx, LIF and HH kernels take them from the
+// compile-time MASK (EDLUT_SYN_* bits), or, when MASK has kRtMask set, from
+// the run-time mask of the parameters (one kernel for every combination).
+constexpr unsigned kRtMask = 16u;
+template <unsigned MASK>
+__device__ __forceinline__ bool syn(uint32_t rt, unsigned bit) {
+    return (MASK & kRtMask) ? (rt & bit) != 0u : (MASK & bit) != 0u;
+}
+
 // ---------------------------------------------------------------------------
 // Model descriptors.  Each exposes NV (state variables), ND (differential
 // variables), the variable rows that hold g_exc/g_inh/g_nmda/ext_I, the
@@ -64,12 +73,14 @@ struct AdExParams {
         glk_thr, inv_c_m, nmda_aux;
     float cf[3];     // decay factors for the integrator's elapsed time
     float init[6];   // initial state (reset target of the NaN check)
+    uint32_t mask;   // run-time synapse mask (kernels compiled with kRtMask)
 };
 
 struct LIFParams {
     float e_exc, e_inh, e_leak, v_thr, g_leak, inv_c_m, nmda_aux;
     float cf[3];
     float init[5];
+    uint32_t mask;
 };
 
 struct HHParams {
@@ -78,6 +89,7 @@ struct HHParams {
     float init[8];
     double inv_c_m_d;   // FP64 Newton mode: 1/c_m and decay factors in double
     double cf_d[3];
+    uint32_t mask;
 };
 
 // Granule and Purkinje cells keep the enabled synapse types as a run-time
@@ -149,26 +161,26 @@ struct AdExModel {
         const float V = s[0];
         if (V <= p.v_cap) {
             float cur = 0.0f;
-            if (MASK & 1u) cur += s[EXC] * (p.e_exc - V);
-            if (MASK & 2u) cur += s[INH] * (p.e_inh - V);
-            if (MASK & 4u) {
+            if (syn<MASK>(p.mask, 1u)) cur += s[EXC] * (p.e_exc - V);
+            if (syn<MASK>(p.mask, 2u)) cur += s[INH] * (p.e_inh - V);
+            if (syn<MASK>(p.mask, 4u)) {
                 const float gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, V);
                 cur += s[NMDA] * gi * (p.e_exc - V);
             }
-            if (MASK & 8u) cur += s[EXT];
+            if (syn<MASK>(p.mask, 8u)) cur += s[EXT];
             d[0] = (p.g_leak * (p.e_leak - V) + p.glk_thr * lut_exp(c.exp_lut, (V - p.v_thr) * p.inv_thr) -
                     s[1] + cur) * p.inv_c_m;
             d[1] = (p.a * (V - p.e_leak) - s[1]) * p.inv_tau_w;
         } else if (V <= 0.0f) {
             const float Vm = p.v_cap;
             float cur = 0.0f;
-            if (MASK & 1u) cur += s[EXC] * (p.e_exc - Vm);
-            if (MASK & 2u) cur += s[INH] * (p.e_inh - Vm);
-            if (MASK & 4u) {
+            if (syn<MASK>(p.mask, 1u)) cur += s[EXC] * (p.e_exc - Vm);
+            if (syn<MASK>(p.mask, 2u)) cur += s[INH] * (p.e_inh - Vm);
+            if (syn<MASK>(p.mask, 4u)) {
                 const float gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, Vm);
                 cur += s[NMDA] * gi * (p.e_exc - Vm);
             }
-            if (MASK & 8u) cur += s[EXT];
+            if (syn<MASK>(p.mask, 8u)) cur += s[EXT];
             d[0] = (p.g_leak * (p.e_leak - Vm) + p.glk_thr * lut_exp(c.exp_lut, (Vm - p.v_thr) * p.inv_thr) -
                     s[1] + cur) * p.inv_c_m;
             d[1] = (p.a * (V - p.e_leak) - s[1]) * p.inv_tau_w;
@@ -179,9 +191,10 @@ struct AdExModel {
     }
     template <unsigned MASK, typename T = float>
     __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
-        if (MASK & 1u) s[EXC] = decay_g(s[EXC], mp.adex.cf[0]);
-        if (MASK & 2u) s[INH] = decay_g(s[INH], mp.adex.cf[1]);
-        if (MASK & 4u) s[NMDA] = decay_g(s[NMDA], mp.adex.cf[2]);
+        const AdExParams &p = mp.adex;
+        if (syn<MASK>(p.mask, 1u)) s[EXC] = decay_g(s[EXC], mp.adex.cf[0]);
+        if (syn<MASK>(p.mask, 2u)) s[INH] = decay_g(s[INH], mp.adex.cf[1]);
+        if (syn<MASK>(p.mask, 4u)) s[NMDA] = decay_g(s[NMDA], mp.adex.cf[2]);
     }
     // AdExTimeDrivenModel.cpp:159-168.
     template <typename T = float>
@@ -215,13 +228,13 @@ struct LIFModel {
         const LIFParams &p = mp.lif;
         const float V = s[0];
         float cur = 0.0f;
-        if (MASK & 1u) cur += s[EXC] * (p.e_exc - V);
-        if (MASK & 2u) cur += s[INH] * (p.e_inh - V);
-        if (MASK & 4u) {
+        if (syn<MASK>(p.mask, 1u)) cur += s[EXC] * (p.e_exc - V);
+        if (syn<MASK>(p.mask, 2u)) cur += s[INH] * (p.e_inh - V);
+        if (syn<MASK>(p.mask, 4u)) {
             const float gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, V);
             cur += s[NMDA] * gi * (p.e_exc - V);
         }
-        if (MASK & 8u) cur += s[EXT];
+        if (syn<MASK>(p.mask, 8u)) cur += s[EXT];
         if (c.since_spike >= c.refr_len) {
             d[0] = (cur + p.g_leak * (p.e_leak - V)) * p.inv_c_m;
         } else {
@@ -231,9 +244,10 @@ struct LIFModel {
     }
     template <unsigned MASK, typename T = float>
     __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
-        if (MASK & 1u) s[EXC] = decay_g(s[EXC], mp.lif.cf[0]);
-        if (MASK & 2u) s[INH] = decay_g(s[INH], mp.lif.cf[1]);
-        if (MASK & 4u) s[NMDA] = decay_g(s[NMDA], mp.lif.cf[2]);
+        const LIFParams &p = mp.lif;
+        if (syn<MASK>(p.mask, 1u)) s[EXC] = decay_g(s[EXC], mp.lif.cf[0]);
+        if (syn<MASK>(p.mask, 2u)) s[INH] = decay_g(s[INH], mp.lif.cf[1]);
+        if (syn<MASK>(p.mask, 4u)) s[NMDA] = decay_g(s[NMDA], mp.lif.cf[2]);
     }
     // LIFTimeDrivenModel.cpp:158-168.
     template <typename T = float>
@@ -284,13 +298,13 @@ struct HHModel {
         pos = pos < 0 ? 0 : (pos > kHHTable - 1 ? kHHTable - 1 : pos);
         const float *v = c.chan + (int64_t)pos * 6;
         T cur = 0.0f;
-        if (MASK & 1u) cur += s[EXC] * ((T)p.e_exc - s[0]);
-        if (MASK & 2u) cur += s[INH] * ((T)p.e_inh - s[0]);
-        if (MASK & 4u) {
+        if (syn<MASK>(p.mask, 1u)) cur += s[EXC] * ((T)p.e_exc - s[0]);
+        if (syn<MASK>(p.mask, 2u)) cur += s[INH] * ((T)p.e_inh - s[0]);
+        if (syn<MASK>(p.mask, 4u)) {
             const T gi = nmda_inf(c.nmda_lut, p.nmda_aux, p.e_inh, (float)s[0]);
             cur += s[NMDA] * gi * ((T)p.e_exc - s[0]);
         }
-        if (MASK & 8u) cur += s[EXT];
+        if (syn<MASK>(p.mask, 8u)) cur += s[EXT];
         const T inv_c_m = sizeof(T) == sizeof(double) ? (T)p.inv_c_m_d : (T)p.inv_c_m;
         d[0] = (cur + (T)p.g_leak * ((T)p.e_leak - V) + (T)p.g_na * m * m * m * h * ((T)p.e_na - V) +
                 (T)p.g_kd * n * n * n * n * ((T)p.e_k - V)) * inv_c_m;
@@ -302,9 +316,10 @@ struct HHModel {
     template <unsigned MASK, typename T = float>
     __device__ static __forceinline__ void decay(const ModelParams &mp, T *s) {
         const bool dbl = sizeof(T) == sizeof(double);
-        if (MASK & 1u) s[EXC] = decay_g(s[EXC], dbl ? (T)mp.hh.cf_d[0] : (T)mp.hh.cf[0]);
-        if (MASK & 2u) s[INH] = decay_g(s[INH], dbl ? (T)mp.hh.cf_d[1] : (T)mp.hh.cf[1]);
-        if (MASK & 4u) s[NMDA] = decay_g(s[NMDA], dbl ? (T)mp.hh.cf_d[2] : (T)mp.hh.cf[2]);
+        const HHParams &p = mp.hh;
+        if (syn<MASK>(p.mask, 1u)) s[EXC] = decay_g(s[EXC], dbl ? (T)mp.hh.cf_d[0] : (T)mp.hh.cf[0]);
+        if (syn<MASK>(p.mask, 2u)) s[INH] = decay_g(s[INH], dbl ? (T)mp.hh.cf_d[1] : (T)mp.hh.cf[1]);
+        if (syn<MASK>(p.mask, 4u)) s[NMDA] = decay_g(s[NMDA], dbl ? (T)mp.hh.cf_d[2] : (T)mp.hh.cf[2]);
     }
     // HHTimeDrivenModel.cpp:187-193: upward crossing of -30 mV, no reset.
     template <typename T = float>

@@ -269,6 +269,7 @@ inline size_t padded_tiles(int n_tiles) {
 // ---------------------------------------------------------------------------
 struct edlut_population {
     int model = 0, integ = 0, bdf_order = 0;
+    int n_sub = 1;                   // Bifixed methods: sub-steps per update
     int n = 0, nv = 0, nd = 0;
     int64_t stride = 0;
     int n_tiles = 0;
@@ -556,7 +557,7 @@ struct GroupedTable {
 // synapse types are a run-time mask), ITEMS neurons per thread.
 template <class M, int ITEMS>
 struct MaskFreeTable {
-    UpdateFn fn[4 * 2];
+    UpdateFn fn[7 * 2];   // by integrator (edlut_integrator) and drive
     MaskFreeTable() {
         fn[0] = k_update<M, kEuler, 15u, false, ITEMS, false>;
         fn[1] = k_update<M, kEuler, 15u, true, ITEMS, false>;
@@ -566,6 +567,27 @@ struct MaskFreeTable {
         fn[5] = k_update<M, kRK4, 15u, true, ITEMS, false>;
         fn[6] = k_update_bdf<M, 15u, false, float, ITEMS>;
         fn[7] = k_update_bdf<M, 15u, true, float, ITEMS>;
+        fn[8] = k_update<M, kBifEuler, 15u, false, ITEMS, false>;
+        fn[9] = k_update<M, kBifEuler, 15u, true, ITEMS, false>;
+        fn[10] = k_update<M, kBifRK2, 15u, false, ITEMS, false>;
+        fn[11] = k_update<M, kBifRK2, 15u, true, ITEMS, false>;
+        fn[12] = k_update<M, kBifRK4, 15u, false, ITEMS, false>;
+        fn[13] = k_update<M, kBifRK4, 15u, true, ITEMS, false>;
+    }
+};
+
+// Bifixed kernels of AdEx, LIF and HH: the synapse types are a run-time mask.
+template <class M>
+struct BifixedTable {
+    UpdateFn fn[3 * 2];
+    BifixedTable() {
+        constexpr unsigned RT = 15u | kRtMask;
+        fn[0] = k_update<M, kBifEuler, RT, false, kItems, false>;
+        fn[1] = k_update<M, kBifEuler, RT, true, kItems, false>;
+        fn[2] = k_update<M, kBifRK2, RT, false, kItems, false>;
+        fn[3] = k_update<M, kBifRK2, RT, true, kItems, false>;
+        fn[4] = k_update<M, kBifRK4, RT, false, kItems, false>;
+        fn[5] = k_update<M, kBifRK4, RT, true, kItems, false>;
     }
 };
 
@@ -619,6 +641,16 @@ UpdateFn pick_update(const edlut_population *p, bool ext) {
         return model == EDLUT_GRANULE ? gr.fn[idx] : pk.fn[idx];
     }
     if (p->items != kItems) return nullptr;
+    if (integ >= EDLUT_BIFIXED_EULER) {
+        static BifixedTable<AdExModel> adex_s;
+        static BifixedTable<LIFModel> lif_s;
+        static BifixedTable<HHModel> hh_s;
+        const int idx = (integ - EDLUT_BIFIXED_EULER) * 2 + (ext ? 1 : 0);
+        if (model == EDLUT_ADEX) return adex_s.fn[idx];
+        if (model == EDLUT_LIF) return lif_s.fn[idx];
+        if (model == EDLUT_HH) return hh_s.fn[idx];
+        return nullptr;
+    }
     if (integ == EDLUT_BDF) {
         static BdfTable<AdExModel, float> adex_b;
         static BdfTable<LIFModel, float> lif_b;
@@ -664,7 +696,9 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     // the granule and Purkinje kernels read the enabled synapse types at run
     // time: take them from the population (connections may have enabled more)
     a.mp.gran.mask = a.mp.pkj.mask = a.mp.izh.mask = a.mp.lifis.mask = p->mask;
+    a.mp.adex.mask = a.mp.lif.mask = a.mp.hh.mask = p->mask;   // run-time-mask kernels
     a.k = p->k;
+    a.n_sub = p->n_sub;
     a.exp_lut = p->exp_lut;
     a.nmda_lut = p->nmda_lut;
     a.refr = p->refr;
@@ -936,17 +970,25 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     *out = nullptr;
     if (model < EDLUT_LIF || model > EDLUT_LIF_IS)
         return fail(EDLUT_E_ARG, "neuron model %d is not available in this build", model);
-    if (integrator < EDLUT_EULER || integrator > EDLUT_BDF)
+    if (integrator < EDLUT_EULER || integrator > EDLUT_BIFIXED_RK4)
         return fail(EDLUT_E_ARG, "integration method %d is not available for this model", integrator);
     if (integrator == EDLUT_BDF && (bdf_order < 1 || bdf_order > 6))
         return fail(EDLUT_E_ARG, "bdf_order must be in [1, 6] (BDFn_GPU_C_INTERFACE.cuh:275-288)");
+    const bool bifixed = integrator >= EDLUT_BIFIXED_EULER;
+    if (bifixed && bdf_order < 1)
+        return fail(EDLUT_E_ARG, "n_steps must be positive (BifixedStep_GPU_C_INTERFACE.cuh:144-152)");
+    // the base method's sub-step: seconds divided in float
+    // (BifixedStep_GPU_C_INTERFACE.cuh:157-158)
+    const int n_sub = bifixed ? bdf_order : 1;
+    const int base_integ = bifixed ? integrator - EDLUT_BIFIXED_EULER : integrator;
+    const float step_s = bifixed ? dt_s / (float)n_sub : dt_s;
     const int need = model_param_count(model);
     if (n_params != need || (need > 0 && !params))
         return fail(EDLUT_E_ARG, "model %d takes %d parameters, got %d", model, need, n_params);
     if (!(dt_s > 0.0f)) return fail(EDLUT_E_ARG, "integration step must be positive");
     if (n_groups <= 0 || !sizes) return fail(EDLUT_E_ARG, "no groups");
     if (grouped) {
-        if ((model != EDLUT_ADEX && model != EDLUT_LIF) || integrator == EDLUT_BDF)
+        if ((model != EDLUT_ADEX && model != EDLUT_LIF) || integrator > EDLUT_RK4)
             return fail(EDLUT_E_ARG, "grouped populations take AdEx or LIF under Euler, RK2 or RK4");
     }
     int64_t total = 0;
@@ -969,7 +1011,10 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     // (milliseconds; seconds for the SI-unit LIF)
     const float time_scale = model == EDLUT_LIF_IS ? 1.0f : 1000.0f;
     const float dt_ms = dt_s * time_scale;
-    model_params(model, integrator, params, dt_ms, mp0, init, e_exc, e_inh);
+    // the integrator's step in the model scale (Bifixed: the sub-step's
+    // seconds times the time scale, BifixedStep_GPU2.cuh:81-82)
+    const float int_ms = step_s * time_scale;
+    model_params(model, base_integ, params, int_ms, mp0, init, e_exc, e_inh);
     std::vector<GroupInfo> gtab;
     std::vector<float> st_init;   // per group initial state
     if (grouped) {
@@ -994,15 +1039,16 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     edlut_population *p = new edlut_population;
     p->model = model;
     p->integ = integrator;
-    p->bdf_order = bdf_order;
+    p->bdf_order = bifixed ? 0 : bdf_order;
+    p->n_sub = n_sub;
     p->n = n_neurons;
     p->device = device;
     p->params.assign(params, params + (size_t)n_params * n_groups);
     p->dt_s = dt_s;
     p->dt_ms = dt_ms;
-    p->k.dt = p->dt_ms;
-    p->k.h = p->dt_ms * 0.5f;
-    p->k.s6 = p->dt_ms * 0.166666666667f;
+    p->k.dt = int_ms;
+    p->k.h = int_ms * 0.5f;
+    p->k.s6 = int_ms * 0.166666666667f;
     p->grouped = grouped;
     p->g_start = start;
     p->g_size.assign(sizes, sizes + n_groups);
@@ -1033,9 +1079,10 @@ int create_population(int model, int integrator, int bdf_order, const float *par
     if (model == EDLUT_PURKINJE) chan = build_purkinje_table();
     std::vector<float> expt = build_exp_table();
     std::vector<float> refr;
-    if (model == EDLUT_LIF) refr = build_refractory(dt_s, p->dt_ms, params[7]);
-    if (model == EDLUT_LIF_IS) refr = build_refractory(dt_s, p->dt_ms, params[7], 1.0f);
-    if (model == EDLUT_PURKINJE) refr = build_purkinje_clock(dt_s, p->dt_ms, params[2], params[3], params[7]);
+    // clock tables over the integrator's (sub-)steps since the last spike
+    if (model == EDLUT_LIF) refr = build_refractory(step_s, int_ms, params[7]);
+    if (model == EDLUT_LIF_IS) refr = build_refractory(step_s, int_ms, params[7], 1.0f);
+    if (model == EDLUT_PURKINJE) refr = build_purkinje_clock(step_s, int_ms, params[2], params[3], params[7]);
     // entries: LIF one fraction each, Purkinje a (mode, V) pair each
     p->refr_len = (int)refr.size() / (model == EDLUT_PURKINJE ? 2 : 1);
     p->lut_sums[0] = fnv1a(expt.data(), expt.size() * sizeof(float));

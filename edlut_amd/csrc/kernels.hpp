@@ -12,7 +12,10 @@
 
 namespace edl {
 
-enum Integ { kEuler = 0, kRK2 = 1, kRK4 = 2 };
+// kBif*: the GPU Bifixed methods -- UpdateArgs::n_sub steps of the base
+// method (INTEG - kBifixed) per update, at the sub-step's dt.
+enum Integ { kEuler = 0, kRK2 = 1, kRK4 = 2, kBifEuler = 3, kBifRK2 = 4, kBifRK4 = 5 };
+constexpr int kBifixed = 3;
 
 // ---------------------------------------------------------------------------
 // Ordered conductance accumulation.
@@ -210,7 +213,8 @@ struct UpdateArgs {
     const float *nmda_lut;
     const float *refr;
     int refr_len;
-    int32_t *last_spike;   // LIF, Purkinje: update index of the last spike
+    int32_t *last_spike;   // LIF, Purkinje: update (Bifixed: sub-step) index of the last spike
+    int n_sub;             // Bifixed kernels: sub-steps per update
     BdfState bdf;          // BDF integrator history
     ExtDrive ext;
     TileOut out;
@@ -377,21 +381,51 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
         for (int n = 0; n < ITEMS; n++)
             for (int q = 0; q < cnt[n]; q++) s[n][M::EXC] += a.ext.w;
     }
-    float prev_v[ITEMS];
-#pragma unroll
-    for (int n = 0; n < ITEMS; n++) prev_v[n] = s[n][0];
-    if (INTEG == kEuler) integrate_euler<M, MASK, ITEMS>(mp, c, a.k, s);
-    else if (INTEG == kRK2) integrate_rk2<M, MASK, ITEMS>(mp, c, a.k, s);
-    else integrate_rk4<M, MASK, ITEMS>(mp, c, a.k, s);
-
+    // Fixed methods take one step.  The Bifixed methods take n_sub steps of
+    // the base method, each followed by the spike condition, and report a
+    // neuron once per update (Bifixed_RK4_GPU2.cuh:100-150); their spike clock
+    // counts sub-steps, numbered from 1 over the run (update k covers
+    // (k-1)*n_sub+1 .. k*n_sub), and the clock tables are built for the
+    // sub-step.
+    constexpr bool SUB = INTEG >= kBifixed;
+    constexpr int BASE = SUB ? INTEG - kBifixed : INTEG;
+    const int n_sub = SUB ? a.n_sub : 1;
+    const int32_t g0 = SUB ? (step - 1) * n_sub : step - 1;
     bool spk[ITEMS];
+    int32_t last[ITEMS];
+#pragma unroll
+    for (int n = 0; n < ITEMS; n++) {
+        spk[n] = false;
+        last[n] = (SUB && a.last_spike) ? a.last_spike[(uint32_t)idx[n]] : 0;
+    }
+    for (int it = 0; it < n_sub; it++) {
+        const int32_t g = g0 + it + 1;
+        float prev_v[ITEMS];
+#pragma unroll
+        for (int n = 0; n < ITEMS; n++) {
+            if (SUB && a.last_spike) c[n].since_spike = g - 1 - last[n];
+            prev_v[n] = s[n][0];
+        }
+        if (BASE == kEuler) integrate_euler<M, MASK, ITEMS>(mp, c, a.k, s);
+        else if (BASE == kRK2) integrate_rk2<M, MASK, ITEMS>(mp, c, a.k, s);
+        else integrate_rk4<M, MASK, ITEMS>(mp, c, a.k, s);
+#pragma unroll
+        for (int n = 0; n < ITEMS; n++) {
+            const bool f = live[n] && M::spike(mp, prev_v[n], s[n]);
+            if constexpr (M::kClockOverride)
+                if (live[n]) M::clock_override(c[n], f, s[n]);
+            if (f) {
+                spk[n] = true;
+                last[n] = g;
+            }
+        }
+    }
+
 #pragma unroll
     for (int n = 0; n < ITEMS; n++) {
         const uint32_t i = (uint32_t)idx[n];
-        spk[n] = live[n] && M::spike(mp, prev_v[n], s[n]);
         if (!live[n]) continue;
-        if constexpr (M::kClockOverride) M::clock_override(c[n], spk[n], s[n]);
-        if (spk[n] && a.last_spike) a.last_spike[i] = step;
+        if (spk[n] && a.last_spike) a.last_spike[i] = last[n];
         if (s[n][0] != s[n][0]) {
             // TimeDrivenNeuronModel.cpp:75-88: NaN -> whole state to initial.
 #pragma unroll

@@ -52,11 +52,13 @@ inline void check(int rc) {
 using ParamMap = std::map<std::string, std::any>;
 
 // Integration-method description (IntegrationMethod_GPU_C_INTERFACE: "name",
-// "step" in seconds, "bdf_order" for BDF, BDFn_GPU_C_INTERFACE.cuh:275-288).
+// "step" in seconds, "bdf_order" for BDF, BDFn_GPU_C_INTERFACE.cuh:275-288;
+// "n_steps" for the Bifixed methods, BifixedStep_GPU_C_INTERFACE.cuh:144-171).
 struct ModelDescription {
     std::string name = "Euler";
     float step = 1e-3f;
     int bdf_order = 1;
+    int n_steps = 2;
 };
 
 inline int integrator_id(const std::string &name) {
@@ -64,6 +66,9 @@ inline int integrator_id(const std::string &name) {
     if (name == "RK2") return EDLUT_RK2;
     if (name == "RK4") return EDLUT_RK4;
     if (name == "BDF") return EDLUT_BDF;
+    if (name == "Bifixed_Euler") return EDLUT_BIFIXED_EULER;
+    if (name == "Bifixed_RK2") return EDLUT_BIFIXED_RK2;
+    if (name == "Bifixed_RK4") return EDLUT_BIFIXED_RK4;
     // IntegrationMethodFactory_GPU_C_INTERFACE.cu:107-110
     throw EDLUTException(EDLUT_E_ARG, "unknown integration method " + name);
 }
@@ -112,8 +117,10 @@ class TimeDrivenNeuronModel_GPU {
                     throw EDLUTException(EDLUT_E_ARG, "model name does not match " + GetName());
             } else if (kv.first == "int_meth") {
                 ModelDescription d = std::any_cast<ModelDescription>(kv.second);
-                (void)integrator_id(d.name);
+                const int id = integrator_id(d.name);
                 if (!(d.step > 0.0f)) throw EDLUTException(EDLUT_E_ARG, "integration step must be positive");
+                if (id >= EDLUT_BIFIXED_EULER && d.n_steps <= 0)
+                    throw EDLUTException(EDLUT_E_ARG, "n_steps must be positive");
                 int_meth_ = d;
             } else {
                 size_t i = 0;
@@ -127,7 +134,8 @@ class TimeDrivenNeuronModel_GPU {
 
     void InitializeStates(int N_neurons, int /*OpenMPQueueIndex*/ = 0, int device = 0) {
         if (pop_) throw EDLUTException(EDLUT_E_STATE, "states already initialised");
-        check(edlut_population_create(model_, integrator_id(int_meth_.name), int_meth_.bdf_order,
+        const int integ = integrator_id(int_meth_.name);
+        check(edlut_population_create(model_, integ, integ >= EDLUT_BIFIXED_EULER ? int_meth_.n_steps : int_meth_.bdf_order,
                                       values_.data(), (int)values_.size(), int_meth_.step, N_neurons,
                                       device, &pop_));
         n_ = N_neurons;

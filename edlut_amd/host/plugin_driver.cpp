@@ -6,7 +6,7 @@
 //   plugin_driver selftest
 //       parameter handling only (no device): defaults, SetParameters
 //       errors, integration-method names; prints "ok".
-//   plugin_driver run <model> <integrator> <bdf_order> <in.bin> <out.bin>
+//   plugin_driver run <model> <integrator> <bdf_order|n_steps> <in.bin> <out.bin>
 //       in.bin : int32 n_neurons, int32 n_steps, float32 ext_current[n],
 //                then per step: int32 count, int32 target[count],
 //                int32 type[count], float32 weight[count]
@@ -53,6 +53,14 @@ static int selftest() {
         threw = true;
     }
     if (!threw) return 3;
+    threw = false;
+    try {
+        m.SetParameters({{"int_meth", ModelDescription{"Bifixed_RK4", 1e-4f, 1, 0}}});
+    } catch (const EDLUTException &e) {
+        threw = e.code() == EDLUT_E_ARG;
+    }
+    if (!threw) return 8;
+    m.SetParameters({{"int_meth", ModelDescription{"Bifixed_Euler", 1e-4f, 1, 3}}});
     m.SetParameters({{"g_leak", 12.5}, {"int_meth", ModelDescription{"RK4", 1e-4f, 1}}});
     ParamMap p = m.GetParameters();
     if (std::any_cast<float>(p["g_leak"]) != 12.5f) return 4;
@@ -75,7 +83,8 @@ static int run(int argc, char **argv) {
     if (argc < 7) return 64;
     auto m = make(argv[2]);
     const float dt = 1e-4f;
-    m->SetParameters({{"int_meth", ModelDescription{argv[3], dt, std::atoi(argv[4])}}});
+    const int iparam = std::atoi(argv[4]);   // BDF order, or the Bifixed methods' n_steps
+    m->SetParameters({{"int_meth", ModelDescription{argv[3], dt, iparam, iparam}}});
     FILE *in = std::fopen(argv[5], "rb");
     if (!in) return 66;
     int32_t n = 0, steps = 0;
@@ -134,6 +143,6 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "EDLUTException: %s\n", e.what());
         return 70;
     }
-    std::fprintf(stderr, "usage: plugin_driver selftest | run <model> <integrator> <bdf_order> <in> <out>\n");
+    std::fprintf(stderr, "usage: plugin_driver selftest | run <model> <integrator> <bdf_order|n_steps> <in> <out>\n");
     return 64;
 }

@@ -3,7 +3,8 @@
 Each class mirrors one of EDLUT's GPU plug-ins (TimeDrivenNeuronModel_GPU_C_INTERFACE
 subclasses, Kernel/include/neuron_model/TimeDrivenNeuronModel_GPU_C_INTERFACE.cuh:64-423):
 same registry name, same parameter names and defaults, same integration-method
-description ({"name", "step", "bdf_order"}), and the same calls
+description ({"name", "step", "bdf_order"}; {"name", "step", "n_steps"} for the
+Bifixed methods), and the same calls
 (InitializeStates, CheckSynapseType, ProcessInputSpike, ProcessInputCurrent,
 UpdateState).  UpdateState returns False like the reference; the spikes of the
 step are read with GetInternalSpikeIndexs() (the reference exposes a bool array
@@ -20,13 +21,24 @@ import numpy as np
 from . import _capi as C
 from ._capi import EDLUTException
 
-_INTEGRATORS = {"Euler": C.EDLUT_EULER, "RK2": C.EDLUT_RK2, "RK4": C.EDLUT_RK4, "BDF": C.EDLUT_BDF}
+_INTEGRATORS = {"Euler": C.EDLUT_EULER, "RK2": C.EDLUT_RK2, "RK4": C.EDLUT_RK4, "BDF": C.EDLUT_BDF,
+                "Bifixed_Euler": C.EDLUT_BIFIXED_EULER, "Bifixed_RK2": C.EDLUT_BIFIXED_RK2,
+                "Bifixed_RK4": C.EDLUT_BIFIXED_RK4}
 
 
-def integration_method(name: str = "Euler", step: float = 1e-3, bdf_order: int = 1) -> dict:
+def integration_method(name: str = "Euler", step: float = 1e-3, bdf_order: int = 1, n_steps: int = 2) -> dict:
     """Integration-method description (IntegrationMethod.h GetDefaultParameters:
-    step 0.001 s; BDFn_GPU_C_INTERFACE.cuh:275-288 for bdf_order)."""
+    step 0.001 s; BDFn_GPU_C_INTERFACE.cuh:275-288 for bdf_order; the Bifixed
+    methods' n_steps, sub-steps per update, default 2,
+    BifixedStep_GPU_C_INTERFACE.cuh:144-171)."""
+    if name.startswith("Bifixed_"):
+        return {"name": name, "step": float(step), "n_steps": int(n_steps)}
     return {"name": name, "step": float(step), "bdf_order": int(bdf_order)}
+
+
+def _int_param(m: dict) -> int:
+    """The integrator's integer parameter of the C ABI: n_steps or bdf_order."""
+    return int(m["n_steps"]) if m["name"].startswith("Bifixed_") else int(m["bdf_order"])
 
 
 class TimeDrivenNeuronModel_GPU:
@@ -84,7 +96,10 @@ class TimeDrivenNeuronModel_GPU:
                     raise EDLUTException(C.EDLUT_E_ARG, f"unknown integration method {m.get('name')!r}")
                 if not float(m.get("step", 0.0)) > 0.0:
                     raise EDLUTException(C.EDLUT_E_ARG, "integration step must be positive")
-                self._int = integration_method(m["name"], m.get("step", 1e-3), m.get("bdf_order", 1))
+                if m["name"].startswith("Bifixed_") and int(m.get("n_steps", 2)) <= 0:
+                    raise EDLUTException(C.EDLUT_E_ARG, "n_steps must be positive")
+                self._int = integration_method(m["name"], m.get("step", 1e-3), m.get("bdf_order", 1),
+                                               m.get("n_steps", 2))
             elif k in self._p:
                 self._p[k] = float(np.float32(v))
             else:
@@ -100,7 +115,7 @@ class TimeDrivenNeuronModel_GPU:
             C.lib().edlut_population_create(
                 self.MODEL,
                 _INTEGRATORS[self._int["name"]],
-                self._int["bdf_order"],
+                _int_param(self._int),
                 C.ptr(params),
                 len(params),
                 float(np.float32(self._int["step"])),

@@ -31,8 +31,13 @@ enum edlut_model {
     EDLUT_IZHIKEVICH = 5, /* "IzhikevichTimeDrivenModel_GPU" (IzhikevichTimeDrivenModel_GPU_C_INTERFACE.cu) */
     EDLUT_LIF_IS = 6     /* "LIFTimeDrivenModel_IS_GPU" (LIFTimeDrivenModel_IS_GPU_C_INTERFACE.cu), SI units */
 };
-/* Integration methods; names of IntegrationMethodFactory_GPU_C_INTERFACE.cu:40-145. */
-enum edlut_integrator { EDLUT_EULER = 0, EDLUT_RK2 = 1, EDLUT_RK4 = 2, EDLUT_BDF = 3 };
+/* Integration methods; names of IntegrationMethodFactory_GPU_C_INTERFACE.cu:40-145.
+ * The Bifixed methods (Bifixed_{Euler,RK2,RK4}_GPU2.cuh) take n_steps steps of
+ * the base method, of step/n_steps each, in every update. */
+enum edlut_integrator {
+    EDLUT_EULER = 0, EDLUT_RK2 = 1, EDLUT_RK4 = 2, EDLUT_BDF = 3,
+    EDLUT_BIFIXED_EULER = 4, EDLUT_BIFIXED_RK2 = 5, EDLUT_BIFIXED_RK4 = 6
+};
 /* Synapse-type bits (CheckSynapseType, AdExTimeDrivenModel_GPU_C_INTERFACE.cu:239-290). */
 enum edlut_synapse { EDLUT_SYN_EXC = 1, EDLUT_SYN_INH = 2, EDLUT_SYN_NMDA = 4, EDLUT_SYN_EXT_I = 8 };
 
@@ -71,8 +76,10 @@ const char *edlut_last_error(void);
  *         tau_inh tau_nmda (IzhikevichTimeDrivenModel.cpp:389-401)
  *   LIF_IS (LIFTimeDrivenModel_IS_GPU): the LIF order in SI units (V, F, s, S)
  *         (LIFTimeDrivenModel_IS.cpp:392-404)
- * bdf_order (1..6) is the BDF method's order ("bdf_order" parameter,
- * BDFn_GPU_C_INTERFACE.cuh:275-288); ignored by the other integrators.
+ * bdf_order is the integrator's integer parameter: the BDF method's order
+ * (1..6, "bdf_order", BDFn_GPU_C_INTERFACE.cuh:275-288), or the Bifixed
+ * methods' sub-steps per update (>= 1, "n_steps",
+ * BifixedStep_GPU_C_INTERFACE.cuh:144-158); ignored by Euler, RK2 and RK4.
  * dt_s is the integration method's "step" parameter (seconds).  State lives on
  * `device` as structure-of-arrays: variable v of neuron i at v*stride+i.
  */

@@ -1211,9 +1211,28 @@ template <typename M, typename R> struct Pop : PopIface {
     int integrator = ORC_EULER;
     BdfStore<R> bdf;
     bool round_to_float = false;   /* FP64 Newton mode: state kept in float between steps */
+    /* Bifixed methods (Bifixed_{Euler,RK2,RK4}_GPU2.cuh): every update takes
+     * n_sub steps of the base method with the step's seconds divided by
+     * n_sub in float (BifixedStep_GPU_C_INTERFACE.cuh:157-158); the model
+     * scale is that times the time scale (BifixedStep_GPU2.cuh:81-82), the
+     * decay factors are those of the sub-step (Bifixed_RK4_GPU2.cuh:237-241),
+     * the spike clock gains the sub-step and the spike condition is checked
+     * after every sub-step (Bifixed_RK4_GPU2.cuh:143-149); a neuron is
+     * reported once per update (a flag per neuron), and the NaN check runs
+     * once, after the last sub-step. */
+    int n_sub = 1;
+    float update_dt_s = 0.0f;
+    std::vector<uint8_t> fired_flag;
 
     Pop(int integ, int bdf_order, const float *params, float dt_s, int n, uint32_t mask) {
+        if (integ >= ORC_BIFIXED_EULER) {
+            n_sub = bdf_order;
+            integ -= ORC_BIFIXED_EULER;
+            fired_flag.assign(n, 0);
+        }
         integrator = integ;
+        update_dt_s = dt_s;
+        dt_s = dt_s / (float)n_sub;
         m.set_params(params);
         m.N = n;
         m.dt_s = dt_s;
@@ -1252,14 +1271,30 @@ template <typename M, typename R> struct Pop : PopIface {
         for (int i = 0; i < m.N; i++) m.state[(size_t)i * m.nvars + m.ndiff + 3] = (R)I[i];
     }
     bool exc_enabled() override { return m.EXC; }
-    float dt_s() override { return m.dt_s; }
-    /* AdExTimeDrivenModel.cpp:92-101 + TimeDrivenNeuronModel.cpp:75-88. */
-    int update(int64_t /*step*/, int32_t *spikes) override {
-        m.spikes.clear();
+    float dt_s() override { return update_dt_s; }
+    void step_once() {
         if (integrator == ORC_EULER) step_euler<M, R>(m);
         else if (integrator == ORC_RK2) step_rk2<M, R>(m);
         else if (integrator == ORC_RK4) step_rk4<M, R>(m);
         else step_bdf<M, R>(m, bdf);
+    }
+    /* AdExTimeDrivenModel.cpp:92-101 + TimeDrivenNeuronModel.cpp:75-88. */
+    int update(int64_t /*step*/, int32_t *spikes) override {
+        m.spikes.clear();
+        if (fired_flag.empty()) {
+            step_once();
+        } else {
+            for (int it = 0; it < n_sub; it++) {
+                step_once();
+                for (int32_t i : m.spikes) fired_flag[i] = 1;
+                m.spikes.clear();
+            }
+            for (int i = 0; i < m.N; i++)
+                if (fired_flag[i]) {
+                    m.spikes.push_back(i);
+                    fired_flag[i] = 0;
+                }
+        }
         if (round_to_float)
             for (auto &v : m.state) v = (R)(float)v;
         /* A NaN membrane potential resets that neuron to its initial state. */
@@ -1508,6 +1543,8 @@ uint32_t orc_drive_hash(uint64_t seed, int64_t step, uint32_t neuron) {
 void *orc_pop_create(int model, int integrator, int bdf_order, const float *params,
                      int /*n_params*/, float dt_s, int n_neurons, uint32_t syn_mask, int fp64) {
     if (integrator == ORC_BDF && (bdf_order < 1 || bdf_order > 6)) return nullptr;
+    if (integrator < ORC_EULER || integrator > ORC_BIFIXED_RK4) return nullptr;
+    if (integrator >= ORC_BIFIXED_EULER && bdf_order < 1) return nullptr;
     /* fp64: 0 float restatement, 1 FP64 oracle, 2 FP64 with the state rounded
      * to float after every step (the HIP path's FP64 Newton mode for HH). */
     PopIface *pi = make_pop(model, integrator, bdf_order, params, dt_s, n_neurons, syn_mask, fp64 != 0);

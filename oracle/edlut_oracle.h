@@ -27,7 +27,9 @@ extern "C" {
 /* Model and integrator identifiers (same numbering as include/edlut_amd.h). */
 enum { ORC_LIF = 0, ORC_ADEX = 1, ORC_HH = 2, ORC_GRANULE = 3, ORC_PURKINJE = 4, ORC_IZHIKEVICH = 5,
        ORC_LIF_IS = 6 };
-enum { ORC_EULER = 0, ORC_RK2 = 1, ORC_RK4 = 2, ORC_BDF = 3 };
+enum { ORC_EULER = 0, ORC_RK2 = 1, ORC_RK4 = 2, ORC_BDF = 3,
+       /* GPU Bifixed methods: n_steps sub-steps of the base method per update */
+       ORC_BIFIXED_EULER = 4, ORC_BIFIXED_RK2 = 5, ORC_BIFIXED_RK4 = 6 };
 
 /* ---- look-up tables ---------------------------------------------------- */
 /* ExponentialTable::LookUpTable (16384 floats). */
@@ -55,7 +57,8 @@ void orc_lcg_seeds(uint32_t seed, int n, uint32_t *out);
 void orc_lcg_frand(uint32_t x0, int n, float *out);
 
 /* ---- one population ----------------------------------------------------- */
-/* fp64 != 0 builds the FP64 oracle (double arithmetic, exact exp). */
+/* fp64 != 0 builds the FP64 oracle (double arithmetic, exact exp).
+ * bdf_order is the BDF order, or n_steps (>= 1) for the Bifixed methods. */
 void *orc_pop_create(int model, int integrator, int bdf_order, const float *params,
                      int n_params, float dt_s, int n_neurons, uint32_t syn_mask, int fp64);
 void orc_pop_destroy(void *pop);

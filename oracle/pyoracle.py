@@ -19,7 +19,7 @@ REF_DUMP = os.path.join(HERE, "_ref", "ref_dump")
 MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1, "HHTimeDrivenModel_GPU": 2,
           "EgidioGranuleCell_TimeDriven_GPU": 3, "TimeDrivenPurkinjeCell_GPU": 4,
           "IzhikevichTimeDrivenModel_GPU": 5, "LIFTimeDrivenModel_IS_GPU": 6}
-INTEGRATORS = {"Euler": 0, "RK2": 1, "RK4": 2, "BDF": 3}
+INTEGRATORS = {"Euler": 0, "RK2": 1, "RK4": 2, "BDF": 3, "Bifixed_Euler": 4, "Bifixed_RK2": 5, "Bifixed_RK4": 6}
 
 P = ctypes.c_void_p
 _lib = None
@@ -160,10 +160,12 @@ class Population:
     """One population of the oracle (float restatement, or FP64 with fp64=True)."""
 
     def __init__(self, model: str, integrator: str, params, n: int, dt_s=np.float32(1e-4),
-                 mask: int = 3, fp64=False, bdf_order: int = 1):
+                 mask: int = 3, fp64=False, bdf_order: int = 1, n_steps: int = 2):
+        # Bifixed methods take n_steps (sub-steps per update) where BDF takes its order
         p = np.ascontiguousarray(params, np.float32)
         self.n = n
-        self.h = lib().orc_pop_create(MODELS[model], INTEGRATORS[integrator], bdf_order, _p(p), len(p),
+        iparam = n_steps if integrator.startswith("Bifixed") else bdf_order
+        self.h = lib().orc_pop_create(MODELS[model], INTEGRATORS[integrator], iparam, _p(p), len(p),
                                       float(np.float32(dt_s)), n, mask, int(fp64))
         if not self.h:
             raise ValueError("oracle does not implement this model")

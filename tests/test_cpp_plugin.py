@@ -59,6 +59,7 @@ def _read_outputs(path, n):
     ("AdExTimeDrivenModel_GPU", "RK4", 1, ADEX_DEFAULTS, 6.0, 150.0),
     ("LIFTimeDrivenModel_GPU", "Euler", 1, LIF_DEFAULTS, 0.3, 1.0),
     ("HHTimeDrivenModel_GPU", "BDF", 2, HH_DEFAULTS, 6.0, 100.0),
+    ("LIFTimeDrivenModel_GPU", "Bifixed_RK2", 3, LIF_DEFAULTS, 0.3, 1.0),
 ])
 def test_cpp_plugin_matches_oracle(oracle, tmp_path, model, integ, order, params, wmax, cur):
     n, steps = 700, 400
@@ -70,7 +71,7 @@ def test_cpp_plugin_matches_oracle(oracle, tmp_path, model, integ, order, params
                         str(tmp_path / "out.bin")], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     st, ix, state = _read_outputs(tmp_path / "out.bin", n)
-    pop = oracle.Population(model, integ, params, n, mask=15, bdf_order=order)
+    pop = oracle.Population(model, integ, params, n, mask=15, bdf_order=order, n_steps=order)
     pop.set_ext_current(ext)
     ost, oix = [], []
     for k, (tg, ty, w) in enumerate(events, start=1):
