@@ -38,6 +38,8 @@ EXPORTS = (
     "edlut_network_add_poisson",
     "edlut_network_set_external",
     "edlut_network_set_poisson_rates",
+    "edlut_network_add_spike_train",
+    "edlut_network_set_spike_train",
     "edlut_network_add_projection",
     "edlut_circuit_create",
     "edlut_circuit_run",
@@ -121,6 +123,8 @@ def _sig(lib):
         "edlut_network_add_poisson": (I32, [P, I32, P, F32, P, P, P]),
         "edlut_network_set_external": (I32, [P, I32, P, F32, U64]),
         "edlut_network_set_poisson_rates": (I32, [P, I32, P]),
+        "edlut_network_add_spike_train": (I32, [P, I32, I64, P, P, P, P, P]),
+        "edlut_network_set_spike_train": (I32, [P, I32, I64, P, P]),
         "edlut_network_add_projection": (I32, [P, P, P, P, P]),
         "edlut_circuit_create": (I32, [P, I32, P]),
         "edlut_circuit_run": (I32, [P, I64, I32]),

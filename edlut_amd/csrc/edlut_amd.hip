@@ -394,6 +394,13 @@ struct Source {
     // its spikes are that network's own ring
     edlut_network *from = nullptr;
     float *rates = nullptr;       // per-generator Poisson frequencies (or null)
+    // spike train (edlut_network_add_spike_train): spikes of update k are
+    // train_idx[train_ptr[k-1] .. train_ptr[k])
+    bool train = false;
+    int64_t *train_ptr = nullptr;
+    int32_t *train_idx = nullptr;
+    int64_t train_steps = 0;
+    int train_blocks = 1;
     int64_t n = 0;
     int32_t *ring_idx = nullptr;
     int32_t *ring_cnt = nullptr;
@@ -1457,7 +1464,7 @@ int edlut_network_destroy(edlut_network *n) {
         const bool own_ring = !s.self && !s.from;
         void *dev[] = {own_ring ? s.ring_idx : nullptr, own_ring ? s.ring_cnt : nullptr, s.row_ptr,
                        s.tgt_type, s.w, s.lcg, s.tile_count, s.staging, s.syn.pre, s.syn.post,
-                       s.syn.last, s.in_ptr, s.in_syn, s.rates};
+                       s.syn.last, s.in_ptr, s.in_syn, s.rates, s.train_ptr, s.train_idx};
         for (void *q : dev)
             if (q) (void)hipFree(q);
         if (s.from) s.from->links_out--;
@@ -1553,6 +1560,86 @@ int edlut_network_add_poisson(edlut_network *n, int n_src, const uint32_t *lcg_s
     return n->pop->reserve_events(n->total_syn + 1);
 }
 
+// Validates a spike schedule and uploads it into the train arrays of `s`
+// (replacing earlier ones).
+static int load_train(edlut_network *n, Source &s, int64_t n_steps, const int64_t *step_ptr,
+                      const int32_t *idx) {
+    if (n_steps > 0 && !step_ptr) return fail(EDLUT_E_ARG, "step_ptr is null");
+    if (n_steps < 0 || n_steps >= (int64_t)INT32_MAX) return fail(EDLUT_E_ARG, "n_steps out of range");
+    int64_t longest = 0;
+    if (n_steps > 0) {
+        if (step_ptr[0] != 0) return fail(EDLUT_E_ARG, "step_ptr[0] must be 0");
+        for (int64_t k = 1; k <= n_steps; k++) {
+            const int64_t b = step_ptr[k - 1], e = step_ptr[k];
+            if (e < b) return fail(EDLUT_E_ARG, "step_ptr must be non-decreasing");
+            if (e - b > s.n) return fail(EDLUT_E_ARG, "update %lld: more spikes than sources", (long long)k);
+            if (e > b && !idx) return fail(EDLUT_E_ARG, "idx is null");
+            for (int64_t j = b; j < e; j++) {
+                if (idx[j] < 0 || idx[j] >= s.n)
+                    return fail(EDLUT_E_ARG, "update %lld: source %d out of range", (long long)k, idx[j]);
+                // one spike per source and update, in ascending order (the
+                // delivery order and the fold's fixed segments rely on it)
+                if (j > b && idx[j] <= idx[j - 1])
+                    return fail(EDLUT_E_ARG, "update %lld: spike sources must ascend without repeats",
+                                (long long)k);
+            }
+            longest = std::max(longest, e - b);
+        }
+    }
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    if (s.train_ptr) (void)hipFree(s.train_ptr);
+    if (s.train_idx) (void)hipFree(s.train_idx);
+    s.train_ptr = nullptr;
+    s.train_idx = nullptr;
+    s.train_steps = 0;
+    const int64_t total = n_steps > 0 ? step_ptr[n_steps] : 0;
+    int rc;
+    if ((rc = dalloc(&s.train_ptr, (size_t)n_steps + 1)) ||
+        (rc = dalloc(&s.train_idx, (size_t)std::max<int64_t>(total, 1))))
+        return rc;
+    if (n_steps > 0) {
+        HIP_TRY(hipMemcpy(s.train_ptr, step_ptr, sizeof(int64_t) * (size_t)(n_steps + 1), hipMemcpyHostToDevice));
+        if (total > 0) HIP_TRY(hipMemcpy(s.train_idx, idx, sizeof(int32_t) * (size_t)total, hipMemcpyHostToDevice));
+    }
+    s.train_steps = n_steps;
+    s.train_blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (longest + kBlock - 1) / kBlock));
+    n->dirty = true;   // captured kernels carry the train's arrays
+    return 0;
+}
+
+int edlut_network_add_spike_train(edlut_network *n, int n_src, int64_t n_steps, const int64_t *step_ptr,
+                                  const int32_t *idx, const int64_t *row_ptr, const uint32_t *tgt_type,
+                                  const float *w) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (n_src <= 0) return fail(EDLUT_E_ARG, "n_src must be positive");
+    if ((int)n->sources.size() >= kMaxSources) return fail(EDLUT_E_CAP, "too many sources");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    Source s;
+    s.self = false;
+    s.train = true;
+    s.n = n_src;
+    int rc = add_synapses(n, s, row_ptr, tgt_type, w);
+    if (rc) return rc;
+    if ((rc = load_train(n, s, n_steps, step_ptr, idx))) return rc;
+    if ((rc = dalloc(&s.ring_idx, (size_t)n->ring * n_src)) || (rc = dalloc(&s.ring_cnt, n->ring))) return rc;
+    HIP_TRY(hipMemset(s.ring_cnt, 0, sizeof(int32_t) * n->ring));
+    n->sources.push_back(s);
+    n->dirty = true;
+    n->seg_dirty = true;
+    n->total_syn += (uint64_t)row_ptr[(size_t)s.n * n->max_delay];
+    return n->pop->reserve_events(n->total_syn + 1);
+}
+
+int edlut_network_set_spike_train(edlut_network *n, int source, int64_t n_steps, const int64_t *step_ptr,
+                                  const int32_t *idx) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    Source &s = n->sources[(size_t)source];
+    if (!s.train) return fail(EDLUT_E_ARG, "source %d is not a spike train", source);
+    return load_train(n, s, n_steps, step_ptr, idx);
+}
+
 int edlut_network_add_projection(edlut_network *n, edlut_network *from, const int64_t *row_ptr,
                                  const uint32_t *tgt_type, const float *w) {
     if (!n || !from) return fail(EDLUT_E_ARG, "network is null");
@@ -1582,7 +1669,7 @@ int edlut_network_set_poisson_rates(edlut_network *n, int source, const float *r
     if (!n || !rates_hz) return fail(EDLUT_E_ARG, "null argument");
     if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
     Source &s = n->sources[(size_t)source];
-    if (s.self || s.from) return fail(EDLUT_E_ARG, "source %d is not a Poisson device", source);
+    if (s.self || s.from || s.train) return fail(EDLUT_E_ARG, "source %d is not a Poisson device", source);
     for (int64_t i = 0; i < s.n; i++)
         if (!std::isfinite(rates_hz[i])) return fail(EDLUT_E_ARG, "rate %lld is not finite", (long long)i);
     HIP_TRY(hipSetDevice(n->pop->device));
@@ -1623,6 +1710,19 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     // Poisson devices fire at update k (their spikes feed later updates).
     for (Source &s : n->sources) {
         if (s.self || s.from) continue;
+        if (s.train) {
+            TrainArgs ta;
+            ta.ptr = s.train_ptr;
+            ta.idx = s.train_idx;
+            ta.n_steps = s.train_steps;
+            ta.ring_idx = s.ring_idx;
+            ta.ring_cnt = s.ring_cnt;
+            ta.n_src = s.n;
+            ta.ring = n->ring;
+            ta.step = step;
+            hipLaunchKernelGGL(k_train, dim3(s.train_blocks), dim3(kBlock), 0, p->stream, ta);
+            continue;
+        }
         PoissonArgs pa;
         std::memset(&pa, 0, sizeof(pa));
         pa.lcg = s.lcg;

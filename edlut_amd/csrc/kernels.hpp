@@ -592,6 +592,38 @@ __global__ __launch_bounds__(kBlock) void k_poisson(PoissonArgs a) {
     tile_spikes<1>(a.out, tile, spk, idx);
 }
 
+// ---------------------------------------------------------------------------
+// Spike-train source: input neurons (InputSpikeNeuronModel driven by
+// FileInputSpikeDriver.cpp:45-85 or Simulation_API::AddExternalSpikeActivity)
+// whose spikes the host has placed on update steps.  Update k copies its
+// ascending, duplicate-free list idx[ptr[k-1] .. ptr[k]) into ring slot
+// k % ring; steps past the train have no spikes.
+// ---------------------------------------------------------------------------
+struct TrainArgs {
+    const int64_t *ptr;     // [n_steps + 1]
+    const int32_t *idx;
+    int64_t n_steps;
+    int32_t *ring_idx;      // [ring][n_src]
+    int32_t *ring_cnt;      // [ring]
+    int64_t n_src;
+    int ring;
+    StepRef step;
+};
+
+__global__ __launch_bounds__(kBlock) void k_train(TrainArgs a) {
+    const int32_t k = a.step.get();
+    const int slot = k % a.ring;
+    int64_t b = 0, e = 0;
+    if (k >= 1 && (int64_t)k <= a.n_steps) {
+        b = a.ptr[k - 1];
+        e = a.ptr[k];
+    }
+    int32_t *out = a.ring_idx + (int64_t)slot * a.n_src;
+    for (int64_t j = b + (int64_t)blockIdx.x * kBlock + threadIdx.x; j < e; j += (int64_t)gridDim.x * kBlock)
+        out[j - b] = a.idx[j];
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.ring_cnt[slot] = (int32_t)(e - b);
+}
+
 // Events pushed by the host: order = position in the host's list.
 struct PushEvents {
     const int32_t *tgt;

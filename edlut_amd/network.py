@@ -100,6 +100,33 @@ class Network:
         r = np.ascontiguousarray(rates_hz, np.float32)
         C.check(C.lib().edlut_network_set_poisson_rates(self._h, int(source), C.ptr(r)))
 
+    def add_spike_train(self, n_src: int, step_ptr, idx, row_ptr, tgt_type, weight):
+        """Input neurons that fire at given updates: update k (1-based) emits
+        idx[step_ptr[k-1]:step_ptr[k]] (ascending, no repeats).  The events
+        of a spike at update s through a synapse of delay d are consumed by
+        update s + d + 1, as for every other source."""
+        step_ptr = np.ascontiguousarray(step_ptr, np.int64)
+        idx = np.ascontiguousarray(idx, np.int32)
+        row_ptr = np.ascontiguousarray(row_ptr, np.int64)
+        tgt_type = np.ascontiguousarray(tgt_type, np.uint32)
+        weight = np.ascontiguousarray(weight, np.float32)
+        n_steps = max(len(step_ptr) - 1, 0)
+        C.check(
+            C.lib().edlut_network_add_spike_train(
+                self._h, int(n_src), n_steps, C.ptr(step_ptr) if n_steps else None,
+                C.ptr(idx) if len(idx) else None, C.ptr(row_ptr), C.ptr(tgt_type), C.ptr(weight)
+            )
+        )
+
+    def set_spike_train(self, source: int, step_ptr, idx):
+        """Replaces the schedule of spike-train source `source`."""
+        step_ptr = np.ascontiguousarray(step_ptr, np.int64)
+        idx = np.ascontiguousarray(idx, np.int32)
+        n_steps = max(len(step_ptr) - 1, 0)
+        C.check(C.lib().edlut_network_set_spike_train(self._h, int(source), n_steps,
+                                                      C.ptr(step_ptr) if n_steps else None,
+                                                      C.ptr(idx) if len(idx) else None))
+
     def add_projection(self, source: "Network", row_ptr, tgt_type, weight):
         """Synapses from `source`'s population onto this network's population
         (rows over the source neurons).  Networks with projections run in a

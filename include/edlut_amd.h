@@ -195,6 +195,27 @@ int edlut_network_add_poisson(edlut_network *net, int n_src, const uint32_t *lcg
 int edlut_network_set_poisson_rates(edlut_network *net, int source, const float *rates_hz);
 
 /*
+ * Spike-train source of n_src input neurons: the spikes that the reference's
+ * InputSpikeNeuronModel layers receive from FileInputSpikeDriver
+ * (FileInputSpikeDriver.cpp:45-85) or Simulation_API::AddExternalSpikeActivity
+ * (Simulation_API.cpp), placed by the host on update steps.  Update k
+ * (1..n_steps) emits idx[step_ptr[k-1] .. step_ptr[k]), ascending and without
+ * repeats; its events reach the population like a Poisson device's, one
+ * update of lookahead plus the synapse's delay later.  Later updates emit
+ * nothing.
+ */
+int edlut_network_add_spike_train(edlut_network *net, int n_src, int64_t n_steps,
+                                  const int64_t *step_ptr, const int32_t *idx,
+                                  const int64_t *row_ptr, const uint32_t *tgt_type,
+                                  const float *weight);
+/* Replaces the schedule of spike-train source `source` (same layout, steps
+ * counted from update 1; updates already run are not replayed).  The
+ * reference inserts external spikes into its event queue between runs
+ * (Simulation_API::AddExternalSpikeActivity). */
+int edlut_network_set_spike_train(edlut_network *net, int source, int64_t n_steps,
+                                  const int64_t *step_ptr, const int32_t *idx);
+
+/*
  * Projection of the population of network `from` onto the population of
  * `net` (another neuron model, e.g. granule -> Purkinje): CSR rows over the
  * source population's neurons, the same layout as edlut_network_add_recurrent.

@@ -1349,6 +1349,11 @@ struct Source {
     std::vector<uint32_t> lcg;
     float rate = 0.0f;
     std::vector<float> rates;
+    /* spike train (input neurons placed on update steps): the spikes of
+     * update k are train_idx[train_ptr[k-1] .. train_ptr[k]) */
+    bool train = false;
+    std::vector<int64_t> train_ptr;
+    std::vector<int32_t> train_idx;
     /* projection from another network's population: its spikes */
     struct Net *from = nullptr;
     /* spikes per step: ring indexed by step % ring */
@@ -1620,6 +1625,19 @@ int orc_net_add_source_poisson(void *net, int n_src, const uint32_t *lcg_state, 
     return add_source(n, s, row_ptr, tgt_type, w);
 }
 
+int orc_net_add_source_train(void *net, int n_src, int64_t n_steps, const int64_t *step_ptr,
+                             const int32_t *idx, const int64_t *row_ptr, const uint32_t *tgt_type,
+                             const float *w) {
+    Net *n = (Net *)net;
+    Source s;
+    s.self = false;
+    s.train = true;
+    s.n = n_src;
+    s.train_ptr.assign(step_ptr, step_ptr + n_steps + 1);
+    s.train_idx.assign(idx, idx + (n_steps > 0 ? step_ptr[n_steps] : 0));
+    return add_source(n, s, row_ptr, tgt_type, w);
+}
+
 void orc_net_set_external(void *net, int n_thr, const uint32_t *thresholds, float w_ext,
                           uint64_t seed) {
     Net *n = (Net *)net;
@@ -1729,6 +1747,14 @@ static int net_step(Net *n, int64_t k, std::vector<int32_t> &spk) {
         if (src.self || src.from) continue;
         std::vector<int32_t> &list = src.ring[k % n->ring];
         list.clear();
+        if (src.train) {
+            /* InputSpikeNeuronModel: the spikes the host placed on update k */
+            const int64_t steps = (int64_t)src.train_ptr.size() - 1;
+            if (k >= 1 && k <= steps)
+                for (int64_t j = src.train_ptr[(size_t)k - 1]; j < src.train_ptr[(size_t)k]; j++)
+                    list.push_back(src.train_idx[(size_t)j]);
+            continue;
+        }
         const double step_s = (double)pop->dt_s();
         for (int i = 0; i < src.n; i++) {
             const float f = src.rates.empty() ? src.rate : src.rates[(size_t)i];

@@ -118,6 +118,10 @@ uint32_t orc_drive_hash(uint64_t seed, int64_t step, uint32_t neuron);
  * source's neurons, same layout as the other sources); same max_delay. */
 int orc_net_add_link(void *net, void *source, const int64_t *row_ptr, const uint32_t *tgt_type,
                      const float *w);
+/* Spike train: update k (1..n_steps) emits idx[step_ptr[k-1] .. step_ptr[k]). */
+int orc_net_add_source_train(void *net, int n_src, int64_t n_steps, const int64_t *step_ptr,
+                             const int32_t *idx, const int64_t *row_ptr, const uint32_t *tgt_type,
+                             const float *w);
 /* Per-generator frequencies (Hz) of Poisson source `source`. */
 int orc_net_set_poisson_rates(void *net, int source, const float *rates);
 /* Lockstep run: update k of every network, in the order given, before update

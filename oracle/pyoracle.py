@@ -83,6 +83,7 @@ def lib():
         l.orc_net_get_trace.argtypes = [P, P, P, P, P, ctypes.c_int64]
         l.orc_net_add_link.argtypes = [P, P, P, P, P]
         l.orc_net_set_poisson_rates.argtypes = [P, ctypes.c_int, P]
+        l.orc_net_add_source_train.argtypes = [P, ctypes.c_int, ctypes.c_int64, P, P, P, P, P]
         l.orc_circuit_run.argtypes = [P, ctypes.c_int, ctypes.c_int64, ctypes.c_int]
         l.orc_net_take_log.restype = ctypes.c_int64
         l.orc_net_take_log.argtypes = [P, P, P, ctypes.c_int64]
@@ -222,6 +223,18 @@ class Net:
         a = [np.ascontiguousarray(row_ptr, np.int64), np.ascontiguousarray(tgt_type, np.uint32),
              np.ascontiguousarray(w, np.float32)]
         lib().orc_net_add_source_poisson(self.h, len(s), _p(s), float(rate), *[_p(x) for x in a])
+
+    def add_train(self, n_src, step_ptr, idx, row_ptr, tgt_type, w):
+        """Input neurons firing at given updates: update k emits
+        idx[step_ptr[k-1]:step_ptr[k]]."""
+        sp = np.ascontiguousarray(step_ptr, np.int64)
+        ix = np.ascontiguousarray(idx, np.int32)
+        if len(sp) == 0:
+            sp = np.zeros(1, np.int64)
+        a = [np.ascontiguousarray(row_ptr, np.int64), np.ascontiguousarray(tgt_type, np.uint32),
+             np.ascontiguousarray(w, np.float32)]
+        lib().orc_net_add_source_train(self.h, int(n_src), len(sp) - 1, _p(sp), _p(ix) if len(ix) else None,
+                                       *[_p(x) for x in a])
 
     def set_external(self, thr, w_ext, seed):
         t = np.ascontiguousarray(thr, np.uint32)
