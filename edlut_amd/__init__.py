@@ -19,6 +19,7 @@ from .models import (
     integration_method,
 )
 from .network import Circuit, Network, binomial_thresholds, build_out_csr, from_spec
+from .simulation import RandomGenerator, Simulation_API
 
 __all__ = [
     "EDLUTException",
@@ -39,4 +40,6 @@ __all__ = [
     "binomial_thresholds",
     "build_out_csr",
     "from_spec",
+    "Simulation_API",
+    "RandomGenerator",
 ]
