@@ -515,6 +515,14 @@ int plain_items(int model, int integrator) {
     return model == EDLUT_GRANULE ? 2 : kItems;
 }
 
+// Plain AdEx and LIF populations with spike-driven synapse types only
+// (mask 0..3) under Euler, RK2 or RK4 also have a two-neurons-per-thread
+// update kernel; the autotuner picks between the two tile shapes.
+bool plain_two_items(const edlut_population *p) {
+    return !p->grouped && (p->model == EDLUT_ADEX || p->model == EDLUT_LIF) && (p->mask & ~3u) == 0 &&
+           (p->integ == EDLUT_EULER || p->integ == EDLUT_RK2 || p->integ == EDLUT_RK4);
+}
+
 template <class M, int I, unsigned MASK>
 void fill_one(UpdateFn *t) {
     t[(I * 16 + (int)MASK) * 2 + 0] = k_update<M, I, MASK, false, kItems, false>;
@@ -525,6 +533,25 @@ template <class M, int I, unsigned... MS>
 void fill_masks(UpdateFn *t, std::integer_sequence<unsigned, MS...>) {
     (fill_one<M, I, MS>(t), ...);
 }
+
+template <class M, int I, unsigned MASK>
+void fill_two_one(UpdateFn *t) {
+    t[(I * 4 + (int)MASK) * 2 + 0] = k_update<M, I, MASK, false, 2, false>;
+    t[(I * 4 + (int)MASK) * 2 + 1] = k_update<M, I, MASK, true, 2, false>;
+}
+template <class M, int I, unsigned... MS>
+void fill_two(UpdateFn *t, std::integer_sequence<unsigned, MS...>) {
+    (fill_two_one<M, I, MS>(t), ...);
+}
+template <class M>
+struct TwoItemTable {
+    UpdateFn fn[3 * 4 * 2];
+    TwoItemTable() {
+        fill_two<M, kEuler>(fn, std::make_integer_sequence<unsigned, 4>{});
+        fill_two<M, kRK2>(fn, std::make_integer_sequence<unsigned, 4>{});
+        fill_two<M, kRK4>(fn, std::make_integer_sequence<unsigned, 4>{});
+    }
+};
 
 template <class M>
 struct Table {
@@ -646,6 +673,12 @@ UpdateFn pick_update(const edlut_population *p, bool ext) {
         if (model == EDLUT_GRANULE && integ == EDLUT_BDF && !lds_ok) return nullptr;
         const int idx = integ * 2 + (ext ? 1 : 0);
         return model == EDLUT_GRANULE ? gr.fn[idx] : pk.fn[idx];
+    }
+    if (p->items == 2 && plain_two_items(p)) {
+        static TwoItemTable<AdExModel> adex2;
+        static TwoItemTable<LIFModel> lif2;
+        const int idx = (integ * 4 + m) * 2 + (ext ? 1 : 0);
+        return model == EDLUT_ADEX ? adex2.fn[idx] : lif2.fn[idx];
     }
     if (p->items != kItems) return nullptr;
     if (integ >= EDLUT_BIFIXED_EULER) {
@@ -2465,8 +2498,9 @@ int edlut_network_set_launch(edlut_network *n, int items, int fold_lanes, int fo
     if (!n) return fail(EDLUT_E_ARG, "network is null");
     edlut_population *p = n->pop;
     const int fixed = plain_items(p->model, p->integ);
-    if (p->grouped ? (items != 2 && items != 4) : items != fixed)
-        return fail(EDLUT_E_ARG, "items per thread must be %s", p->grouped ? "2 or 4" : (fixed == 2 ? "2" : "4"));
+    const bool two_or_four = p->grouped || plain_two_items(p);
+    if (two_or_four ? (items != 2 && items != 4) : items != fixed)
+        return fail(EDLUT_E_ARG, "items per thread must be %s", two_or_four ? "2 or 4" : (fixed == 2 ? "2" : "4"));
     if (!pick_ring_fold(kCount, fold_lanes)) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
     if (fold_blocks < 1 || fold_blocks > 65536) return fail(EDLUT_E_ARG, "fold blocks must be in [1, 65536]");
     p->items = items;
@@ -2493,7 +2527,8 @@ int edlut_network_autotune(edlut_network *n, int trial_steps, int *items, int *f
     if (trial_steps < 16 || trial_steps > 100000) return fail(EDLUT_E_ARG, "trial_steps must be in [16, 100000]");
     edlut_population *p = n->pop;
     HIP_TRY(hipSetDevice(p->device));
-    std::vector<int> it_c = p->grouped ? std::vector<int>{2, 4} : std::vector<int>{plain_items(p->model, p->integ)};
+    std::vector<int> it_c = (p->grouped || plain_two_items(p)) ? std::vector<int>{2, 4}
+                                                                : std::vector<int>{plain_items(p->model, p->integ)};
     const int lanes_c[] = {16, 32, 64};
     const int blocks_c[] = {64, 256, 1024, 4096};
     hipEvent_t e0, e1;

@@ -91,3 +91,24 @@ def test_graph_replay_equals_direct_launches():
     assert np.array_equal(a, c) and np.array_equal(b, d)
     for x, y in zip(sa, sc):
         assert np.array_equal(bits(x), bits(y))
+
+
+@pytest.mark.parametrize("kind", ["adex", "lif"])
+def test_two_items_per_thread_launch(oracle, kind):
+    """Plain AdEx/LIF networks also run with two neurons per thread (the
+    autotuner's other tile shape): same raster and state as the oracle."""
+    if kind == "adex":
+        spec, params = workloads.adex_recurrent(n=20000, steps=1500, seed=5, w_ext=1.0), ADEX_DEFAULTS
+    else:
+        spec, params = workloads.lif_poisson(n=1000, steps=3000), LIF_DEFAULTS
+    print(f"case two-items {kind}", flush=True)
+    model, net = E.from_spec(spec)
+    net.set_launch(2, 32, 256)
+    net.run(spec_steps := (1500 if kind == "adex" else 3000))
+    st, ix = net.spikes()
+    pop, onet = oracle.from_spec(spec, params, np_out_csr)
+    ost, oix = onet.run(1, spec_steps)
+    assert len(ost) > 100
+    assert np.array_equal(st, ost) and np.array_equal(ix, oix)
+    for v in range(pop.nv):
+        assert np.array_equal(bits(model.GetVectorNeuronState(v)), bits(pop.state(v))), f"variable {v}"
