@@ -473,3 +473,40 @@ def parse_input_spikes(path: str, n_neurons: int | None = None):
                 times.append(float(tt))
                 cells.append(nn + r)
     return np.array(times, np.float64), np.array(cells, np.int64)
+
+
+def parse_input_currents(path: str, n_neurons: int | None = None):
+    """FileInputCurrentDriver::LoadInputs (FileInputCurrentDriver.cpp:45-90): a
+    count, then lines ``time ncurrents interval first_neuron nneurons
+    current``; change j of a line happens at float(time + j * interval) on
+    each of its neurons.  Returns (times float64, neuron indices int64,
+    currents float32) in file order."""
+    f = _CFile(path)
+    f.skip_comments()
+    ninputs = f.scan_i()
+    if ninputs is None:
+        f.fail("the number of input currents cannot be read")
+    times, cells, vals = [], [], []
+    i = 0
+    while i < ninputs:
+        f.skip_comments()
+        t = f.scan_f()
+        ns = f.scan_i()
+        iv = f.scan_f()
+        nn = f.scan_i()
+        nr = f.scan_i()
+        cur = f.scan_f()
+        if t is None or ns is None or iv is None or nn is None or nr is None or cur is None:
+            f.fail("fewer input currents than declared")
+        if nn < 0 or (n_neurons is not None and nn + nr > n_neurons):
+            f.fail("input current neuron index out of range")
+        i += ns * nr
+        if i > ninputs:
+            f.fail("more input currents than declared")
+        for it in range(ns):
+            tt = np.float32(t + np.float32(it) * iv)
+            for r in range(nr):
+                times.append(float(tt))
+                cells.append(nn + r)
+                vals.append(cur)
+    return np.array(times, np.float64), np.array(cells, np.int64), np.array(vals, np.float32)

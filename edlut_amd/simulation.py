@@ -119,7 +119,7 @@ class _Type:
 def gpu_model_name(model: NF.ModelDescription):
     """The GPU plug-in that runs a layer's model, or None for an input device."""
     name = model.model_name
-    if name in (NF.INPUT_SPIKE, NF.POISSON):
+    if name in (NF.INPUT_SPIKE, NF.POISSON, NF.INPUT_CURRENT):
         return None
     if name in REGISTRY:
         return name
@@ -130,7 +130,7 @@ def gpu_model_name(model: NF.ModelDescription):
                                  "GPU ones; use the model's _GPU plug-in name")
         return NF.CPU_TIME_DRIVEN[name]
     raise EDLUTException(C.EDLUT_E_ARG, f"neuron model {name!r} is not available (GPU time-driven models and their "
-                         "CPU names, InputSpikeNeuronModel, PoissonGeneratorDeviceVector)")
+                         "CPU names, InputSpikeNeuronModel, InputCurrentNeuronModel, PoissonGeneratorDeviceVector)")
 
 
 def _same_model(a: NF.ModelDescription, b: NF.ModelDescription) -> bool:
@@ -166,6 +166,10 @@ class Simulation_API:
         self.initialized = False
         self.simulation_properties = {"num_threads": 1, "num_simulation_queues": 1,
                                       "resolution": -1.0}
+        self._cur_changes: list[tuple] = []   # pending current-synapse changes (k, t, seq, type, j, value)
+        self._cur_value = {}         # per population: current of each current synapse (pA)
+        self._cur_seq = 0
+        self.current_log: list[tuple] = []    # (update k, population type, total currents) applied
         self._input_new_from = 0     # input spikes from here on are not uploaded yet
         self._trains_dirty = False
         self._input_times = np.zeros(0, np.float64)
@@ -250,7 +254,7 @@ class Simulation_API:
 
     @staticmethod
     def GetAvailableNeuronModels() -> list:
-        return sorted(REGISTRY) + [NF.INPUT_SPIKE, NF.POISSON]
+        return sorted(REGISTRY) + [NF.INPUT_SPIKE, NF.INPUT_CURRENT, NF.POISSON]
 
     @staticmethod
     def GetNeuronModelDefParams(model_name: str) -> dict:
@@ -258,8 +262,8 @@ class Simulation_API:
             return REGISTRY[model_name].GetDefaultParameters()
         if model_name == NF.POISSON:
             return {"frequency": 10.0, "name": NF.POISSON}
-        if model_name == NF.INPUT_SPIKE:
-            return {"name": NF.INPUT_SPIKE}
+        if model_name in (NF.INPUT_SPIKE, NF.INPUT_CURRENT):
+            return {"name": model_name}
         raise EDLUTException(C.EDLUT_E_ARG, f"Neuron model \"{model_name}\" does not exist")
 
     @staticmethod
@@ -276,7 +280,8 @@ class Simulation_API:
 
     @classmethod
     def from_files(cls, netfile: str, wfile: str, input_spike_file: str | None = None,
-                   base_dir: str | None = None, device: int = 0) -> "Simulation_API":
+                   base_dir: str | None = None, device: int = 0,
+                   input_current_file: str | None = None) -> "Simulation_API":
         """A simulation built from a network file and a weights file
         (Network::ParseNet; the weights file's random weights take one
         RandomGenerator), with the input spikes of a FileInputSpikeDriver file
@@ -297,6 +302,9 @@ class Simulation_API:
         if input_spike_file is not None:
             t, c = NF.parse_input_spikes(input_spike_file, sim.num_neuron_created)
             sim.AddExternalSpikeActivity(t, c)
+        if input_current_file is not None:
+            t, c, v = NF.parse_input_currents(input_current_file, sim.num_neuron_created)
+            sim.AddExternalCurrentActivity(t, c, v)
         return sim
 
     # ---- initialisation ---------------------------------------------------
@@ -361,19 +369,26 @@ class Simulation_API:
         ttype = plan.type_of[cat["target"]] if nsyn else np.zeros(0, np.int32)
         for q in np.nonzero(np.isin(ttype, [t.index for t in plan.types if t not in gpu]))[0][:1]:
             raise EDLUTException(C.EDLUT_E_ARG, f"synapse {q}: its target is an input device")
-        if nsyn and ((cat["type"] < 0) | (cat["type"] > 2)).any():
-            raise EDLUTException(C.EDLUT_E_ARG, "synapse types 0 (AMPA), 1 (GABA) and 2 (NMDA) take spikes; "
-                                 "current synapses (type 3) are not available in device-resident networks")
+        # CheckSynapseType (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:239-290):
+        # types 0 (AMPA), 1 (GABA) and 2 (NMDA) take spikes, type 3 currents
+        if nsyn and ((cat["type"] < 0) | (cat["type"] > 3)).any():
+            raise EDLUTException(C.EDLUT_E_ARG, "synapse types are 0 (AMPA), 1 (GABA), 2 (NMDA) and 3 (current)")
+        cur_src = np.array([plan.types[t].model.model_name == NF.INPUT_CURRENT for t in stype], bool)
+        is_cur = cat["type"] == 3 if nsyn else np.zeros(0, bool)
+        if nsyn and (is_cur != cur_src).any():
+            q = int(np.nonzero(is_cur != cur_src)[0][0])
+            raise EDLUTException(C.EDLUT_E_ARG, f"synapse {q}: type 3 takes currents from InputCurrentNeuronModel "
+                                 "neurons, types 0-2 take spikes")
         dround = round_propagation_delay(cat["delay"])
         dsteps = np.zeros(nsyn, np.int32)
-        for D in np.unique(dround):
+        for D in np.unique(dround[~is_cur]):
             d = ring_delay(float(D), fdt)
             if d < 1:
                 raise EDLUTException(C.EDLUT_E_ARG, f"a delay of {D:g} s is consumed by the update right after the "
                                      f"spike at step {fdt:g} s; the device networks need delays of at least two "
                                      "updates' spacing minus the step")
             dsteps[dround == D] = d
-        plan.max_delay = int(dsteps.max()) if nsyn else 1
+        plan.max_delay = max(int(dsteps.max()), 1) if nsyn else 1
         if plan.max_delay > 256:
             raise EDLUTException(C.EDLUT_E_ARG, "delays above 256 updates are not available")
         plan.syn_dsteps = dsteps
@@ -381,10 +396,19 @@ class Simulation_API:
         plan.syn = cat
         plan.syn_stype = stype
         plan.syn_ttype = ttype
-        self._syn_pos = np.zeros((nsyn, 3), np.int64)
+        self._syn_pos = np.full((nsyn, 3), -1, np.int64)
+        # current synapses stay on the host: per target population, in the
+        # order they were created, which is each target's CurrentSynapseModel
+        # subindex order (SetSubindexType in CheckSynapseType)
+        plan.current = {}
+        for t in gpu:
+            cq = np.nonzero((ttype == t.index) & is_cur)[0]
+            if len(cq):
+                plan.current[t.index] = dict(syn=cq, src=cat["source"][cq].astype(np.int64),
+                                             tgt=plan.local_of[cat["target"][cq]], delay=dround[cq])
         for t in gpu:
             srcs = []
-            onto = np.nonzero(ttype == t.index)[0]
+            onto = np.nonzero((ttype == t.index) & ~is_cur)[0]
             for st in sorted(set(stype[onto].tolist())):
                 sel = onto[stype[onto] == st]
                 groups = [("static", sel[cat["wchange"][sel] < 0])]
@@ -439,6 +463,8 @@ class Simulation_API:
             for s in plan.sources.get(t.index, []):
                 for ty in np.unique(s["edges"][3]):
                     m.CheckSynapseType(int(ty))
+            if t.index in plan.current:
+                m.CheckSynapseType(3, source_emits_spikes=False)
             m.InitializeStates(len(t.globals), 0, self.device)
             self.models[t.index] = m
             self.nets[t.index] = Network(m, plan.max_delay)
@@ -471,6 +497,8 @@ class Simulation_API:
                 pos = np.empty(len(order), np.int64)
                 pos[order] = np.arange(len(order))
                 self._syn_pos[s["syn"]] = np.stack([np.full(len(pos), t.index), np.full(len(pos), slot), pos], 1)
+        for tidx, cur in plan.current.items():
+            self._cur_value[tidx] = np.zeros(len(cur["syn"]), np.float32)
         if gpu:
             self.circuit = Circuit([self.nets[t.index] for t in gpu])
         self._log_read = {t.index: 0 for t in gpu}
@@ -494,6 +522,58 @@ class Simulation_API:
         self._input_times = np.concatenate([self._input_times, t])
         self._input_cells = np.concatenate([self._input_cells, c])
         self._trains_dirty = True
+
+    def AddExternalCurrentActivity(self, event_time_list, neuron_index_list, current_value_list) -> None:
+        """Simulation_API.cpp:155-171: currents (pA) of InputCurrentNeuronModel
+        neurons from the given times on.  Each reaches the populations through
+        its current synapses (type 3) after their delay: from the first update
+        after the arrival, the synapse carries the new current, and a target's
+        external current is the float sum of its current synapses' currents in
+        the order they were created (CurrentSynapseModel::GetTotalCurrent,
+        CurrentSynapseModel.cpp:76-82; ProcessInputCurrent,
+        LIFTimeDrivenModel_GPU_C_INTERFACE.cu:72-79)."""
+        self._after_init("AddExternalCurrentActivity")
+        t = np.asarray(event_time_list, np.float64).ravel()
+        c = np.asarray(neuron_index_list, np.int64).ravel()
+        v = np.asarray(current_value_list, np.float32).ravel()
+        if not (len(t) == len(c) == len(v)):
+            raise EDLUTException(C.EDLUT_E_ARG, "time, neuron and current lists differ in length")
+        if len(c) and (c.min() < 0 or c.max() >= self.num_neuron_created):
+            raise EDLUTException(C.EDLUT_E_ARG, "input current neuron index out of range")
+        plan = self.plan
+        for q in range(len(c)):
+            if plan.types[plan.type_of[c[q]]].model.model_name != NF.INPUT_CURRENT:
+                raise EDLUTException(C.EDLUT_E_ARG, f"neuron {c[q]} is not an InputCurrentNeuronModel neuron")
+        if len(t) == 0:
+            return
+        fdt = float(np.float32(plan.dt))
+        dmax = max((float(cur["delay"].max()) for cur in plan.current.values()), default=0.0)
+        grid = update_times(fdt, int(np.ceil((t.max() + dmax) / fdt)) + 8)
+        new = []
+        for q in range(len(t)):
+            for tidx, cur in plan.current.items():
+                for j in np.nonzero(cur["src"] == c[q])[0]:
+                    k = int(np.searchsorted(grid, t[q] + float(cur["delay"][j]), side="right"))
+                    if k <= self.current_step:
+                        raise EDLUTException(C.EDLUT_E_ARG, f"input current at {t[q]!r} s of neuron {c[q]} reaches "
+                                             "its target in updates already simulated")
+                    new.append((k, float(t[q]), self._cur_seq, tidx, int(j), np.float32(v[q])))
+                    self._cur_seq += 1
+        self._cur_changes += new
+
+    def _apply_currents(self, k: int, changes) -> None:
+        touched = set()
+        for (_, _, _, tidx, j, value) in changes:
+            self._cur_value[tidx][j] = value
+            touched.add(tidx)
+        for tidx in sorted(touched):
+            cur = self.plan.current[tidx]
+            total = np.zeros(len(self.plan.types[tidx].globals), np.float32)
+            vals = self._cur_value[tidx]
+            for j in range(len(cur["syn"])):
+                total[cur["tgt"][j]] = np.float32(total[cur["tgt"][j]] + vals[j])
+            self.models[tidx].ProcessInputCurrent(total)
+            self.current_log.append((k, tidx, total.copy()))
 
     def place_input_spikes(self, times, cells):
         """Update step of each input spike (see the module notes): 0 for a
@@ -567,6 +647,15 @@ class Simulation_API:
         k_end = int(np.searchsorted(grid, end_simulation_time, side="right")) - 1
         if self._trains_dirty:
             self._upload_trains()
+        # current changes split the run: updates before k, then the new
+        # currents, then update k onward
+        due = sorted(x for x in self._cur_changes if x[0] <= k_end)
+        self._cur_changes = [x for x in self._cur_changes if x[0] > k_end]
+        for k in sorted({x[0] for x in due}):
+            if k - 1 > self.current_step and self.circuit is not None:
+                self.circuit.run(k - 1 - self.current_step, first_step=self.current_step + 1)
+                self.current_step = k - 1
+            self._apply_currents(k, [x for x in due if x[0] == k])
         n = k_end - self.current_step
         if n > 0 and self.circuit is not None:
             self.circuit.run(n, first_step=self.current_step + 1)
@@ -624,6 +713,9 @@ class Simulation_API:
         cache = {}
         for q in range(n):
             tidx, slot, pos = self._syn_pos[q]
+            if tidx < 0:
+                w[q] = self.plan.syn["weight"][q]      # current synapse (host side)
+                continue
             key = (int(tidx), int(slot))
             if key not in cache:
                 cache[key] = self.nets[key[0]].synapses(key[1])[2]

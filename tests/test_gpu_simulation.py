@@ -41,7 +41,7 @@ def _oracle(oracle, sim):
             continue
         im = t.model.param_map["int_meth"]
         params = np.array([t.model.param_map.get(k, v) for k, v in REGISTRY[name].PARAMS], np.float32)
-        mask = 0
+        mask = 8 if t.index in plan.current else 0
         for s in plan.sources.get(t.index, []):
             for ty in np.unique(s["edges"][3]):
                 mask |= 1 << int(ty)
@@ -72,7 +72,15 @@ def _oracle(oracle, sim):
 
 def _compare(oracle, sim, n_steps):
     pops, nets, order = _oracle(oracle, sim)
-    oracle.circuit_run([nets[i] for i in order], 1, n_steps)
+    # the external currents the front-end set before update k, replayed
+    done = 0
+    for k, tidx, total in sim.current_log:
+        if k - 1 > done:
+            oracle.circuit_run([nets[i] for i in order], done + 1, k - 1 - done)
+            done = k - 1
+        pops[tidx].set_ext_current(total)
+    if n_steps > done:
+        oracle.circuit_run([nets[i] for i in order], done + 1, n_steps - done)
     total = 0
     for ti in order:
         gs, gi = sim.nets[ti].spikes()
@@ -192,4 +200,37 @@ def test_from_files_reference_inputs(oracle, tmp_path):
     n = _compare(oracle, sim, sim.current_step)
     assert n > 0
     assert sim.GetWeights() == [np.float32(0.9)] * 2 + [0.5] * 2 + [np.float32(0.7)] * 2
+    sim.close()
+
+
+def test_reference_example_with_currents(oracle, tmp_path):
+    """The reference's example network (Kernel/utils/edlut_input_files:
+    three input neurons on AMPA, GABA and NMDA synapses of one neuron, an
+    input-current neuron on a current synapse of another) with GPU LIF
+    neurons at a 0.1 ms step, driven by the reference's Input_spikes.cfg and
+    Input_currents.cfg.  Currents change only between updates: the run is
+    split where they reach the population."""
+    (tmp_path / "lif.cfg").write_text("0.0\n-80.0\n-65.0\n-50.0\n2.0\n1.0\n10.0\n1.0\n1.0\n14.0\nEuler\n0.0001\n")
+    net = tmp_path / "net.cfg"
+    net.write_text("3\n6\n3 InputSpikeNeuronModel Null 0 0\n1 InputCurrentNeuronModel Null 0 0\n"
+                   "2 LIFTimeDrivenModel_GPU lif 1 0\n0\n4\n0 1 4 1 1 0.001 0 0 1.0 -1\n1 1 4 1 1 0.001 0 1 1.0 -1\n"
+                   "2 1 4 1 1 0.001 0 2 1.0 -1\n3 1 5 1 1 0.001 0 3 1.0 -1\n")
+    w = tmp_path / "w.cfg"
+    w.write_text("3 1.5\n1 1.0\n")
+    S.RandomGenerator.srand(1)
+    sim = S.Simulation_API.from_files(str(net), str(w), os.path.join(GOLD, "Input_spikes.cfg"), base_dir=str(tmp_path),
+                                      input_current_file=os.path.join(GOLD, "Input_currents.cfg"))
+    sim.RunSimulation(0.3)
+    sim.RunSimulation(1.25)
+    # 20 current changes reach neuron 5 at the first update after t + 1 ms
+    grid = S.update_times(float(np.float32(1e-4)), sim.current_step)
+    t, _, v = S.NF.parse_input_currents(os.path.join(GOLD, "Input_currents.cfg"), 6)
+    D = float(S.round_propagation_delay(np.float32(0.001)))
+    want = sorted({int(np.searchsorted(grid, x + D, side="right")) for x in t})
+    assert [k for k, _, _ in sim.current_log] == want
+    assert all(tot[1] in (np.float32(300.0), np.float32(0.0)) and tot[0] == 0.0 for _, _, tot in sim.current_log)
+    n = _compare(oracle, sim, sim.current_step)
+    st, ix = sim.nets[2].spikes()
+    assert (ix == 1).sum() > 20, "the 300 pA steps should drive neuron 5"
+    assert n > 0
     sim.close()

@@ -226,7 +226,7 @@ def test_plan_refusals():
     sim = S.Simulation_API()
     a = sim.AddNeuronLayer(2, "LIFTimeDrivenModel_GPU", {"int_meth": {"name": "Euler", "step": 1e-4}})
     sim.AddSynapticLayer(a, a[::-1], {"type": 3})
-    with pytest.raises(EDLUTException, match="current synapses"):
+    with pytest.raises(EDLUTException, match="type 3 takes currents"):
         sim.build_plan()
     sim = S.Simulation_API()
     sim.AddNeuronLayer(2, "LIFTimeDrivenModel", {"int_meth": {"name": "Bifixed_RK2", "step": 1e-3, "n_steps": 3}})
@@ -256,3 +256,43 @@ def test_input_spike_placement_brute_force():
     assert sim.place_input_spikes([0.01], [2]).tolist() == [0]
     with pytest.raises(EDLUTException, match="before the first update"):
         sim.place_input_spikes([0.00001], [0])
+
+
+def test_reference_example_input_currents():
+    """FileInputCurrentDriver format (FileInputCurrentDriver.cpp:45-90) on the
+    reference's own example: 300 pA on neuron 3 every 50 ms from 10 ms, back
+    to 0 pA every 50 ms from 60 ms."""
+    t, c, v = NF.parse_input_currents(os.path.join(GOLD, "Input_currents.cfg"), 6)
+    assert len(t) == 20 and (c == 3).all()
+    j = np.arange(10, dtype=np.float32)
+    assert np.array_equal(t[:10], (np.float32(0.010) + j * np.float32(0.050)).astype(np.float64))
+    assert np.array_equal(t[10:], (np.float32(0.060) + j * np.float32(0.050)).astype(np.float64))
+    assert (v[:10] == np.float32(300.0)).all() and (v[10:] == 0.0).all()
+
+
+def test_current_synapses_need_current_sources():
+    sim = S.Simulation_API()
+    inp = sim.AddNeuronLayer(1, "InputSpikeNeuronModel")
+    cur = sim.AddNeuronLayer(1, "InputCurrentNeuronModel")
+    lif = sim.AddNeuronLayer(2, "LIFTimeDrivenModel_GPU", {"int_meth": {"name": "Euler", "step": 1e-4}})
+    sim.AddSynapticLayer(inp, lif[:1], {"type": 3})
+    with pytest.raises(EDLUTException, match="type 3 takes currents"):
+        sim.build_plan()
+    sim = S.Simulation_API()
+    inp = sim.AddNeuronLayer(1, "InputSpikeNeuronModel")
+    cur = sim.AddNeuronLayer(1, "InputCurrentNeuronModel")
+    lif = sim.AddNeuronLayer(2, "LIFTimeDrivenModel_GPU", {"int_meth": {"name": "Euler", "step": 1e-4}})
+    sim.AddSynapticLayer(cur, lif[:1], {"type": 0})
+    with pytest.raises(EDLUTException, match="type 3 takes currents"):
+        sim.build_plan()
+    # accepted: currents on type 3, kept on the host in creation order per target
+    sim = S.Simulation_API()
+    inp = sim.AddNeuronLayer(1, "InputSpikeNeuronModel")
+    cur = sim.AddNeuronLayer(2, "InputCurrentNeuronModel")
+    lif = sim.AddNeuronLayer(2, "LIFTimeDrivenModel_GPU", {"int_meth": {"name": "Euler", "step": 1e-4}})
+    sim.AddSynapticLayer(inp, lif[:1], {"type": 0})
+    sim.AddSynapticLayer([cur[1], cur[0], cur[1]], [lif[1], lif[1], lif[0]], {"type": 3, "delay": 0.0005})
+    plan = sim.build_plan()
+    c = plan.current[2]
+    assert c["syn"].tolist() == [1, 2, 3] and c["src"].tolist() == [2, 1, 2] and c["tgt"].tolist() == [1, 1, 0]
+    assert [s["kind"] for s in plan.sources[2]] == ["train"]
