@@ -321,7 +321,9 @@ def main():
     model, net = E.from_spec(spec)
     print(f"setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
-    items, lanes, blocks, tune_ms = net.autotune(64)   # launch shape (the network is reset after)
+    # launch shape (the network is reset after); 256 steps per trial so that
+    # the trials see the network's recurrent activity, not only its quiet start
+    items, lanes, blocks, tune_ms = net.autotune(256)
     print(f"autotune -> items {items} lanes {lanes} blocks {blocks} ({tune_ms:.4f} ms/step)",
           file=sys.stderr, flush=True)
     print("timed region", file=sys.stderr, flush=True)
