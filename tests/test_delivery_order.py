@@ -103,3 +103,39 @@ def test_large_groups_split_at_1024_sources(oracle):
             assert a[n_last:] + a[:n_last] == b, k
             permuted += 1
     assert permuted >= 1, "the heap model should show at least one reordered split"
+
+
+def test_spike_train_source_delivers_like_the_poisson_device(oracle):
+    """The oracle's spike-train source (input neurons placed on updates):
+    fed the raster of a Poisson device, it applies the same increments in
+    the same order as that device (itself checked against the event-queue
+    model above); a lone spike at update s through a delay of d updates
+    arrives at update s + d + 1."""
+    rng = np.random.default_rng(11)
+    n_src, n_tgt, steps = 40, 30, 300
+    states, edges = make_source(rng, n_src, n_tgt, 6, 9, seed=4)
+    want = oracle_trace(oracle, n_tgt, [(states, 120.0, edges)], steps)
+    raster = lcg_raster(states, 120.0, steps)
+    ptr = np.cumsum([0] + [len(raster[k]) for k in range(1, steps + 1)]).astype(np.int64)
+    idx = np.array([i for k in range(1, steps + 1) for i in raster[k]], np.int32)
+    pop = oracle.Population("LIFTimeDrivenModel_GPU", "Euler", LIF_DEFAULTS, n_tgt, mask=7)
+    net = oracle.Net(pop, 16)
+    net.add_train(n_src, ptr, idx, *np_out_csr(n_src, n_tgt, 16, *edges))
+    net.set_trace(True)
+    net.run(1, steps)
+    st, tg, ty, w = net.trace()
+    got = {k: list(zip(tg[st == k].tolist(), ty[st == k].tolist(), w[st == k].astype(float).tolist()))
+           for k in range(1, steps + 1)}
+    assert sum(len(v) for v in want.values()) > 100
+    compare(got, want, steps)
+    # a lone spike
+    pop = oracle.Population("LIFTimeDrivenModel_GPU", "Euler", LIF_DEFAULTS, 2, mask=7)
+    net = oracle.Net(pop, 16)
+    ptr = np.zeros(8, np.int64)
+    ptr[5:] = 1                                   # source 0 fires at update 5
+    net.add_train(1, ptr, np.array([0], np.int32),
+                  *np_out_csr(1, 2, 16, [0], [1], [3], np.array([1], np.uint8), np.array([0.25], np.float32)))
+    net.set_trace(True)
+    net.run(1, 20)
+    st, tg, ty, w = net.trace()
+    assert st.tolist() == [5 + 3 + 1] and tg.tolist() == [1] and ty.tolist() == [1] and w.tolist() == [0.25]
