@@ -133,8 +133,26 @@ def gpu_model_name(model: NF.ModelDescription):
                          "CPU names, InputSpikeNeuronModel, InputCurrentNeuronModel, PoissonGeneratorDeviceVector)")
 
 
+def _canonical(model: NF.ModelDescription) -> tuple:
+    """The model's parameters with its defaults filled in (float parameters
+    as floats, the integration method as integration_method builds it), so
+    that layers compare like the reference's NeuronModel::compare."""
+    name = model.model_name
+    pm = model.param_map
+    g = gpu_model_name(model)
+    if g is None:
+        if name == NF.POISSON:
+            return (name, float(np.float32(pm.get("frequency", 10.0))))
+        return (name,)
+    vals = tuple(float(np.float32(pm.get(k, v))) for k, v in REGISTRY[g].PARAMS)
+    im = dict(pm.get("int_meth", integration_method()))
+    imc = integration_method(im.get("name", "Euler"), float(np.float32(im.get("step", 1e-3))),
+                             int(im.get("bdf_order", 1)), int(im.get("n_steps", 2)))
+    return (name, vals, tuple(sorted(imc.items())))
+
+
 def _same_model(a: NF.ModelDescription, b: NF.ModelDescription) -> bool:
-    return a.model_name == b.model_name and a.param_map == b.param_map
+    return _canonical(a) == _canonical(b)
 
 
 class SimulationPlan:

@@ -296,3 +296,17 @@ def test_current_synapses_need_current_sources():
     c = plan.current[2]
     assert c["syn"].tolist() == [1, 2, 3] and c["src"].tolist() == [2, 1, 2] and c["tgt"].tolist() == [1, 1, 0]
     assert [s["kind"] for s in plan.sources[2]] == ["train"]
+
+
+def test_layers_with_equal_parameters_share_a_population():
+    """Network::CreateNeuronLayers merges layers whose models compare equal
+    (NeuronModel::compare): defaults given explicitly or left out are the
+    same model; another value or another step is not."""
+    sim = S.Simulation_API()
+    im = {"int_meth": {"name": "RK4", "step": 1e-4}}
+    a = sim.AddNeuronLayer(2, "AdExTimeDrivenModel_GPU", dict(im))
+    b = sim.AddNeuronLayer(2, "AdExTimeDrivenModel_GPU", dict(im, a=1.0, b=9.0))
+    c = sim.AddNeuronLayer(2, "AdExTimeDrivenModel_GPU", dict(im, a=2.0))
+    d = sim.AddNeuronLayer(2, "AdExTimeDrivenModel_GPU", {"int_meth": {"name": "RK4", "step": 1e-4, "bdf_order": 1}})
+    plan = sim.build_plan()
+    assert plan.type_of[a + b + c + d].tolist() == [0, 0, 0, 0, 1, 1, 0, 0]
