@@ -310,3 +310,34 @@ def test_layers_with_equal_parameters_share_a_population():
     d = sim.AddNeuronLayer(2, "AdExTimeDrivenModel_GPU", {"int_meth": {"name": "RK4", "step": 1e-4, "bdf_order": 1}})
     plan = sim.build_plan()
     assert plan.type_of[a + b + c + d].tolist() == [0, 0, 0, 0, 1, 1, 0, 0]
+
+
+def test_simulation_api_call_order_and_queries():
+    """Simulation_API's call-order checks and registry queries
+    (Simulation_API.cpp:63-443): construction calls before Initialize, run
+    calls after it; unknown simulation parameters and learning rules raise."""
+    sim = S.Simulation_API()
+    with pytest.raises(EDLUTException, match="after Initialize"):
+        sim.RunSimulation(0.1)
+    with pytest.raises(EDLUTException, match="after Initialize"):
+        sim.AddExternalSpikeActivity([0.01], [0])
+    with pytest.raises(EDLUTException, match="after Initialize"):
+        sim.GetSpikeActivity()
+    with pytest.raises(EDLUTException, match="invalid simulation parameter"):
+        sim.SetSimulationParameters({"no_such_parameter": 1})
+    sim.SetSimulationParameters({"num_threads": 2})
+    assert sim.simulation_properties["num_threads"] == 2
+    assert sim.AddNeuronLayer(3, "InputSpikeNeuronModel") == [0, 1, 2]
+    assert sim.AddNeuronLayer(2, "LIFTimeDrivenModel_GPU") == [3, 4]
+    assert sim.AddSynapticLayer([0, 1], [3, 4], {"weight": [0.5, 0.25]}) == [0, 1]
+    assert sim.AddSynapticLayer([2], [4], {}) == [2]
+    with pytest.raises(EDLUTException, match="unknown synapse parameters"):
+        sim.AddSynapticLayer([2], [3], {"colour": 1})
+    models = S.Simulation_API.GetAvailableNeuronModels()
+    assert "AdExTimeDrivenModel_GPU" in models and "InputCurrentNeuronModel" in models
+    assert S.Simulation_API.GetNeuronModelDefParams("PoissonGeneratorDeviceVector")["frequency"] == 10.0
+    assert S.Simulation_API.GetNeuronModelDefParams("LIFTimeDrivenModel_GPU")["tau_ref"] == 1.0
+    assert S.Simulation_API.GetLearningRuleDefParams("STDP")["tau_LTP"] == 0.100
+    assert S.Simulation_API.GetIntegrationMethodDefParams("Bifixed_RK4")["n_steps"] == 2
+    with pytest.raises(EDLUTException):
+        S.Simulation_API.GetNeuronModelDefParams("NoSuchModel")
