@@ -50,6 +50,8 @@ EXPORTS = (
     "edlut_network_synchronize",
     "edlut_network_spike_log",
     "edlut_network_clear_log",
+    "edlut_network_set_source_log",
+    "edlut_network_source_spike_log",
     "edlut_network_set_log_capacity",
     "edlut_network_spike_count",
     "edlut_network_set_timing",
@@ -135,6 +137,8 @@ def _sig(lib):
         "edlut_network_synchronize": (I32, [P]),
         "edlut_network_spike_log": (I32, [P, P, P, I64, P]),
         "edlut_network_clear_log": (I32, [P]),
+        "edlut_network_set_source_log": (I32, [P, I32, I64]),
+        "edlut_network_source_spike_log": (I32, [P, I32, P, P, I64, P]),
         "edlut_network_set_log_capacity": (I32, [P, I64]),
         "edlut_network_spike_count": (I32, [P, P]),
         "edlut_network_set_timing": (I32, [P, I64]),

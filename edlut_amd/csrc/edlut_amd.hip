@@ -397,6 +397,12 @@ struct Source {
     // spike train (edlut_network_add_spike_train): spikes of update k are
     // train_idx[train_ptr[k-1] .. train_ptr[k])
     bool train = false;
+    // optional spike log of a Poisson device (edlut_network_set_source_log),
+    // counted like the network's own log
+    int32_t *log_step = nullptr;
+    int32_t *log_idx = nullptr;
+    int64_t *log_count = nullptr;
+    int64_t log_cap = 0;
     int64_t *train_ptr = nullptr;
     int32_t *train_idx = nullptr;
     int64_t train_steps = 0;
@@ -1497,7 +1503,8 @@ int edlut_network_destroy(edlut_network *n) {
         const bool own_ring = !s.self && !s.from;
         void *dev[] = {own_ring ? s.ring_idx : nullptr, own_ring ? s.ring_cnt : nullptr, s.row_ptr,
                        s.tgt_type, s.w, s.lcg, s.tile_count, s.staging, s.syn.pre, s.syn.post,
-                       s.syn.last, s.in_ptr, s.in_syn, s.rates, s.train_ptr, s.train_idx};
+                       s.syn.last, s.in_ptr, s.in_syn, s.rates, s.train_ptr, s.train_idx,
+                       s.log_step, s.log_idx, s.log_count};
         for (void *q : dev)
             if (q) (void)hipFree(q);
         if (s.from) s.from->links_out--;
@@ -1767,7 +1774,7 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         pa.out = TileOut{s.tile_count, s.staging};
         hipLaunchKernelGGL(k_poisson, dim3(pa.n_tiles), dim3(kBlock), 0, p->stream, pa);
         launch_compact(p->stream, s.tile_count, s.staging, pa.n_tiles, kBlock, s.ring_idx, s.ring_cnt,
-                       n->ring, s.n, nullptr, nullptr, nullptr, 0, p->err, step);
+                       n->ring, s.n, s.log_step, s.log_idx, s.log_count, s.log_cap, p->err, step);
     }
     // Deliver the events consumed by update k into their keys' segments.
     FoldTarget ft = p->fold_target(0, true);
@@ -1944,6 +1951,13 @@ static int run_prepare(edlut_network *n, int64_t first_step, int n_steps, Update
         lc[0] = lc[1] = total;
         HIP_TRY(hipMemcpyAsync(n->log_count, lc, sizeof(lc), hipMemcpyHostToDevice, p->stream));
         HIP_TRY(hipStreamSynchronize(p->stream));
+        for (Source &s : n->sources) {
+            if (!s.log_count) continue;
+            int64_t sc[2] = {0, 0};
+            HIP_TRY(hipMemcpy(sc, s.log_count, sizeof(sc), hipMemcpyDeviceToHost));
+            sc[0] = sc[1] = sc[(n->last_step + 1) & 1];
+            HIP_TRY(hipMemcpy(s.log_count, sc, sizeof(sc), hipMemcpyHostToDevice));
+        }
     }
     HIP_TRY(hipMemsetAsync(p->bump, 0, 2 * sizeof(uint64_t), p->stream));
     if (n->plastic) {
@@ -2227,6 +2241,54 @@ int edlut_network_spike_log(edlut_network *n, int32_t *step, int32_t *idx, int64
     if (cnt > 0) {
         HIP_TRY(hipMemcpy(step, n->log_step, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
         HIP_TRY(hipMemcpy(idx, n->log_idx, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+    }
+    return 0;
+}
+
+int edlut_network_set_source_log(edlut_network *n, int source, int64_t cap) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    Source &s = n->sources[(size_t)source];
+    if (s.self || s.from || s.train) return fail(EDLUT_E_ARG, "source %d is not a Poisson device", source);
+    if (cap < 0 || cap > (1ll << 31)) return fail(EDLUT_E_ARG, "log capacity out of range");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    void *old[] = {s.log_step, s.log_idx, s.log_count};
+    for (void *q : old)
+        if (q) (void)hipFree(q);
+    s.log_step = s.log_idx = nullptr;
+    s.log_count = nullptr;
+    s.log_cap = 0;
+    if (cap > 0) {
+        int rc;
+        if ((rc = dalloc(&s.log_step, (size_t)cap)) || (rc = dalloc(&s.log_idx, (size_t)cap)) ||
+            (rc = dalloc(&s.log_count, 2)))
+            return rc;
+        HIP_TRY(hipMemset(s.log_count, 0, 2 * sizeof(int64_t)));
+        s.log_cap = cap;
+    }
+    n->dirty = true;   // captured kernels carry the log
+    return 0;
+}
+
+int edlut_network_source_spike_log(edlut_network *n, int source, int32_t *step, int32_t *idx, int64_t cap,
+                                   int64_t *n_out) {
+    if (!n || !n_out) return fail(EDLUT_E_ARG, "null argument");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    const Source &s = n->sources[(size_t)source];
+    if (!s.log_count) return fail(EDLUT_E_STATE, "source %d keeps no spike log", source);
+    int rc = edlut_network_synchronize(n);
+    if (rc) return rc;
+    int64_t lc[2] = {0, 0};
+    HIP_TRY(hipMemcpy(lc, s.log_count, sizeof(lc), hipMemcpyDeviceToHost));
+    int64_t cnt = lc[(n->last_step + 1) & 1];
+    if (cnt > s.log_cap) return fail(EDLUT_E_CAP, "source %d produced %lld spikes, its log holds %lld", source,
+                                     (long long)cnt, (long long)s.log_cap);
+    *n_out = cnt;
+    if (cnt > cap) return fail(EDLUT_E_CAP, "spike log has %lld entries, buffer holds %lld", (long long)cnt, (long long)cap);
+    if (cnt > 0) {
+        HIP_TRY(hipMemcpy(step, s.log_step, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(idx, s.log_idx, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost));
     }
     return 0;
 }

@@ -231,6 +231,24 @@ class Network:
         C.check(C.lib().edlut_network_spike_log(self._h, C.ptr(st), C.ptr(ix), m, C.byref(n)))
         return st[: n.value].copy(), ix[: n.value].copy()
 
+    def set_source_log(self, source: int, cap: int):
+        """Keep a spike log of Poisson source `source` (cap entries; 0 drops it)."""
+        C.check(C.lib().edlut_network_set_source_log(self._h, int(source), int(cap)))
+
+    def source_spikes(self, source: int):
+        """(step, generator index) arrays of Poisson source `source`'s logged spikes."""
+        cnt = ctypes.c_int64(0)
+        rc = C.lib().edlut_network_source_spike_log(self._h, int(source), None, None, 0, C.byref(cnt))
+        if rc not in (0, C.EDLUT_E_CAP):
+            C.check(rc)
+        m = cnt.value
+        st = np.zeros(max(m, 1), np.int32)
+        ix = np.zeros(max(m, 1), np.int32)
+        n = ctypes.c_int64(0)
+        C.check(C.lib().edlut_network_source_spike_log(self._h, int(source), C.ptr(st), C.ptr(ix), m,
+                                                        C.byref(n)))
+        return st[: n.value].copy(), ix[: n.value].copy()
+
     def set_log_capacity(self, cap: int):
         """Spike log capacity in entries; 0 keeps only the spike count."""
         C.check(C.lib().edlut_network_set_log_capacity(self._h, int(cap)))

@@ -52,6 +52,7 @@ from .network import Circuit, Network, build_out_csr
 MIN_PROPAGATION_DELAY = 0.0001   # Network::minpropagationdelay
 MAX_SOURCES = 4                  # sources per population in the library
 MAX_RAND = np.float32(2147483647)
+POISSON_LOG_CAP = 1 << 22        # logged spikes per Poisson layer with output neurons
 
 
 class RandomGenerator:
@@ -517,9 +518,22 @@ class Simulation_API:
                 self._syn_pos[s["syn"]] = np.stack([np.full(len(pos), t.index), np.full(len(pos), slot), pos], 1)
         for tidx, cur in plan.current.items():
             self._cur_value[tidx] = np.zeros(len(cur["syn"]), np.float32)
+        # Poisson layers with output neurons: one of their device copies logs
+        # the generators' spikes
+        self._poisson_log = {}
+        for t in plan.types:
+            if t.model.model_name != NF.POISSON or not t.output.any():
+                continue
+            for tidx, srcs in plan.sources.items():
+                hit = [s2 for s2 in srcs if s2["kind"] == "poisson" and s2["src_type"] == t.index]
+                if hit:
+                    self.nets[tidx].set_source_log(hit[0]["slot"], POISSON_LOG_CAP)
+                    self._poisson_log[t.index] = (tidx, hit[0]["slot"])
+                    self._log_read[("poisson", t.index)] = 0
+                    break
         if gpu:
             self.circuit = Circuit([self.nets[t.index] for t in gpu])
-        self._log_read = {t.index: 0 for t in gpu}
+        self._log_read.update({t.index: 0 for t in gpu})
         self.initialized = True
 
     # ---- run --------------------------------------------------------------
@@ -703,6 +717,17 @@ class Simulation_API:
             keep = T.output[ix]
             g = np.asarray(T.globals, np.int64)[ix[keep]]
             rows.append((grid[st[keep]], np.full(len(g), tidx), ix[keep], g))
+        # Poisson layers with output neurons: their generators' spikes
+        for ptype, (tidx, slot) in self._poisson_log.items():
+            st, ix = self.nets[tidx].source_spikes(slot)
+            key = ("poisson", ptype)
+            new = slice(self._log_read[key], len(st))
+            self._log_read[key] = len(st)
+            st, ix = st[new], ix[new]
+            T = plan.types[ptype]
+            keep = T.output[ix]
+            rows.append((grid[st[keep]], np.full(int(keep.sum()), ptype), ix[keep],
+                         np.asarray(T.globals, np.int64)[ix[keep]]))
         # input neurons with output activity: their spikes up to the current time
         if len(self._input_times):
             prev = getattr(self, "_in_cursor", -np.inf)

@@ -265,6 +265,14 @@ int edlut_network_synchronize(edlut_network *net);
 int edlut_network_spike_log(edlut_network *net, int32_t *step, int32_t *idx, int64_t cap,
                             int64_t *n_out);
 int edlut_network_clear_log(edlut_network *net);
+/* Spike log of Poisson device `source` (cap entries, 0 to drop it): the
+ * generators that fired and when, for output of a PoissonGeneratorDeviceVector
+ * layer (the reference hands its spikes to the output drivers like any
+ * neuron's, TimeDrivenInternalSpike.cpp:66-104).  Read with
+ * edlut_network_source_spike_log (step, index), E_CAP when it overflowed. */
+int edlut_network_set_source_log(edlut_network *net, int source, int64_t cap);
+int edlut_network_source_spike_log(edlut_network *net, int source, int32_t *step, int32_t *idx,
+                                   int64_t cap, int64_t *n_out);
 /* Spike log capacity in entries (default 2^26); 0 keeps only the count.
  * Clears the log. */
 int edlut_network_set_log_capacity(edlut_network *net, int64_t cap);

@@ -102,7 +102,7 @@ def _driven_sim(seed=7):
     sim = S.Simulation_API()
     rk4 = {"int_meth": {"name": "RK4", "step": 1e-4, "bdf_order": 1}}
     inp = sim.AddNeuronLayer(4, "InputSpikeNeuronModel", output_activity=True)
-    poi = sim.AddNeuronLayer(30, "PoissonGeneratorDeviceVector", {"frequency": 150.0})
+    poi = sim.AddNeuronLayer(30, "PoissonGeneratorDeviceVector", {"frequency": 150.0}, output_activity=True)
     adex = sim.AddNeuronLayer(40, "AdExTimeDrivenModel_GPU", rk4, output_activity=True)
     lif = sim.AddNeuronLayer(12, "LIFTimeDrivenModel_GPU", {"int_meth": {"name": "Euler", "step": 1e-4}})
     adex2 = sim.AddNeuronLayer(10, "AdExTimeDrivenModel_GPU", rk4, output_activity=True)
@@ -120,7 +120,7 @@ def _driven_sim(seed=7):
                          {"delay": 0.0005, "weight": 0.5, "wchange": r, "max_weight": 2.0})
     sim.AddSynapticLayer(np.array(lif)[rng.integers(0, 12, 60)], np.array(adex2)[rng.integers(0, 10, 60)],
                          {"delay": 0.0002, "type": 1, "weight": 3.0})
-    return sim, inp
+    return sim, inp, poi
 
 
 def _input_spikes(seed, n, lo, hi, first):
@@ -136,7 +136,7 @@ def _input_spikes(seed, n, lo, hi, first):
 
 
 def test_simulation_api_network_bit_exact(oracle):
-    sim, inp = _driven_sim()
+    sim, inp, poi = _driven_sim()
     sim.Initialize()
     t, c = _input_spikes(1, 80, 0.0005, 0.15, inp[0])
     sim.AddExternalSpikeActivity(t, c)
@@ -154,6 +154,18 @@ def test_simulation_api_network_bit_exact(oracle):
         st, ix = sim.nets[ti].spikes()
         T = sim.plan.types[ti]
         want += [(float(grid[s]), int(T.globals[i])) for s, i in zip(st, ix) if T.output[i]]
+    # the Poisson layer's generators, replayed on the host from their seeds
+    # (RandomGenerator LCG; fire when frequency * step > frand())
+    x = sim.plan.poisson_states[1].astype(np.uint64)
+    p = float(np.float32(150.0)) * float(np.float32(1e-4))
+    n_poi = 0
+    for k in range(1, sim.current_step + 1):
+        x = (x * np.uint64(1103515245) + np.uint64(12345)) & np.uint64(0x7FFFFFFF)
+        u = x.astype(np.float32) / np.float32(2147483647.0)
+        for i in np.nonzero(p > u.astype(np.float64))[0]:
+            want.append((float(grid[k]), int(poi[0] + i)))
+            n_poi += 1
+    assert n_poi > 100
     assert sorted(want) == sorted(zip(times, cells))
     assert sim.GetSpikeActivity() == ([], [])      # the buffer was emptied
     w = sim.GetWeights()
@@ -164,7 +176,7 @@ def test_simulation_api_network_bit_exact(oracle):
 def test_external_spikes_between_runs(oracle):
     """Spikes added between runs (the closed-loop use of
     AddExternalSpikeActivity) give the run that had them from the start."""
-    sim, inp = _driven_sim(seed=11)
+    sim, inp, _ = _driven_sim(seed=11)
     sim.Initialize()
     t1, c1 = _input_spikes(5, 30, 0.001, 0.05, inp[0])
     t2, c2 = _input_spikes(6, 30, 0.06, 0.1, inp[0])
