@@ -108,6 +108,19 @@ def ring_delay(delay_rounded: float, fdt: float) -> int:
     return m - 1
 
 
+def compress_weights(weights):
+    """Run-length encoding of consecutive equal float weights, as
+    Network::GetCompressedWeights builds it (Network.cpp:1204-1231)."""
+    counts, values = [], []
+    for w in np.asarray(weights, np.float32).tolist():
+        if values and values[-1] == w:
+            counts[-1] += 1
+        else:
+            counts.append(1)
+            values.append(w)
+    return counts, values
+
+
 class _Type:
     def __init__(self, model: NF.ModelDescription, index: int):
         self.model = model
@@ -764,6 +777,12 @@ class Simulation_API:
                 cache[key] = self.nets[key[0]].synapses(key[1])[2]
             w[q] = cache[key][pos]
         return w.tolist()
+
+    def GetCompressedWeights(self):
+        """Simulation_API.cpp:313-324 / Network::GetCompressedWeights
+        (Network.cpp:1204-1231): runs of equal consecutive weights as
+        (counts, weights)."""
+        return compress_weights(self.GetWeights())
 
     def GetSelectedWeights(self, indexes) -> list:
         w = self.GetWeights()

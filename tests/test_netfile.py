@@ -341,3 +341,12 @@ def test_simulation_api_call_order_and_queries():
     assert S.Simulation_API.GetIntegrationMethodDefParams("Bifixed_RK4")["n_steps"] == 2
     with pytest.raises(EDLUTException):
         S.Simulation_API.GetNeuronModelDefParams("NoSuchModel")
+
+
+def test_compressed_weights():
+    """Network::GetCompressedWeights (Network.cpp:1204-1231): runs of equal
+    consecutive weights, a leading zero weight included."""
+    assert S.compress_weights([]) == ([], [])
+    assert S.compress_weights([0.0, 0.0, 1.5, 1.5, 1.5, 0.0, 2.0]) == ([2, 3, 1, 1], [0.0, 1.5, 0.0, 2.0])
+    c, v = S.compress_weights(np.float32([0.1, 0.1, 0.2]))
+    assert c == [2, 1] and v == [float(np.float32(0.1)), float(np.float32(0.2))]
