@@ -275,11 +275,7 @@ def parse_neuron_model(model_name: str, ident: str, base_dir: str | None = None)
         params[pname] = float(v)
     if "c_m" in params and not params["c_m"] > 0.0:
         f.fail("c_m must be positive")
-    f.skip_comments()
-    try:
-        params["int_meth"] = parse_integration_method(f)
-    except EDLUTFileException:
-        raise
+    params["int_meth"] = parse_integration_method(f)
     return ModelDescription(model_name, params)
 
 

@@ -211,8 +211,7 @@ class Simulation_API:
         self.nets = {}
         self.circuit = None
         self.current_step = 0       # last update run
-        self._out_cursor = 0.0      # spikes up to this time were returned
-        self._out_pending: list[tuple[np.ndarray, np.ndarray]] = []
+        self._in_cursor = -np.inf   # input spikes up to this time were returned
         self._log_read = {}         # per type: spikes of its log already read
         self._syn_pos = None        # per synapse: (type, source slot, position) for GetWeights
 
@@ -743,7 +742,7 @@ class Simulation_API:
                          np.asarray(T.globals, np.int64)[ix[keep]]))
         # input neurons with output activity: their spikes up to the current time
         if len(self._input_times):
-            prev = getattr(self, "_in_cursor", -np.inf)
+            prev = self._in_cursor
             sel = (self._input_times > prev) & (self._input_times <= t_now)
             for q in np.nonzero(sel)[0]:
                 T = plan.types[plan.type_of[self._input_cells[q]]]
