@@ -199,6 +199,34 @@ class Network:
         self.next_step = 1
         return a.value, b.value, c.value, ms.value
 
+    def autotune_settled(self, settle_steps: int, trial_steps: int, items=(2, 4), lanes=(16, 32, 64),
+                         blocks=(64, 256, 1024, 4096)):
+        """Launch-shape search in the network's settled regime: `settle_steps`
+        steps first, then every shape times `trial_steps` steps continuing the
+        same run (no reset between trials, so each trial sees the activity the
+        network has reached).  The network is reset afterwards.  Returns
+        (items, lanes, blocks, ms_per_step)."""
+        import time
+
+        self.run(settle_steps)
+        best = None
+        for it in items:
+            for ln in lanes:
+                for bl in blocks:
+                    try:
+                        self.set_launch(it, ln, bl)
+                    except C.EDLUTException:
+                        continue
+                    self.run(16)
+                    t0 = time.perf_counter()
+                    self.run(trial_steps)
+                    ms = (time.perf_counter() - t0) * 1e3 / trial_steps
+                    if best is None or ms < best[3]:
+                        best = (it, ln, bl, ms)
+        self.set_launch(*best[:3])
+        self.reset()
+        return best
+
     def set_external(self, thresholds, w_ext, seed):
         thr = np.ascontiguousarray(thresholds, np.uint32)
         C.check(C.lib().edlut_network_set_external(self._h, len(thr), C.ptr(thr), float(w_ext), int(seed)))

@@ -127,6 +127,7 @@ template <typename R> struct PopBase {
     R dt_model = 0;                /* elapsedTimeInNeuronModelScale */
     bool EXC = false, INH = false, NMDA = false, EXT_I = false;
     std::vector<int32_t> spikes;   /* InternalSpikeIndexs of the last update */
+    std::vector<uint8_t> fired_tmp; /* per-neuron spike flags of a parallel update */
     std::vector<int> bdf_state;     /* BDFn::state, empty for other integrators */
     /* integration_method->resetState(index) (BDFn.h resetState: state = 0) */
     void reset_integrator(int i) {
@@ -975,28 +976,42 @@ template <typename R> struct Izhikevich : PopBase<R> {
 /* ------------------------------------------------------------------------ */
 const int kMaxVars = 32;
 
+/* Neurons are independent within an update: the loop runs on the OpenMP
+ * threads set by orc_set_threads (one by default), and the spike list is
+ * gathered in ascending index afterwards, as a serial loop would build it. */
+inline int g_threads = 1;
+template <typename M, typename F> void for_each_neuron(M &m, F &&body) {
+    m.fired_tmp.assign((size_t)m.N, 0);
+    uint8_t *f = m.fired_tmp.data();
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1 && m.N >= 2048)
+    for (int index = 0; index < m.N; index++)
+        if (body(index)) f[index] = 1;
+    for (int i = 0; i < m.N; i++)
+        if (f[i]) m.spikes.push_back(i);
+}
+
 /* Euler: Kernel/include/integration_method/Euler.h:123-146, decay factors
  * for elapsed = dt (Euler.h:230-234). */
 template <typename M, typename R> void step_euler(M &m) {
-    R D[kMaxVars];
     const R dt = m.dt_model;
-    for (int index = 0; index < m.N; index++) {
+    for_each_neuron(m, [&](int index) {
+        R D[kMaxVars];
         R *S = m.at(index);
         R prev = S[0];
         m.deriv(S, D, index, dt);
         for (int j = 0; j < m.ndiff; j++) S[j] += dt * D[j];
         m.decay(S);
         m.add_elapsed(index);
-        if (m.spike(prev, S, index)) m.spikes.push_back(index);
-    }
+        return m.spike(prev, S, index);
+    });
 }
 
 /* RK2 (Heun): Kernel/include/integration_method/RK2.h:133-165, decay factors
  * for elapsed = dt (RK2.h:257-261). */
 template <typename M, typename R> void step_rk2(M &m) {
-    R D1[kMaxVars], D2[kMaxVars];
     const R dt = m.dt_model;
-    for (int index = 0; index < m.N; index++) {
+    for_each_neuron(m, [&](int index) {
+        R D1[kMaxVars], D2[kMaxVars];
         R *S = m.at(index);
         R prev = S[0];
         m.deriv(S, D1, index, dt);
@@ -1005,18 +1020,18 @@ template <typename M, typename R> void step_rk2(M &m) {
         m.deriv(S, D2, index, dt);
         for (int j = 0; j < m.ndiff; j++) S[j] += (-D1[j] + D2[j]) * dt * 0.5f;
         m.add_elapsed(index);
-        if (m.spike(prev, S, index)) m.spikes.push_back(index);
-    }
+        return m.spike(prev, S, index);
+    });
 }
 
 /* RK4: Kernel/include/integration_method/RK4.h:167-231, decay factors for
  * elapsed = dt*0.5 applied twice (RK4.h:354-358). */
 template <typename M, typename R> void step_rk4(M &m) {
-    R A[kMaxVars], D1[kMaxVars], D2[kMaxVars], D3[kMaxVars], D4[kMaxVars];
     const R dt = m.dt_model;
     const R dt_0_5 = dt * 0.5f;
     const R dt_0_16 = dt * 0.166666666667f;
-    for (int index = 0; index < m.N; index++) {
+    for_each_neuron(m, [&](int index) {
+        R A[kMaxVars], D1[kMaxVars], D2[kMaxVars], D3[kMaxVars], D4[kMaxVars];
         R *S = m.at(index);
         R prev = S[0];
         m.deriv(S, D1, index, dt);
@@ -1033,8 +1048,8 @@ template <typename M, typename R> void step_rk4(M &m) {
             S[j] += (D1[j] + 2.0f * (D2[j] + D3[j]) + D4[j]) * dt_0_16;
         for (int j = m.ndiff; j < m.nvars; j++) S[j] = A[j];
         m.add_elapsed(index);
-        if (m.spike(prev, S, index)) m.spikes.push_back(index);
-    }
+        return m.spike(prev, S, index);
+    });
 }
 
 
@@ -1113,10 +1128,10 @@ template <typename R> void invermat(int n, const R *a, R *ainv) {
 template <typename M, typename R> void step_bdf(M &m, BdfStore<R> &b) {
     const int nd = m.ndiff, nv = m.nvars;
     const R dt = m.dt_model;
-    R F[kMaxVars], P[kMaxVars], P1[kMaxVars], Cc[kMaxVars];
-    R jac[kMaxVars * kMaxVars], J[kMaxVars * kMaxVars], iJ[kMaxVars * kMaxVars];
-    R Jx[kMaxVars], Jpos[kMaxVars], Jneg[kMaxVars];
-    for (int index = 0; index < m.N; index++) {
+    for_each_neuron(m, [&](int index) {
+        R F[kMaxVars], P[kMaxVars], P1[kMaxVars], Cc[kMaxVars];
+        R jac[kMaxVars * kMaxVars], J[kMaxVars * kMaxVars], iJ[kMaxVars * kMaxVars];
+        R Jx[kMaxVars], Jpos[kMaxVars], Jneg[kMaxVars];
         R *S = m.at(index);
         R prev = S[0];
         int &st = m.bdf_state[index];
@@ -1185,8 +1200,8 @@ template <typename M, typename R> void step_bdf(M &m, BdfStore<R> &b) {
         for (int j = 0; j < nd; j++) S[j] = P[j];
         m.decay(S);
         m.add_elapsed(index);
-        if (m.spike(prev, S, index)) m.spikes.push_back(index);
-    }
+        return m.spike(prev, S, index);
+    });
 }
 
 /* ------------------------------------------------------------------------ */
@@ -1347,6 +1362,7 @@ struct Source {
     /* Poisson device (PoissonGeneratorDeviceVector: one frequency per
      * generator, `rates`, or `rate` for all) */
     std::vector<uint32_t> lcg;
+    std::vector<uint8_t> fired_tmp;
     float rate = 0.0f;
     std::vector<float> rates;
     /* spike train (input neurons placed on update steps): the spikes of
@@ -1489,6 +1505,8 @@ PopIface *make_pop(int model, int integ, int bdf_order, const float *p, float dt
 
 /* ======================================================================== */
 extern "C" {
+
+void orc_set_threads(int n) { g_threads = n < 1 ? 1 : n; }
 
 void orc_exp_table(float *out) { std::memcpy(out, exp_table().v, sizeof(float) * kExpSize); }
 
@@ -1756,13 +1774,18 @@ static int net_step(Net *n, int64_t k, std::vector<int32_t> &spk) {
             continue;
         }
         const double step_s = (double)pop->dt_s();
+        src.fired_tmp.assign((size_t)src.n, 0);
+        uint8_t *fl = src.fired_tmp.data();
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1 && src.n >= 4096)
         for (int i = 0; i < src.n; i++) {
             const float f = src.rates.empty() ? src.rate : src.rates[(size_t)i];
             if (f > 0.0f) {
                 src.lcg[i] = lcg_next(src.lcg[i]);
-                if ((double)f * step_s > (double)lcg_frand(src.lcg[i])) list.push_back(i);
+                if ((double)f * step_s > (double)lcg_frand(src.lcg[i])) fl[i] = 1;
             }
         }
+        for (int i = 0; i < src.n; i++)
+            if (fl[i]) list.push_back(i);
     }
     /* 4. population update k */
     int ns = pop->update(k, spk.data());

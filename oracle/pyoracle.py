@@ -53,6 +53,7 @@ def lib():
         l.orc_pop_add_inputs.argtypes = [P, P, P, P, ctypes.c_int]
         l.orc_pop_set_ext_current.argtypes = [P, P]
         l.orc_pop_update.argtypes = [P, ctypes.c_int64, P]
+        l.orc_set_threads.argtypes = [ctypes.c_int]
         l.orc_exp_table.argtypes = [P]
         l.orc_exp_lut.argtypes = [P, P, ctypes.c_int]
         l.orc_nmda_table.argtypes = [ctypes.c_float, ctypes.c_float, P]
@@ -95,6 +96,11 @@ def lib():
 
 def _p(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def set_threads(n: int) -> None:
+    """OpenMP threads of the population updates (1: serial)."""
+    lib().orc_set_threads(int(n))
 
 
 def exp_table() -> np.ndarray:

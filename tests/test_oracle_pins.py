@@ -18,7 +18,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from tests.helpers import ADEX_DEFAULTS, LIF_DEFAULTS
+from tests.helpers import ADEX_DEFAULTS, LIF_DEFAULTS, np_out_csr
 
 REF_DUMP = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_dump")
 
@@ -272,3 +272,25 @@ def test_lif_refractory_against_double_clock(oracle):
             break
         assert abs(pop.state(0)[0] - v) < 1e-9, j
         clock += float(np.float32(1e-4))
+
+
+def test_threaded_oracle_equals_serial(oracle):
+    """The oracle's population updates run on several OpenMP threads (the
+    multi-core CPU baseline); the spikes and states are those of the serial
+    loop, bit for bit."""
+    from edlut_amd import workloads
+
+    spec = workloads.adex_recurrent(n=6000, in_degree=50, steps=300, w_ext=1.0)
+    out = []
+    for threads in (1, 4):
+        oracle.set_threads(threads)
+        try:
+            pop, net = oracle.from_spec(spec, ADEX_DEFAULTS, np_out_csr)
+            st, ix = net.run(1, 300)
+            out.append((st, ix, [pop.state(v) for v in range(4)]))
+        finally:
+            oracle.set_threads(1)
+    assert len(out[0][0]) > 100
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    for a, b in zip(out[0][2], out[1][2]):
+        assert np.array_equal(a.view(np.uint64), b.view(np.uint64))
