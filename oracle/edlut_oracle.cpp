@@ -1370,6 +1370,15 @@ struct Source {
     bool train = false;
     std::vector<int64_t> train_ptr;
     std::vector<int32_t> train_idx;
+    /* event engine: sources of several networks with the same key are one
+     * emitting model (one layer type of the reference); rank orders the
+     * models' first update events (the network file's type order).  A train
+     * may carry the input spikes' times, in load order. */
+    int emitter_key = -1;
+    int rank = -1;
+    std::vector<std::pair<int32_t, int32_t>> poisson_log;
+    std::vector<double> train_time;
+    std::vector<int32_t> train_neuron;
     /* projection from another network's population: its spikes */
     struct Net *from = nullptr;
     /* spikes per step: ring indexed by step % ring */
@@ -1442,6 +1451,12 @@ struct Net {
      * projections into other networks), and the spikes logged by circuit runs */
     std::vector<std::vector<int32_t>> out_ring;
     std::vector<std::pair<int32_t, int32_t>> log;
+    /* event engine: the simulation this network runs in, the rank of its
+     * population among the models, and an explicit stop time for its next run */
+    struct Sim *sim = nullptr;
+    int rank = -1;
+    int model_rank = -1;
+    double stop_time = -1.0;
     /* update times: t[k] = t[k-1] + double(float dt), t[0] = 0
      * (TimeEventUpdateNeurons.cpp:78, Simulation.cpp:228-268) */
     std::vector<double> t;
@@ -1806,11 +1821,479 @@ static int net_step(Net *n, int64_t k, std::vector<int32_t> &spk) {
     return ns;
 }
 
+}  // extern "C"
+
+namespace {
+
+/* ======================================================================== */
+/* Event engine: the reference's event loop on one queue.                    */
+/*                                                                          */
+/* Every time-driven model (a population, a Poisson generator layer) has a  */
+/* TimeEventUpdateNeurons stream at its own step (TimeEventUpdateNeurons.cpp */
+/* 45-79, first event at one step, Simulation.cpp:228-268); an update       */
+/* gathers its spiking neurons, in ascending index, into one                */
+/* PropagatedSpikeGroup per delay of the model's delay structure, queueing  */
+/* a group as soon as it holds 1,024 sources and the rest afterwards in     */
+/* ascending delay (TimeDrivenInternalSpike.cpp:66-104,163-178,             */
+/* PropagatedSpikeGroup.h:54); a group delivers, source by source, the      */
+/* source's synapses of its delay (PropagatedSpikeGroup.cpp:66-108).  Input */
+/* neurons' spikes are InputSpike events that chain one PropagatedSpike     */
+/* per delay of the neuron (InputSpike.cpp, PropagatedSpike.cpp:119-160).   */
+/* All of them share one binary heap (EventQueue.cpp:121-198) ordered by   */
+/* time, then EventPriority (Event.h:37); equal keys leave in the order the */
+/* heap's layout gives.  A run is RunSimulationSlot (Simulation.cpp:322-373):*/
+/* a StopSimulationEvent at the stop time, then events until it leaves.     */
+/* ======================================================================== */
+namespace evq {
+
+enum { PRIO_STOP = 1, PRIO_SPIKE = 8, PRIO_TIME = 10 };   /* Event.h:37 */
+enum { K_TIME = 0, K_GROUP = 1, K_INPUT = 2, K_PROP = 3, K_STOP = 4 };
+enum { E_POP = 0, E_POISSON = 1, E_TRAIN = 2 };
+
+struct Item {
+    double t;
+    int prio;
+    int32_t ev;
+};
+
+/* EventQueue::InsertEvent / RemoveEvent restricted to one queue. */
+struct Heap {
+    std::vector<Item> e{Item{0.0, 0, -1}};
+    size_t size() const { return e.size() - 1; }
+    void insert(const Item &it) {
+        e.push_back(it);
+        for (size_t c = size(); c > 1 && (e[c / 2].t > e[c].t || (e[c / 2].t == e[c].t && e[c / 2].prio < e[c].prio));
+             c /= 2)
+            std::swap(e[c], e[c / 2]);
+    }
+    bool remove(Item &out) {
+        const size_t n = e.size();
+        if (n > 2) {
+            out = e[1];
+            e[1] = e[size()];
+            e.pop_back();
+            size_t p = 1, c;
+            for (c = p * 2; c < size(); p = c, c = p * 2) {
+                if (e[c].t > e[c + 1].t || (e[c].t == e[c + 1].t && e[c].prio < e[c + 1].prio)) c++;
+                if (e[c].t < e[p].t || (e[c].t == e[p].t && e[c].prio > e[p].prio)) std::swap(e[p], e[c]);
+                else break;
+            }
+            if (c == size() && (e[c].t < e[p].t || (e[c].t == e[p].t && e[c].prio > e[p].prio)))
+                std::swap(e[p], e[c]);
+            return true;
+        }
+        if (n == 2) {
+            out = e[1];
+            e.pop_back();
+            return true;
+        }
+        return false;
+    }
+};
+
+struct Ev {
+    int kind = 0;
+    int em = 0;
+    int d = 0;          /* delay (steps) of a group */
+    int32_t neuron = 0; /* input spike: source neuron */
+    int j = 0;          /* propagated spike: index into the neuron's delays */
+    std::vector<int32_t> srcs;
+};
+
+struct Emitter {
+    int kind = E_POP;
+    int rank = 0;
+    int key = -1;
+    Net *net = nullptr;                              /* population: its network */
+    std::vector<std::pair<Net *, int>> copies;      /* (network, source index) it feeds */
+    double h = 0.0;                                  /* step (s) of its update events */
+    int64_t k = 1;                                   /* update index of its next update */
+    int n = 0;
+    int max_delay = 1;
+    float dt_s = 1e-4f;                              /* step of the networks it feeds */
+    std::vector<int> delays;                         /* model delay structure, ascending */
+    std::vector<std::vector<int>> ndel;              /* delays of each neuron, ascending */
+};
+
+/* PoissonGeneratorDeviceVector samples every sampling_period = 0.1f ms of
+ * its millisecond time scale: 0.1f / 1000.0f in float, held as a double
+ * (PoissonGeneratorDeviceVector.cpp:26,137). */
+inline double poisson_step() { return (double)(0.1f / 1000.0f); }
+
+}  // namespace evq
+
+struct Sim {
+    std::vector<Net *> nets;
+    std::vector<evq::Emitter> em;
+    evq::Heap heap;
+    std::vector<evq::Ev> pool;
+    std::vector<int32_t> free_ev;
+    int64_t next_first = 1;
+    int32_t new_ev(int kind, int em) {
+        int32_t id;
+        if (!free_ev.empty()) {
+            id = free_ev.back();
+            free_ev.pop_back();
+        } else {
+            id = (int32_t)pool.size();
+            pool.emplace_back();
+        }
+        evq::Ev &e = pool[(size_t)id];
+        e.kind = kind;
+        e.em = em;
+        e.d = 0;
+        e.neuron = 0;
+        e.j = 0;
+        e.srcs.clear();
+        return id;
+    }
+};
+
+namespace evq {
+
+/* The delays (steps) of neuron i of a source, in ascending order. */
+inline void add_neuron_delays(Emitter &E, const Net *n, const Source &s) {
+    const int D = n->max_delay;
+    for (int i = 0; i < E.n; i++)
+        for (int d = 1; d <= D; d++)
+            if (s.row_ptr[(size_t)i * D + d] > s.row_ptr[(size_t)i * D + d - 1]) E.ndel[(size_t)i].push_back(d);
+}
+
+inline void finish_delays(Emitter &E) {
+    std::vector<char> any((size_t)E.max_delay + 1, 0);
+    for (auto &v : E.ndel) {
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        for (int d : v) any[(size_t)d] = 1;
+    }
+    for (int d = 1; d <= E.max_delay; d++)
+        if (any[(size_t)d]) E.delays.push_back(d);
+}
+
+/* Delivery of neuron `id`'s synapses of delay d in every network the
+ * emitter feeds, at event time t (ProcessInputSpike + pre-synaptic STDP). */
+inline void deliver(Emitter &E, int32_t id, int d, double t) {
+    for (auto &cp : E.copies) {
+        Net *n = cp.first;
+        Source &src = n->sources[(size_t)cp.second];
+        const int D = n->max_delay;
+        const int64_t b = src.row_ptr[(size_t)id * D + d - 1];
+        const int64_t e = src.row_ptr[(size_t)id * D + d];
+        const int32_t k_next = (int32_t)n->sim->em[(size_t)n->rank].k;
+        for (int64_t q = b; q < e; q++) {
+            const uint32_t tt = src.tgt_type[(size_t)q];
+            n->pop->add_input((int)(tt & 0x3fffffffu), (int)(tt >> 30), src.w[(size_t)q]);
+            if (n->trace) n->trace_buf.push_back({k_next, (int32_t)(tt & 0x3fffffffu), (uint8_t)(tt >> 30), src.w[(size_t)q]});
+            if (src.stdp) stdp_pre(src, q, t);
+        }
+    }
+}
+
+/* TimeDrivenInternalSpike::ProcessEvent: groups of the spiking neurons. */
+inline void queue_groups(Sim &S, int e, const std::vector<int32_t> &spk, double t) {
+    Emitter &E = S.em[(size_t)e];
+    std::vector<int32_t> open((size_t)E.max_delay + 1, -1);
+    auto push = [&](int d, int32_t id) {
+        if (open[(size_t)d] < 0) {
+            open[(size_t)d] = S.new_ev(K_GROUP, e);
+            S.pool[(size_t)open[(size_t)d]].d = d;
+        }
+        const int32_t g = open[(size_t)d];
+        S.pool[(size_t)g].srcs.push_back(id);
+        if (S.pool[(size_t)g].srcs.size() >= 1024) {
+            S.heap.insert(Item{t + delay_seconds(d, E.dt_s), PRIO_SPIKE, g});
+            open[(size_t)d] = -1;
+        }
+    };
+    for (int32_t id : spk)
+        for (int d : E.ndel[(size_t)id]) push(d, id);
+    for (int d : E.delays)
+        if (open[(size_t)d] >= 0) S.heap.insert(Item{t + delay_seconds(d, E.dt_s), PRIO_SPIKE, open[(size_t)d]});
+}
+
+}  // namespace evq
+
+/* Population update k of network n at time t (external drive, update,
+ * post-synaptic learning); returns the spikes. */
+static void engine_update_pop(Net *n, int64_t k, double t, std::vector<int32_t> &spk) {
+    PopIface *pop = n->pop;
+    const int N = pop->size();
+    if (n->has_ext) {
+        for (int i = 0; i < N; i++) {
+            uint32_t u = drive_hash(n->ext_seed, k, (uint32_t)i);
+            int cnt = 0;
+            while (cnt < (int)n->thr.size() && u < n->thr[cnt]) cnt++;
+            for (int c = 0; c < cnt; c++) pop->add_input(i, 0, n->w_ext);
+        }
+    }
+    spk.resize((size_t)N);
+    const int ns = pop->update(k, spk.data());
+    spk.resize((size_t)ns);
+    for (int32_t i : spk) n->log.push_back({(int32_t)k, i});
+    for (Source &src : n->sources) {
+        if (!src.stdp) continue;
+        for (int32_t i : spk)
+            for (int64_t syn : src.in_syn[(size_t)i]) stdp_post(src, syn, t);
+    }
+}
+
+/* Builds the simulation of a set of networks: one emitter per population
+ * and per Poisson / spike-train model; the initial events. */
+static Sim *engine_create(Net **nets, int n_nets) {
+    using namespace evq;
+    Sim *S = new Sim;
+    S->nets.assign(nets, nets + n_nets);
+    for (int i = 0; i < n_nets; i++) {
+        Net *n = nets[i];
+        n->sim = S;
+        n->rank = i;
+        Emitter E;
+        E.kind = E_POP;
+        E.rank = n->model_rank >= 0 ? n->model_rank : i;
+        E.net = n;
+        E.h = (double)n->pop->dt_s();
+        E.n = n->pop->size();
+        E.max_delay = n->max_delay;
+        E.dt_s = n->pop->dt_s();
+        E.ndel.assign((size_t)E.n, {});
+        S->em.push_back(E);
+    }
+    int next_key = 1 << 20;
+    for (int i = 0; i < n_nets; i++) {
+        Net *n = nets[i];
+        for (size_t si = 0; si < n->sources.size(); si++) {
+            Source &s = n->sources[si];
+            int e = -1;
+            if (s.self) e = i;
+            else if (s.from) e = s.from->rank;   /* projections: the source network is in the circuit */
+            else {
+                const int key = s.emitter_key >= 0 ? s.emitter_key : next_key++;
+                for (size_t q = (size_t)n_nets; q < S->em.size(); q++)
+                    if (S->em[q].key == key) e = (int)q;
+                if (e < 0) {
+                    Emitter E;
+                    E.kind = s.train ? E_TRAIN : E_POISSON;
+                    E.key = key;
+                    E.rank = s.rank >= 0 ? s.rank : 1000 + (int)S->em.size();
+                    E.h = s.train ? 0.0 : poisson_step();
+                    E.n = s.n;
+                    E.max_delay = n->max_delay;
+                    E.dt_s = n->pop->dt_s();
+                    E.ndel.assign((size_t)E.n, {});
+                    S->em.push_back(E);
+                    e = (int)S->em.size() - 1;
+                }
+            }
+            S->em[(size_t)e].copies.push_back({n, (int)si});
+            add_neuron_delays(S->em[(size_t)e], n, s);
+        }
+    }
+    for (Emitter &E : S->em) finish_delays(E);
+    /* Simulation::InitSimulation: the input spikes first (GetInput), then one
+     * update event per time-driven model in type order. */
+    for (size_t e = 0; e < S->em.size(); e++) {
+        Emitter &E = S->em[e];
+        if (E.kind != E_TRAIN) continue;
+        const Source &s = E.copies[0].first->sources[(size_t)E.copies[0].second];
+        const Net *n0 = E.copies[0].first;
+        std::vector<std::pair<double, int32_t>> in;
+        if (!s.train_time.empty()) {
+            for (size_t q = 0; q < s.train_time.size(); q++) in.push_back({s.train_time[q], s.train_neuron[q]});
+        } else {
+            /* spikes placed on updates: an input spike at the time of update k */
+            double t = 0.0;
+            const int64_t steps = (int64_t)s.train_ptr.size() - 1;
+            for (int64_t k = 1; k <= steps; k++) {
+                t += (double)n0->pop->dt_s();
+                for (int64_t j = s.train_ptr[(size_t)k - 1]; j < s.train_ptr[(size_t)k]; j++)
+                    in.push_back({t, s.train_idx[(size_t)j]});
+            }
+        }
+        for (auto &x : in) {
+            if (E.ndel[(size_t)x.second].empty()) continue;
+            const int32_t id = S->new_ev(K_INPUT, (int)e);
+            S->pool[(size_t)id].neuron = x.second;
+            S->heap.insert(Item{x.first, PRIO_SPIKE, id});
+        }
+    }
+    std::vector<int> order;
+    for (size_t e = 0; e < S->em.size(); e++)
+        if (S->em[e].kind != E_TRAIN) order.push_back((int)e);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return S->em[(size_t)a].rank < S->em[(size_t)b].rank; });
+    for (int e : order) S->heap.insert(Item{S->em[(size_t)e].h, PRIO_TIME, S->new_ev(K_TIME, e)});
+    return S;
+}
+
+/* RunSimulationSlot(stop): updates first..first+n_steps-1 of the
+ * populations; the stop time defaults to the last update's time. */
+static int engine_run(Sim *S, int64_t first, int n_steps, double stop) {
+    using namespace evq;
+    if (first != S->next_first) return -1;
+    const int64_t last = first + n_steps - 1;
+    if (stop < 0) {
+        const Emitter &P = S->em[0];
+        double t = 0.0;
+        for (int64_t k = 1; k <= last; k++) t += P.h;
+        stop = t;
+    }
+    const int32_t stop_id = S->new_ev(K_STOP, -1);
+    S->heap.insert(Item{stop, PRIO_STOP, stop_id});
+    std::vector<int32_t> spk;
+    Item it;
+    while (S->heap.remove(it)) {
+        const int kind = S->pool[(size_t)it.ev].kind;
+        const int e = S->pool[(size_t)it.ev].em;
+        if (kind == K_STOP) {
+            S->free_ev.push_back(it.ev);
+            break;
+        }
+        if (kind == K_TIME) {
+            Emitter &E = S->em[(size_t)e];
+            const int64_t k = E.k;
+            if (E.kind == E_POP) {
+                engine_update_pop(E.net, k, it.t, spk);
+            } else {
+                /* PoissonGeneratorDeviceVector::UpdateState (.cpp:56-70) with
+                 * the first copy's generators; the other copies follow */
+                Source &s0 = E.copies[0].first->sources[(size_t)E.copies[0].second];
+                s0.fired_tmp.assign((size_t)s0.n, 0);
+                uint8_t *fl = s0.fired_tmp.data();
+#pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1 && s0.n >= 4096)
+                for (int i = 0; i < s0.n; i++) {
+                    const float f = s0.rates.empty() ? s0.rate : s0.rates[(size_t)i];
+                    if (f > 0.0f) {
+                        s0.lcg[(size_t)i] = lcg_next(s0.lcg[(size_t)i]);
+                        if ((double)f * E.h > (double)lcg_frand(s0.lcg[(size_t)i])) fl[i] = 1;
+                    }
+                }
+                spk.clear();
+                for (int i = 0; i < s0.n; i++)
+                    if (fl[i]) spk.push_back(i);
+                for (size_t c = 1; c < E.copies.size(); c++)
+                    E.copies[c].first->sources[(size_t)E.copies[c].second].lcg = s0.lcg;
+                for (auto &cp : E.copies) {
+                    Source &s = cp.first->sources[(size_t)cp.second];
+                    for (int32_t i : spk) s.poisson_log.push_back({(int32_t)k, i});
+                }
+            }
+            queue_groups(*S, e, spk, it.t);
+            E.k = k + 1;
+            S->heap.insert(Item{it.t + E.h, PRIO_TIME, it.ev});
+            continue;
+        }
+        Ev &ev = S->pool[(size_t)it.ev];
+        Emitter &E = S->em[(size_t)e];
+        if (kind == K_GROUP) {
+            for (int32_t id : ev.srcs) deliver(E, id, ev.d, it.t);
+            S->free_ev.push_back(it.ev);
+        } else if (kind == K_INPUT) {
+            /* InputSpike::ProcessEvent: the first delay of the neuron */
+            ev.kind = K_PROP;
+            ev.j = 0;
+            S->heap.insert(Item{it.t + delay_seconds(E.ndel[(size_t)ev.neuron][0], E.dt_s), PRIO_SPIKE, it.ev});
+        } else {
+            /* PropagatedSpike::ProcessEvent: this delay, then the next one at
+             * time - delay_j + delay_{j+1} */
+            const std::vector<int> &dl = E.ndel[(size_t)ev.neuron];
+            deliver(E, ev.neuron, dl[(size_t)ev.j], it.t);
+            if (ev.j + 1 < (int)dl.size()) {
+                const double t2 = it.t - delay_seconds(dl[(size_t)ev.j], E.dt_s) + delay_seconds(dl[(size_t)ev.j + 1], E.dt_s);
+                ev.j++;
+                S->heap.insert(Item{t2, PRIO_SPIKE, it.ev});
+            } else {
+                S->free_ev.push_back(it.ev);
+            }
+        }
+    }
+    S->next_first = last + 1;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+/* Order of delivery: 1 the reference's event queue, 0 (default) the
+ * canonical order (delay, source, spiking index, synapse). */
+static int g_order = 0;
+void orc_set_order(int mode) { g_order = mode; }
+
+/* Sources with the same key (across networks) are one model; rank orders
+ * the models' first update events. */
+int orc_net_set_source_model(void *net, int source, int key, int rank) {
+    Net *n = (Net *)net;
+    if (source < 0 || source >= (int)n->sources.size() || n->sim) return -1;
+    n->sources[(size_t)source].emitter_key = key;
+    n->sources[(size_t)source].rank = rank;
+    return 0;
+}
+
+/* Input spikes of a spike-train source as (time s, neuron) in load order:
+ * the reference's InputSpike events, queued when the simulation starts. */
+int orc_net_set_train_times(void *net, int source, int64_t n_spikes, const double *time, const int32_t *neuron) {
+    Net *n = (Net *)net;
+    if (source < 0 || source >= (int)n->sources.size() || n->sim || !n->sources[(size_t)source].train) return -1;
+    Source &s = n->sources[(size_t)source];
+    s.train_time.assign(time, time + n_spikes);
+    s.train_neuron.assign(neuron, neuron + n_spikes);
+    return 0;
+}
+
+/* The event heap alone: op[i] = 1 inserts an element (time t[i], priority
+ * prio[i], id = number of earlier inserts), op[i] = 0 removes the top and
+ * writes its id to removed[]; returns the number removed. */
+int64_t orc_heap_replay(int64_t n_ops, const int8_t *op, const double *t, const int32_t *prio,
+                        int32_t *removed) {
+    evq::Heap h;
+    int32_t next = 0;
+    int64_t r = 0;
+    for (int64_t i = 0; i < n_ops; i++) {
+        if (op[i]) {
+            h.insert(evq::Item{t[i], prio[i], next++});
+        } else {
+            evq::Item it;
+            if (h.remove(it)) removed[r++] = it.ev;
+        }
+    }
+    return r;
+}
+
+/* Rank of the network's population among the models (default: its
+ * position in the circuit). */
+int orc_net_set_model_rank(void *net, int rank) {
+    Net *n = (Net *)net;
+    if (n->sim) return -1;
+    n->model_rank = rank;
+    return 0;
+}
+
+/* Stop time (s) of the network's (or circuit's) next run. */
+void orc_net_set_stop_time(void *net, double t) { ((Net *)net)->stop_time = t; }
+
 int64_t orc_net_run(void *net, int64_t first_step, int n_steps, int32_t *spike_step,
                     int32_t *spike_idx, int64_t cap) {
     Net *n = (Net *)net;
-    std::vector<int32_t> spk(n->pop->size());
     int64_t total = 0;
+    if (g_order == 1) {
+        if (!n->sim) {
+            for (const Source &s : n->sources)
+                if (s.from) return -2;   /* projections run in a circuit */
+            engine_create(&n, 1);
+        }
+        const size_t l0 = n->log.size();
+        if (engine_run(n->sim, first_step, n_steps, n->stop_time) != 0) return -2;
+        n->stop_time = -1.0;
+        for (size_t q = l0; q < n->log.size(); q++) {
+            if (total < cap) {
+                spike_step[total] = n->log[q].first;
+                spike_idx[total] = n->log[q].second;
+            }
+            total++;
+        }
+        n->log.resize(l0);
+        return total > cap ? -1 : total;
+    }
+    std::vector<int32_t> spk(n->pop->size());
     for (int64_t k = first_step; k < first_step + n_steps; k++) {
         const int ns = net_step(n, k, spk);
         for (int q = 0; q < ns; q++) {
@@ -1853,6 +2336,13 @@ int orc_net_set_poisson_rates(void *net, int source, const float *rates) {
 /* Lockstep run of several networks: update k of every network (in the order
  * given) before update k + 1 of any; the spikes go to each network's log. */
 void orc_circuit_run(void **nets, int n_nets, int64_t first_step, int n_steps) {
+    if (g_order == 1) {
+        Net *n0 = (Net *)nets[0];
+        if (!n0->sim) engine_create((Net **)nets, n_nets);
+        engine_run(n0->sim, first_step, n_steps, n0->stop_time);
+        n0->stop_time = -1.0;
+        return;
+    }
     std::vector<std::vector<int32_t>> spk((size_t)n_nets);
     for (int i = 0; i < n_nets; i++) spk[(size_t)i].resize(((Net *)nets[i])->pop->size());
     for (int64_t k = first_step; k < first_step + n_steps; k++)
@@ -1861,6 +2351,22 @@ void orc_circuit_run(void **nets, int n_nets, int64_t first_step, int n_steps) {
             const int ns = net_step(n, k, spk[(size_t)i]);
             for (int q = 0; q < ns; q++) n->log.push_back({(int32_t)k, spk[(size_t)i][(size_t)q]});
         }
+}
+
+/* Spikes of a Poisson source's generators at each of its updates (event
+ * engine); moved out when cap holds them all; returns the number logged. */
+int64_t orc_net_take_poisson_log(void *net, int source, int32_t *step, int32_t *idx, int64_t cap) {
+    Net *n = (Net *)net;
+    if (source < 0 || source >= (int)n->sources.size()) return -1;
+    auto &lg = n->sources[(size_t)source].poisson_log;
+    const int64_t m = (int64_t)lg.size();
+    if (cap < m) return m;
+    for (int64_t i = 0; i < m; i++) {
+        step[i] = lg[(size_t)i].first;
+        idx[i] = lg[(size_t)i].second;
+    }
+    lg.clear();
+    return m;
 }
 
 /* Moves the logged spikes out when cap holds them all (else leaves them);

@@ -15,6 +15,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
 REF_DUMP = os.path.join(HERE, "_ref", "ref_dump")
+REF_EVENTQUEUE = os.path.join(HERE, "_ref", "ref_eventqueue")
 
 MODELS = {"LIFTimeDrivenModel_GPU": 0, "AdExTimeDrivenModel_GPU": 1, "HHTimeDrivenModel_GPU": 2,
           "EgidioGranuleCell_TimeDriven_GPU": 3, "TimeDrivenPurkinjeCell_GPU": 4,
@@ -54,6 +55,15 @@ def lib():
         l.orc_pop_set_ext_current.argtypes = [P, P]
         l.orc_pop_update.argtypes = [P, ctypes.c_int64, P]
         l.orc_set_threads.argtypes = [ctypes.c_int]
+        l.orc_set_order.argtypes = [ctypes.c_int]
+        l.orc_net_set_source_model.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        l.orc_net_set_train_times.argtypes = [P, ctypes.c_int, ctypes.c_int64, P, P]
+        l.orc_net_set_stop_time.argtypes = [P, ctypes.c_double]
+        l.orc_net_set_model_rank.argtypes = [P, ctypes.c_int]
+        l.orc_net_take_poisson_log.restype = ctypes.c_int64
+        l.orc_net_take_poisson_log.argtypes = [P, ctypes.c_int, P, P, ctypes.c_int64]
+        l.orc_heap_replay.restype = ctypes.c_int64
+        l.orc_heap_replay.argtypes = [ctypes.c_int64, P, P, P, P]
         l.orc_exp_table.argtypes = [P]
         l.orc_exp_lut.argtypes = [P, P, ctypes.c_int]
         l.orc_nmda_table.argtypes = [ctypes.c_float, ctypes.c_float, P]
@@ -101,6 +111,23 @@ def _p(a):
 def set_threads(n: int) -> None:
     """OpenMP threads of the population updates (1: serial)."""
     lib().orc_set_threads(int(n))
+
+
+def set_order(mode: int) -> None:
+    """Delivery order of network runs: 1 the reference's event queue, 0 the
+    canonical order (delay, source, spiking index, synapse)."""
+    lib().orc_set_order(int(mode))
+
+
+def heap_replay(op, t, prio) -> np.ndarray:
+    """Ids in removal order of a sequence of heap operations (op 1 inserts
+    (t, prio) with id = number of earlier inserts, op 0 removes the top)."""
+    op = np.ascontiguousarray(op, np.int8)
+    t = np.ascontiguousarray(t, np.float64)
+    prio = np.ascontiguousarray(prio, np.int32)
+    out = np.zeros(max(int((op == 0).sum()), 1), np.int32)
+    r = lib().orc_heap_replay(len(op), _p(op), _p(t), _p(prio), _p(out))
+    return out[:r].copy()
 
 
 def exp_table() -> np.ndarray:
@@ -241,6 +268,38 @@ class Net:
              np.ascontiguousarray(w, np.float32)]
         lib().orc_net_add_source_train(self.h, int(n_src), len(sp) - 1, _p(sp), _p(ix) if len(ix) else None,
                                        *[_p(x) for x in a])
+
+    def set_source_model(self, source: int, key: int, rank: int = -1):
+        """Sources with the same key, across networks, are one model (one
+        layer type of the reference); rank orders the models' first updates."""
+        if lib().orc_net_set_source_model(self.h, int(source), int(key), int(rank)) != 0:
+            raise ValueError("bad source or simulation already started")
+
+    def set_train_times(self, source: int, times, neurons):
+        """Input spikes (time in s, neuron) of a spike-train source, in load order."""
+        t = np.ascontiguousarray(times, np.float64)
+        n = np.ascontiguousarray(neurons, np.int32)
+        self._keep += [t, n]
+        if lib().orc_net_set_train_times(self.h, int(source), len(t), _p(t), _p(n)) != 0:
+            raise ValueError("not a spike-train source, or simulation already started")
+
+    def set_model_rank(self, rank: int):
+        """Rank of this network's population among the models (update-event
+        order at start; default: its position in the circuit)."""
+        if lib().orc_net_set_model_rank(self.h, int(rank)) != 0:
+            raise ValueError("simulation already started")
+
+    def set_stop_time(self, t: float):
+        """Stop time (s) of the next run (default: its last update's time)."""
+        lib().orc_net_set_stop_time(self.h, float(t))
+
+    def take_poisson_log(self, source: int):
+        """(update, generator) of every spike of a Poisson source since the last call."""
+        m = lib().orc_net_take_poisson_log(self.h, int(source), None, None, 0)
+        st = np.zeros(max(m, 1), np.int32)
+        ix = np.zeros(max(m, 1), np.int32)
+        lib().orc_net_take_poisson_log(self.h, int(source), _p(st), _p(ix), m)
+        return st[:m].copy(), ix[:m].copy()
 
     def set_external(self, thr, w_ext, seed):
         t = np.ascontiguousarray(thr, np.uint32)

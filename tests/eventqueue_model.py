@@ -6,9 +6,13 @@ which order input spikes reach a time-driven population:
   * EventQueue: a 1-based binary heap keyed by (time, priority), with the
     sift-up / sift-down rules of Kernel/src/simulation/EventQueue.cpp:121-198
     (equal times: higher EventPriority first, Kernel/include/simulation/Event.h:37);
-  * TimeEventUpdateNeurons: one event per model, next one at t + double(float dt)
-    (Kernel/src/simulation/TimeEventUpdateNeurons.cpp:45-79), first at t = dt
-    (Kernel/src/simulation/Simulation.cpp:228-268);
+  * TimeEventUpdateNeurons: one event per model, next one at t + its step
+    (Kernel/src/simulation/TimeEventUpdateNeurons.cpp:45-79), first at one
+    step (Kernel/src/simulation/Simulation.cpp:228-268); a population's step
+    is double(float dt), a PoissonGeneratorDeviceVector's 0.1f / 1000.0f in
+    float (PoissonGeneratorDeviceVector.cpp:26,137);
+  * RunSimulationSlot: a StopSimulationEvent at the stop time, events until
+    it leaves (Kernel/src/simulation/Simulation.cpp:322-373);
   * TimeDrivenInternalSpike: spiking sources in ascending index, one
     PropagatedSpikeGroup per model delay (delays sorted ascending,
     NeuronModelPropagationDelayStructure.cpp:42-73), a group is queued as soon
@@ -28,8 +32,9 @@ import math
 import numpy as np
 
 # Event.h:37
-ENDSIMULATIONEVENT, PROPAGATEDSPIKE, TIMEEVENT = 0, 8, 10
+ENDSIMULATIONEVENT, STOPSIMULATIONEVENT, PROPAGATEDSPIKE, TIMEEVENT = 0, 1, 8, 10
 MAX_GROUP = 1024
+POISSON_STEP = float(np.float32(0.1) / np.float32(1000.0))
 
 
 class Heap:
@@ -107,33 +112,45 @@ class SourceModel:
             self.src_delays.setdefault(s, set()).add(d)
 
 
-def simulate(models, n_updates: int, dt_s=np.float32(1e-4), target_first: bool = True):
-    """Run the event loop; `models` are SourceModel objects updated at every
-    step like the target population.  Returns {update k: [(tgt, type, w), ...]}."""
+def simulate(models, n_updates: int, dt_s=np.float32(1e-4), target_first: bool = True, steps=None,
+             runs=None, self_model=None):
+    """Run the event loop; `models` are SourceModel objects, each updated by
+    its own update events at steps[i] seconds (default: the target's step),
+    like the target population.  `runs` splits the updates into successive
+    RunSimulationSlot calls, each stopping at the time of its last update.
+    `self_model` is the target's own recurrent source (its raster is the
+    target's spikes), emitted by the target's update events.
+    Returns {update k: [(tgt, type, w), ...]} for the target's updates."""
     step = float(np.float32(dt_s))
+    steps = [step] * len(models) if steps is None else [float(x) for x in steps]
+    runs = [n_updates] if runs is None else list(runs)
     h = Heap()
-    h.insert(1e9, ENDSIMULATIONEVENT, ("end",))
     order = ["target"] + list(range(len(models))) if target_first else list(range(len(models))) + ["target"]
     for m in order:
-        h.insert(step, TIMEEVENT, ("time", m, 1))
+        h.insert(step if m == "target" else steps[m], TIMEEVENT, ("time", m, 1))
     out = {}
-    k_target = 0
     pending = []
-    while True:
-        ev = h.remove()
-        t, prio, payload = ev
-        if payload[0] == "end":
-            break
-        if payload[0] == "time":
-            _, m, s = payload
-            if m == "target":
-                k_target = s
-                out[s] = pending
-                pending = []
-                if s == n_updates:
-                    break
-            else:
-                mod = models[m]
+    done = 0
+    for run in runs:
+        last = done + run
+        t_stop = 0.0
+        for _ in range(last):
+            t_stop += step
+        h.insert(t_stop, STOPSIMULATIONEVENT, ("stop",))
+        while True:
+            ev = h.remove()
+            t, prio, payload = ev
+            if payload[0] == "stop":
+                break
+            if payload[0] == "time":
+                _, m, s = payload
+                if m == "target":
+                    out[s] = pending
+                    pending = []
+                    if self_model is None:
+                        h.insert(t + step, TIMEEVENT, ("time", m, s + 1))
+                        continue
+                mod = models[m] if m != "target" else self_model
                 fired = mod.raster[s] if s < len(mod.raster) else []
                 groups = {d: [] for d in mod.delays}
                 for src in fired:
@@ -145,10 +162,11 @@ def simulate(models, n_updates: int, dt_s=np.float32(1e-4), target_first: bool =
                 for d in mod.delays:
                     if groups[d]:
                         h.insert(t + d, PROPAGATEDSPIKE, ("group", m, groups[d], d))
-            h.insert(t + step, TIMEEVENT, ("time", m, s + 1))
-        else:
-            _, m, srcs, d = payload
-            mod = models[m]
-            for src in srcs:
-                pending.extend(mod.syn[(src, d)])
+                h.insert(t + (step if m == "target" else steps[m]), TIMEEVENT, ("time", m, s + 1))
+            else:
+                _, m, srcs, d = payload
+                mod = models[m] if m != "target" else self_model
+                for src in srcs:
+                    pending.extend(mod.syn[(src, d)])
+        done = last
     return out
