@@ -41,6 +41,12 @@ EXPORTS = (
     "edlut_network_add_spike_train",
     "edlut_network_set_spike_train",
     "edlut_network_add_projection",
+    "edlut_network_set_source_model",
+    "edlut_network_set_model_rank",
+    "edlut_network_set_event_order",
+    "edlut_network_set_stop_time",
+    "edlut_network_set_spike_train_times",
+    "edlut_circuit_set_stop_time",
     "edlut_circuit_create",
     "edlut_circuit_run",
     "edlut_circuit_synchronize",
@@ -128,6 +134,12 @@ def _sig(lib):
         "edlut_network_add_spike_train": (I32, [P, I32, I64, P, P, P, P, P]),
         "edlut_network_set_spike_train": (I32, [P, I32, I64, P, P]),
         "edlut_network_add_projection": (I32, [P, P, P, P, P]),
+        "edlut_network_set_source_model": (I32, [P, I32, I32, I32]),
+        "edlut_network_set_model_rank": (I32, [P, I32]),
+        "edlut_network_set_event_order": (I32, [P, I32]),
+        "edlut_network_set_stop_time": (I32, [P, ctypes.c_double]),
+        "edlut_network_set_spike_train_times": (I32, [P, I32, I64, P, P, P]),
+        "edlut_circuit_set_stop_time": (I32, [P, ctypes.c_double]),
         "edlut_circuit_create": (I32, [P, I32, P]),
         "edlut_circuit_run": (I32, [P, I64, I32]),
         "edlut_circuit_synchronize": (I32, [P]),

@@ -18,6 +18,7 @@
 #include "adjoint.hpp"
 #include "generate.hpp"
 #include "kernels.hpp"
+#include "eventq.hpp"
 
 using namespace edl;
 
@@ -425,7 +426,21 @@ struct Source {
     int64_t nsyn = 0;
     int32_t *tile_count = nullptr;
     int32_t *staging = nullptr;
+    // event-queue order: the model this source belongs to (sources of several
+    // networks with the same key are one model) and its type rank; a spike
+    // train's input spikes as records in load order (time, neuron, position
+    // in its update's list, load sequence number)
+    int eq_key = -1, eq_rank = -1;
+    std::vector<double> tr_t;
+    std::vector<int32_t> tr_neuron, tr_pos;
+    std::vector<int64_t> tr_seq;
+    std::vector<int64_t> tr_load;   // 0: queued when the simulation starts; k: by the run starting at update k
+    int32_t tr_longest = 0;
+    std::vector<int64_t> h_train_ptr;   // host copy of the schedule
+    std::vector<int32_t> h_train_idx;
 };
+
+struct EqSim;
 
 struct edlut_network {
     edlut_population *pop = nullptr;
@@ -478,6 +493,19 @@ struct edlut_network {
     double *d_delay_s = nullptr;     // [max_delay + 1]
     int64_t t_cache_k = 0;           // host copy of t_k for k = t_cache_k
     double t_cache = 0.0;
+    // delivery in the order of the reference's event queue (eventq.hpp):
+    // the replayed queue of a network that runs alone, the emitter of each
+    // source and its rank tables (set when the queue is built)
+    bool eq_on = true;
+    bool eq_stale = true;            // sources changed since the queue was built
+    EqSim *eq = nullptr;
+    int model_rank = -1;
+    double stop_next = -1.0;
+    std::vector<int> src_em;
+    const uint32_t *eq_rank[kMaxSources] = {};
+    int32_t eq_cmax[kMaxSources] = {};
+    const int32_t *eq_bnd[kMaxSources] = {};
+    const uint8_t *eq_bnd_on[kMaxSources] = {};
     double time_of(int64_t k) {
         // t_k = t_{k-1} + double(float dt), t_0 = 0 (TimeEventUpdateNeurons.cpp:78)
         if (k < t_cache_k) {
@@ -488,6 +516,429 @@ struct edlut_network {
         return t_cache;
     }
 };
+
+// ---------------------------------------------------------------------------
+// The replayed event queue (eventq.hpp) of a network that runs alone or of a
+// circuit: one emitter per population, Poisson model and spike-train model.
+struct EqHostEm {
+    int kind = kEqPop, rank = 0, key = -1;
+    std::vector<std::pair<edlut_network *, int>> copies;   // (network, source) it feeds
+    EqEmitter dev{};
+    uint32_t *dmask = nullptr;
+    uint32_t *rank_tab = nullptr;
+    int32_t *bnd = nullptr;
+    uint8_t *bnd_on = nullptr;
+    double *in_t = nullptr;
+    int32_t *in_neuron = nullptr, *in_pos = nullptr, *nd_ptr = nullptr;
+    uint16_t *nd_list = nullptr;
+    int64_t *in_seq = nullptr;
+    int32_t in_loaded = 0;          // records already in the queue
+    std::vector<int32_t> h_nd_ptr;  // trains: delays per neuron (host copy of nd_ptr)
+};
+
+struct EqSim {
+    std::vector<EqHostEm> em;
+    EqEmitter *d_em = nullptr;
+    int max_delay = 1, ring = 3, words = 1;
+    const double *delay_s = nullptr;
+    EqItem *heap = nullptr;
+    int32_t *st = nullptr;
+    int32_t cap = 0;
+    uint32_t *err = nullptr;
+    int64_t next_first = 1;
+    double h_pop = 0.0;
+    int64_t t_k = 0;                // t_cache holds the time of update t_k
+    double t_cache = 0.0;
+    double time_of(int64_t k) {
+        if (k < t_k) {
+            t_k = 0;
+            t_cache = 0.0;
+        }
+        for (; t_k < k; t_k++) t_cache += h_pop;
+        return t_cache;
+    }
+    void free_all() {
+        for (EqHostEm &E : em) {
+            void *d[] = {E.dmask, E.rank_tab, E.bnd, E.bnd_on, E.in_t, E.in_neuron, E.in_pos, E.nd_ptr, E.nd_list,
+                         E.in_seq};
+            for (void *q : d)
+                if (q) (void)hipFree(q);
+        }
+        em.clear();
+        void *d[] = {d_em, heap, st};
+        for (void *q : d)
+            if (q) (void)hipFree(q);
+        d_em = nullptr;
+        heap = nullptr;
+        st = nullptr;
+    }
+    ~EqSim() { free_all(); }
+};
+
+namespace {
+
+// PoissonGeneratorDeviceVector samples every 0.1f ms of its millisecond time
+// scale: 0.1f / 1000.0f in float (PoissonGeneratorDeviceVector.cpp:26,137).
+inline double poisson_step_s() { return (double)(0.1f / 1000.0f); }
+
+int64_t g_train_seq = 0;   // load order of input spikes across spike trains
+
+// Spike-train records in load order from a step-major schedule: an input
+// spike at the time of its update unless `time` gives it (aligned with idx),
+// loaded in step-major order unless `order` ranks the entries.
+void train_records(Source &s, double dt, int64_t n_steps, const int64_t *step_ptr, const int32_t *idx,
+                   const double *time, const int64_t *order, const int64_t *load = nullptr) {
+    const int64_t total = n_steps > 0 ? step_ptr[n_steps] : 0;
+    std::vector<double> t((size_t)total);
+    std::vector<int32_t> nr((size_t)total), pos((size_t)total);
+    double tk = 0.0;
+    int32_t longest = 0;
+    for (int64_t k = 1; k <= n_steps; k++) {
+        tk += dt;
+        for (int64_t j = step_ptr[k - 1]; j < step_ptr[k]; j++) {
+            t[(size_t)j] = time ? time[j] : tk;
+            nr[(size_t)j] = idx[j];
+            pos[(size_t)j] = (int32_t)(j - step_ptr[k - 1]);
+        }
+        longest = std::max<int32_t>(longest, (int32_t)(step_ptr[k] - step_ptr[k - 1]));
+    }
+    std::vector<int64_t> perm((size_t)total);
+    for (int64_t j = 0; j < total; j++) perm[(size_t)j] = j;
+    if (order) std::stable_sort(perm.begin(), perm.end(), [&](int64_t a, int64_t b) { return order[a] < order[b]; });
+    const size_t kept = std::min(s.tr_t.size(), (size_t)total);   // records queued before stay first
+    s.tr_t.resize((size_t)total);
+    s.tr_neuron.resize((size_t)total);
+    s.tr_pos.resize((size_t)total);
+    s.tr_seq.resize((size_t)total);
+    s.tr_load.resize((size_t)total);
+    for (size_t q = kept; q < (size_t)total; q++) {
+        const int64_t j = perm[q];
+        s.tr_t[q] = t[(size_t)j];
+        s.tr_neuron[q] = nr[(size_t)j];
+        s.tr_pos[q] = pos[(size_t)j];
+        s.tr_seq[q] = order ? order[j] : g_train_seq++;
+        s.tr_load[q] = load ? load[j] : 0;
+    }
+    // entries kept from an earlier schedule keep their records, but their
+    // positions follow the new lists
+    for (size_t q = 0; q < kept; q++) {
+        for (int64_t j = 0; j < total; j++)
+            if (nr[(size_t)j] == s.tr_neuron[q] && t[(size_t)j] == s.tr_t[q]) {
+                s.tr_pos[q] = pos[(size_t)j];
+                break;
+            }
+    }
+    s.tr_longest = longest;
+}
+
+// Builds the replayed queue of `nets` (a network alone, or a circuit's
+// members in order): emitters, delay masks, rank tables, the initial heap
+// (queued input spikes, then one update event per model in type order,
+// Simulation::InitSimulation, Simulation.cpp:228-268).
+int eq_build(EqSim *S, const std::vector<edlut_network *> &nets, uint32_t *err, hipStream_t st) {
+    S->free_all();
+    const int m = (int)nets.size();
+    const int D = nets[0]->max_delay;
+    S->max_delay = D;
+    S->ring = nets[0]->ring;
+    S->words = (D + 31) / 32;
+    S->delay_s = nets[0]->d_delay_s;
+    S->err = err;
+    S->h_pop = (double)nets[0]->pop->dt_s;
+    S->t_k = 0;
+    S->t_cache = 0.0;
+    if (D > kMaxDelayEq) return fail(EDLUT_E_ARG, "the event-queue order supports delays up to %d steps", kMaxDelayEq);
+    for (int i = 0; i < m; i++) {
+        edlut_network *n = nets[i];
+        if (n->pop->grouped) return fail(EDLUT_E_STATE, "grouped populations run in the canonical order");
+        if (n->max_delay != D || n->pop->dt_s != nets[0]->pop->dt_s)
+            return fail(EDLUT_E_ARG, "networks of one event queue share max_delay and step");
+        EqHostEm E;
+        E.kind = kEqPop;
+        E.rank = n->model_rank >= 0 ? n->model_rank : i;
+        E.dev.kind = kEqPop;
+        E.dev.n = n->pop->n;
+        E.dev.h = (double)n->pop->dt_s;
+        E.dev.ring_idx = n->own_ring_idx;
+        E.dev.ring_cnt = n->own_ring_cnt;
+        E.dev.slot_len = n->pop->n;
+        S->em.push_back(E);
+    }
+    int next_key = 1 << 24;
+    for (int i = 0; i < m; i++) {
+        edlut_network *n = nets[i];
+        n->src_em.assign(n->sources.size(), -1);
+        for (size_t si = 0; si < n->sources.size(); si++) {
+            Source &s = n->sources[si];
+            int e = -1;
+            if (s.self) e = i;
+            else if (s.from) {
+                for (int j = 0; j < m; j++)
+                    if (nets[j] == s.from) e = j;
+                if (e < 0) return fail(EDLUT_E_STATE, "a projection's source network is not in the circuit");
+            } else {
+                const int key = s.eq_key >= 0 ? s.eq_key : next_key++;
+                for (size_t q = (size_t)m; q < S->em.size(); q++)
+                    if (S->em[q].key == key) e = (int)q;
+                if (e >= 0) {
+                    EqHostEm &E0 = S->em[(size_t)e];
+                    if ((E0.kind == kEqTrain) != s.train || E0.dev.n != s.n)
+                        return fail(EDLUT_E_ARG, "sources of one model must be alike");
+                } else {
+                    if ((int)S->em.size() >= kEqMaxEmitters) return fail(EDLUT_E_CAP, "at most %d models", kEqMaxEmitters);
+                    EqHostEm E;
+                    E.kind = s.train ? kEqTrain : kEqPoisson;
+                    E.key = key;
+                    E.rank = s.eq_rank >= 0 ? s.eq_rank : 1000 + (int)S->em.size();
+                    E.dev.kind = E.kind;
+                    E.dev.n = (int)s.n;
+                    E.dev.h = s.train ? 0.0 : poisson_step_s();
+                    E.dev.ring_idx = s.ring_idx;
+                    E.dev.ring_cnt = s.ring_cnt;
+                    E.dev.slot_len = s.n;
+                    S->em.push_back(E);
+                    e = (int)S->em.size() - 1;
+                }
+            }
+            if ((int)S->em.size() > kEqMaxEmitters) return fail(EDLUT_E_CAP, "at most %d models", kEqMaxEmitters);
+            S->em[(size_t)e].copies.push_back({n, (int)si});
+            n->src_em[si] = e;
+        }
+    }
+    const int W = S->words;
+    int rc;
+    for (EqHostEm &E : S->em) {
+        const int64_t nn = E.dev.n;
+        if ((rc = dalloc(&E.dmask, (size_t)nn * W))) return rc;
+        HIP_TRY(hipMemsetAsync(E.dmask, 0, sizeof(uint32_t) * nn * W, st));
+        for (auto &cp : E.copies) {
+            const Source &s = cp.first->sources[(size_t)cp.second];
+            const int nb = (int)std::min<int64_t>(4096, (nn + kBlock - 1) / kBlock);
+            hipLaunchKernelGGL(k_delay_mask, dim3(nb), dim3(kBlock), 0, st, s.row_ptr, nn, D, W, E.dmask);
+        }
+        HIP_TRY(hipGetLastError());
+        std::vector<uint32_t> h((size_t)nn * W);
+        HIP_TRY(hipMemcpyAsync(h.data(), E.dmask, sizeof(uint32_t) * h.size(), hipMemcpyDeviceToHost, st));
+        HIP_TRY(hipStreamSynchronize(st));
+        for (int w = 0; w < kEqMaskWords; w++) E.dev.full[w] = 0u;
+        for (int64_t i = 0; i < nn; i++)
+            for (int w = 0; w < W; w++) E.dev.full[w] |= h[(size_t)i * W + w];
+        E.dev.dmask = E.dmask;
+        if (E.kind == kEqTrain) {
+            const Source &s0 = E.copies[0].first->sources[(size_t)E.copies[0].second];
+            std::vector<int32_t> ptr((size_t)nn + 1, 0);
+            std::vector<uint16_t> lst;
+            for (int64_t i = 0; i < nn; i++) {
+                for (int d = 1; d <= D; d++)
+                    if ((h[(size_t)i * W + (d - 1) / 32] >> ((d - 1) % 32)) & 1u) lst.push_back((uint16_t)d);
+                ptr[(size_t)i + 1] = (int32_t)lst.size();
+            }
+            if (lst.empty()) lst.push_back(0);
+            E.h_nd_ptr = ptr;
+            const size_t nr = std::max<size_t>(s0.tr_t.size(), 1);
+            if ((rc = dalloc(&E.nd_ptr, ptr.size())) || (rc = dalloc(&E.nd_list, lst.size())) ||
+                (rc = dalloc(&E.in_t, nr)) || (rc = dalloc(&E.in_neuron, nr)) || (rc = dalloc(&E.in_pos, nr)) ||
+                (rc = dalloc(&E.in_seq, nr)))
+                return rc;
+            HIP_TRY(hipMemcpy(E.nd_ptr, ptr.data(), sizeof(int32_t) * ptr.size(), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(E.nd_list, lst.data(), sizeof(uint16_t) * lst.size(), hipMemcpyHostToDevice));
+            if (!s0.tr_t.empty()) {
+                HIP_TRY(hipMemcpy(E.in_t, s0.tr_t.data(), sizeof(double) * s0.tr_t.size(), hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(E.in_neuron, s0.tr_neuron.data(), sizeof(int32_t) * s0.tr_t.size(),
+                                  hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(E.in_pos, s0.tr_pos.data(), sizeof(int32_t) * s0.tr_t.size(), hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(E.in_seq, s0.tr_seq.data(), sizeof(int64_t) * s0.tr_t.size(), hipMemcpyHostToDevice));
+            }
+            E.dev.in_t = E.in_t;
+            E.dev.in_neuron = E.in_neuron;
+            E.dev.in_pos = E.in_pos;
+            E.dev.in_seq = E.in_seq;
+            E.dev.nd_ptr = E.nd_ptr;
+            E.dev.nd_list = E.nd_list;
+            E.dev.cmax = std::max<int32_t>(1, s0.tr_longest);
+        } else {
+            E.dev.cmax = (int)std::max<int64_t>(1, (nn + kEqChunk - 1) / kEqChunk);
+        }
+        if (E.dev.cmax >= (1 << kOrdChunkBits)) return fail(EDLUT_E_CAP, "too many spike groups per update");
+        const size_t nrank = (size_t)2 * D * E.dev.cmax, nbnd = (size_t)S->ring * D * E.dev.cmax;
+        if ((rc = dalloc(&E.rank_tab, nrank)) || (rc = dalloc(&E.bnd, nbnd)) || (rc = dalloc(&E.bnd_on, (size_t)S->ring)))
+            return rc;
+        HIP_TRY(hipMemsetAsync(E.rank_tab, 0, sizeof(uint32_t) * nrank, st));
+        HIP_TRY(hipMemsetAsync(E.bnd_on, 0, (size_t)S->ring, st));
+        E.dev.rank = E.rank_tab;
+        E.dev.bnd = E.bnd;
+        E.dev.bnd_on = E.bnd_on;
+    }
+    // rank tables and chunk boundaries of every source's model
+    for (edlut_network *n : nets)
+        for (size_t si = 0; si < n->sources.size(); si++) {
+            const EqHostEm &E = S->em[(size_t)n->src_em[si]];
+            n->eq_rank[si] = E.rank_tab;
+            n->eq_cmax[si] = E.dev.cmax;
+            n->eq_bnd[si] = E.bnd;
+            n->eq_bnd_on[si] = E.bnd_on;
+        }
+    // heap capacity: the groups of every model for every update in flight,
+    // the input spikes, the update and stop events
+    int64_t cap = 256;
+    for (EqHostEm &E : S->em) {
+        cap += (int64_t)D * (S->ring + 2) * (E.kind == kEqTrain ? 1 : E.dev.cmax + 1);
+        if (E.kind == kEqTrain) {
+            const Source &s0 = E.copies[0].first->sources[(size_t)E.copies[0].second];
+            cap += (int64_t)s0.tr_t.size();
+        }
+    }
+    if (cap > (1ll << 28)) return fail(EDLUT_E_CAP, "event queue too large");
+    S->cap = (int32_t)cap;
+    if ((rc = dalloc(&S->heap, (size_t)cap + 1)) || (rc = dalloc(&S->st, 3 + S->em.size())) ||
+        (rc = dalloc(&S->d_em, S->em.size())))
+        return rc;
+    // initial heap: the input spikes queued before the first run in load
+    // order, then the update events
+    std::vector<EqItem> h((size_t)cap + 1);
+    int32_t hn = 0;
+    {
+        std::vector<size_t> cur(S->em.size(), 0);
+        for (;;) {
+            int be = -1;
+            int64_t bs = 0;
+            for (size_t e = 0; e < S->em.size(); e++) {
+                EqHostEm &E = S->em[e];
+                if (E.kind != kEqTrain) continue;
+                const Source &s0 = E.copies[0].first->sources[(size_t)E.copies[0].second];
+                if (cur[e] >= s0.tr_t.size()) continue;
+                if (s0.tr_load[cur[e]] != 0) continue;
+                if (be < 0 || s0.tr_seq[cur[e]] < bs) {
+                    be = (int)e;
+                    bs = s0.tr_seq[cur[e]];
+                }
+            }
+            if (be < 0) break;
+            EqHostEm &E = S->em[(size_t)be];
+            const Source &s0 = E.copies[0].first->sources[(size_t)E.copies[0].second];
+            const size_t q = cur[(size_t)be]++;
+            const int32_t nr = s0.tr_neuron[q];
+            // a neuron without output synapses propagates nothing
+            if (E.h_nd_ptr[(size_t)nr + 1] > E.h_nd_ptr[(size_t)nr])
+                eq_insert(h.data(), &hn, eq_item(s0.tr_t[q], kEqPrioSpike, kEqInput, (uint32_t)be, 0, (uint32_t)q));
+        }
+        for (size_t e = 0; e < S->em.size(); e++) {
+            EqHostEm &E = S->em[e];
+            if (E.kind != kEqTrain) continue;
+            const Source &s0 = E.copies[0].first->sources[(size_t)E.copies[0].second];
+            int32_t q = 0;
+            while (q < (int32_t)s0.tr_t.size() && s0.tr_load[(size_t)q] == 0) q++;
+            E.in_loaded = q;
+        }
+    }
+    std::vector<int> order;
+    for (size_t e = 0; e < S->em.size(); e++)
+        if (S->em[e].kind != kEqTrain) order.push_back((int)e);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return S->em[(size_t)a].rank < S->em[(size_t)b].rank; });
+    for (int e : order) eq_insert(h.data(), &hn, eq_item(S->em[(size_t)e].dev.h, kEqPrioTime, kEqTime, (uint32_t)e, 0, 0));
+    HIP_TRY(hipMemcpy(S->heap, h.data(), sizeof(EqItem) * ((size_t)hn + 1), hipMemcpyHostToDevice));
+    std::vector<int32_t> stv(3 + S->em.size(), 1);
+    stv[0] = hn;
+    stv[1] = 0;
+    stv[2] = 0;
+    HIP_TRY(hipMemcpy(S->st, stv.data(), sizeof(int32_t) * stv.size(), hipMemcpyHostToDevice));
+    std::vector<EqEmitter> dev;
+    for (EqHostEm &E : S->em) {
+        E.dev.in_lo = E.dev.in_hi = 0;
+        dev.push_back(E.dev);
+    }
+    HIP_TRY(hipMemcpy(S->d_em, dev.data(), sizeof(EqEmitter) * dev.size(), hipMemcpyHostToDevice));
+    S->next_first = 1;
+    return 0;
+}
+
+EqArgs eq_args(const EqSim *S, StepRef step, int resume, double stop_t) {
+    EqArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.em = S->d_em;
+    a.n_em = (int)S->em.size();
+    a.words = S->words;
+    a.max_delay = S->max_delay;
+    a.ring = S->ring;
+    a.delay_s = S->delay_s;
+    a.heap = S->heap;
+    a.st = S->st;
+    a.cap = S->cap;
+    a.stop_t = stop_t;
+    a.resume = resume;
+    a.step = step;
+    a.err = S->err;
+    return a;
+}
+
+// Before a run of updates [first, first + n_steps): input spikes added to
+// the trains since the queue last saw them are uploaded, then the resuming
+// replay queues them and the run's stop event and finishes the window
+// before update `first`.
+int eq_resume(EqSim *S, int64_t first, int n_steps, double stop, hipStream_t st) {
+    if (first != S->next_first) return fail(EDLUT_E_STATE, "the event-queue order needs runs that continue each other");
+    const int64_t last = first + n_steps - 1;
+    const double t_last = S->time_of(last);
+    const double t_next = t_last + S->h_pop;
+    const bool with_stop = stop != -2.0;   // -2: the run continues into the next one (no stop event)
+    if (stop < 0) stop = t_last;
+    else if (!(stop >= t_last && stop < t_next))
+        return fail(EDLUT_E_ARG, "stop time must lie between the run's last update and the next one");
+    bool changed = false;
+    for (EqHostEm &E : S->em) {
+        if (E.kind != kEqTrain) continue;
+        const Source &s0 = E.copies[0].first->sources[(size_t)E.copies[0].second];
+        const int32_t total = (int32_t)s0.tr_t.size();
+        if (total < E.in_loaded) return fail(EDLUT_E_STATE, "input spikes already queued were removed from a train");
+        int32_t hi = E.in_loaded;
+        while (hi < total && s0.tr_load[(size_t)hi] <= first) hi++;
+        // re-upload the records (and positions, which follow the lists)
+        int rc;
+        if (total > 0) {
+            if (E.in_t) (void)hipFree(E.in_t);
+            if (E.in_neuron) (void)hipFree(E.in_neuron);
+            if (E.in_pos) (void)hipFree(E.in_pos);
+            if (E.in_seq) (void)hipFree(E.in_seq);
+            E.in_t = nullptr;
+            E.in_neuron = E.in_pos = nullptr;
+            E.in_seq = nullptr;
+            if ((rc = dalloc(&E.in_t, (size_t)total)) || (rc = dalloc(&E.in_neuron, (size_t)total)) ||
+                (rc = dalloc(&E.in_pos, (size_t)total)) || (rc = dalloc(&E.in_seq, (size_t)total)))
+                return rc;
+            HIP_TRY(hipMemcpy(E.in_t, s0.tr_t.data(), sizeof(double) * total, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(E.in_neuron, s0.tr_neuron.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(E.in_pos, s0.tr_pos.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(E.in_seq, s0.tr_seq.data(), sizeof(int64_t) * total, hipMemcpyHostToDevice));
+        }
+        if (s0.tr_longest > E.dev.cmax) return fail(EDLUT_E_CAP, "a train's update lists outgrew its rank table");
+        E.dev.in_t = E.in_t;
+        E.dev.in_neuron = E.in_neuron;
+        E.dev.in_pos = E.in_pos;
+        E.dev.in_seq = E.in_seq;
+        E.dev.in_lo = E.in_loaded;
+        E.dev.in_hi = hi;
+        E.in_loaded = hi;
+        changed = true;
+    }
+    std::vector<EqEmitter> dev, dev2;
+    for (EqHostEm &E : S->em) dev.push_back(E.dev);
+    HIP_TRY(hipMemcpy(S->d_em, dev.data(), sizeof(EqEmitter) * dev.size(), hipMemcpyHostToDevice));
+    EqArgs ra = eq_args(S, StepRef{nullptr, (int32_t)(first - 1)}, 1, stop);
+    ra.with_stop = with_stop ? 1 : 0;
+    hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, st, ra);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+    if (changed) {
+        // the run's own replays queue no input spikes
+        for (EqHostEm &E : S->em) E.dev.in_lo = E.dev.in_hi = 0;
+        for (EqHostEm &E : S->em) dev2.push_back(E.dev);
+        HIP_TRY(hipMemcpy(S->d_em, dev2.data(), sizeof(EqEmitter) * dev2.size(), hipMemcpyHostToDevice));
+    }
+    S->next_first = last + 1;
+    return 0;
+}
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Kernel dispatch tables.
@@ -1514,6 +1965,7 @@ int edlut_network_destroy(edlut_network *n) {
                    n->log_idx, n->log_count, n->d_step, n->d_time, n->d_delay_s, n->seg_off, n->clk};
     for (void *q : dev)
         if (q) (void)hipFree(q);
+    delete n->eq;
     n->pop->net = nullptr;
     delete n;
     return 0;
@@ -1538,6 +1990,8 @@ static int add_synapses(edlut_network *n, Source &s, const int64_t *row_ptr,
     }
     // CheckSynapseType (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:239-262)
     n->pop->mask |= types;
+    if (nsyn >= (int64_t)(1ull << kOrdSynBits)) return fail(EDLUT_E_ARG, "too many synapses in one source");
+    n->eq_stale = true;
     int rc;
     if ((rc = dalloc(&s.row_ptr, nrow)) || (rc = dalloc(&s.tgt_type, nsyn)) || (rc = dalloc(&s.w, nsyn)))
         return rc;
@@ -1577,6 +2031,10 @@ int edlut_network_add_poisson(edlut_network *n, int n_src, const uint32_t *lcg_s
     if (n_src <= 0) return fail(EDLUT_E_ARG, "n_src must be positive");
     if ((int)n->sources.size() >= kMaxSources) return fail(EDLUT_E_CAP, "too many sources");
     if (!std::isfinite(rate_hz)) return fail(EDLUT_E_ARG, "rate must be finite");
+    // PoissonGeneratorDeviceVector samples every 0.1 ms: exactly one of its
+    // updates falls between two population updates only at that step
+    if (n->pop->dt_s != 1e-4f)
+        return fail(EDLUT_E_ARG, "Poisson devices sample every 0.1 ms: the population's step must be 0.1 ms");
     HIP_TRY(hipSetDevice(n->pop->device));
     Source s;
     s.self = false;
@@ -1644,6 +2102,10 @@ static int load_train(edlut_network *n, Source &s, int64_t n_steps, const int64_
     }
     s.train_steps = n_steps;
     s.train_blocks = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (longest + kBlock - 1) / kBlock));
+    s.h_train_ptr.assign(step_ptr, step_ptr + (n_steps > 0 ? n_steps + 1 : 0));
+    s.h_train_idx.assign(idx, idx + total);
+    if (s.h_train_ptr.empty()) s.h_train_ptr.push_back(0);
+    train_records(s, (double)n->pop->dt_s, n_steps, s.h_train_ptr.data(), s.h_train_idx.data(), nullptr, nullptr);
     n->dirty = true;   // captured kernels carry the train's arrays
     return 0;
 }
@@ -1720,6 +2182,74 @@ int edlut_network_set_poisson_rates(edlut_network *n, int source, const float *r
     return 0;
 }
 
+int edlut_network_set_source_model(edlut_network *n, int source, int key, int rank) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    Source &s = n->sources[(size_t)source];
+    if (s.self || s.from) return fail(EDLUT_E_ARG, "source %d is a population's own spikes", source);
+    if (key < 0 || key >= (1 << 24)) return fail(EDLUT_E_ARG, "model key out of range");
+    s.eq_key = key;
+    s.eq_rank = rank;
+    n->eq_stale = true;
+    return 0;
+}
+
+int edlut_network_set_model_rank(edlut_network *n, int rank) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    n->model_rank = rank;
+    n->eq_stale = true;
+    return 0;
+}
+
+int edlut_network_set_event_order(edlut_network *n, int on) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if ((n->eq_on != (on != 0))) {
+        n->eq_on = on != 0;
+        n->eq_stale = true;
+        n->dirty = true;
+    }
+    return 0;
+}
+
+int edlut_network_set_stop_time(edlut_network *n, double t) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    n->stop_next = t;
+    return 0;
+}
+
+int edlut_network_set_spike_train_times(edlut_network *n, int source, int64_t n_spikes, const double *time,
+                                        const int64_t *order, const int64_t *load_first) {
+    if (!n) return fail(EDLUT_E_ARG, "network is null");
+    if (source < 0 || source >= (int)n->sources.size()) return fail(EDLUT_E_ARG, "no source %d", source);
+    Source &s = n->sources[(size_t)source];
+    if (!s.train) return fail(EDLUT_E_ARG, "source %d is not a spike train", source);
+    if (n_spikes != (int64_t)s.h_train_idx.size())
+        return fail(EDLUT_E_ARG, "the schedule has %lld spikes", (long long)s.h_train_idx.size());
+    if (n_spikes > 0 && !time) return fail(EDLUT_E_ARG, "time is null");
+    for (int64_t j = 0; j < n_spikes; j++)
+        if (!std::isfinite(time[j])) return fail(EDLUT_E_ARG, "spike %lld: time is not finite", (long long)j);
+    if (order) {
+        std::vector<int64_t> o(order, order + n_spikes);
+        std::sort(o.begin(), o.end());
+        for (int64_t j = 1; j < n_spikes; j++)
+            if (o[(size_t)j] == o[(size_t)j - 1]) return fail(EDLUT_E_ARG, "load order has repeats");
+    }
+    s.tr_t.clear();   // every record is rebuilt (those already queued keep their place)
+    const size_t queued = s.tr_seq.size();
+    std::vector<int64_t> seq = s.tr_seq;
+    if (load_first)
+        for (int64_t j = 0; j < n_spikes; j++)
+            if (load_first[j] < 0) return fail(EDLUT_E_ARG, "spike %lld: load update is negative", (long long)j);
+    train_records(s, (double)n->pop->dt_s, s.train_steps, s.h_train_ptr.data(), s.h_train_idx.data(), time, order,
+                  load_first);
+    for (size_t q = 1; q < s.tr_load.size(); q++)
+        if (s.tr_load[q] < s.tr_load[q - 1])
+            return fail(EDLUT_E_ARG, "input spikes must be loaded in the order of their load updates");
+    // records already queued keep their sequence numbers
+    for (size_t q = 0; q < queued && q < s.tr_seq.size(); q++) s.tr_seq[q] = seq[q];
+    return 0;
+}
+
 int edlut_network_set_external(edlut_network *n, int n_thr, const uint32_t *thresholds, float w_ext,
                                uint64_t seed) {
     if (!n) return fail(EDLUT_E_ARG, "network is null");
@@ -1742,8 +2272,29 @@ int edlut_network_set_external(edlut_network *n, int n_thr, const uint32_t *thre
 // fold_done: the fold of update k's events already ran (fused into the
 // previous step's compaction).  Returns through *fused whether this step's
 // compaction also folded the events of update k + 1.
+// The fold target of a network, with the rank tables of the event-queue order
+// when the network runs in that order.
+static FoldTarget net_fold_target(const edlut_network *n) {
+    FoldTarget ft = n->pop->fold_target(0, true);
+    ft.seg_off = n->seg_off;
+    for (int q = 0; q < kMaxSources; q++) {
+        ft.rank[q] = nullptr;
+        ft.rank_cmax[q] = 0;
+    }
+    ft.rank_delays = n->max_delay;
+    if (n->eq_on && !n->pop->grouped && !n->sources.empty() && n->eq_rank[0]) {
+        for (size_t q = 0; q < n->sources.size(); q++) {
+            ft.rank[q] = n->eq_rank[q];
+            ft.rank_cmax[q] = n->eq_cmax[q];
+        }
+    }
+    return ft;
+}
+
+// eq: the replayed event queue to advance after this step (a network that
+// runs alone), or null.
 static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingEvents &re0,
-                        bool fold_done, bool *fused) {
+                        bool fold_done, bool *fused, const EqSim *eq = nullptr) {
     edlut_population *p = n->pop;
     if (n->plastic)
         hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, p->stream, step, n->d_time, n->ring, (double)p->dt_s);
@@ -1770,15 +2321,14 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         pa.n_tiles = (int)((s.n + kBlock - 1) / kBlock);
         pa.rate = s.rate;
         pa.rates = s.rates;
-        pa.step_s = (double)p->dt_s;
+        pa.step_s = poisson_step_s();
         pa.out = TileOut{s.tile_count, s.staging};
         hipLaunchKernelGGL(k_poisson, dim3(pa.n_tiles), dim3(kBlock), 0, p->stream, pa);
         launch_compact(p->stream, s.tile_count, s.staging, pa.n_tiles, kBlock, s.ring_idx, s.ring_cnt,
                        n->ring, s.n, s.log_step, s.log_idx, s.log_count, s.log_cap, p->err, step);
     }
     // Deliver the events consumed by update k into their keys' segments.
-    FoldTarget ft = p->fold_target(0, true);
-    ft.seg_off = n->seg_off;
+    FoldTarget ft = net_fold_target(n);
     const bool ring = re0.n_sources > 0;
     if (ring && !fold_done) {
         RingEvents re = re0;
@@ -1832,6 +2382,7 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         pa.exp_lut = p->exp_lut;
         hipLaunchKernelGGL(k_stdp_post<64>, dim3(64), dim3(kBlock), 0, p->stream, pa);
     }
+    if (eq) hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, p->stream, eq_args(eq, step, 0, 0.0));
     return 0;
 }
 
@@ -1847,12 +2398,33 @@ static RingEvents ring_events(edlut_network *n) {
     for (int q = 0; q < re.n_sources; q++) {
         const Source &s = n->sources[q];
         re.src[q] = RingSource{s.ring_idx, s.ring_cnt, s.row_ptr, s.tgt_type, s.w, s.n,
-                               s.stdp ? 1 : 0, s.syn};
+                               s.stdp ? 1 : 0, s.syn, s.train ? 1 : 0, nullptr, nullptr, 1};
+        if (n->eq_on && !n->pop->grouped && n->eq_rank[q]) {
+            re.src[q].bnd = n->eq_bnd[q];
+            re.src[q].bnd_on = n->eq_bnd_on[q];
+            re.src[q].cmax = n->eq_cmax[q];
+        }
     }
     return re;
 }
 
 static constexpr int kGraphChunk = 16;
+
+// The replayed event queue of a network that runs alone, built at update 1
+// (and whenever a run restarts there).
+static int eq_prepare_net(edlut_network *n, int64_t first) {
+    if (!n->eq) n->eq = new EqSim;
+    if (first == 1 || n->eq->em.empty() || n->eq_stale) {
+        if (first != 1) return fail(EDLUT_E_STATE, "sources changed: the event queue restarts at update 1 (reset)");
+        int rc = eq_build(n->eq, {n}, n->pop->err, n->pop->stream);
+        if (rc) return rc;
+        n->eq_stale = false;
+        n->dirty = true;
+    }
+    return 0;
+}
+
+static EqSim *net_eq(edlut_network *n) { return (n->eq_on && !n->pop->grouped && !n->circuit) ? n->eq : nullptr; }
 
 // Captures kGraphChunk steps; step i of the chunk reads its update index as
 // d_step + i, and the chunk's last node advances d_step by kGraphChunk.
@@ -1871,7 +2443,7 @@ static int build_graph(edlut_network *n, UpdateFn fn) {
     const bool will_fuse = re.n_sources > 0 && !n->plastic;
     bool fused = will_fuse;
     for (int i = 0; i < kGraphChunk && rc == 0; i++)
-        rc = enqueue_step(n, StepRef{n->d_step, i}, fn, re, fused, &fused);
+        rc = enqueue_step(n, StepRef{n->d_step, i}, fn, re, fused, &fused, net_eq(n));
     n->graph_fused = will_fuse;
     hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, p->stream, n->d_step, (int32_t)kGraphChunk);
     hipError_t e = hipStreamEndCapture(p->stream, &g);
@@ -1995,9 +2567,7 @@ static int fold_ahead(edlut_network *n, const RingEvents &re, int64_t k) {
     if (!n->graph_fused || n->fold_ready == k) return 0;
     RingEvents rk = re;
     rk.step = StepRef{nullptr, (int32_t)k};
-    FoldTarget ft = n->pop->fold_target(0, true);
-    ft.seg_off = n->seg_off;
-    return launch_ring_fold(ft, n->pop->stream, rk, n->fold_lanes, n->fold_blocks);
+    return launch_ring_fold(net_fold_target(n), n->pop->stream, rk, n->fold_lanes, n->fold_blocks);
 }
 
 static void run_finish(edlut_network *n, int64_t first_step, int n_steps) {
@@ -2017,6 +2587,11 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     UpdateFn fn = nullptr;
     int rc = run_prepare(n, first_step, n_steps, &fn);
     if (rc) return rc;
+    if (n->eq_on && !p->grouped) {
+        if ((rc = eq_prepare_net(n, first_step)) || (rc = eq_resume(n->eq, first_step, n_steps, n->stop_next, p->stream)))
+            return rc;
+        n->stop_next = -1.0;
+    }
     const RingEvents re = ring_events(n);
     int64_t k = first_step;
     if (n->use_graph && n_steps >= kGraphChunk) {
@@ -2035,7 +2610,8 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     }
     for (; k < first_step + n_steps; k++) {
         bool fused = false;
-        if ((rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, n->fold_ready == k, &fused))) return rc;
+        if ((rc = enqueue_step(n, StepRef{nullptr, (int32_t)k}, fn, re, n->fold_ready == k, &fused, net_eq(n))))
+            return rc;
         n->fold_ready = fused ? k + 1 : -1;
     }
     HIP_TRY(hipGetLastError());
@@ -2055,6 +2631,11 @@ struct edlut_circuit {
     int32_t *d_step = nullptr;
     hipGraphExec_t graph = nullptr;
     std::vector<UpdateFn> graph_fns;
+    // delivery in the event queue's order: one replayed queue for all members
+    EqSim *eq = nullptr;
+    bool eq_on = true;
+    bool eq_stale = true;
+    double stop_next = -1.0;
 };
 
 int edlut_circuit_create(edlut_network *const *nets, int n_nets, edlut_circuit **out) {
@@ -2112,6 +2693,7 @@ int edlut_circuit_destroy(edlut_circuit *c) {
     }
     if (c->graph) (void)hipGraphExecDestroy(c->graph);
     if (c->d_step) (void)hipFree(c->d_step);
+    delete c->eq;
     (void)hipStreamDestroy(c->stream);
     delete c;
     return 0;
@@ -2119,7 +2701,7 @@ int edlut_circuit_destroy(edlut_circuit *c) {
 
 // Captures kGraphChunk lockstep steps of every member network.
 static int build_circuit_graph(edlut_circuit *c, const std::vector<UpdateFn> &fns,
-                               const std::vector<RingEvents> &res) {
+                               const std::vector<RingEvents> &res, const EqSim *eq) {
     if (c->graph) {
         (void)hipGraphExecDestroy(c->graph);
         c->graph = nullptr;
@@ -2130,12 +2712,14 @@ static int build_circuit_graph(edlut_circuit *c, const std::vector<UpdateFn> &fn
     hipGraph_t g = nullptr;
     HIP_TRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
     int rc = 0;
-    for (int i = 0; i < kGraphChunk && rc == 0; i++)
+    for (int i = 0; i < kGraphChunk && rc == 0; i++) {
         for (size_t j = 0; j < m && rc == 0; j++) {
             bool f = fused[j];
             rc = enqueue_step(c->nets[j], StepRef{c->d_step, i}, fns[j], res[j], f, &f);
             fused[j] = f;
         }
+        if (eq) hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, c->stream, eq_args(eq, StepRef{c->d_step, i}, 0, 0.0));
+    }
     hipLaunchKernelGGL(k_advance_step, dim3(1), dim3(64), 0, c->stream, c->d_step, (int32_t)kGraphChunk);
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (rc) {
@@ -2178,8 +2762,34 @@ int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps) {
     rebuild = rebuild || fns != c->graph_fns;
     int64_t k = first_step;
     int rc = 0;
+    bool grouped = false;
+    for (edlut_network *n : c->nets) grouped = grouped || n->pop->grouped;
+    bool all_on = true;
+    for (edlut_network *n : c->nets) all_on = all_on && n->eq_on;
+    if (all_on != c->eq_on) {
+        c->eq_on = all_on;
+        c->eq_stale = true;
+        rebuild = true;
+    }
+    const bool eq_on = c->eq_on && !grouped;
+    if (eq_on) {
+        if (!c->eq) c->eq = new EqSim;
+        bool stale = c->eq_stale || c->eq->em.empty();
+        for (edlut_network *n : c->nets) stale = stale || n->eq_stale;
+        if (first_step == 1 || stale) {
+            if (first_step != 1) return fail(EDLUT_E_STATE, "sources changed: the event queue restarts at update 1 (reset)");
+            if ((rc = eq_build(c->eq, c->nets, c->nets[0]->pop->err, c->stream))) return rc;
+            c->eq_stale = false;
+            for (edlut_network *n : c->nets) n->eq_stale = false;
+            rebuild = true;
+            for (size_t j = 0; j < m; j++) res[j] = ring_events(c->nets[j]);
+        }
+        if ((rc = eq_resume(c->eq, first_step, n_steps, c->stop_next, c->stream))) return rc;
+        c->stop_next = -1.0;
+    }
+    const EqSim *eq = eq_on ? c->eq : nullptr;
     if (use_graph && n_steps >= kGraphChunk) {
-        if (rebuild && (rc = build_circuit_graph(c, fns, res))) return rc;
+        if (rebuild && (rc = build_circuit_graph(c, fns, res, eq))) return rc;
         for (size_t j = 0; j < m; j++)
             if ((rc = fold_ahead(c->nets[j], res[j], k))) return rc;
         int32_t k32 = (int32_t)k;
@@ -2191,7 +2801,7 @@ int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps) {
         }
         for (size_t j = 0; j < m; j++) c->nets[j]->fold_ready = c->nets[j]->graph_fused ? k : -1;
     }
-    for (; k < first_step + n_steps; k++)
+    for (; k < first_step + n_steps; k++) {
         for (size_t j = 0; j < m; j++) {
             edlut_network *n = c->nets[j];
             bool fused = false;
@@ -2199,8 +2809,16 @@ int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps) {
                 return rc;
             n->fold_ready = fused ? k + 1 : -1;
         }
+        if (eq) hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, c->stream, eq_args(eq, StepRef{nullptr, (int32_t)k}, 0, 0.0));
+    }
     HIP_TRY(hipGetLastError());
     for (size_t j = 0; j < m; j++) run_finish(c->nets[j], first_step, n_steps);
+    return 0;
+}
+
+int edlut_circuit_set_stop_time(edlut_circuit *c, double t) {
+    if (!c) return fail(EDLUT_E_ARG, "circuit is null");
+    c->stop_next = t;
     return 0;
 }
 

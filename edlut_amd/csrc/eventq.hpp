@@ -1,0 +1,388 @@
+// eventq.hpp -- the order in which the reference's event queue delivers a
+// step's spike groups, replayed on the device.
+//
+// The reference runs one binary heap of events (EventQueue.cpp:121-198)
+// ordered by time, then EventPriority (Event.h:37); equal keys leave in the
+// order the heap's layout gives.  Every time-driven model has a stream of
+// update events at its own step; an update gathers the model's spiking
+// neurons, in ascending index, into one PropagatedSpikeGroup per delay of the
+// model's delay structure, queueing a group as soon as it holds 1,024 sources
+// and the rest afterwards in ascending delay (TimeDrivenInternalSpike.cpp
+// 66-104,163-178, PropagatedSpikeGroup.h:54).  Input neurons' spikes are
+// InputSpike events that chain one PropagatedSpike per delay of the neuron
+// (InputSpike.cpp, PropagatedSpike.cpp:119-160).  A run is RunSimulationSlot
+// (Simulation.cpp:322-373): a stop event at the stop time, then events until
+// it leaves.
+//
+// k_eventq replays exactly these heap operations, once per update k, after
+// the spikes of update k are compacted: it pops the events that reach the
+// populations between update k and update k + 1 and gives each group (or
+// propagated input spike) its rank in that window.  The fold of update k + 1
+// sorts each target's events by (rank, synapse), which is the reference's
+// delivery order.  The heap holds only keys and small payloads, so one lane
+// runs it; the other lanes count the step's spikes per delay first.
+#pragma once
+#include "device_common.hpp"
+
+namespace edl {
+
+constexpr uint32_t kEqPrioStop = 1, kEqPrioSpike = 8, kEqPrioTime = 10;
+enum EqKind : uint32_t { kEqTime = 0, kEqGroup = 1, kEqInput = 2, kEqProp = 3, kEqStop = 4 };
+enum EqEmitterKind : int { kEqPop = 0, kEqPoisson = 1, kEqTrain = 2 };
+constexpr int kEqMaxEmitters = 16;
+constexpr int kEqMaskWords = 8;          // delays up to 256 steps
+constexpr int kEqChunk = 1024;           // PropagatedSpikeGroup::MaxSize
+constexpr int kEqLdsItems = 2048;        // heap entries staged in LDS (larger heaps stay in memory)
+
+// Heap entry: time, then a = priority<<28 | kind<<24 | emitter<<16 | d (or j),
+// b = chunk (groups) or input record (input and propagated spikes).
+struct EqItem {
+    double t;
+    uint32_t a;
+    uint32_t b;
+};
+
+__host__ __device__ inline EqItem eq_item(double t, uint32_t prio, uint32_t kind, uint32_t em, uint32_t d, uint32_t b) {
+    return EqItem{t, (prio << 28) | (kind << 24) | (em << 16) | (d & 0xffffu), b};
+}
+__host__ __device__ inline uint32_t eq_prio(const EqItem &x) { return x.a >> 28; }
+__host__ __device__ inline uint32_t eq_kind(const EqItem &x) { return (x.a >> 24) & 0xfu; }
+__host__ __device__ inline uint32_t eq_em(const EqItem &x) { return (x.a >> 16) & 0xffu; }
+__host__ __device__ inline uint32_t eq_d(const EqItem &x) { return x.a & 0xffffu; }
+
+// EventQueue::InsertEvent (EventQueue.cpp:121-141) on a 1-based array of
+// `*n` entries.
+__host__ __device__ inline void eq_insert(EqItem *h, int32_t *n, const EqItem &x) {
+    int32_t c = ++*n;
+    h[c] = x;
+    while (c > 1) {
+        const EqItem &p = h[c / 2];
+        const EqItem &q = h[c];
+        if (p.t > q.t || (p.t == q.t && eq_prio(p) < eq_prio(q))) {
+            const EqItem tmp = h[c / 2];
+            h[c / 2] = h[c];
+            h[c] = tmp;
+            c /= 2;
+        } else {
+            break;
+        }
+    }
+}
+
+// EventQueue::RemoveEvent (EventQueue.cpp:162-198).
+__host__ __device__ inline bool eq_remove(EqItem *h, int32_t *n, EqItem *out) {
+    if (*n >= 2) {
+        *out = h[1];
+        h[1] = h[*n];
+        --*n;
+        const int32_t size = *n;
+        int32_t p = 1, c;
+        for (c = 2; c < size; p = c, c = p * 2) {
+            if (h[c].t > h[c + 1].t || (h[c].t == h[c + 1].t && eq_prio(h[c]) < eq_prio(h[c + 1]))) c++;
+            if (h[c].t < h[p].t || (h[c].t == h[p].t && eq_prio(h[c]) > eq_prio(h[p]))) {
+                const EqItem tmp = h[p];
+                h[p] = h[c];
+                h[c] = tmp;
+            } else {
+                break;
+            }
+        }
+        if (c == size && (h[c].t < h[p].t || (h[c].t == h[p].t && eq_prio(h[c]) > eq_prio(h[p])))) {
+            const EqItem tmp = h[p];
+            h[p] = h[c];
+            h[c] = tmp;
+        }
+        return true;
+    }
+    if (*n == 1) {
+        *out = h[1];
+        *n = 0;
+        return true;
+    }
+    return false;
+}
+
+// One model whose events the replay follows.
+struct EqEmitter {
+    int kind;                  // EqEmitterKind
+    int n;                     // neurons
+    double h;                  // update step (s): populations double(float dt), Poisson 0.1f/1000.0f
+    const int32_t *ring_idx;   // spike lists by update slot (populations, Poisson devices)
+    const int32_t *ring_cnt;
+    int64_t slot_len;
+    const uint32_t *dmask;     // [n][words]: bit d-1 set when the neuron has synapses of delay d
+    uint32_t full[kEqMaskWords];   // the model's delay structure (union of the neurons')
+    int cmax;                  // rank-table entries per delay (chunks, or input spikes per update)
+    uint32_t *rank;            // [2][max_delay][cmax]: rank in its window, by consuming-update parity
+    int32_t *bnd;              // [ring][max_delay][cmax]: list position where chunk c starts
+    uint8_t *bnd_on;           // [ring]: the slot's chunks are in bnd (else every 1,024 list entries)
+    // input spikes (trains): records in load order
+    const double *in_t;
+    const int32_t *in_neuron;
+    const int32_t *in_pos;     // position in its update's list
+    const int32_t *nd_ptr;     // per neuron: its delays (ascending), CSR into nd_list
+    const uint16_t *nd_list;
+    const int64_t *in_seq;     // load sequence number of each record (queue insertion order)
+    int32_t in_lo, in_hi;      // records inserted by a resuming call
+};
+
+struct EqArgs {
+    const EqEmitter *em;       // [n_em] in device memory
+    int n_em;
+    int words;                 // mask words per neuron
+    int max_delay;
+    int ring;
+    const double *delay_s;     // [max_delay + 1] RoundPropagationDelay of d steps
+    EqItem *heap;              // [cap + 1], 1-based
+    int32_t *st;               // [0] heap size, [1] rank counter, [2] unused, [3..3+n_em) next update per emitter
+    int32_t cap;
+    double stop_t;             // resuming call: the run's stop time
+    int resume;                // 1: insert the pending input spikes and the stop event first
+    int with_stop;             // resuming call: 0 when the run continues the previous one (no stop event)
+    StepRef step;              // update k just run (resuming call: the update before the run)
+    uint32_t *err;             // bit 4: heap capacity, bit 5: event of an unknown kind
+};
+
+__device__ inline bool eq_has(const uint32_t *m, int d) { return (m[(d - 1) >> 5] >> ((d - 1) & 31)) & 1u; }
+
+constexpr int kMaxDelayEq = 32 * kEqMaskWords;
+
+// The replay of one window.  One workgroup of kBlock threads.
+__global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
+    __shared__ EqItem s_heap[kEqLdsItems + 1];
+    __shared__ uint32_t s_cnt[kEqMaxEmitters][kMaxDelayEq];
+    __shared__ uint32_t s_partial[kEqMaxEmitters];
+    __shared__ uint32_t s_cur[kMaxDelayEq];
+    __shared__ int32_t s_scan[kBlock];
+    __shared__ int32_t s_run;
+    __shared__ int32_t s_size;
+    __shared__ int32_t s_in_lds;
+    const int32_t k = a.step.get();
+    if (threadIdx.x == 0) s_size = a.st[0];
+    for (int i = threadIdx.x; i < kEqMaxEmitters * kMaxDelayEq; i += kBlock) (&s_cnt[0][0])[i] = 0u;
+    if (threadIdx.x < kEqMaxEmitters) s_partial[threadIdx.x] = 0u;
+    __syncthreads();
+    // 1. spikes per delay of every model whose update at k (or before, when
+    // resuming) is still to leave the heap; s_partial: some spiking neuron
+    // lacks a delay of its model's structure.
+    for (int e = 0; e < a.n_em; e++) {
+        const EqEmitter &E = a.em[e];
+        if (E.kind == kEqTrain) continue;
+        const int32_t ke = a.st[3 + e];
+        if (ke > k || ke < 1) continue;
+        const int slot = ke % a.ring;
+        const int32_t ns = E.ring_cnt[slot];
+        for (int32_t r = threadIdx.x; r < ns; r += kBlock) {
+            const int32_t id = E.ring_idx[(int64_t)slot * E.slot_len + r];
+            const uint32_t *m = E.dmask + (int64_t)id * a.words;
+            bool part = false;
+            for (int w = 0; w < a.words; w++) {
+                uint32_t bits = m[w];
+                part = part || bits != E.full[w];
+                while (bits) {
+                    const int b = __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    atomicAdd(&s_cnt[e][w * 32 + b], 1u);
+                }
+            }
+            if (part) s_partial[e] = 1u;
+        }
+    }
+    __syncthreads();
+    // 2. last member of every full chunk, where a delay has 1,024 spikes or
+    // more and not every spiking neuron has every delay (else chunk c ends at
+    // list entry 1024c + 1023): a block scan per such delay.  An entry past
+    // the last full chunk holds INT32_MAX.
+    for (int e = 0; e < a.n_em; e++) {
+        const EqEmitter &E = a.em[e];
+        if (E.kind == kEqTrain) continue;
+        const int32_t ke = a.st[3 + e];
+        if (ke > k || ke < 1) continue;
+        const int slot = ke % a.ring;
+        if (threadIdx.x == 0) E.bnd_on[slot] = (uint8_t)s_partial[e];
+        if (!s_partial[e]) continue;
+        const int32_t ns = E.ring_cnt[slot];
+        for (int d = 1; d <= a.max_delay; d++) {
+            const uint32_t full_c = s_cnt[e][d - 1] / (uint32_t)kEqChunk;
+            if (full_c == 0) continue;
+            int32_t *bd = E.bnd + ((int64_t)slot * a.max_delay + (d - 1)) * E.cmax;
+            if (threadIdx.x == 0) {
+                s_run = 0;
+                if ((int)full_c < E.cmax) bd[full_c] = 0x7fffffff;
+            }
+            __syncthreads();
+            for (int32_t r0 = 0; r0 < ns; r0 += kBlock) {
+                const int32_t r = r0 + (int32_t)threadIdx.x;
+                int32_t f = 0;
+                if (r < ns) {
+                    const int32_t id = E.ring_idx[(int64_t)slot * E.slot_len + r];
+                    f = eq_has(E.dmask + (int64_t)id * a.words, d) ? 1 : 0;
+                }
+                s_scan[threadIdx.x] = f;
+                __syncthreads();
+                for (int off = 1; off < kBlock; off <<= 1) {
+                    const int32_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
+                    __syncthreads();
+                    s_scan[threadIdx.x] += v;
+                    __syncthreads();
+                }
+                const int32_t upto = s_run + s_scan[threadIdx.x];   // members up to entry r
+                if (f && upto % kEqChunk == 0 && upto / kEqChunk <= (int32_t)full_c) bd[upto / kEqChunk - 1] = r;
+                __syncthreads();
+                if (threadIdx.x == kBlock - 1) s_run += s_scan[kBlock - 1];
+                __syncthreads();
+            }
+        }
+    }
+    // 3. stage the heap in LDS when it cannot outgrow it in this call
+    if (threadIdx.x == 0) {
+        int64_t bound = (int64_t)s_size + a.n_em + 2;
+        for (int e = 0; e < a.n_em; e++) {
+            const EqEmitter &E = a.em[e];
+            if (E.kind == kEqTrain) {
+                if (a.resume) bound += E.in_hi - E.in_lo;
+                continue;
+            }
+            for (int d = 1; d <= a.max_delay; d++) bound += s_cnt[e][d - 1] / kEqChunk + 1;
+        }
+        s_in_lds = bound <= kEqLdsItems ? 1 : 0;
+    }
+    __syncthreads();
+    const bool in_lds = s_in_lds != 0;
+    EqItem *h = in_lds ? s_heap : a.heap;
+    if (in_lds)
+        for (int i = threadIdx.x; i <= s_size; i += kBlock) s_heap[i] = a.heap[i];
+    __syncthreads();
+    // 4. the heap operations, on one lane
+    if (threadIdx.x == 0) {
+        int32_t n = s_size;
+        int32_t *next_k = a.st + 3;
+        uint32_t r = (uint32_t)a.st[1];
+        const int par = (k + 1) & 1;          // parity of the update that consumes this window
+        bool overflow = false, unknown = false;
+        auto push = [&](const EqItem &x) {
+            if (n >= a.cap) {
+                overflow = true;
+                return;
+            }
+            eq_insert(h, &n, x);
+        };
+        if (a.resume) {
+            // input spikes added since the last run (Simulation_API::
+            // AddExternalSpikeActivity) are queued in load order, then the
+            // run's stop event
+            int32_t cur[kEqMaxEmitters];
+            for (int e = 0; e < a.n_em; e++) cur[e] = a.em[e].in_lo;
+            for (;;) {
+                int be = -1;
+                int64_t bs = 0;
+                for (int e = 0; e < a.n_em; e++) {
+                    const EqEmitter &E = a.em[e];
+                    if (E.kind != kEqTrain || cur[e] >= E.in_hi) continue;
+                    if (be < 0 || E.in_seq[cur[e]] < bs) {
+                        be = e;
+                        bs = E.in_seq[cur[e]];
+                    }
+                }
+                if (be < 0) break;
+                const EqEmitter &E = a.em[be];
+                const int32_t q = cur[be]++;
+                if (E.nd_ptr[E.in_neuron[q] + 1] > E.nd_ptr[E.in_neuron[q]])
+                    push(eq_item(E.in_t[q], kEqPrioSpike, kEqInput, (uint32_t)be, 0, (uint32_t)q));
+            }
+            if (a.with_stop) push(eq_item(a.stop_t, kEqPrioStop, kEqStop, 0, 0, 0));
+        }
+        EqItem it;
+        while (n > 0 && !overflow) {
+            const EqItem top = h[1];
+            const uint32_t kind = eq_kind(top), e = eq_em(top);
+            // the window ends where the populations' next update starts
+            if (kind == kEqTime && a.em[e].kind == kEqPop && next_k[e] > k) break;
+            eq_remove(h, &n, &it);
+            if (kind == kEqStop) break;
+            const EqEmitter &E = a.em[e];
+            if (kind == kEqTime) {
+                // TimeDrivenInternalSpike::ProcessEvent: full groups as they
+                // fill (ascending list position, then ascending delay), then
+                // the rest in ascending delay
+                const int32_t ke = next_k[e];
+                if (E.kind == kEqPop) r = 0;   // a window starts with the populations' updates
+                const int slot = ke % a.ring;
+                const bool use_bnd = E.bnd_on[slot] != 0;
+                for (int d = 1; d <= a.max_delay; d++) s_cur[d - 1] = 0u;
+                for (;;) {
+                    int32_t best = 0x7fffffff;
+                    int bd_ = 0;
+                    for (int d = 1; d <= a.max_delay; d++) {
+                        const uint32_t c = s_cur[d - 1];
+                        if (c >= s_cnt[e][d - 1] / (uint32_t)kEqChunk) continue;
+                        const int32_t end = use_bnd ? E.bnd[((int64_t)slot * a.max_delay + (d - 1)) * E.cmax + c]
+                                                    : (int32_t)(c * kEqChunk + kEqChunk - 1);
+                        if (end < best) {
+                            best = end;
+                            bd_ = d;
+                        }
+                    }
+                    if (!bd_) break;
+                    push(eq_item(it.t + a.delay_s[bd_], kEqPrioSpike, kEqGroup, e, (uint32_t)bd_, s_cur[bd_ - 1]));
+                    s_cur[bd_ - 1]++;
+                }
+                for (int d = 1; d <= a.max_delay; d++)
+                    if (s_cnt[e][d - 1] % (uint32_t)kEqChunk)
+                        push(eq_item(it.t + a.delay_s[d], kEqPrioSpike, kEqGroup, e, (uint32_t)d, s_cur[d - 1]));
+                next_k[e] = ke + 1;
+                push(eq_item(it.t + E.h, kEqPrioTime, kEqTime, e, 0, 0));
+            } else if (kind == kEqGroup) {
+                E.rank[((int64_t)par * a.max_delay + (eq_d(it) - 1)) * E.cmax + it.b] = r++;
+            } else if (kind == kEqInput) {
+                // InputSpike::ProcessEvent: the neuron's first delay
+                const int32_t nr = E.in_neuron[it.b];
+                const int d0 = E.nd_list[E.nd_ptr[nr]];
+                push(eq_item(it.t + a.delay_s[d0], kEqPrioSpike, kEqProp, e, 0, it.b));
+            } else if (kind == kEqProp) {
+                // PropagatedSpike::ProcessEvent: this delay, then the next one
+                // at time - delay_j + delay_{j+1}
+                const int32_t nr = E.in_neuron[it.b];
+                const uint32_t j = eq_d(it);
+                const int d = E.nd_list[E.nd_ptr[nr] + (int32_t)j];
+                E.rank[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + E.in_pos[it.b]] = r++;
+                if (E.nd_ptr[nr] + (int32_t)j + 1 < E.nd_ptr[nr + 1]) {
+                    const int d2 = E.nd_list[E.nd_ptr[nr] + (int32_t)j + 1];
+                    push(eq_item(it.t - a.delay_s[d] + a.delay_s[d2], kEqPrioSpike, kEqProp, e, j + 1, it.b));
+                }
+            } else {
+                unknown = true;
+                break;
+            }
+        }
+        s_size = n;
+        a.st[0] = n;
+        a.st[1] = (int32_t)r;
+        if (overflow) atomicOr(a.err, 16u);
+        if (unknown) atomicOr(a.err, 32u);
+    }
+    __syncthreads();
+    if (in_lds)
+        for (int i = threadIdx.x; i <= s_size; i += kBlock) a.heap[i] = s_heap[i];
+}
+
+}  // namespace edl
+
+namespace edl {
+// Delay bitmask of every neuron of a source: bit d-1 when its row of delay d
+// is not empty (OR-ed into the emitter's masks, one source at a time).
+__global__ __launch_bounds__(kBlock) void k_delay_mask(const int64_t *row_ptr, int64_t n, int max_delay, int words,
+                                                       uint32_t *mask) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        for (int w = 0; w < words; w++) {
+            uint32_t bits = 0;
+            for (int b = 0; b < 32; b++) {
+                const int d = w * 32 + b + 1;
+                if (d > max_delay) break;
+                if (row_ptr[i * max_delay + d] > row_ptr[i * max_delay + d - 1]) bits |= 1u << b;
+            }
+            mask[i * words + w] |= bits;
+        }
+    }
+}
+}  // namespace edl

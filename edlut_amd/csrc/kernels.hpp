@@ -62,7 +62,25 @@ struct FoldTarget {
     // network mode: fixed segment of each key (prefix of the keys' synapse
     // counts; a synapse carries at most one event per step), null otherwise
     const uint64_t *seg_off;
+    // network mode with the event queue's order: rank of each group of the
+    // window that update k consumes, per source, [2][max_delay][cmax] by the
+    // parity of k (rank[0] null: canonical order)
+    const uint32_t *rank[4];
+    int32_t rank_cmax[4];
+    int32_t rank_delays;
 };
+
+// Network-mode order key of an event: delay, source, chunk (the group of 1,024
+// spiking sources it belongs to, or the input spike's position in its update),
+// synapse position.  Sorting by it gives the canonical order (delay, source,
+// spiking index, synapse); the event queue's order replaces (delay, source,
+// chunk) by the group's rank in its window.
+constexpr int kOrdSynBits = 36;
+constexpr uint64_t kOrdSynMask = (1ull << kOrdSynBits) - 1ull;
+constexpr int kOrdChunkBits = 18;
+__host__ __device__ __forceinline__ uint64_t ord_key(int d, int p, uint32_t c, uint64_t q) {
+    return ((uint64_t)(d - 1) << 56) | ((uint64_t)p << 54) | ((uint64_t)c << kOrdSynBits) | q;
+}
 
 __device__ __forceinline__ float *g_ptr(const FoldTarget &t, uint64_t key) {
     const uint64_t type = key / (uint64_t)t.stride;
@@ -125,10 +143,24 @@ __device__ __forceinline__ void fold_event(const FoldTarget &t, int par, uint64_
 
 // Network mode: folds the m >= 1 events of `key` (its fixed segment) into g
 // in order and clears the key's fill count.
-__device__ __attribute__((noinline)) float fold_static(uint32_t *fill, const uint64_t *seg_off,
-                                                       uint64_t *b_ord, float *b_w, uint64_t key,
-                                                       uint32_t m, float g) {
-    const uint64_t b = seg_off[key];
+__device__ __attribute__((noinline)) float fold_static(const FoldTarget &t, int par, uint64_t key, uint32_t m,
+                                                       float g) {
+    uint32_t *fill = t.fill;
+    uint64_t *b_ord = t.b_ord;
+    float *b_w = t.b_w;
+    const uint64_t b = t.seg_off[key];
+    if (t.rank[0] && m >= 2u) {
+        // (delay, source, chunk) -> the group's rank in the event queue's window
+        for (uint32_t j = 0; j < m; j++) {
+            const uint64_t o = b_ord[b + j];
+            const int d = (int)(o >> 56) + 1;
+            const int p = (int)((o >> 54) & 3u);
+            const uint32_t c = (uint32_t)((o >> kOrdSynBits) & ((1u << kOrdChunkBits) - 1u));
+            const uint32_t r =
+                t.rank[p][((int64_t)par * t.rank_delays + (d - 1)) * t.rank_cmax[p] + c];
+            b_ord[b + j] = ((uint64_t)r << kOrdSynBits) | (o & kOrdSynMask);
+        }
+    }
     for (uint32_t j = 1; j < m; j++) {
         const uint64_t ko = b_ord[b + j];
         const float kw = b_w[b + j];
@@ -341,9 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
                 const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)idx[n];
                 if (a.fold.seg_off) {
                     const uint32_t m = a.fold.fill[key];
-                    if (m >= 1u)
-                        s[n][M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
-                                                        key, m, s[n][M::EXC + ty]);
+                    if (m >= 1u) s[n][M::EXC + ty] = fold_static(a.fold, step & 1, key, m, s[n][M::EXC + ty]);
                     continue;
                 }
                 const uint32_t m = a.fold.cnt[key];
@@ -503,9 +533,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
                 const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)i;
                 if (a.fold.seg_off) {
                     const uint32_t m = a.fold.fill[key];
-                    if (m >= 1u)
-                        sf[M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
-                                                      key, m, sf[M::EXC + ty]);
+                    if (m >= 1u) sf[M::EXC + ty] = fold_static(a.fold, step & 1, key, m, sf[M::EXC + ty]);
                     continue;
                 }
                 const uint32_t m = a.fold.cnt[key];
@@ -664,6 +692,14 @@ struct RingSource {
     int64_t n_src;
     int stdp;                  // plastic synapses (StdpSyn below)
     StdpSyn syn;
+    // groups of 1,024 spiking sources (the reference's PropagatedSpikeGroup):
+    // the chunk of a list entry is its position / 1024 unless bnd_on[slot],
+    // then the number of chunks whose last member comes before it
+    // (bnd [ring][max_delay][cmax]); a spike train's events are one per spike
+    int train;
+    const int32_t *bnd;
+    const uint8_t *bnd_on;
+    int32_t cmax;
 };
 struct RingEvents {
     RingSource src[kMaxSources];
@@ -696,9 +732,9 @@ __device__ __forceinline__ void stdp_increment(const StdpSyn &s, float *w, int64
 // Work item = one spike of one source at emission step k-1-d; a group of
 // LANES lanes walks its synapses of delay d.  Items are enumerated in
 // (d, source, rank) order from a prefix over the per-(d, source) counts.
-// order = ((d-1)*n_sources + source) << 40 | synapse position; for a fixed
-// (d, source) the synapse position rises with the source index and then the
-// target index, which is the reference's delivery order.
+// The order key (ord_key) holds delay, source, chunk and synapse position;
+// within a group the synapse position rises with the source index and then
+// the target index, which is the reference's delivery order in a group.
 template <int PASS, int LANES>
 __device__ __forceinline__ void fold_ring_body(const FoldTarget &t, const RingEvents &r, int bid, int nb) {
     __shared__ int64_t s_pref[kMaxDelay * kMaxSources + 1];
@@ -744,7 +780,19 @@ __device__ __forceinline__ void fold_ring_body(const FoldTarget &t, const RingEv
         const int32_t id = src.ring_idx[(int64_t)s_slot[g] * src.n_src + rank];
         const int64_t b = src.row_ptr[(int64_t)id * r.max_delay + d - 1];
         const int64_t e = src.row_ptr[(int64_t)id * r.max_delay + d];
-        const uint64_t ord_hi = ((uint64_t)((d - 1) * r.n_sources + p)) << 40;
+        uint32_t chunk = 0;
+        if (src.train) {
+            chunk = (uint32_t)rank;
+        } else if (s_pref[g + 1] - s_pref[g] >= 1024) {
+            const int slot = s_slot[g];
+            if (src.bnd_on && src.bnd_on[slot]) {
+                const int32_t *bd = src.bnd + ((int64_t)slot * r.max_delay + (d - 1)) * src.cmax;
+                while ((int32_t)chunk < src.cmax && bd[chunk] < rank) chunk++;
+            } else {
+                chunk = (uint32_t)(rank >> 10);
+            }
+        }
+        const uint64_t ord_hi = ord_key(d, p, chunk, 0);
         double t_arr = 0.0;
         if (PASS != kCount && src.stdp) t_arr = r.time_ring[s_slot[g]] + r.delay_s[d];
         for (int64_t q = b + lane; q < e; q += LANES) {

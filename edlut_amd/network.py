@@ -94,6 +94,37 @@ class Network:
             )
         )
 
+    def set_source_model(self, source: int, key: int, rank: int = -1):
+        """Event-queue order: sources of any network with the same key are one
+        model (one layer type of the reference); `rank` orders the models'
+        first update events (the network file's type order)."""
+        C.check(C.lib().edlut_network_set_source_model(self._h, int(source), int(key), int(rank)))
+
+    def set_model_rank(self, rank: int):
+        """Rank of this population among the models (default: its position)."""
+        C.check(C.lib().edlut_network_set_model_rank(self._h, int(rank)))
+
+    def set_event_order(self, on: bool):
+        """Deliver in the reference event queue's order (default) or in the
+        canonical order (delay, source, spiking index, synapse)."""
+        C.check(C.lib().edlut_network_set_event_order(self._h, int(bool(on))))
+
+    def set_stop_time(self, t: float):
+        """Stop time (s) of the next run (default: its last update's time)."""
+        C.check(C.lib().edlut_network_set_stop_time(self._h, float(t)))
+
+    def set_spike_train_times(self, source: int, times, order=None, load_first=None):
+        """Times (s) of a spike train's input spikes, aligned with its
+        schedule's entries, their load order (order[j] ranks entry j) and the
+        first update of the run that queues each (0: the simulation's start)."""
+        t = np.ascontiguousarray(times, np.float64)
+        o = None if order is None else np.ascontiguousarray(order, np.int64)
+        lf = None if load_first is None else np.ascontiguousarray(load_first, np.int64)
+        C.check(C.lib().edlut_network_set_spike_train_times(self._h, int(source), len(t),
+                                                            C.ptr(t) if len(t) else None,
+                                                            C.ptr(o) if o is not None and len(o) else None,
+                                                            C.ptr(lf) if lf is not None and len(lf) else None))
+
     def set_poisson_rates(self, source: int, rates_hz):
         """One frequency per generator of Poisson source `source`
         (PoissonGeneratorDeviceVector)."""
@@ -371,6 +402,10 @@ class Circuit:
         C.check(C.lib().edlut_circuit_create(arr, len(self.nets), C.byref(h)))
         self._h = h
         self.next_step = 1
+
+    def set_stop_time(self, t: float):
+        """Stop time (s) of the next run (default: its last update's time)."""
+        C.check(C.lib().edlut_circuit_set_stop_time(self._h, float(t)))
 
     def run(self, n_steps: int, first_step: int | None = None, sync: bool = True):
         if first_step is None:

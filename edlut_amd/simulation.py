@@ -206,6 +206,9 @@ class Simulation_API:
         self._trains_dirty = False
         self._input_times = np.zeros(0, np.float64)
         self._input_cells = np.zeros(0, np.int64)
+        self._input_load = np.zeros(0, np.int64)   # per input spike: first update of the run that queues it
+        self._file_inputs = False    # AddExternalSpikeActivity is loading an input-spike file
+        self.run_log: list[tuple] = []   # device runs: (first update, updates, stop time or -2)
         self.plan: SimulationPlan | None = None
         self.models = {}
         self.nets = {}
@@ -332,7 +335,11 @@ class Simulation_API:
         sim.Initialize()
         if input_spike_file is not None:
             t, c = NF.parse_input_spikes(input_spike_file, sim.num_neuron_created)
-            sim.AddExternalSpikeActivity(t, c)
+            sim._file_inputs = True
+            try:
+                sim.AddExternalSpikeActivity(t, c)
+            finally:
+                sim._file_inputs = False
         if input_current_file is not None:
             t, c, v = NF.parse_input_currents(input_current_file, sim.num_neuron_created)
             sim.AddExternalCurrentActivity(t, c, v)
@@ -501,6 +508,8 @@ class Simulation_API:
             self.nets[t.index] = Network(m, plan.max_delay)
         for t in gpu:
             net = self.nets[t.index]
+            # event-queue order: models are the layer types, ranked in type order
+            net.set_model_rank(t.index)
             for slot, s in enumerate(plan.sources.get(t.index, [])):
                 rp, tt, ww = build_out_csr(s["n_src"], len(t.globals), plan.max_delay, *s["edges"])
                 s["csr"] = (rp, tt, ww)
@@ -515,6 +524,8 @@ class Simulation_API:
                     net.set_poisson_rates(slot, rates)
                 else:
                     net.add_spike_train(s["n_src"], np.zeros(1, np.int64), np.zeros(0, np.int32), rp, tt, ww)
+                if s["kind"] in ("poisson", "train"):
+                    net.set_source_model(slot, s["src_type"], s["src_type"])
                 if s["stdp"] is not None:
                     p, wmax = s["stdp"]
                     net.set_stdp(slot, float(p["max_LTP"]), float(p["tau_LTP"]), float(p["max_LTD"]),
@@ -565,7 +576,14 @@ class Simulation_API:
                 raise EDLUTException(C.EDLUT_E_ARG, f"neuron {c[q]} is not an InputSpikeNeuronModel neuron")
         self._input_times = np.concatenate([self._input_times, t])
         self._input_cells = np.concatenate([self._input_cells, c])
+        # queued by the next run (Simulation_API::AddExternalSpikeActivity
+        # inserts them into the event queue between runs); a file's spikes
+        # are queued when the simulation starts (FileInputSpikeDriver)
+        self._input_load = np.concatenate([self._input_load, np.full(len(t), self._load_update(), np.int64)])
         self._trains_dirty = True
+
+    def _load_update(self) -> int:
+        return 0 if self._file_inputs else self.current_step + 1
 
     def AddExternalCurrentActivity(self, event_time_list, neuron_index_list, current_value_list) -> None:
         """Simulation_API.cpp:155-171: currents (pA) of InputCurrentNeuronModel
@@ -664,22 +682,31 @@ class Simulation_API:
             for s in srcs:
                 if s["kind"] != "train":
                     continue
-                mine = (plan.type_of[self._input_cells] == s["src_type"]) & (steps >= 1)
-                loc = plan.local_of[self._input_cells[mine]]
-                stp = steps[mine]
-                if len(stp) == 0:
-                    self.nets[tidx].set_spike_train(s["slot"], np.zeros(1, np.int64), np.zeros(0, np.int32))
-                    continue
-                pairs = np.unique(np.stack([stp, loc], 1), axis=0)
-                if len(pairs) != len(stp):
-                    raise EDLUTException(C.EDLUT_E_ARG, "two spikes of one input neuron fall on the same update")
-                n_steps = int(pairs[:, 0].max())
-                ptr = np.zeros(n_steps + 1, np.int64)
-                np.add.at(ptr, pairs[:, 0], 1)
-                ptr = np.cumsum(ptr)
-                self.nets[tidx].set_spike_train(s["slot"], ptr, pairs[:, 1].astype(np.int32))
+                ptr, idx, times, rank, load = self.train_schedule(s["src_type"], steps)
+                self.nets[tidx].set_spike_train(s["slot"], ptr, idx)
+                # the InputSpike events: times, load order (the order the
+                # spikes were given in) and the run that queues them
+                self.nets[tidx].set_spike_train_times(s["slot"], times, rank, load)
         self._input_new_from = len(self._input_times)
         self._trains_dirty = False
+
+    def train_schedule(self, src_type: int, steps):
+        """Spike-train schedule of input layer `src_type` (updates from 1,
+        ascending neuron within an update) and, per entry, the spike's time
+        and load rank (its index among the spikes given)."""
+        plan = self.plan
+        q = np.nonzero((plan.type_of[self._input_cells] == src_type) & (steps >= 1))[0]
+        loc = plan.local_of[self._input_cells[q]]
+        stp = steps[q]
+        order = np.lexsort((loc, stp))
+        q, loc, stp = q[order], loc[order], stp[order]
+        if len(q) > 1 and ((stp[1:] == stp[:-1]) & (loc[1:] == loc[:-1])).any():
+            raise EDLUTException(C.EDLUT_E_ARG, "two spikes of one input neuron fall on the same update")
+        n_steps = int(stp.max()) if len(stp) else 0
+        ptr = np.zeros(n_steps + 1, np.int64)
+        np.add.at(ptr, stp, 1)
+        return (np.cumsum(ptr), loc.astype(np.int32), self._input_times[q].astype(np.float64), q.astype(np.int64),
+                self._input_load[q].astype(np.int64))
 
     def RunSimulation(self, end_simulation_time: float) -> None:
         """Simulation_API.cpp:428-442: every update with t_k <= end (an update
@@ -695,16 +722,24 @@ class Simulation_API:
         # currents, then update k onward
         due = sorted(x for x in self._cur_changes if x[0] <= k_end)
         self._cur_changes = [x for x in self._cur_changes if x[0] > k_end]
+        # one reference run: the device runs it in pieces (before each current
+        # change) that carry no stop event, and the last piece stops at `end`
+        # (its stop event shares the event queue, Simulation.cpp:322-373)
         for k in sorted({x[0] for x in due}):
             if k - 1 > self.current_step and self.circuit is not None:
-                self.circuit.run(k - 1 - self.current_step, first_step=self.current_step + 1)
+                self._run_piece(k - 1 - self.current_step, -2.0)
                 self.current_step = k - 1
             self._apply_currents(k, [x for x in due if x[0] == k])
         n = k_end - self.current_step
-        if n > 0 and self.circuit is not None:
-            self.circuit.run(n, first_step=self.current_step + 1)
+        if self.circuit is not None:
+            self._run_piece(max(n, 0), float(end_simulation_time))
         if n > 0:
             self.current_step = k_end
+
+    def _run_piece(self, n: int, stop: float) -> None:
+        self.circuit.set_stop_time(stop)
+        self.circuit.run(n, first_step=self.current_step + 1)
+        self.run_log.append((self.current_step + 1, n, stop))
 
     def update_time(self, k: int) -> float:
         return float(update_times(float(np.float32(self.plan.dt)), k)[k])

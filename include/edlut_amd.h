@@ -164,11 +164,21 @@ int edlut_population_lut_checksums(const edlut_population *pop, uint64_t *out3);
  * Spikes stay on the device: each source keeps a ring of compacted spike
  * lists, and before update k every event of a spike emitted at update
  * s = k-1-d through a synapse of delay d is folded into its target's
- * conductance, in the order the reference's event queue delivers them
- * (ascending delay, then source model, then source index, then target index;
+ * conductance, in the order the reference's event queue delivers them.  The
+ * queue's operations are replayed on the device after every update
+ * (eventq.hpp): one update-event stream per model at its own step, spike
+ * groups of up to 1,024 sources per delay queued as they fill, input spikes
+ * chaining one propagated spike per delay, a stop event per run, all in one
+ * binary heap whose equal keys leave in its layout's order
+ * (Kernel/src/simulation/EventQueue.cpp:121-198,
+ * Kernel/src/spike/TimeDrivenInternalSpike.cpp:66-104,163-178,
  * Kernel/src/spike/PropagatedSpikeGroup.cpp:66-108,
- * Kernel/src/spike/Network.cpp:45-68,
- * Kernel/src/simulation/EventQueue.cpp:121-198).
+ * Kernel/src/simulation/Simulation.cpp:322-373); within a group, source index
+ * then target index (Kernel/src/spike/Network.cpp:45-68).  A population's
+ * sources are one model each unless edlut_network_set_source_model joins
+ * them; Poisson devices sample every 0.1 ms and need a 0.1 ms population
+ * step.  Grouped populations run in the canonical order (ascending delay,
+ * then source, then source index, then target index).
  *
  * Synapse arrays use a per-(source, delay) row pointer: synapses of source s
  * with delay d (1..max_delay steps) occupy [row_ptr[s*max_delay+d-1],
@@ -230,6 +240,33 @@ int edlut_network_set_spike_train(edlut_network *net, int source, int64_t n_step
 int edlut_network_add_projection(edlut_network *net, edlut_network *from, const int64_t *row_ptr,
                                  const uint32_t *tgt_type, const float *w);
 
+/* Event-queue order.  Sources (Poisson devices, spike trains) of any network
+ * with the same key are one model, as the layers of one neuron type are one
+ * model in the reference (Network.cpp:1016-1110); rank orders the models'
+ * first update events as the network file orders its types
+ * (Simulation::InitSimulation, Simulation.cpp:228-268).  Defaults: every
+ * source its own model, populations first in circuit order. */
+int edlut_network_set_source_model(edlut_network *net, int source, int key, int rank);
+int edlut_network_set_model_rank(edlut_network *net, int rank);
+/* on = 0: canonical order instead of the event queue's (no counterpart in
+ * the reference; for comparisons). */
+int edlut_network_set_event_order(edlut_network *net, int on);
+/* Stop time (s) of the next run, between its last update's time and the
+ * next update's (Simulation_API::RunSimulation's end time, whose stop event
+ * shares the queue; Simulation.cpp:322-373).  Default: the last update's
+ * time; -2: no stop event (the run and the next one are one reference run). */
+int edlut_network_set_stop_time(edlut_network *net, double t);
+/* Times (s) of a spike train's input spikes, aligned with the entries of its
+ * schedule, their load order (order[j] ranks entry j; null: schedule order)
+ * and when they are queued (load_first[j] = 0: when the simulation starts,
+ * before the models' first update events, as FileInputSpikeDriver's spikes
+ * (Simulation.cpp:228-268); k: by the run starting at update k, as
+ * Simulation_API::AddExternalSpikeActivity's; null: all 0): the InputSpike
+ * events of the reference (FileInputSpikeDriver.cpp:45-85, InputSpike.cpp).
+ * Without it an input spike sits at its update's time. */
+int edlut_network_set_spike_train_times(edlut_network *net, int source, int64_t n_spikes,
+                                        const double *time, const int64_t *order, const int64_t *load_first);
+
 /*
  * Circuit: networks advanced in lockstep on one stream, update k of every
  * member (in the order given) before update k + 1 of any; every projection
@@ -240,6 +277,8 @@ int edlut_network_add_projection(edlut_network *net, edlut_network *from, const 
  */
 int edlut_circuit_create(edlut_network *const *nets, int n_nets, edlut_circuit **out);
 int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps);
+/* Stop time (s) of the circuit's next run (see edlut_network_set_stop_time). */
+int edlut_circuit_set_stop_time(edlut_circuit *c, double t);
 int edlut_circuit_synchronize(edlut_circuit *c);
 int edlut_circuit_destroy(edlut_circuit *c);
 

@@ -1377,8 +1377,9 @@ struct Source {
     int emitter_key = -1;
     int rank = -1;
     std::vector<std::pair<int32_t, int32_t>> poisson_log;
-    std::vector<double> train_time;
-    std::vector<int32_t> train_neuron;
+    std::vector<double> train_time;      /* per schedule entry (train_idx order) */
+    std::vector<int64_t> train_order;    /* load rank of each entry */
+    std::vector<int64_t> train_load;     /* first update of the run that queues it */
     /* projection from another network's population: its spikes */
     struct Net *from = nullptr;
     /* spikes per step: ring indexed by step % ring */
@@ -1427,6 +1428,10 @@ inline double delay_seconds(int d, float dt_s) {
     double r = std::floor((double)d * (double)dt_s * (1.0 / 0.0001) + m * 0.5) * m;
     return r <= 0 ? m : r;
 }
+
+/* PoissonGeneratorDeviceVector's step: 0.1f / 1000.0f in float
+ * (PoissonGeneratorDeviceVector.cpp:26,137). */
+inline double evq_poisson_step() { return (double)(0.1f / 1000.0f); }
 
 struct TraceRec {
     int32_t step;
@@ -1788,7 +1793,7 @@ static int net_step(Net *n, int64_t k, std::vector<int32_t> &spk) {
                     list.push_back(src.train_idx[(size_t)j]);
             continue;
         }
-        const double step_s = (double)pop->dt_s();
+        const double step_s = evq_poisson_step();
         src.fired_tmp.assign((size_t)src.n, 0);
         uint8_t *fl = src.fired_tmp.data();
 #pragma omp parallel for schedule(static) num_threads(g_threads) if (g_threads > 1 && src.n >= 4096)
@@ -2037,6 +2042,53 @@ static void engine_update_pop(Net *n, int64_t k, double t, std::vector<int32_t> 
     }
 }
 
+/* Input spikes of the spike trains queued when the simulation starts (load
+ * update 0, before the models' update events, as FileInputSpikeDriver's) or
+ * by the run starting at update `first` (Simulation_API::
+ * AddExternalSpikeActivity), merged across trains by load rank.  Without
+ * explicit times a spike sits at its update's time, is loaded at the start,
+ * and the schedule order is its load order. */
+static void engine_queue_inputs(Sim &S, int64_t first, bool initial) {
+    using namespace evq;
+    struct Rec {
+        int64_t rank;
+        int em;
+        double t;
+        int32_t neuron;
+    };
+    std::vector<Rec> recs;
+    int64_t base = 0;
+    for (size_t e = 0; e < S.em.size(); e++) {
+        Emitter &E = S.em[e];
+        if (E.kind != E_TRAIN) continue;
+        const Source &s = E.copies[0].first->sources[(size_t)E.copies[0].second];
+        const Net *n0 = E.copies[0].first;
+        const int64_t steps = (int64_t)s.train_ptr.size() - 1;
+        double t = 0.0;
+        for (int64_t k = 1; k <= steps; k++) {
+            t += (double)n0->pop->dt_s();
+            for (int64_t j = s.train_ptr[(size_t)k - 1]; j < s.train_ptr[(size_t)k]; j++) {
+                const int64_t load = s.train_load.empty() ? 0 : s.train_load[(size_t)j];
+                if (initial ? load != 0 : load != first) continue;
+                Rec r;
+                r.rank = s.train_order.empty() ? base + j : s.train_order[(size_t)j];
+                r.em = (int)e;
+                r.t = s.train_time.empty() ? t : s.train_time[(size_t)j];
+                r.neuron = s.train_idx[(size_t)j];
+                recs.push_back(r);
+            }
+        }
+        base += (int64_t)s.train_idx.size();
+    }
+    std::stable_sort(recs.begin(), recs.end(), [](const Rec &a, const Rec &b) { return a.rank < b.rank; });
+    for (const Rec &r : recs) {
+        if (S.em[(size_t)r.em].ndel[(size_t)r.neuron].empty()) continue;
+        const int32_t id = S.new_ev(K_INPUT, r.em);
+        S.pool[(size_t)id].neuron = r.neuron;
+        S.heap.insert(Item{r.t, PRIO_SPIKE, id});
+    }
+}
+
 /* Builds the simulation of a set of networks: one emitter per population
  * and per Poisson / spike-train model; the initial events. */
 static Sim *engine_create(Net **nets, int n_nets) {
@@ -2091,31 +2143,7 @@ static Sim *engine_create(Net **nets, int n_nets) {
     for (Emitter &E : S->em) finish_delays(E);
     /* Simulation::InitSimulation: the input spikes first (GetInput), then one
      * update event per time-driven model in type order. */
-    for (size_t e = 0; e < S->em.size(); e++) {
-        Emitter &E = S->em[e];
-        if (E.kind != E_TRAIN) continue;
-        const Source &s = E.copies[0].first->sources[(size_t)E.copies[0].second];
-        const Net *n0 = E.copies[0].first;
-        std::vector<std::pair<double, int32_t>> in;
-        if (!s.train_time.empty()) {
-            for (size_t q = 0; q < s.train_time.size(); q++) in.push_back({s.train_time[q], s.train_neuron[q]});
-        } else {
-            /* spikes placed on updates: an input spike at the time of update k */
-            double t = 0.0;
-            const int64_t steps = (int64_t)s.train_ptr.size() - 1;
-            for (int64_t k = 1; k <= steps; k++) {
-                t += (double)n0->pop->dt_s();
-                for (int64_t j = s.train_ptr[(size_t)k - 1]; j < s.train_ptr[(size_t)k]; j++)
-                    in.push_back({t, s.train_idx[(size_t)j]});
-            }
-        }
-        for (auto &x : in) {
-            if (E.ndel[(size_t)x.second].empty()) continue;
-            const int32_t id = S->new_ev(K_INPUT, (int)e);
-            S->pool[(size_t)id].neuron = x.second;
-            S->heap.insert(Item{x.first, PRIO_SPIKE, id});
-        }
-    }
+    engine_queue_inputs(*S, 0, true);
     std::vector<int> order;
     for (size_t e = 0; e < S->em.size(); e++)
         if (S->em[e].kind != E_TRAIN) order.push_back((int)e);
@@ -2130,17 +2158,26 @@ static int engine_run(Sim *S, int64_t first, int n_steps, double stop) {
     using namespace evq;
     if (first != S->next_first) return -1;
     const int64_t last = first + n_steps - 1;
+    /* input spikes loaded for this run, then its stop event; stop = -2: no
+     * stop event (the run is continued by the next one as one reference run) */
+    engine_queue_inputs(*S, first, false);
+    const bool with_stop = stop != -2.0;
     if (stop < 0) {
         const Emitter &P = S->em[0];
         double t = 0.0;
         for (int64_t k = 1; k <= last; k++) t += P.h;
         stop = t;
     }
-    const int32_t stop_id = S->new_ev(K_STOP, -1);
-    S->heap.insert(Item{stop, PRIO_STOP, stop_id});
+    if (with_stop) S->heap.insert(Item{stop, PRIO_STOP, S->new_ev(K_STOP, -1)});
     std::vector<int32_t> spk;
     Item it;
-    while (S->heap.remove(it)) {
+    while (S->heap.size() > 0) {
+        /* the window after update `last` ends where the populations' next
+         * update starts */
+        const Item top = S->heap.e[1];
+        const Ev &tev = S->pool[(size_t)top.ev];
+        if (tev.kind == K_TIME && S->em[(size_t)tev.em].kind == E_POP && S->em[(size_t)tev.em].k > last) break;
+        S->heap.remove(it);
         const int kind = S->pool[(size_t)it.ev].kind;
         const int e = S->pool[(size_t)it.ev].em;
         if (kind == K_STOP) {
@@ -2213,9 +2250,9 @@ static int engine_run(Sim *S, int64_t first, int n_steps, double stop) {
 
 extern "C" {
 
-/* Order of delivery: 1 the reference's event queue, 0 (default) the
+/* Order of delivery: 1 (default) the reference's event queue, 0 the
  * canonical order (delay, source, spiking index, synapse). */
-static int g_order = 0;
+static int g_order = 1;
 void orc_set_order(int mode) { g_order = mode; }
 
 /* Sources with the same key (across networks) are one model; rank orders
@@ -2228,14 +2265,21 @@ int orc_net_set_source_model(void *net, int source, int key, int rank) {
     return 0;
 }
 
-/* Input spikes of a spike-train source as (time s, neuron) in load order:
- * the reference's InputSpike events, queued when the simulation starts. */
-int orc_net_set_train_times(void *net, int source, int64_t n_spikes, const double *time, const int32_t *neuron) {
+/* Times (s), load ranks and load runs of a spike train's input spikes,
+ * aligned with its schedule's entries (null: the update's time, schedule
+ * order, queued at the start). */
+int orc_net_set_train_times(void *net, int source, int64_t n_spikes, const double *time, const int64_t *order,
+                            const int64_t *load_first) {
     Net *n = (Net *)net;
-    if (source < 0 || source >= (int)n->sources.size() || n->sim || !n->sources[(size_t)source].train) return -1;
+    if (source < 0 || source >= (int)n->sources.size() || !n->sources[(size_t)source].train) return -1;
     Source &s = n->sources[(size_t)source];
-    s.train_time.assign(time, time + n_spikes);
-    s.train_neuron.assign(neuron, neuron + n_spikes);
+    if (n_spikes != (int64_t)s.train_idx.size()) return -1;
+    if (time) s.train_time.assign(time, time + n_spikes);
+    else s.train_time.clear();
+    if (order) s.train_order.assign(order, order + n_spikes);
+    else s.train_order.clear();
+    if (load_first) s.train_load.assign(load_first, load_first + n_spikes);
+    else s.train_load.clear();
     return 0;
 }
 
@@ -2256,6 +2300,19 @@ int64_t orc_heap_replay(int64_t n_ops, const int8_t *op, const double *t, const 
         }
     }
     return r;
+}
+
+/* Replaces a spike train's schedule (its times, ranks and load runs reset). */
+int orc_net_set_train(void *net, int source, int64_t n_steps, const int64_t *step_ptr, const int32_t *idx) {
+    Net *n = (Net *)net;
+    if (source < 0 || source >= (int)n->sources.size() || !n->sources[(size_t)source].train) return -1;
+    Source &s = n->sources[(size_t)source];
+    s.train_ptr.assign(step_ptr, step_ptr + n_steps + 1);
+    s.train_idx.assign(idx, idx + (n_steps > 0 ? step_ptr[n_steps] : 0));
+    s.train_time.clear();
+    s.train_order.clear();
+    s.train_load.clear();
+    return 0;
 }
 
 /* Rank of the network's population among the models (default: its

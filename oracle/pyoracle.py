@@ -57,7 +57,8 @@ def lib():
         l.orc_set_threads.argtypes = [ctypes.c_int]
         l.orc_set_order.argtypes = [ctypes.c_int]
         l.orc_net_set_source_model.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
-        l.orc_net_set_train_times.argtypes = [P, ctypes.c_int, ctypes.c_int64, P, P]
+        l.orc_net_set_train_times.argtypes = [P, ctypes.c_int, ctypes.c_int64, P, P, P]
+        l.orc_net_set_train.argtypes = [P, ctypes.c_int, ctypes.c_int64, P, P]
         l.orc_net_set_stop_time.argtypes = [P, ctypes.c_double]
         l.orc_net_set_model_rank.argtypes = [P, ctypes.c_int]
         l.orc_net_take_poisson_log.restype = ctypes.c_int64
@@ -114,8 +115,8 @@ def set_threads(n: int) -> None:
 
 
 def set_order(mode: int) -> None:
-    """Delivery order of network runs: 1 the reference's event queue, 0 the
-    canonical order (delay, source, spiking index, synapse)."""
+    """Delivery order of network runs: 1 the reference's event queue
+    (default), 0 the canonical order (delay, source, spiking index, synapse)."""
     lib().orc_set_order(int(mode))
 
 
@@ -275,13 +276,24 @@ class Net:
         if lib().orc_net_set_source_model(self.h, int(source), int(key), int(rank)) != 0:
             raise ValueError("bad source or simulation already started")
 
-    def set_train_times(self, source: int, times, neurons):
-        """Input spikes (time in s, neuron) of a spike-train source, in load order."""
-        t = np.ascontiguousarray(times, np.float64)
-        n = np.ascontiguousarray(neurons, np.int32)
-        self._keep += [t, n]
-        if lib().orc_net_set_train_times(self.h, int(source), len(t), _p(t), _p(n)) != 0:
-            raise ValueError("not a spike-train source, or simulation already started")
+    def set_train_times(self, source: int, times=None, order=None, load_first=None):
+        """Times (s), load ranks and first update of the loading run (0: when
+        the simulation starts) of a spike train's input spikes, aligned with
+        its schedule's entries (None: the update's time, schedule order, 0)."""
+        arrs = [None if x is None else np.ascontiguousarray(x, dt) for x, dt in
+                ((times, np.float64), (order, np.int64), (load_first, np.int64))]
+        m = next((len(a) for a in arrs if a is not None), 0)
+        self._keep += [a for a in arrs if a is not None]
+        if lib().orc_net_set_train_times(self.h, int(source), m, *[_p(a) if a is not None else None for a in arrs]) != 0:
+            raise ValueError("not a spike-train source, or the times do not match its schedule")
+
+    def set_train(self, source: int, step_ptr, idx):
+        """Replaces a spike train's schedule (times, ranks and load runs reset)."""
+        sp = np.ascontiguousarray(step_ptr, np.int64)
+        ix = np.ascontiguousarray(idx, np.int32)
+        self._keep += [sp, ix]
+        if lib().orc_net_set_train(self.h, int(source), len(sp) - 1, _p(sp), _p(ix) if len(ix) else None) != 0:
+            raise ValueError("not a spike-train source")
 
     def set_model_rank(self, rank: int):
         """Rank of this network's population among the models (update-event
@@ -290,7 +302,8 @@ class Net:
             raise ValueError("simulation already started")
 
     def set_stop_time(self, t: float):
-        """Stop time (s) of the next run (default: its last update's time)."""
+        """Stop time (s) of the next run (default: its last update's time;
+        -2: no stop event, the next run continues this one)."""
         lib().orc_net_set_stop_time(self.h, float(t))
 
     def take_poisson_log(self, source: int):

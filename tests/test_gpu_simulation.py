@@ -22,19 +22,12 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "edlut_input_files")
 
 
-def _train(sim, plan, src_type):
-    steps = sim.place_input_spikes(sim._input_times, sim._input_cells)
-    mine = (plan.type_of[sim._input_cells] == src_type) & (steps >= 1)
-    pairs = np.unique(np.stack([steps[mine], plan.local_of[sim._input_cells[mine]]], 1), axis=0)
-    n = int(pairs[:, 0].max()) if len(pairs) else 0
-    ptr = np.zeros(n + 1, np.int64)
-    np.add.at(ptr, pairs[:, 0], 1)
-    return np.cumsum(ptr), pairs[:, 1].astype(np.int32)
-
-
 def _oracle(oracle, sim):
+    """The oracle circuit of the same mapping, models ranked and input spikes
+    timed and loaded as the front-end gives them to the device."""
     plan = sim.plan
     pops, nets, order = {}, {}, []
+    steps = sim.place_input_spikes(sim._input_times, sim._input_cells)
     for t in plan.types:
         name = S.gpu_model_name(t.model)
         if name is None:
@@ -48,6 +41,7 @@ def _oracle(oracle, sim):
         pops[t.index] = oracle.Population(name, im["name"], params, len(t.globals), dt_s=np.float32(im["step"]),
                                           mask=mask, bdf_order=im.get("bdf_order", 1), n_steps=im.get("n_steps", 2))
         nets[t.index] = oracle.Net(pops[t.index], plan.max_delay)
+        nets[t.index].set_model_rank(t.index)
         order.append(t.index)
     for ti in order:
         net = nets[ti]
@@ -61,8 +55,12 @@ def _oracle(oracle, sim):
                 net.add_poisson(plan.poisson_states[s["src_type"]], 0.0, *csr)
                 f = plan.types[s["src_type"]].model.param_map.get("frequency", 10.0)
                 net.set_poisson_rates(slot, np.full(s["n_src"], np.float32(f), np.float32))
+                net.set_source_model(slot, s["src_type"], s["src_type"])
             else:
-                net.add_train(s["n_src"], *_train(sim, plan, s["src_type"]), *csr)
+                ptr, idx, times, rank, load = sim.train_schedule(s["src_type"], steps)
+                net.add_train(s["n_src"], ptr, idx, *csr)
+                net.set_source_model(slot, s["src_type"], s["src_type"])
+                net.set_train_times(slot, times, rank, load)
             if s["stdp"] is not None:
                 p, wmax = s["stdp"]
                 net.set_stdp(slot, float(p["max_LTP"]), float(p["tau_LTP"]), float(p["max_LTD"]),
@@ -72,15 +70,17 @@ def _oracle(oracle, sim):
 
 def _compare(oracle, sim, n_steps):
     pops, nets, order = _oracle(oracle, sim)
-    # the external currents the front-end set before update k, replayed
-    done = 0
-    for k, tidx, total in sim.current_log:
-        if k - 1 > done:
-            oracle.circuit_run([nets[i] for i in order], done + 1, k - 1 - done)
-            done = k - 1
-        pops[tidx].set_ext_current(total)
-    if n_steps > done:
-        oracle.circuit_run([nets[i] for i in order], done + 1, n_steps - done)
+    # the device runs (pieces of RunSimulation, with their stop events) and
+    # the external currents the front-end set before them, replayed
+    ci = 0
+    for first, n, stop in sim.run_log:
+        while ci < len(sim.current_log) and sim.current_log[ci][0] <= first:
+            _, tidx, total = sim.current_log[ci]
+            pops[tidx].set_ext_current(total)
+            ci += 1
+        nets[order[0]].set_stop_time(stop)
+        oracle.circuit_run([nets[i] for i in order], first, n)
+    assert sum(n for _, n, _ in sim.run_log) == n_steps
     total = 0
     for ti in order:
         gs, gi = sim.nets[ti].spikes()
@@ -157,7 +157,7 @@ def test_simulation_api_network_bit_exact(oracle):
     # the Poisson layer's generators, replayed on the host from their seeds
     # (RandomGenerator LCG; fire when frequency * step > frand())
     x = sim.plan.poisson_states[1].astype(np.uint64)
-    p = float(np.float32(150.0)) * float(np.float32(1e-4))
+    p = float(np.float32(150.0)) * float(np.float32(0.1) / np.float32(1000.0))   # the device's own 0.1 ms step
     n_poi = 0
     for k in range(1, sim.current_step + 1):
         x = (x * np.uint64(1103515245) + np.uint64(12345)) & np.uint64(0x7FFFFFFF)
@@ -175,7 +175,8 @@ def test_simulation_api_network_bit_exact(oracle):
 
 def test_external_spikes_between_runs(oracle):
     """Spikes added between runs (the closed-loop use of
-    AddExternalSpikeActivity) give the run that had them from the start."""
+    AddExternalSpikeActivity) join the event queue when the next run starts,
+    as in the reference; the oracle queues them at the same point."""
     sim, inp, _ = _driven_sim(seed=11)
     sim.Initialize()
     t1, c1 = _input_spikes(5, 30, 0.001, 0.05, inp[0])
@@ -187,6 +188,7 @@ def test_external_spikes_between_runs(oracle):
         sim.RunSimulation(0.06)
     sim._input_times = sim._input_times[:30]
     sim._input_cells = sim._input_cells[:30]
+    sim._input_load = sim._input_load[:30]
     sim._trains_dirty = False
     sim.AddExternalSpikeActivity(t2, c2)
     sim.RunSimulation(0.1)
