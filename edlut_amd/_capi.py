@@ -47,6 +47,8 @@ EXPORTS = (
     "edlut_network_set_stop_time",
     "edlut_network_set_spike_train_times",
     "edlut_circuit_set_stop_time",
+    "edlut_network_event_queue",
+    "edlut_circuit_event_queue",
     "edlut_circuit_create",
     "edlut_circuit_run",
     "edlut_circuit_synchronize",
@@ -140,6 +142,8 @@ def _sig(lib):
         "edlut_network_set_stop_time": (I32, [P, ctypes.c_double]),
         "edlut_network_set_spike_train_times": (I32, [P, I32, I64, P, P, P]),
         "edlut_circuit_set_stop_time": (I32, [P, ctypes.c_double]),
+        "edlut_network_event_queue": (I32, [P, P, P, I64, P]),
+        "edlut_circuit_event_queue": (I32, [P, P, P, I64, P]),
         "edlut_circuit_create": (I32, [P, I32, P]),
         "edlut_circuit_run": (I32, [P, I64, I32]),
         "edlut_circuit_synchronize": (I32, [P]),

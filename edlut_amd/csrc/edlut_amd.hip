@@ -339,6 +339,7 @@ struct edlut_population {
     FoldTarget fold_target(int par, bool by_step = false) const {
         FoldTarget t;
         t.seg_off = nullptr;
+        t.rtab = nullptr;
         t.par_by_step = by_step ? 1 : 0;
         t.state = state;
         t.stride = stride;
@@ -504,6 +505,9 @@ struct edlut_network {
     std::vector<int> src_em;
     const uint32_t *eq_rank[kMaxSources] = {};
     int32_t eq_cmax[kMaxSources] = {};
+    RankTab *d_rtab = nullptr;       // the rank tables of the sources, in device memory
+    double *d_time_p = nullptr;      // [ring] update times of the Poisson devices' clock
+    uint32_t fold_rank_lo = 0;       // the next run's first fold skips groups ranked below (delivered at a stop)
     const int32_t *eq_bnd[kMaxSources] = {};
     const uint8_t *eq_bnd_on[kMaxSources] = {};
     double time_of(int64_t k) {
@@ -770,14 +774,21 @@ int eq_build(EqSim *S, const std::vector<edlut_network *> &nets, uint32_t *err, 
         E.dev.bnd_on = E.bnd_on;
     }
     // rank tables and chunk boundaries of every source's model
-    for (edlut_network *n : nets)
+    for (edlut_network *n : nets) {
+        RankTab rt{};
+        rt.delays = D;
         for (size_t si = 0; si < n->sources.size(); si++) {
             const EqHostEm &E = S->em[(size_t)n->src_em[si]];
             n->eq_rank[si] = E.rank_tab;
             n->eq_cmax[si] = E.dev.cmax;
             n->eq_bnd[si] = E.bnd;
             n->eq_bnd_on[si] = E.bnd_on;
+            rt.rank[si] = E.rank_tab;
+            rt.cmax[si] = E.dev.cmax;
         }
+        HIP_TRY(hipMemcpy(n->d_rtab, &rt, sizeof(RankTab), hipMemcpyHostToDevice));
+        n->fold_rank_lo = 0;
+    }
     // heap capacity: the groups of every model for every update in flight,
     // the input spikes, the update and stop events
     int64_t cap = 256;
@@ -1178,6 +1189,8 @@ int check_err(edlut_population *p) {
     if (e & 4u) return fail(EDLUT_E_CAP, "input event buffer capacity exceeded");
     if (e & 8u) return fail(EDLUT_E_STATE, "input fold segment wait exceeded its spin limit");
     if (e & 16u) return fail(EDLUT_E_CAP, "BDF fallback working storage exhausted in one step");
+    if (e & 64u) return fail(EDLUT_E_CAP, "event queue capacity exceeded");
+    if (e & 128u) return fail(EDLUT_E_STATE, "event queue holds an event of an unknown kind");
     return 0;
 }
 
@@ -1918,7 +1931,8 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
         (rc = dalloc(&n->staging, (size_t)pop->stride)) ||
         (rc = dalloc(&n->log_step, n->log_cap)) || (rc = dalloc(&n->log_idx, n->log_cap)) ||
         (rc = dalloc(&n->log_count, 2)) || (rc = dalloc(&n->d_step, 1)) ||
-        (rc = dalloc(&n->d_time, n->ring)) || (rc = dalloc(&n->d_delay_s, max_delay + 1))) {
+        (rc = dalloc(&n->d_time, n->ring)) || (rc = dalloc(&n->d_time_p, n->ring)) ||
+        (rc = dalloc(&n->d_rtab, 1)) || (rc = dalloc(&n->d_delay_s, max_delay + 1))) {
         delete n;
         return rc;
     }
@@ -1932,6 +1946,8 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
         }
         HIP_TRY(hipMemcpy(n->d_delay_s, ds.data(), sizeof(double) * ds.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemset(n->d_time, 0, sizeof(double) * n->ring));
+        HIP_TRY(hipMemset(n->d_time_p, 0, sizeof(double) * n->ring));
+        HIP_TRY(hipMemset(n->d_rtab, 0, sizeof(RankTab)));
     }
     HIP_TRY(hipMemset(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring));
     HIP_TRY(hipMemset(n->log_count, 0, 2 * sizeof(int64_t)));
@@ -1962,7 +1978,8 @@ int edlut_network_destroy(edlut_network *n) {
     }
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
-                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_delay_s, n->seg_off, n->clk};
+                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_time_p, n->d_rtab, n->d_delay_s,
+                   n->seg_off, n->clk};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     delete n->eq;
@@ -2211,6 +2228,31 @@ int edlut_network_set_event_order(edlut_network *n, int on) {
     return 0;
 }
 
+// The replayed event queue's heap as stored (1-based array order): time and
+// the packed key (priority << 28 | kind << 24 | model << 16 | delay).
+static int eq_dump(EqSim *S, hipStream_t st, double *t, uint32_t *key, int64_t cap, int64_t *size) {
+    *size = 0;
+    if (!S || !S->st) return 0;
+    HIP_TRY(hipStreamSynchronize(st));
+    int32_t n = 0;
+    HIP_TRY(hipMemcpy(&n, S->st, sizeof(int32_t), hipMemcpyDeviceToHost));
+    *size = n;
+    if (cap < n) return fail(EDLUT_E_CAP, "the queue holds %d events", n);
+    std::vector<EqItem> h((size_t)n + 1);
+    HIP_TRY(hipMemcpy(h.data(), S->heap, sizeof(EqItem) * ((size_t)n + 1), hipMemcpyDeviceToHost));
+    for (int32_t i = 0; i < n; i++) {
+        t[i] = h[(size_t)i + 1].t;
+        key[i] = h[(size_t)i + 1].a;
+    }
+    return 0;
+}
+
+int edlut_network_event_queue(edlut_network *n, double *t, uint32_t *key, int64_t cap, int64_t *size) {
+    if (!n || !size) return fail(EDLUT_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    return eq_dump(n->circuit ? nullptr : n->eq, n->pop->stream, t, key, cap, size);
+}
+
 int edlut_network_set_stop_time(edlut_network *n, double t) {
     if (!n) return fail(EDLUT_E_ARG, "network is null");
     n->stop_next = t;
@@ -2277,17 +2319,7 @@ int edlut_network_set_external(edlut_network *n, int n_thr, const uint32_t *thre
 static FoldTarget net_fold_target(const edlut_network *n) {
     FoldTarget ft = n->pop->fold_target(0, true);
     ft.seg_off = n->seg_off;
-    for (int q = 0; q < kMaxSources; q++) {
-        ft.rank[q] = nullptr;
-        ft.rank_cmax[q] = 0;
-    }
-    ft.rank_delays = n->max_delay;
-    if (n->eq_on && !n->pop->grouped && !n->sources.empty() && n->eq_rank[0]) {
-        for (size_t q = 0; q < n->sources.size(); q++) {
-            ft.rank[q] = n->eq_rank[q];
-            ft.rank_cmax[q] = n->eq_cmax[q];
-        }
-    }
+    ft.rtab = (n->eq_on && !n->pop->grouped && !n->sources.empty() && n->eq_rank[0]) ? n->d_rtab : nullptr;
     return ft;
 }
 
@@ -2297,7 +2329,8 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
                         bool fold_done, bool *fused, const EqSim *eq = nullptr) {
     edlut_population *p = n->pop;
     if (n->plastic)
-        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, p->stream, step, n->d_time, n->ring, (double)p->dt_s);
+        hipLaunchKernelGGL(k_tick, dim3(1), dim3(64), 0, p->stream, step, n->d_time, n->d_time_p, n->ring,
+                           (double)p->dt_s, poisson_step_s());
     // Poisson devices fire at update k (their spikes feed later updates).
     for (Source &s : n->sources) {
         if (s.self || s.from) continue;
@@ -2393,12 +2426,17 @@ static RingEvents ring_events(edlut_network *n) {
     re.max_delay = n->max_delay;
     re.ring = n->ring;
     re.time_ring = n->d_time;
+    re.time_ring_p = n->d_time_p;
     re.delay_s = n->d_delay_s;
+    re.rtab = nullptr;
+    re.rank_lo = 0;
+    re.rank_hi = 0xffffffffu;
     re.exp_lut = n->pop->exp_lut;
     for (int q = 0; q < re.n_sources; q++) {
         const Source &s = n->sources[q];
         re.src[q] = RingSource{s.ring_idx, s.ring_cnt, s.row_ptr, s.tgt_type, s.w, s.n,
-                               s.stdp ? 1 : 0, s.syn, s.train ? 1 : 0, nullptr, nullptr, 1};
+                               s.stdp ? 1 : 0, s.syn, s.train ? 1 : 0, nullptr, nullptr, 1,
+                               (!s.self && !s.from && !s.train) ? 1 : 0};
         if (n->eq_on && !n->pop->grouped && n->eq_rank[q]) {
             re.src[q].bnd = n->eq_bnd[q];
             re.src[q].bnd_on = n->eq_bnd_on[q];
@@ -2535,7 +2573,11 @@ static int run_prepare(edlut_network *n, int64_t first_step, int n_steps, Update
     if (n->plastic) {
         // update time of the step before the run, in its ring slot
         const double t0 = n->time_of(first_step - 1);
+        double tp = 0.0;
+        for (int64_t k = 1; k < first_step; k++) tp += poisson_step_s();
         HIP_TRY(hipMemcpyAsync(n->d_time + (first_step - 1) % n->ring, &t0, sizeof(double),
+                               hipMemcpyHostToDevice, p->stream));
+        HIP_TRY(hipMemcpyAsync(n->d_time_p + (first_step - 1) % n->ring, &tp, sizeof(double),
                                hipMemcpyHostToDevice, p->stream));
         HIP_TRY(hipStreamSynchronize(p->stream));
     }
@@ -2570,6 +2612,61 @@ static int fold_ahead(edlut_network *n, const RingEvents &re, int64_t k) {
     return launch_ring_fold(net_fold_target(n), n->pop->stream, rk, n->fold_lanes, n->fold_blocks);
 }
 
+// A run whose stop event leaves inside the window after its last update
+// (Simulation_API::RunSimulation with an end time past that update) delivers,
+// in the reference, the window's groups that come before the stop event.
+// Here: the fold of the next update restricted to the groups ranked below
+// r_stop, added to the conductances now; the next run's first fold takes the
+// rest (fold_rank_lo).
+static int stop_delivery(edlut_network *n, int64_t next, uint32_t r_stop) {
+    edlut_population *p = n->pop;
+    if (n->sources.empty()) return 0;
+    if (n->fold_ready >= 0) {
+        // events folded ahead for the next update: folded again, in two parts
+        HIP_TRY(hipMemsetAsync(p->fill, 0, sizeof(uint32_t) * 3 * p->stride, p->stream));
+        HIP_TRY(hipMemsetAsync(p->tile_flag, 0, sizeof(uint32_t) * (p->stride / kFlagTile), p->stream));
+        n->fold_ready = -1;
+    }
+    RingEvents re = ring_events(n);
+    re.step = StepRef{nullptr, (int32_t)next};
+    re.rtab = n->d_rtab;
+    re.rank_lo = 0;
+    re.rank_hi = r_stop;
+    FoldTarget ft = net_fold_target(n);
+    int rc = launch_ring_fold(ft, p->stream, re, n->fold_lanes, n->fold_blocks);
+    if (rc) return rc;
+    const int nb = (int)std::min<int64_t>(4096, ((int64_t)p->n + kBlock - 1) / kBlock);
+    hipLaunchKernelGGL(k_apply_fold, dim3(nb), dim3(kBlock), 0, p->stream, ft, (int32_t)p->n, (int)(next & 1));
+    HIP_TRY(hipGetLastError());
+    n->fold_rank_lo = r_stop;
+    return 0;
+}
+
+// The next run's first fold, without the groups delivered at the last stop.
+static int resume_delivery(edlut_network *n, int64_t first) {
+    if (n->fold_rank_lo == 0 || n->sources.empty()) return 0;
+    RingEvents re = ring_events(n);
+    re.step = StepRef{nullptr, (int32_t)first};
+    re.rtab = n->d_rtab;
+    re.rank_lo = n->fold_rank_lo;
+    re.rank_hi = 0xffffffffu;
+    int rc = launch_ring_fold(net_fold_target(n), n->pop->stream, re, n->fold_lanes, n->fold_blocks);
+    if (rc) return rc;
+    n->fold_ready = first;
+    n->fold_rank_lo = 0;
+    return 0;
+}
+
+// After a run with an explicit stop time: the rank counter when its stop
+// event left (0 or less: no group came before it).
+static int stop_rank(EqSim *S, hipStream_t st, uint32_t *r_stop) {
+    int32_t v = -1;
+    HIP_TRY(hipMemcpyAsync(&v, S->st + 2, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    *r_stop = v > 0 ? (uint32_t)v : 0u;
+    return 0;
+}
+
 static void run_finish(edlut_network *n, int64_t first_step, int n_steps) {
     if (n->clk) n->timed = n->clk_count;
     n->last_step = first_step + n_steps - 1;
@@ -2587,10 +2684,12 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     UpdateFn fn = nullptr;
     int rc = run_prepare(n, first_step, n_steps, &fn);
     if (rc) return rc;
+    const double stop = n->stop_next;
+    n->stop_next = -1.0;
     if (n->eq_on && !p->grouped) {
-        if ((rc = eq_prepare_net(n, first_step)) || (rc = eq_resume(n->eq, first_step, n_steps, n->stop_next, p->stream)))
+        if ((rc = eq_prepare_net(n, first_step)) || (rc = eq_resume(n->eq, first_step, n_steps, stop, p->stream)) ||
+            (rc = resume_delivery(n, first_step)))
             return rc;
-        n->stop_next = -1.0;
     }
     const RingEvents re = ring_events(n);
     int64_t k = first_step;
@@ -2616,6 +2715,11 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     }
     HIP_TRY(hipGetLastError());
     run_finish(n, first_step, n_steps);
+    if (n->eq_on && !p->grouped && stop >= 0) {
+        uint32_t r_stop = 0;
+        if ((rc = stop_rank(n->eq, p->stream, &r_stop))) return rc;
+        if (r_stop > 0 && (rc = stop_delivery(n, first_step + n_steps, r_stop))) return rc;
+    }
     return 0;
 }
 
@@ -2785,8 +2889,11 @@ int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps) {
             for (size_t j = 0; j < m; j++) res[j] = ring_events(c->nets[j]);
         }
         if ((rc = eq_resume(c->eq, first_step, n_steps, c->stop_next, c->stream))) return rc;
-        c->stop_next = -1.0;
+        for (edlut_network *n : c->nets)
+            if ((rc = resume_delivery(n, first_step))) return rc;
     }
+    const double stop = c->stop_next;
+    c->stop_next = -1.0;
     const EqSim *eq = eq_on ? c->eq : nullptr;
     if (use_graph && n_steps >= kGraphChunk) {
         if (rebuild && (rc = build_circuit_graph(c, fns, res, eq))) return rc;
@@ -2813,7 +2920,20 @@ int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps) {
     }
     HIP_TRY(hipGetLastError());
     for (size_t j = 0; j < m; j++) run_finish(c->nets[j], first_step, n_steps);
+    if (eq && stop >= 0) {
+        uint32_t r_stop = 0;
+        if ((rc = stop_rank(c->eq, c->stream, &r_stop))) return rc;
+        if (r_stop > 0)
+            for (edlut_network *n : c->nets)
+                if ((rc = stop_delivery(n, first_step + n_steps, r_stop))) return rc;
+    }
     return 0;
+}
+
+int edlut_circuit_event_queue(edlut_circuit *c, double *t, uint32_t *key, int64_t cap, int64_t *size) {
+    if (!c || !size) return fail(EDLUT_E_ARG, "null argument");
+    HIP_TRY(hipSetDevice(c->device));
+    return eq_dump(c->eq, c->stream, t, key, cap, size);
 }
 
 int edlut_circuit_set_stop_time(edlut_circuit *c, double t) {
@@ -3156,6 +3276,8 @@ int edlut_network_reset(edlut_network *n) {
     HIP_TRY(hipMemsetAsync(n->log_count, 0, 2 * sizeof(int64_t), p->stream));
     HIP_TRY(hipMemsetAsync(p->err, 0, sizeof(uint32_t), p->stream));
     HIP_TRY(hipMemsetAsync(n->d_time, 0, sizeof(double) * n->ring, p->stream));
+    HIP_TRY(hipMemsetAsync(n->d_time_p, 0, sizeof(double) * n->ring, p->stream));
+    n->fold_rank_lo = 0;
     for (Source &s : n->sources) {
         if (!s.stdp) continue;
         if (s.w0.size() == (size_t)s.nsyn && s.nsyn)

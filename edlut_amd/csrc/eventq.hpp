@@ -134,31 +134,153 @@ struct EqArgs {
     int ring;
     const double *delay_s;     // [max_delay + 1] RoundPropagationDelay of d steps
     EqItem *heap;              // [cap + 1], 1-based
-    int32_t *st;               // [0] heap size, [1] rank counter, [2] unused, [3..3+n_em) next update per emitter
+    int32_t *st;               // [0] heap size, [1] rank counter, [2] rank counter when the run's stop event
+                               // left (-1: not yet), [3..3+n_em) next update per emitter
     int32_t cap;
     double stop_t;             // resuming call: the run's stop time
     int resume;                // 1: insert the pending input spikes and the stop event first
     int with_stop;             // resuming call: 0 when the run continues the previous one (no stop event)
     StepRef step;              // update k just run (resuming call: the update before the run)
-    uint32_t *err;             // bit 4: heap capacity, bit 5: event of an unknown kind
+    uint32_t *err;             // bit 6: heap capacity, bit 7: event of an unknown kind
 };
 
 __device__ inline bool eq_has(const uint32_t *m, int d) { return (m[(d - 1) >> 5] >> ((d - 1) & 31)) & 1u; }
 
 constexpr int kMaxDelayEq = 32 * kEqMaskWords;
 
-// The replay of one window.  One workgroup of kBlock threads.
+// The heap operations of one window, on one lane (always inlined, so that a
+// heap staged in LDS is addressed as LDS).
+struct EqWin {
+    int32_t n;          // heap size
+    uint32_t r;         // rank counter of the window
+    int32_t stop_r;     // rank counter when the stop event left (-1: it did not)
+    bool overflow, unknown;
+};
+
+__device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &a, const EqEmitter *em,
+                                          const uint32_t (*cnt)[kMaxDelayEq], uint32_t *cur, int32_t *next_k,
+                                          int32_t k, EqWin &w) {
+    const int par = (k + 1) & 1;          // parity of the update that consumes this window
+    auto push = [&](const EqItem &x) {
+        if (w.n >= a.cap) {
+            w.overflow = true;
+            return;
+        }
+        eq_insert(h, &w.n, x);
+    };
+    if (a.resume) {
+        // input spikes added since the last run (Simulation_API::
+        // AddExternalSpikeActivity) are queued in load order, then the run's
+        // stop event
+        int32_t pos[kEqMaxEmitters];
+        for (int e = 0; e < a.n_em; e++) pos[e] = em[e].in_lo;
+        for (;;) {
+            int be = -1;
+            int64_t bs = 0;
+            for (int e = 0; e < a.n_em; e++) {
+                if (em[e].kind != kEqTrain || pos[e] >= em[e].in_hi) continue;
+                const int64_t sq = em[e].in_seq[pos[e]];
+                if (be < 0 || sq < bs) {
+                    be = e;
+                    bs = sq;
+                }
+            }
+            if (be < 0) break;
+            const EqEmitter &E = em[be];
+            const int32_t q = pos[be]++;
+            if (E.nd_ptr[E.in_neuron[q] + 1] > E.nd_ptr[E.in_neuron[q]])
+                push(eq_item(E.in_t[q], kEqPrioSpike, kEqInput, (uint32_t)be, 0, (uint32_t)q));
+        }
+        if (a.with_stop) push(eq_item(a.stop_t, kEqPrioStop, kEqStop, 0, 0, 0));
+    }
+    EqItem it;
+    while (w.n > 0 && !w.overflow) {
+        const uint32_t top_a = h[1].a;
+        const uint32_t kind = (top_a >> 24) & 0xfu, e = (top_a >> 16) & 0xffu;
+        // the window ends where the populations' next update starts
+        if (kind == kEqTime && em[e].kind == kEqPop && next_k[e] > k) break;
+        eq_remove(h, &w.n, &it);
+        if (kind == kEqStop) {
+            w.stop_r = (int32_t)w.r;
+            break;
+        }
+        const EqEmitter &E = em[e];
+        if (kind == kEqTime) {
+            // TimeDrivenInternalSpike::ProcessEvent: full groups as they fill
+            // (ascending list position, then ascending delay), then the rest
+            // in ascending delay
+            const int32_t ke = next_k[e];
+            if (E.kind == kEqPop) w.r = 0;   // a window starts with the populations' updates
+            const int slot = ke % a.ring;
+            const bool use_bnd = E.bnd_on[slot] != 0;
+            const uint32_t *c_e = cnt[e];
+            for (int d = 1; d <= a.max_delay; d++) cur[d - 1] = 0u;
+            for (;;) {
+                int32_t best = 0x7fffffff;
+                int bd = 0;
+                for (int d = 1; d <= a.max_delay; d++) {
+                    const uint32_t c = cur[d - 1];
+                    if (c >= c_e[d - 1] / (uint32_t)kEqChunk) continue;
+                    const int32_t end = use_bnd ? E.bnd[((int64_t)slot * a.max_delay + (d - 1)) * E.cmax + c]
+                                                : (int32_t)(c * kEqChunk + kEqChunk - 1);
+                    if (end < best) {
+                        best = end;
+                        bd = d;
+                    }
+                }
+                if (!bd) break;
+                push(eq_item(it.t + a.delay_s[bd], kEqPrioSpike, kEqGroup, e, (uint32_t)bd, cur[bd - 1]));
+                cur[bd - 1]++;
+            }
+            for (int d = 1; d <= a.max_delay; d++)
+                if (c_e[d - 1] % (uint32_t)kEqChunk)
+                    push(eq_item(it.t + a.delay_s[d], kEqPrioSpike, kEqGroup, e, (uint32_t)d, cur[d - 1]));
+            next_k[e] = ke + 1;
+            push(eq_item(it.t + E.h, kEqPrioTime, kEqTime, e, 0, 0));
+        } else if (kind == kEqGroup) {
+            E.rank[((int64_t)par * a.max_delay + (eq_d(it) - 1)) * E.cmax + it.b] = w.r++;
+        } else if (kind == kEqInput) {
+            // InputSpike::ProcessEvent: the neuron's first delay
+            const int32_t nr = E.in_neuron[it.b];
+            const int d0 = E.nd_list[E.nd_ptr[nr]];
+            push(eq_item(it.t + a.delay_s[d0], kEqPrioSpike, kEqProp, e, 0, it.b));
+        } else if (kind == kEqProp) {
+            // PropagatedSpike::ProcessEvent: this delay, then the next one at
+            // time - delay_j + delay_{j+1}
+            const int32_t nr = E.in_neuron[it.b];
+            const uint32_t j = eq_d(it);
+            const int d = E.nd_list[E.nd_ptr[nr] + (int32_t)j];
+            E.rank[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + E.in_pos[it.b]] = w.r++;
+            if (E.nd_ptr[nr] + (int32_t)j + 1 < E.nd_ptr[nr + 1]) {
+                const int d2 = E.nd_list[E.nd_ptr[nr] + (int32_t)j + 1];
+                push(eq_item(it.t - a.delay_s[d] + a.delay_s[d2], kEqPrioSpike, kEqProp, e, j + 1, it.b));
+            }
+        } else {
+            w.unknown = true;
+            break;
+        }
+    }
+}
+
+// The replay of one window.  One workgroup of kBlock threads: the spikes per
+// delay are counted by all lanes, the heap operations run on one.
 __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
     __shared__ EqItem s_heap[kEqLdsItems + 1];
     __shared__ uint32_t s_cnt[kEqMaxEmitters][kMaxDelayEq];
     __shared__ uint32_t s_partial[kEqMaxEmitters];
     __shared__ uint32_t s_cur[kMaxDelayEq];
+    __shared__ int32_t s_next[kEqMaxEmitters];
+    __shared__ EqEmitter s_em[kEqMaxEmitters];
     __shared__ int32_t s_scan[kBlock];
     __shared__ int32_t s_run;
     __shared__ int32_t s_size;
     __shared__ int32_t s_in_lds;
     const int32_t k = a.step.get();
     if (threadIdx.x == 0) s_size = a.st[0];
+    if (threadIdx.x < a.n_em) {
+        s_em[threadIdx.x] = a.em[threadIdx.x];
+        s_next[threadIdx.x] = a.st[3 + threadIdx.x];
+    }
     for (int i = threadIdx.x; i < kEqMaxEmitters * kMaxDelayEq; i += kBlock) (&s_cnt[0][0])[i] = 0u;
     if (threadIdx.x < kEqMaxEmitters) s_partial[threadIdx.x] = 0u;
     __syncthreads();
@@ -166,9 +288,9 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
     // resuming) is still to leave the heap; s_partial: some spiking neuron
     // lacks a delay of its model's structure.
     for (int e = 0; e < a.n_em; e++) {
-        const EqEmitter &E = a.em[e];
+        const EqEmitter &E = s_em[e];
         if (E.kind == kEqTrain) continue;
-        const int32_t ke = a.st[3 + e];
+        const int32_t ke = s_next[e];
         if (ke > k || ke < 1) continue;
         const int slot = ke % a.ring;
         const int32_t ns = E.ring_cnt[slot];
@@ -191,12 +313,12 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
     __syncthreads();
     // 2. last member of every full chunk, where a delay has 1,024 spikes or
     // more and not every spiking neuron has every delay (else chunk c ends at
-    // list entry 1024c + 1023): a block scan per such delay.  An entry past
+    // list entry 1024c + 1023): a block scan per such delay.  The entry past
     // the last full chunk holds INT32_MAX.
     for (int e = 0; e < a.n_em; e++) {
-        const EqEmitter &E = a.em[e];
+        const EqEmitter &E = s_em[e];
         if (E.kind == kEqTrain) continue;
-        const int32_t ke = a.st[3 + e];
+        const int32_t ke = s_next[e];
         if (ke > k || ke < 1) continue;
         const int slot = ke % a.ring;
         if (threadIdx.x == 0) E.bnd_on[slot] = (uint8_t)s_partial[e];
@@ -234,11 +356,22 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
             }
         }
     }
-    // 3. stage the heap in LDS when it cannot outgrow it in this call
+    // 3. a call that is not resuming starts a new window: its rank table
+    // (by the parity of the update that consumes it) is cleared first, so
+    // that groups still queued when a stop event leaves hold no rank
+    if (!a.resume) {
+        const int par = (k + 1) & 1;
+        for (int e = 0; e < a.n_em; e++) {
+            const EqEmitter &E = s_em[e];
+            const int64_t half = (int64_t)a.max_delay * E.cmax;
+            for (int64_t i = threadIdx.x; i < half; i += kBlock) E.rank[par * half + i] = 0xffffffffu;
+        }
+    }
+    // 4. stage the heap in LDS when it cannot outgrow it in this call
     if (threadIdx.x == 0) {
         int64_t bound = (int64_t)s_size + a.n_em + 2;
         for (int e = 0; e < a.n_em; e++) {
-            const EqEmitter &E = a.em[e];
+            const EqEmitter &E = s_em[e];
             if (E.kind == kEqTrain) {
                 if (a.resume) bound += E.in_hi - E.in_lo;
                 continue;
@@ -249,117 +382,21 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
     }
     __syncthreads();
     const bool in_lds = s_in_lds != 0;
-    EqItem *h = in_lds ? s_heap : a.heap;
     if (in_lds)
         for (int i = threadIdx.x; i <= s_size; i += kBlock) s_heap[i] = a.heap[i];
     __syncthreads();
-    // 4. the heap operations, on one lane
+    // 5. the heap operations, on one lane
     if (threadIdx.x == 0) {
-        int32_t n = s_size;
-        int32_t *next_k = a.st + 3;
-        uint32_t r = (uint32_t)a.st[1];
-        const int par = (k + 1) & 1;          // parity of the update that consumes this window
-        bool overflow = false, unknown = false;
-        auto push = [&](const EqItem &x) {
-            if (n >= a.cap) {
-                overflow = true;
-                return;
-            }
-            eq_insert(h, &n, x);
-        };
-        if (a.resume) {
-            // input spikes added since the last run (Simulation_API::
-            // AddExternalSpikeActivity) are queued in load order, then the
-            // run's stop event
-            int32_t cur[kEqMaxEmitters];
-            for (int e = 0; e < a.n_em; e++) cur[e] = a.em[e].in_lo;
-            for (;;) {
-                int be = -1;
-                int64_t bs = 0;
-                for (int e = 0; e < a.n_em; e++) {
-                    const EqEmitter &E = a.em[e];
-                    if (E.kind != kEqTrain || cur[e] >= E.in_hi) continue;
-                    if (be < 0 || E.in_seq[cur[e]] < bs) {
-                        be = e;
-                        bs = E.in_seq[cur[e]];
-                    }
-                }
-                if (be < 0) break;
-                const EqEmitter &E = a.em[be];
-                const int32_t q = cur[be]++;
-                if (E.nd_ptr[E.in_neuron[q] + 1] > E.nd_ptr[E.in_neuron[q]])
-                    push(eq_item(E.in_t[q], kEqPrioSpike, kEqInput, (uint32_t)be, 0, (uint32_t)q));
-            }
-            if (a.with_stop) push(eq_item(a.stop_t, kEqPrioStop, kEqStop, 0, 0, 0));
-        }
-        EqItem it;
-        while (n > 0 && !overflow) {
-            const EqItem top = h[1];
-            const uint32_t kind = eq_kind(top), e = eq_em(top);
-            // the window ends where the populations' next update starts
-            if (kind == kEqTime && a.em[e].kind == kEqPop && next_k[e] > k) break;
-            eq_remove(h, &n, &it);
-            if (kind == kEqStop) break;
-            const EqEmitter &E = a.em[e];
-            if (kind == kEqTime) {
-                // TimeDrivenInternalSpike::ProcessEvent: full groups as they
-                // fill (ascending list position, then ascending delay), then
-                // the rest in ascending delay
-                const int32_t ke = next_k[e];
-                if (E.kind == kEqPop) r = 0;   // a window starts with the populations' updates
-                const int slot = ke % a.ring;
-                const bool use_bnd = E.bnd_on[slot] != 0;
-                for (int d = 1; d <= a.max_delay; d++) s_cur[d - 1] = 0u;
-                for (;;) {
-                    int32_t best = 0x7fffffff;
-                    int bd_ = 0;
-                    for (int d = 1; d <= a.max_delay; d++) {
-                        const uint32_t c = s_cur[d - 1];
-                        if (c >= s_cnt[e][d - 1] / (uint32_t)kEqChunk) continue;
-                        const int32_t end = use_bnd ? E.bnd[((int64_t)slot * a.max_delay + (d - 1)) * E.cmax + c]
-                                                    : (int32_t)(c * kEqChunk + kEqChunk - 1);
-                        if (end < best) {
-                            best = end;
-                            bd_ = d;
-                        }
-                    }
-                    if (!bd_) break;
-                    push(eq_item(it.t + a.delay_s[bd_], kEqPrioSpike, kEqGroup, e, (uint32_t)bd_, s_cur[bd_ - 1]));
-                    s_cur[bd_ - 1]++;
-                }
-                for (int d = 1; d <= a.max_delay; d++)
-                    if (s_cnt[e][d - 1] % (uint32_t)kEqChunk)
-                        push(eq_item(it.t + a.delay_s[d], kEqPrioSpike, kEqGroup, e, (uint32_t)d, s_cur[d - 1]));
-                next_k[e] = ke + 1;
-                push(eq_item(it.t + E.h, kEqPrioTime, kEqTime, e, 0, 0));
-            } else if (kind == kEqGroup) {
-                E.rank[((int64_t)par * a.max_delay + (eq_d(it) - 1)) * E.cmax + it.b] = r++;
-            } else if (kind == kEqInput) {
-                // InputSpike::ProcessEvent: the neuron's first delay
-                const int32_t nr = E.in_neuron[it.b];
-                const int d0 = E.nd_list[E.nd_ptr[nr]];
-                push(eq_item(it.t + a.delay_s[d0], kEqPrioSpike, kEqProp, e, 0, it.b));
-            } else if (kind == kEqProp) {
-                // PropagatedSpike::ProcessEvent: this delay, then the next one
-                // at time - delay_j + delay_{j+1}
-                const int32_t nr = E.in_neuron[it.b];
-                const uint32_t j = eq_d(it);
-                const int d = E.nd_list[E.nd_ptr[nr] + (int32_t)j];
-                E.rank[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + E.in_pos[it.b]] = r++;
-                if (E.nd_ptr[nr] + (int32_t)j + 1 < E.nd_ptr[nr + 1]) {
-                    const int d2 = E.nd_list[E.nd_ptr[nr] + (int32_t)j + 1];
-                    push(eq_item(it.t - a.delay_s[d] + a.delay_s[d2], kEqPrioSpike, kEqProp, e, j + 1, it.b));
-                }
-            } else {
-                unknown = true;
-                break;
-            }
-        }
-        s_size = n;
-        a.st[0] = n;
-        a.st[1] = (int32_t)r;
-        if (overflow) atomicOr(a.err, 16u);
-        if (unknown) atomicOr(a.err, 32u);
+        EqWin w{s_size, (uint32_t)a.st[1], -1, false, false};
+        if (in_lds) eq_window(s_heap, a, s_em, s_cnt, s_cur, s_next, k, w);
+        else eq_window(a.heap, a, s_em, s_cnt, s_cur, s_next, k, w);
+        s_size = w.n;
+        a.st[0] = w.n;
+        a.st[1] = (int32_t)w.r;
+        if (a.resume || w.stop_r >= 0) a.st[2] = w.stop_r;
+        for (int e = 0; e < a.n_em; e++) a.st[3 + e] = s_next[e];
+        if (w.overflow) atomicOr(a.err, 64u);
+        if (w.unknown) atomicOr(a.err, 128u);
     }
     __syncthreads();
     if (in_lds)

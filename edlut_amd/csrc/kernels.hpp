@@ -62,12 +62,17 @@ struct FoldTarget {
     // network mode: fixed segment of each key (prefix of the keys' synapse
     // counts; a synapse carries at most one event per step), null otherwise
     const uint64_t *seg_off;
-    // network mode with the event queue's order: rank of each group of the
-    // window that update k consumes, per source, [2][max_delay][cmax] by the
-    // parity of k (rank[0] null: canonical order)
+    // network mode with the event queue's order: the rank tables (null:
+    // canonical order)
+    const struct RankTab *rtab;
+};
+
+// Rank of each group of the window that update k consumes, per source,
+// [2][delays][cmax] by the parity of k (device memory, one per network).
+struct RankTab {
     const uint32_t *rank[4];
-    int32_t rank_cmax[4];
-    int32_t rank_delays;
+    int32_t cmax[4];
+    int32_t delays;
 };
 
 // Network-mode order key of an event: delay, source, chunk (the group of 1,024
@@ -143,22 +148,24 @@ __device__ __forceinline__ void fold_event(const FoldTarget &t, int par, uint64_
 
 // Network mode: folds the m >= 1 events of `key` (its fixed segment) into g
 // in order and clears the key's fill count.
-__device__ __attribute__((noinline)) float fold_static(const FoldTarget &t, int par, uint64_t key, uint32_t m,
-                                                       float g) {
-    uint32_t *fill = t.fill;
-    uint64_t *b_ord = t.b_ord;
-    float *b_w = t.b_w;
-    const uint64_t b = t.seg_off[key];
-    if (t.rank[0] && m >= 2u) {
+// Rank of the group an order key belongs to, in the window of the update of
+// parity par.
+__device__ __forceinline__ uint32_t key_rank(const RankTab *rt, int par, uint64_t o) {
+    const int d = (int)(o >> 56) + 1;
+    const int p = (int)((o >> 54) & 3u);
+    const uint32_t c = (uint32_t)((o >> kOrdSynBits) & ((1u << kOrdChunkBits) - 1u));
+    return rt->rank[p][((int64_t)par * rt->delays + (d - 1)) * rt->cmax[p] + c];
+}
+
+__device__ __attribute__((noinline)) float fold_static(uint32_t *fill, const uint64_t *seg_off, uint64_t *b_ord,
+                                                       float *b_w, const RankTab *rt, int par, uint64_t key,
+                                                       uint32_t m, float g) {
+    const uint64_t b = seg_off[key];
+    if (rt && m >= 2u) {
         // (delay, source, chunk) -> the group's rank in the event queue's window
         for (uint32_t j = 0; j < m; j++) {
             const uint64_t o = b_ord[b + j];
-            const int d = (int)(o >> 56) + 1;
-            const int p = (int)((o >> 54) & 3u);
-            const uint32_t c = (uint32_t)((o >> kOrdSynBits) & ((1u << kOrdChunkBits) - 1u));
-            const uint32_t r =
-                t.rank[p][((int64_t)par * t.rank_delays + (d - 1)) * t.rank_cmax[p] + c];
-            b_ord[b + j] = ((uint64_t)r << kOrdSynBits) | (o & kOrdSynMask);
+            b_ord[b + j] = ((uint64_t)key_rank(rt, par, o) << kOrdSynBits) | (o & kOrdSynMask);
         }
     }
     for (uint32_t j = 1; j < m; j++) {
@@ -373,7 +380,9 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
                 const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)idx[n];
                 if (a.fold.seg_off) {
                     const uint32_t m = a.fold.fill[key];
-                    if (m >= 1u) s[n][M::EXC + ty] = fold_static(a.fold, step & 1, key, m, s[n][M::EXC + ty]);
+                    if (m >= 1u)
+                        s[n][M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
+                                                        a.fold.rtab, step & 1, key, m, s[n][M::EXC + ty]);
                     continue;
                 }
                 const uint32_t m = a.fold.cnt[key];
@@ -533,7 +542,9 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
                 const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)i;
                 if (a.fold.seg_off) {
                     const uint32_t m = a.fold.fill[key];
-                    if (m >= 1u) sf[M::EXC + ty] = fold_static(a.fold, step & 1, key, m, sf[M::EXC + ty]);
+                    if (m >= 1u)
+                        sf[M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
+                                                      a.fold.rtab, step & 1, key, m, sf[M::EXC + ty]);
                     continue;
                 }
                 const uint32_t m = a.fold.cnt[key];
@@ -700,6 +711,7 @@ struct RingSource {
     const int32_t *bnd;
     const uint8_t *bnd_on;
     int32_t cmax;
+    int poisson;               // a Poisson device: its groups' times are on its own clock
 };
 struct RingEvents {
     RingSource src[kMaxSources];
@@ -707,10 +719,17 @@ struct RingEvents {
     int max_delay;
     int ring;
     StepRef step;              // update index about to run
-    // STDP only: update time of each ring slot, delay of d steps in seconds
+    // STDP only: update time of each ring slot (populations; Poisson
+    // devices on their own 0.1 ms clock), delay of d steps in seconds
     const double *time_ring;   // [ring]
+    const double *time_ring_p; // [ring]
     const double *delay_s;     // [max_delay + 1]
     const float *exp_lut;
+    // only the groups whose rank in the window lies in [rank_lo, rank_hi)
+    // (a run that stops inside the window delivers the groups before its stop
+    // event at its end, the rest with the next update); rtab null: all
+    const RankTab *rtab;
+    uint32_t rank_lo, rank_hi;
 };
 
 // STDPState::SetNewUpdateTime (STDPState.cpp:74-82): both traces decay by
@@ -793,8 +812,13 @@ __device__ __forceinline__ void fold_ring_body(const FoldTarget &t, const RingEv
             }
         }
         const uint64_t ord_hi = ord_key(d, p, chunk, 0);
+        if (r.rtab) {
+            const uint32_t rk = key_rank(r.rtab, step & 1, ord_hi);
+            if (rk < r.rank_lo || rk >= r.rank_hi) continue;
+        }
         double t_arr = 0.0;
-        if (PASS != kCount && src.stdp) t_arr = r.time_ring[s_slot[g]] + r.delay_s[d];
+        if (PASS != kCount && src.stdp)
+            t_arr = (src.poisson ? r.time_ring_p[s_slot[g]] : r.time_ring[s_slot[g]]) + r.delay_s[d];
         for (int64_t q = b + lane; q < e; q += LANES) {
             const uint32_t tt = src.tgt_type[q];
             const uint64_t key =
@@ -884,10 +908,27 @@ __global__ __launch_bounds__(kBlock) void k_stdp_post(StdpPostArgs a) {
 
 // Update time of step k: t_k = t_{k-1} + double(float dt), kept per ring slot
 // (TimeEventUpdateNeurons.cpp:78).  First node of a step of a plastic network.
-__global__ void k_tick(StepRef step, double *time_ring, int ring, double dt) {
+// The Poisson devices' clock advances by their own step in a second ring.
+__global__ void k_tick(StepRef step, double *time_ring, double *time_ring_p, int ring, double dt, double dt_p) {
     if (threadIdx.x == 0) {
         const int32_t k = step.get();
         time_ring[k % ring] = time_ring[(k - 1) % ring] + dt;
+        time_ring_p[k % ring] = time_ring_p[(k - 1) % ring] + dt_p;
+    }
+}
+
+// Adds every queued event of the population's keys to its conductances, in
+// order (the groups delivered before a run's stop event), and clears the
+// keys and fold flags.
+__global__ __launch_bounds__(kBlock) void k_apply_fold(FoldTarget t, int32_t n, int par) {
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        for (int ty = 0; ty < 3; ty++) {
+            const uint64_t key = (uint64_t)ty * (uint64_t)t.stride + (uint64_t)i;
+            const uint32_t m = t.fill[key];
+            if (m == 0u) continue;
+            float *g = g_ptr(t, key);
+            *g = fold_static(t.fill, t.seg_off, t.b_ord, t.b_w, t.rtab, par, key, m, *g);
+        }
     }
 }
 

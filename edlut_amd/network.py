@@ -113,6 +113,11 @@ class Network:
         """Stop time (s) of the next run (default: its last update's time)."""
         C.check(C.lib().edlut_network_set_stop_time(self._h, float(t)))
 
+    def event_queue(self):
+        """The replayed event queue's heap as stored: (times, keys), key =
+        priority << 28 | kind << 24 | model << 16 | delay."""
+        return _queue(C.lib().edlut_network_event_queue, self._h)
+
     def set_spike_train_times(self, source: int, times, order=None, load_first=None):
         """Times (s) of a spike train's input spikes, aligned with its
         schedule's entries, their load order (order[j] ranks entry j) and the
@@ -391,6 +396,16 @@ def cerebellum_from_spec(spec, device: int = 0):
     return (gr, go, pc), (ng, no, npc), circuit
 
 
+def _queue(fn, h):
+    size = ctypes.c_int64(0)
+    fn(h, None, None, 0, C.byref(size))   # the size (an error when the queue is not empty)
+    n = size.value
+    t = np.zeros(max(n, 1), np.float64)
+    k = np.zeros(max(n, 1), np.uint32)
+    C.check(fn(h, C.ptr(t), C.ptr(k), n, C.byref(size)))
+    return t[:n], k[:n]
+
+
 class Circuit:
     """Networks advanced in lockstep on one stream: update k of every member,
     in the order given, before update k + 1 of any (edlut_circuit_*)."""
@@ -406,6 +421,10 @@ class Circuit:
     def set_stop_time(self, t: float):
         """Stop time (s) of the next run (default: its last update's time)."""
         C.check(C.lib().edlut_circuit_set_stop_time(self._h, float(t)))
+
+    def event_queue(self):
+        """The circuit's replayed event queue (see Network.event_queue)."""
+        return _queue(C.lib().edlut_circuit_event_queue, self._h)
 
     def run(self, n_steps: int, first_step: int | None = None, sync: bool = True):
         if first_step is None:

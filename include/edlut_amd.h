@@ -256,6 +256,15 @@ int edlut_network_set_event_order(edlut_network *net, int on);
  * shares the queue; Simulation.cpp:322-373).  Default: the last update's
  * time; -2: no stop event (the run and the next one are one reference run). */
 int edlut_network_set_stop_time(edlut_network *net, double t);
+/* The replayed event queue of a network that runs alone (of a circuit:
+ * edlut_circuit_event_queue) as its heap stores it: event times and keys
+ * (priority << 28 | kind << 24 | model << 16 | delay; kind 0 update event,
+ * 1 spike group, 2 input spike, 3 propagated spike (delay: the index of the
+ * neuron's delay), 4 stop), *size events (an error when cap is smaller).
+ * No counterpart in the reference, whose queue is EventQueue's heap; for
+ * checking the replay against it. */
+int edlut_network_event_queue(edlut_network *net, double *t, uint32_t *key, int64_t cap, int64_t *size);
+int edlut_circuit_event_queue(edlut_circuit *c, double *t, uint32_t *key, int64_t cap, int64_t *size);
 /* Times (s) of a spike train's input spikes, aligned with the entries of its
  * schedule, their load order (order[j] ranks entry j; null: schedule order)
  * and when they are queued (load_first[j] = 0: when the simulation starts,

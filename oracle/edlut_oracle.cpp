@@ -2315,6 +2315,28 @@ int orc_net_set_train(void *net, int source, int64_t n_steps, const int64_t *ste
     return 0;
 }
 
+/* The event queue of the simulation the network runs in, as stored (heap
+ * array order): time, priority, kind (0 update, 1 group, 2 input spike,
+ * 3 propagated spike, 4 stop), model and delay (steps; propagated spikes:
+ * the delay's index).  Returns the number of events. */
+int64_t orc_net_event_queue(void *net, double *t, int32_t *prio, int32_t *kind, int32_t *em, int32_t *d,
+                            int64_t cap) {
+    Net *n = (Net *)net;
+    if (!n->sim) return 0;
+    const auto &h = n->sim->heap.e;
+    const int64_t m = (int64_t)h.size() - 1;
+    for (int64_t i = 0; i < m && i < cap; i++) {
+        const evq::Item &it = h[(size_t)i + 1];
+        const evq::Ev &ev = n->sim->pool[(size_t)it.ev];
+        t[i] = it.t;
+        prio[i] = it.prio;
+        kind[i] = ev.kind;
+        em[i] = ev.em < 0 ? 0 : ev.em;
+        d[i] = ev.kind == evq::K_PROP ? ev.j : ev.d;
+    }
+    return m;
+}
+
 /* Rank of the network's population among the models (default: its
  * position in the circuit). */
 int orc_net_set_model_rank(void *net, int rank) {

@@ -61,6 +61,8 @@ def lib():
         l.orc_net_set_train.argtypes = [P, ctypes.c_int, ctypes.c_int64, P, P]
         l.orc_net_set_stop_time.argtypes = [P, ctypes.c_double]
         l.orc_net_set_model_rank.argtypes = [P, ctypes.c_int]
+        l.orc_net_event_queue.restype = ctypes.c_int64
+        l.orc_net_event_queue.argtypes = [P, P, P, P, P, P, ctypes.c_int64]
         l.orc_net_take_poisson_log.restype = ctypes.c_int64
         l.orc_net_take_poisson_log.argtypes = [P, ctypes.c_int, P, P, ctypes.c_int64]
         l.orc_heap_replay.restype = ctypes.c_int64
@@ -300,6 +302,15 @@ class Net:
         order at start; default: its position in the circuit)."""
         if lib().orc_net_set_model_rank(self.h, int(rank)) != 0:
             raise ValueError("simulation already started")
+
+    def event_queue(self):
+        """The simulation's event heap as stored: (time, priority, kind,
+        model, delay) arrays (kind 0 update, 1 group, 2 input, 3 propagated
+        spike, 4 stop)."""
+        m = lib().orc_net_event_queue(self.h, None, None, None, None, None, 0)
+        out = [np.zeros(m, np.float64)] + [np.zeros(m, np.int32) for _ in range(4)]
+        lib().orc_net_event_queue(self.h, *[_p(x) for x in out], m)
+        return tuple(out)
 
     def set_stop_time(self, t: float):
         """Stop time (s) of the next run (default: its last update's time;

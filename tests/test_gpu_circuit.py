@@ -102,4 +102,9 @@ def test_circuit_rules():
         na.run(5)
     c.run(20)
     c.close()
-    na.run(5, first_step=21)   # on its own again once the circuit is gone
+    # the event queue belonged to the circuit: the network does not continue
+    # that run on its own, but runs again from a reset
+    with pytest.raises(E.EDLUTException, match="restarts at update 1"):
+        na.run(5, first_step=21)
+    na.reset()
+    na.run(5)

@@ -15,14 +15,54 @@ pytestmark = pytest.mark.gpu
 
 
 def _run_both(oracle, spec, params, steps, chunks=1):
+    """The same runs on both sides (each run is one RunSimulationSlot: its
+    stop event is part of the event queue)."""
     model, net = E.from_spec(spec)
     pop, onet = oracle.from_spec(spec, params, np_out_csr)
     per = steps // chunks
+    ost, oix = [], []
     for c in range(chunks):
         net.run(per)
+        a, b = onet.run(1 + c * per, per)
+        ost.append(a)
+        oix.append(b)
     st, ix = net.spikes()
-    ost, oix = onet.run(1, per * chunks)
-    return model, net, pop, (st, ix), (ost, oix)
+    return model, net, pop, (st, ix), (np.concatenate(ost), np.concatenate(oix))
+
+
+def _keys(oracle_queue):
+    t, prio, kind, em, d = oracle_queue
+    return t, ((prio.astype(np.uint32) << 28) | (kind.astype(np.uint32) << 24) | (em.astype(np.uint32) << 16)
+               | d.astype(np.uint32))
+
+
+def test_event_queue_replay_equals_oracle(oracle):
+    """The device's replay of the reference's event queue holds the same heap
+    (times, priorities, kinds, models, delays, in heap-array order) as the
+    oracle's event loop after every run, through bursts of more than 1,024
+    spikes per update (groups split and tied) and runs of uneven lengths."""
+    spec = workloads.adex_recurrent(n=6000, steps=700, seed=3, w_ext=1.0)
+    print("case event queue replay n=6000", flush=True)
+    model, net = E.from_spec(spec)
+    pop, onet = oracle.from_spec(spec, ADEX_DEFAULTS, np_out_csr)
+    k = 1
+    burst = 0
+    for n in (1, 37, 100, 16, 200, 3, 250, 93):
+        net.run(n)
+        ost, oix = onet.run(k, n)
+        k += n
+        t_d, key_d = net.event_queue()
+        t_o, key_o = _keys(onet.event_queue())
+        assert len(t_d) == len(t_o), f"queue sizes differ after update {k - 1}"
+        assert np.array_equal(t_d.view(np.uint64), t_o.view(np.uint64)), f"event times differ after update {k - 1}"
+        assert np.array_equal(key_d, key_o), f"event keys differ after update {k - 1}"
+        if len(ost):
+            burst = max(burst, int(np.bincount(ost).max()))
+    st, ix = net.spikes()
+    ost, oix = onet.take_log() if False else (None, None)
+    assert burst > 1024, "the workload should burst past one group"
+    for v in range(pop.nv):
+        assert np.array_equal(bits(model.GetVectorNeuronState(v)), bits(pop.state(v))), f"variable {v}"
 
 
 def test_lif_poisson_config(oracle):
