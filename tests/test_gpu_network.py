@@ -58,8 +58,6 @@ def test_event_queue_replay_equals_oracle(oracle):
         assert np.array_equal(key_d, key_o), f"event keys differ after update {k - 1}"
         if len(ost):
             burst = max(burst, int(np.bincount(ost).max()))
-    st, ix = net.spikes()
-    ost, oix = onet.take_log() if False else (None, None)
     assert burst > 1024, "the workload should burst past one group"
     for v in range(pop.nv):
         assert np.array_equal(bits(model.GetVectorNeuronState(v)), bits(pop.state(v))), f"variable {v}"
