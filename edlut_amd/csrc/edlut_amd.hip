@@ -507,7 +507,7 @@ struct edlut_network {
     int32_t eq_cmax[kMaxSources] = {};
     RankTab *d_rtab = nullptr;       // the rank tables of the sources, in device memory
     double *d_time_p = nullptr;      // [ring] update times of the Poisson devices' clock
-    uint32_t fold_rank_lo = 0;       // the next run's first fold skips groups ranked below (delivered at a stop)
+    uint32_t *d_skip = nullptr;      // [2]: update whose fold skips the groups ranked below [1] (delivered at a stop)
     const int32_t *eq_bnd[kMaxSources] = {};
     const uint8_t *eq_bnd_on[kMaxSources] = {};
     double time_of(int64_t k) {
@@ -787,7 +787,7 @@ int eq_build(EqSim *S, const std::vector<edlut_network *> &nets, uint32_t *err, 
             rt.cmax[si] = E.dev.cmax;
         }
         HIP_TRY(hipMemcpy(n->d_rtab, &rt, sizeof(RankTab), hipMemcpyHostToDevice));
-        n->fold_rank_lo = 0;
+        HIP_TRY(hipMemset(n->d_skip, 0, 2 * sizeof(uint32_t)));
     }
     // heap capacity: the groups of every model for every update in flight,
     // the input spikes, the update and stop events
@@ -1932,7 +1932,7 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
         (rc = dalloc(&n->log_step, n->log_cap)) || (rc = dalloc(&n->log_idx, n->log_cap)) ||
         (rc = dalloc(&n->log_count, 2)) || (rc = dalloc(&n->d_step, 1)) ||
         (rc = dalloc(&n->d_time, n->ring)) || (rc = dalloc(&n->d_time_p, n->ring)) ||
-        (rc = dalloc(&n->d_rtab, 1)) || (rc = dalloc(&n->d_delay_s, max_delay + 1))) {
+        (rc = dalloc(&n->d_rtab, 1)) || (rc = dalloc(&n->d_skip, 2)) || (rc = dalloc(&n->d_delay_s, max_delay + 1))) {
         delete n;
         return rc;
     }
@@ -1948,6 +1948,7 @@ int edlut_network_create(edlut_population *pop, int max_delay, edlut_network **o
         HIP_TRY(hipMemset(n->d_time, 0, sizeof(double) * n->ring));
         HIP_TRY(hipMemset(n->d_time_p, 0, sizeof(double) * n->ring));
         HIP_TRY(hipMemset(n->d_rtab, 0, sizeof(RankTab)));
+        HIP_TRY(hipMemset(n->d_skip, 0, 2 * sizeof(uint32_t)));
     }
     HIP_TRY(hipMemset(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring));
     HIP_TRY(hipMemset(n->log_count, 0, 2 * sizeof(int64_t)));
@@ -1978,7 +1979,7 @@ int edlut_network_destroy(edlut_network *n) {
     }
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
-                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_time_p, n->d_rtab, n->d_delay_s,
+                   n->log_idx, n->log_count, n->d_step, n->d_time, n->d_time_p, n->d_rtab, n->d_skip, n->d_delay_s,
                    n->seg_off, n->clk};
     for (void *q : dev)
         if (q) (void)hipFree(q);
@@ -2428,8 +2429,8 @@ static RingEvents ring_events(edlut_network *n) {
     re.time_ring = n->d_time;
     re.time_ring_p = n->d_time_p;
     re.delay_s = n->d_delay_s;
-    re.rtab = nullptr;
-    re.rank_lo = 0;
+    re.rtab = (n->eq_on && !n->pop->grouped && !n->sources.empty() && n->eq_rank[0]) ? n->d_rtab : nullptr;
+    re.skip = n->d_skip;
     re.rank_hi = 0xffffffffu;
     re.exp_lut = n->pop->exp_lut;
     for (int q = 0; q < re.n_sources; q++) {
@@ -2616,8 +2617,8 @@ static int fold_ahead(edlut_network *n, const RingEvents &re, int64_t k) {
 // (Simulation_API::RunSimulation with an end time past that update) delivers,
 // in the reference, the window's groups that come before the stop event.
 // Here: the fold of the next update restricted to the groups ranked below
-// r_stop, added to the conductances now; the next run's first fold takes the
-// rest (fold_rank_lo).
+// r_stop, added to the conductances now; the next update's fold skips them
+// (d_skip).
 static int stop_delivery(edlut_network *n, int64_t next, uint32_t r_stop) {
     edlut_population *p = n->pop;
     if (n->sources.empty()) return 0;
@@ -2627,10 +2628,10 @@ static int stop_delivery(edlut_network *n, int64_t next, uint32_t r_stop) {
         HIP_TRY(hipMemsetAsync(p->tile_flag, 0, sizeof(uint32_t) * (p->stride / kFlagTile), p->stream));
         n->fold_ready = -1;
     }
+    const uint32_t skip[2] = {(uint32_t)next, r_stop};
     RingEvents re = ring_events(n);
     re.step = StepRef{nullptr, (int32_t)next};
-    re.rtab = n->d_rtab;
-    re.rank_lo = 0;
+    re.skip = nullptr;
     re.rank_hi = r_stop;
     FoldTarget ft = net_fold_target(n);
     int rc = launch_ring_fold(ft, p->stream, re, n->fold_lanes, n->fold_blocks);
@@ -2638,22 +2639,8 @@ static int stop_delivery(edlut_network *n, int64_t next, uint32_t r_stop) {
     const int nb = (int)std::min<int64_t>(4096, ((int64_t)p->n + kBlock - 1) / kBlock);
     hipLaunchKernelGGL(k_apply_fold, dim3(nb), dim3(kBlock), 0, p->stream, ft, (int32_t)p->n, (int)(next & 1));
     HIP_TRY(hipGetLastError());
-    n->fold_rank_lo = r_stop;
-    return 0;
-}
-
-// The next run's first fold, without the groups delivered at the last stop.
-static int resume_delivery(edlut_network *n, int64_t first) {
-    if (n->fold_rank_lo == 0 || n->sources.empty()) return 0;
-    RingEvents re = ring_events(n);
-    re.step = StepRef{nullptr, (int32_t)first};
-    re.rtab = n->d_rtab;
-    re.rank_lo = n->fold_rank_lo;
-    re.rank_hi = 0xffffffffu;
-    int rc = launch_ring_fold(net_fold_target(n), n->pop->stream, re, n->fold_lanes, n->fold_blocks);
-    if (rc) return rc;
-    n->fold_ready = first;
-    n->fold_rank_lo = 0;
+    HIP_TRY(hipMemcpyAsync(n->d_skip, skip, sizeof(skip), hipMemcpyHostToDevice, p->stream));
+    HIP_TRY(hipStreamSynchronize(p->stream));
     return 0;
 }
 
@@ -2687,8 +2674,7 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     const double stop = n->stop_next;
     n->stop_next = -1.0;
     if (n->eq_on && !p->grouped) {
-        if ((rc = eq_prepare_net(n, first_step)) || (rc = eq_resume(n->eq, first_step, n_steps, stop, p->stream)) ||
-            (rc = resume_delivery(n, first_step)))
+        if ((rc = eq_prepare_net(n, first_step)) || (rc = eq_resume(n->eq, first_step, n_steps, stop, p->stream)))
             return rc;
     }
     const RingEvents re = ring_events(n);
@@ -2889,8 +2875,6 @@ int edlut_circuit_run(edlut_circuit *c, int64_t first_step, int n_steps) {
             for (size_t j = 0; j < m; j++) res[j] = ring_events(c->nets[j]);
         }
         if ((rc = eq_resume(c->eq, first_step, n_steps, c->stop_next, c->stream))) return rc;
-        for (edlut_network *n : c->nets)
-            if ((rc = resume_delivery(n, first_step))) return rc;
     }
     const double stop = c->stop_next;
     c->stop_next = -1.0;
@@ -3277,7 +3261,7 @@ int edlut_network_reset(edlut_network *n) {
     HIP_TRY(hipMemsetAsync(p->err, 0, sizeof(uint32_t), p->stream));
     HIP_TRY(hipMemsetAsync(n->d_time, 0, sizeof(double) * n->ring, p->stream));
     HIP_TRY(hipMemsetAsync(n->d_time_p, 0, sizeof(double) * n->ring, p->stream));
-    n->fold_rank_lo = 0;
+    HIP_TRY(hipMemsetAsync(n->d_skip, 0, 2 * sizeof(uint32_t), p->stream));
     for (Source &s : n->sources) {
         if (!s.stdp) continue;
         if (s.w0.size() == (size_t)s.nsyn && s.nsyn)

@@ -159,7 +159,7 @@ struct EqWin {
 
 __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &a, const EqEmitter *em,
                                           const uint32_t (*cnt)[kMaxDelayEq], uint32_t *cur, int32_t *next_k,
-                                          int32_t k, EqWin &w) {
+                                          const double *delay_s, int32_t k, EqWin &w) {
     const int par = (k + 1) & 1;          // parity of the update that consumes this window
     auto push = [&](const EqItem &x) {
         if (w.n >= a.cap) {
@@ -212,10 +212,14 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             const int32_t ke = next_k[e];
             if (E.kind == kEqPop) w.r = 0;   // a window starts with the populations' updates
             const int slot = ke % a.ring;
-            const bool use_bnd = E.bnd_on[slot] != 0;
             const uint32_t *c_e = cnt[e];
-            for (int d = 1; d <= a.max_delay; d++) cur[d - 1] = 0u;
-            for (;;) {
+            bool any_full = false;
+            for (int d = 1; d <= a.max_delay; d++) {
+                cur[d - 1] = 0u;
+                any_full = any_full || c_e[d - 1] >= (uint32_t)kEqChunk;
+            }
+            const bool use_bnd = any_full && E.bnd_on[slot] != 0;
+            while (any_full) {
                 int32_t best = 0x7fffffff;
                 int bd = 0;
                 for (int d = 1; d <= a.max_delay; d++) {
@@ -229,12 +233,12 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
                     }
                 }
                 if (!bd) break;
-                push(eq_item(it.t + a.delay_s[bd], kEqPrioSpike, kEqGroup, e, (uint32_t)bd, cur[bd - 1]));
+                push(eq_item(it.t + delay_s[bd], kEqPrioSpike, kEqGroup, e, (uint32_t)bd, cur[bd - 1]));
                 cur[bd - 1]++;
             }
             for (int d = 1; d <= a.max_delay; d++)
                 if (c_e[d - 1] % (uint32_t)kEqChunk)
-                    push(eq_item(it.t + a.delay_s[d], kEqPrioSpike, kEqGroup, e, (uint32_t)d, cur[d - 1]));
+                    push(eq_item(it.t + delay_s[d], kEqPrioSpike, kEqGroup, e, (uint32_t)d, cur[d - 1]));
             next_k[e] = ke + 1;
             push(eq_item(it.t + E.h, kEqPrioTime, kEqTime, e, 0, 0));
         } else if (kind == kEqGroup) {
@@ -243,7 +247,7 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             // InputSpike::ProcessEvent: the neuron's first delay
             const int32_t nr = E.in_neuron[it.b];
             const int d0 = E.nd_list[E.nd_ptr[nr]];
-            push(eq_item(it.t + a.delay_s[d0], kEqPrioSpike, kEqProp, e, 0, it.b));
+            push(eq_item(it.t + delay_s[d0], kEqPrioSpike, kEqProp, e, 0, it.b));
         } else if (kind == kEqProp) {
             // PropagatedSpike::ProcessEvent: this delay, then the next one at
             // time - delay_j + delay_{j+1}
@@ -253,7 +257,7 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             E.rank[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + E.in_pos[it.b]] = w.r++;
             if (E.nd_ptr[nr] + (int32_t)j + 1 < E.nd_ptr[nr + 1]) {
                 const int d2 = E.nd_list[E.nd_ptr[nr] + (int32_t)j + 1];
-                push(eq_item(it.t - a.delay_s[d] + a.delay_s[d2], kEqPrioSpike, kEqProp, e, j + 1, it.b));
+                push(eq_item(it.t - delay_s[d] + delay_s[d2], kEqPrioSpike, kEqProp, e, j + 1, it.b));
             }
         } else {
             w.unknown = true;
@@ -275,7 +279,9 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
     __shared__ int32_t s_run;
     __shared__ int32_t s_size;
     __shared__ int32_t s_in_lds;
+    __shared__ double s_delay[kMaxDelayEq + 1];
     const int32_t k = a.step.get();
+    for (int d = threadIdx.x; d <= a.max_delay; d += kBlock) s_delay[d] = a.delay_s[d];
     if (threadIdx.x == 0) s_size = a.st[0];
     if (threadIdx.x < a.n_em) {
         s_em[threadIdx.x] = a.em[threadIdx.x];
@@ -388,8 +394,8 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
     // 5. the heap operations, on one lane
     if (threadIdx.x == 0) {
         EqWin w{s_size, (uint32_t)a.st[1], -1, false, false};
-        if (in_lds) eq_window(s_heap, a, s_em, s_cnt, s_cur, s_next, k, w);
-        else eq_window(a.heap, a, s_em, s_cnt, s_cur, s_next, k, w);
+        if (in_lds) eq_window(s_heap, a, s_em, s_cnt, s_cur, s_next, s_delay, k, w);
+        else eq_window(a.heap, a, s_em, s_cnt, s_cur, s_next, s_delay, k, w);
         s_size = w.n;
         a.st[0] = w.n;
         a.st[1] = (int32_t)w.r;

@@ -725,11 +725,13 @@ struct RingEvents {
     const double *time_ring_p; // [ring]
     const double *delay_s;     // [max_delay + 1]
     const float *exp_lut;
-    // only the groups whose rank in the window lies in [rank_lo, rank_hi)
-    // (a run that stops inside the window delivers the groups before its stop
-    // event at its end, the rest with the next update); rtab null: all
+    // a run that stops inside a window delivers the groups ranked before its
+    // stop event at its end (rank_hi) and the rest with the next update:
+    // that update's fold skips the groups ranked below skip[1] when the
+    // update is skip[0] (device memory, set by the host); rtab null: all
     const RankTab *rtab;
-    uint32_t rank_lo, rank_hi;
+    const uint32_t *skip;
+    uint32_t rank_hi;
 };
 
 // STDPState::SetNewUpdateTime (STDPState.cpp:74-82): both traces decay by
@@ -761,6 +763,7 @@ __device__ __forceinline__ void fold_ring_body(const FoldTarget &t, const RingEv
     const int nseg = r.max_delay * r.n_sources;
     const int32_t step = r.step.get();
     const int par = t.par_by_step ? (step & 1) : t.par;
+    const uint32_t rank_lo = (r.rtab && r.skip && r.skip[0] == (uint32_t)step) ? r.skip[1] : 0u;
     for (int g = threadIdx.x; g < nseg; g += kBlock) {
         const int d = g / r.n_sources + 1;
         const int p = g % r.n_sources;
@@ -812,9 +815,9 @@ __device__ __forceinline__ void fold_ring_body(const FoldTarget &t, const RingEv
             }
         }
         const uint64_t ord_hi = ord_key(d, p, chunk, 0);
-        if (r.rtab) {
+        if (rank_lo > 0u || r.rank_hi != 0xffffffffu) {
             const uint32_t rk = key_rank(r.rtab, step & 1, ord_hi);
-            if (rk < r.rank_lo || rk >= r.rank_hi) continue;
+            if (rk < rank_lo || rk >= r.rank_hi) continue;
         }
         double t_arr = 0.0;
         if (PASS != kCount && src.stdp)
