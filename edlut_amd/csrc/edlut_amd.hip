@@ -18,7 +18,6 @@
 #include "adjoint.hpp"
 #include "generate.hpp"
 #include "kernels.hpp"
-#include "eventq.hpp"
 
 using namespace edl;
 
@@ -526,6 +525,7 @@ struct edlut_network {
 // circuit: one emitter per population, Poisson model and spike-train model.
 struct EqHostEm {
     int kind = kEqPop, rank = 0, key = -1;
+    edlut_network *net = nullptr;   // a population's network (its update tiles)
     std::vector<std::pair<edlut_network *, int>> copies;   // (network, source) it feeds
     EqEmitter dev{};
     uint32_t *dmask = nullptr;
@@ -666,6 +666,7 @@ int eq_build(EqSim *S, const std::vector<edlut_network *> &nets, uint32_t *err, 
         E.dev.ring_idx = n->own_ring_idx;
         E.dev.ring_cnt = n->own_ring_cnt;
         E.dev.slot_len = n->pop->n;
+        E.net = n;
         S->em.push_back(E);
     }
     int next_key = 1 << 24;
@@ -801,7 +802,7 @@ int eq_build(EqSim *S, const std::vector<edlut_network *> &nets, uint32_t *err, 
     }
     if (cap > (1ll << 28)) return fail(EDLUT_E_CAP, "event queue too large");
     S->cap = (int32_t)cap;
-    if ((rc = dalloc(&S->heap, (size_t)cap + 1)) || (rc = dalloc(&S->st, 3 + S->em.size())) ||
+    if ((rc = dalloc(&S->heap, (size_t)cap + 1)) || (rc = dalloc(&S->st, kEqSt + S->em.size())) ||
         (rc = dalloc(&S->d_em, S->em.size())))
         return rc;
     // initial heap: the input spikes queued before the first run in load
@@ -848,10 +849,11 @@ int eq_build(EqSim *S, const std::vector<edlut_network *> &nets, uint32_t *err, 
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return S->em[(size_t)a].rank < S->em[(size_t)b].rank; });
     for (int e : order) eq_insert(h.data(), &hn, eq_item(S->em[(size_t)e].dev.h, kEqPrioTime, kEqTime, (uint32_t)e, 0, 0));
     HIP_TRY(hipMemcpy(S->heap, h.data(), sizeof(EqItem) * ((size_t)hn + 1), hipMemcpyHostToDevice));
-    std::vector<int32_t> stv(3 + S->em.size(), 1);
+    std::vector<int32_t> stv(kEqSt + S->em.size(), 1);
     stv[0] = hn;
     stv[1] = 0;
-    stv[2] = 0;
+    stv[2] = -1;
+    stv[3] = -1;
     HIP_TRY(hipMemcpy(S->st, stv.data(), sizeof(int32_t) * stv.size(), hipMemcpyHostToDevice));
     std::vector<EqEmitter> dev;
     for (EqHostEm &E : S->em) {
@@ -877,6 +879,8 @@ EqArgs eq_args(const EqSim *S, StepRef step, int resume, double stop_t) {
     a.cap = S->cap;
     a.stop_t = stop_t;
     a.resume = resume;
+    a.with_stop = 1;
+    a.clear_step = -1;
     a.step = step;
     a.err = S->err;
     return a;
@@ -931,11 +935,23 @@ int eq_resume(EqSim *S, int64_t first, int n_steps, double stop, hipStream_t st)
         E.in_loaded = hi;
         changed = true;
     }
+    // the populations' update tiles (their layout follows the launch shape)
+    for (EqHostEm &E : S->em) {
+        if (E.kind != kEqPop) continue;
+        edlut_population *p = E.net->pop;
+        E.dev.tile_count = E.net->tile_count;
+        E.dev.staging = E.net->staging;
+        E.dev.n_tiles = p->update_tiles();
+        E.dev.tile_size = kBlock * p->items;
+    }
     std::vector<EqEmitter> dev, dev2;
     for (EqHostEm &E : S->em) dev.push_back(E.dev);
     HIP_TRY(hipMemcpy(S->d_em, dev.data(), sizeof(EqEmitter) * dev.size(), hipMemcpyHostToDevice));
     EqArgs ra = eq_args(S, StepRef{nullptr, (int32_t)(first - 1)}, 1, stop);
     ra.with_stop = with_stop ? 1 : 0;
+    // an explicit stop time may fall inside the last window: its groups
+    // still queued at the stop must hold no rank
+    ra.clear_step = (with_stop && stop > t_last) ? (int32_t)last : -1;
     hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, st, ra);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(st));
@@ -1274,6 +1290,16 @@ RingFoldFn pick_ring_fold(int pass, int lanes) {
         case 16: return ring_fold_fn<16>(pass);
         case 32: return ring_fold_fn<32>(pass);
         case 64: return ring_fold_fn<64>(pass);
+        default: return nullptr;
+    }
+}
+using CompactFoldEqFn = void (*)(CompactArgs, FoldTarget, RingEvents, int, EqArgs);
+CompactFoldEqFn pick_compact_fold_eq(int lanes) {
+    switch (lanes) {
+        case 8: return k_compact_fold_eq<8>;
+        case 16: return k_compact_fold_eq<16>;
+        case 32: return k_compact_fold_eq<32>;
+        case 64: return k_compact_fold_eq<64>;
         default: return nullptr;
     }
 }
@@ -2390,12 +2416,22 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     // STDP orders the post-synaptic learning of update k before the arrivals
     // of update k + 1, so plastic networks fold in their own step.
     *fused = ring && !n->plastic;
+    // the event-queue replay of this update's window runs in the compaction
+    // kernel's last workgroup when its sizes fit there
+    const bool eq_fused = *fused && eq && eq->max_delay <= kEqFusedDelays;
     if (*fused) {
         RingEvents rn = re0;
         rn.step = StepRef{step.ptr, step.imm + 1};
-        CompactFoldFn f = pick_compact_fold(n->fold_lanes);
-        if (!f) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
-        hipLaunchKernelGGL(f, dim3(nbc + n->fold_blocks), dim3(kBlock), 0, p->stream, c, ft, rn, nbc);
+        if (eq_fused) {
+            CompactFoldEqFn f = pick_compact_fold_eq(n->fold_lanes);
+            if (!f) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
+            hipLaunchKernelGGL(f, dim3(nbc + n->fold_blocks + 1), dim3(kBlock), 0, p->stream, c, ft, rn, nbc,
+                               eq_args(eq, step, 0, 0.0));
+        } else {
+            CompactFoldFn f = pick_compact_fold(n->fold_lanes);
+            if (!f) return fail(EDLUT_E_ARG, "fold lanes must be 8, 16, 32 or 64");
+            hipLaunchKernelGGL(f, dim3(nbc + n->fold_blocks), dim3(kBlock), 0, p->stream, c, ft, rn, nbc);
+        }
     } else {
         hipLaunchKernelGGL(k_compact, dim3(nbc), dim3(kBlock), 0, p->stream, c);
     }
@@ -2416,7 +2452,7 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         pa.exp_lut = p->exp_lut;
         hipLaunchKernelGGL(k_stdp_post<64>, dim3(64), dim3(kBlock), 0, p->stream, pa);
     }
-    if (eq) hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, p->stream, eq_args(eq, step, 0, 0.0));
+    if (eq && !eq_fused) hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, p->stream, eq_args(eq, step, 0, 0.0));
     return 0;
 }
 

@@ -51,47 +51,52 @@ __host__ __device__ inline uint32_t eq_em(const EqItem &x) { return (x.a >> 16) 
 __host__ __device__ inline uint32_t eq_d(const EqItem &x) { return x.a & 0xffffu; }
 
 // EventQueue::InsertEvent (EventQueue.cpp:121-141) on a 1-based array of
-// `*n` entries.
+// `*n` entries: the reference swaps the new element up while its parent is
+// later (or as late and of lower priority); the same moves, with the new
+// element held aside until its place is known.
+__host__ __device__ inline bool eq_later(const EqItem &p, const EqItem &c) {
+    return p.t > c.t || (p.t == c.t && eq_prio(p) < eq_prio(c));
+}
 __host__ __device__ inline void eq_insert(EqItem *h, int32_t *n, const EqItem &x) {
     int32_t c = ++*n;
-    h[c] = x;
     while (c > 1) {
-        const EqItem &p = h[c / 2];
-        const EqItem &q = h[c];
-        if (p.t > q.t || (p.t == q.t && eq_prio(p) < eq_prio(q))) {
-            const EqItem tmp = h[c / 2];
-            h[c / 2] = h[c];
-            h[c] = tmp;
-            c /= 2;
-        } else {
-            break;
-        }
+        const EqItem p = h[c / 2];
+        if (!eq_later(p, x)) break;
+        h[c] = p;
+        c /= 2;
     }
+    h[c] = x;
 }
 
-// EventQueue::RemoveEvent (EventQueue.cpp:162-198).
+// EventQueue::RemoveEvent (EventQueue.cpp:162-198): the last element moves
+// to the top and is swapped down with the earlier child (ties: the left one
+// unless the right has higher priority) while that child comes before it;
+// again with the moving element held aside.
 __host__ __device__ inline bool eq_remove(EqItem *h, int32_t *n, EqItem *out) {
     if (*n >= 2) {
         *out = h[1];
-        h[1] = h[*n];
+        const EqItem x = h[*n];
         --*n;
         const int32_t size = *n;
         int32_t p = 1, c;
         for (c = 2; c < size; p = c, c = p * 2) {
-            if (h[c].t > h[c + 1].t || (h[c].t == h[c + 1].t && eq_prio(h[c]) < eq_prio(h[c + 1]))) c++;
-            if (h[c].t < h[p].t || (h[c].t == h[p].t && eq_prio(h[c]) > eq_prio(h[p]))) {
-                const EqItem tmp = h[p];
-                h[p] = h[c];
-                h[c] = tmp;
-            } else {
-                break;
+            EqItem a = h[c];
+            const EqItem b = h[c + 1];
+            if (eq_later(a, b)) {
+                c++;
+                a = b;
+            }
+            if (a.t < x.t || (a.t == x.t && eq_prio(a) > eq_prio(x))) h[p] = a;
+            else break;
+        }
+        if (c == size) {
+            const EqItem a = h[c];
+            if (a.t < x.t || (a.t == x.t && eq_prio(a) > eq_prio(x))) {
+                h[p] = a;
+                p = c;
             }
         }
-        if (c == size && (h[c].t < h[p].t || (h[c].t == h[p].t && eq_prio(h[c]) > eq_prio(h[p])))) {
-            const EqItem tmp = h[p];
-            h[p] = h[c];
-            h[c] = tmp;
-        }
+        h[p] = x;
         return true;
     }
     if (*n == 1) {
@@ -110,11 +115,17 @@ struct EqEmitter {
     const int32_t *ring_idx;   // spike lists by update slot (populations, Poisson devices)
     const int32_t *ring_cnt;
     int64_t slot_len;
+    // the update's spikes still in the update kernel's tiles (a population
+    // replayed inside its own compaction kernel): tile counts and staged
+    // indices, ascending within a tile (tile_count null: the ring lists)
+    const int32_t *tile_count;
+    const int32_t *staging;
+    int32_t n_tiles, tile_size;
     const uint32_t *dmask;     // [n][words]: bit d-1 set when the neuron has synapses of delay d
     uint32_t full[kEqMaskWords];   // the model's delay structure (union of the neurons')
     int cmax;                  // rank-table entries per delay (chunks, or input spikes per update)
     uint32_t *rank;            // [2][max_delay][cmax]: rank in its window, by consuming-update parity
-    int32_t *bnd;              // [ring][max_delay][cmax]: list position where chunk c starts
+    int32_t *bnd;              // [ring][max_delay][cmax]: list position of the last member of chunk c
     uint8_t *bnd_on;           // [ring]: the slot's chunks are in bnd (else every 1,024 list entries)
     // input spikes (trains): records in load order
     const double *in_t;
@@ -126,6 +137,12 @@ struct EqEmitter {
     int32_t in_lo, in_hi;      // records inserted by a resuming call
 };
 
+// Replay state in device memory: [0] heap size, [1] rank counter of the
+// window, [2] the rank counter when the run's stop event left (-1: not yet),
+// [3] the update whose window clears its rank table first, [4..4+n_em) the
+// next update of each model.
+constexpr int kEqSt = 4;
+
 struct EqArgs {
     const EqEmitter *em;       // [n_em] in device memory
     int n_em;
@@ -134,12 +151,12 @@ struct EqArgs {
     int ring;
     const double *delay_s;     // [max_delay + 1] RoundPropagationDelay of d steps
     EqItem *heap;              // [cap + 1], 1-based
-    int32_t *st;               // [0] heap size, [1] rank counter, [2] rank counter when the run's stop event
-                               // left (-1: not yet), [3..3+n_em) next update per emitter
+    int32_t *st;
     int32_t cap;
     double stop_t;             // resuming call: the run's stop time
     int resume;                // 1: insert the pending input spikes and the stop event first
     int with_stop;             // resuming call: 0 when the run continues the previous one (no stop event)
+    int32_t clear_step;        // resuming call: the run's last update when its stop time is explicit, else -1
     StepRef step;              // update k just run (resuming call: the update before the run)
     uint32_t *err;             // bit 6: heap capacity, bit 7: event of an unknown kind
 };
@@ -157,8 +174,9 @@ struct EqWin {
     bool overflow, unknown;
 };
 
+template <int MAXD>
 __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &a, const EqEmitter *em,
-                                          const uint32_t (*cnt)[kMaxDelayEq], uint32_t *cur, int32_t *next_k,
+                                          const uint32_t (*cnt)[MAXD], uint32_t *cur, int32_t *next_k,
                                           const double *delay_s, int32_t k, EqWin &w) {
     const int par = (k + 1) & 1;          // parity of the update that consumes this window
     auto push = [&](const EqItem &x) {
@@ -266,30 +284,38 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
     }
 }
 
-// The replay of one window.  One workgroup of kBlock threads: the spikes per
-// delay are counted by all lanes, the heap operations run on one.
-__global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
-    __shared__ EqItem s_heap[kEqLdsItems + 1];
-    __shared__ uint32_t s_cnt[kEqMaxEmitters][kMaxDelayEq];
+// The replay of one window by one workgroup of kBlock threads: the spikes
+// per delay are counted by all lanes, the heap operations run on one.
+// LDS_ITEMS heap entries are staged in LDS when the heap cannot outgrow
+// them; MAXD bounds the delays (host-checked).
+template <int LDS_ITEMS, int MAXD>
+__device__ __forceinline__ void eq_body(const EqArgs &a) {
+    __shared__ EqItem s_heap[LDS_ITEMS + 1];
+    __shared__ uint32_t s_cnt[kEqMaxEmitters][MAXD];
     __shared__ uint32_t s_partial[kEqMaxEmitters];
-    __shared__ uint32_t s_cur[kMaxDelayEq];
+    __shared__ uint32_t s_cur[MAXD];
     __shared__ int32_t s_next[kEqMaxEmitters];
     __shared__ EqEmitter s_em[kEqMaxEmitters];
-    __shared__ int32_t s_scan[kBlock];
-    __shared__ int32_t s_run;
-    __shared__ int32_t s_size;
+    __shared__ int32_t s_scan[kBlock], s_scan2[kBlock];
+    __shared__ int32_t s_run, s_base;
+    __shared__ int32_t s_st[kEqSt];
     __shared__ int32_t s_in_lds;
-    __shared__ double s_delay[kMaxDelayEq + 1];
+    __shared__ double s_delay[MAXD + 1];
     const int32_t k = a.step.get();
-    for (int d = threadIdx.x; d <= a.max_delay; d += kBlock) s_delay[d] = a.delay_s[d];
-    if (threadIdx.x == 0) s_size = a.st[0];
+    // every independent load at once: state, models, delays
+    if (threadIdx.x < kEqSt) s_st[threadIdx.x] = a.st[threadIdx.x];
     if (threadIdx.x < a.n_em) {
         s_em[threadIdx.x] = a.em[threadIdx.x];
-        s_next[threadIdx.x] = a.st[3 + threadIdx.x];
+        s_next[threadIdx.x] = a.st[kEqSt + threadIdx.x];
     }
-    for (int i = threadIdx.x; i < kEqMaxEmitters * kMaxDelayEq; i += kBlock) (&s_cnt[0][0])[i] = 0u;
+    for (int d = threadIdx.x; d <= a.max_delay; d += kBlock) s_delay[d] = a.delay_s[d];
+    for (int i = threadIdx.x; i < kEqMaxEmitters * MAXD; i += kBlock) (&s_cnt[0][0])[i] = 0u;
     if (threadIdx.x < kEqMaxEmitters) s_partial[threadIdx.x] = 0u;
     __syncthreads();
+    const int32_t size0 = s_st[0];
+    // stage the heap in LDS speculatively (the check below may discard it)
+    if (size0 + 1 <= LDS_ITEMS)
+        for (int i = threadIdx.x; i <= size0; i += kBlock) s_heap[i] = a.heap[i];
     // 1. spikes per delay of every model whose update at k (or before, when
     // resuming) is still to leave the heap; s_partial: some spiking neuron
     // lacks a delay of its model's structure.
@@ -299,9 +325,7 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
         const int32_t ke = s_next[e];
         if (ke > k || ke < 1) continue;
         const int slot = ke % a.ring;
-        const int32_t ns = E.ring_cnt[slot];
-        for (int32_t r = threadIdx.x; r < ns; r += kBlock) {
-            const int32_t id = E.ring_idx[(int64_t)slot * E.slot_len + r];
+        auto count = [&](int32_t id) {
             const uint32_t *m = E.dmask + (int64_t)id * a.words;
             bool part = false;
             for (int w = 0; w < a.words; w++) {
@@ -314,13 +338,23 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
                 }
             }
             if (part) s_partial[e] = 1u;
+        };
+        if (E.tile_count && ke == k) {
+            for (int32_t t = threadIdx.x; t < E.n_tiles; t += kBlock) {
+                const int32_t c = E.tile_count[t];
+                for (int32_t j = 0; j < c; j++) count(E.staging[(int64_t)t * E.tile_size + j]);
+            }
+        } else {
+            const int32_t ns = E.ring_cnt[slot];
+            for (int32_t r = threadIdx.x; r < ns; r += kBlock) count(E.ring_idx[(int64_t)slot * E.slot_len + r]);
         }
     }
     __syncthreads();
-    // 2. last member of every full chunk, where a delay has 1,024 spikes or
+    // 2. last member of every full chunk where a delay has 1,024 spikes or
     // more and not every spiking neuron has every delay (else chunk c ends at
-    // list entry 1024c + 1023): a block scan per such delay.  The entry past
-    // the last full chunk holds INT32_MAX.
+    // list entry 1024c + 1023): a block scan per such delay over the list
+    // (ring list, or tiles in ascending order).  The entry past the last
+    // full chunk holds INT32_MAX.
     for (int e = 0; e < a.n_em; e++) {
         const EqEmitter &E = s_em[e];
         if (E.kind == kEqTrain) continue;
@@ -329,43 +363,72 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
         const int slot = ke % a.ring;
         if (threadIdx.x == 0) E.bnd_on[slot] = (uint8_t)s_partial[e];
         if (!s_partial[e]) continue;
-        const int32_t ns = E.ring_cnt[slot];
+        const bool tiles = E.tile_count && ke == k;
+        const int32_t ns = tiles ? E.n_tiles : E.ring_cnt[slot];   // list entries, or tiles
         for (int d = 1; d <= a.max_delay; d++) {
             const uint32_t full_c = s_cnt[e][d - 1] / (uint32_t)kEqChunk;
             if (full_c == 0) continue;
             int32_t *bd = E.bnd + ((int64_t)slot * a.max_delay + (d - 1)) * E.cmax;
             if (threadIdx.x == 0) {
                 s_run = 0;
+                s_base = 0;
                 if ((int)full_c < E.cmax) bd[full_c] = 0x7fffffff;
             }
             __syncthreads();
             for (int32_t r0 = 0; r0 < ns; r0 += kBlock) {
                 const int32_t r = r0 + (int32_t)threadIdx.x;
-                int32_t f = 0;
+                // members (f) and list entries (g) of this thread's entry or tile
+                int32_t f = 0, g = 0;
                 if (r < ns) {
-                    const int32_t id = E.ring_idx[(int64_t)slot * E.slot_len + r];
-                    f = eq_has(E.dmask + (int64_t)id * a.words, d) ? 1 : 0;
+                    if (tiles) {
+                        g = E.tile_count[r];
+                        for (int32_t j = 0; j < g; j++)
+                            f += eq_has(E.dmask + (int64_t)E.staging[(int64_t)r * E.tile_size + j] * a.words, d) ? 1 : 0;
+                    } else {
+                        g = 1;
+                        f = eq_has(E.dmask + (int64_t)E.ring_idx[(int64_t)slot * E.slot_len + r] * a.words, d) ? 1 : 0;
+                    }
                 }
                 s_scan[threadIdx.x] = f;
+                s_scan2[threadIdx.x] = g;
                 __syncthreads();
                 for (int off = 1; off < kBlock; off <<= 1) {
                     const int32_t v = threadIdx.x >= (unsigned)off ? s_scan[threadIdx.x - off] : 0;
+                    const int32_t v2 = threadIdx.x >= (unsigned)off ? s_scan2[threadIdx.x - off] : 0;
                     __syncthreads();
                     s_scan[threadIdx.x] += v;
+                    s_scan2[threadIdx.x] += v2;
                     __syncthreads();
                 }
-                const int32_t upto = s_run + s_scan[threadIdx.x];   // members up to entry r
-                if (f && upto % kEqChunk == 0 && upto / kEqChunk <= (int32_t)full_c) bd[upto / kEqChunk - 1] = r;
+                int32_t upto = s_run + s_scan[threadIdx.x] - f;          // members before this entry or tile
+                const int32_t pos0 = s_base + s_scan2[threadIdx.x] - g;  // its first list position
+                if (r < ns) {
+                    if (tiles) {
+                        for (int32_t j = 0; j < g; j++) {
+                            if (!eq_has(E.dmask + (int64_t)E.staging[(int64_t)r * E.tile_size + j] * a.words, d))
+                                continue;
+                            upto++;
+                            if (upto % kEqChunk == 0 && upto / kEqChunk <= (int32_t)full_c) bd[upto / kEqChunk - 1] = pos0 + j;
+                        }
+                    } else if (f) {
+                        upto++;
+                        if (upto % kEqChunk == 0 && upto / kEqChunk <= (int32_t)full_c) bd[upto / kEqChunk - 1] = pos0;
+                    }
+                }
                 __syncthreads();
-                if (threadIdx.x == kBlock - 1) s_run += s_scan[kBlock - 1];
+                if (threadIdx.x == kBlock - 1) {
+                    s_run += s_scan[kBlock - 1];
+                    s_base += s_scan2[kBlock - 1];
+                }
                 __syncthreads();
             }
         }
     }
-    // 3. a call that is not resuming starts a new window: its rank table
-    // (by the parity of the update that consumes it) is cleared first, so
-    // that groups still queued when a stop event leaves hold no rank
-    if (!a.resume) {
+    // 3. the window's rank table is cleared first when the run stops inside
+    // it (explicit stop time): groups still queued at the stop hold no rank
+    if (threadIdx.x == 0 && a.resume) s_st[3] = a.clear_step;
+    __syncthreads();
+    if (!a.resume && s_st[3] == k) {
         const int par = (k + 1) & 1;
         for (int e = 0; e < a.n_em; e++) {
             const EqEmitter &E = s_em[e];
@@ -373,9 +436,9 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
             for (int64_t i = threadIdx.x; i < half; i += kBlock) E.rank[par * half + i] = 0xffffffffu;
         }
     }
-    // 4. stage the heap in LDS when it cannot outgrow it in this call
+    // 4. the heap stays in LDS when it cannot outgrow it in this call
     if (threadIdx.x == 0) {
-        int64_t bound = (int64_t)s_size + a.n_em + 2;
+        int64_t bound = (int64_t)size0 + a.n_em + 2;
         for (int e = 0; e < a.n_em; e++) {
             const EqEmitter &E = s_em[e];
             if (E.kind == kEqTrain) {
@@ -384,30 +447,35 @@ __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) {
             }
             for (int d = 1; d <= a.max_delay; d++) bound += s_cnt[e][d - 1] / kEqChunk + 1;
         }
-        s_in_lds = bound <= kEqLdsItems ? 1 : 0;
+        s_in_lds = bound <= LDS_ITEMS ? 1 : 0;
     }
     __syncthreads();
     const bool in_lds = s_in_lds != 0;
-    if (in_lds)
-        for (int i = threadIdx.x; i <= s_size; i += kBlock) s_heap[i] = a.heap[i];
-    __syncthreads();
     // 5. the heap operations, on one lane
     if (threadIdx.x == 0) {
-        EqWin w{s_size, (uint32_t)a.st[1], -1, false, false};
-        if (in_lds) eq_window(s_heap, a, s_em, s_cnt, s_cur, s_next, s_delay, k, w);
-        else eq_window(a.heap, a, s_em, s_cnt, s_cur, s_next, s_delay, k, w);
-        s_size = w.n;
+        EqWin w{size0, (uint32_t)s_st[1], -1, false, false};
+        if (in_lds) eq_window<MAXD>(s_heap, a, s_em, s_cnt, s_cur, s_next, s_delay, k, w);
+        else eq_window<MAXD>(a.heap, a, s_em, s_cnt, s_cur, s_next, s_delay, k, w);
+        s_st[0] = w.n;
         a.st[0] = w.n;
         a.st[1] = (int32_t)w.r;
         if (a.resume || w.stop_r >= 0) a.st[2] = w.stop_r;
-        for (int e = 0; e < a.n_em; e++) a.st[3 + e] = s_next[e];
+        if (a.resume) a.st[3] = a.clear_step;
+        for (int e = 0; e < a.n_em; e++) a.st[kEqSt + e] = s_next[e];
         if (w.overflow) atomicOr(a.err, 64u);
         if (w.unknown) atomicOr(a.err, 128u);
     }
     __syncthreads();
     if (in_lds)
-        for (int i = threadIdx.x; i <= s_size; i += kBlock) a.heap[i] = s_heap[i];
+        for (int i = threadIdx.x; i <= s_st[0]; i += kBlock) a.heap[i] = s_heap[i];
 }
+
+// The replay as a kernel of its own (circuits, plastic networks, resuming calls).
+__global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) { eq_body<2048, kMaxDelayEq>(a); }
+
+// Sizes of the replay fused into a network's compaction kernel.
+constexpr int kEqFusedItems = 1024;
+constexpr int kEqFusedDelays = 64;
 
 }  // namespace edl
 

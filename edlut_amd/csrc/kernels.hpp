@@ -9,6 +9,7 @@
 #include <type_traits>
 #include "device_common.hpp"
 #include "device_models.hpp"
+#include "eventq.hpp"
 
 namespace edl {
 
@@ -854,6 +855,17 @@ __global__ __launch_bounds__(kBlock) void k_compact_fold(CompactArgs c, FoldTarg
                                                          int nb_compact) {
     if ((int)blockIdx.x < nb_compact) compact_body(c, blockIdx.x, nb_compact);
     else fold_ring_body<kStatic, LANES>(t, r, blockIdx.x - nb_compact, gridDim.x - nb_compact);
+}
+
+// The same with the event-queue replay of update k's window in the last
+// workgroup: it reads update k's spikes from the update kernel's tiles, so it
+// depends on nothing the other workgroups write and runs beside them.
+template <int LANES>
+__global__ __launch_bounds__(kBlock) void k_compact_fold_eq(CompactArgs c, FoldTarget t, RingEvents r,
+                                                            int nb_compact, EqArgs q) {
+    if ((int)blockIdx.x == (int)gridDim.x - 1) eq_body<kEqFusedItems, kEqFusedDelays>(q);
+    else if ((int)blockIdx.x < nb_compact) compact_body(c, blockIdx.x, nb_compact);
+    else fold_ring_body<kStatic, LANES>(t, r, blockIdx.x - nb_compact, gridDim.x - 1 - nb_compact);
 }
 
 // Synapses per (type, target) key of one source, for the fixed segments of
