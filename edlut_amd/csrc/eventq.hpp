@@ -340,9 +340,18 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
             if (part) s_partial[e] = 1u;
         };
         if (E.tile_count && ke == k) {
-            for (int32_t t = threadIdx.x; t < E.n_tiles; t += kBlock) {
-                const int32_t c = E.tile_count[t];
-                for (int32_t j = 0; j < c; j++) count(E.staging[(int64_t)t * E.tile_size + j]);
+            // tile counts eight at a time (the array is padded to a multiple
+            // of 8 * kBlock entries), so their loads are in flight together
+            for (int32_t t0 = 0; t0 < E.n_tiles; t0 += 8 * kBlock) {
+                int32_t c8[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) c8[u] = E.tile_count[t0 + u * kBlock + (int32_t)threadIdx.x];
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int32_t t = t0 + u * kBlock + (int32_t)threadIdx.x;
+                    if (t >= E.n_tiles) continue;
+                    for (int32_t j = 0; j < c8[u]; j++) count(E.staging[(int64_t)t * E.tile_size + j]);
+                }
             }
         } else {
             const int32_t ns = E.ring_cnt[slot];

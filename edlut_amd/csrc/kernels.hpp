@@ -857,15 +857,17 @@ __global__ __launch_bounds__(kBlock) void k_compact_fold(CompactArgs c, FoldTarg
     else fold_ring_body<kStatic, LANES>(t, r, blockIdx.x - nb_compact, gridDim.x - nb_compact);
 }
 
-// The same with the event-queue replay of update k's window in the last
-// workgroup: it reads update k's spikes from the update kernel's tiles, so it
-// depends on nothing the other workgroups write and runs beside them.
+// The same with the event-queue replay of update k's window in the first
+// workgroup (its chain of dependent steps is the longest): it reads update
+// k's spikes from the update kernel's tiles, so it depends on nothing the
+// other workgroups write and runs beside them.
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void k_compact_fold_eq(CompactArgs c, FoldTarget t, RingEvents r,
                                                             int nb_compact, EqArgs q) {
-    if ((int)blockIdx.x == (int)gridDim.x - 1) eq_body<kEqFusedItems, kEqFusedDelays>(q);
-    else if ((int)blockIdx.x < nb_compact) compact_body(c, blockIdx.x, nb_compact);
-    else fold_ring_body<kStatic, LANES>(t, r, blockIdx.x - nb_compact, gridDim.x - 1 - nb_compact);
+    const int b = (int)blockIdx.x - 1;
+    if (b < 0) eq_body<kEqFusedItems, kEqFusedDelays>(q);
+    else if (b < nb_compact) compact_body(c, b, nb_compact);
+    else fold_ring_body<kStatic, LANES>(t, r, b - nb_compact, (int)gridDim.x - 1 - nb_compact);
 }
 
 // Synapses per (type, target) key of one source, for the fixed segments of
