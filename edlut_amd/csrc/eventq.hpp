@@ -57,7 +57,7 @@ __host__ __device__ inline uint32_t eq_d(const EqItem &x) { return x.a & 0xffffu
 __host__ __device__ inline bool eq_later(const EqItem &p, const EqItem &c) {
     return p.t > c.t || (p.t == c.t && eq_prio(p) < eq_prio(c));
 }
-__host__ __device__ inline void eq_insert(EqItem *h, int32_t *n, const EqItem &x) {
+__host__ __device__ __forceinline__ void eq_insert(EqItem *h, int32_t *n, const EqItem &x) {
     int32_t c = ++*n;
     while (c > 1) {
         const EqItem p = h[c / 2];
@@ -72,7 +72,7 @@ __host__ __device__ inline void eq_insert(EqItem *h, int32_t *n, const EqItem &x
 // to the top and is swapped down with the earlier child (ties: the left one
 // unless the right has higher priority) while that child comes before it;
 // again with the moving element held aside.
-__host__ __device__ inline bool eq_remove(EqItem *h, int32_t *n, EqItem *out) {
+__host__ __device__ __forceinline__ bool eq_remove(EqItem *h, int32_t *n, EqItem *out) {
     if (*n >= 2) {
         *out = h[1];
         const EqItem x = h[*n];
@@ -161,7 +161,18 @@ struct EqArgs {
     uint32_t *err;             // bit 6: heap capacity, bit 7: event of an unknown kind
 };
 
-__device__ inline bool eq_has(const uint32_t *m, int d) { return (m[(d - 1) >> 5] >> ((d - 1) & 31)) & 1u; }
+// Pointers read from LDS (the staged model descriptions) are generic; the
+// replay casts them to global memory so that their loads and stores are not
+// flat accesses, which the LDS waits of the heap operations would also wait on.
+#define EQ_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ EQ_GLOBAL T *eq_g(T *p) {
+    return (EQ_GLOBAL T *)p;
+}
+
+__device__ inline bool eq_has(const uint32_t *m, int d) {
+    return (eq_g(m)[(d - 1) >> 5] >> ((d - 1) & 31)) & 1u;
+}
 
 constexpr int kMaxDelayEq = 32 * kEqMaskWords;
 
@@ -176,7 +187,8 @@ struct EqWin {
 
 template <int MAXD>
 __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &a, const EqEmitter *em,
-                                          const uint32_t (*cnt)[MAXD], uint32_t *cur, int32_t *next_k,
+                                          const uint32_t (*cnt)[MAXD], const uint32_t (*pmask)[MAXD / 32 + 1],
+                                          const uint32_t *anyfull, uint32_t *cur, int32_t *next_k,
                                           const double *delay_s, int32_t k, EqWin &w) {
     const int par = (k + 1) & 1;          // parity of the update that consumes this window
     auto push = [&](const EqItem &x) {
@@ -197,7 +209,7 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             int64_t bs = 0;
             for (int e = 0; e < a.n_em; e++) {
                 if (em[e].kind != kEqTrain || pos[e] >= em[e].in_hi) continue;
-                const int64_t sq = em[e].in_seq[pos[e]];
+                const int64_t sq = eq_g(em[e].in_seq)[pos[e]];
                 if (be < 0 || sq < bs) {
                     be = e;
                     bs = sq;
@@ -206,8 +218,8 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             if (be < 0) break;
             const EqEmitter &E = em[be];
             const int32_t q = pos[be]++;
-            if (E.nd_ptr[E.in_neuron[q] + 1] > E.nd_ptr[E.in_neuron[q]])
-                push(eq_item(E.in_t[q], kEqPrioSpike, kEqInput, (uint32_t)be, 0, (uint32_t)q));
+            if (eq_g(E.nd_ptr)[eq_g(E.in_neuron)[q] + 1] > eq_g(E.nd_ptr)[eq_g(E.in_neuron)[q]])
+                push(eq_item(eq_g(E.in_t)[q], kEqPrioSpike, kEqInput, (uint32_t)be, 0, (uint32_t)q));
         }
         if (a.with_stop) push(eq_item(a.stop_t, kEqPrioStop, kEqStop, 0, 0, 0));
     }
@@ -231,19 +243,17 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             if (E.kind == kEqPop) w.r = 0;   // a window starts with the populations' updates
             const int slot = ke % a.ring;
             const uint32_t *c_e = cnt[e];
-            bool any_full = false;
-            for (int d = 1; d <= a.max_delay; d++) {
-                cur[d - 1] = 0u;
-                any_full = any_full || c_e[d - 1] >= (uint32_t)kEqChunk;
-            }
-            const bool use_bnd = any_full && E.bnd_on[slot] != 0;
+            const bool any_full = anyfull[e] != 0u;
+            if (any_full)
+                for (int d = 1; d <= a.max_delay; d++) cur[d - 1] = 0u;
+            const bool use_bnd = any_full && eq_g(E.bnd_on)[slot] != 0;
             while (any_full) {
                 int32_t best = 0x7fffffff;
                 int bd = 0;
                 for (int d = 1; d <= a.max_delay; d++) {
                     const uint32_t c = cur[d - 1];
                     if (c >= c_e[d - 1] / (uint32_t)kEqChunk) continue;
-                    const int32_t end = use_bnd ? E.bnd[((int64_t)slot * a.max_delay + (d - 1)) * E.cmax + c]
+                    const int32_t end = use_bnd ? eq_g(E.bnd)[((int64_t)slot * a.max_delay + (d - 1)) * E.cmax + c]
                                                 : (int32_t)(c * kEqChunk + kEqChunk - 1);
                     if (end < best) {
                         best = end;
@@ -254,27 +264,35 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
                 push(eq_item(it.t + delay_s[bd], kEqPrioSpike, kEqGroup, e, (uint32_t)bd, cur[bd - 1]));
                 cur[bd - 1]++;
             }
-            for (int d = 1; d <= a.max_delay; d++)
-                if (c_e[d - 1] % (uint32_t)kEqChunk)
-                    push(eq_item(it.t + delay_s[d], kEqPrioSpike, kEqGroup, e, (uint32_t)d, cur[d - 1]));
+            // the partial groups: delays whose count is not a multiple of 1,024
+            for (int wd = 0; wd <= (a.max_delay - 1) / 32; wd++) {
+                uint32_t bits = pmask[e][wd];
+                while (bits) {
+                    const int d = wd * 32 + __builtin_ctz(bits) + 1;
+                    bits &= bits - 1u;
+                    push(eq_item(it.t + delay_s[d], kEqPrioSpike, kEqGroup, e, (uint32_t)d,
+                                 any_full ? cur[d - 1] : 0u));
+                }
+            }
             next_k[e] = ke + 1;
             push(eq_item(it.t + E.h, kEqPrioTime, kEqTime, e, 0, 0));
         } else if (kind == kEqGroup) {
-            E.rank[((int64_t)par * a.max_delay + (eq_d(it) - 1)) * E.cmax + it.b] = w.r++;
+            eq_g(E.rank)[((int64_t)par * a.max_delay + (eq_d(it) - 1)) * E.cmax + it.b] = w.r++;
         } else if (kind == kEqInput) {
             // InputSpike::ProcessEvent: the neuron's first delay
-            const int32_t nr = E.in_neuron[it.b];
-            const int d0 = E.nd_list[E.nd_ptr[nr]];
+            const int32_t nr = eq_g(E.in_neuron)[it.b];
+            const int d0 = eq_g(E.nd_list)[eq_g(E.nd_ptr)[nr]];
             push(eq_item(it.t + delay_s[d0], kEqPrioSpike, kEqProp, e, 0, it.b));
         } else if (kind == kEqProp) {
             // PropagatedSpike::ProcessEvent: this delay, then the next one at
             // time - delay_j + delay_{j+1}
-            const int32_t nr = E.in_neuron[it.b];
+            const int32_t nr = eq_g(E.in_neuron)[it.b];
             const uint32_t j = eq_d(it);
-            const int d = E.nd_list[E.nd_ptr[nr] + (int32_t)j];
-            E.rank[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + E.in_pos[it.b]] = w.r++;
-            if (E.nd_ptr[nr] + (int32_t)j + 1 < E.nd_ptr[nr + 1]) {
-                const int d2 = E.nd_list[E.nd_ptr[nr] + (int32_t)j + 1];
+            const int32_t np0 = eq_g(E.nd_ptr)[nr];
+            const int d = eq_g(E.nd_list)[np0 + (int32_t)j];
+            eq_g(E.rank)[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + eq_g(E.in_pos)[it.b]] = w.r++;
+            if (np0 + (int32_t)j + 1 < eq_g(E.nd_ptr)[nr + 1]) {
+                const int d2 = eq_g(E.nd_list)[np0 + (int32_t)j + 1];
                 push(eq_item(it.t - delay_s[d] + delay_s[d2], kEqPrioSpike, kEqProp, e, j + 1, it.b));
             }
         } else {
@@ -301,6 +319,8 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
     __shared__ int32_t s_st[kEqSt];
     __shared__ int32_t s_in_lds;
     __shared__ double s_delay[MAXD + 1];
+    __shared__ uint32_t s_pmask[kEqMaxEmitters][MAXD / 32 + 1];
+    __shared__ uint32_t s_anyfull[kEqMaxEmitters];
     const int32_t k = a.step.get();
     // every independent load at once: state, models, delays
     if (threadIdx.x < kEqSt) s_st[threadIdx.x] = a.st[threadIdx.x];
@@ -326,7 +346,7 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
         if (ke > k || ke < 1) continue;
         const int slot = ke % a.ring;
         auto count = [&](int32_t id) {
-            const uint32_t *m = E.dmask + (int64_t)id * a.words;
+            const EQ_GLOBAL uint32_t *m = eq_g(E.dmask) + (int64_t)id * a.words;
             bool part = false;
             for (int w = 0; w < a.words; w++) {
                 uint32_t bits = m[w];
@@ -345,20 +365,35 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
             for (int32_t t0 = 0; t0 < E.n_tiles; t0 += 8 * kBlock) {
                 int32_t c8[8];
 #pragma unroll
-                for (int u = 0; u < 8; u++) c8[u] = E.tile_count[t0 + u * kBlock + (int32_t)threadIdx.x];
+                for (int u = 0; u < 8; u++) c8[u] = eq_g(E.tile_count)[t0 + u * kBlock + (int32_t)threadIdx.x];
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
                     const int32_t t = t0 + u * kBlock + (int32_t)threadIdx.x;
                     if (t >= E.n_tiles) continue;
-                    for (int32_t j = 0; j < c8[u]; j++) count(E.staging[(int64_t)t * E.tile_size + j]);
+                    for (int32_t j = 0; j < c8[u]; j++) count(eq_g(E.staging)[(int64_t)t * E.tile_size + j]);
                 }
             }
         } else {
-            const int32_t ns = E.ring_cnt[slot];
-            for (int32_t r = threadIdx.x; r < ns; r += kBlock) count(E.ring_idx[(int64_t)slot * E.slot_len + r]);
+            const int32_t ns = eq_g(E.ring_cnt)[slot];
+            for (int32_t r = threadIdx.x; r < ns; r += kBlock) count(eq_g(E.ring_idx)[(int64_t)slot * E.slot_len + r]);
         }
     }
     __syncthreads();
+    // per model: delays with a partial group, and whether any group is full
+    for (int i = threadIdx.x; i < kEqMaxEmitters * (MAXD / 32 + 1); i += kBlock) {
+        const int e = i / (MAXD / 32 + 1), wd = i % (MAXD / 32 + 1);
+        uint32_t pm = 0u;
+        for (int b = 0; b < 32; b++) {
+            const int d = wd * 32 + b + 1;
+            if (d <= a.max_delay && (s_cnt[e][d - 1] % (uint32_t)kEqChunk) != 0u) pm |= 1u << b;
+        }
+        s_pmask[e][wd] = pm;
+    }
+    if (threadIdx.x < kEqMaxEmitters) {
+        uint32_t f = 0u;
+        for (int d = 1; d <= a.max_delay; d++) f |= s_cnt[threadIdx.x][d - 1] >= (uint32_t)kEqChunk ? 1u : 0u;
+        s_anyfull[threadIdx.x] = f;
+    }
     // 2. last member of every full chunk where a delay has 1,024 spikes or
     // more and not every spiking neuron has every delay (else chunk c ends at
     // list entry 1024c + 1023): a block scan per such delay over the list
@@ -370,14 +405,14 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
         const int32_t ke = s_next[e];
         if (ke > k || ke < 1) continue;
         const int slot = ke % a.ring;
-        if (threadIdx.x == 0) E.bnd_on[slot] = (uint8_t)s_partial[e];
+        if (threadIdx.x == 0) eq_g(E.bnd_on)[slot] = (uint8_t)s_partial[e];
         if (!s_partial[e]) continue;
         const bool tiles = E.tile_count && ke == k;
-        const int32_t ns = tiles ? E.n_tiles : E.ring_cnt[slot];   // list entries, or tiles
+        const int32_t ns = tiles ? E.n_tiles : eq_g(E.ring_cnt)[slot];   // list entries, or tiles
         for (int d = 1; d <= a.max_delay; d++) {
             const uint32_t full_c = s_cnt[e][d - 1] / (uint32_t)kEqChunk;
             if (full_c == 0) continue;
-            int32_t *bd = E.bnd + ((int64_t)slot * a.max_delay + (d - 1)) * E.cmax;
+            EQ_GLOBAL int32_t *bd = eq_g(E.bnd) + ((int64_t)slot * a.max_delay + (d - 1)) * E.cmax;
             if (threadIdx.x == 0) {
                 s_run = 0;
                 s_base = 0;
@@ -390,12 +425,12 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
                 int32_t f = 0, g = 0;
                 if (r < ns) {
                     if (tiles) {
-                        g = E.tile_count[r];
+                        g = eq_g(E.tile_count)[r];
                         for (int32_t j = 0; j < g; j++)
-                            f += eq_has(E.dmask + (int64_t)E.staging[(int64_t)r * E.tile_size + j] * a.words, d) ? 1 : 0;
+                            f += eq_has(E.dmask + (int64_t)eq_g(E.staging)[(int64_t)r * E.tile_size + j] * a.words, d) ? 1 : 0;
                     } else {
                         g = 1;
-                        f = eq_has(E.dmask + (int64_t)E.ring_idx[(int64_t)slot * E.slot_len + r] * a.words, d) ? 1 : 0;
+                        f = eq_has(E.dmask + (int64_t)eq_g(E.ring_idx)[(int64_t)slot * E.slot_len + r] * a.words, d) ? 1 : 0;
                     }
                 }
                 s_scan[threadIdx.x] = f;
@@ -414,7 +449,7 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
                 if (r < ns) {
                     if (tiles) {
                         for (int32_t j = 0; j < g; j++) {
-                            if (!eq_has(E.dmask + (int64_t)E.staging[(int64_t)r * E.tile_size + j] * a.words, d))
+                            if (!eq_has(E.dmask + (int64_t)eq_g(E.staging)[(int64_t)r * E.tile_size + j] * a.words, d))
                                 continue;
                             upto++;
                             if (upto % kEqChunk == 0 && upto / kEqChunk <= (int32_t)full_c) bd[upto / kEqChunk - 1] = pos0 + j;
@@ -442,7 +477,7 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
         for (int e = 0; e < a.n_em; e++) {
             const EqEmitter &E = s_em[e];
             const int64_t half = (int64_t)a.max_delay * E.cmax;
-            for (int64_t i = threadIdx.x; i < half; i += kBlock) E.rank[par * half + i] = 0xffffffffu;
+            for (int64_t i = threadIdx.x; i < half; i += kBlock) eq_g(E.rank)[par * half + i] = 0xffffffffu;
         }
     }
     // 4. the heap stays in LDS when it cannot outgrow it in this call
@@ -463,8 +498,8 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
     // 5. the heap operations, on one lane
     if (threadIdx.x == 0) {
         EqWin w{size0, (uint32_t)s_st[1], -1, false, false};
-        if (in_lds) eq_window<MAXD>(s_heap, a, s_em, s_cnt, s_cur, s_next, s_delay, k, w);
-        else eq_window<MAXD>(a.heap, a, s_em, s_cnt, s_cur, s_next, s_delay, k, w);
+        if (in_lds) eq_window<MAXD>(s_heap, a, s_em, s_cnt, s_pmask, s_anyfull, s_cur, s_next, s_delay, k, w);
+        else eq_window<MAXD>(a.heap, a, s_em, s_cnt, s_pmask, s_anyfull, s_cur, s_next, s_delay, k, w);
         s_st[0] = w.n;
         a.st[0] = w.n;
         a.st[1] = (int32_t)w.r;
