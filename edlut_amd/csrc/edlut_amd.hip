@@ -1201,7 +1201,9 @@ int check_err(edlut_population *p) {
     HIP_TRY(hipMemcpyAsync(&e, p->err, sizeof(uint32_t), hipMemcpyDeviceToHost, p->stream));
     HIP_TRY(hipStreamSynchronize(p->stream));
     if (e & 1u) return fail(EDLUT_E_STATE, "spike compaction look-back exceeded its spin limit");
-    if (e & 2u) return fail(EDLUT_E_CAP, "spike log capacity exceeded");
+    if (e & 2u)
+        return fail(EDLUT_E_CAP, "spike log capacity exceeded (read and clear the log between runs, or raise "
+                                 "its capacity)");
     if (e & 4u) return fail(EDLUT_E_CAP, "input event buffer capacity exceeded");
     if (e & 8u) return fail(EDLUT_E_STATE, "input fold segment wait exceeded its spin limit");
     if (e & 16u) return fail(EDLUT_E_CAP, "BDF fallback working storage exhausted in one step");
@@ -3055,6 +3057,8 @@ int edlut_network_clear_log(edlut_network *n) {
     if (!n) return fail(EDLUT_E_ARG, "network is null");
     HIP_TRY(hipSetDevice(n->pop->device));
     HIP_TRY(hipMemsetAsync(n->log_count, 0, 2 * sizeof(int64_t), n->pop->stream));
+    for (Source &s : n->sources)
+        if (s.log_count) HIP_TRY(hipMemsetAsync(s.log_count, 0, 2 * sizeof(int64_t), n->pop->stream));
     HIP_TRY(hipStreamSynchronize(n->pop->stream));
     return 0;
 }
@@ -3291,6 +3295,7 @@ int edlut_network_reset(edlut_network *n) {
         if (!s.self && !s.lcg0.empty())
             HIP_TRY(hipMemcpy(s.lcg, s.lcg0.data(), sizeof(uint32_t) * s.lcg0.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemsetAsync(s.ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
+        if (s.log_count) HIP_TRY(hipMemsetAsync(s.log_count, 0, 2 * sizeof(int64_t), p->stream));
     }
     HIP_TRY(hipMemsetAsync(n->own_ring_cnt, 0, sizeof(int32_t) * n->ring, p->stream));
     HIP_TRY(hipMemsetAsync(n->log_count, 0, 2 * sizeof(int64_t), p->stream));

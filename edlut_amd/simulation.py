@@ -188,8 +188,13 @@ class SimulationPlan:
 class Simulation_API:
     """Simulation_API (Simulation_API.cpp) on device-resident networks."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, full_spike_log: bool = False):
+        """full_spike_log keeps every neuron's spikes in the device logs for
+        the whole simulation (Network.spikes() of each population, for
+        monitoring); by default only populations with output neurons keep a
+        log, and GetSpikeActivity empties it on each read."""
         self.device = device
+        self.full_spike_log = bool(full_spike_log)
         self.neuron_layer_list: list[NF.NeuronLayerDescription] = []
         self.learning_rule_list: list[NF.ModelDescription] = []
         self.synaptic_layers: list[dict] = []
@@ -315,7 +320,7 @@ class Simulation_API:
     @classmethod
     def from_files(cls, netfile: str, wfile: str, input_spike_file: str | None = None,
                    base_dir: str | None = None, device: int = 0,
-                   input_current_file: str | None = None) -> "Simulation_API":
+                   input_current_file: str | None = None, full_spike_log: bool = False) -> "Simulation_API":
         """A simulation built from a network file and a weights file
         (Network::ParseNet; the weights file's random weights take one
         RandomGenerator), with the input spikes of a FileInputSpikeDriver file
@@ -323,7 +328,7 @@ class Simulation_API:
         (default: the working directory, as the reference's fopen)."""
         rng = RandomGenerator()
         layers, rules, syn = NF.parse_net(netfile, wfile, base_dir=base_dir, rng=rng)
-        sim = cls(device)
+        sim = cls(device, full_spike_log)
         for L in layers:
             sim.AddNeuronLayer(L.num_neurons, L.neuron_model.model_name, L.neuron_model.param_map,
                                L.log_activity, L.output_activity)
@@ -391,6 +396,12 @@ class Simulation_API:
                                  f"(found {sorted(steps)})")
         plan.dt = steps.pop() if steps else float(np.float32(1e-3))
         fdt = float(np.float32(plan.dt))
+        # PoissonGeneratorDeviceVector samples on its own 0.1 ms clock
+        # (PoissonGeneratorDeviceVector.cpp:26,137); the device draws one
+        # sample per population update, so the two clocks must agree
+        if any(t.model.model_name == NF.POISSON for t in plan.types) and fdt != float(np.float32(1e-4)):
+            raise EDLUTException(C.EDLUT_E_ARG, "Poisson generator layers sample every 0.1 ms: the GPU neuron "
+                                 f"models must use an integration step of 0.0001 s (found {fdt:g})")
 
         cat = {k: np.concatenate([L[k] for L in self.synaptic_layers]) if self.synaptic_layers else
                np.zeros(0, np.int32) for k in ("source", "target", "delay", "type", "weight", "max_weight",
@@ -542,7 +553,9 @@ class Simulation_API:
         for tidx, cur in plan.current.items():
             self._cur_value[tidx] = np.zeros(len(cur["syn"]), np.float32)
         # Poisson layers with output neurons: one of their device copies logs
-        # the generators' spikes
+        # the generators' spikes; a layer with no synapse onto a GPU
+        # population gets a copy without synapses, on the first population's
+        # network, that only logs
         self._poisson_log = {}
         for t in plan.types:
             if t.model.model_name != NF.POISSON or not t.output.any():
@@ -550,10 +563,37 @@ class Simulation_API:
             for tidx, srcs in plan.sources.items():
                 hit = [s2 for s2 in srcs if s2["kind"] == "poisson" and s2["src_type"] == t.index]
                 if hit:
-                    self.nets[tidx].set_source_log(hit[0]["slot"], POISSON_LOG_CAP)
-                    self._poisson_log[t.index] = (tidx, hit[0]["slot"])
-                    self._log_read[("poisson", t.index)] = 0
                     break
+            else:
+                if not gpu:
+                    raise EDLUTException(C.EDLUT_E_ARG, "Poisson generator layers need a GPU neuron population "
+                                         "to run beside")
+                tidx = gpu[0].index
+                srcs = plan.sources.setdefault(tidx, [])
+                if len(srcs) >= MAX_SOURCES:
+                    raise EDLUTException(C.EDLUT_E_CAP, f"population of {gpu[0].model.model_name} has no source "
+                                         "slot left for logging a Poisson output layer")
+                n_src = len(t.globals)
+                empty = (np.zeros(0, np.int32), np.zeros(0, np.int32), np.zeros(0, np.int32),
+                         np.zeros(0, np.uint8), np.zeros(0, np.float32))
+                rp, tt, ww = build_out_csr(n_src, len(gpu[0].globals), plan.max_delay, *empty)
+                net = self.nets[tidx]
+                slot = len(srcs)
+                net.add_poisson(plan.poisson_states[t.index], 0.0, rp, tt, ww)
+                net.set_poisson_rates(slot, np.full(n_src, np.float32(t.model.param_map.get("frequency", 10.0)),
+                                                    np.float32))
+                net.set_source_model(slot, t.index, t.index)
+                hit = [dict(kind="poisson", src_type=t.index, n_src=n_src, syn=np.zeros(0, np.int64), edges=empty,
+                            stdp=None, csr=(rp, tt, ww), slot=slot)]
+                srcs.append(hit[0])
+            self.nets[tidx].set_source_log(hit[0]["slot"], POISSON_LOG_CAP)
+            self._poisson_log[t.index] = (tidx, hit[0]["slot"])
+            self._log_read[("poisson", t.index)] = 0
+        # spike logs: only populations with output neurons keep one
+        if not self.full_spike_log:
+            for t in gpu:
+                if not t.output.any():
+                    self.nets[t.index].set_log_capacity(0)
         if gpu:
             self.circuit = Circuit([self.nets[t.index] for t in gpu])
         self._log_read.update({t.index: 0 for t in gpu})
@@ -756,11 +796,13 @@ class Simulation_API:
         t_now = grid[self.current_step] if self.current_step else 0.0
         rows = []
         for tidx, net in self.nets.items():
+            T = plan.types[tidx]
+            if not T.output.any():
+                continue
             st, ix = net.spikes()
             new = slice(self._log_read[tidx], len(st))
             self._log_read[tidx] = len(st)
             st, ix = st[new], ix[new]
-            T = plan.types[tidx]
             keep = T.output[ix]
             g = np.asarray(T.globals, np.int64)[ix[keep]]
             rows.append((grid[st[keep]], np.full(len(g), tidx), ix[keep], g))
@@ -775,6 +817,14 @@ class Simulation_API:
             keep = T.output[ix]
             rows.append((grid[st[keep]], np.full(int(keep.sum()), ptype), ix[keep],
                          np.asarray(T.globals, np.int64)[ix[keep]]))
+        if not self.full_spike_log:
+            # the logs are drained, as the output drivers' buffers
+            # (GetBufferedSpikes empties its buffer on each read)
+            for tidx, net in self.nets.items():
+                if plan.types[tidx].output.any() or any(v[0] == tidx for v in self._poisson_log.values()):
+                    net.clear_log()
+            for key in self._log_read:
+                self._log_read[key] = 0
         # input neurons with output activity: their spikes up to the current time
         if len(self._input_times):
             prev = self._in_cursor

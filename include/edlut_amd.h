@@ -312,6 +312,9 @@ int edlut_network_synchronize(edlut_network *net);
 /* Spike log (step, index) pairs, ascending within a step; returns the count. */
 int edlut_network_spike_log(edlut_network *net, int32_t *step, int32_t *idx, int64_t cap,
                             int64_t *n_out);
+/* Empties the population's spike log and the logs of its Poisson sources
+ * (the output drivers' GetBufferedSpikes empties its buffer on each read,
+ * Simulation_API.cpp:173-204); runs continue unaffected. */
 int edlut_network_clear_log(edlut_network *net);
 /* Spike log of Poisson device `source` (cap entries, 0 to drop it): the
  * generators that fired and when, for output of a PoissonGeneratorDeviceVector
@@ -375,8 +378,8 @@ int edlut_network_get_stdp(edlut_network *net, int source, float *pre, float *po
 /* Back to the initial state: neuron state as after construction (the
  * external-current row, an input, is kept), Poisson
  * generators at their initial states, plastic weights and traces as when
- * STDP was switched on, empty spike rings and log; the next run starts at
- * update 1. */
+ * STDP was switched on, empty spike rings and logs (its Poisson sources'
+ * logs included); the next run starts at update 1. */
 int edlut_network_reset(edlut_network *net);
 /* Launch shape: neurons per thread of the update kernel (4; 2 or 4 for a
  * grouped population), lanes per spike and workgroups of the ring fold

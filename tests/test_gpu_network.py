@@ -150,3 +150,26 @@ def test_two_items_per_thread_launch(oracle, kind):
     assert np.array_equal(st, ost) and np.array_equal(ix, oix)
     for v in range(pop.nv):
         assert np.array_equal(bits(model.GetVectorNeuronState(v)), bits(pop.state(v))), f"variable {v}"
+
+
+def test_reset_and_clear_log_empty_source_logs():
+    """reset and clear_log empty the Poisson sources' spike logs with the
+    population's: a run after a reset logs the same spikes as the first run,
+    with nothing left over from it."""
+    spec = workloads.lif_poisson(n=200, steps=0)
+    print("case source logs reset", flush=True)
+    model, net = E.from_spec(spec)
+    net.set_source_log(0, 1 << 16)
+    net.run(300)
+    first = net.source_spikes(0)
+    assert len(first[0]) > 100
+    net.reset()
+    assert len(net.source_spikes(0)[0]) == 0 and len(net.spikes()[0]) == 0
+    net.run(300)
+    again = net.source_spikes(0)
+    assert np.array_equal(first[0], again[0]) and np.array_equal(first[1], again[1])
+    net.clear_log()
+    assert len(net.source_spikes(0)[0]) == 0 and len(net.spikes()[0]) == 0
+    net.run(50)
+    st, _ = net.source_spikes(0)
+    assert len(st) > 0 and st.min() == 301

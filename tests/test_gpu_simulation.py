@@ -97,9 +97,9 @@ def _compare(oracle, sim, n_steps):
     return total
 
 
-def _driven_sim(seed=7):
+def _driven_sim(seed=7, full=True):
     S.RandomGenerator.srand(seed)
-    sim = S.Simulation_API()
+    sim = S.Simulation_API(full_spike_log=full)
     rk4 = {"int_meth": {"name": "RK4", "step": 1e-4, "bdf_order": 1}}
     inp = sim.AddNeuronLayer(4, "InputSpikeNeuronModel", output_activity=True)
     poi = sim.AddNeuronLayer(30, "PoissonGeneratorDeviceVector", {"frequency": 150.0}, output_activity=True)
@@ -209,7 +209,7 @@ def test_from_files_reference_inputs(oracle, tmp_path):
     w.write_text("2 0.9\n2 0.5\n0 0.7\n")
     S.RandomGenerator.srand(1)
     sim = S.Simulation_API.from_files(str(net), str(w), os.path.join(GOLD, "Input_spikes.cfg"),
-                                      base_dir=str(tmp_path))
+                                      base_dir=str(tmp_path), full_spike_log=True)
     sim.RunSimulation(1.25)
     n = _compare(oracle, sim, sim.current_step)
     assert n > 0
@@ -233,7 +233,8 @@ def test_reference_example_with_currents(oracle, tmp_path):
     w.write_text("3 1.5\n1 1.0\n")
     S.RandomGenerator.srand(1)
     sim = S.Simulation_API.from_files(str(net), str(w), os.path.join(GOLD, "Input_spikes.cfg"), base_dir=str(tmp_path),
-                                      input_current_file=os.path.join(GOLD, "Input_currents.cfg"))
+                                      input_current_file=os.path.join(GOLD, "Input_currents.cfg"),
+                                      full_spike_log=True)
     sim.RunSimulation(0.3)
     sim.RunSimulation(1.25)
     # 20 current changes reach neuron 5 at the first update after t + 1 ms
@@ -248,3 +249,84 @@ def test_reference_example_with_currents(oracle, tmp_path):
     assert (ix == 1).sum() > 20, "the 300 pA steps should drive neuron 5"
     assert n > 0
     sim.close()
+
+
+def _poisson_replay(states, freq, steps):
+    """(update, generator) of every spike of a Poisson layer, replayed on the
+    host from its seeds (RandomGenerator LCG; fire when frequency * 0.1 ms >
+    frand(), PoissonGeneratorDeviceVector.cpp:137)."""
+    x = np.asarray(states).astype(np.uint64)
+    p = float(np.float32(freq)) * float(np.float32(0.1) / np.float32(1000.0))
+    out = []
+    for k in range(1, steps + 1):
+        x = (x * np.uint64(1103515245) + np.uint64(12345)) & np.uint64(0x7FFFFFFF)
+        u = x.astype(np.float32) / np.float32(2147483647.0)
+        out += [(k, int(i)) for i in np.nonzero(p > u.astype(np.float64))[0]]
+    return out
+
+
+def test_spike_activity_drains_logs():
+    """GetSpikeActivity empties the device logs on each read, as the output
+    drivers' GetBufferedSpikes (Simulation_API.cpp:173-204): the pieces read
+    after each run join to one read of a fully logged simulation, logs far
+    smaller than the whole simulation's spikes do not overflow, a population
+    without output neurons keeps no log, and a log that is never read
+    overflows with an error."""
+    print("case drain logs", flush=True)
+    full, _, _ = _driven_sim(seed=5)
+    full.Initialize()
+    for q in range(1, 13):
+        full.RunSimulation(0.01 * q)
+    want = full.GetSpikeActivity()
+    # the largest number of spikes one population logs in one run
+    ends = [first + n - 1 for first, n, _ in full.run_log]
+    piece = 0
+    for ti in full.nets:
+        st = full.nets[ti].spikes()[0]
+        piece = max([piece] + np.diff(np.searchsorted(st, [0] + ends, side="right")).tolist())
+    full.close()
+    sim, _, _ = _driven_sim(seed=5, full=False)
+    sim.Initialize()
+    outs = [ti for ti in sim.nets if sim.plan.types[ti].output.any()]
+    quiet = [ti for ti in sim.nets if not sim.plan.types[ti].output.any()]
+    assert outs and quiet
+    cap = 2 * piece
+    for ti in outs:
+        sim.nets[ti].set_log_capacity(cap)
+    got_t, got_c = [], []
+    for q in range(1, 13):
+        sim.RunSimulation(0.01 * q)
+        t, c = sim.GetSpikeActivity()
+        got_t += t
+        got_c += c
+    assert len(want[0]) > 3 * cap
+    assert sorted(zip(got_t, got_c)) == sorted(zip(*want))
+    for ti in quiet:
+        assert len(sim.nets[ti].spikes()[0]) == 0 and sim.nets[ti].spike_count() > 0
+    with pytest.raises(EDLUTException, match="capacity"):
+        sim.RunSimulation(0.36)
+    sim.close()
+
+
+def test_poisson_output_layer_without_synapses():
+    """A Poisson layer with output neurons and no synapse still reports its
+    generators' spikes (a copy without synapses logs them)."""
+    print("case poisson output without synapses", flush=True)
+    S.RandomGenerator.srand(3)
+    sim = S.Simulation_API()
+    lone = sim.AddNeuronLayer(25, "PoissonGeneratorDeviceVector", {"frequency": 120.0}, output_activity=True)
+    drive = sim.AddNeuronLayer(10, "PoissonGeneratorDeviceVector", {"frequency": 80.0})
+    lif = sim.AddNeuronLayer(5, "LIFTimeDrivenModel_GPU", {"int_meth": {"name": "Euler", "step": 1e-4}})
+    sim.AddSynapticLayer(np.array(drive)[np.arange(20) % 10], np.array(lif)[np.arange(20) % 5],
+                         {"delay": 0.0003, "weight": 1.0})
+    sim.Initialize()
+    sim.RunSimulation(0.05)
+    t, c = sim.GetSpikeActivity()
+    ty = sim.plan.type_of[lone[0]]
+    grid = S.update_times(float(np.float32(1e-4)), sim.current_step)
+    want = [(float(grid[k]), lone[0] + i) for k, i in _poisson_replay(sim.plan.poisson_states[ty], 120.0,
+                                                                       sim.current_step)]
+    assert len(want) > 50
+    assert sorted(zip(t, c)) == sorted(want)
+    sim.close()
+

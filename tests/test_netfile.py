@@ -235,6 +235,18 @@ def test_plan_refusals():
     sim = S.Simulation_API()
     with pytest.raises(EDLUTException, match="not available"):
         sim.AddLearningRule("ExpOptimisedBufferedAdditiveKernel", {})
+    # Poisson generators sample every 0.1 ms (PoissonGeneratorDeviceVector.cpp:26,137):
+    # a 1 ms integration step is refused, 0.1 ms is not
+    for step, refused in ((1e-3, True), (1e-4, False)):
+        sim = S.Simulation_API()
+        p = sim.AddNeuronLayer(5, "PoissonGeneratorDeviceVector", {"frequency": 10.0})
+        a = sim.AddNeuronLayer(5, "LIFTimeDrivenModel_GPU", {"int_meth": {"name": "Euler", "step": step}})
+        sim.AddSynapticLayer(p[:2], a[:2], {"delay": 0.001, "weight": 1.0})
+        if refused:
+            with pytest.raises(EDLUTException, match="0.1 ms"):
+                sim.build_plan()
+        else:
+            sim.build_plan()
 
 
 def test_input_spike_placement_brute_force():
