@@ -27,6 +27,7 @@ EXPORTS = (
     "edlut_population_set_ext_current",
     "edlut_population_update",
     "edlut_population_get_spikes",
+    "edlut_population_update_noalloc",
     "edlut_population_get_state",
     "edlut_population_set_state",
     "edlut_population_set_newton_fp64",
@@ -122,6 +123,7 @@ def _sig(lib):
         "edlut_population_set_ext_current": (I32, [P, P]),
         "edlut_population_update": (I32, [P, I64, P]),
         "edlut_population_get_spikes": (I32, [P, P, I32]),
+        "edlut_population_update_noalloc": (I32, [P, I64, P, P, P, I32, P, P, P]),
         "edlut_population_get_state": (I32, [P, I32, P]),
         "edlut_population_set_state": (I32, [P, I32, P]),
         "edlut_population_set_newton_fp64": (I32, [P, I32]),

@@ -12,6 +12,7 @@
 #include <cstring>
 #include <limits>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "../../include/edlut_amd.h"
@@ -1209,6 +1210,9 @@ int check_err(edlut_population *p) {
     if (e & 16u) return fail(EDLUT_E_CAP, "BDF fallback working storage exhausted in one step");
     if (e & 64u) return fail(EDLUT_E_CAP, "event queue capacity exceeded");
     if (e & 128u) return fail(EDLUT_E_STATE, "event queue holds an event of an unknown kind");
+    if (e & 256u)
+        return fail(EDLUT_E_ARG, "an input event in device memory names a target out of range or a synapse "
+                                 "type that is not enabled (dropped)");
     return 0;
 }
 
@@ -1852,7 +1856,7 @@ int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
         HIP_TRY(hipMemcpyAsync(p->e_type, p->q_type.data(), ne, hipMemcpyHostToDevice, p->stream));
         HIP_TRY(hipMemcpyAsync(p->e_w, p->q_w.data(), sizeof(float) * ne, hipMemcpyHostToDevice, p->stream));
         if ((rc0 = p->reserve_events((uint64_t)ne))) return rc0;
-        PushEvents pe{p->e_tgt, p->e_type, p->e_w, ne};
+        PushEvents pe{p->e_tgt, p->e_type, p->e_w, ne, (int32_t)p->n, p->mask & 7u, p->err};
         int rc = launch_push_fold(p, par, ne, &pe);
         if (rc) return rc;
     }
@@ -1883,6 +1887,86 @@ int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
     if (rc) return rc;
     p->last_n_spikes = ns;
     if (n_spikes) *n_spikes = ns;
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// Allocation-free update: the input fold's event buffer (order keys and
+// weights of the step's events, grouped by target) lives in this file-scope
+// device array instead of the population's growable buffer.  Each device
+// holds its own copy at its own address, looked up once per device.
+constexpr int64_t kNoAllocMaxEvents = 1 << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocMaxEvents * (sizeof(uint64_t) + sizeof(float))];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices] = {};
+
+// The device address of `scratch` on `device` (the current device).
+int scratch_ptr(int device, void **out) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (device < 0 || device >= kScratchDevices) return fail(EDLUT_E_ARG, "device %d out of range", device);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            return fail(EDLUT_E_HIP, "hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+        g_scratch_ptr[device] = ptr;
+    }
+    *out = g_scratch_ptr[device];
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int edlut_population_update_noalloc(edlut_population *p, int64_t step, const int32_t *target, const uint8_t *type,
+                                    const float *weight, int n_events, int32_t *spk_idx, int32_t *spk_cnt,
+                                    void *stream) {
+    if (!p) return fail(EDLUT_E_ARG, "population is null");
+    if (step < 1 || step > (1ll << 30)) return fail(EDLUT_E_ARG, "update index out of range");
+    if (n_events < 0 || (n_events > 0 && (!target || !type || !weight))) return fail(EDLUT_E_ARG, "bad event arrays");
+    if (n_events > kNoAllocMaxEvents)
+        return fail(EDLUT_E_CAP, "%d input events exceed the allocation-free update's maximum of %lld", n_events,
+                    (long long)kNoAllocMaxEvents);
+    if (!spk_idx || !spk_cnt) return fail(EDLUT_E_ARG, "null spike output");
+    if (!p->q_tgt.empty())
+        return fail(EDLUT_E_STATE, "input events queued by edlut_population_push_inputs: use edlut_population_update");
+    UpdateFn fn = pick_update(p, false);
+    if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
+    HIP_TRY(hipSetDevice(p->device));
+    void *base = nullptr;
+    int rc = scratch_ptr(p->device, &base);
+    if (rc) return rc;
+    const hipStream_t st = stream ? (hipStream_t)stream : p->stream;
+    // 1. ordered fold of the events (position in the arrays = delivery order)
+    const int par = (int)(++p->upd_seq & 1u);
+    FoldTarget t = p->fold_target(par);
+    t.b_ord = (uint64_t *)base;
+    t.b_w = (float *)((uint8_t *)base + kNoAllocMaxEvents * sizeof(uint64_t));
+    t.b_cap = (uint64_t)kNoAllocMaxEvents;
+    if (n_events > 0) {
+        const PushEvents pe{target, type, weight, n_events, (int32_t)p->n, p->mask & 7u, p->err};
+        int nb = (n_events + kBlock - 1) / kBlock;
+        if (nb > 4096) nb = 4096;
+        hipLaunchKernelGGL((k_fold_push<kCount>), dim3(nb), dim3(kBlock), 0, st, t, pe);
+        hipLaunchKernelGGL((k_fold_push<kScatter>), dim3(nb), dim3(kBlock), 0, st, t, pe);
+    }
+    // 2. integrate, detect, compact into the caller's spike list
+    UpdateArgs a = make_update_args(p, (int32_t)step);
+    a.out = TileOut{p->tile_count, p->staging};
+    a.fold = t;
+    p->started = true;
+    const int tiles = p->update_tiles();
+    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), st, a);
+    launch_compact(st, p->tile_count, p->staging, tiles, kBlock * p->items, spk_idx, spk_cnt, 1, 0, nullptr,
+                   nullptr, nullptr, 0, p->err, StepRef{nullptr, (int32_t)step});
+    HIP_TRY(hipGetLastError());
     return 0;
 }
 

@@ -664,18 +664,29 @@ __global__ __launch_bounds__(kBlock) void k_train(TrainArgs a) {
     if (blockIdx.x == 0 && threadIdx.x == 0) a.ring_cnt[slot] = (int32_t)(e - b);
 }
 
-// Events pushed by the host: order = position in the host's list.
+// Events pushed by the host: order = position in the host's list.  Events
+// handed over in device memory are checked here (target in range, synapse
+// type enabled); one that fails is dropped and sets err bit 8.
 struct PushEvents {
     const int32_t *tgt;
     const uint8_t *type;
     const float *w;
     int n;
+    int32_t n_tgt;
+    uint32_t mask;       // enabled synapse types taking spikes (bits 0-2)
+    uint32_t *err;
 };
 
 template <int PASS>
 __global__ __launch_bounds__(kBlock) void k_fold_push(FoldTarget t, PushEvents e) {
     for (int q = blockIdx.x * kBlock + threadIdx.x; q < e.n; q += gridDim.x * kBlock) {
-        const uint64_t key = (uint64_t)e.type[q] * (uint64_t)t.stride + (uint64_t)e.tgt[q];
+        const int32_t tg = e.tgt[q];
+        const uint32_t ty = e.type[q];
+        if ((uint32_t)tg >= (uint32_t)e.n_tgt || ty > 2u || !((e.mask >> ty) & 1u)) {
+            if (PASS == kCount) atomicOr(e.err, 256u);
+            continue;
+        }
+        const uint64_t key = (uint64_t)ty * (uint64_t)t.stride + (uint64_t)tg;
         fold_event<PASS>(t, t.par, key, (uint64_t)q, e.w[q]);
     }
 }

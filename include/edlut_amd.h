@@ -141,6 +141,29 @@ int edlut_population_set_ext_current(edlut_population *pop, const float *current
 int edlut_population_update(edlut_population *pop, int64_t step, int *n_spikes);
 int edlut_population_get_spikes(const edlut_population *pop, int32_t *idx, int cap);
 
+/*
+ * Allocation-free update.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  The same step as
+ * edlut_population_push_inputs + edlut_population_update, with the input
+ * events already in device memory (target, type, weight arrays of n_events
+ * entries, delivered in array order) and the spikes written to device memory
+ * (spk_idx: room for every neuron, ascending; spk_cnt: one int32).
+ * Asynchronous on `stream` (null: the population's stream); calls no
+ * hipMalloc, hipFree or host synchronisation.  The input fold's event buffer
+ * is a library-wide device array of EDLUT_NOALLOC_MAX_EVENTS events (12 MiB,
+ * held by each GPU the library runs on until the process ends): n_events
+ * above it is refused (EDLUT_E_CAP).  Every call on one GPU shares that
+ * array, so calls that overlap on one GPU race; serialise them.  An event
+ * whose target is out of range or whose synapse type is not enabled is
+ * dropped and reported (EDLUT_E_ARG) by the population's next synchronous
+ * call.  Refused while events queued by edlut_population_push_inputs wait.
+ */
+#define EDLUT_NOALLOC_MAX_EVENTS (1 << 20)
+int edlut_population_update_noalloc(edlut_population *pop, int64_t step, const int32_t *target,
+                                    const uint8_t *type, const float *weight, int n_events,
+                                    int32_t *spk_idx, int32_t *spk_cnt, void *stream);
+
 /* Copy one state variable to / from host memory (n_neurons floats); replaces
  * the monitored-state copies (AdExTimeDrivenModel_GPU_C_INTERFACE.cu:99-103). */
 int edlut_population_get_state(const edlut_population *pop, int var, float *out);
