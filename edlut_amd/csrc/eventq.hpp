@@ -19,8 +19,9 @@
 // populations between update k and update k + 1 and gives each group (or
 // propagated input spike) its rank in that window.  The fold of update k + 1
 // sorts each target's events by (rank, synapse), which is the reference's
-// delivery order.  The heap holds only keys and small payloads, so one lane
-// runs it; the other lanes count the step's spikes per delay first.
+// delivery order.  The heap holds only keys and small payloads; one wave
+// runs its operations lane-parallel, after the workgroup has counted the
+// step's spikes per delay.
 #pragma once
 #include "device_common.hpp"
 
@@ -176,8 +177,20 @@ __device__ inline bool eq_has(const uint32_t *m, int d) {
 
 constexpr int kMaxDelayEq = 32 * kEqMaskWords;
 
-// The heap operations of one window, on one lane (always inlined, so that a
-// heap staged in LDS is addressed as LDS).
+// The heap operations of one window, run by one wave with wave-uniform
+// control flow (always inlined, so that a heap staged in LDS is addressed as
+// LDS).  Each heap operation is the reference's, lane-parallel:
+//  - removal: every lane takes one node p of the top 64 and picks its child
+//    as the reference's sift-down would (the earlier child, the left one on
+//    ties), and whether that child moves up past the displaced last element;
+//    the path is then a chain of readlanes from the root, and the nodes on
+//    it take their chosen child's entry in one write.  Nodes below the top
+//    64 (heaps of more than 129 entries) continue one level at a time.
+//  - insertion: lane j reads the j-th ancestor of the new position; the
+//    ancestors later than the new entry up to the first that is not (the
+//    reference's sift-up stops there) move down one level in one write.
+// Children of node p are read before any entry on the path is written, as in
+// the reference, where the entries written lie above them.
 struct EqWin {
     int32_t n;          // heap size
     uint32_t r;         // rank counter of the window
@@ -185,18 +198,109 @@ struct EqWin {
     bool overflow, unknown;
 };
 
+__device__ __forceinline__ bool eq_before(const EqItem &a, const EqItem &x) {
+    return a.t < x.t || (a.t == x.t && eq_prio(a) > eq_prio(x));
+}
+
+// The same comparisons without short-circuit branches (lane-parallel code).
+__device__ __forceinline__ bool eq_later_v(const EqItem &p, const EqItem &c) {
+    return (p.t > c.t) | ((p.t == c.t) & (eq_prio(p) < eq_prio(c)));
+}
+__device__ __forceinline__ bool eq_before_v(const EqItem &a, const EqItem &x) {
+    return (a.t < x.t) | ((a.t == x.t) & (eq_prio(a) > eq_prio(x)));
+}
+
+__device__ __forceinline__ int eq_lane() { return (int)(threadIdx.x & 63u); }
+
+__device__ __forceinline__ void eq_insert_wave(EqItem *__restrict__ h, int32_t &n, const EqItem &x) {
+    const int lane = eq_lane();
+    const int32_t c = ++n;
+    const int depth = 31 - __builtin_clz((uint32_t)c);     // ancestors of c
+    // lanes past the root read entry 0 (unused) and take no part
+    const EqItem anc = h[lane < depth ? c >> (lane + 1) : 0];
+    const bool later = (lane < depth) & eq_later_v(anc, x);
+    const uint64_t lm = __ballot(later);
+    const int m = (int)__builtin_ctzll(~lm);                // ancestors that move down
+    if (lane < m) h[c >> lane] = anc;
+    if (lane == 0) h[c >> m] = x;
+}
+
+// The entries a removal reads (the last entry and the children of the top
+// 64 nodes), loaded before the caller decides to remove.
+struct EqRemoveLoads {
+    EqItem x, ch;
+    int32_t c;
+    bool has_l;
+};
+
+__device__ __forceinline__ EqRemoveLoads eq_remove_loads(const EqItem *__restrict__ h, int32_t n) {
+    const int32_t p0 = eq_lane() + 1;
+    const int32_t size = n - 1 > 0 ? n - 1 : 1;
+    // both children of node p0, read unconditionally (indices clamped into
+    // the heap), then chosen with selects
+    const int32_t cl = 2 * p0, cr = 2 * p0 + 1;
+    const EqItem L = h[cl <= size ? cl : size];
+    const EqItem R = h[cr <= size ? cr : size];
+    const bool has_r = cr <= n - 1;
+    const bool pick_r = has_r & eq_later_v(L, R);
+    EqRemoveLoads r;
+    r.x = h[n];
+    r.ch.t = pick_r ? R.t : L.t;
+    r.ch.a = pick_r ? R.a : L.a;
+    r.ch.b = pick_r ? R.b : L.b;
+    r.c = pick_r ? cr : cl;
+    r.has_l = cl <= n - 1;
+    return r;
+}
+
+// Removes the top (n >= 1) with the entries `eq_remove_loads` read.
+__device__ __forceinline__ void eq_remove_wave(EqItem *__restrict__ h, int32_t &n, const EqRemoveLoads &r) {
+    const int lane = eq_lane();
+    if (n == 1) {
+        n = 0;
+        return;
+    }
+    const int32_t size = --n;
+    const bool ok = r.has_l & eq_before_v(r.ch, r.x);
+    const uint64_t okm = __ballot(ok);
+    uint64_t path = 0;
+    int32_t p = 1;
+    while (p <= 64 && ((okm >> (p - 1)) & 1ull)) {
+        path |= 1ull << (p - 1);
+        p = __builtin_amdgcn_readlane(r.c, p - 1);
+    }
+    if ((path >> lane) & 1ull) h[lane + 1] = r.ch;
+    // below the top 64 nodes: one level at a time
+    while (p > 64 && 2 * p <= size) {
+        int32_t cc = 2 * p;
+        EqItem a = h[cc];
+        if (cc + 1 <= size) {
+            const EqItem b = h[cc + 1];
+            if (eq_later(a, b)) {
+                a = b;
+                cc++;
+            }
+        }
+        if (!eq_before(a, r.x)) break;
+        if (lane == 0) h[p] = a;
+        p = cc;
+    }
+    if (lane == 0) h[p] = r.x;
+}
+
 template <int MAXD>
 __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &a, const EqEmitter *em,
                                           const uint32_t (*cnt)[MAXD], const uint32_t (*pmask)[MAXD / 32 + 1],
                                           const uint32_t *anyfull, uint32_t *cur, int32_t *next_k,
                                           const double *delay_s, int32_t k, EqWin &w) {
+    const int lane = eq_lane();
     const int par = (k + 1) & 1;          // parity of the update that consumes this window
     auto push = [&](const EqItem &x) {
         if (w.n >= a.cap) {
             w.overflow = true;
             return;
         }
-        eq_insert(h, &w.n, x);
+        eq_insert_wave(h, w.n, x);
     };
     if (a.resume) {
         // input spikes added since the last run (Simulation_API::
@@ -223,13 +327,16 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
         }
         if (a.with_stop) push(eq_item(a.stop_t, kEqPrioStop, kEqStop, 0, 0, 0));
     }
-    EqItem it;
+    uint32_t pops = 0;                    // models that are populations
+    for (int e = 0; e < a.n_em; e++) pops |= (em[e].kind == kEqPop ? 1u : 0u) << e;
     while (w.n > 0 && !w.overflow) {
-        const uint32_t top_a = h[1].a;
+        const EqItem it = h[1];
+        const EqRemoveLoads rl = eq_remove_loads(h, w.n);
+        const uint32_t top_a = __builtin_amdgcn_readfirstlane(it.a);
         const uint32_t kind = (top_a >> 24) & 0xfu, e = (top_a >> 16) & 0xffu;
         // the window ends where the populations' next update starts
-        if (kind == kEqTime && em[e].kind == kEqPop && next_k[e] > k) break;
-        eq_remove(h, &w.n, &it);
+        if (kind == kEqTime && ((pops >> e) & 1u) && next_k[e] > k) break;
+        eq_remove_wave(h, w.n, rl);
         if (kind == kEqStop) {
             w.stop_r = (int32_t)w.r;
             break;
@@ -240,12 +347,14 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             // (ascending list position, then ascending delay), then the rest
             // in ascending delay
             const int32_t ke = next_k[e];
-            if (E.kind == kEqPop) w.r = 0;   // a window starts with the populations' updates
+            if ((pops >> e) & 1u) w.r = 0;   // a window starts with the populations' updates
             const int slot = ke % a.ring;
             const uint32_t *c_e = cnt[e];
             const bool any_full = anyfull[e] != 0u;
-            if (any_full)
-                for (int d = 1; d <= a.max_delay; d++) cur[d - 1] = 0u;
+            if (any_full) {
+                if (lane == 0)
+                    for (int d = 1; d <= a.max_delay; d++) cur[d - 1] = 0u;
+            }
             const bool use_bnd = any_full && eq_g(E.bnd_on)[slot] != 0;
             while (any_full) {
                 int32_t best = 0x7fffffff;
@@ -261,8 +370,9 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
                     }
                 }
                 if (!bd) break;
-                push(eq_item(it.t + delay_s[bd], kEqPrioSpike, kEqGroup, e, (uint32_t)bd, cur[bd - 1]));
-                cur[bd - 1]++;
+                const uint32_t cb = cur[bd - 1];
+                push(eq_item(it.t + delay_s[bd], kEqPrioSpike, kEqGroup, e, (uint32_t)bd, cb));
+                if (lane == 0) cur[bd - 1] = cb + 1u;
             }
             // the partial groups: delays whose count is not a multiple of 1,024
             for (int wd = 0; wd <= (a.max_delay - 1) / 32; wd++) {
@@ -274,10 +384,11 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
                                  any_full ? cur[d - 1] : 0u));
                 }
             }
-            next_k[e] = ke + 1;
+            if (lane == 0) next_k[e] = ke + 1;
             push(eq_item(it.t + E.h, kEqPrioTime, kEqTime, e, 0, 0));
         } else if (kind == kEqGroup) {
-            eq_g(E.rank)[((int64_t)par * a.max_delay + (eq_d(it) - 1)) * E.cmax + it.b] = w.r++;
+            if (lane == 0) eq_g(E.rank)[((int64_t)par * a.max_delay + (eq_d(it) - 1)) * E.cmax + it.b] = w.r;
+            w.r++;
         } else if (kind == kEqInput) {
             // InputSpike::ProcessEvent: the neuron's first delay
             const int32_t nr = eq_g(E.in_neuron)[it.b];
@@ -290,7 +401,8 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             const uint32_t j = eq_d(it);
             const int32_t np0 = eq_g(E.nd_ptr)[nr];
             const int d = eq_g(E.nd_list)[np0 + (int32_t)j];
-            eq_g(E.rank)[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + eq_g(E.in_pos)[it.b]] = w.r++;
+            if (lane == 0) eq_g(E.rank)[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + eq_g(E.in_pos)[it.b]] = w.r;
+            w.r++;
             if (np0 + (int32_t)j + 1 < eq_g(E.nd_ptr)[nr + 1]) {
                 const int d2 = eq_g(E.nd_list)[np0 + (int32_t)j + 1];
                 push(eq_item(it.t - delay_s[d] + delay_s[d2], kEqPrioSpike, kEqProp, e, j + 1, it.b));
@@ -303,7 +415,7 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
 }
 
 // The replay of one window by one workgroup of kBlock threads: the spikes
-// per delay are counted by all lanes, the heap operations run on one.
+// per delay are counted by all lanes, the heap operations run on the first wave.
 // LDS_ITEMS heap entries are staged in LDS when the heap cannot outgrow
 // them; MAXD bounds the delays (host-checked).
 template <int LDS_ITEMS, int MAXD>
@@ -379,20 +491,23 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
         }
     }
     __syncthreads();
-    // per model: delays with a partial group, and whether any group is full
-    for (int i = threadIdx.x; i < kEqMaxEmitters * (MAXD / 32 + 1); i += kBlock) {
-        const int e = i / (MAXD / 32 + 1), wd = i % (MAXD / 32 + 1);
-        uint32_t pm = 0u;
-        for (int b = 0; b < 32; b++) {
-            const int d = wd * 32 + b + 1;
-            if (d <= a.max_delay && (s_cnt[e][d - 1] % (uint32_t)kEqChunk) != 0u) pm |= 1u << b;
+    // per model (one wave each): delays with a partial group, and whether
+    // any group is full, from ballots over 64 delays at a time
+    for (int e = (int)(threadIdx.x >> 6); e < kEqMaxEmitters; e += kBlock / 64) {
+        const int lane = eq_lane();
+        bool full = false;
+#pragma unroll
+        for (int j = 0; j < (MAXD + 63) / 64; j++) {
+            const int d = j * 64 + lane + 1;
+            const uint32_t c = d <= MAXD && d <= a.max_delay ? s_cnt[e][(d - 1) % MAXD] : 0u;
+            const uint64_t pm = __ballot((c % (uint32_t)kEqChunk) != 0u);
+            full = full || __ballot(c >= (uint32_t)kEqChunk) != 0ull;
+            if (lane == 0) {
+                if (2 * j < MAXD / 32 + 1) s_pmask[e][2 * j] = (uint32_t)pm;
+                if (2 * j + 1 < MAXD / 32 + 1) s_pmask[e][2 * j + 1] = (uint32_t)(pm >> 32);
+            }
         }
-        s_pmask[e][wd] = pm;
-    }
-    if (threadIdx.x < kEqMaxEmitters) {
-        uint32_t f = 0u;
-        for (int d = 1; d <= a.max_delay; d++) f |= s_cnt[threadIdx.x][d - 1] >= (uint32_t)kEqChunk ? 1u : 0u;
-        s_anyfull[threadIdx.x] = f;
+        if (lane == 0) s_anyfull[e] = full ? 1u : 0u;
     }
     // 2. last member of every full chunk where a delay has 1,024 spikes or
     // more and not every spiking neuron has every delay (else chunk c ends at
@@ -495,19 +610,21 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
     }
     __syncthreads();
     const bool in_lds = s_in_lds != 0;
-    // 5. the heap operations, on one lane
-    if (threadIdx.x == 0) {
+    // 5. the heap operations, by the first wave
+    if (threadIdx.x < 64) {
         EqWin w{size0, (uint32_t)s_st[1], -1, false, false};
         if (in_lds) eq_window<MAXD>(s_heap, a, s_em, s_cnt, s_pmask, s_anyfull, s_cur, s_next, s_delay, k, w);
         else eq_window<MAXD>(a.heap, a, s_em, s_cnt, s_pmask, s_anyfull, s_cur, s_next, s_delay, k, w);
-        s_st[0] = w.n;
-        a.st[0] = w.n;
-        a.st[1] = (int32_t)w.r;
-        if (a.resume || w.stop_r >= 0) a.st[2] = w.stop_r;
-        if (a.resume) a.st[3] = a.clear_step;
-        for (int e = 0; e < a.n_em; e++) a.st[kEqSt + e] = s_next[e];
-        if (w.overflow) atomicOr(a.err, 64u);
-        if (w.unknown) atomicOr(a.err, 128u);
+        if (threadIdx.x == 0) {
+            s_st[0] = w.n;
+            a.st[0] = w.n;
+            a.st[1] = (int32_t)w.r;
+            if (a.resume || w.stop_r >= 0) a.st[2] = w.stop_r;
+            if (a.resume) a.st[3] = a.clear_step;
+            for (int e = 0; e < a.n_em; e++) a.st[kEqSt + e] = s_next[e];
+            if (w.overflow) atomicOr(a.err, 64u);
+            if (w.unknown) atomicOr(a.err, 128u);
+        }
     }
     __syncthreads();
     if (in_lds)
