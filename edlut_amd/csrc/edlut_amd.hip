@@ -729,6 +729,10 @@ int eq_build(EqSim *S, const std::vector<edlut_network *> &nets, uint32_t *err, 
         for (int w = 0; w < kEqMaskWords; w++) E.dev.full[w] = 0u;
         for (int64_t i = 0; i < nn; i++)
             for (int w = 0; w < W; w++) E.dev.full[w] |= h[(size_t)i * W + w];
+        E.dev.all_full = 1;
+        for (int64_t i = 0; i < nn && E.dev.all_full; i++)
+            for (int w = 0; w < W; w++)
+                if (h[(size_t)i * W + w] != E.dev.full[w]) E.dev.all_full = 0;
         E.dev.dmask = E.dmask;
         if (E.kind == kEqTrain) {
             const Source &s0 = E.copies[0].first->sources[(size_t)E.copies[0].second];
@@ -884,6 +888,13 @@ EqArgs eq_args(const EqSim *S, StepRef step, int resume, double stop_t) {
     a.clear_step = -1;
     a.step = step;
     a.err = S->err;
+    a.pf_em = -1;
+    for (size_t e = 0; e < S->em.size(); e++)
+        if (S->em[e].kind == kEqPop) {
+            a.pf_em = (int)e;
+            a.pf_tiles = S->em[e].net->tile_count;
+            break;
+        }
     return a;
 }
 

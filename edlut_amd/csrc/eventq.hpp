@@ -124,6 +124,7 @@ struct EqEmitter {
     int32_t n_tiles, tile_size;
     const uint32_t *dmask;     // [n][words]: bit d-1 set when the neuron has synapses of delay d
     uint32_t full[kEqMaskWords];   // the model's delay structure (union of the neurons')
+    int all_full;              // every neuron has every delay of `full` (counts per delay = spikes)
     int cmax;                  // rank-table entries per delay (chunks, or input spikes per update)
     uint32_t *rank;            // [2][max_delay][cmax]: rank in its window, by consuming-update parity
     int32_t *bnd;              // [ring][max_delay][cmax]: list position of the last member of chunk c
@@ -159,6 +160,8 @@ struct EqArgs {
     int with_stop;             // resuming call: 0 when the run continues the previous one (no stop event)
     int32_t clear_step;        // resuming call: the run's last update when its stop time is explicit, else -1
     StepRef step;              // update k just run (resuming call: the update before the run)
+    const int32_t *pf_tiles;   // tile counts of model pf_em, loaded at the start (null: none)
+    int pf_em;
     uint32_t *err;             // bit 6: heap capacity, bit 7: event of an unknown kind
 };
 
@@ -212,7 +215,20 @@ __device__ __forceinline__ bool eq_before_v(const EqItem &a, const EqItem &x) {
 
 __device__ __forceinline__ int eq_lane() { return (int)(threadIdx.x & 63u); }
 
-__device__ __forceinline__ void eq_insert_wave(EqItem *__restrict__ h, int32_t &n, const EqItem &x) {
+// Lane 0's entry, in scalar registers.
+__device__ __forceinline__ EqItem eq_lane0(const EqItem &v) {
+    const uint64_t tb = (uint64_t)__double_as_longlong(v.t);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)tb);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(tb >> 32));
+    EqItem r;
+    r.t = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+    r.a = (uint32_t)__builtin_amdgcn_readfirstlane((int)v.a);
+    r.b = (uint32_t)__builtin_amdgcn_readfirstlane((int)v.b);
+    return r;
+}
+
+// Inserts x; `top` follows the heap's first entry.
+__device__ __forceinline__ void eq_insert_wave(EqItem *__restrict__ h, int32_t &n, const EqItem &x, EqItem &top) {
     const int lane = eq_lane();
     const int32_t c = ++n;
     const int depth = 31 - __builtin_clz((uint32_t)c);     // ancestors of c
@@ -223,6 +239,7 @@ __device__ __forceinline__ void eq_insert_wave(EqItem *__restrict__ h, int32_t &
     const int m = (int)__builtin_ctzll(~lm);                // ancestors that move down
     if (lane < m) h[c >> lane] = anc;
     if (lane == 0) h[c >> m] = x;
+    if ((c >> m) == 1) top = x;
 }
 
 // The entries a removal reads (the last entry and the children of the top
@@ -253,8 +270,13 @@ __device__ __forceinline__ EqRemoveLoads eq_remove_loads(const EqItem *__restric
     return r;
 }
 
-// Removes the top (n >= 1) with the entries `eq_remove_loads` read.
-__device__ __forceinline__ void eq_remove_wave(EqItem *__restrict__ h, int32_t &n, const EqRemoveLoads &r) {
+// Removes the top (n >= 1) with the entries `eq_remove_loads` read; `top`
+// becomes the new first entry.  Two ballots give, for each of the top 64
+// nodes, whether its chosen child moves up and whether that child is the
+// right one; every lane then tests its node's ancestors against those bits,
+// so the path comes out without a chain of dependent steps.
+__device__ __forceinline__ void eq_remove_wave(EqItem *__restrict__ h, int32_t &n, const EqRemoveLoads &r,
+                                               EqItem &top) {
     const int lane = eq_lane();
     if (n == 1) {
         n = 0;
@@ -262,14 +284,32 @@ __device__ __forceinline__ void eq_remove_wave(EqItem *__restrict__ h, int32_t &
     }
     const int32_t size = --n;
     const bool ok = r.has_l & eq_before_v(r.ch, r.x);
-    const uint64_t okm = __ballot(ok);
-    uint64_t path = 0;
-    int32_t p = 1;
-    while (p <= 64 && ((okm >> (p - 1)) & 1ull)) {
-        path |= 1ull << (p - 1);
-        p = __builtin_amdgcn_readlane(r.c, p - 1);
+    const uint64_t moves = __ballot(ok);
+    const uint64_t right = __ballot((r.c & 1) != 0);
+    // node p0 (lane + 1) is on the path when every ancestor moves its child
+    // up and chose the child toward p0
+    const uint32_t p0 = (uint32_t)lane + 1u;
+    bool on = true;
+#pragma unroll
+    for (int j = 1; j <= 6; j++) {
+        const uint32_t q = p0 >> j;          // ancestor j levels up (0: none)
+        const uint32_t sh = (q - 1u) & 63u;
+        const bool step = (q == 0u) | ((((moves >> sh) & 1ull) != 0ull) &
+                                       ((uint32_t)((right >> sh) & 1ull) == ((p0 >> (j - 1)) & 1u)));
+        on = on & step;
+    }
+    const uint64_t path = __ballot(on & ok);
+    const uint64_t ends = __ballot(on & !ok);
+    int32_t p;
+    if (ends) {
+        p = (int32_t)__builtin_ctzll(ends) + 1;
+    } else {
+        // the path leaves the top 64 nodes below its deepest node there
+        const uint64_t out = __ballot(on & ok & (r.c > 64));
+        p = __builtin_amdgcn_readlane(r.c, (int)__builtin_ctzll(out));
     }
     if ((path >> lane) & 1ull) h[lane + 1] = r.ch;
+    top = (moves & 1ull) ? eq_lane0(r.ch) : r.x;
     // below the top 64 nodes: one level at a time
     while (p > 64 && 2 * p <= size) {
         int32_t cc = 2 * p;
@@ -288,19 +328,41 @@ __device__ __forceinline__ void eq_remove_wave(EqItem *__restrict__ h, int32_t &
     if (lane == 0) h[p] = r.x;
 }
 
+// Ranks given in a window, listed in LDS and written to the models' rank
+// tables by the whole wave (entries: model << 16 | delay, chunk or input
+// record, rank, 1 for a propagated input spike).
+constexpr int kEqRankList = 256;
+
 template <int MAXD>
 __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &a, const EqEmitter *em,
                                           const uint32_t (*cnt)[MAXD], const uint32_t (*pmask)[MAXD / 32 + 1],
                                           const uint32_t *anyfull, uint32_t *cur, int32_t *next_k,
-                                          const double *delay_s, int32_t k, EqWin &w) {
+                                          const double *delay_s, uint4 *rk, int32_t k, EqWin &w) {
     const int lane = eq_lane();
     const int par = (k + 1) & 1;          // parity of the update that consumes this window
+    EqItem top = h[1];                    // the heap's first entry, kept in registers
+    int32_t nrk = 0;
+    auto flush = [&]() {
+        for (int i = lane; i < nrk; i += 64) {
+            const uint4 q = rk[i];
+            const EqEmitter &F = em[q.x >> 16];
+            const int d = (int)(q.x & 0xffffu);
+            const int32_t c = q.w ? eq_g(F.in_pos)[q.y] : (int32_t)q.y;
+            eq_g(F.rank)[((int64_t)par * a.max_delay + (d - 1)) * F.cmax + c] = q.z;
+        }
+        nrk = 0;
+    };
+    auto give_rank = [&](uint32_t em_d, uint32_t b, uint32_t prop) {
+        if (lane == 0) rk[nrk] = make_uint4(em_d, b, w.r, prop);
+        w.r++;
+        if (++nrk == kEqRankList) flush();
+    };
     auto push = [&](const EqItem &x) {
         if (w.n >= a.cap) {
             w.overflow = true;
             return;
         }
-        eq_insert_wave(h, w.n, x);
+        eq_insert_wave(h, w.n, x, top);
     };
     if (a.resume) {
         // input spikes added since the last run (Simulation_API::
@@ -330,13 +392,13 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
     uint32_t pops = 0;                    // models that are populations
     for (int e = 0; e < a.n_em; e++) pops |= (em[e].kind == kEqPop ? 1u : 0u) << e;
     while (w.n > 0 && !w.overflow) {
-        const EqItem it = h[1];
+        const EqItem it = top;
         const EqRemoveLoads rl = eq_remove_loads(h, w.n);
         const uint32_t top_a = __builtin_amdgcn_readfirstlane(it.a);
         const uint32_t kind = (top_a >> 24) & 0xfu, e = (top_a >> 16) & 0xffu;
         // the window ends where the populations' next update starts
         if (kind == kEqTime && ((pops >> e) & 1u) && next_k[e] > k) break;
-        eq_remove_wave(h, w.n, rl);
+        eq_remove_wave(h, w.n, rl, top);
         if (kind == kEqStop) {
             w.stop_r = (int32_t)w.r;
             break;
@@ -374,21 +436,30 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
                 push(eq_item(it.t + delay_s[bd], kEqPrioSpike, kEqGroup, e, (uint32_t)bd, cb));
                 if (lane == 0) cur[bd - 1] = cb + 1u;
             }
-            // the partial groups: delays whose count is not a multiple of 1,024
-            for (int wd = 0; wd <= (a.max_delay - 1) / 32; wd++) {
-                uint32_t bits = pmask[e][wd];
+            // the partial groups: delays whose count is not a multiple of
+            // 1,024, 64 delays at a time (lane L holds the time of delay L + 1)
+            for (int j = 0; j * 64 < a.max_delay; j++) {
+                uint64_t bits = (uint64_t)pmask[e][2 * j];
+                if (2 * j + 1 < MAXD / 32 + 1) bits |= (uint64_t)pmask[e][2 * j + 1] << 32;
+                if (!bits) continue;
+                const int dl = j * 64 + lane + 1;
+                const uint64_t tb = (uint64_t)__double_as_longlong(it.t + delay_s[dl <= a.max_delay ? dl : 0]);
                 while (bits) {
-                    const int d = wd * 32 + __builtin_ctz(bits) + 1;
-                    bits &= bits - 1u;
-                    push(eq_item(it.t + delay_s[d], kEqPrioSpike, kEqGroup, e, (uint32_t)d,
-                                 any_full ? cur[d - 1] : 0u));
+                    const int b = __builtin_ctzll(bits);
+                    bits &= bits - 1ull;
+                    const int d = j * 64 + b + 1;
+                    const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)tb, b);
+                    const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(tb >> 32), b);
+                    const double td = __longlong_as_double((long long)(((uint64_t)hi << 32) | lo));
+                    uint32_t ch = 0u;
+                    if (any_full) ch = cur[d - 1];
+                    push(eq_item(td, kEqPrioSpike, kEqGroup, e, (uint32_t)d, ch));
                 }
             }
             if (lane == 0) next_k[e] = ke + 1;
             push(eq_item(it.t + E.h, kEqPrioTime, kEqTime, e, 0, 0));
         } else if (kind == kEqGroup) {
-            if (lane == 0) eq_g(E.rank)[((int64_t)par * a.max_delay + (eq_d(it) - 1)) * E.cmax + it.b] = w.r;
-            w.r++;
+            give_rank((e << 16) | eq_d(it), it.b, 0u);
         } else if (kind == kEqInput) {
             // InputSpike::ProcessEvent: the neuron's first delay
             const int32_t nr = eq_g(E.in_neuron)[it.b];
@@ -401,8 +472,7 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             const uint32_t j = eq_d(it);
             const int32_t np0 = eq_g(E.nd_ptr)[nr];
             const int d = eq_g(E.nd_list)[np0 + (int32_t)j];
-            if (lane == 0) eq_g(E.rank)[((int64_t)par * a.max_delay + (d - 1)) * E.cmax + eq_g(E.in_pos)[it.b]] = w.r;
-            w.r++;
+            give_rank((e << 16) | (uint32_t)d, it.b, 1u);
             if (np0 + (int32_t)j + 1 < eq_g(E.nd_ptr)[nr + 1]) {
                 const int d2 = eq_g(E.nd_list)[np0 + (int32_t)j + 1];
                 push(eq_item(it.t - delay_s[d] + delay_s[d2], kEqPrioSpike, kEqProp, e, j + 1, it.b));
@@ -412,6 +482,7 @@ __device__ __forceinline__ void eq_window(EqItem *__restrict__ h, const EqArgs &
             break;
         }
     }
+    flush();
 }
 
 // The replay of one window by one workgroup of kBlock threads: the spikes
@@ -433,8 +504,16 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
     __shared__ double s_delay[MAXD + 1];
     __shared__ uint32_t s_pmask[kEqMaxEmitters][MAXD / 32 + 1];
     __shared__ uint32_t s_anyfull[kEqMaxEmitters];
+    __shared__ uint4 s_rk[kEqRankList];
     const int32_t k = a.step.get();
-    // every independent load at once: state, models, delays
+    // every independent load at once: state, models, delays, and the first
+    // tile counts of the population replayed from its tiles (the array is
+    // padded to a multiple of 8 * kBlock entries)
+    int32_t pf[8];
+    if (a.pf_tiles) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) pf[u] = eq_g(a.pf_tiles)[u * kBlock + (int32_t)threadIdx.x];
+    }
     if (threadIdx.x < kEqSt) s_st[threadIdx.x] = a.st[threadIdx.x];
     if (threadIdx.x < a.n_em) {
         s_em[threadIdx.x] = a.em[threadIdx.x];
@@ -457,6 +536,35 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
         const int32_t ke = s_next[e];
         if (ke > k || ke < 1) continue;
         const int slot = ke % a.ring;
+        if (E.all_full) {
+            // every spike joins a group of every delay: only the spike count
+            int32_t tot = 0;
+            if (E.tile_count && ke == k) {
+                for (int32_t t0 = 0; t0 < E.n_tiles; t0 += 8 * kBlock) {
+                    const bool pre = t0 == 0 && e == a.pf_em && a.pf_tiles == E.tile_count;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int32_t t = t0 + u * kBlock + (int32_t)threadIdx.x;
+                        const int32_t c = pre ? pf[u] : eq_g(E.tile_count)[t];
+                        tot += t < E.n_tiles ? c : 0;
+                    }
+                }
+            } else if (threadIdx.x == 0) {
+                tot = eq_g(E.ring_cnt)[slot];
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) tot += __shfl_xor(tot, off);
+            if ((threadIdx.x & 63u) == 0 && tot > 0)
+                for (int w = 0; w < a.words; w++) {
+                    uint32_t bits = E.full[w];
+                    while (bits) {
+                        const int b = __builtin_ctz(bits);
+                        bits &= bits - 1u;
+                        atomicAdd(&s_cnt[e][(w * 32 + b) % MAXD], (uint32_t)tot);
+                    }
+                }
+            continue;
+        }
         auto count = [&](int32_t id) {
             const EQ_GLOBAL uint32_t *m = eq_g(E.dmask) + (int64_t)id * a.words;
             bool part = false;
@@ -476,8 +584,10 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
             // of 8 * kBlock entries), so their loads are in flight together
             for (int32_t t0 = 0; t0 < E.n_tiles; t0 += 8 * kBlock) {
                 int32_t c8[8];
+                const bool pre = t0 == 0 && e == a.pf_em && a.pf_tiles == E.tile_count;
 #pragma unroll
-                for (int u = 0; u < 8; u++) c8[u] = eq_g(E.tile_count)[t0 + u * kBlock + (int32_t)threadIdx.x];
+                for (int u = 0; u < 8; u++)
+                    c8[u] = pre ? pf[u] : eq_g(E.tile_count)[t0 + u * kBlock + (int32_t)threadIdx.x];
 #pragma unroll
                 for (int u = 0; u < 8; u++) {
                     const int32_t t = t0 + u * kBlock + (int32_t)threadIdx.x;
@@ -613,8 +723,8 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
     // 5. the heap operations, by the first wave
     if (threadIdx.x < 64) {
         EqWin w{size0, (uint32_t)s_st[1], -1, false, false};
-        if (in_lds) eq_window<MAXD>(s_heap, a, s_em, s_cnt, s_pmask, s_anyfull, s_cur, s_next, s_delay, k, w);
-        else eq_window<MAXD>(a.heap, a, s_em, s_cnt, s_pmask, s_anyfull, s_cur, s_next, s_delay, k, w);
+        if (in_lds) eq_window<MAXD>(s_heap, a, s_em, s_cnt, s_pmask, s_anyfull, s_cur, s_next, s_delay, s_rk, k, w);
+        else eq_window<MAXD>(a.heap, a, s_em, s_cnt, s_pmask, s_anyfull, s_cur, s_next, s_delay, s_rk, k, w);
         if (threadIdx.x == 0) {
             s_st[0] = w.n;
             a.st[0] = w.n;
