@@ -478,6 +478,10 @@ struct edlut_network {
     // kernels of steps [clk_first, clk_first + clk_count)
     unsigned long long *clk = nullptr;
     int64_t clk_cap = 0, clk_tiles_max = 0, clk_first = 0, clk_count = 0, clk_blocks = 0;
+    // event-timed runs: HIP events recorded on the stream before and after
+    // each of the next ev_a.size() update kernels (steps launched directly)
+    std::vector<hipEvent_t> ev_a, ev_b;
+    int64_t ev_count = 0;
     int64_t fold_ready = -1;         // update whose events are already folded
     bool dirty = true;
     bool use_graph = true;
@@ -2101,6 +2105,8 @@ int edlut_network_destroy(edlut_network *n) {
         if (s.from) s.from->links_out--;
     }
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
+    for (hipEvent_t ev : n->ev_a) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : n->ev_b) (void)hipEventDestroy(ev);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
                    n->log_idx, n->log_count, n->d_step, n->d_time, n->d_time_p, n->d_rtab, n->d_skip, n->d_delay_s,
                    n->seg_off, n->clk};
@@ -2505,7 +2511,10 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         a.clk_first = (int32_t)n->clk_first;
         a.clk_count = (int32_t)n->clk_count;
     }
+    const bool ev_timed = !step.ptr && n->ev_count < (int64_t)n->ev_a.size();
+    if (ev_timed) HIP_TRY(hipEventRecord(n->ev_a[(size_t)n->ev_count], p->stream));
     hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
+    if (ev_timed) HIP_TRY(hipEventRecord(n->ev_b[(size_t)n->ev_count++], p->stream));
     CompactArgs c = compact_args(n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
                                  n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
                                  n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);
@@ -2812,7 +2821,8 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     }
     const RingEvents re = ring_events(n);
     int64_t k = first_step;
-    if (n->use_graph && n_steps >= kGraphChunk) {
+    const bool ev_pending = n->ev_count < (int64_t)n->ev_a.size();   // event-timed steps launch directly
+    if (n->use_graph && n_steps >= kGraphChunk && !ev_pending) {
         if (n->dirty || !n->graph || n->graph_fn != fn) {
             if ((rc = build_graph(n, fn))) return rc;
         }
@@ -3214,6 +3224,40 @@ int edlut_network_set_timing(edlut_network *n, int64_t max_steps) {
     int rc;
     if ((rc = dalloc(&n->clk, 2 * (size_t)max_steps * (size_t)n->clk_tiles_max))) return rc;
     n->clk_cap = max_steps;
+    return 0;
+}
+
+int edlut_network_set_event_timing(edlut_network *n, int64_t max_steps) {
+    if (!n || max_steps < 0 || max_steps > (1 << 20)) return fail(EDLUT_E_ARG, "bad argument");
+    HIP_TRY(hipSetDevice(n->pop->device));
+    HIP_TRY(hipStreamSynchronize(n->pop->stream));
+    for (hipEvent_t ev : n->ev_a) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : n->ev_b) (void)hipEventDestroy(ev);
+    n->ev_a.clear();
+    n->ev_b.clear();
+    n->ev_count = 0;
+    for (int64_t i = 0; i < max_steps; i++) {
+        hipEvent_t a = nullptr, b = nullptr;
+        HIP_TRY(hipEventCreate(&a));
+        n->ev_a.push_back(a);
+        HIP_TRY(hipEventCreate(&b));
+        n->ev_b.push_back(b);
+    }
+    return 0;
+}
+
+int edlut_network_event_timing(edlut_network *n, double *ms, int64_t *launches) {
+    if (!n || !ms || !launches) return fail(EDLUT_E_ARG, "null argument");
+    int rc = edlut_network_synchronize(n);
+    if (rc) return rc;
+    double t = 0.0;
+    for (int64_t i = 0; i < n->ev_count; i++) {
+        float e = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&e, n->ev_a[(size_t)i], n->ev_b[(size_t)i]));
+        t += e;
+    }
+    *ms = t;
+    *launches = n->ev_count;
     return 0;
 }
 

@@ -330,6 +330,18 @@ class Network:
         (per-workgroup device-clock stamps; graph replays included)."""
         C.check(C.lib().edlut_network_set_timing(self._h, int(max_steps)))
 
+    def set_event_timing(self, max_steps: int):
+        """Record HIP events around each of the next max_steps update kernels
+        (those steps launch directly, without graph replay)."""
+        C.check(C.lib().edlut_network_set_event_timing(self._h, int(max_steps)))
+
+    def event_timing(self):
+        """(total ms, launches) of the event-timed update kernels."""
+        ms = ctypes.c_double(0)
+        n = ctypes.c_int64(0)
+        C.check(C.lib().edlut_network_event_timing(self._h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
     def update_timing(self):
         """(total ms, launches) of the timed update kernels of the last run
         (device-clock span from each launch's first workgroup start to its last

@@ -68,3 +68,35 @@ def test_binomial_thresholds_distribution():
     tails = [sum(pmf[j:]) for j in range(1, len(thr) + 1)]
     assert np.allclose(thr, tails, rtol=0, atol=2**-32)
     assert np.all(np.diff(thr) <= 0)
+
+
+def test_build_flags_keep_packed_fp32_out(tmp_path):
+    """The library's hipcc flags drop packed FP32 instructions from device
+    code: a kernel whose float pairs hipcc -O3 would otherwise fuse into
+    v_pk_fma_f32 compiles to none under them (the warning hipcc prints about
+    the feature comes from its host pass)."""
+    import shutil
+    import subprocess
+
+    from edlut_amd import build
+    if not os.path.exists(build.HIPCC) and not shutil.which("hipcc"):
+        import pytest
+        pytest.skip("hipcc not available")
+    src = tmp_path / "pk.hip"
+    src.write_text("#include <hip/hip_runtime.h>\n"
+                   "__global__ void k(float2 *a, const float2 *b, int n) {\n"
+                   "  int i = blockIdx.x * blockDim.x + threadIdx.x;\n"
+                   "  if (i < n) { float2 x = a[i], y = b[i]; x.x = x.x * y.x + y.y; x.y = x.y * y.y + y.x; a[i] = x; }\n"
+                   "}\n")
+
+    def device_asm(flags):
+        out = tmp_path / "pk.s"
+        r = subprocess.run([build.HIPCC, "--offload-arch=gfx950", "-O3", "--cuda-device-only", "-S", "-o", str(out),
+                            str(src), *flags], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr
+        return out.read_text()
+
+    lib_flags = [f for f in build.FLAGS if f not in ("-shared", "-fPIC") and not f.startswith("--offload-arch")]
+    lib_flags = [f for f in lib_flags if f != "-I" and not f.endswith("include")]
+    assert "v_pk_fma_f32" in device_asm(["-ffp-contract=fast"])       # the pattern does fuse without the flag
+    assert not re.search(r"v_pk_(fma|add|mul)_f32", device_asm(lib_flags + ["-ffp-contract=fast"]))
