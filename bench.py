@@ -63,16 +63,6 @@ def read_profile_value(name: str, key: str):
     return None
 
 
-def read_traffic(path: str):
-    """HBM bytes per update-kernel launch from the committed rocprof summary."""
-    if not os.path.exists(path):
-        return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":", 1)[1])
-    return None
-
-
 def calibration_copy(n_bytes: int) -> None:
     """One device-to-device copy of n_bytes (HIP runtime blit kernel), so that a
     counter profile of this run holds a kernel of known traffic to scale the
@@ -482,6 +472,8 @@ def main():
     ap.add_argument("--calibrate-copy-mb", type=int, default=0,
                     help="also run one device copy of this size (for counter profiles)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--launch", default="",
+                    help="items,fold_lanes,fold_blocks of the main leg instead of the autotuner's choice")
     ap.add_argument("--sweep-networks", type=int, default=4096)
     ap.add_argument("--sweep-steps", type=int, default=5000)
     ap.add_argument("--seq-sample", type=int, default=16)
@@ -526,10 +518,15 @@ def main():
     model, net = E.from_spec(spec)
     print(f"setup {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
 
-    # launch shape (the network is reset after); 256 steps per trial so that
-    # the trials see the network's recurrent activity, not only its quiet start
-    items, lanes, blocks, tune_ms = net.autotune(256)
-    print(f"autotune -> items {items} lanes {lanes} blocks {blocks} ({tune_ms:.4f} ms/step)",
+    # launch shape: timed in the settled regime the timed region runs in
+    # (the network is reset after), or given by --launch
+    if args.launch:
+        items, lanes, blocks = (int(x) for x in args.launch.split(","))
+        net.set_launch(items, lanes, blocks)
+        tune_ms = None
+    else:
+        items, lanes, blocks, tune_ms = net.autotune_settled(args.settle, 256)
+    print(f"launch -> items {items} lanes {lanes} blocks {blocks} (autotune {tune_ms} ms/step)",
           file=sys.stderr, flush=True)
     net.run(args.settle)                       # settle into the asynchronous state
     net.run(args.warmup)                       # untimed warmup (synchronised)
@@ -546,7 +543,7 @@ def main():
     ms_active, launches_active = net.update_timing()
     net.set_timing(0)
     # the update kernel's launch duration as a profiler sees it: HIP events
-    # on its stream around each of kernel_steps further steps (launched
+    # bound to the dispatches of kernel_steps further steps (launched
     # directly, same settled network)
     net.set_event_timing(args.kernel_steps)
     net.run(args.kernel_steps)
@@ -562,7 +559,7 @@ def main():
     value = updates / wall
     avg_launch_s = ms_launch / 1e3 / max(launches, 1)
     achieved = BYTES_PER_UPDATE * spec.n / avg_launch_s
-    traffic = read_traffic(os.path.join(ROOT, "profiles", "adex_rk4_update_1m_pmc.txt"))
+    traffic = read_profile_value("adex_rk4_update_1m_pmc.txt", f"traffic_bytes_per_launch_items{items}")
 
     out = {
         "metric": "neuron-updates/sec (fp32, RK4) at 1 GPU",
@@ -588,9 +585,10 @@ def main():
             "settle_rate_hz": round(float(settle_rate), 3),
             "update_kernel_us": round(avg_launch_s * 1e6, 3),
             "update_kernel_timed_launches": launches,
-            "update_kernel_timing": "HIP events on the kernel's stream, steps after the timed region",
+            "update_kernel_timing": "HIP events of the dispatch (hipExtLaunchKernel), steps after the timed region",
             "update_kernel_active_us": round(ms_active / 1e3 / max(launches_active, 1) * 1e6, 3),
-            "launch": {"items_per_thread": items, "fold_lanes": lanes, "fold_blocks": blocks},
+            "launch": {"items_per_thread": items, "fold_lanes": lanes, "fold_blocks": blocks,
+                       "autotune": "24 shapes, 256 steps each, after settling" if tune_ms else "given"},
         },
         "roofline": {
             "bound": "hbm",

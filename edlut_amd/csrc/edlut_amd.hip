@@ -4,6 +4,7 @@
 // objects with device-side `new`, AdExTimeDrivenModel_GPU_C_INTERFACE.cu:
 // 171-208; here they are plain host structs passed to kernels by value).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdarg>
@@ -2512,9 +2513,14 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         a.clk_count = (int32_t)n->clk_count;
     }
     const bool ev_timed = !step.ptr && n->ev_count < (int64_t)n->ev_a.size();
-    if (ev_timed) HIP_TRY(hipEventRecord(n->ev_a[(size_t)n->ev_count], p->stream));
-    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
-    if (ev_timed) HIP_TRY(hipEventRecord(n->ev_b[(size_t)n->ev_count++], p->stream));
+    if (ev_timed) {
+        // the events take the dispatch's own start and end times
+        hipExtLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), (uint32_t)update_lds_bytes(p), p->stream,
+                              n->ev_a[(size_t)n->ev_count], n->ev_b[(size_t)n->ev_count], 0u, a);
+        n->ev_count++;
+    } else {
+        hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
+    }
     CompactArgs c = compact_args(n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
                                  n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
                                  n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);

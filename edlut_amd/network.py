@@ -331,8 +331,8 @@ class Network:
         C.check(C.lib().edlut_network_set_timing(self._h, int(max_steps)))
 
     def set_event_timing(self, max_steps: int):
-        """Record HIP events around each of the next max_steps update kernels
-        (those steps launch directly, without graph replay)."""
+        """HIP events bound to the dispatches of the next max_steps update
+        kernels (those steps launch directly, without graph replay)."""
         C.check(C.lib().edlut_network_set_event_timing(self._h, int(max_steps)))
 
     def event_timing(self):

@@ -461,6 +461,11 @@ template <typename R> struct HH : PopBase<R> {
 /* series through s^23, e ln 2 with a split ln 2, rounded to float.  The    */
 /* pin tests measure how often glibc's logf differs from it (by one ulp).   */
 /* ------------------------------------------------------------------------ */
+/* g_glibc_logf (orc_set_glibc_logf): the Nernst potential calls the C
+ * library's logf as the reference does, to measure how often the two
+ * choices change a raster.  Off by default (parity runs). */
+int g_glibc_logf = 0;
+
 float log_cr(float x) {
     if (!(x > 0.0f)) return x == 0.0f ? -INFINITY : NAN;
     if (std::isinf(x)) return x;
@@ -622,7 +627,10 @@ template <typename R> struct Granule : PopBase<R> {
     R nernst(R ci) const {
         using namespace gr;
         const float k = 1000 * (R_gas() * (temper + 273.15f) / F) / 2.0f;
-        if (sizeof(R) == sizeof(float)) return (R)(k * log_cr(std::fabs(cao / (float)ci)));
+        if (sizeof(R) == sizeof(float)) {
+            const float r = std::fabs(cao / (float)ci);
+            return (R)(k * (g_glibc_logf ? ::logf(r) : log_cr(r)));
+        }
         return (R)k * std::log(std::fabs((double)cao / (double)ci));
     }
     static float R_gas() { return gr::R; }
@@ -1557,6 +1565,8 @@ void orc_purkinje_table(float *out) {
     t.build();
     std::memcpy(out, t.v.data(), sizeof(float) * t.v.size());
 }
+
+void orc_set_glibc_logf(int on) { g_glibc_logf = on ? 1 : 0; }
 
 void orc_log_cr(const float *x, float *out, int n) {
     for (int i = 0; i < n; i++) out[i] = log_cr(x[i]);

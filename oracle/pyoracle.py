@@ -55,6 +55,7 @@ def lib():
         l.orc_pop_set_ext_current.argtypes = [P, P]
         l.orc_pop_update.argtypes = [P, ctypes.c_int64, P]
         l.orc_set_threads.argtypes = [ctypes.c_int]
+        l.orc_set_glibc_logf.argtypes = [ctypes.c_int]
         l.orc_set_order.argtypes = [ctypes.c_int]
         l.orc_net_set_source_model.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         l.orc_net_set_train_times.argtypes = [P, ctypes.c_int, ctypes.c_int64, P, P, P]
@@ -109,6 +110,12 @@ def lib():
 
 def _p(a):
     return ctypes.c_void_p(a.ctypes.data)
+
+
+def set_glibc_logf(on: bool) -> None:
+    """The granule's Nernst logarithm through the C library's logf (as the
+    reference) instead of the correctly rounded one (default)."""
+    lib().orc_set_glibc_logf(int(bool(on)))
 
 
 def set_threads(n: int) -> None:
