@@ -62,8 +62,7 @@ EXPORTS = (
     "edlut_network_set_source_log",
     "edlut_network_source_spike_log",
     "edlut_network_set_log_capacity",
-    "edlut_network_set_event_timing",
-    "edlut_network_event_timing",
+    "edlut_network_update_dispatch_timing",
     "edlut_network_spike_count",
     "edlut_network_set_timing",
     "edlut_network_update_timing",
@@ -160,8 +159,7 @@ def _sig(lib):
         "edlut_network_set_source_log": (I32, [P, I32, I64]),
         "edlut_network_source_spike_log": (I32, [P, I32, P, P, I64, P]),
         "edlut_network_set_log_capacity": (I32, [P, I64]),
-        "edlut_network_set_event_timing": (I32, [P, I64]),
-        "edlut_network_event_timing": (I32, [P, P, P]),
+        "edlut_network_update_dispatch_timing": (I32, [P, P, P]),
         "edlut_network_spike_count": (I32, [P, P]),
         "edlut_network_set_timing": (I32, [P, I64]),
         "edlut_network_update_timing": (I32, [P, P, P]),

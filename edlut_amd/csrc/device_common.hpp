@@ -95,7 +95,21 @@ struct CompactArgs {
     int64_t log_cap;
     uint32_t *err;         // bit 1: log overflow
     StepRef step;
+    // timed runs: the earliest workgroup start of this launch, per step in
+    // [clk_first, clk_first + clk_count), on the device wall clock (null:
+    // untimed).  It ends the previous update kernel's launch as a kernel
+    // trace sees it (back-to-back dispatches).
+    unsigned long long *clk_next;
+    int32_t clk_first;
+    int32_t clk_count;
 };
+
+__device__ __forceinline__ void clock_next_kernel(const CompactArgs &c) {
+    if (c.clk_next && threadIdx.x == 0) {
+        const int32_t s = c.step.get() - c.clk_first;
+        if (s >= 0 && s < c.clk_count) atomicMin(&c.clk_next[s], (unsigned long long)wall_clock64());
+    }
+}
 
 constexpr int kCompactTiles = kBlock;   // tiles per compaction workgroup
 
@@ -170,7 +184,10 @@ __device__ __forceinline__ void compact_body(const CompactArgs &a, int bid, int 
     }
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) { compact_body(a, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(kBlock) void k_compact(CompactArgs a) {
+    clock_next_kernel(a);
+    compact_body(a, blockIdx.x, gridDim.x);
+}
 
 // Advances a device step base (last node of a captured chunk).
 __global__ void k_advance_step(int32_t *step, int32_t by) {

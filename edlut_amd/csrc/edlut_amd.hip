@@ -4,7 +4,6 @@
 // objects with device-side `new`, AdExTimeDrivenModel_GPU_C_INTERFACE.cu:
 // 171-208; here they are plain host structs passed to kernels by value).
 #include <hip/hip_runtime.h>
-#include <hip/hip_ext.h>
 
 #include <cmath>
 #include <cstdarg>
@@ -479,10 +478,7 @@ struct edlut_network {
     // kernels of steps [clk_first, clk_first + clk_count)
     unsigned long long *clk = nullptr;
     int64_t clk_cap = 0, clk_tiles_max = 0, clk_first = 0, clk_count = 0, clk_blocks = 0;
-    // event-timed runs: HIP events recorded on the stream before and after
-    // each of the next ev_a.size() update kernels (steps launched directly)
-    std::vector<hipEvent_t> ev_a, ev_b;
-    int64_t ev_count = 0;
+    unsigned long long *clk_next = nullptr;   // per timed step: the next kernel's first start
     int64_t fold_ready = -1;         // update whose events are already folded
     bool dirty = true;
     bool use_graph = true;
@@ -1285,6 +1281,9 @@ CompactArgs compact_args(int32_t *tile_count, int32_t *staging, int n_tiles, int
     c.log_cap = log_cap;
     c.err = err;
     c.step = step;
+    c.clk_next = nullptr;
+    c.clk_first = 0;
+    c.clk_count = 0;
     return c;
 }
 
@@ -2106,11 +2105,9 @@ int edlut_network_destroy(edlut_network *n) {
         if (s.from) s.from->links_out--;
     }
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
-    for (hipEvent_t ev : n->ev_a) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : n->ev_b) (void)hipEventDestroy(ev);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
                    n->log_idx, n->log_count, n->d_step, n->d_time, n->d_time_p, n->d_rtab, n->d_skip, n->d_delay_s,
-                   n->seg_off, n->clk};
+                   n->seg_off, n->clk, n->clk_next};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     delete n->eq;
@@ -2512,18 +2509,15 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         a.clk_first = (int32_t)n->clk_first;
         a.clk_count = (int32_t)n->clk_count;
     }
-    const bool ev_timed = !step.ptr && n->ev_count < (int64_t)n->ev_a.size();
-    if (ev_timed) {
-        // the events take the dispatch's own start and end times
-        hipExtLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), (uint32_t)update_lds_bytes(p), p->stream,
-                              n->ev_a[(size_t)n->ev_count], n->ev_b[(size_t)n->ev_count], 0u, a);
-        n->ev_count++;
-    } else {
-        hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
-    }
+    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
     CompactArgs c = compact_args(n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
                                  n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
                                  n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);
+    if (n->clk) {
+        c.clk_next = n->clk_next;
+        c.clk_first = (int32_t)n->clk_first;
+        c.clk_count = (int32_t)n->clk_count;
+    }
     const int nbc = compact_blocks(tiles);
     // STDP orders the post-synaptic learning of update k before the arrivals
     // of update k + 1, so plastic networks fold in their own step.
@@ -2748,6 +2742,7 @@ static int run_prepare(edlut_network *n, int64_t first_step, int n_steps, Update
         n->dirty = true;   // captured kernel arguments carry the timing window
         HIP_TRY(hipMemsetAsync(n->clk, 0, sizeof(unsigned long long) * 2 * n->clk_cap * n->clk_tiles_max,
                                p->stream));
+        HIP_TRY(hipMemsetAsync(n->clk_next, 0xff, sizeof(unsigned long long) * n->clk_cap, p->stream));
     }
     return 0;
 }
@@ -2827,8 +2822,7 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     }
     const RingEvents re = ring_events(n);
     int64_t k = first_step;
-    const bool ev_pending = n->ev_count < (int64_t)n->ev_a.size();   // event-timed steps launch directly
-    if (n->use_graph && n_steps >= kGraphChunk && !ev_pending) {
+    if (n->use_graph && n_steps >= kGraphChunk) {
         if (n->dirty || !n->graph || n->graph_fn != fn) {
             if ((rc = build_graph(n, fn))) return rc;
         }
@@ -3221,49 +3215,19 @@ int edlut_network_set_timing(edlut_network *n, int64_t max_steps) {
     HIP_TRY(hipSetDevice(n->pop->device));
     HIP_TRY(hipStreamSynchronize(n->pop->stream));
     if (n->clk) (void)hipFree(n->clk);
+    if (n->clk_next) (void)hipFree(n->clk_next);
     n->clk = nullptr;
+    n->clk_next = nullptr;
     n->timed = 0;
     n->clk_cap = 0;
     n->dirty = true;
     if (max_steps == 0) return 0;
     n->clk_tiles_max = n->pop->stride / kFlagTile;   // the most workgroups of any launch shape
     int rc;
-    if ((rc = dalloc(&n->clk, 2 * (size_t)max_steps * (size_t)n->clk_tiles_max))) return rc;
+    if ((rc = dalloc(&n->clk, 2 * (size_t)max_steps * (size_t)n->clk_tiles_max)) ||
+        (rc = dalloc(&n->clk_next, (size_t)max_steps)))
+        return rc;
     n->clk_cap = max_steps;
-    return 0;
-}
-
-int edlut_network_set_event_timing(edlut_network *n, int64_t max_steps) {
-    if (!n || max_steps < 0 || max_steps > (1 << 20)) return fail(EDLUT_E_ARG, "bad argument");
-    HIP_TRY(hipSetDevice(n->pop->device));
-    HIP_TRY(hipStreamSynchronize(n->pop->stream));
-    for (hipEvent_t ev : n->ev_a) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : n->ev_b) (void)hipEventDestroy(ev);
-    n->ev_a.clear();
-    n->ev_b.clear();
-    n->ev_count = 0;
-    for (int64_t i = 0; i < max_steps; i++) {
-        hipEvent_t a = nullptr, b = nullptr;
-        HIP_TRY(hipEventCreate(&a));
-        n->ev_a.push_back(a);
-        HIP_TRY(hipEventCreate(&b));
-        n->ev_b.push_back(b);
-    }
-    return 0;
-}
-
-int edlut_network_event_timing(edlut_network *n, double *ms, int64_t *launches) {
-    if (!n || !ms || !launches) return fail(EDLUT_E_ARG, "null argument");
-    int rc = edlut_network_synchronize(n);
-    if (rc) return rc;
-    double t = 0.0;
-    for (int64_t i = 0; i < n->ev_count; i++) {
-        float e = 0.0f;
-        HIP_TRY(hipEventElapsedTime(&e, n->ev_a[(size_t)i], n->ev_b[(size_t)i]));
-        t += e;
-    }
-    *ms = t;
-    *launches = n->ev_count;
     return 0;
 }
 
@@ -3292,6 +3256,36 @@ int edlut_network_update_timing(edlut_network *n, double *ms_clock, int64_t *lau
         }
     }
     if (ms_clock) *ms_clock = t_clock;
+    return 0;
+}
+
+int edlut_network_update_dispatch_timing(edlut_network *n, double *ms, int64_t *launches) {
+    if (!n || !ms || !launches) return fail(EDLUT_E_ARG, "null argument");
+    int rc = edlut_network_synchronize(n);
+    if (rc) return rc;
+    *launches = n->timed;
+    double t = 0.0;
+    if (n->timed > 0) {
+        const size_t nb = (size_t)n->clk_blocks;
+        std::vector<unsigned long long> c(2 * (size_t)n->timed * nb), nx((size_t)n->timed);
+        HIP_TRY(hipMemcpy(c.data(), n->clk, sizeof(unsigned long long) * c.size(), hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(nx.data(), n->clk_next, sizeof(unsigned long long) * nx.size(), hipMemcpyDeviceToHost));
+        int khz = 0;
+        HIP_TRY(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, n->pop->device));
+        if (khz <= 0) return fail(EDLUT_E_HIP, "device wall-clock rate unavailable");
+        for (int64_t i = 0; i < n->timed; i++) {
+            unsigned long long lo = ~0ull;
+            for (size_t b = 0; b < nb; b++) {
+                const unsigned long long s0 = c[(i * nb + b) * 2];
+                if (s0 == 0) return fail(EDLUT_E_STATE, "a timed launch left no clock stamp");
+                lo = s0 < lo ? s0 : lo;
+            }
+            if (nx[(size_t)i] == ~0ull || nx[(size_t)i] < lo)
+                return fail(EDLUT_E_STATE, "timed update %lld has no following compaction stamp", (long long)i);
+            t += (double)(nx[(size_t)i] - lo) / (double)khz;   // ms
+        }
+    }
+    *ms = t;
     return 0;
 }
 

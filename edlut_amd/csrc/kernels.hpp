@@ -864,6 +864,7 @@ __global__ __launch_bounds__(kBlock) void k_fold_ring(FoldTarget t, RingEvents r
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void k_compact_fold(CompactArgs c, FoldTarget t, RingEvents r,
                                                          int nb_compact) {
+    clock_next_kernel(c);
     if ((int)blockIdx.x < nb_compact) compact_body(c, blockIdx.x, nb_compact);
     else fold_ring_body<kStatic, LANES>(t, r, blockIdx.x - nb_compact, gridDim.x - nb_compact);
 }
@@ -875,6 +876,7 @@ __global__ __launch_bounds__(kBlock) void k_compact_fold(CompactArgs c, FoldTarg
 template <int LANES>
 __global__ __launch_bounds__(kBlock) void k_compact_fold_eq(CompactArgs c, FoldTarget t, RingEvents r,
                                                             int nb_compact, EqArgs q) {
+    clock_next_kernel(c);
     const int b = (int)blockIdx.x - 1;
     if (b < 0) eq_body<kEqFusedItems, kEqFusedDelays>(q);
     else if (b < nb_compact) compact_body(c, b, nb_compact);

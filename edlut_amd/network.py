@@ -330,16 +330,13 @@ class Network:
         (per-workgroup device-clock stamps; graph replays included)."""
         C.check(C.lib().edlut_network_set_timing(self._h, int(max_steps)))
 
-    def set_event_timing(self, max_steps: int):
-        """HIP events bound to the dispatches of the next max_steps update
-        kernels (those steps launch directly, without graph replay)."""
-        C.check(C.lib().edlut_network_set_event_timing(self._h, int(max_steps)))
-
-    def event_timing(self):
-        """(total ms, launches) of the event-timed update kernels."""
+    def update_dispatch_timing(self):
+        """(total ms, launches) of the timed update kernels from each launch's
+        first workgroup start to the next compaction launch's (the duration a
+        kernel trace reports for back-to-back dispatches)."""
         ms = ctypes.c_double(0)
         n = ctypes.c_int64(0)
-        C.check(C.lib().edlut_network_event_timing(self._h, C.byref(ms), C.byref(n)))
+        C.check(C.lib().edlut_network_update_dispatch_timing(self._h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
     def update_timing(self):

@@ -360,13 +360,11 @@ int edlut_network_set_timing(edlut_network *net, int64_t max_steps);
 /* Total milliseconds of the timed update-kernel launches of the last run and
  * their count. */
 int edlut_network_update_timing(edlut_network *net, double *ms, int64_t *launches);
-/* HIP events that take the start and end times of each of the next
- * max_steps update-kernel dispatches (hipExtLaunchKernel; 0 frees them);
- * those steps are launched directly, not from captured graphs.  edlut_network_event_timing returns
- * the summed event time (ms) and the launches timed.  A measurement aid with
- * no counterpart in the reference. */
-int edlut_network_set_event_timing(edlut_network *net, int64_t max_steps);
-int edlut_network_event_timing(edlut_network *net, double *ms, int64_t *launches);
+/* The timed update kernels' launch durations as a kernel trace measures them
+ * (the dispatch and the end-of-kernel cache write-back included): summed
+ * from each launch's first workgroup start to the first workgroup start of
+ * the compaction launch that follows it on the stream.  Plain networks. */
+int edlut_network_update_dispatch_timing(edlut_network *net, double *ms, int64_t *launches);
 
 /*
  * Synthetic recurrent wiring generated on the device (replaces building and

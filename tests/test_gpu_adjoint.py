@@ -14,6 +14,12 @@ from edlut_amd.adjoint import FitProblem, refractory_table
 
 pytestmark = pytest.mark.gpu
 MODELS = ["LIFTimeDrivenModel_GPU", "AdExTimeDrivenModel_GPU"]
+# GPU forward against the float oracle, on neurons with equal spike trains:
+# (share of neurons whose trace stays within 1e-5 relative, largest relative
+# difference).  They differ only in their libms' expf; LIF's traces come out
+# identical, AdEx's exponential term amplifies a one-ulp expf difference on
+# the upswing before a spike on a few neurons.
+FWD_TOL = {"LIFTimeDrivenModel_GPU": (1.0, 1e-5), "AdExTimeDrivenModel_GPU": (0.9, 5e-3)}
 KEYS = ("g_leak", "e_leak", "tau_exc", "tau_inh", "weight")
 
 
@@ -34,10 +40,12 @@ def test_gpu_adjoint_matches_fp64(oracle, model):
     """The adjoint kernel's gradients equal the FP64 adjoint evaluated along
     the GPU's own forward trajectory to 1e-4 (relative, per parameter
     group).  Against the all-FP64 run (its own FP64 trajectory) LIF agrees to
-    1e-4 as well; for AdEx the gradient through the exponential upswing is
-    ill-conditioned: float rounding of the trajectory alone moves it by tens
-    of percent (the oracle reproduces this on the CPU), so that comparison is
-    reported, not asserted."""
+    1e-4 as well.  For AdEx the gradient through the exponential upswing is
+    ill-conditioned: float rounding of the trajectory alone moves the exact
+    adjoint by tens of percent (the conditioning, printed; the oracle
+    reproduces it on the CPU), so the assertion is that the GPU's distance
+    from the all-FP64 run is no more than that plus 1e-4, and 1e-4 on the
+    neurons that never spike."""
     n, steps = 256, 10000
     print(f"case adjoint {model} n={n} steps={steps}", flush=True)
     spec, prob, o = _setup(oracle, model, n, steps)
@@ -48,10 +56,18 @@ def test_gpu_adjoint_matches_fp64(oracle, model):
         v, spk_gpu, traj, since = prob.forward_raw(*x, spikes=True, trajectory=True)
     torch.cuda.synchronize()
     vg = v.cpu().numpy()
-    # forward against the float oracle (expf of different libms: state tolerance)
+    # forward against the float oracle: the two differ only by their libms'
+    # expf (decay factors, AdEx exponential term), so on neurons whose spike
+    # trains agree the traces agree to the state tolerance of the contract
     vf, spk_f = o.forward(*[spec.start[k] for k in KEYS], fp64=False)
-    close = np.max(np.abs(vg - vf) / np.maximum(np.abs(vf), 1.0), axis=0) < 1e-3
-    assert close.mean() > 0.9, close.mean()
+    agree = np.all(spk_gpu.cpu().numpy() == spk_f, axis=0)
+    rel = np.max(np.abs(vg - vf) / np.maximum(np.abs(vf), 1.0), axis=0)
+    within = (rel[agree] < 1e-5).mean()
+    print(f"{model}: forward vs float oracle, {agree.mean():.3f} of neurons with equal spikes, relative state "
+          f"difference there: median {np.median(rel[agree]):.2e}, {within:.3f} within 1e-5, "
+          f"max {rel[agree].max():.2e}")
+    assert agree.mean() > 0.8, agree.mean()
+    assert within >= FWD_TOL[model][0] and rel[agree].max() < FWD_TOL[model][1], (within, rel[agree].max())
     # GPU gradient through the autograd function
     xs = [t.clone().requires_grad_(True) for t in x]
     vv = prob.simulate(*xs)
@@ -74,11 +90,24 @@ def test_gpu_adjoint_matches_fp64(oracle, model):
     gp64, gw64 = o.backward(*[spec.start[k] for k in KEYS], gv64)
     same = np.all(spk_gpu.cpu().numpy() == spk64, axis=0)
     assert same.mean() > 0.8, same.mean()
-    for name, a, b in zip(KEYS, g, [gp64[0], gp64[1], gp64[2], gp64[3], gw64]):
+    quiet = same & (spk64.sum(axis=0) == 0)
+    for name, a, r, b in zip(KEYS, g, ref, [gp64[0], gp64[1], gp64[2], gp64[3], gw64]):
         err = np.linalg.norm(a[same] - b[same]) / np.linalg.norm(b[same])
-        print(f"{model} {name}: relative error vs the all-FP64 run {err:.2e}")
+        # conditioning: how far float rounding of the trajectory alone moves
+        # the exact (FP64) adjoint
+        cond = np.linalg.norm(r[same] - b[same]) / np.linalg.norm(b[same])
+        print(f"{model} {name}: relative error vs the all-FP64 run {err:.2e}, of which the trajectory's "
+              f"rounding (FP64 adjoint along the GPU trajectory vs all-FP64) {cond:.2e}")
         if model == "LIFTimeDrivenModel_GPU":
             assert err < 1e-4, (name, err)
+        else:
+            # AdEx: what the float trajectory explains, plus the kernel's own 1e-4
+            assert err <= cond + 1e-4, (name, err, cond)
+            if quiet.sum() >= 8:
+                # neurons that never spike are well conditioned
+                errq = np.linalg.norm(a[quiet] - b[quiet]) / np.linalg.norm(b[quiet])
+                print(f"{model} {name}: {quiet.sum()} silent neurons, relative error {errq:.2e}")
+                assert errq < 1e-4, (name, errq)
 
 
 @pytest.mark.parametrize("model", MODELS)
@@ -121,3 +150,49 @@ def test_gpu_adjoint_cost_ratio_and_fit(oracle, model):
                 xs[j].clamp_(min=1e-3)
         losses.append(float(loss))
     assert losses[-1] < losses[0], losses
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_gpu_gradient_matches_finite_differences(oracle, model):
+    """The GPU adjoint's directional derivative of the trace loss equals a
+    central finite difference of the GPU forward itself, for each parameter
+    group: perturbations of 1e-4 relative, restricted to neurons whose spike
+    trains the perturbation leaves unchanged (a spike-stable window)."""
+    n, steps = 256, 2000
+    print(f"case finite differences {model}", flush=True)
+    spec, prob, o = _setup(oracle, model, n, steps)
+    x = _dev(spec.start)
+    with torch.no_grad():
+        target = prob.forward_raw(*_dev(spec.target)).double()
+        v0, s0 = prob.forward_raw(*x, spikes=True)
+    xs = [t.clone().requires_grad_(True) for t in x]
+    loss = ((prob.simulate(*xs).double() - target) ** 2).mean()
+    loss.backward()
+    g = [t.grad.double() for t in xs]
+
+    def run(vals):
+        with torch.no_grad():
+            v, spk = prob.forward_raw(*vals, spikes=True)
+        return ((v.double() - target) ** 2).mean().item(), spk
+
+    for j, name in enumerate(KEYS):
+        d = x[j] * 1e-4
+        for _ in range(4):
+            xp = [t.clone() for t in x]
+            xm = [t.clone() for t in x]
+            xp[j] = x[j] + d
+            xm[j] = x[j] - d
+            lp, sp = run(xp)
+            lm, sm = run(xm)
+            moved = torch.any((sp != s0) | (sm != s0), dim=0)
+            if not bool(moved.any()):
+                break
+            d = d.clone()
+            d[moved] = 0.0     # neurons whose spikes moved leave the direction
+        assert not bool(moved.any()), f"{name}: no spike-stable direction"
+        dd = (xp[j] - x[j]).double() - (xm[j] - x[j]).double()    # the perturbation as stored in float
+        fd = (lp - lm)
+        an = float((g[j] * dd).sum())
+        print(f"{model} {name}: finite difference {fd:.6e}, adjoint {an:.6e}, "
+              f"{int((d != 0).sum())} of {d.numel()} entries perturbed")
+        assert abs(fd - an) <= 2e-2 * abs(fd) + 1e-12, (name, fd, an)

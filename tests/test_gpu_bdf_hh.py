@@ -1,6 +1,6 @@
 """GPU parity of the HH model and the BDF integrator against the oracle.
 
-HH runs under Euler, RK2, RK4 and BDF orders 1-3; AdEx and LIF under BDF2.
+HH runs under Euler, RK2, RK4 and BDF orders 1-6; AdEx and LIF under BDF2.
 Every update's spike list and the final state must be bit-identical to the
 oracle (float Newton).  The FP64 Newton mode of HH is checked bit-exactly
 against the oracle's FP64 variant with the state rounded to float after
@@ -50,7 +50,7 @@ def _run(oracle, model, integ, order, n, steps, fp64_newton=False, wmax=None, cu
 
 
 @pytest.mark.parametrize("integ,order", [("Euler", 1), ("RK2", 1), ("RK4", 1), ("BDF", 1), ("BDF", 2),
-                                         ("BDF", 3)])
+                                         ("BDF", 3), ("BDF", 4), ("BDF", 5), ("BDF", 6)])
 def test_hh_bit_exact(oracle, integ, order):
     print(f"case HH {integ} order={order}", flush=True)
     total = _run(oracle, "HHTimeDrivenModel_GPU", integ, order, 513, 400, wmax=3.0, cur_scale=400.0)
@@ -64,9 +64,10 @@ def test_bdf2_adex_lif_bit_exact(oracle, model, wmax):
     assert total > 0
 
 
-def test_hh_fp64_newton_bit_exact(oracle):
-    print("case HH BDF2 fp64 Newton", flush=True)
-    total = _run(oracle, "HHTimeDrivenModel_GPU", "BDF", 2, 257, 400, fp64_newton=True, wmax=3.0,
+@pytest.mark.parametrize("order", [2, 6])
+def test_hh_fp64_newton_bit_exact(oracle, order):
+    print(f"case HH BDF{order} fp64 Newton", flush=True)
+    total = _run(oracle, "HHTimeDrivenModel_GPU", "BDF", order, 257, 400, fp64_newton=True, wmax=3.0,
                  cur_scale=400.0)
     assert total > 0
 

@@ -73,6 +73,34 @@ def test_cerebellum_circuit_bit_exact(oracle, pc_model, graph):
     circuit.close()
 
 
+def test_cerebellum_full_in_degree(oracle):
+    """Config 3's connectivity at a reduced population: 2,500 granule cells,
+    Purkinje in-degree 2,000 (config 3's), Golgi in-degree 100, mossy-fibre
+    in-degree 4, default weights; every raster, state and plastic synapse
+    bit-identical to the oracle circuit."""
+    steps = 300
+    print(f"case cerebellum full in-degree steps={steps}", flush=True)
+    spec = W.cerebellum(n_grc=2500, n_goc=50, n_pc=10, n_mf=500, k_pc=2000, steps=steps)
+    models, nets, circuit = cerebellum_from_spec(spec)
+    circuit.run(steps)
+    opops, onets = _oracle_circuit(oracle, spec)
+    oracle.circuit_run(list(onets), 1, steps)
+    total = {}
+    for name, m, n, op, on in zip(("granule", "Golgi", "Purkinje"), models, nets, opops, onets):
+        st, ix = n.spikes()
+        ost, oix = on.take_log()
+        assert np.array_equal(st, ost) and np.array_equal(ix, oix), f"{name} raster"
+        total[name] = len(ost)
+        for v in range(op.nv):
+            assert np.array_equal(bits(m.GetVectorNeuronState(v)), bits(op.state(v))), f"{name} variable {v}"
+    print("spikes:", total)
+    assert total["granule"] > 50 and total["Purkinje"] > 0, total
+    w_o, pre_o, post_o, last_o = onets[2].synapses(0)
+    _, _, w_g = nets[2].synapses(0)
+    assert np.array_equal(bits(w_g), bits(w_o))
+    circuit.close()
+
+
 def _lif_net(max_delay, n=10):
     dt = float(np.float32(1e-4))
     m = E.CreateNeuronModel("LIFTimeDrivenModel_GPU", {"int_meth": E.integration_method("Euler", dt)})

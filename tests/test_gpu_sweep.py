@@ -33,26 +33,26 @@ def _params(model, n_groups, seed=0):
     return p
 
 
-def _grouped(model, integ, params, sizes, w_ext, items=None):
+def _grouped(model, integ, params, sizes, w_ext, items=None, in_deg=IN_DEG, max_d=MAX_D):
     m = E.CreateNeuronModel(model)
     m.SetParameters({"int_meth": E.integration_method(integ, 1e-4)})
     m.InitializeGroups(sizes, params)
-    net = Network(m, MAX_D)
-    net.generate_recurrent(IN_DEG, 0.8, W_MU, W_SIG, SEED)
+    net = Network(m, max_d)
+    net.generate_recurrent(in_deg, 0.8, W_MU, W_SIG, SEED)
     net.set_external(binomial_thresholds(100, 5.0 * 1e-4), w_ext, EXT_SEED)
     if items is not None:
         net.set_launch(*items)
     return m, net
 
 
-def _plain(model, integ, params_g, n, g, w_ext):
+def _plain(model, integ, params_g, n, g, w_ext, in_deg=IN_DEG, max_d=MAX_D):
     m = E.CreateNeuronModel(model)
     names = [k for k, _ in type(m).PARAMS]
     m.SetParameters({k: float(v) for k, v in zip(names, params_g)})
     m.SetParameters({"int_meth": E.integration_method(integ, 1e-4)})
     m.InitializeStates(int(n))
-    net = Network(m, MAX_D)
-    net.generate_recurrent(IN_DEG, 0.8, W_MU, W_SIG, SEED + g)
+    net = Network(m, max_d)
+    net.generate_recurrent(in_deg, 0.8, W_MU, W_SIG, SEED + g)
     net.set_external(binomial_thresholds(100, 5.0 * 1e-4), w_ext, EXT_SEED + g)
     net.set_event_order(False)   # grouped populations deliver in the canonical order
     return m, net
@@ -105,6 +105,50 @@ def test_groups_equal_separate_networks(oracle, model, integ, w_ext):
         inside |= (gix >= start[g]) & (gix < start[g] + SIZES[g])
     assert inside.all()
     assert total == len(gst) and total > 200, "the groups should spike"
+
+
+def test_full_shape_groups_equal_separate_networks(oracle):
+    """Config 4's shape: groups of 1,000 to 20,000 AdEx neurons at in-degree
+    100 with delays of 1-10 steps, driven hard enough that some groups
+    ignite.  Every group equals its network built alone; the smallest also
+    equals the oracle."""
+    model, integ, w_ext, steps, in_deg, max_d = "AdExTimeDrivenModel_GPU", "RK4", 1.0, 400, 100, 10
+    sizes = np.array([1000, 20000, 4096], np.int32)
+    params = _params(model, len(sizes), seed=21)
+    print(f"case full-shape groups {sizes.tolist()} in-degree {in_deg}", flush=True)
+    gm, gnet = _grouped(model, integ, params, sizes, w_ext, in_deg=in_deg, max_d=max_d)
+    gnet.run(steps)
+    gst, gix = gnet.spikes()
+    start, _ = gm.GetGroups()
+    gstate = [gm.GetVectorNeuronState(v) for v in range(type(gm).N_DIFF + 2)]
+    rates = []
+    for g in range(len(sizes)):
+        sel = (gix >= start[g]) & (gix < start[g] + sizes[g])
+        pm, pnet = _plain(model, integ, params[g], sizes[g], g, w_ext, in_deg=in_deg, max_d=max_d)
+        pnet.run(steps)
+        pst, pix = pnet.spikes()
+        assert np.array_equal(gst[sel], pst) and np.array_equal(gix[sel] - start[g], pix), f"group {g} raster"
+        for v, full in enumerate(gstate):
+            assert np.array_equal(bits(full[start[g]:start[g] + sizes[g]]),
+                                  bits(pm.GetVectorNeuronState(v))), f"group {g} variable {v}"
+        rates.append(len(pst) / (sizes[g] * steps * 1e-4))
+        if g == 0:
+            pop = oracle.Population(model, integ, params[g], int(sizes[g]), mask=3)
+            onet = oracle.Net(pop, max_d)
+            onet.add_self(*pnet.synapses(0))
+            onet.set_external(binomial_thresholds(100, 5.0 * 1e-4), w_ext, EXT_SEED + g)
+            oracle.set_order(0)
+            try:
+                ost, oix = onet.run(1, steps)
+            finally:
+                oracle.set_order(1)
+            assert np.array_equal(ost, pst) and np.array_equal(oix, pix), "oracle raster"
+            for v in range(pop.nv - 2):
+                assert np.array_equal(bits(pop.state(v)), bits(pm.GetVectorNeuronState(v))), f"oracle var {v}"
+        pnet.close()
+        pm.close()
+    print("rates (Hz):", [round(r, 1) for r in rates])
+    assert max(rates) > 50.0, "some group should ignite"
 
 
 def test_generated_wiring_properties():

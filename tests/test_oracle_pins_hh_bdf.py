@@ -115,10 +115,13 @@ def test_hh_rhs_one_euler_step(oracle, S0, I):
 
 BDF_TEXTBOOK = {1: (1.0, [1.0]), 2: (2.0 / 3.0, [4.0 / 3.0, -1.0 / 3.0]),
                 3: (6.0 / 11.0, [18.0 / 11.0, -9.0 / 11.0, 2.0 / 11.0]),
-                4: (12.0 / 25.0, [48.0 / 25.0, -36.0 / 25.0, 16.0 / 25.0, -3.0 / 25.0])}
+                4: (12.0 / 25.0, [48.0 / 25.0, -36.0 / 25.0, 16.0 / 25.0, -3.0 / 25.0]),
+                5: (60.0 / 137.0, [300.0 / 137.0, -300.0 / 137.0, 200.0 / 137.0, -75.0 / 137.0, 12.0 / 137.0]),
+                6: (60.0 / 147.0, [360.0 / 147.0, -450.0 / 147.0, 400.0 / 147.0, -225.0 / 147.0, 72.0 / 147.0,
+                                   -10.0 / 147.0])}
 
 
-@pytest.mark.parametrize("order", [1, 2, 3, 4])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6])
 def test_bdf_linear_ode_follows_textbook_recursion(oracle, order):
     """dV/dt = (I + g_leak (e_leak - V)) / c_m; step n uses BDF-k with
     k = max(1, min(n - 1, order)) (the variable-order start)."""
