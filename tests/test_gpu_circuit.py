@@ -76,11 +76,12 @@ def test_cerebellum_circuit_bit_exact(oracle, pc_model, graph):
 def test_cerebellum_full_in_degree(oracle):
     """Config 3's connectivity at a reduced population: 2,500 granule cells,
     Purkinje in-degree 2,000 (config 3's), Golgi in-degree 100, mossy-fibre
-    in-degree 4, default weights; every raster, state and plastic synapse
-    bit-identical to the oracle circuit."""
-    steps = 300
+    in-degree 4, with the stronger granule weights of the small circuit (so
+    the few granule spikes of a short run reach the targets); every raster,
+    state and plastic synapse bit-identical to the oracle circuit."""
+    steps = 800
     print(f"case cerebellum full in-degree steps={steps}", flush=True)
-    spec = W.cerebellum(n_grc=2500, n_goc=50, n_pc=10, n_mf=500, k_pc=2000, steps=steps)
+    spec = W.cerebellum(n_grc=2500, n_goc=50, n_pc=10, n_mf=500, k_pc=2000, steps=steps, w_goc=0.3, w_pc=2.0)
     models, nets, circuit = cerebellum_from_spec(spec)
     circuit.run(steps)
     opops, onets = _oracle_circuit(oracle, spec)
@@ -94,7 +95,7 @@ def test_cerebellum_full_in_degree(oracle):
         for v in range(op.nv):
             assert np.array_equal(bits(m.GetVectorNeuronState(v)), bits(op.state(v))), f"{name} variable {v}"
     print("spikes:", total)
-    assert total["granule"] > 50 and total["Purkinje"] > 0, total
+    assert total["granule"] > 50 and total["Golgi"] > 0 and total["Purkinje"] > 0, total
     w_o, pre_o, post_o, last_o = onets[2].synapses(0)
     _, _, w_g = nets[2].synapses(0)
     assert np.array_equal(bits(w_g), bits(w_o))

@@ -45,7 +45,7 @@ def _grouped(model, integ, params, sizes, w_ext, items=None, in_deg=IN_DEG, max_
     return m, net
 
 
-def _plain(model, integ, params_g, n, g, w_ext, in_deg=IN_DEG, max_d=MAX_D):
+def _plain(model, integ, params_g, n, g, w_ext, in_deg=IN_DEG, max_d=MAX_D, event_order=False):
     m = E.CreateNeuronModel(model)
     names = [k for k, _ in type(m).PARAMS]
     m.SetParameters({k: float(v) for k, v in zip(names, params_g)})
@@ -54,7 +54,8 @@ def _plain(model, integ, params_g, n, g, w_ext, in_deg=IN_DEG, max_d=MAX_D):
     net = Network(m, max_d)
     net.generate_recurrent(in_deg, 0.8, W_MU, W_SIG, SEED + g)
     net.set_external(binomial_thresholds(100, 5.0 * 1e-4), w_ext, EXT_SEED + g)
-    net.set_event_order(False)   # grouped populations deliver in the canonical order
+    if not event_order:
+        net.set_event_order(False)   # grouped populations deliver in the canonical order
     return m, net
 
 
@@ -149,6 +150,35 @@ def test_full_shape_groups_equal_separate_networks(oracle):
         pm.close()
     print("rates (Hz):", [round(r, 1) for r in rates])
     assert max(rates) > 50.0, "some group should ignite"
+
+
+def test_igniting_sweep_network_in_event_order(oracle):
+    """One network of the sweep's shape (config 4: AdEx RK4, in-degree 100,
+    delays 1-10 steps) whose E_leak (-45.5 mV, the sweep's +30% corner) lies
+    above threshold, so it ignites: run alone it delivers in the reference's
+    event-queue order (updates with more than 1,024 spikes split into groups
+    of 1,024), bit for bit with the oracle's event loop."""
+    model, integ, w_ext, steps, n = "AdExTimeDrivenModel_GPU", "RK4", 0.7, 400, 20000
+    p = ADEX_DEFAULTS.copy()
+    p[8] = np.float32(p[8] * 0.7)
+    print(f"case igniting sweep network n={n} event order", flush=True)
+    pm, pnet = _plain(model, integ, p, n, 3, w_ext, in_deg=100, max_d=10, event_order=True)
+    pnet.run(steps)
+    pst, pix = pnet.spikes()
+    pop = oracle.Population(model, integ, p, n, mask=3)
+    onet = oracle.Net(pop, 10)
+    onet.add_self(*pnet.synapses(0))
+    onet.set_external(binomial_thresholds(100, 5.0 * 1e-4), w_ext, EXT_SEED + 3)
+    ost, oix = onet.run(1, steps)
+    assert np.array_equal(ost, pst) and np.array_equal(oix, pix), "raster"
+    for v in range(pop.nv - 2):
+        assert np.array_equal(bits(pop.state(v)), bits(pm.GetVectorNeuronState(v))), f"variable {v}"
+    rate = len(pst) / (n * steps * 1e-4)
+    burst = int(np.bincount(pst).max())
+    print(f"rate {rate:.0f} Hz, largest update {burst} spikes")
+    assert rate > 100.0, rate
+    pnet.close()
+    pm.close()
 
 
 def test_generated_wiring_properties():
