@@ -157,13 +157,15 @@ def test_gpu_gradient_matches_finite_differences(oracle, model):
     """The GPU adjoint's directional derivative of the trace loss equals a
     central finite difference of the GPU forward itself, for each parameter
     group: perturbations of 1e-4 relative, restricted to neurons whose spike
-    trains the perturbation leaves unchanged (a spike-stable window).  LIF
-    agrees to 2e-2 over 2,000 steps.  AdEx's exponential upswing amplifies
-    the forward's float rounding into the difference quotient (the
-    conditioning measured against the FP64 adjoint above), so it is checked
-    over 1,000 steps to 5e-2."""
+    trains the perturbation leaves unchanged (a spike-stable window), 2,000
+    steps.  LIF agrees to 2e-2 (measured: within 6e-3).  AdEx's map is
+    ill-conditioned on the exponential upswing before each spike (rounding
+    of the trajectory alone moves the exact adjoint by ~20%, see above), and
+    its difference quotients carry that: they agree to 1e-1 (measured: 2-7%,
+    always larger in magnitude than the adjoint)."""
     n = 256
-    steps, tol = (2000, 2e-2) if model == "LIFTimeDrivenModel_GPU" else (1000, 5e-2)
+    steps = 2000
+    tol = 2e-2 if model == "LIFTimeDrivenModel_GPU" else 1e-1
     print(f"case finite differences {model}", flush=True)
     spec, prob, o = _setup(oracle, model, n, steps)
     x = _dev(spec.start)
