@@ -479,11 +479,6 @@ struct edlut_network {
     unsigned long long *clk = nullptr;
     int64_t clk_cap = 0, clk_tiles_max = 0, clk_first = 0, clk_count = 0, clk_blocks = 0;
     unsigned long long *clk_next = nullptr;   // per timed step: the next kernel's first start
-    // plain networks: the event-queue replay of window k - 1 runs in the
-    // first workgroup of update kernel k (eq_done: the update whose ranks
-    // are written); the run's last window is replayed after it
-    bool eq_upd = false;
-    int32_t *eq_done = nullptr;
     int64_t fold_ready = -1;         // update whose events are already folded
     bool dirty = true;
     bool use_graph = true;
@@ -1138,43 +1133,6 @@ struct BdfTable {
     BdfTable() { fill_bdf<M, T>(fn, std::make_integer_sequence<unsigned, 16>{}); }
 };
 
-// Update kernels with the event-queue replay in workgroup 0: AdEx and LIF,
-// Euler/RK2/RK4, spike synapses (mask 0..3), with and without the drive,
-// two or four neurons per lane.
-template <class M, int I, unsigned MASK>
-void fill_eq_one(UpdateFn *t) {
-    t[((I * 4 + (int)MASK) * 2 + 0) * 2 + 0] = k_update<M, I, MASK, false, 2, false, true>;
-    t[((I * 4 + (int)MASK) * 2 + 1) * 2 + 0] = k_update<M, I, MASK, true, 2, false, true>;
-    t[((I * 4 + (int)MASK) * 2 + 0) * 2 + 1] = k_update<M, I, MASK, false, 4, false, true>;
-    t[((I * 4 + (int)MASK) * 2 + 1) * 2 + 1] = k_update<M, I, MASK, true, 4, false, true>;
-}
-template <class M, int I, unsigned... MS>
-void fill_eq(UpdateFn *t, std::integer_sequence<unsigned, MS...>) {
-    (fill_eq_one<M, I, MS>(t), ...);
-}
-template <class M>
-struct EqUpdTable {
-    UpdateFn fn[3 * 4 * 2 * 2];
-    EqUpdTable() {
-        fill_eq<M, kEuler>(fn, std::make_integer_sequence<unsigned, 4>{});
-        fill_eq<M, kRK2>(fn, std::make_integer_sequence<unsigned, 4>{});
-        fill_eq<M, kRK4>(fn, std::make_integer_sequence<unsigned, 4>{});
-    }
-};
-
-// The update kernel with the replay in workgroup 0, or null when no such
-// kernel covers the population (the replay then runs after the step).
-UpdateFn pick_update_eq(const edlut_population *p, bool ext) {
-    if (p->grouped || (p->mask & 12u) || p->integ > EDLUT_RK4) return nullptr;
-    if (p->model != EDLUT_ADEX && p->model != EDLUT_LIF) return nullptr;
-    const int it = p->items == 2 ? 0 : (p->items == 4 ? 1 : -1);
-    if (it < 0) return nullptr;
-    static EqUpdTable<AdExModel> adex_e;
-    static EqUpdTable<LIFModel> lif_e;
-    const int idx = ((p->integ * 4 + (int)(p->mask & 3u)) * 2 + (ext ? 1 : 0)) * 2 + it;
-    return (p->model == EDLUT_ADEX ? adex_e.fn : lif_e.fn)[idx];
-}
-
 // Update kernel of a population: null when no kernel covers the combination
 // (grouped populations with NMDA or external-current synapses).
 UpdateFn pick_update(const edlut_population *p, bool ext) {
@@ -1267,7 +1225,6 @@ int check_err(edlut_population *p) {
     if (e & 256u)
         return fail(EDLUT_E_ARG, "an input event in device memory names a target out of range or a synapse "
                                  "type that is not enabled (dropped)");
-    if (e & 512u) return fail(EDLUT_E_STATE, "event-queue ranks were not ready within the spin limit");
     return 0;
 }
 
@@ -2150,7 +2107,7 @@ int edlut_network_destroy(edlut_network *n) {
     if (n->graph) (void)hipGraphExecDestroy(n->graph);
     void *dev[] = {n->own_ring_idx, n->own_ring_cnt, n->tile_count, n->staging, n->log_step,
                    n->log_idx, n->log_count, n->d_step, n->d_time, n->d_time_p, n->d_rtab, n->d_skip, n->d_delay_s,
-                   n->seg_off, n->clk, n->clk_next, n->eq_done};
+                   n->seg_off, n->clk, n->clk_next};
     for (void *q : dev)
         if (q) (void)hipFree(q);
     delete n->eq;
@@ -2546,22 +2503,13 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     a.out = TileOut{n->tile_count, n->staging};
     a.fold = ft;
     const int tiles = p->update_tiles();
-    const bool eq_upd = eq && n->eq_upd && eq == n->eq && !n->circuit;
-    if (eq_upd) {
-        // workgroup 0 replays the previous update's window from the ring lists
-        a.eq = eq_args(eq, StepRef{step.ptr, step.imm - 1}, 0, 0.0);
-        a.eq.no_tiles = 1;
-        a.eq.pf_tiles = nullptr;
-        a.eq.pf_em = -1;
-        a.eq_done = n->eq_done;
-    }
     if (n->clk) {
         // timed run: every workgroup stamps its start and end
         a.clk = n->clk;
         a.clk_first = (int32_t)n->clk_first;
         a.clk_count = (int32_t)n->clk_count;
     }
-    hipLaunchKernelGGL(fn, dim3(tiles + (eq_upd ? 1 : 0)), dim3(kBlock), update_lds_bytes(p), p->stream, a);
+    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
     CompactArgs c = compact_args(n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
                                  n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
                                  n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);
@@ -2576,7 +2524,7 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
     *fused = ring && !n->plastic;
     // the event-queue replay of this update's window runs in the compaction
     // kernel's last workgroup when its sizes fit there
-    const bool eq_fused = *fused && eq && !eq_upd && eq->max_delay <= kEqFusedDelays;
+    const bool eq_fused = *fused && eq && eq->max_delay <= kEqFusedDelays;
     if (*fused) {
         RingEvents rn = re0;
         rn.step = StepRef{step.ptr, step.imm + 1};
@@ -2610,8 +2558,7 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         pa.exp_lut = p->exp_lut;
         hipLaunchKernelGGL(k_stdp_post<64>, dim3(64), dim3(kBlock), 0, p->stream, pa);
     }
-    if (eq && !eq_fused && !eq_upd)
-        hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, p->stream, eq_args(eq, step, 0, 0.0));
+    if (eq && !eq_fused) hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, p->stream, eq_args(eq, step, 0, 0.0));
     return 0;
 }
 
@@ -2734,27 +2681,6 @@ static int build_segments(edlut_network *n) {
 // Everything a run needs before its first step: stale folds dropped, rings
 // cleared when the run does not continue the previous one, segments built,
 // the timing window set.  Returns the update kernel through *fn.
-// The replay moves into the update kernel where the step fuses the fold into
-// the compaction (plain network, not plastic, delays within the fused
-// replay's) and an update kernel with the replay exists.
-static int eq_update_mode(edlut_network *n, UpdateFn *fn) {
-    edlut_population *p = n->pop;
-    n->eq_upd = false;
-    EqSim *eq = net_eq(n);
-    if (!eq || n->sources.empty() || n->plastic || eq->max_delay > kEqFusedDelays) return 0;
-    UpdateFn fe = pick_update_eq(p, n->has_ext);
-    if (!fe) return 0;
-    if (!n->eq_done) {
-        int rc = dalloc(&n->eq_done, 1);
-        if (rc) return rc;
-    }
-    HIP_TRY(hipMemsetAsync(n->eq_done, 0, sizeof(int32_t), p->stream));
-    *fn = fe;
-    n->eq_upd = true;
-    if (n->clk) n->clk_blocks = p->update_tiles() + 1;
-    return 0;
-}
-
 static int run_prepare(edlut_network *n, int64_t first_step, int n_steps, UpdateFn *fn) {
     edlut_population *p = n->pop;
     if (!p->q_tgt.empty()) return fail(EDLUT_E_STATE, "queued host inputs are not used in network mode");
@@ -2800,7 +2726,6 @@ static int run_prepare(edlut_network *n, int64_t first_step, int n_steps, Update
     }
     *fn = pick_update(p, n->has_ext);
     if (!*fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
-    n->eq_upd = false;   // decided by edlut_network_run once the queue is built
     if (n->seg_dirty) {
         int rc = build_segments(n);
         if (rc) return rc;
@@ -2894,7 +2819,6 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     if (n->eq_on && !p->grouped) {
         if ((rc = eq_prepare_net(n, first_step)) || (rc = eq_resume(n->eq, first_step, n_steps, stop, p->stream)))
             return rc;
-        if ((rc = eq_update_mode(n, &fn))) return rc;
     }
     const RingEvents re = ring_events(n);
     int64_t k = first_step;
@@ -2920,12 +2844,6 @@ int edlut_network_run(edlut_network *n, int64_t first_step, int n_steps) {
     }
     HIP_TRY(hipGetLastError());
     run_finish(n, first_step, n_steps);
-    if (n->eq_upd && n_steps > 0) {
-        // the run's last window (the next run's first update would find it done)
-        hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, p->stream,
-                           eq_args(n->eq, StepRef{nullptr, (int32_t)(first_step + n_steps - 1)}, 0, 0.0));
-        HIP_TRY(hipGetLastError());
-    }
     if (n->eq_on && !p->grouped && stop >= 0) {
         uint32_t r_stop = 0;
         if ((rc = stop_rank(n->eq, p->stream, &r_stop))) return rc;
@@ -3304,7 +3222,7 @@ int edlut_network_set_timing(edlut_network *n, int64_t max_steps) {
     n->clk_cap = 0;
     n->dirty = true;
     if (max_steps == 0) return 0;
-    n->clk_tiles_max = n->pop->stride / kFlagTile + 1;   // the most workgroups of any launch shape
+    n->clk_tiles_max = n->pop->stride / kFlagTile;   // the most workgroups of any launch shape
     int rc;
     if ((rc = dalloc(&n->clk, 2 * (size_t)max_steps * (size_t)n->clk_tiles_max)) ||
         (rc = dalloc(&n->clk_next, (size_t)max_steps)))

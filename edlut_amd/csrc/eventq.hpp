@@ -162,7 +162,6 @@ struct EqArgs {
     StepRef step;              // update k just run (resuming call: the update before the run)
     const int32_t *pf_tiles;   // tile counts of model pf_em, loaded at the start (null: none)
     int pf_em;
-    int no_tiles;              // count from the ring lists only (the tiles are being rewritten)
     uint32_t *err;             // bit 6: heap capacity, bit 7: event of an unknown kind
 };
 
@@ -537,11 +536,10 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
         const int32_t ke = s_next[e];
         if (ke > k || ke < 1) continue;
         const int slot = ke % a.ring;
-        const bool from_tiles = E.tile_count && ke == k && !a.no_tiles;
         if (E.all_full) {
             // every spike joins a group of every delay: only the spike count
             int32_t tot = 0;
-            if (from_tiles) {
+            if (E.tile_count && ke == k) {
                 for (int32_t t0 = 0; t0 < E.n_tiles; t0 += 8 * kBlock) {
                     const bool pre = t0 == 0 && e == a.pf_em && a.pf_tiles == E.tile_count;
 #pragma unroll
@@ -581,7 +579,7 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
             }
             if (part) s_partial[e] = 1u;
         };
-        if (from_tiles) {
+        if (E.tile_count && ke == k) {
             // tile counts eight at a time (the array is padded to a multiple
             // of 8 * kBlock entries), so their loads are in flight together
             for (int32_t t0 = 0; t0 < E.n_tiles; t0 += 8 * kBlock) {
@@ -634,7 +632,7 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
         const int slot = ke % a.ring;
         if (threadIdx.x == 0) eq_g(E.bnd_on)[slot] = (uint8_t)s_partial[e];
         if (!s_partial[e]) continue;
-        const bool tiles = E.tile_count && ke == k && !a.no_tiles;
+        const bool tiles = E.tile_count && ke == k;
         const int32_t ns = tiles ? E.n_tiles : eq_g(E.ring_cnt)[slot];   // list entries, or tiles
         for (int d = 1; d <= a.max_delay; d++) {
             const uint32_t full_c = s_cnt[e][d - 1] / (uint32_t)kEqChunk;
@@ -746,12 +744,9 @@ __device__ __forceinline__ void eq_body(const EqArgs &a) {
 // The replay as a kernel of its own (circuits, plastic networks, resuming calls).
 __global__ __launch_bounds__(kBlock) void k_eventq(EqArgs a) { eq_body<2048, kMaxDelayEq>(a); }
 
-// Sizes of the replay fused into a network's compaction kernel, and of the
-// replay in the first workgroup of the update kernel (its LDS is allocated
-// to every workgroup of that launch, so its heap stage is smaller).
+// Sizes of the replay fused into a network's compaction kernel.
 constexpr int kEqFusedItems = 1024;
 constexpr int kEqFusedDelays = 64;
-constexpr int kEqUpdItems = 512;
 
 }  // namespace edl
 

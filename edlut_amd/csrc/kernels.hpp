@@ -266,11 +266,6 @@ struct UpdateArgs {
     unsigned long long *clk;
     int32_t clk_first;
     int32_t clk_count;
-    // update kernels with the event-queue replay in workgroup 0 (EQ): the
-    // replay of the previous update's window, and the word it sets to this
-    // update's index once the window's ranks are written
-    EqArgs eq;
-    int32_t *eq_done;
 };
 
 __device__ __forceinline__ unsigned long long *clock_slot(const UpdateArgs &a, int32_t step) {
@@ -312,47 +307,14 @@ __device__ __forceinline__ void clear_tile_flag(uint32_t *flags, int tile) {
 // integrated stage-major, and the stores come last, so every thread keeps
 // ITEMS independent dependency chains in flight.  GROUPED takes the model
 // parameters, the live range and the drive seed from the wave's group.
-// Waits until the event-queue replay in workgroup 0 has written the ranks
-// this update sorts by (bounded: error bit 9 past the spin limit).
-__device__ inline void eq_wait_ranks(const int32_t *done, int32_t step, uint32_t *err) {
-    uint32_t spins = 0;
-    while (__hip_atomic_load(done, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != step) {
-        if (++spins > (1u << 22)) {
-            atomicOr(err, 512u);
-            return;
-        }
-        __builtin_amdgcn_s_sleep(2);
-    }
-}
-
-// EQ: workgroup 0 replays the event queue's window of the previous update
-// (from the ring lists: the tiles are being rewritten) while the others
-// integrate; a target with two or more events this update waits for the
-// window's ranks before sorting them.  Workgroup 0 is dispatched first and
-// waits on nothing, so every wait ends.
-template <class M, int INTEG, unsigned MASK, bool EXT, int ITEMS, bool GROUPED, bool EQ = false>
+template <class M, int INTEG, unsigned MASK, bool EXT, int ITEMS, bool GROUPED>
 __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     constexpr int T = kBlock * ITEMS;
     static_assert(T % kFlagTile == 0, "update tiles cover whole fold flags");
     static_assert(kGroupAlign % (ITEMS * kWave) == 0, "a wave's span lies in one group");
+    const int tile = blockIdx.x;
     const int32_t step = a.step.get();
     clock_begin(a, step);
-    if constexpr (EQ) {
-        if (blockIdx.x == 0) {
-            // the replay's chain of dependent steps is the step's longest:
-            // its waves take issue priority over the update waves on their SIMDs
-            __builtin_amdgcn_s_setprio(3);
-            eq_body<kEqUpdItems, kEqFusedDelays>(a.eq);
-            __syncthreads();
-            if (threadIdx.x == 0) {
-                __threadfence();
-                __hip_atomic_store(a.eq_done, step, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            clock_finish(a, step);
-            return;
-        }
-    }
-    const int tile = EQ ? (int)blockIdx.x - 1 : (int)blockIdx.x;
     const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
     __shared__ uint32_t s_flag;
     // the tile's fold flag is read alongside the state loads (the barrier
@@ -410,7 +372,6 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
     __syncthreads();
     if (s_flag) {
         // events of keys with several inputs this step, in delivery order
-        bool ranks_ready = !EQ;
 #pragma unroll
         for (int n = 0; n < ITEMS; n++) {
             if (!live[n]) continue;
@@ -420,10 +381,6 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
                 const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)idx[n];
                 if (a.fold.seg_off) {
                     const uint32_t m = a.fold.fill[key];
-                    if (m >= 2u && !ranks_ready) {
-                        eq_wait_ranks(a.eq_done, step, a.fold.err);
-                        ranks_ready = true;
-                    }
                     if (m >= 1u)
                         s[n][M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
                                                         a.fold.rtab, step & 1, key, m, s[n][M::EXC + ty]);
