@@ -844,6 +844,10 @@ struct BdfState {
     uint32_t *ws_count;
     int ws_cap;
     uint32_t *err;
+    // granule: spike bits of the launch, one word per 32 neurons, and the
+    // count of finished workgroups (both zero between launches)
+    uint32_t *spk_bits;
+    uint32_t *done;
 };
 
 template <class M, unsigned MASK, typename T>
@@ -983,6 +987,14 @@ __device__ __forceinline__ bool invert_in_place(float *A) {
     return ok && fin;
 }
 
+template <int N>
+__device__ __forceinline__ bool any_nan(const float *x) {
+    bool r = false;
+#pragma unroll
+    for (int q = 0; q < N; q++) r = r || x[q] != x[q];
+    return r;
+}
+
 // Floats of one working-storage slot: la, li (15 x 15 each), X (19), Fp, Fn,
 // Cc, P1 (15 each).
 constexpr int kGrWsFloats = 2 * 225 + 19 + 4 * 15;
@@ -1048,17 +1060,22 @@ __device__ __forceinline__ void granule_newton_reference(const ModelParams &mp, 
     }
 }
 
+// The granule BDF step in three parts, so that the update kernel can hand a
+// lane its next neuron as soon as that lane's corrector has converged:
+// granule_bdf_predict (the order-st predictor), granule_bdf_newton (one
+// corrector iteration, repeated while epsi > 1e-6 and k < 10) and
+// granule_bdf_commit (history shift and new state).  Run in that order they
+// are the step of integrate_bdf above, BDFn.h:288-423.
 template <unsigned MASK>
-__device__ __forceinline__ void integrate_bdf_granule(const ModelParams &mp, const RhsCtx &c, float dt,
-                                                      const BdfState &b, int par, int64_t ld, uint32_t i,
-                                                      int &st, float *S) {
+__device__ __forceinline__ void granule_bdf_predict(const ModelParams &mp, const RhsCtx &c, float dt,
+                                                    const BdfState &b, int64_t ld, uint32_t i, int st,
+                                                    const float *S, float *P) {
     using G = GranuleModel;
     constexpr int ND = G::ND, NV = G::NV;
-    float *Dh = (float *)b.d;
-    float *Ph = (float *)b.prev;
-    float F[ND], P[NV], P1[ND], Cc[ND], A[ND * ND];
+    const float *Dh = (const float *)b.d;
     if (st == 0) {
-        G::rhs<MASK>(mp, c, S, F);
+        float F[ND];
+        G::rhs<MASK>(mp, c, (float *)S, F);
 #pragma unroll
         for (int j = 0; j < ND; j++) P[j] = S[j] + dt * F[j];
     } else {
@@ -1071,13 +1088,21 @@ __device__ __forceinline__ void integrate_bdf_granule(const ModelParams &mp, con
 #pragma unroll
     for (int j = ND; j < NV; j++) P[j] = S[j];
     G::decay<MASK>(mp, P);
-    float epsi = 1.0f;
-    int k = 0;
+}
+
+template <unsigned MASK>
+__device__ __forceinline__ void granule_bdf_newton(const ModelParams &mp, const RhsCtx &c, float dt,
+                                                   const BdfState &b, int par, int64_t ld, uint32_t i, int st,
+                                                   const float *S, float *P, float &epsi, int &k) {
+    using G = GranuleModel;
+    constexpr int ND = G::ND, NV = G::NV;
+    const float *Ph = (const float *)b.prev;
+    float F[ND], P1[ND], Cc[ND], A[ND * ND];
     const float *Cs = b.C + st * 7;
     const float c0 = Cs[0], c0dt = c0 * dt;
     const float eps = dt * 0.1f;
     const float inv_eps = 1.0f / eps;
-    while (epsi > 1e-6 && k < 10) {
+    {
         G::Terms t;
         G::prep(mp, c, P, t);
         G::rhs_terms(mp, c, P, t, F);
@@ -1134,6 +1159,17 @@ __device__ __forceinline__ void integrate_bdf_granule(const ModelParams &mp, con
                 for (int q = 0; q < ND; q++) aux += A[z * ND + q] * (P[q] - Cc[q]);
                 P1[z] = aux + P[z];
             }
+        } else if (any_nan<ND>(P)) {
+            // An iterate with a NaN component corrects to all NaN whatever
+            // the Jacobian: every P1[z] = sum_t iJ[z][t] * (P[t] - Cc[t]) +
+            // P[z] takes the NaN term (and the max-norm change below is then
+            // 0: no comparison with a NaN holds).  This is the standing case
+            // of a neuron whose step once went NaN: the NaN reset restores
+            // the state but not the BDF history (TimeDrivenNeuronModel.cpp:
+            // 75-88), so every later predictor is NaN.  Only the NaN payload
+            // can differ from the full correction's.
+#pragma unroll
+            for (int q = 0; q < ND; q++) P1[q] = __builtin_nanf("");
         } else {
             const uint32_t slot = atomicAdd(&b.ws_count[par], 1u);
             if (slot < (uint32_t)b.ws_cap) {
@@ -1163,6 +1199,15 @@ __device__ __forceinline__ void integrate_bdf_granule(const ModelParams &mp, con
         epsi = aux2;
         k++;
     }
+}
+
+template <unsigned MASK>
+__device__ __forceinline__ void granule_bdf_commit(const ModelParams &mp, const BdfState &b, int64_t ld,
+                                                   uint32_t i, int &st, float *S, const float *P) {
+    using G = GranuleModel;
+    constexpr int ND = G::ND;
+    float *Dh = (float *)b.d;
+    float *Ph = (float *)b.prev;
     if (st < b.order) st++;
 #pragma unroll
     for (int j = 0; j < ND; j++) {

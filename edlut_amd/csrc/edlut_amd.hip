@@ -297,6 +297,9 @@ struct edlut_population {
     int refr_len = 0;
     float *bdf_ws = nullptr;          // granule BDF: fallback working storage
     uint32_t *bdf_ws_count = nullptr; // [2] slots handed out, by step parity
+    uint32_t *bdf_bits = nullptr;     // granule BDF: spike bits of a launch [stride / 32]
+    uint32_t *bdf_done = nullptr;     // granule BDF: finished workgroups of a launch
+    int n_cu = 0;                     // compute units of the device
     uint64_t lut_sums[3] = {0, 0, 0};
 
     // spikes of the last update (tile counts + staging, then compacted)
@@ -335,6 +338,13 @@ struct edlut_population {
     int items = kItems;              // neurons per thread of the update launch
 
     int update_tiles() const { return (int)(stride / ((int64_t)kBlock * items)); }
+    // workgroups of the update launch: one per tile, except the granule's
+    // BDF kernel, whose waves take neurons from equal ranges (at most one
+    // workgroup per CU fits)
+    int update_blocks() const {
+        const int t = update_tiles();
+        return (model == EDLUT_GRANULE && integ == EDLUT_BDF && n_cu > 0 && n_cu < t) ? n_cu : t;
+    }
 
     FoldTarget fold_target(int par, bool by_step = false) const {
         FoldTarget t;
@@ -1096,8 +1106,13 @@ struct MaskFreeTable {
         fn[3] = k_update<M, kRK2, 15u, true, ITEMS, false>;
         fn[4] = k_update<M, kRK4, 15u, false, ITEMS, false>;
         fn[5] = k_update<M, kRK4, 15u, true, ITEMS, false>;
-        fn[6] = k_update_bdf<M, 15u, false, float, ITEMS>;
-        fn[7] = k_update_bdf<M, 15u, true, float, ITEMS>;
+        if constexpr (std::is_same<M, GranuleModel>::value) {
+            fn[6] = k_update_granule_bdf<false>;
+            fn[7] = k_update_granule_bdf<true>;
+        } else {
+            fn[6] = k_update_bdf<M, 15u, false, float, ITEMS>;
+            fn[7] = k_update_bdf<M, 15u, true, float, ITEMS>;
+        }
         fn[8] = k_update<M, kBifEuler, 15u, false, ITEMS, false>;
         fn[9] = k_update<M, kBifEuler, 15u, true, ITEMS, false>;
         fn[10] = k_update<M, kBifRK2, 15u, false, ITEMS, false>;
@@ -1234,7 +1249,7 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.state = p->state;
     a.stride = p->stride;
     a.n = p->n;
-    a.n_tiles = p->n_tiles;
+    a.n_tiles = p->update_tiles();
     a.step = StepRef{nullptr, step};
     a.mp = p->mp;
     // the granule and Purkinje kernels read the enabled synapse types at run
@@ -1258,6 +1273,8 @@ UpdateArgs make_update_args(edlut_population *p, int32_t step) {
     a.bdf.ws_count = p->bdf_ws_count;
     a.bdf.ws_cap = p->bdf_ws ? kBdfWsSlots : 0;
     a.bdf.err = p->err;
+    a.bdf.spk_bits = p->bdf_bits;
+    a.bdf.done = p->bdf_done;
     a.grp.tab = p->d_groups;
     a.grp.chunk_group = p->d_chunk_group;
     return a;
@@ -1691,9 +1708,13 @@ int create_population(int model, int integrator, int bdf_order, const float *par
             hipMemset(p->bdf_st, 0, sizeof(int32_t) * p->stride) != hipSuccess)
             return cleanup(fail(EDLUT_E_HIP, "BDF initialisation failed"));
         if (model == EDLUT_GRANULE) {
-            if ((rc = dalloc(&p->bdf_ws, (size_t)kBdfWsSlots * kGrWsFloats)) || (rc = dalloc(&p->bdf_ws_count, 2)))
+            if ((rc = dalloc(&p->bdf_ws, (size_t)kBdfWsSlots * kGrWsFloats)) || (rc = dalloc(&p->bdf_ws_count, 2)) ||
+                (rc = dalloc(&p->bdf_bits, (size_t)(p->stride / 32))) || (rc = dalloc(&p->bdf_done, 1)))
                 return cleanup(rc);
-            if (hipMemset(p->bdf_ws_count, 0, 2 * sizeof(uint32_t)) != hipSuccess)
+            if (hipMemset(p->bdf_ws_count, 0, 2 * sizeof(uint32_t)) != hipSuccess ||
+                hipMemset(p->bdf_bits, 0, sizeof(uint32_t) * (size_t)(p->stride / 32)) != hipSuccess ||
+                hipMemset(p->bdf_done, 0, sizeof(uint32_t)) != hipSuccess ||
+                hipDeviceGetAttribute(&p->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
                 return cleanup(fail(EDLUT_E_HIP, "BDF initialisation failed"));
         }
     }
@@ -1788,6 +1809,7 @@ int edlut_population_destroy(edlut_population *p) {
     (void)hipSetDevice(p->device);
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     void *dev[] = {p->chan, p->bdf_c, p->bdf_d, p->bdf_prev, p->bdf_st, p->bdf_ws, p->bdf_ws_count,
+                   p->bdf_bits, p->bdf_done,
                    p->state, p->last_spike, p->exp_lut, p->nmda_lut, p->refr, p->tile_count,
                    p->staging, p->spk_idx, p->spk_cnt, p->err, p->cnt, p->fill, p->seg,
                    p->tile_flag, p->bump, p->b_ord, p->b_w, p->e_tgt,
@@ -1883,7 +1905,7 @@ int edlut_population_update(edlut_population *p, int64_t step, int *n_spikes) {
     if (!fn) return fail(EDLUT_E_ARG, "no update kernel for this model, integrator and synapse set");
     p->started = true;
     const int tiles = p->update_tiles();
-    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
+    hipLaunchKernelGGL(fn, dim3(p->update_blocks()), dim3(kBlock), update_lds_bytes(p), p->stream, a);
     launch_compact(p->stream, p->tile_count, p->staging, tiles, kBlock * p->items, p->spk_idx, p->spk_cnt,
                    1, 0, nullptr, nullptr, nullptr, 0, p->err, StepRef{nullptr, (int32_t)step});
     HIP_TRY(hipGetLastError());
@@ -1978,7 +2000,7 @@ int edlut_population_update_noalloc(edlut_population *p, int64_t step, const int
     a.fold = t;
     p->started = true;
     const int tiles = p->update_tiles();
-    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), st, a);
+    hipLaunchKernelGGL(fn, dim3(p->update_blocks()), dim3(kBlock), update_lds_bytes(p), st, a);
     launch_compact(st, p->tile_count, p->staging, tiles, kBlock * p->items, spk_idx, spk_cnt, 1, 0, nullptr,
                    nullptr, nullptr, 0, p->err, StepRef{nullptr, (int32_t)step});
     HIP_TRY(hipGetLastError());
@@ -2509,7 +2531,7 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         a.clk_first = (int32_t)n->clk_first;
         a.clk_count = (int32_t)n->clk_count;
     }
-    hipLaunchKernelGGL(fn, dim3(tiles), dim3(kBlock), update_lds_bytes(p), p->stream, a);
+    hipLaunchKernelGGL(fn, dim3(p->update_blocks()), dim3(kBlock), update_lds_bytes(p), p->stream, a);
     CompactArgs c = compact_args(n->tile_count, n->staging, tiles, kBlock * p->items, n->own_ring_idx,
                                  n->own_ring_cnt, n->ring, p->n, n->log_cap ? n->log_step : nullptr,
                                  n->log_cap ? n->log_idx : nullptr, n->log_count, n->log_cap, p->err, step);
@@ -2738,7 +2760,7 @@ static int run_prepare(edlut_network *n, int64_t first_step, int n_steps, Update
     if (n->clk) {
         n->clk_first = first_step;
         n->clk_count = n_steps < n->clk_cap ? n_steps : n->clk_cap;
-        n->clk_blocks = p->update_tiles();
+        n->clk_blocks = p->update_blocks();
         n->dirty = true;   // captured kernel arguments carry the timing window
         HIP_TRY(hipMemsetAsync(n->clk, 0, sizeof(unsigned long long) * 2 * n->clk_cap * n->clk_tiles_max,
                                p->stream));

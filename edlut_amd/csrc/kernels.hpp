@@ -483,6 +483,63 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
 }
 
 
+// Loads the state of neuron i at the start of a BDF step into sf: the row
+// values, the ordered fold of its multi-event keys when its tile's fold flag
+// is set, and the external drive.
+template <class M, unsigned MASK, bool EXT>
+__device__ __forceinline__ void bdf_load(const UpdateArgs &a, float *__restrict__ const *row, int32_t step,
+                                         uint32_t step_key, uint32_t flag, uint32_t i, float *sf) {
+    const int64_t ld = a.stride;
+#pragma unroll
+    for (int j = 0; j < M::ND; j++) sf[j] = row[j][i];
+    sf[M::EXC] = (MASK & 1u) ? row[M::EXC][i] : 0.0f;
+    sf[M::INH] = (MASK & 2u) ? row[M::INH][i] : 0.0f;
+    sf[M::NMDA] = (MASK & 4u) ? row[M::NMDA][i] : 0.0f;
+    sf[M::EXT] = (MASK & 8u) ? row[M::EXT][i] : 0.0f;
+    if (flag) {
+#pragma unroll
+        for (int ty = 0; ty < 3; ty++) {
+            if (!(MASK & (1u << ty))) continue;
+            const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)i;
+            if (a.fold.seg_off) {
+                const uint32_t m = a.fold.fill[key];
+                if (m >= 1u)
+                    sf[M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
+                                                  a.fold.rtab, step & 1, key, m, sf[M::EXC + ty]);
+                continue;
+            }
+            const uint32_t m = a.fold.cnt[key];
+            if (m >= 2u)
+                sf[M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ord,
+                                               a.fold.b_w, a.fold.b_cap, a.fold.err, key, m,
+                                               sf[M::EXC + ty]);
+        }
+    }
+    if (EXT) {
+        const uint32_t u = drive_hash_keyed(step_key, i);
+        int cnt = 0;
+        while (cnt < a.ext.n_thr && u < a.ext.thr[cnt]) cnt++;
+        for (int q = 0; q < cnt; q++) sf[M::EXC] += a.ext.w;
+    }
+}
+
+// Stores neuron i's state after the step (TimeDrivenNeuronModel.cpp:75-88:
+// a NaN membrane potential puts the whole state back to its initial values).
+template <class M, unsigned MASK>
+__device__ __forceinline__ void bdf_store(const UpdateArgs &a, float *__restrict__ const *row, uint32_t i,
+                                          const float *out) {
+    if (out[0] != out[0]) {
+#pragma unroll
+        for (int j = 0; j < M::NV; j++) row[j][i] = M::init(a.mp, j);
+    } else {
+#pragma unroll
+        for (int j = 0; j < M::ND; j++) row[j][i] = out[j];
+        if (MASK & 1u) row[M::EXC][i] = out[M::EXC];
+        if (MASK & 2u) row[M::INH][i] = out[M::INH];
+        if (MASK & 4u) row[M::NMDA][i] = out[M::NMDA];
+    }
+}
+
 // BDF update: one neuron at a time per thread (data-dependent Newton
 // iterations), kItems neurons per thread in the same tile layout as k_update.
 // T = double runs the whole BDF step in double (FP64 Newton mode); the state
@@ -499,11 +556,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     if (threadIdx.x == 0) {
         s_flag = take_tile_flag<TILE>(a.fold.tile_flag, tile);
         if (s_flag) clear_tile_flag<TILE>(a.fold.tile_flag, tile);
-        if (tile == 0) {
-            a.fold.bump[par ^ 1] = 0ull;
-            // the next step hands out working-storage slots from the other parity
-            if (a.bdf.ws_count) a.bdf.ws_count[par ^ 1] = 0u;
-        }
+        if (tile == 0) a.fold.bump[par ^ 1] = 0ull;
     }
     __syncthreads();
     float *__restrict__ row[M::NV];
@@ -511,17 +564,6 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     for (int j = 0; j < M::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
     const int64_t ld = a.stride;
     const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, step) : 0u;
-    // The granule's channel table (104 KB) is staged in LDS: its workgroups
-    // run one per CU (the Newton step fills the register file), so the LDS
-    // is free, and every right-hand side gathers a 26-float row from it.
-    const float *chan = a.chan;
-    if constexpr (std::is_same<M, GranuleModel>::value) {
-        extern __shared__ float4 s_chan[];
-        const float4 *g4 = (const float4 *)a.chan;
-        for (int q = threadIdx.x; q < kGrTable * kGrVars / 4; q += kBlock) s_chan[q] = g4[q];
-        __syncthreads();
-        chan = (const float *)s_chan;
-    }
     bool spk[ITEMS];
     int idx[ITEMS];
     for (int n = 0; n < ITEMS; n++) {
@@ -530,53 +572,20 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         if (idx[n] >= a.n) continue;
         const uint32_t i = (uint32_t)idx[n];
         float sf[M::NV];
-#pragma unroll
-        for (int j = 0; j < M::ND; j++) sf[j] = row[j][i];
-        sf[M::EXC] = (MASK & 1u) ? row[M::EXC][i] : 0.0f;
-        sf[M::INH] = (MASK & 2u) ? row[M::INH][i] : 0.0f;
-        sf[M::NMDA] = (MASK & 4u) ? row[M::NMDA][i] : 0.0f;
-        sf[M::EXT] = (MASK & 8u) ? row[M::EXT][i] : 0.0f;
-        if (s_flag) {
-#pragma unroll
-            for (int ty = 0; ty < 3; ty++) {
-                if (!(MASK & (1u << ty))) continue;
-                const uint64_t key = (uint64_t)ty * (uint64_t)ld + (uint64_t)i;
-                if (a.fold.seg_off) {
-                    const uint32_t m = a.fold.fill[key];
-                    if (m >= 1u)
-                        sf[M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
-                                                      a.fold.rtab, step & 1, key, m, sf[M::EXC + ty]);
-                    continue;
-                }
-                const uint32_t m = a.fold.cnt[key];
-                if (m >= 2u)
-                    sf[M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ord,
-                                                   a.fold.b_w, a.fold.b_cap, a.fold.err, key, m,
-                                                   sf[M::EXC + ty]);
-            }
-        }
-        if (EXT) {
-            const uint32_t u = drive_hash_keyed(step_key, i);
-            int cnt = 0;
-            while (cnt < a.ext.n_thr && u < a.ext.thr[cnt]) cnt++;
-            for (int q = 0; q < cnt; q++) sf[M::EXC] += a.ext.w;
-        }
+        bdf_load<M, MASK, EXT>(a, row, step, step_key, s_flag, i, sf);
         T S[M::NV];
 #pragma unroll
         for (int j = 0; j < M::NV; j++) S[j] = sf[j];
         const T prev = S[0];
         RhsCtx c;
-        c.chan = chan;
+        c.chan = a.chan;
         c.exp_lut = a.exp_lut;
         c.nmda_lut = a.nmda_lut;
         c.refr = a.refr;
         c.refr_len = a.refr_len;
         c.since_spike = a.last_spike ? (step - 1 - a.last_spike[i]) : (1 << 30);
         int st = a.bdf.st[i];
-        if constexpr (std::is_same<M, GranuleModel>::value)
-            integrate_bdf_granule<MASK>(a.mp, c, a.k.dt, a.bdf, par, ld, i, st, S);
-        else
-            integrate_bdf<M, MASK, T>(a.mp, c, (T)a.k.dt, a.bdf, ld, i, st, S);
+        integrate_bdf<M, MASK, T>(a.mp, c, (T)a.k.dt, a.bdf, ld, i, st, S);
         const bool fired = M::spike(a.mp, prev, S);
         if constexpr (M::kClockOverride) M::clock_override(c, fired, S);
         if (fired && M::kResetsIntegrator) st = 0;
@@ -586,18 +595,135 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         float out[M::NV];
 #pragma unroll
         for (int j = 0; j < M::NV; j++) out[j] = (float)S[j];
-        if (out[0] != out[0]) {
-#pragma unroll
-            for (int j = 0; j < M::NV; j++) row[j][i] = M::init(a.mp, j);
-        } else {
-#pragma unroll
-            for (int j = 0; j < M::ND; j++) row[j][i] = out[j];
-            if (MASK & 1u) row[M::EXC][i] = out[M::EXC];
-            if (MASK & 2u) row[M::INH][i] = out[M::INH];
-            if (MASK & 4u) row[M::NMDA][i] = out[M::NMDA];
-        }
+        bdf_store<M, MASK>(a, row, i, out);
     }
     tile_spikes<ITEMS>(a.out, tile, spk, idx);
+    clock_finish(a, step);
+}
+
+// Granule BDF update (EgidioGranuleCell_TimeDriven with BDFn, the
+// reference's per-neuron step).  The corrector runs a data-dependent number
+// of Newton iterations per neuron (about 3 on average, but 10 for the few
+// percent whose iterate ends in a one-ulp cycle), so a wave that gave each
+// lane a fixed neuron would run each item for its slowest lane's count.
+// Here each wave owns a contiguous range of neurons and a lane takes the
+// range's next neuron as soon as its own corrector has ended; a neuron's
+// arithmetic is the same either way.  The grid is at most one workgroup per
+// CU (the Newton step fills the register file); the ranges are equal.
+//   * The channel table (104 KB) is staged in LDS: every right-hand side
+//     gathers a 26-float row from it.
+//   * Spikes set bits of a.bdf.spk_bits; the last workgroup to finish turns
+//     them into the per-tile spike lists (ascending, kFlagTile neurons per
+//     tile) that the compaction reads, clears the bits and the tiles' fold
+//     flags, and resets the workgroup count.
+//   * A lane reads its neuron's fold flag when it takes the neuron.
+constexpr int kGrBatch = 32;
+template <bool EXT>
+__global__ __launch_bounds__(kBlock) void k_update_granule_bdf(UpdateArgs a) {
+    using G = GranuleModel;
+    constexpr unsigned MASK = 15u;   // the synapse types are the run-time mask
+    constexpr int TILE = kFlagTile;
+    const int32_t step = a.step.get();
+    clock_begin(a, step);
+    const int par = a.fold.par_by_step ? (step & 1) : a.fold.par;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        a.fold.bump[par ^ 1] = 0ull;
+        // the next step hands out working-storage slots from the other parity
+        if (a.bdf.ws_count) a.bdf.ws_count[par ^ 1] = 0u;
+    }
+    float *__restrict__ row[G::NV];
+#pragma unroll
+    for (int j = 0; j < G::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
+    const int64_t ld = a.stride;
+    const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, step) : 0u;
+    extern __shared__ float4 s_chan[];
+    {
+        const float4 *g4 = (const float4 *)a.chan;
+        for (int q = threadIdx.x; q < kGrTable * kGrVars / 4; q += kBlock) s_chan[q] = g4[q];
+    }
+    __syncthreads();
+    RhsCtx c;
+    c.chan = (const float *)s_chan;
+    c.exp_lut = a.exp_lut;
+    c.nmda_lut = a.nmda_lut;
+    c.refr = a.refr;
+    c.refr_len = a.refr_len;
+    c.since_spike = 1 << 30;
+    const float dt = a.k.dt;
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerBlock + threadIdx.x / kWave));
+    const int64_t n_waves = (int64_t)gridDim.x * kWavesPerBlock;
+    int cursor = (int)((int64_t)a.n * wave / n_waves);
+    const int end = (int)((int64_t)a.n * (wave + 1) / n_waves);
+    int i = -1;          // neuron on this lane (-1: none)
+    int st = 0, k = 0;
+    float epsi = 1.0f, prev = 0.0f;
+    float S[G::NV], P[G::NV];
+    for (;;) {
+        bool busy = i >= 0 && epsi > 1e-6 && k < 10;   // the corrector's loop condition
+        const uint64_t run = __ballot(busy);
+        // lanes whose corrector has ended wait until kGrBatch lanes are free,
+        // then all commit and take new neurons together
+        if (__popcll(run) <= kWave - kGrBatch) {
+            if (i >= 0 && !busy) {
+                granule_bdf_commit<MASK>(a.mp, a.bdf, ld, (uint32_t)i, st, S, P);
+                const bool fired = G::spike(a.mp, prev, S);
+                a.bdf.st[i] = st;
+                if (fired) {
+                    if (a.last_spike) a.last_spike[i] = step;
+                    atomicOr(&a.bdf.spk_bits[i >> 5], 1u << (i & 31));
+                }
+                bdf_store<G, MASK>(a, row, (uint32_t)i, S);
+                i = -1;
+            }
+            if (cursor < end) {
+                const uint64_t idle = __ballot(i < 0);
+                if (i < 0) {
+                    const int j = cursor + __popcll(idle & below);
+                    if (j < end) {
+                        i = j;
+                        const uint32_t flag = a.fold.tile_flag[j / TILE];
+                        bdf_load<G, MASK, EXT>(a, row, step, step_key, flag, (uint32_t)i, S);
+                        prev = S[0];
+                        st = a.bdf.st[i];
+                        if (a.last_spike) c.since_spike = step - 1 - a.last_spike[i];
+                        granule_bdf_predict<MASK>(a.mp, c, dt, a.bdf, ld, (uint32_t)i, st, S, P);
+                        epsi = 1.0f;
+                        k = 0;
+                        busy = true;
+                    }
+                }
+                cursor += __popcll(idle);
+            }
+        }
+        if (!__any(busy)) break;
+        if (busy) granule_bdf_newton<MASK>(a.mp, c, dt, a.bdf, par, ld, (uint32_t)i, st, S, P, epsi, k);
+    }
+    __shared__ uint32_t s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        s_last = atomicAdd(a.bdf.done, 1u) == gridDim.x - 1u;
+    }
+    __syncthreads();
+    if (s_last) {
+        __threadfence();
+        for (int t = threadIdx.x; t < a.n_tiles; t += kBlock) {
+            int32_t cnt = 0;
+            for (int w = 0; w < TILE / 32; w++) {
+                uint32_t bits = atomicExch(&a.bdf.spk_bits[t * (TILE / 32) + w], 0u);
+                while (bits) {
+                    const int b = __builtin_ctz(bits);
+                    bits &= bits - 1u;
+                    a.out.staging[(int64_t)t * TILE + cnt++] = t * TILE + w * 32 + b;
+                }
+            }
+            a.out.tile_count[t] = cnt;
+            clear_tile_flag<TILE>(a.fold.tile_flag, t);
+        }
+        if (threadIdx.x == 0) *a.bdf.done = 0u;
+    }
     clock_finish(a, step);
 }
 

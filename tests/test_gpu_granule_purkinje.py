@@ -22,7 +22,8 @@ GR = "EgidioGranuleCell_TimeDriven_GPU"
 PK = "TimeDrivenPurkinjeCell_GPU"
 
 
-def _run(oracle, model, integ, order, n, steps, dt, wmax, cur_scale, types=(0, 0, 0, 1, 2), rate=0.5):
+def _run(oracle, model, integ, order, n, steps, dt, wmax, cur_scale, types=(0, 0, 0, 1, 2), rate=0.5,
+         nan_at=None):
     print(f"case {model} {integ} order={order} dt={dt} n={n} steps={steps}", flush=True)
     rng = np.random.default_rng(zlib.crc32(f"{model}/{integ}/{order}/{dt}".encode()))
     m = E.CreateNeuronModel(model, {"int_meth": E.integration_method(integ, dt, order)})
@@ -38,7 +39,16 @@ def _run(oracle, model, integ, order, n, steps, dt, wmax, cur_scale, types=(0, 0
     m.ProcessInputCurrent(cur)
     o.set_ext_current(cur)
     counts = np.zeros(n, np.int64)
+    late = np.zeros(n, np.int64)   # spikes from the NaN update on
     for k in range(1, steps + 1):
+        if nan_at and k == nan_at[0]:   # {variable: neurons} set to NaN before update k
+            for var, idx in nan_at[1].items():
+                v = m.GetVectorNeuronState(var)
+                v[idx] = np.nan
+                m.SetVectorNeuronState(var, v)
+                ov = o.state(var)
+                ov[idx] = np.nan
+                o.set_state(var, ov)
         ne = int(rng.integers(0, int(n * rate) + 1))
         tgt = rng.integers(0, n, ne).astype(np.int32)
         tgt[: ne // 8] = 3
@@ -50,8 +60,12 @@ def _run(oracle, model, integ, order, n, steps, dt, wmax, cur_scale, types=(0, 0
         ref = o.update(k)
         assert np.array_equal(m.GetInternalSpikeIndexs(), ref), f"update {k}"
         counts[ref] += 1
+        if nan_at and k >= nan_at[0]:
+            late[ref] += 1
     for v in range(o.nv):
         assert np.array_equal(bits(m.GetVectorNeuronState(v)), bits(o.state(v))), f"var {v}"
+    if nan_at:
+        return counts, late, [m.GetVectorNeuronState(v) for v in range(o.nv)]
     return counts
 
 
@@ -59,6 +73,24 @@ def _run(oracle, model, integ, order, n, steps, dt, wmax, cur_scale, types=(0, 0
 def test_granule_bdf_bit_exact(oracle, order):
     counts = _run(oracle, GR, "BDF", order, 300, 600, 1e-4, wmax=1.0, cur_scale=10.0)
     assert counts.sum() > 50 and (counts > 1).sum() > 10
+
+
+def test_granule_bdf_standing_nan(oracle):
+    """A granule cell whose BDF step goes NaN is put back to its initial
+    state, but its BDF history keeps the NaN, so every later step is NaN
+    again and resets it again: the neuron stays at its initial state and
+    never fires.  NaN in V, in a gate and in the calcium concentration, set
+    before update 150 of 400."""
+    nan = {0: [3, 77], 5: [150], 14: [299]}
+    counts, late, state = _run(oracle, GR, "BDF", 2, 300, 400, 1e-4, wmax=1.0, cur_scale=10.0,
+                               nan_at=(150, nan))
+    stuck = [3, 77, 150, 299]
+    live = np.setdiff1d(np.arange(300), stuck)
+    assert late[live].sum() > 50 and late[stuck].sum() == 0
+    fresh = E.CreateNeuronModel(GR, {"int_meth": E.integration_method("BDF", 1e-4, 2)})
+    fresh.InitializeStates(1, 0, 0)
+    for v, x in enumerate(state):   # every step ends in the NaN reset
+        assert np.array_equal(bits(x[stuck]), np.repeat(bits(fresh.GetVectorNeuronState(v)), 4)), f"var {v}"
 
 
 @pytest.mark.parametrize("integ", ["Euler", "RK2", "RK4"])
