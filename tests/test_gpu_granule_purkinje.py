@@ -110,3 +110,44 @@ def test_granule_synapse_subsets(oracle):
     excitatory-only population equals the oracle's with the same mask."""
     counts = _run(oracle, GR, "BDF", 2, 200, 300, 1e-4, wmax=1.0, cur_scale=5.0, types=(0,))
     assert counts.sum() > 0
+
+
+@pytest.mark.parametrize("n,steps", [(1, 120), (70, 120), (140_000, 40)])
+def test_granule_bdf_lane_refill_shapes(oracle, n, steps):
+    """The granule BDF kernel splits the population into equal ranges, one
+    per wave of at most one workgroup per CU, and lanes take a range's
+    neurons as their correctors end.  One neuron (most waves empty), a
+    population inside one wave's reach, and 140,000 cells: more 512-neuron
+    tiles than CUs, so the ranges cross tiles and the last workgroup writes
+    the spike lists of all 274 tiles.  Spike lists of every update and the
+    final state against the oracle, under a drive that fires within tens of
+    updates."""
+    import os
+    print(f"case granule BDF2 n={n} steps={steps}", flush=True)
+    rng = np.random.default_rng(n)
+    m = E.CreateNeuronModel(GR, {"int_meth": E.integration_method("BDF", 1e-4, 2)})
+    assert m.CheckSynapseType(0)
+    m.InitializeStates(n, 0, 0)
+    o = oracle.Population(GR, "BDF", defaults(GR), n, dt_s=np.float32(1e-4), mask=1 | 8, bdf_order=2)
+    cur = rng.uniform(0.0, 1.0, n).astype(np.float32) * 200.0
+    m.ProcessInputCurrent(cur)
+    o.set_ext_current(cur)
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    total = 0
+    try:
+        for k in range(1, steps + 1):
+            ne = max(1, n // 4)
+            tgt = rng.integers(0, n, ne).astype(np.int32)
+            w = rng.uniform(0.0, 1.0, ne).astype(np.float32)
+            typ = np.zeros(ne, np.uint8)
+            m.ProcessInputSpikes(tgt, typ, w)
+            o.add_inputs(tgt, typ, w)
+            m.UpdateState(step=k)
+            ref = o.update(k)
+            assert np.array_equal(m.GetInternalSpikeIndexs(), ref), f"update {k}"
+            total += len(ref)
+    finally:
+        oracle.set_threads(1)
+    for v in range(o.nv):
+        assert np.array_equal(bits(m.GetVectorNeuronState(v)), bits(o.state(v))), f"var {v}"
+    assert total > 0
