@@ -434,6 +434,7 @@ struct Source {
     std::vector<float> w0;        // weights when STDP was switched on (reset)
     int64_t *in_ptr = nullptr;
     int64_t *in_syn = nullptr;
+    int64_t in_max = 0;           // longest per-target list
     int64_t nsyn = 0;
     int32_t *tile_count = nullptr;
     int32_t *staging = nullptr;
@@ -2574,11 +2575,13 @@ static int enqueue_step(edlut_network *n, StepRef step, UpdateFn fn, const RingE
         pa.step = step;
         pa.in_ptr = s.in_ptr;
         pa.in_syn = s.in_syn;
+        pa.chunks = (int)((s.in_max + kWave - 1) / kWave);
+        if (pa.chunks < 1) pa.chunks = 1;
         pa.w = s.w;
         pa.syn = s.syn;
         pa.time_ring = n->d_time;
         pa.exp_lut = p->exp_lut;
-        hipLaunchKernelGGL(k_stdp_post<64>, dim3(64), dim3(kBlock), 0, p->stream, pa);
+        hipLaunchKernelGGL(k_stdp_post, dim3(64), dim3(kBlock), 0, p->stream, pa);
     }
     if (eq && !eq_fused) hipLaunchKernelGGL(k_eventq, dim3(1), dim3(kBlock), 0, p->stream, eq_args(eq, step, 0, 0.0));
     return 0;
@@ -3593,7 +3596,11 @@ int edlut_network_set_stdp(edlut_network *n, int source, float max_ltp, float ta
     }
     std::vector<int64_t> ptr((size_t)p->n + 1, 0), syn((size_t)s.nsyn);
     for (int64_t q = 0; q < s.nsyn; q++) ptr[(tt[(size_t)q] & 0x3fffffffu) + 1]++;
-    for (int i = 0; i < p->n; i++) ptr[(size_t)i + 1] += ptr[(size_t)i];
+    s.in_max = 0;
+    for (int i = 0; i < p->n; i++) {
+        if (ptr[(size_t)i + 1] > s.in_max) s.in_max = ptr[(size_t)i + 1];
+        ptr[(size_t)i + 1] += ptr[(size_t)i];
+    }
     std::vector<int64_t> fillp(ptr.begin(), ptr.end() - 1);
     for (int64_t q = 0; q < s.nsyn; q++) syn[(size_t)fillp[tt[(size_t)q] & 0x3fffffffu]++] = q;
     int rc;

@@ -1039,25 +1039,31 @@ struct StdpPostArgs {
     StepRef step;
     const int64_t *in_ptr;     // [n_tgt + 1]
     const int64_t *in_syn;     // synapse positions grouped by target
+    int chunks;                // kWave-synapse chunks of the longest list
     float *w;
     StdpSyn syn;
     const double *time_ring;
     const float *exp_lut;
 };
 
-template <int LANES>
+// Post-synaptic learning of the neurons that fired at update k: every
+// incoming synapse of each of them (STDPWeightChange.cpp:58-76).  A work
+// item is one wave-sized chunk of one neuron's list, so a neuron with 2,000
+// inputs is spread over 32 waves rather than walked by one; the synapses are
+// distinct, so the order of their updates does not matter.
 __global__ __launch_bounds__(kBlock) void k_stdp_post(StdpPostArgs a) {
     const int32_t step = a.step.get();
     const int slot = step % a.ring;
-    const int32_t n = a.spk_cnt[slot];
+    const int64_t n = a.spk_cnt[slot];
     const int32_t *ids = a.spk_idx + (int64_t)slot * a.slot_len;
     const double tk = a.time_ring[slot];
-    const int lane = threadIdx.x % LANES;
-    const int64_t group = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / LANES;
-    const int64_t n_groups = (int64_t)gridDim.x * kBlock / LANES;
-    for (int64_t it = group; it < n; it += n_groups) {
-        const int32_t i = ids[it];
-        for (int64_t x = a.in_ptr[i] + lane; x < a.in_ptr[i + 1]; x += LANES) {
+    const int lane = threadIdx.x % kWave;
+    const int64_t wave = (int64_t)(blockIdx.x * kBlock + threadIdx.x) / kWave;
+    const int64_t n_waves = (int64_t)gridDim.x * kBlock / kWave;
+    for (int64_t it = wave; it < n * a.chunks; it += n_waves) {
+        const int32_t i = ids[it / a.chunks];
+        const int64_t x = a.in_ptr[i] + (it % a.chunks) * kWave + lane;
+        if (x < a.in_ptr[i + 1]) {
             const int64_t q = a.in_syn[x];
             stdp_decay(a.syn, a.exp_lut, q, tk);
             a.syn.post[q] = a.syn.post[q] + 1.0f;
