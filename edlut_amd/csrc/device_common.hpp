@@ -76,6 +76,38 @@ __device__ __forceinline__ void tile_spikes(const TileOut &o, int tile, const bo
     if (threadIdx.x == 0) o.tile_count[tile] = (int32_t)local;
 }
 
+// The same output for the item-major layout of the BDF kernels (item j of
+// thread t of tile `tile` is neuron tile*kBlock*ITEMS + j*kBlock + t), whose
+// ascending order is item, then wave, then lane: a population smaller than
+// a workgroup gets one neuron per thread.
+template <int ITEMS>
+__device__ __forceinline__ void tile_spikes_item_major(const TileOut &o, int tile, const bool *spike,
+                                                       const int *idx) {
+    __shared__ uint32_t s_wave[ITEMS * kWavesPerBlock];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wid = threadIdx.x / kWave;
+    uint64_t ballot[ITEMS];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        ballot[j] = __ballot(spike[j]);
+        if (lane == 0) s_wave[j * kWavesPerBlock + wid] = (uint32_t)__popcll(ballot[j]);
+    }
+    __syncthreads();
+    uint32_t local = 0;
+#pragma unroll
+    for (int q = 0; q < ITEMS * kWavesPerBlock; q++) local += s_wave[q];
+#pragma unroll
+    for (int j = 0; j < ITEMS; j++) {
+        if (spike[j]) {
+            uint32_t base = 0;
+            for (int q = 0; q < j * kWavesPerBlock + wid; q++) base += s_wave[q];
+            const uint32_t rank = (uint32_t)__popcll(ballot[j] & ((1ull << lane) - 1ull));
+            o.staging[(int64_t)tile * kBlock * ITEMS + base + rank] = idx[j];
+        }
+    }
+    if (threadIdx.x == 0) o.tile_count[tile] = (int32_t)local;
+}
+
 struct CompactArgs {
     const int32_t *tile_count;
     const int32_t *staging;

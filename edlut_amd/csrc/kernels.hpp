@@ -545,7 +545,8 @@ __device__ __forceinline__ void bdf_store(const UpdateArgs &a, float *__restrict
 }
 
 // BDF update: one neuron at a time per thread (data-dependent Newton
-// iterations), kItems neurons per thread in the same tile layout as k_update.
+// iterations), ITEMS neurons per thread in the item-major layout of
+// tile_spikes_item_major (a small population takes one neuron per thread).
 // T = double runs the whole BDF step in double (FP64 Newton mode); the state
 // is stored back in float.
 template <class M, unsigned MASK, bool EXT, typename T, int ITEMS = kItems>
@@ -571,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
     bool spk[ITEMS];
     int idx[ITEMS];
     for (int n = 0; n < ITEMS; n++) {
-        idx[n] = tile_neuron<ITEMS>(tile, n);
+        idx[n] = tile * TILE + n * kBlock + (int)threadIdx.x;
         spk[n] = false;
         if (idx[n] >= a.n) continue;
         const uint32_t i = (uint32_t)idx[n];
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
         for (int j = 0; j < M::NV; j++) out[j] = (float)S[j];
         bdf_store<M, MASK>(a, row, i, out);
     }
-    tile_spikes<ITEMS>(a.out, tile, spk, idx);
+    tile_spikes_item_major<ITEMS>(a.out, tile, spk, idx);
     clock_finish(a, step);
 }
 
