@@ -316,8 +316,7 @@ struct edlut_population {
     uint32_t *cnt = nullptr, *fill = nullptr, *tile_flag = nullptr;
     uint64_t *seg = nullptr, *bump = nullptr;
     uint64_t upd_seq = 0;   // plug-in mode: parity of the fold counters
-    uint64_t *b_ord = nullptr;
-    float *b_w = nullptr;
+    FoldEntry *b_ent = nullptr;
     uint64_t b_cap = 0;
     std::vector<int32_t> q_tgt;
     std::vector<uint8_t> q_type;
@@ -360,8 +359,7 @@ struct edlut_population {
         t.tile_flag = tile_flag;
         t.bump = bump;
         t.par = par;
-        t.b_ord = b_ord;
-        t.b_w = b_w;
+        t.b_ent = b_ent;
         t.b_cap = b_cap;
         t.err = err;
         return t;
@@ -385,13 +383,11 @@ struct edlut_population {
     int reserve_events(uint64_t events) {
         if (events <= b_cap) return 0;
         (void)hipStreamSynchronize(stream);
-        if (b_ord) (void)hipFree(b_ord);
-        if (b_w) (void)hipFree(b_w);
-        b_ord = nullptr;
-        b_w = nullptr;
+        if (b_ent) (void)hipFree(b_ent);
+        b_ent = nullptr;
         b_cap = 0;
         int rc;
-        if ((rc = dalloc(&b_ord, events)) || (rc = dalloc(&b_w, events))) return rc;
+        if ((rc = dalloc(&b_ent, events))) return rc;
         b_cap = events;
         return 0;
     }
@@ -1813,7 +1809,7 @@ int edlut_population_destroy(edlut_population *p) {
                    p->bdf_bits, p->bdf_done,
                    p->state, p->last_spike, p->exp_lut, p->nmda_lut, p->refr, p->tile_count,
                    p->staging, p->spk_idx, p->spk_cnt, p->err, p->cnt, p->fill, p->seg,
-                   p->tile_flag, p->bump, p->b_ord, p->b_w, p->e_tgt,
+                   p->tile_flag, p->bump, p->b_ent, p->e_tgt,
                    p->e_type, p->e_w, p->d_groups, p->d_chunk_group};
     for (void *q : dev)
         if (q) (void)hipFree(q);
@@ -1938,7 +1934,7 @@ namespace {
 // device array instead of the population's growable buffer.  Each device
 // holds its own copy at its own address, looked up once per device.
 constexpr int64_t kNoAllocMaxEvents = 1 << 20;
-static __device__ __align__(256) uint8_t scratch[kNoAllocMaxEvents * (sizeof(uint64_t) + sizeof(float))];
+static __device__ __align__(256) uint8_t scratch[kNoAllocMaxEvents * sizeof(FoldEntry)];
 
 constexpr int kScratchDevices = 64;
 std::mutex g_scratch_mu;
@@ -1985,8 +1981,7 @@ int edlut_population_update_noalloc(edlut_population *p, int64_t step, const int
     // 1. ordered fold of the events (position in the arrays = delivery order)
     const int par = (int)(++p->upd_seq & 1u);
     FoldTarget t = p->fold_target(par);
-    t.b_ord = (uint64_t *)base;
-    t.b_w = (float *)((uint8_t *)base + kNoAllocMaxEvents * sizeof(uint64_t));
+    t.b_ent = (FoldEntry *)base;
     t.b_cap = (uint64_t)kNoAllocMaxEvents;
     if (n_events > 0) {
         const PushEvents pe{target, type, weight, n_events, (int32_t)p->n, p->mask & 7u, p->err};

@@ -45,6 +45,14 @@ constexpr int kGroupAlign = 256;             // grouped populations: groups star
 constexpr uint64_t kSegEmpty = ~0ull;
 constexpr uint64_t kSegBusy = ~0ull - 1ull;
 
+// One queued event of the fold: its order key and weight in one 16-byte
+// entry, so that scattering an event writes one line.
+struct FoldEntry {
+    uint64_t ord;
+    float w;
+    uint32_t pad;
+};
+
 struct FoldTarget {
     float *state;
     int64_t stride;
@@ -56,8 +64,7 @@ struct FoldTarget {
     uint64_t *bump;      // [2] event-buffer allocator, by step parity
     int par;             // parity of this step, unless par_by_step
     int par_by_step;     // parity = update index & 1 (network mode)
-    uint64_t *b_ord;     // event buffer: order
-    float *b_w;          //               weight
+    FoldEntry *b_ent;    // event buffer (order key, weight)
     uint64_t b_cap;
     uint32_t *err;       // bit 2: event buffer overflow, bit 3: spin limit
     // network mode: fixed segment of each key (prefix of the keys' synapse
@@ -103,8 +110,7 @@ __device__ __forceinline__ void fold_event(const FoldTarget &t, int par, uint64_
         // network mode, one pass: the key's fixed segment, next free entry
         const uint32_t pos = atomicAdd(&t.fill[key], 1u);
         const uint64_t at = t.seg_off[key] + pos;
-        t.b_ord[at] = ord;
-        t.b_w[at] = w;
+        t.b_ent[at] = FoldEntry{ord, w, 0u};
         if (pos == 0u) t.tile_flag[(key % (uint64_t)t.stride) / kFlagTile] = 1u;
         return;
     }
@@ -140,8 +146,7 @@ __device__ __forceinline__ void fold_event(const FoldTarget &t, int par, uint64_
     }
     const uint64_t pos = base + atomicAdd(&t.fill[key], 1u);
     if (pos < t.b_cap) {
-        t.b_ord[pos] = ord;
-        t.b_w[pos] = w;
+        t.b_ent[pos] = FoldEntry{ord, w, 0u};
     } else {
         atomicOr(t.err, 4u);
     }
@@ -158,30 +163,27 @@ __device__ __forceinline__ uint32_t key_rank(const RankTab *rt, int par, uint64_
     return rt->rank[p][((int64_t)par * rt->delays + (d - 1)) * rt->cmax[p] + c];
 }
 
-__device__ __attribute__((noinline)) float fold_static(uint32_t *fill, const uint64_t *seg_off, uint64_t *b_ord,
-                                                       float *b_w, const RankTab *rt, int par, uint64_t key,
-                                                       uint32_t m, float g) {
+__device__ __attribute__((noinline)) float fold_static(uint32_t *fill, const uint64_t *seg_off, FoldEntry *e,
+                                                       const RankTab *rt, int par, uint64_t key, uint32_t m,
+                                                       float g) {
     const uint64_t b = seg_off[key];
     if (rt && m >= 2u) {
         // (delay, source, chunk) -> the group's rank in the event queue's window
         for (uint32_t j = 0; j < m; j++) {
-            const uint64_t o = b_ord[b + j];
-            b_ord[b + j] = ((uint64_t)key_rank(rt, par, o) << kOrdSynBits) | (o & kOrdSynMask);
+            const uint64_t o = e[b + j].ord;
+            e[b + j].ord = ((uint64_t)key_rank(rt, par, o) << kOrdSynBits) | (o & kOrdSynMask);
         }
     }
     for (uint32_t j = 1; j < m; j++) {
-        const uint64_t ko = b_ord[b + j];
-        const float kw = b_w[b + j];
+        const FoldEntry k = e[b + j];
         int64_t i = (int64_t)j - 1;
-        while (i >= 0 && b_ord[b + i] > ko) {
-            b_ord[b + i + 1] = b_ord[b + i];
-            b_w[b + i + 1] = b_w[b + i];
+        while (i >= 0 && e[b + i].ord > k.ord) {
+            e[b + i + 1] = e[b + i];
             --i;
         }
-        b_ord[b + i + 1] = ko;
-        b_w[b + i + 1] = kw;
+        e[b + i + 1] = k;
     }
-    for (uint32_t j = 0; j < m; j++) g = g + b_w[b + j];
+    for (uint32_t j = 0; j < m; j++) g = g + e[b + j].w;
     fill[key] = 0u;
     return g;
 }
@@ -191,8 +193,7 @@ __device__ __attribute__((noinline)) float fold_static(uint32_t *fill, const uin
 // line so the common path keeps its small register budget; the segment is
 // sorted in place in the event buffer (insertion sort: segments are short).
 __device__ __attribute__((noinline)) float fold_segment(uint32_t *cnt, uint32_t *fill,
-                                                        uint64_t *seg, uint64_t *b_ord,
-                                                        float *b_w, uint64_t b_cap,
+                                                        uint64_t *seg, FoldEntry *e, uint64_t b_cap,
                                                         uint32_t *err, uint64_t key,
                                                         uint32_t m, float g) {
     const uint64_t b = seg[key];
@@ -200,18 +201,15 @@ __device__ __attribute__((noinline)) float fold_segment(uint32_t *cnt, uint32_t 
         atomicOr(err, 4u);
     } else {
         for (uint32_t j = 1; j < m; j++) {
-            const uint64_t ko = b_ord[b + j];
-            const float kw = b_w[b + j];
+            const FoldEntry k = e[b + j];
             int64_t i = (int64_t)j - 1;
-            while (i >= 0 && b_ord[b + i] > ko) {
-                b_ord[b + i + 1] = b_ord[b + i];
-                b_w[b + i + 1] = b_w[b + i];
+            while (i >= 0 && e[b + i].ord > k.ord) {
+                e[b + i + 1] = e[b + i];
                 --i;
             }
-            b_ord[b + i + 1] = ko;
-            b_w[b + i + 1] = kw;
+            e[b + i + 1] = k;
         }
-        for (uint32_t j = 0; j < m; j++) g = g + b_w[b + j];
+        for (uint32_t j = 0; j < m; j++) g = g + e[b + j].w;
     }
     cnt[key] = 0u;
     fill[key] = 0u;
@@ -382,14 +380,14 @@ __global__ __launch_bounds__(kBlock) void k_update(UpdateArgs a) {
                 if (a.fold.seg_off) {
                     const uint32_t m = a.fold.fill[key];
                     if (m >= 1u)
-                        s[n][M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
+                        s[n][M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ent,
                                                         a.fold.rtab, step & 1, key, m, s[n][M::EXC + ty]);
                     continue;
                 }
                 const uint32_t m = a.fold.cnt[key];
                 if (m >= 2u)
-                    s[n][M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ord,
-                                                     a.fold.b_w, a.fold.b_cap, a.fold.err, key, m,
+                    s[n][M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ent,
+                                                     a.fold.b_cap, a.fold.err, key, m,
                                                      s[n][M::EXC + ty]);
             }
         }
@@ -508,14 +506,14 @@ __device__ __forceinline__ void bdf_load(const UpdateArgs &a, float *__restrict_
             if (a.fold.seg_off) {
                 const uint32_t m = a.fold.fill[key];
                 if (m >= 1u)
-                    sf[M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ord, a.fold.b_w,
+                    sf[M::EXC + ty] = fold_static(a.fold.fill, a.fold.seg_off, a.fold.b_ent,
                                                   a.fold.rtab, step & 1, key, m, sf[M::EXC + ty]);
                 continue;
             }
             const uint32_t m = a.fold.cnt[key];
             if (m >= 2u)
-                sf[M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ord,
-                                               a.fold.b_w, a.fold.b_cap, a.fold.err, key, m,
+                sf[M::EXC + ty] = fold_segment(a.fold.cnt, a.fold.fill, a.fold.seg, a.fold.b_ent,
+                                               a.fold.b_cap, a.fold.err, key, m,
                                                sf[M::EXC + ty]);
         }
     }
@@ -1029,8 +1027,8 @@ __global__ __launch_bounds__(kBlock) void k_key_degree(const uint32_t *tgt_type,
 // STDPWeightChange.cpp:57-76, called for every neuron that fired at update k,
 // TimeDrivenInternalSpike.cpp:94-103): every plastic input synapse of a
 // spiking neuron decays to the update time, gains a postsynaptic trace unit
-// and potentiates by max_LTP times its presynaptic trace.  LANES lanes walk
-// one spiking neuron's inputs (in_syn lists synapse positions by target).
+// and potentiates by max_LTP times its presynaptic trace (in_syn lists
+// synapse positions by target).
 // ---------------------------------------------------------------------------
 struct StdpPostArgs {
     const int32_t *spk_idx;    // own spike ring, slot k % ring
@@ -1048,7 +1046,7 @@ struct StdpPostArgs {
 };
 
 // Post-synaptic learning of the neurons that fired at update k: every
-// incoming synapse of each of them (STDPWeightChange.cpp:58-76).  A work
+// incoming synapse of each of them.  A work
 // item is one wave-sized chunk of one neuron's list, so a neuron with 2,000
 // inputs is spread over 32 waves rather than walked by one; the synapses are
 // distinct, so the order of their updates does not matter.
@@ -1094,7 +1092,7 @@ __global__ __launch_bounds__(kBlock) void k_apply_fold(FoldTarget t, int32_t n, 
             const uint32_t m = t.fill[key];
             if (m == 0u) continue;
             float *g = g_ptr(t, key);
-            *g = fold_static(t.fill, t.seg_off, t.b_ord, t.b_w, t.rtab, par, key, m, *g);
+            *g = fold_static(t.fill, t.seg_off, t.b_ent, t.rtab, par, key, m, *g);
         }
     }
 }

@@ -151,7 +151,7 @@ int edlut_population_get_spikes(const edlut_population *pop, int32_t *idx, int c
  * (spk_idx: room for every neuron, ascending; spk_cnt: one int32).
  * Asynchronous on `stream` (null: the population's stream); calls no
  * hipMalloc, hipFree or host synchronisation.  The input fold's event buffer
- * is a library-wide device array of EDLUT_NOALLOC_MAX_EVENTS events (12 MiB,
+ * is a library-wide device array of EDLUT_NOALLOC_MAX_EVENTS events (16 MiB,
  * held by each GPU the library runs on until the process ends): n_events
  * above it is refused (EDLUT_E_CAP).  Every call on one GPU shares that
  * array, so calls that overlap on one GPU race; serialise them.  An event
