@@ -13,18 +13,25 @@ from tests.helpers import bits
 pytestmark = pytest.mark.gpu
 
 
-def _pop(n):
-    m = E.CreateNeuronModel("AdExTimeDrivenModel_GPU", {"int_meth": E.integration_method("RK4", 1e-4)})
+CASES = [("AdExTimeDrivenModel_GPU", "RK4", 12.0),
+         # the granule's BDF kernel: lanes refilled from equal ranges, spike lists
+         # written by the last workgroup
+         ("EgidioGranuleCell_TimeDriven_GPU", "BDF", 1.0)]
+
+
+def _pop(n, model="AdExTimeDrivenModel_GPU", integ="RK4"):
+    m = E.CreateNeuronModel(model, {"int_meth": E.integration_method(integ, 1e-4, 2 if integ == "BDF" else 1)})
     for t in (0, 1, 2):
         assert m.CheckSynapseType(t)
     m.InitializeStates(n, 0, 0)
     return m
 
 
-def test_update_noalloc_equals_update():
+@pytest.mark.parametrize("model,integ,wmax", CASES)
+def test_update_noalloc_equals_update(model, integ, wmax):
     n = 4000
-    print("case noalloc update n=4000", flush=True)
-    ref, m = _pop(n), _pop(n)
+    print(f"case noalloc update {model} {integ} n=4000", flush=True)
+    ref, m = _pop(n, model, integ), _pop(n, model, integ)
     rng = np.random.default_rng(11)
     idx = torch.full((n,), -1, dtype=torch.int32, device="cuda")
     cnt = torch.zeros(1, dtype=torch.int32, device="cuda")
@@ -34,7 +41,7 @@ def test_update_noalloc_equals_update():
         tgt = rng.integers(0, n, ne).astype(np.int32)
         tgt[: ne // 10] = 7                     # one hot target
         typ = rng.choice(np.array([0, 0, 0, 1, 2], np.uint8), ne)
-        w = rng.uniform(0.0, 12.0, ne).astype(np.float32)
+        w = rng.uniform(0.0, wmax, ne).astype(np.float32)
         ref.ProcessInputSpikes(tgt, typ, w)
         ref.UpdateState(step=k)
         want = ref.GetInternalSpikeIndexs()
