@@ -1065,7 +1065,9 @@ __device__ __forceinline__ void granule_newton_reference(const ModelParams &mp, 
 // granule_bdf_predict (the order-st predictor), granule_bdf_newton (one
 // corrector iteration, repeated while epsi > 1e-6 and k < 10) and
 // granule_bdf_commit (history shift and new state).  Run in that order they
-// are the step of integrate_bdf above, BDFn.h:288-423.
+// are the step of integrate_bdf above, BDFn.h:288-423.  The corrector reads
+// the first previous-state row from ph0 (a lane's copy in LDS, stride kBlock,
+// taken when the lane took the neuron), the older rows from memory.
 template <unsigned MASK>
 __device__ __forceinline__ void granule_bdf_predict(const ModelParams &mp, const RhsCtx &c, float dt,
                                                     const BdfState &b, int64_t ld, uint32_t i, int st,
@@ -1093,7 +1095,8 @@ __device__ __forceinline__ void granule_bdf_predict(const ModelParams &mp, const
 template <unsigned MASK>
 __device__ __forceinline__ void granule_bdf_newton(const ModelParams &mp, const RhsCtx &c, float dt,
                                                    const BdfState &b, int par, int64_t ld, uint32_t i, int st,
-                                                   const float *S, float *P, float &epsi, int &k) {
+                                                   const float *S, float *P, float &epsi, int &k,
+                                                   const float *ph0) {
     using G = GranuleModel;
     constexpr int ND = G::ND, NV = G::NV;
     const float *Ph = (const float *)b.prev;
@@ -1110,7 +1113,8 @@ __device__ __forceinline__ void granule_bdf_newton(const ModelParams &mp, const 
 #pragma unroll
         for (int j = 0; j < ND; j++) {
             Cc[j] = c0 * dt * F[j] + Cs[1] * S[j];
-            for (int q = 1; q < st; q++) Cc[j] += Cs[q + 1] * Ph[((int64_t)(q - 1) * ND + j) * ld + i];
+            if (st > 1) Cc[j] += Cs[2] * ph0[j * kBlock];
+            for (int q = 2; q < st; q++) Cc[j] += Cs[q + 1] * Ph[((int64_t)(q - 1) * ND + j) * ld + i];
             fin = fin && __builtin_isfinite(F[j]);
         }
         // J = c0*dt*jac - I, jac[z][j] = (F[z] - Fn_j[z]) / eps (one-sided, at P)

@@ -614,7 +614,8 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
 // arithmetic is the same either way.  The grid is at most one workgroup per
 // CU (the Newton step fills the register file); the ranges are equal.
 //   * The channel table (104 KB) is staged in LDS: every right-hand side
-//     gathers a 26-float row from it.
+//     gathers a 26-float row from it.  A lane's first previous-state row
+//     (15 floats) is copied to LDS when it takes a neuron.
 //   * Spikes set bits of a.bdf.spk_bits; the last workgroup to finish turns
 //     them into the per-tile spike lists (ascending, kFlagTile neurons per
 //     tile) that the compaction reads, clears the bits and the tiles' fold
@@ -639,6 +640,10 @@ __global__ __launch_bounds__(kBlock) void k_update_granule_bdf(UpdateArgs a) {
     for (int j = 0; j < G::NV; j++) row[j] = a.state + (int64_t)j * a.stride;
     const int64_t ld = a.stride;
     const uint32_t step_key = EXT ? drive_step_key(a.ext.seed, step) : 0u;
+    // each lane's copy of its neuron's first previous-state row, which every
+    // corrector iteration reads (2.4% of the cerebellum step against reading
+    // it from memory once lanes have drifted apart in their range)
+    __shared__ float s_ph0[G::ND * kBlock];
     extern __shared__ float4 s_chan[];
     {
         const float4 *g4 = (const float4 *)a.chan;
@@ -692,6 +697,11 @@ __global__ __launch_bounds__(kBlock) void k_update_granule_bdf(UpdateArgs a) {
                         st = a.bdf.st[i];
                         if (a.last_spike) c.since_spike = step - 1 - a.last_spike[i];
                         granule_bdf_predict<MASK>(a.mp, c, dt, a.bdf, ld, (uint32_t)i, st, S, P);
+                        if (st > 1) {
+                            const float *Ph = (const float *)a.bdf.prev;
+#pragma unroll
+                            for (int q = 0; q < G::ND; q++) s_ph0[q * kBlock + threadIdx.x] = Ph[(int64_t)q * ld + i];
+                        }
                         epsi = 1.0f;
                         k = 0;
                         busy = true;
@@ -701,7 +711,8 @@ __global__ __launch_bounds__(kBlock) void k_update_granule_bdf(UpdateArgs a) {
             }
         }
         if (!__any(busy)) break;
-        if (busy) granule_bdf_newton<MASK>(a.mp, c, dt, a.bdf, par, ld, (uint32_t)i, st, S, P, epsi, k);
+        if (busy)
+            granule_bdf_newton<MASK>(a.mp, c, dt, a.bdf, par, ld, (uint32_t)i, st, S, P, epsi, k, s_ph0 + threadIdx.x);
     }
     __shared__ uint32_t s_last;
     __syncthreads();
