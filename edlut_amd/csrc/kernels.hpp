@@ -615,7 +615,8 @@ __global__ __launch_bounds__(kBlock) void k_update_bdf(UpdateArgs a) {
 // CU (the Newton step fills the register file); the ranges are equal.
 //   * The channel table (104 KB) is staged in LDS: every right-hand side
 //     gathers a 26-float row from it.  A lane's first previous-state row
-//     (15 floats) is copied to LDS when it takes a neuron.
+//     (15 floats) is copied to LDS when it takes a neuron, and the BDF
+//     coefficients are staged there too.
 //   * Spikes set bits of a.bdf.spk_bits; the last workgroup to finish turns
 //     them into the per-tile spike lists (ascending, kFlagTile neurons per
 //     tile) that the compaction reads, clears the bits and the tiles' fold
@@ -644,12 +645,17 @@ __global__ __launch_bounds__(kBlock) void k_update_granule_bdf(UpdateArgs a) {
     // corrector iteration reads (2.4% of the cerebellum step against reading
     // it from memory once lanes have drifted apart in their range)
     __shared__ float s_ph0[G::ND * kBlock];
+    // the BDF coefficient table (7 x 7), read by every corrector iteration
+    __shared__ float s_bdfc[49];
     extern __shared__ float4 s_chan[];
     {
         const float4 *g4 = (const float4 *)a.chan;
         for (int q = threadIdx.x; q < kGrTable * kGrVars / 4; q += kBlock) s_chan[q] = g4[q];
+        if (threadIdx.x < 49) s_bdfc[threadIdx.x] = a.bdf.C[threadIdx.x];
     }
     __syncthreads();
+    BdfState bl = a.bdf;
+    bl.C = s_bdfc;
     RhsCtx c;
     c.chan = (const float *)s_chan;
     c.exp_lut = a.exp_lut;
@@ -712,7 +718,7 @@ __global__ __launch_bounds__(kBlock) void k_update_granule_bdf(UpdateArgs a) {
         }
         if (!__any(busy)) break;
         if (busy)
-            granule_bdf_newton<MASK>(a.mp, c, dt, a.bdf, par, ld, (uint32_t)i, st, S, P, epsi, k, s_ph0 + threadIdx.x);
+            granule_bdf_newton<MASK>(a.mp, c, dt, bl, par, ld, (uint32_t)i, st, S, P, epsi, k, s_ph0 + threadIdx.x);
     }
     __shared__ uint32_t s_last;
     __syncthreads();
